@@ -1,0 +1,254 @@
+"""Python binding of the MI355X continuum-LDOS library (libcqpi_amd.so).
+
+``AmdQPI`` mirrors the reference's GPU plug-in interface, class GPUQPI
+(include/gpuqpi.h:18-32) as implemented by HipQPI (src/qpihipkernel.h:64-88):
+same constructor arguments, ``wannierldos(scat, g0)``, ``retrieveResult``,
+``printinfo``; errors raise ``CqpiError`` instead of exiting the process
+(src/qpihipkernel.cc:372-380).  The whole-chain extension (``set_model``,
+``set_impurity``, ``run``) moves the eigensolve, resolvent, k->R transform and
+T-matrix (src/qpi.cc:454-486, :557-578, :833-854) onto the device too.
+
+The compute path is the HIP library only; there is no CPU fallback.  Loading
+fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import synth
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(_HERE), "libcqpi_amd.so")
+_LIB = None
+
+PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED}
+STAGES = ("eigensolve", "resolvent+transform_ky", "transform_kx", "tmatrix+bulk", "contraction")
+
+_sz = C.c_size_t
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# every symbol include/cqpi_amd.h declares, with its ctypes signature
+SIGNATURES = {
+    "cqpi_last_error": (C.c_char_p, []),
+    "cqpi_version": (C.c_char_p, []),
+    "cqpi_device_count": (C.c_int, []),
+    "cqpi_create": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _sz, C.c_int, _dp, C.POINTER(_vp)]),
+    "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
+    "cqpi_retrieve": (C.c_int, [_vp, _dp]),
+    "cqpi_printinfo": (C.c_int, [_vp, C.c_char_p, _sz]),
+    "cqpi_destroy": (None, [_vp]),
+    "cqpi_set_precision": (C.c_int, [_vp, C.c_int]),
+    "cqpi_set_model": (C.c_int, [_vp, _vp, _sz, C.c_double, C.c_int]),
+    "cqpi_set_impurity": (C.c_int, [_vp, _dp]),
+    "cqpi_compute_eigen": (C.c_int, [_vp]),
+    "cqpi_run": (C.c_int, [_vp, _dp, _sz, C.c_double, _dp]),
+    "cqpi_retrieve_layers": (C.c_int, [_vp, _sz, _sz, _dp]),
+    "cqpi_device_ldos": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_sz)]),
+    "cqpi_window_half": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "cqpi_retrieve_g0_window": (C.c_int, [_vp, _sz, _dp]),
+    "cqpi_retrieve_tmatrix": (C.c_int, [_vp, _sz, _dp]),
+    "cqpi_stage_times": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64)]),
+    "cqpi_set_timing": (C.c_int, [_vp, C.c_int]),
+    "cqpi_read_hr": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_sz),
+                               C.c_char_p, _sz]),
+    "cqpi_free": (None, [_vp]),
+}
+
+
+class CqpiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cqpi error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded libcqpi_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libcqpi_amd.so not built (run __graft_entry__.build() or "
+                               "make -C calcqpi-release_amd): %s" % LIBPATH)
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc:
+        raise CqpiError(rc, lib().cqpi_last_error().decode(errors="replace"))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def device_count() -> int:
+    return int(lib().cqpi_device_count())
+
+
+def version() -> str:
+    return lib().cqpi_version().decode()
+
+
+def read_hr(path, scmodel=False):
+    """Reads a Wannier90 _hr.dat file with the library's reader; returns
+    (hops as synth.HOP_DTYPE array, bands, name)."""
+    L = lib()
+    hp = _vp()
+    nh = _sz()
+    nb = _sz()
+    name = C.create_string_buffer(1024)
+    _check(L.cqpi_read_hr(os.fsencode(path), int(scmodel), C.byref(hp), C.byref(nh), C.byref(nb), name, 1024))
+    n = nh.value
+    try:
+        raw = C.string_at(hp.value, n * synth.HOP_DTYPE.itemsize) if n else b""
+    finally:
+        L.cqpi_free(hp)
+    return np.frombuffer(raw, dtype=synth.HOP_DTYPE).copy(), nb.value, name.value.decode(errors="replace")
+
+
+class AmdQPI:
+    """GPUQPI drop-in plus the whole-chain extension (one HIP context)."""
+
+    def __init__(self, wanniern, kpoints, n, window, oversamp, bands, maxband, spin, wf):
+        self.W = int(wanniern)
+        self.K = int(kpoints)
+        self.n = int(n)
+        self.window = int(window)
+        self.oversamp = int(oversamp)
+        self.bands = int(bands)
+        self.maxband = int(maxband) if maxband else int(bands)
+        self.spin = bool(spin)
+        T = (2 * self.window + 1) * self.oversamp
+        wf = np.ascontiguousarray(wf, dtype=np.float64)
+        if wf.shape != (self.maxband, T, T):
+            raise ValueError("wf must have shape (maxband, T, T) = %s, got %s" % ((self.maxband, T, T), wf.shape))
+        self._wf = wf
+        self._ctx = _vp()
+        _check(lib().cqpi_create(self.W, self.K, self.n, self.window, self.oversamp, self.bands,
+                                 self.maxband, int(self.spin), _d(wf), C.byref(self._ctx)))
+        self.nlayers = 0
+
+    # -- GPUQPI ------------------------------------------------------------
+    def wannierldos(self, scat, g0):
+        """scat: T (b x b complex); g0: G0(R) on the k-grid, shape (K, K, b, b)
+        complex, g0[ix, iy] = G0(R = (ix, iy) mod K)."""
+        b = self.bands
+        scat = np.ascontiguousarray(scat, dtype=np.complex128)
+        g0 = np.ascontiguousarray(g0, dtype=np.complex128)
+        if scat.shape != (b, b) or g0.shape != (self.K, self.K, b, b):
+            raise ValueError("shape mismatch")
+        base = g0.ctypes.data
+        ptrs = (base + np.arange(self.K * self.K, dtype=np.uint64) * np.uint64(b * b * 16)).astype(np.uint64)
+        _check(lib().cqpi_wannierldos(self._ctx, _d(scat.view(np.float64)),
+                                      ptrs.ctypes.data_as(C.POINTER(_vp)), b))
+        self.nlayers = 1
+
+    def retrieveResult(self, ldos=None):
+        out = np.empty(self.W * self.W) if ldos is None else ldos
+        _check(lib().cqpi_retrieve(self._ctx, _d(out)))
+        return out
+
+    def printinfo(self) -> str:
+        buf = C.create_string_buffer(2048)
+        _check(lib().cqpi_printinfo(self._ctx, buf, 2048))
+        return buf.value.decode()
+
+    # -- whole chain --------------------------------------------------------
+    def set_precision(self, mode):
+        mode = PRECISIONS.get(mode, mode)
+        _check(lib().cqpi_set_precision(self._ctx, int(mode)))
+
+    def set_model(self, hops, efermi=0.0, scmodel=False):
+        hops = np.ascontiguousarray(hops, dtype=synth.HOP_DTYPE)
+        _check(lib().cqpi_set_model(self._ctx, hops.ctypes.data if len(hops) else None, len(hops),
+                                    float(efermi), int(scmodel)))
+
+    def set_impurity(self, V):
+        V = np.ascontiguousarray(V, dtype=np.complex128)
+        if V.shape != (self.bands, self.bands):
+            raise ValueError("V must be b x b")
+        _check(lib().cqpi_set_impurity(self._ctx, _d(V.view(np.float64))))
+
+    def compute_eigen(self):
+        _check(lib().cqpi_compute_eigen(self._ctx))
+
+    def set_timing(self, on=True):
+        _check(lib().cqpi_set_timing(self._ctx, int(on)))
+
+    def run(self, omegas, eta, host=True):
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        E = len(om)
+        out = np.empty((E, self.W * self.W)) if host else None
+        _check(lib().cqpi_run(self._ctx, _d(om), E, float(eta), _d(out) if host else None))
+        self.nlayers = E
+        return out
+
+    def retrieve_layers(self, first, count):
+        out = np.empty((count, self.W * self.W))
+        _check(lib().cqpi_retrieve_layers(self._ctx, first, count, _d(out)))
+        return out
+
+    def window_half(self):
+        h = _sz()
+        _check(lib().cqpi_window_half(self._ctx, C.byref(h)))
+        return h.value
+
+    def g0_window(self, e):
+        h = self.window_half()
+        nw = 2 * h + 1
+        out = np.empty((nw, nw, self.bands, self.bands), np.complex128)
+        _check(lib().cqpi_retrieve_g0_window(self._ctx, e, _d(out.view(np.float64))))
+        return out
+
+    def tmatrix(self, e):
+        out = np.empty((self.bands, self.bands), np.complex128)
+        _check(lib().cqpi_retrieve_tmatrix(self._ctx, e, _d(out.view(np.float64))))
+        return out
+
+    def stage_times(self):
+        ms = np.zeros(5)
+        n = (C.c_int64 * 5)()
+        _check(lib().cqpi_stage_times(self._ctx, _d(ms), n))
+        return {STAGES[i]: (float(ms[i]), int(n[i])) for i in range(5)}
+
+    def device_ldos(self):
+        p = _vp()
+        cnt = _sz()
+        _check(lib().cqpi_device_ldos(self._ctx, C.byref(p), C.byref(cnt)))
+        return p.value, cnt.value
+
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            lib().cqpi_destroy(self._ctx)
+            self._ctx = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def engine_for(wl: "synth.Workload", precision="fp64") -> AmdQPI:
+    """A context for a synth.Workload with model and impurity uploaded."""
+    q = AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
+    q.set_model(wl.hops, wl.efermi, wl.scmodel)
+    q.set_impurity(wl.V)
+    q.set_precision(precision)
+    return q

@@ -1,0 +1,601 @@
+// C ABI of the continuum-LDOS library: context, GPUQPI drop-in entry points
+// and the whole-chain device path.  See include/cqpi_amd.h for the contract.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "cqpi_common.h"
+
+using namespace cqpi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(CQPI_E_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RETCHK(call)                                                         \
+    do {                                                                     \
+        int r_ = (call);                                                     \
+        if (r_) {                                                            \
+            if (g_err.empty()) g_err = std::string(#call " failed");         \
+            return r_;                                                       \
+        }                                                                    \
+    } while (0)
+
+template <class T>
+int dev_grow(T *&ptr, size_t &cap, size_t count, size_t &gpumem)
+{
+    if (count <= cap && ptr) return 0;
+    if (ptr) {
+        (void)hipFree(ptr);
+        gpumem -= cap * sizeof(T);
+        ptr = nullptr;
+        cap = 0;
+    }
+    hipError_t e = hipMalloc((void **)&ptr, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return fail(CQPI_E_HIP, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                    " bytes failed: " + hipGetErrorString(e));
+    }
+    cap = std::max<size_t>(count, 1);
+    gpumem += cap * sizeof(T);
+    return 0;
+}
+
+bool is_supported_bands(size_t b)
+{
+    return (b >= 1 && b <= 12) || b == 14 || b == 16;
+}
+
+}  // namespace
+
+namespace cqpi {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace cqpi
+
+struct cqpi_ctx {
+    Geom g{};
+    size_t wanniern = 0, kpoints = 0;
+    int precision = CQPI_PREC_FP64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t gpumem = 0;
+
+    // tiles
+    double *d_tiles = nullptr;
+    size_t c_tiles = 0;
+    std::vector<double> h_tiles;
+
+    // model
+    bool has_model = false;
+    DevRGroup *d_groups = nullptr;
+    size_t c_groups = 0;
+    DevHop *d_hops = nullptr;
+    size_t c_hops = 0;
+    int ngroups = 0;
+    double efermi = 0.0;
+    int scmax = 0;
+    double *d_eval = nullptr;
+    size_t c_eval = 0;
+    double2 *d_evec = nullptr;
+    size_t c_evec = 0;
+    bool eigen_valid = false;
+
+    // impurity
+    double2 *d_V = nullptr;
+    size_t c_V = 0;
+
+    // transform
+    double2 *d_tw = nullptr;
+    size_t c_tw = 0;
+
+    // per-chunk buffers
+    double2 *d_F1 = nullptr;
+    size_t c_F1 = 0;
+    double2 *d_G0w = nullptr;
+    size_t c_G0w = 0;
+    double *d_bulk = nullptr;
+    size_t c_bulk = 0;
+    // per-run buffers
+    double2 *d_T = nullptr;
+    size_t c_T = 0;
+    int *d_status = nullptr;
+    size_t c_status = 0;
+    double *d_omega = nullptr;
+    size_t c_omega = 0;
+    double *d_ldos = nullptr;
+    size_t c_ldos = 0;
+    size_t nlayers = 0;
+    size_t last_chunk_e0 = 0, last_chunk_ec = 0;
+
+    // host staging for the drop-in path (pinned)
+    double2 *h_win = nullptr;
+    size_t c_hwin = 0;
+    double2 *h_T = nullptr;
+    hipEvent_t copy_done = nullptr;
+
+    // timing
+    bool timing = false;
+    double stage_ms[CQPI_NSTAGES] = {0};
+    int64_t stage_n[CQPI_NSTAGES] = {0};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+namespace {
+
+void release(cqpi_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw,
+                    c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos};
+    for (void *p : bufs)
+        if (p) hipFree(p);
+    if (c->h_win) hipHostFree(c->h_win);
+    if (c->h_T) hipHostFree(c->h_T);
+    for (auto &p : c->pending) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    if (c->copy_done) hipEventDestroy(c->copy_done);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// stage timing: begin returns an index into pending, end records the stop event
+int stage_begin(cqpi_ctx *c, int stage)
+{
+    if (!c->timing) return -1;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return -1;
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return -1;
+    }
+    hipEventRecord(a, c->stream);
+    c->pending.push_back({stage, {a, b}});
+    return (int)c->pending.size() - 1;
+}
+
+void stage_end(cqpi_ctx *c, int idx)
+{
+    if (idx < 0) return;
+    hipEventRecord(c->pending[idx].second.second, c->stream);
+}
+
+void stage_collect(cqpi_ctx *c)
+{
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.second.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            c->stage_ms[p.first] += ms;
+            c->stage_n[p.first] += 1;
+        }
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    c->pending.clear();
+}
+
+void reset_stages(cqpi_ctx *c)
+{
+    for (int s = 0; s < CQPI_NSTAGES; s++) {
+        c->stage_ms[s] = 0.0;
+        c->stage_n[s] = 0;
+    }
+}
+
+int upload_identity_V(cqpi_ctx *c)
+{
+    int B = c->g.B;
+    std::vector<double2> V((size_t)B * B, make_double2(0.0, 0.0));
+    for (int i = 0; i < B; i++) V[(size_t)i * B + i] = make_double2(1.0, 0.0);
+    RETCHK(dev_grow(c->d_V, c->c_V, V.size(), c->gpumem));
+    HIPCHK(hipMemcpy(c->d_V, V.data(), V.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// bulk table + contraction for E energies whose G0 window, T are on the device
+int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos)
+{
+    const Geom &g = c->g;
+    RETCHK(dev_grow(c->d_bulk, c->c_bulk, (size_t)E * g.os * g.os, c->gpumem));
+    int st = stage_begin(c, 3);
+    RETCHK(launch_bulk(g, G0w, c->d_tiles, E, c->d_bulk, c->stream));
+    stage_end(c, st);
+    st = stage_begin(c, 4);
+    int r;
+    switch (c->precision) {
+    default:
+        r = launch_contract_fp64(g, G0w, T, c->d_bulk, c->d_tiles, E, ldos, c->stream);
+        break;
+    }
+    stage_end(c, st);
+    if (r) return fail(r, "contraction launch failed");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cqpi_last_error(void) { return g_err.c_str(); }
+
+const char *cqpi_version(void) { return "cqpi_amd 0.1 (gfx950)"; }
+
+int cqpi_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands,
+                size_t maxband, int spin, const double *wf, cqpi_ctx **out)
+{
+    g_err.clear();
+    if (!out) return fail(CQPI_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!n || !kpoints || !oversamp || !bands) return fail(CQPI_E_ARG, "n, kpoints, oversamp and bands must be > 0");
+    if (wanniern != oversamp * n) return fail(CQPI_E_ARG, "wanniern must equal oversamp*n");
+    if (!is_supported_bands(bands)) return fail(CQPI_E_ARG, "unsupported band count (1-12, 14, 16)");
+    if (maxband == 0) maxband = bands;
+    if (maxband > bands) return fail(CQPI_E_ARG, "maxband > bands");
+    if (spin && maxband < 2) return fail(CQPI_E_ARG, "spin needs maxband >= 2");
+    if (!wf) return fail(CQPI_E_ARG, "wf is NULL");
+    if (kpoints * sizeof(double2) > 80 * 1024) return fail(CQPI_E_ARG, "kpoints > 5120 not supported");
+    if (cqpi_device_count() <= 0) return fail(CQPI_E_NODEV, "no HIP device visible");
+    cqpi_ctx *c = new cqpi_ctx();
+    Geom &g = c->g;
+    g.K = (int)kpoints;
+    g.n = (int)n;
+    g.w = (int)window;
+    g.os = (int)oversamp;
+    g.B = (int)bands;
+    g.maxb = (int)maxband;
+    g.spin = spin ? 1 : 0;
+    int n2 = g.n >> 1;
+    g.h = std::max(n2 + g.w, 2 * g.w);
+    g.nw = 2 * g.h + 1;
+    g.T = (2 * g.w + 1) * g.os;
+    g.W = g.os * g.n;
+    c->wanniern = wanniern;
+    c->kpoints = kpoints;
+    c->scmax = g.B;
+    hipGetDevice(&c->device);
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        release(c);
+        return fail(CQPI_E_HIP, "hipStreamCreate failed");
+    }
+    if (hipEventCreateWithFlags(&c->copy_done, hipEventDisableTiming) != hipSuccess) {
+        release(c);
+        return fail(CQPI_E_HIP, "hipEventCreate failed");
+    }
+    size_t ntile = (size_t)maxband * g.T * g.T;
+    c->h_tiles.assign(wf, wf + ntile);
+    int r = dev_grow(c->d_tiles, c->c_tiles, ntile, c->gpumem);
+    if (!r && hipMemcpy(c->d_tiles, wf, ntile * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "tile upload failed");
+    // twiddles exp(+2 pi i m / K), m < K
+    if (!r) {
+        std::vector<double2> tw(kpoints);
+        for (size_t m = 0; m < kpoints; m++) {
+            long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)kpoints;
+            tw[m] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        r = dev_grow(c->d_tw, c->c_tw, kpoints, c->gpumem);
+        if (!r && hipMemcpy(c->d_tw, tw.data(), kpoints * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+            r = fail(CQPI_E_HIP, "twiddle upload failed");
+    }
+    if (!r) r = upload_identity_V(c);
+    if (!r) r = dev_grow(c->d_ldos, c->c_ldos, (size_t)g.W * g.W, c->gpumem);
+    if (r) {
+        release(c);
+        return r;
+    }
+    *out = c;
+    return 0;
+}
+
+void cqpi_destroy(cqpi_ctx *ctx) { release(ctx); }
+
+int cqpi_set_precision(cqpi_ctx *c, int precision)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (precision != CQPI_PREC_FP64)
+        return fail(CQPI_E_ARG, "precision mode not available");
+    c->precision = precision;
+    return 0;
+}
+
+int cqpi_set_timing(cqpi_ctx *c, int on)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    c->timing = on != 0;
+    return 0;
+}
+
+int cqpi_stage_times(cqpi_ctx *c, double *ms, int64_t *launches)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    for (int s = 0; s < CQPI_NSTAGES; s++) {
+        if (ms) ms[s] = c->stage_ms[s];
+        if (launches) launches[s] = c->stage_n[s];
+    }
+    return 0;
+}
+
+int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda)
+{
+    g_err.clear();
+    if (!c || !scat || !g0) return fail(CQPI_E_ARG, "NULL argument");
+    const Geom &g = c->g;
+    const int B = g.B, NP = B * B, K = g.K;
+    if (tda < (size_t)B) return fail(CQPI_E_ARG, "tda < bands");
+    size_t nwin = (size_t)g.nw * g.nw * NP;
+    // previous work must be done before the pinned staging and window buffer are reused
+    HIPCHK(hipStreamSynchronize(c->stream));
+    reset_stages(c);
+    if (nwin > c->c_hwin) {
+        if (c->h_win) hipHostFree(c->h_win);
+        c->h_win = nullptr;
+        c->c_hwin = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_win, nwin * sizeof(double2)));
+        c->c_hwin = nwin;
+    }
+    if (!c->h_T) HIPCHK(hipHostMalloc((void **)&c->h_T, 16 * 16 * sizeof(double2)));
+    // window of G0(R), R = slot - h wrapped onto the k-grid (src/qpihipkernel.cc:383-393)
+    for (int sx = 0; sx < g.nw; sx++) {
+        int ix = ((sx - g.h) % K + K) % K;
+        for (int sy = 0; sy < g.nw; sy++) {
+            int iy = ((sy - g.h) % K + K) % K;
+            const double *m = g0[(size_t)ix * K + iy];
+            if (!m) return fail(CQPI_E_ARG, "NULL matrix in g0 table");
+            double2 *dst = c->h_win + ((size_t)sx * g.nw + sy) * NP;
+            for (int r = 0; r < B; r++)
+                for (int cc = 0; cc < B; cc++)
+                    dst[r * B + cc] = make_double2(m[2 * (r * tda + cc)], m[2 * (r * tda + cc) + 1]);
+        }
+    }
+    for (int i = 0; i < NP; i++) c->h_T[i] = make_double2(scat[2 * i], scat[2 * i + 1]);
+    RETCHK(dev_grow(c->d_G0w, c->c_G0w, nwin, c->gpumem));
+    RETCHK(dev_grow(c->d_T, c->c_T, (size_t)NP, c->gpumem));
+    RETCHK(dev_grow(c->d_ldos, c->c_ldos, (size_t)g.W * g.W, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_G0w, c->h_win, nwin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_T, c->h_T, NP * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->copy_done, c->stream));
+    RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos));
+    c->nlayers = 1;
+    c->last_chunk_e0 = 0;
+    c->last_chunk_ec = 1;
+    // the caller may overwrite g0 and scat once the copies have landed
+    HIPCHK(hipEventSynchronize(c->copy_done));
+    return 0;
+}
+
+int cqpi_retrieve(cqpi_ctx *c, double *ldos) { return cqpi_retrieve_layers(c, 0, 1, ldos); }
+
+int cqpi_retrieve_layers(cqpi_ctx *c, size_t first, size_t count, double *ldos)
+{
+    g_err.clear();
+    if (!c || !ldos) return fail(CQPI_E_ARG, "NULL argument");
+    if (first + count > c->nlayers) return fail(CQPI_E_STATE, "layer range beyond the last run");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_collect(c);
+    size_t WW = (size_t)c->g.W * c->g.W;
+    HIPCHK(hipMemcpy(ldos, c->d_ldos + first * WW, count * WW * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cqpi_device_ldos(cqpi_ctx *c, const double **dptr, size_t *count)
+{
+    if (!c || !dptr) return fail(CQPI_E_ARG, "NULL argument");
+    *dptr = c->d_ldos;
+    if (count) *count = c->nlayers * (size_t)c->g.W * c->g.W;
+    return 0;
+}
+
+int cqpi_printinfo(cqpi_ctx *c, char *buf, size_t len)
+{
+    if (!c || !buf || !len) return fail(CQPI_E_ARG, "NULL argument");
+    hipDeviceProp_t props;
+    std::string name = "unknown";
+    if (hipGetDeviceProperties(&props, c->device) == hipSuccess) name = props.name;
+    char tmp[1024];
+    snprintf(tmp, sizeof tmp,
+             "Running continuum QPI on HIP GPU %d: %s (%s)\n"
+             "Additional memory requirements:\n"
+             "GPU memory:       %.2fGB\n"
+             "Map:              %dx%d pixels, %d bands, window %d, oversamp %d, k-grid %d\n"
+             "G0 window:        %dx%d (R in [-%d,%d])\n",
+             c->device, name.c_str(), cqpi_version(), (double)c->gpumem / 1073741824.0, c->g.W, c->g.W,
+             c->g.B, c->g.w, c->g.os, c->g.K, c->g.nw, c->g.nw, c->g.h, c->g.h);
+    strncpy(buf, tmp, len - 1);
+    buf[len - 1] = 0;
+    return 0;
+}
+
+int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double efermi, int scmodel)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (nhops && !hops) return fail(CQPI_E_ARG, "hops is NULL");
+    const int B = c->g.B;
+    // group by lattice vector (rx, ry); kz = 0 so rz drops out of the phase
+    std::map<std::pair<double, double>, std::vector<DevHop>> groups;
+    std::vector<std::pair<double, double>> order;
+    for (size_t i = 0; i < nhops; i++) {
+        const cqpi_hopping &h = hops[i];
+        if (h.o1 < 0 || h.o2 < 0 || h.o1 >= B || h.o2 >= B)
+            return fail(CQPI_E_ARG, "hopping orbital index out of range at term " + std::to_string(i));
+        auto key = std::make_pair(h.rx + 0.0, h.ry + 0.0);
+        auto it = groups.find(key);
+        if (it == groups.end()) {
+            order.push_back(key);
+            it = groups.emplace(key, std::vector<DevHop>()).first;
+        }
+        DevHop d;
+        d.o1 = h.o1;
+        d.o2 = h.o2;
+        d.tre = h.tre;
+        d.tim = h.tim;
+        d.anti = h.anti ? 1 : 0;
+        d.pad = 0;
+        it->second.push_back(d);
+    }
+    std::vector<DevRGroup> G;
+    std::vector<DevHop> H;
+    for (auto &key : order) {
+        auto &v = groups[key];
+        DevRGroup r;
+        r.rx = key.first;
+        r.ry = key.second;
+        r.first = (int)H.size();
+        r.count = (int)v.size();
+        G.push_back(r);
+        H.insert(H.end(), v.begin(), v.end());
+    }
+    RETCHK(dev_grow(c->d_groups, c->c_groups, std::max<size_t>(G.size(), 1), c->gpumem));
+    RETCHK(dev_grow(c->d_hops, c->c_hops, std::max<size_t>(H.size(), 1), c->gpumem));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!G.empty()) HIPCHK(hipMemcpy(c->d_groups, G.data(), G.size() * sizeof(DevRGroup), hipMemcpyHostToDevice));
+    if (!H.empty()) HIPCHK(hipMemcpy(c->d_hops, H.data(), H.size() * sizeof(DevHop), hipMemcpyHostToDevice));
+    c->ngroups = (int)G.size();
+    c->efermi = efermi;
+    c->scmax = scmodel ? (B >> 1) : B;
+    c->has_model = true;
+    c->eigen_valid = false;
+    return 0;
+}
+
+int cqpi_set_impurity(cqpi_ctx *c, const double *V)
+{
+    g_err.clear();
+    if (!c || !V) return fail(CQPI_E_ARG, "NULL argument");
+    int B = c->g.B;
+    RETCHK(dev_grow(c->d_V, c->c_V, (size_t)B * B, c->gpumem));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_V, V, (size_t)B * B * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cqpi_compute_eigen(cqpi_ctx *c)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    const Geom &g = c->g;
+    size_t KK = (size_t)g.K * g.K;
+    RETCHK(dev_grow(c->d_eval, c->c_eval, KK * g.B, c->gpumem));
+    RETCHK(dev_grow(c->d_evec, c->c_evec, KK * g.B * g.B, c->gpumem));
+    int st = stage_begin(c, 0);
+    RETCHK(launch_eigen(g.B, c->d_groups, c->ngroups, c->d_hops, c->efermi, c->scmax, g.K, c->d_eval,
+                        c->d_evec, c->stream));
+    stage_end(c, st);
+    c->eigen_valid = true;
+    return 0;
+}
+
+int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ldos_host)
+{
+    g_err.clear();
+    if (!c || (!omegas && E)) return fail(CQPI_E_ARG, "NULL argument");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    if (E == 0) return fail(CQPI_E_ARG, "no energies");
+    const Geom &g = c->g;
+    const int NP = g.B * g.B;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_collect(c);
+    reset_stages(c);
+    if (!c->eigen_valid) RETCHK(cqpi_compute_eigen(c));
+    size_t WW = (size_t)g.W * g.W;
+    RETCHK(dev_grow(c->d_ldos, c->c_ldos, E * WW, c->gpumem));
+    RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
+    RETCHK(dev_grow(c->d_T, c->c_T, E * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_status, c->c_status, E, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_omega, omegas, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // energies per chunk: intermediate of the two-pass transform kept under ~2 GiB
+    size_t per_e = (size_t)g.K * g.nw * NP * sizeof(double2);
+    size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
+    RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
+    for (size_t e0 = 0; e0 < E; e0 += Ec) {
+        int ec = (int)std::min(Ec, E - e0);
+        int st = stage_begin(c, 1);
+        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, eta, c->d_tw, c->d_F1,
+                                   c->stream));
+        stage_end(c, st);
+        st = stage_begin(c, 2);
+        RETCHK(launch_transform_kx(g, c->d_F1, ec, c->d_tw, c->d_G0w, c->stream));
+        stage_end(c, st);
+        st = stage_begin(c, 3);
+        RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
+        stage_end(c, st);
+        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW));
+        c->last_chunk_e0 = e0;
+        c->last_chunk_ec = ec;
+    }
+    c->nlayers = E;
+    std::vector<int> status(E);
+    HIPCHK(hipMemcpyAsync(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_collect(c);
+    for (size_t e = 0; e < E; e++)
+        if (status[e])
+            return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
+    if (ldos_host) HIPCHK(hipMemcpy(ldos_host, c->d_ldos, E * WW * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cqpi_window_half(cqpi_ctx *c, size_t *h)
+{
+    if (!c || !h) return fail(CQPI_E_ARG, "NULL argument");
+    *h = (size_t)c->g.h;
+    return 0;
+}
+
+int cqpi_retrieve_g0_window(cqpi_ctx *c, size_t e, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(CQPI_E_ARG, "NULL argument");
+    if (e < c->last_chunk_e0 || e >= c->last_chunk_e0 + c->last_chunk_ec)
+        return fail(CQPI_E_STATE, "G0 window is kept for the last energy chunk only");
+    size_t nwin = (size_t)c->g.nw * c->g.nw * c->g.B * c->g.B;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_G0w + (e - c->last_chunk_e0) * nwin, nwin * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cqpi_retrieve_tmatrix(cqpi_ctx *c, size_t e, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(CQPI_E_ARG, "NULL argument");
+    if (e >= c->nlayers) return fail(CQPI_E_STATE, "energy index beyond the last run");
+    size_t NP = (size_t)c->g.B * c->g.B;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_T + e * NP, NP * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void cqpi_free(void *p) { free(p); }
+
+}  // extern "C"

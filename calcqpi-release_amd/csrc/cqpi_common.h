@@ -1,0 +1,91 @@
+// Shared device/host helpers of the MI355X continuum-LDOS library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/cqpi_amd.h"
+
+// complex FP64 as double2 (x = re, y = im)
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__host__ __device__ inline double2 cmulc(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+// 1 / z
+__host__ __device__ inline double2 cinv(double2 z)
+{
+    double d = 1.0 / (z.x * z.x + z.y * z.y);
+    return make_double2(z.x * d, -z.y * d);
+}
+__host__ __device__ inline double2 cdiv(double2 a, double2 b) { return cmul(a, cinv(b)); }
+
+// Hopping terms grouped by lattice vector: every term of group g shares
+// (rx, ry, rz) and the phase exp(+-2 pi i R.k) is evaluated once per group.
+struct DevHop {
+    int o1, o2;
+    double tre, tim;
+    int anti;
+    int pad;
+};
+struct DevRGroup {
+    double rx, ry;
+    int first, count;
+};
+
+// Geometry shared by the transform, T-matrix and contraction kernels.
+struct Geom {
+    int K;        // k-grid side
+    int n;        // cells per side
+    int w;        // neighbour window
+    int os;       // oversampling
+    int B;        // bands
+    int maxb;     // bands carrying a Wannier function
+    int spin;     // same-spin contraction blocks
+    int h;        // window half width: G0(R) stored for R in [-h, h]^2
+    int nw;       // 2h+1
+    int T;        // tile side (2w+1)*os
+    int W;        // map side os*n
+};
+
+// launchers (defined next to their kernels)
+namespace cqpi {
+
+int launch_eigen(int B, const DevRGroup *groups, int ngroups, const DevHop *hops, double efermi,
+                 int scmax, int K, double *eval, double2 *evec, hipStream_t s);
+
+// resolvent G0(k,w) from eigen data fused with the transform along ky; writes
+// F1[e][i][slot][pair] for the Ry window.  omegas: device array of the chunk.
+int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
+                        const double *omegas, int E, double eta, const double2 *tw,
+                        double2 *F1, hipStream_t s);
+// transform along kx, window extraction and 1/K^2: G0w[e][sx][sy][pair]
+int launch_transform_kx(const Geom &g, const double2 *F1, int E, const double2 *tw,
+                        double2 *G0w, hipStream_t s);
+// T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular
+int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T,
+                   int *status, hipStream_t s);
+// periodic (bulk) term per sub-pixel: bulk[e][ii*os+jj]
+int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, double *bulk,
+                hipStream_t s);
+// contraction: ldos[e][x + y*W]
+int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
+                         const double *tiles, int E, double *ldos, hipStream_t s);
+int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
+                         const float *tiles32, int E, double *ldos, float2 *scratch32,
+                         hipStream_t s);
+int launch_contract_mixed(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
+                          const void *tilepack, int E, double *ldos, void *work, size_t worksz,
+                          hipStream_t s);
+
+}  // namespace cqpi

@@ -1,0 +1,127 @@
+// T-matrix and the periodic (bulk) term of the LDOS (FP64).
+//
+// T = (1 - V G0(R=0))^{-1} V   (tmatrix::setscatteringmatrix, src/qpi.cc:833-854)
+// solved per energy by Gaussian elimination with partial pivoting on
+// [1 - V G0(0) | V].
+//
+// The bulk term of the reference's contraction,
+//   -Im sum_{R1,R2,o1,o2} w_o1(x-R1) w_o2(x-R2) G0(R2-R1)_{o2,o1}
+// (src/qpi.cc:921, src/qpihipkernel.cc:94), depends on the sub-pixel (ii,jj)
+// only, not on the cell: it is computed once per energy as an os*os table.
+#include "cqpi_common.h"
+
+namespace {
+
+constexpr int kMaxB = 16;
+
+__global__ void k_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
+                          int E, double2 *__restrict__ Tout, int *__restrict__ status)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int B = g.B, NP = B * B;
+    const double2 *G = G0w + ((size_t)e * g.nw * g.nw + (size_t)g.h * g.nw + g.h) * NP;
+    double2 M[kMaxB * kMaxB], X[kMaxB * kMaxB];
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) {
+            double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+            for (int k = 0; k < B; k++) acc = csub(acc, cmul(V[r * B + k], G[k * B + c]));
+            M[r * kMaxB + c] = acc;
+            X[r * kMaxB + c] = V[r * B + c];
+        }
+    int singular = 0;
+    for (int j = 0; j < B && !singular; j++) {
+        int piv = j;
+        double best = cabs2(M[j * kMaxB + j]);
+        for (int i = j + 1; i < B; i++) {
+            double v = cabs2(M[i * kMaxB + j]);
+            if (v > best) { best = v; piv = i; }
+        }
+        if (best == 0.0) { singular = 1; break; }
+        if (piv != j)
+            for (int c = 0; c < B; c++) {
+                double2 t = M[j * kMaxB + c]; M[j * kMaxB + c] = M[piv * kMaxB + c]; M[piv * kMaxB + c] = t;
+                t = X[j * kMaxB + c]; X[j * kMaxB + c] = X[piv * kMaxB + c]; X[piv * kMaxB + c] = t;
+            }
+        double2 invp = cinv(M[j * kMaxB + j]);
+        for (int i = 0; i < B; i++) {
+            if (i == j) continue;
+            double2 l = cmul(M[i * kMaxB + j], invp);
+            if (l.x == 0.0 && l.y == 0.0) continue;
+            for (int c = j; c < B; c++) M[i * kMaxB + c] = csub(M[i * kMaxB + c], cmul(l, M[j * kMaxB + c]));
+            for (int c = 0; c < B; c++) X[i * kMaxB + c] = csub(X[i * kMaxB + c], cmul(l, X[j * kMaxB + c]));
+        }
+        for (int c = 0; c < B; c++) X[j * kMaxB + c] = cmul(X[j * kMaxB + c], invp);
+        for (int c = j; c < B; c++) M[j * kMaxB + c] = cmul(M[j * kMaxB + c], invp);
+    }
+    double2 *T = Tout + (size_t)e * NP;
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) T[r * B + c] = X[r * kMaxB + c];
+    status[e] = singular;
+}
+
+constexpr int kBulkThreads = 256;
+
+__global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__restrict__ G0w,
+                                                      const double *__restrict__ tiles,
+                                                      double *__restrict__ bulk)
+{
+    __shared__ double red[kBulkThreads];
+    const int pix = blockIdx.x, e = blockIdx.y;
+    const int os = g.os, w = g.w, T = g.T, B = g.B, NP = B * B, nw = g.nw, h = g.h;
+    const int ii = pix / os, jj = pix - ii * os;
+    const int x = ii + w * os, y = jj + w * os;
+    const int side = 2 * w + 1, NW = side * side;
+    const int nsets = g.spin ? 2 : 1;
+    const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < NW * NW; t += blockDim.x) {
+        int a1 = t / NW, a2 = t - a1 * NW;
+        int r1x = a1 / side - w, r1y = a1 % side - w;
+        int r2x = a2 / side - w, r2y = a2 % side - w;
+        const double2 *G = G0 + ((size_t)(r2x - r1x + h) * nw + (r2y - r1y + h)) * NP;
+        int x1 = x - r1x * os, y1 = y - r1y * os, x2 = x - r2x * os, y2 = y - r2y * os;
+        for (int sg = 0; sg < nsets; sg++) {
+            int off = sg * nper;
+            for (int o1 = off; o1 < off + nper; o1++) {
+                double f1 = tiles[((size_t)o1 * T + x1) * T + y1];
+                if (f1 == 0.0) continue;
+                for (int o2 = off; o2 < off + nper; o2++) {
+                    double f2 = tiles[((size_t)o2 * T + x2) * T + y2];
+                    acc += f1 * f2 * G[o2 * B + o1].y;
+                }
+            }
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBulkThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bulk[(size_t)e * os * os + pix] = red[0];
+}
+
+}  // namespace
+
+namespace cqpi {
+
+int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T, int *status,
+                   hipStream_t s)
+{
+    if (g.B > kMaxB) return CQPI_E_ARG;
+    int threads = 64;
+    hipLaunchKernelGGL(k_tmatrix, dim3((E + threads - 1) / threads), dim3(threads), 0, s, g, G0w, V, E,
+                       T, status);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, double *bulk,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bulk, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+}  // namespace cqpi

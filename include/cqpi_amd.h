@@ -1,0 +1,182 @@
+/*
+ * cqpi_amd.h -- C ABI of the MI355X continuum-LDOS library (libcqpi_amd.so).
+ *
+ * Plain C types only: sizes, pointers to double (complex numbers are
+ * interleaved re,im pairs), status codes.  Every entry point returns 0 on
+ * success and a nonzero CQPI_E_* code on failure; cqpi_last_error() then
+ * holds a message (per thread).
+ *
+ * The first group replaces the reference's GPU plug-in, class GPUQPI
+ * (include/gpuqpi.h:18-32) as implemented by HipQPI (src/qpihipkernel.h:64-88,
+ * src/qpihipkernel.cc:395-704).  The second group extends it: the whole
+ * chain (eigensolve, resolvent, k->R transform, T-matrix, contraction) runs on
+ * the device, which the reference does on the host in tmatrix
+ * (src/qpi.cc:454-486, :557-578, :833-854).
+ *
+ * Layouts:
+ *   complex b x b matrices: row-major, element (r,c) at [2*(r*ld+c)] with a
+ *     leading dimension ld (= b unless stated)
+ *   tiles wf: maxband x T x T doubles, T = (2*window+1)*oversamp, element
+ *     (o,x,y) at [(o*T+x)*T+y]  (HipQPI packs IDX3C(o,x,y,T), src/qpihipkernel.cc:414-418)
+ *   LDOS map: W x W doubles, W = oversamp*n, element (x,y) at [x + y*W]
+ *     (src/qpihipkernel.cc:96, src/qpi.cc:821-831); a batch of E maps is
+ *     E consecutive maps
+ *
+ * Threading: a context is used by one host thread at a time (the reference's
+ * HipQPI is not thread-safe either).  All device work of a context runs on
+ * the context's own HIP stream.
+ */
+#ifndef CQPI_AMD_H
+#define CQPI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CQPI_OK = 0,
+    CQPI_E_ARG = 1,      /* invalid argument / shape */
+    CQPI_E_HIP = 2,      /* HIP runtime error (allocation, launch, copy) */
+    CQPI_E_STATE = 3,    /* call out of order (e.g. run before set_model) */
+    CQPI_E_SINGULAR = 4, /* 1 - V G0(0) singular at some energy */
+    CQPI_E_IO = 5,       /* file could not be read or parsed */
+    CQPI_E_NODEV = 6     /* no HIP device */
+};
+
+/* precision of the continuum contraction (the other stages are always FP64) */
+enum {
+    CQPI_PREC_FP64 = 0,  /* FP64 everywhere */
+    CQPI_PREC_FP32 = 1,  /* contraction in FP32 */
+    CQPI_PREC_MIXED = 2  /* contraction on MFMA: split-FP16 hi+lo operands, FP32 accumulation */
+};
+
+/* One hopping term, orbitals 0-based, amplitude already divided by the
+ * Wannier90 degeneracy (hoppingterm, src/tightbinding.h:111-164). 56 bytes. */
+typedef struct {
+    double rx, ry, rz;
+    int32_t o1, o2;
+    double tre, tim;
+    int32_t anti;     /* 1: conjugate phase, both orbitals in the hole block */
+    int32_t reserved;
+} cqpi_hopping;
+
+typedef struct cqpi_ctx cqpi_ctx;
+
+/* ---------------------------------------------------------------------- */
+/* Status                                                                  */
+/* ---------------------------------------------------------------------- */
+/* Message of the last failed call on this thread ("" if none). */
+const char *cqpi_last_error(void);
+/* Library version string. */
+const char *cqpi_version(void);
+/* Number of visible HIP devices (0 if none); never fails. */
+int cqpi_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* GPUQPI drop-in (one energy per call, host-computed G0 and T)            */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces HipQPI::HipQPI(wanniern, kpoints, n, window, oversamp, bands,
+ * maxband, spin, wf) (src/qpihipkernel.h:75, src/qpihipkernel.cc:395-429).
+ * wanniern must equal oversamp*n.  wf: maxband tiles (layout above); zero
+ * tiles stand for orbitals without a Wannier function.  Allocates the device
+ * buffers and uploads the tiles once. */
+int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp,
+                size_t bands, size_t maxband, int spin, const double *wf, cqpi_ctx **out);
+
+/* Replaces HipQPI::wannierldos(scat, g0) (include/gpuqpi.h:23,
+ * src/qpihipkernel.cc:577-598).  scat: T, b x b complex row-major (ld=b).
+ * g0: table of kpoints*kpoints pointers, entry [ix*kpoints+iy] pointing to the
+ * b x b complex matrix G0(R = (ix, iy) mod kpoints) with leading dimension
+ * tda (complex elements), i.e. exactly the reference's
+ * vector<vector<gsl_matrix_complex*>> g0[ix][iy] with ->data and ->tda.
+ * Only the window of R actually used is read.  The data is copied before the
+ * call returns (the caller overwrites g0 for the next energy, src/calcqpi.cc:721-750);
+ * the contraction runs asynchronously. */
+int cqpi_wannierldos(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda);
+
+/* Replaces HipQPI::retrieveResult(double *ldos) (src/qpihipkernel.cc:690-704):
+ * waits for the device and copies the W x W map of the last wannierldos call
+ * (or layer 0 of the last cqpi_run) to ldos. */
+int cqpi_retrieve(cqpi_ctx *ctx, double *ldos);
+
+/* Replaces HipQPI::printinfo (src/qpihipkernel.cc:535-560): device name and
+ * memory use as text; writes at most len bytes including the terminator. */
+int cqpi_printinfo(cqpi_ctx *ctx, char *buf, size_t len);
+
+/* Replaces HipQPI::~HipQPI (src/qpihipkernel.cc:562-575).  NULL is a no-op. */
+void cqpi_destroy(cqpi_ctx *ctx);
+
+/* Sets the contraction precision (CQPI_PREC_*) for later calls. */
+int cqpi_set_precision(cqpi_ctx *ctx, int precision);
+
+/* ---------------------------------------------------------------------- */
+/* Whole chain on the device (extension of the plug-in)                    */
+/* ---------------------------------------------------------------------- */
+
+/* Tight-binding model: H(k) = sum t exp(+-2 pi i R.k) - efermi (hole block
+ * +efermi when scmodel), src/tightbinding.cc:210-219.  Invalidates cached
+ * eigen data.  nhops may be 0 (then H(k) is the constant diagonal). */
+int cqpi_set_model(cqpi_ctx *ctx, const cqpi_hopping *hops, size_t nhops, double efermi,
+                   int scmodel);
+
+/* Impurity potential V (b x b complex row-major).  The reference's V is
+ * diagonal, V = diag(vscat) (src/qpi.cc:840-841, src/qpi.h:107-130); a full
+ * matrix is accepted here. */
+int cqpi_set_impurity(cqpi_ctx *ctx, const double *V);
+
+/* Per-k eigensolve of H(k) on the k-grid, cached until the model changes
+ * (tmatrix::calchamiltonian, src/qpi.cc:454-475).  cqpi_run calls it when
+ * needed; calling it explicitly lets a caller time it separately. */
+int cqpi_compute_eigen(cqpi_ctx *ctx);
+
+/* Full cLDOS for E energies with broadening eta: resolvent, k->R transform,
+ * T-matrix and contraction, energies processed in device-sized chunks.  The
+ * maps stay on the device (retrievable with cqpi_retrieve_layers); if
+ * ldos_host is not NULL the E maps are also copied there (E*W*W doubles).
+ * Returns CQPI_E_SINGULAR if 1 - V G0(0) is singular at some energy. */
+int cqpi_run(cqpi_ctx *ctx, const double *omegas, size_t E, double eta, double *ldos_host);
+
+/* Copies maps [first, first+count) of the last cqpi_run to ldos_host. */
+int cqpi_retrieve_layers(cqpi_ctx *ctx, size_t first, size_t count, double *ldos_host);
+
+/* Device pointer of the maps of the last cqpi_run (E*W*W doubles, owned by
+ * the context, valid until the next run or destroy), for zero-copy hand-off. */
+int cqpi_device_ldos(cqpi_ctx *ctx, const double **dptr, size_t *count);
+
+/* Copies G0(R) of energy index e of the last cqpi_run for the window of R
+ * used by the contraction: side nw = 2h+1 with R = slot - h, h as reported
+ * by cqpi_window_half; out is nw*nw*b*b complex, [(sx*nw+sy)*b*b + r*b + c]. */
+int cqpi_window_half(cqpi_ctx *ctx, size_t *h);
+int cqpi_retrieve_g0_window(cqpi_ctx *ctx, size_t e, double *out);
+/* Copies T (b x b complex) of energy index e of the last cqpi_run. */
+int cqpi_retrieve_tmatrix(cqpi_ctx *ctx, size_t e, double *out);
+
+/* Per-stage device time of the last cqpi_run / cqpi_wannierldos, measured
+ * with HIP events on the context stream.  Stages: 0 eigensolve, 1 resolvent +
+ * transform along y, 2 transform along x, 3 T-matrix + periodic term,
+ * 4 contraction.  ms[s] = total milliseconds, launches[s] = launch count. */
+#define CQPI_NSTAGES 5
+int cqpi_stage_times(cqpi_ctx *ctx, double *ms, int64_t *launches);
+/* Enables/disables the per-stage event timing (default off). */
+int cqpi_set_timing(cqpi_ctx *ctx, int on);
+
+/* ---------------------------------------------------------------------- */
+/* Input formats (host)                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* Reads a Wannier90 _hr.dat file (tightbind::readfile, src/tightbinding.cc:70-149).
+ * On success *hops is allocated with malloc (free with cqpi_free) and holds
+ * *nhops terms, *bands is num_wann, name receives line 1 (namelen bytes). */
+int cqpi_read_hr(const char *path, int scmodel, cqpi_hopping **hops, size_t *nhops,
+                 size_t *bands, char *name, size_t namelen);
+void cqpi_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CQPI_AMD_H */

@@ -1,0 +1,195 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+Build with ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+HOP_DTYPE = np.dtype(
+    [("rx", "<f8"), ("ry", "<f8"), ("rz", "<f8"), ("o1", "<i4"), ("o2", "<i4"),
+     ("tre", "<f8"), ("tim", "<f8"), ("anti", "<i4"), ("reserved", "<i4")],
+    align=True,
+)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        sz = C.c_size_t
+        L.or_read_hr.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(sz),
+                                 C.POINTER(sz), C.c_char_p, sz]
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_setmatrix.argtypes = [C.c_void_p, sz, sz, C.c_double, sz, C.c_double, C.c_double, _dp]
+        L.or_eigh.argtypes = [sz, _dp, _dp, _dp]
+        L.or_calchamiltonian.argtypes = [C.c_void_p, sz, sz, C.c_double, sz, sz, _dp, _dp]
+        L.or_setgreensfunction.argtypes = [sz, sz, _dp, _dp, C.c_double, C.c_double, _dp]
+        L.or_realspace.argtypes = [sz, sz, _dp]
+        L.or_tmatrix.argtypes = [sz, _dp, _dp, _dp]
+        L.or_tmatrix.restype = C.c_int
+        L.or_wannierldos.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip, _dp]
+        L.or_cldos_run.argtypes = [C.c_void_p, sz, sz, C.c_double, C.c_int, sz, sz, sz, sz, sz,
+                                   C.c_int, C.c_double, _dp, sz, _dp, _dp, _ip, _dp]
+        L.or_cldos_run.restype = C.c_int
+        L.or_getbias.argtypes = [C.c_double, C.c_double, sz, sz]
+        L.or_getbias.restype = C.c_double
+        L.or_idl_write.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, sz, sz, sz, _dp, _dp]
+        L.or_idl_read_header.argtypes = [C.c_char_p, C.POINTER(sz), _dp]
+        L.or_idl_read.argtypes = [C.c_char_p, _dp, sz]
+        L.or_num_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def read_hr(path, scmodel=False):
+    L = lib()
+    hp = C.c_void_p()
+    nh = C.c_size_t()
+    nb = C.c_size_t()
+    name = C.create_string_buffer(512)
+    err = L.or_read_hr(path.encode(), int(scmodel), C.byref(hp), C.byref(nh), C.byref(nb), name, 512)
+    if err:
+        raise ValueError("or_read_hr failed with code %d" % err)
+    n = nh.value
+    buf = (C.c_char * (n * HOP_DTYPE.itemsize)).from_address(hp.value)
+    hops = np.frombuffer(bytes(buf), dtype=HOP_DTYPE).copy()
+    L.or_free(hp)
+    return hops, nb.value, name.value.decode()
+
+
+def setmatrix(hops, b, efermi, kx, ky, scmodel=False):
+    hops = _c(hops, HOP_DTYPE)
+    H = np.zeros((b, b), np.complex128)
+    scmax = b >> 1 if scmodel else b
+    lib().or_setmatrix(hops.ctypes.data, len(hops), b, efermi, scmax, kx, ky, _d(H.view(np.float64)))
+    return H
+
+
+def eigh(H):
+    H = _c(H, np.complex128)
+    b = H.shape[0]
+    ev = np.zeros(b)
+    V = np.zeros((b, b), np.complex128)
+    lib().or_eigh(b, _d(H.view(np.float64)), _d(ev), _d(V.view(np.float64)))
+    return ev, V
+
+
+def calchamiltonian(hops, b, efermi, K, scmodel=False):
+    hops = _c(hops, HOP_DTYPE)
+    ev = np.zeros((K, K, b))
+    V = np.zeros((K, K, b, b), np.complex128)
+    scmax = b >> 1 if scmodel else b
+    lib().or_calchamiltonian(hops.ctypes.data, len(hops), b, efermi, scmax, K, _d(ev),
+                             _d(V.view(np.float64)))
+    return ev, V
+
+
+def greens_k(ev, V, omega, eta):
+    K, _, b = ev.shape
+    g = np.zeros((K, K, b, b), np.complex128)
+    lib().or_setgreensfunction(K, b, _d(_c(ev, np.float64)), _d(_c(V, np.complex128).view(np.float64)),
+                               omega, eta, _d(g.view(np.float64)))
+    return g
+
+
+def realspace(g0k):
+    g = _c(g0k, np.complex128).copy()
+    K, _, b, _ = g.shape
+    lib().or_realspace(K, b, _d(g.view(np.float64)))
+    return g
+
+
+def tmatrix(V, g00):
+    V = _c(V, np.complex128)
+    b = V.shape[0]
+    T = np.zeros((b, b), np.complex128)
+    err = lib().or_tmatrix(b, _d(V.view(np.float64)), _d(_c(g00, np.complex128).view(np.float64)),
+                           _d(T.view(np.float64)))
+    if err:
+        raise ZeroDivisionError("singular 1 - V G0(0)")
+    return T
+
+
+def wannierldos(g0R, T, tiles, n, window, oversamp, maxband=0, spin=False, wfzero=None):
+    K, _, b, _ = g0R.shape
+    W = n * oversamp
+    out = np.zeros(W * W)
+    tiles = _c(tiles, np.float64)
+    if not maxband:
+        maxband = b
+    wz = None
+    if wfzero is not None:
+        wz = _c(wfzero, np.int32).ctypes.data_as(_ip)
+    lib().or_wannierldos(K, n, window, oversamp, b, maxband, int(spin),
+                         _d(_c(T, np.complex128).view(np.float64)),
+                         _d(_c(g0R, np.complex128).view(np.float64)), _d(tiles), wz, _d(out))
+    return out
+
+
+def cldos_run(wl, energies=None):
+    """Whole FP64 path for a synth.Workload; returns E x W*W (x + y*W)."""
+    energies = wl.energies if energies is None else np.asarray(energies, np.float64)
+    hops = _c(wl.hops, HOP_DTYPE)
+    E = len(energies)
+    out = np.zeros((E, wl.W * wl.W))
+    err = lib().or_cldos_run(hops.ctypes.data, len(hops), wl.bands, wl.efermi, int(wl.scmodel),
+                             wl.K, wl.n, wl.window, wl.oversamp, wl.maxband, int(wl.spin), wl.eta,
+                             _d(_c(energies, np.float64)), E,
+                             _d(_c(wl.V, np.complex128).view(np.float64)),
+                             _d(_c(wl.tiles, np.float64)), None, _d(out))
+    if err:
+        raise ZeroDivisionError("singular 1 - V G0(0)")
+    return out
+
+
+def getbias(lb, hb, layers, i):
+    return lib().or_getbias(lb, hb, layers, i)
+
+
+def idl_write(path, data_xyz, name="", date="", hdr=(1, 1, 0, 0, 0, 0, 0)):
+    sx, sy, sz = data_xyz.shape
+    flat = np.ascontiguousarray(np.transpose(data_xyz, (2, 1, 0)), dtype=np.float64)
+    h = np.array(hdr, np.float64)
+    err = lib().or_idl_write(path.encode(), name.encode(), date.encode(), sx, sy, sz, _d(h), _d(flat))
+    if err:
+        raise IOError("or_idl_write failed")
+
+
+def idl_read(path):
+    L = lib()
+    dims = (C.c_size_t * 3)()
+    h = np.zeros(7)
+    if L.or_idl_read_header(path.encode(), dims, _d(h)):
+        raise IOError("bad idl header")
+    sx, sy, sz = dims
+    flat = np.zeros(sx * sy * sz)
+    if L.or_idl_read(path.encode(), _d(flat), flat.size):
+        raise IOError("bad idl data")
+    return flat.reshape(sz, sy, sx).transpose(2, 1, 0), h
+
+
+def num_threads():
+    return lib().or_num_threads()

@@ -1,0 +1,123 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+symbol include/cqpi_amd.h declares; the library's _hr.dat reader agrees with
+the oracle's restatement; idl files round-trip; error paths report codes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cqpi
+import oracle
+from cqpi import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "cqpi_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cqpi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = cqpi.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(cqpi.SIGNATURES), set(names) ^ set(cqpi.SIGNATURES)
+
+
+def test_hopping_struct_layout():
+    assert synth.HOP_DTYPE.itemsize == 56
+    assert synth.HOP_DTYPE == oracle.HOP_DTYPE
+
+
+def test_no_device_is_reported_not_faked():
+    if cqpi.device_count() > 0:
+        pytest.skip("device present")
+    T = np.zeros((1, 3, 3))
+    with pytest.raises(cqpi.CqpiError) as ei:
+        cqpi.AmdQPI(4, 4, 4, 1, 1, 1, 1, False, T)
+    assert ei.value.code == 6
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3])
+def test_hr_reader_matches_oracle(tmp_path, cfg):
+    wl = {1: synth.config1, 2: lambda: synth.config2(K=8, n=4),
+          3: lambda: synth.config3(K=8, n=4)}[cfg]()
+    p = str(tmp_path / "m_hr.dat")
+    synth.write_hr(p, wl)
+    h1, b1, n1 = cqpi.read_hr(p)
+    h2, b2, n2 = oracle.read_hr(p)
+    assert b1 == b2 == wl.bands and n1 == n2 == wl.name
+    assert len(h1) == len(h2) == len(wl.hops)
+    for f in ("rx", "ry", "rz", "o1", "o2", "tre", "tim", "anti"):
+        assert np.array_equal(h1[f], h2[f]), f
+    # amplitudes survive the text format to the last digit written (%.16e)
+    assert np.allclose(h1["tre"], wl.hops["tre"], rtol=1e-15, atol=1e-300)
+
+
+def test_hr_degeneracy_and_grouping(tmp_path):
+    """Degeneracies spread over several lines divide the amplitudes of their R;
+    R changes are detected line by line."""
+    p = tmp_path / "deg_hr.dat"
+    lines = ["test model", "1", "3", "  2", " 4    1", "0 0 0 1 1 0.8 0.0", "1 0 0 1 1 -1.0 0.0",
+             "1 0 0 1 1 -0.5 0.2", "0 1 0 1 1 2.0 0.0", ""]
+    p.write_text("\n".join(lines))
+    for reader in (cqpi.read_hr, oracle.read_hr):
+        h, b, name = reader(str(p))
+        assert b == 1 and name == "test model"
+        assert np.allclose(h["tre"], [0.4, -0.25, -0.125, 2.0])
+        assert np.allclose(h["tim"], [0.0, 0.0, 0.05, 0.0])
+
+
+def test_hr_scmodel_anti_flag(tmp_path):
+    p = tmp_path / "sc_hr.dat"
+    lines = ["sc", "4", "1", "1"] + ["0 0 0 %d %d 1.0 0.0" % (u, v) for u in range(1, 5) for v in range(1, 5)]
+    p.write_text("\n".join(lines) + "\n")
+    for reader in (cqpi.read_hr, oracle.read_hr):
+        h, _, _ = reader(str(p), scmodel=True)
+        anti = {(int(a), int(b)): int(f) for a, b, f in zip(h["o1"], h["o2"], h["anti"])}
+        assert anti[(2, 3)] == 1 and anti[(0, 3)] == 0 and anti[(1, 1)] == 0
+
+
+@pytest.mark.parametrize("bad", ["badorb", "short", "missing"])
+def test_hr_errors(tmp_path, bad):
+    p = tmp_path / "bad_hr.dat"
+    if bad == "badorb":
+        p.write_text("m\n1\n1\n1\n0 0 0 2 1 1.0 0.0\n")
+    elif bad == "short":
+        p.write_text("m\n1\n1\n1\n0 0 0 1 1 1.0\n")
+    else:
+        p = tmp_path / "does_not_exist"
+    with pytest.raises(cqpi.CqpiError) as ei:
+        cqpi.read_hr(str(p))
+    assert ei.value.code in (1, 5)
+
+
+def test_idl_roundtrip_between_writers(tmp_path):
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=(5, 4, 3)).astype(np.float32).astype(np.float64)
+    p1, p2 = str(tmp_path / "a.idl"), str(tmp_path / "b.idl")
+    # file order: scan range (2), offset (2), bias range (2), setpoint -- as the
+    # cLDOS output map idl(W,W,L,2.0,2.0,lb,hb,0.0,-1.0,-1.0) writes it
+    hdr = (2.0, 2.0, -1.0, -1.0, -0.1, 0.1, 0.0)
+    synth.write_idl(p1, d, name="map", date="today", hdr=hdr)
+    oracle.idl_write(p2, d, name="map", date="today", hdr=hdr)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    got, h = synth.read_idl(p1)
+    assert np.array_equal(got, d) and h["name"] == "map" and h["bias"] == [-0.1, 0.1]
+    got2, h2 = oracle.idl_read(p2)
+    assert np.array_equal(got2, d) and list(h2) == list(hdr)
+
+
+def test_write_inputs(tmp_path):
+    wl = synth.config2(K=16, n=8, E=5)
+    p = synth.write_inputs(str(tmp_path), wl)
+    txt = open(p).read()
+    assert "scattering=(0.5,0.5,0.5);" in txt and "window=2;" in txt
+    tiles, _ = synth.read_idl(str(tmp_path / "orbitals.idl"))
+    assert np.array_equal(np.transpose(tiles, (2, 0, 1)), wl.tiles)

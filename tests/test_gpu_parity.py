@@ -1,0 +1,159 @@
+"""GPU parity: the HIP path against the FP64 oracle (restated reference
+algorithm), called through the C ABI.  FP64 mode must agree to 1e-10 relative
+(max over the map of |rho - rho_oracle| / max(|rho_oracle|, 1e-3 max|rho_oracle|))."""
+import numpy as np
+import pytest
+
+import cqpi
+import oracle
+import refmath as rm
+from cqpi import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _run(wl, precision="fp64"):
+    print("case: %s K=%d n=%d w=%d os=%d E=%d spin=%s prec=%s" % (
+        wl.name, wl.K, wl.n, wl.window, wl.oversamp, len(wl.energies), wl.spin, precision), flush=True)
+    with cqpi.engine_for(wl, precision) as q:
+        return q.run(wl.energies, wl.eta)
+
+
+def _assert_close(got, ref, tol=TOL64):
+    for e in range(ref.shape[0]):
+        mx, rms = rm.rel_errors(got[e], ref[e])
+        assert mx < tol, "energy %d: max_rel %.3e rms_rel %.3e" % (e, mx, rms)
+
+
+def test_full_chain_3orb_small(gpu):
+    wl = synth.config2_like(3, K=32, n=8, E=3, seed=11, w=1, os_=2, elo=-0.1, ehi=0.1, eta=0.01)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_full_chain_config1_reduced(gpu):
+    wl = synth.config1(K=64, n=16, E=3)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_full_chain_config2_shapes_reduced(gpu):
+    # config-2 model, tiles, window and oversampling with a smaller grid and map
+    wl = synth.config2(K=64, n=16, E=3)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_full_chain_spin_hex_reduced(gpu):
+    # config-3 model family (hexagonal, 10 bands, SOC, full Hermitian V, spin blocks)
+    wl = synth.config3(K=16, n=6, E=2)
+    wl.window, wl.oversamp = 1, 2
+    wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_non_power_of_two_kgrid(gpu):
+    wl = synth.config2_like(3, K=24, n=8, E=2, seed=12, w=1, os_=2, eta=0.02)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_map_larger_than_kgrid_odd_n(gpu):
+    # n > K: the map repeats periodic images; odd n exercises the window bounds
+    wl = synth.config2_like(1, K=8, n=11, E=2, seed=13, w=2, os_=2, eta=0.05)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_degenerate_bands(gpu):
+    # two identical decoupled orbitals: every band is exactly two-fold degenerate
+    base = synth.config1(K=32, n=8, E=2)
+    b1 = {(int(h["rx"]), int(h["ry"])): complex(h["tre"], h["tim"]) for h in base.hops}
+    blocks = {R: np.diag([t, t]) for R, t in b1.items()}
+    wl = synth.Workload(name="doubled", bands=2, hops=synth.hops_from_blocks(blocks, 2), K=32, n=8,
+                        window=1, oversamp=2, energies=np.array([-0.2, 0.3]), eta=0.02,
+                        V=np.array([[0.7, 0.2], [0.2, -0.4]], complex),
+                        tiles=np.concatenate([synth.gaussian_tiles(["s"], 0.35, 0.5, 1, 2)] * 2))
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
+def test_g0_window_and_tmatrix(gpu):
+    wl = synth.config2_like(3, K=32, n=8, E=2, seed=14, w=1, os_=2, eta=0.01)
+    with cqpi.engine_for(wl) as q:
+        q.run(wl.energies, wl.eta)
+        h = q.window_half()
+        ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+        for e, om in enumerate(wl.energies):
+            gR = oracle.realspace(oracle.greens_k(ev, V, om, wl.eta))
+            win = q.g0_window(e)
+            idx = (np.arange(2 * h + 1) - h) % wl.K
+            ref = gR[np.ix_(idx, idx)]
+            assert np.max(np.abs(win - ref)) < 1e-12 * np.max(np.abs(ref))
+            T = q.tmatrix(e)
+            Tref = oracle.tmatrix(wl.V, gR[0, 0])
+            assert np.max(np.abs(T - Tref)) < 1e-12 * np.max(np.abs(Tref))
+
+
+def test_dropin_wannierldos_matches_oracle(gpu):
+    """GPUQPI::wannierldos path: host G0(R) and T in, one map out."""
+    wl = synth.config2(K=32, n=12, E=1)
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    ref = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
+    try:
+        q.wannierldos(T, gR)
+        gR[:] = 0  # the caller may overwrite g0 once wannierldos returned
+        got = q.retrieveResult()
+        assert "HIP GPU" in q.printinfo()
+    finally:
+        q.close()
+    mx, _ = rm.rel_errors(got, ref)
+    assert mx < TOL64
+
+
+def test_dropin_repeated_calls_and_zero_tile(gpu):
+    """Two energies through the drop-in path, with one orbital's tile zero
+    (the reference pads missing orbitals with zero functions)."""
+    wl = synth.config2(K=32, n=8, E=2)
+    tiles = wl.tiles.copy()
+    tiles[1] = 0.0
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, False, tiles)
+    try:
+        for om in wl.energies:
+            gR = oracle.realspace(oracle.greens_k(ev, V, om, wl.eta))
+            T = oracle.tmatrix(wl.V, gR[0, 0])
+            ref = oracle.wannierldos(gR, T, tiles, wl.n, wl.window, wl.oversamp,
+                                     wfzero=np.array([0, 1, 0], np.int32))
+            q.wannierldos(T, gR)
+            mx, _ = rm.rel_errors(q.retrieveResult(), ref)
+            assert mx < TOL64
+    finally:
+        q.close()
+
+
+def test_no_impurity_gives_periodic_map(gpu):
+    """V = 0: rho is the lattice-periodic bulk term in every cell."""
+    wl = synth.config2(K=32, n=8, E=1)
+    wl.V = np.zeros((3, 3), complex)
+    rho = _run(wl)[0].reshape(wl.W, wl.W)  # [y, x]
+    cell = rho[: wl.oversamp, : wl.oversamp]
+    tiled = np.tile(cell, (wl.n, wl.n))
+    assert np.max(np.abs(rho - tiled)) < 1e-12 * np.max(np.abs(rho))
+
+
+def test_singular_tmatrix_reported(gpu):
+    """1 - V G0(0) singular: reported as an error, not a silent result."""
+    wl = synth.config1(K=16, n=4, E=1)
+    ev, V = oracle.calchamiltonian(wl.hops, 1, 0.0, wl.K)
+    g00 = oracle.realspace(oracle.greens_k(ev, V, 0.2, 0.05))[0, 0, 0, 0]
+    wl.V = np.array([[1.0 / g00]])  # makes 1 - V G0(0) exactly zero up to rounding
+    wl.energies = np.array([0.2])
+    wl.eta = 0.05
+    with cqpi.engine_for(wl) as q:
+        try:
+            out = q.run(wl.energies, wl.eta)
+        except cqpi.CqpiError as e:
+            assert e.code == 4
+        else:
+            # rounding may leave a tiny nonzero pivot; then the map is huge but finite
+            assert np.all(np.isfinite(out))

@@ -1,0 +1,161 @@
+"""Pins of the FP64 oracle (CPU).  The reference holds no golden vectors, so
+each oracle function is checked against mathematics that fixes it:
+LAPACK inverses, numpy's inverse FFT, closed forms of the one-band model, a
+brute-force dense inverse of the periodic supercell with the impurity, and
+the factorised contraction.  Each check is chosen so that a dropped term, a
+wrong sign or a transposed index fails it."""
+import numpy as np
+import pytest
+
+import oracle
+import refmath as rm
+from cqpi import synth
+
+
+@pytest.fixture(scope="module")
+def small3():
+    wl = synth.config2_like(3, K=8, n=8, E=1, seed=7, w=1, os_=2, eta=0.01)
+    wl.efermi = 0.1
+    wl.V = np.array([[0.5, 0.1 - 0.2j, 0], [0.1 + 0.2j, 0.3, 0.05], [0, 0.05, 0.2]])
+    return wl
+
+
+def test_setmatrix_closed_form_one_band():
+    wl = synth.config1()
+    for kx, ky in [(0.0, 0.0), (0.25, -0.125), (0.5, 0.5), (-0.3, 0.1)]:
+        H = oracle.setmatrix(wl.hops, 1, 0.2, kx, ky)
+        eps = -0.5 + 2 * (-1.0) * (np.cos(2 * np.pi * kx) + np.cos(2 * np.pi * ky)) \
+            + 4 * 0.3 * np.cos(2 * np.pi * kx) * np.cos(2 * np.pi * ky) - 0.2
+        assert abs(H[0, 0] - eps) < 1e-13
+
+
+def test_setmatrix_phase_sign_and_anti():
+    # a single complex hopping at R=(1,0): +2 pi i phase, conjugated when 'anti'
+    hops = np.zeros(1, synth.HOP_DTYPE)
+    hops[0] = (1.0, 0.0, 0.0, 0, 1, 0.3, 0.1, 0, 0)
+    H = oracle.setmatrix(hops, 2, 0.0, 0.1, 0.0)
+    assert abs(H[0, 1] - (0.3 + 0.1j) * np.exp(2j * np.pi * 0.1)) < 1e-15
+    hops[0]["anti"] = 1
+    H = oracle.setmatrix(hops, 2, 0.0, 0.1, 0.0)
+    assert abs(H[0, 1] - (0.3 + 0.1j) * np.exp(-2j * np.pi * 0.1)) < 1e-15
+
+
+def test_eigh_reconstructs_and_matches_lapack():
+    rng = np.random.default_rng(3)
+    for b in (1, 2, 3, 5, 10):
+        A = rng.normal(size=(b, b)) + 1j * rng.normal(size=(b, b))
+        H = A + A.conj().T
+        ev, V = oracle.eigh(H)
+        assert np.allclose(V @ np.diag(ev) @ V.conj().T, H, atol=1e-13 * np.abs(H).max())
+        assert np.allclose(V.conj().T @ V, np.eye(b), atol=1e-14)
+        assert np.allclose(np.sort(ev), np.linalg.eigvalsh(H), atol=1e-13 * np.abs(H).max())
+
+
+def test_eigh_degenerate():
+    H = np.diag([1.0, 1.0, -2.0, 1.0]).astype(complex)
+    U = np.linalg.qr(np.random.default_rng(1).normal(size=(4, 4)) + 0j)[0]
+    H = U @ H @ U.conj().T
+    ev, V = oracle.eigh(H)
+    assert np.allclose(np.sort(ev), [-2, 1, 1, 1], atol=1e-13)
+    assert np.allclose(V @ np.diag(ev) @ V.conj().T, H, atol=1e-13)
+
+
+def test_greens_k_matches_inverse(small3):
+    wl = small3
+    ev, V = oracle.calchamiltonian(wl.hops, 3, wl.efermi, wl.K)
+    g = oracle.greens_k(ev, V, 0.05, wl.eta)
+    ref = rm.g0k_inv(wl.hops, 3, wl.efermi, wl.K, 0.05, wl.eta)
+    assert np.max(np.abs(g - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("K", [8, 16, 6, 12])
+def test_realspace_matches_numpy_ifft2(K):
+    rng = np.random.default_rng(K)
+    g = rng.normal(size=(K, K, 2, 2)) + 1j * rng.normal(size=(K, K, 2, 2))
+    got = oracle.realspace(g)
+    ref = np.fft.ifft2(g, axes=(0, 1))
+    assert np.max(np.abs(got - ref)) < 1e-14 * np.max(np.abs(ref)) * K
+
+
+def test_realspace_sum_rule_and_one_band_closed_form():
+    # sum_R G0(R) = G0(k=0); one band: G0(R) = 1/K^2 sum_k e^{2 pi i kR}/(z - eps_k)
+    wl = synth.config1(K=16)
+    ev, V = oracle.calchamiltonian(wl.hops, 1, 0.0, 16)
+    gk = oracle.greens_k(ev, V, 0.1, 0.05)
+    gR = oracle.realspace(gk)
+    assert abs(gR.sum(axis=(0, 1))[0, 0] - gk[0, 0, 0, 0]) < 1e-13
+    k = rm.kgrid(16)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    eps = -0.5 - 2 * (np.cos(2 * np.pi * KX) + np.cos(2 * np.pi * KY)) \
+        + 1.2 * np.cos(2 * np.pi * KX) * np.cos(2 * np.pi * KY)
+    for R in [(0, 0), (1, 0), (2, -3)]:
+        val = np.sum(np.exp(2j * np.pi * (KX * R[0] + KY * R[1])) / (0.1 + 0.05j - eps)) / 256
+        assert abs(gR[R[0] % 16, R[1] % 16, 0, 0] - val) < 1e-14
+
+
+def test_tmatrix_dyson_identity():
+    rng = np.random.default_rng(5)
+    b = 4
+    V = rng.normal(size=(b, b)) + 1j * rng.normal(size=(b, b))
+    V = V + V.conj().T
+    G = rng.normal(size=(b, b)) + 1j * rng.normal(size=(b, b))
+    T = oracle.tmatrix(V, G)
+    assert np.allclose(T, V + V @ G @ T, atol=1e-12)
+    # one band closed form
+    T1 = oracle.tmatrix(np.array([[1.0]]), np.array([[0.2 - 0.3j]]))
+    assert abs(T1[0, 0] - 1.0 / (1 - (0.2 - 0.3j))) < 1e-15
+
+
+def test_contraction_matches_factorised(small3):
+    wl = small3
+    ev, V = oracle.calchamiltonian(wl.hops, 3, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, 0.05, wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    got = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
+    ref = rm.ldos_factorised(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
+    assert rm.rel_errors(got, ref)[0] < 1e-12
+
+
+def test_contraction_spin_blocks_factorised():
+    wl = synth.config2_like(3, K=6, n=6, E=1, seed=21, w=1, os_=2)
+    # 4 bands, spin: blocks {0,1} and {2,3}
+    rng = np.random.default_rng(2)
+    gR = rng.normal(size=(6, 6, 4, 4)) + 1j * rng.normal(size=(6, 6, 4, 4))
+    T = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    tiles = rng.normal(size=(4, 6, 6))
+    got = oracle.wannierldos(gR, T, tiles, 6, 1, 2, maxband=4, spin=True)
+    ref = rm.ldos_factorised(gR, T, tiles, 6, 1, 2, maxband=4, spin=True)
+    assert rm.rel_errors(got, ref)[0] < 1e-12
+
+
+def test_whole_path_matches_supercell_bruteforce(small3):
+    """Dense inverse of the K x K periodic supercell with the impurity on site 0
+    pins eigensolve, resolvent, transform, T-matrix and contraction together."""
+    wl = small3
+    om = 0.05
+    got = oracle.cldos_run(wl, energies=[om])[0]
+    ref = rm.ldos_bruteforce(wl.hops, 3, wl.efermi, wl.K, wl.n, wl.window, wl.oversamp, om, wl.eta,
+                             wl.V, wl.tiles)
+    mx, rms = rm.rel_errors(got, ref)
+    assert mx < 1e-11, (mx, rms)
+
+
+def test_bruteforce_detects_planted_mistake(small3):
+    """A wrong sign of the bulk term in the restatement would be caught."""
+    wl = small3
+    ev, V = oracle.calchamiltonian(wl.hops, 3, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, 0.05, wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    good = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
+    # bulk-only map (T=0) subtracted twice emulates a flipped bulk sign
+    bulk = oracle.wannierldos(gR, np.zeros_like(T), wl.tiles, wl.n, wl.window, wl.oversamp)
+    planted = good - 2 * bulk
+    ref = rm.ldos_bruteforce(wl.hops, 3, wl.efermi, wl.K, wl.n, wl.window, wl.oversamp, 0.05, wl.eta,
+                             wl.V, wl.tiles)
+    assert rm.rel_errors(planted, ref)[0] > 1e-3
+
+
+def test_getbias():
+    assert oracle.getbias(-0.1, 0.1, 51, 0) == -0.1
+    assert abs(oracle.getbias(-0.1, 0.1, 51, 50) - 0.1) < 1e-17
+    assert oracle.getbias(0.3, 0.5, 1, 0) == 0.3
