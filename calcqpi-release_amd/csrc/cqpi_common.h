@@ -81,11 +81,4 @@ int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, d
 // contraction: ldos[e][x + y*W]
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                          const double *tiles, int E, double *ldos, hipStream_t s);
-int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                         const float *tiles32, int E, double *ldos, float2 *scratch32,
-                         hipStream_t s);
-int launch_contract_mixed(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                          const void *tilepack, int E, double *ldos, void *work, size_t worksz,
-                          hipStream_t s);
-
 }  // namespace cqpi

@@ -1,0 +1,32 @@
+// Geometry and launchers of the mixed-precision (split-fp16 MFMA) contraction.
+#pragma once
+
+#include <vector>
+
+#include "cqpi_common.h"
+
+struct MixGeom {
+    int CB;     // cells per tile along y (CB*os <= 16 pixel rows)
+    int nsets;  // spin blocks
+    int nper;   // orbitals per spin block
+    int NU;     // lattice rows touched by a tile: CB + 2w
+    int NUg;    // groups of 8 rows
+    int KG;     // K groups of 8 per spin block: (2w+1) * nper * NUg
+    int KS;     // K steps of 32
+    int NT;     // N tiles of 16 columns (4b columns: a and b, re/im)
+    int NPL;    // planes per (a|b, hi|lo): b*b*2
+    int LDW;    // alignment of a fragment's 8 halves (in halves)
+    int OFF;    // plane row offset that realises that alignment
+    int Ly;     // plane row length (halves)
+};
+
+namespace cqpi {
+MixGeom mix_geometry(const Geom &g);
+size_t mix_plane_halves(const Geom &g, const MixGeom &mg);
+std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const double *tiles, double &sA);
+int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int E, double *scale,
+                       _Float16 *P, hipStream_t s);
+int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
+                          const double *scale, double sA, const double2 *T, const double *bulk, int E,
+                          double *ldos, hipStream_t s);
+}  // namespace cqpi

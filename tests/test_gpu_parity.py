@@ -157,3 +157,47 @@ def test_singular_tmatrix_reported(gpu):
         else:
             # rounding may leave a tiny nonzero pivot; then the map is huge but finite
             assert np.all(np.isfinite(out))
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (split-fp16 MFMA contraction, FP64 elsewhere): 1e-4 relative
+# ---------------------------------------------------------------------------
+TOLMIX = 1e-4
+
+
+@pytest.mark.parametrize("case", ["c2", "c1", "hexspin", "oddn", "b5"])
+def test_mixed_matches_oracle(gpu, case):
+    if case == "c2":
+        wl = synth.config2(K=64, n=16, E=3)
+    elif case == "c1":
+        wl = synth.config1(K=64, n=16, E=3)
+    elif case == "hexspin":
+        wl = synth.config3(K=16, n=6, E=2)
+        wl.window, wl.oversamp = 1, 2
+        wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
+    elif case == "oddn":
+        wl = synth.config2_like(3, K=16, n=13, E=2, seed=31, w=2, os_=4, eta=0.01)
+    else:
+        wl = synth.config2_like(5, K=32, n=10, E=2, seed=32, w=2, os_=4, eta=0.01)
+    got = _run(wl, "mixed")
+    ref = oracle.cldos_run(wl)
+    for e in range(ref.shape[0]):
+        mx, rms = rm.rel_errors(got[e], ref[e])
+        print("mixed %s energy %d: max_rel %.3e rms_rel %.3e" % (case, e, mx, rms))
+        assert mx < TOLMIX, (case, e, mx, rms)
+
+
+def test_mixed_dropin_path(gpu):
+    wl = synth.config2(K=32, n=12, E=1)
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    ref = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
+    try:
+        q.set_precision("mixed")
+        q.wannierldos(T, gR)
+        got = q.retrieveResult()
+    finally:
+        q.close()
+    assert rm.rel_errors(got, ref)[0] < TOLMIX
