@@ -1,13 +1,19 @@
-"""Continuum-LDOS throughput on one MI355X.
+"""Continuum-LDOS throughput on one MI355X: mixed precision vs FP64.
 
 A step is one whole pass of the device chain over the workload: per-k
 eigensolve of H(k), then for every energy the resolvent, the k->R transform,
 the T-matrix and the continuum contraction into the W x W map (maps stay in
 HBM).  Default workload: the 3-orbital square lattice (K=512, 51 energies,
 128x128 cells, os=4, w=2 -> 512x512 pixels), synthetic model drawn from a
-fixed seed.  Prints one JSON line.
+fixed seed.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision P] [--no-cpu]
+`value` is the mixed-precision path (split-fp16 MFMA contraction, FP64
+eigensolve / resolvent / transform / T-matrix / periodic term).  The same
+line reports the FP64 path, their ratio, the accuracy of both against each
+other and against the FP64 CPU oracle, the roofline of the dominant kernel
+and the CPU baseline.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
 import argparse
 import json
@@ -22,13 +28,13 @@ import numpy as np  # noqa: E402
 
 METRIC = "LDOS pixel-energy points/s, mixed vs FP64 on 1 GPU at <=1e-4 rel err; grad time"
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "mixed": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}   # FP64 vector/matrix; dense fp16 MFMA
 
 
 def flops_per_point(wl):
-    """Algorithmic flops of the contraction per pixel-energy point:
-    the two complex-by-real gathers a and b over (2w+1)^2 * |orbital set| terms
-    for each of the b columns, plus the T sandwich per spin block."""
+    """Algorithmic flops of the contraction per pixel-energy point: the two
+    complex-by-real gathers a and b over (2w+1)^2 x |orbital block| terms for
+    each of the b columns, plus the T sandwich, per spin block."""
     side2 = (2 * wl.window + 1) ** 2
     nsets = 2 if wl.spin else 1
     per = (wl.maxband >> 1) if wl.spin else wl.maxband
@@ -37,7 +43,9 @@ def flops_per_point(wl):
 
 
 def stage_bytes(wl, nw, E):
-    """Algorithmic HBM bytes of the transform stages for E energies."""
+    """Algorithmic HBM bytes of the transform stages for E energies:
+    stage 1 reads the eigen data and writes the ky-transformed rows of the
+    window, stage 2 reads those rows and writes the G0(R) window."""
     K, b = wl.K, wl.bands
     eig = K * K * (16 * b * b + 8 * b)
     f1 = 16 * b * b * K * nw
@@ -45,28 +53,84 @@ def stage_bytes(wl, nw, E):
     return {1: E * (eig + f1), 2: E * (f1 + win)}
 
 
+def rel_errors(x, ref, floor=1e-3):
+    m = np.max(np.abs(ref))
+    den = np.maximum(np.abs(ref), floor * m)
+    return float(np.max(np.abs(x - ref) / den)), float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+
+
 def cpu_baseline(wl, nsample):
-    """Oracle (FP64 C restatement of the reference algorithm, reference loop
-    nest for the contraction) timed on this host: eigensolve once plus
-    nsample energies, extrapolated linearly to all energies."""
+    """The FP64 oracle (C restatement of the reference algorithm with the
+    reference loop nest for the contraction, OpenMP over cells) timed on this
+    host: eigensolve once plus nsample energies, extrapolated linearly to all
+    energies.  Also returns the oracle maps of the sampled energies."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     t0 = time.perf_counter()
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
     t_eig = time.perf_counter() - t0
+    maps = []
     t0 = time.perf_counter()
     for om in wl.energies[:nsample]:
         gR = oracle.realspace(oracle.greens_k(ev, V, om, wl.eta))
         T = oracle.tmatrix(wl.V, gR[0, 0])
-        oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, wl.maxband, wl.spin)
+        maps.append(oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, wl.maxband, wl.spin))
     t_e = (time.perf_counter() - t0) / nsample
     E = len(wl.energies)
     total = t_eig + E * t_e
-    return {"value": wl.points / total, "unit": "points/s", "cores": oracle.num_threads(),
+    info = {"value": wl.points / total, "unit": "points/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": "eigensolve + %d of %d energies at full size, extrapolated "
-                      "(%.2f s eigensolve, %.2f s/energy)" % (nsample, E, t_eig, t_e)}
+            "sample": "oracle eigensolve + %d of %d energies at full size, extrapolated linearly "
+                      "(%.2f s eigensolve, %.2f s/energy, 1 process, %d OpenMP threads)"
+                      % (nsample, E, t_eig, t_e, oracle.num_threads())}
+    return info, np.array(maps)
+
+
+def time_steps(q, wl, steps, warmup, torch):
+    def step():
+        q.compute_eigen()
+        q.run(wl.energies, wl.eta, host=False)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    q.set_timing(True)
+    step()
+    st = q.stage_times()
+    q.set_timing(False)
+    return dt, st
+
+
+def roofline(wl, q, st, precision):
+    names = list(st)
+    dom = max(range(len(names)), key=lambda i: st[names[i]][0])
+    ms, nl = st[names[dom]]
+    avg_s = ms / max(nl, 1) / 1e3
+    E = len(wl.energies)
+    nw = 2 * q.window_half() + 1
+    if dom == 4:
+        per_launch = wl.W * wl.W * E / max(nl, 1) * flops_per_point(wl)
+        achieved = per_launch / avg_s / 1e12
+        peak = PEAK_TFLOPS[precision]
+        roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+    elif dom in (1, 2):
+        per_launch = stage_bytes(wl, nw, E)[dom] / max(nl, 1)
+        achieved = per_launch / avg_s / 1e9
+        roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        roof = {"kernel": names[dom], "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": None, "traffic": None}
+    roof["avg_launch_ms"] = avg_s * 1e3
+    roof["launches_per_step"] = nl
+    return roof
 
 
 def main():
@@ -74,14 +138,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-energies", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (--gpus 1); "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1); the multi-GPU energy sharding is a design, "
+              "not run in this build", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -92,76 +155,58 @@ def main():
     if cqpi.device_count() < 1:
         raise SystemExit("no HIP device")
     wl = synth.by_id(args.config)
-    q = cqpi.engine_for(wl, args.precision)
-    q.set_timing(False)
-
-    def step():
-        q.compute_eigen()
-        q.run(wl.energies, wl.eta, host=False)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    value = wl.points / dt
-
-    # per-stage device time from HIP events on the library stream (one extra step)
-    q.set_timing(True)
-    step()
-    st = q.stage_times()
-    q.set_timing(False)
-    names = list(st)
-    dom = max(range(len(names)), key=lambda i: st[names[i]][0])
-    ms, nl = st[names[dom]]
-    avg_s = ms / max(nl, 1) / 1e3
     E = len(wl.energies)
-    h = q.window_half()
-    nw = 2 * h + 1
-    if dom == 4:
-        per_launch = wl.W * wl.W * E / max(nl, 1) * flops_per_point(wl)
-        achieved = per_launch / avg_s / 1e12
-        roof = {"kernel": names[dom], "bound": "mfma" if args.precision == "mixed" else "valu",
-                "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None}
-    elif dom in (1, 2):
-        per_launch = stage_bytes(wl, nw, E)[dom] / max(nl, 1)
-        achieved = per_launch / avg_s / 1e9
-        roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    else:
-        roof = {"kernel": names[dom], "bound": "latency", "achieved": None, "peak": None,
-                "unit": None, "frac": None, "traffic": None}
-    roof["avg_launch_ms"] = avg_s * 1e3
-    roof["launches_per_step"] = nl
+
+    qm = cqpi.engine_for(wl, "mixed")
+    dt_m, st_m = time_steps(qm, wl, args.steps, args.warmup, torch)
+    q6 = cqpi.engine_for(wl, "fp64")
+    dt_6, st_6 = time_steps(q6, wl, args.steps, args.warmup, torch)
+
+    # accuracy: every energy of the mixed path against the FP64 device path
+    rm_ = qm.run(wl.energies, wl.eta, host=True)
+    r6 = q6.run(wl.energies, wl.eta, host=True)
+    errs = [rel_errors(rm_[e], r6[e]) for e in range(E)]
+    acc = {"mixed_vs_fp64_gpu_max_rel": max(e[0] for e in errs),
+           "mixed_vs_fp64_gpu_rms_rel": max(e[1] for e in errs),
+           "floor": "max|d| / max(|ref|, 1e-3 max|ref|) per energy, max over energies"}
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": wl.points / dt_m,
         "unit": "points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": dt_m * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else args.precision,
-        "data": "synthetic (seeded model, Gaussian x angular tiles)",
+        "dtype": "mixed: fp16x2 split MFMA contraction (f32 accumulate), f64 elsewhere",
+        "data": "synthetic (seeded 3-orbital model, Gaussian x {xz,yz,xy} tiles)",
         "config": {"workload": "square lattice 3 orbitals, K=512, 51 energies in [-0.1,0.1], "
                                "eta=0.002, 128x128 cells, os=4, w=2 (512x512 pixels)"
                    if args.config == 2 else "config %d" % args.config,
                    "kpoints": wl.K, "energies": E, "cells": wl.n, "oversamp": wl.oversamp,
-                   "window": wl.window, "bands": wl.bands, "precision": args.precision},
-        "stages_ms": {k: v[0] for k, v in st.items()},
-        "roofline": roof,
+                   "window": wl.window, "bands": wl.bands},
+        "fp64": {"value": wl.points / dt_6, "ms_per_step": dt_6 * 1e3,
+                 "stages_ms": {k: v[0] for k, v in st_6.items()},
+                 "roofline": roofline(wl, q6, st_6, "fp64")},
+        "speedup_mixed_vs_fp64": dt_6 / dt_m,
+        "stages_ms": {k: v[0] for k, v in st_m.items()},
+        "accuracy": acc,
+        "roofline": roofline(wl, qm, st_m, "mixed"),
     }
-    q.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_energies)
+        info, maps = cpu_baseline(wl, args.cpu_energies)
+        out["cpu_baseline"] = info
+        e6 = [rel_errors(r6[e], maps[e]) for e in range(len(maps))]
+        em = [rel_errors(rm_[e], maps[e]) for e in range(len(maps))]
+        acc["fp64_gpu_vs_oracle_max_rel"] = max(e[0] for e in e6)
+        acc["mixed_vs_oracle_max_rel"] = max(e[0] for e in em)
+        acc["mixed_vs_oracle_rms_rel"] = max(e[1] for e in em)
+        acc["oracle_energies"] = len(maps)
+    qm.close()
+    q6.close()
     print(json.dumps(out), flush=True)
 
 

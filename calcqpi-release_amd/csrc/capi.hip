@@ -229,8 +229,18 @@ int upload_identity_V(cqpi_ctx *c)
     return 0;
 }
 
-// bulk table + contraction for E energies whose G0 window, T are on the device
-int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos)
+// fp16 planes and per-energy scales for the mixed contraction of E energies
+int mix_buffers(cqpi_ctx *c, int E)
+{
+    RETCHK(dev_grow(c->d_mscale, c->c_mscale, (size_t)E, c->gpumem));
+    RETCHK(dev_grow(c->d_planes, c->c_planes, (size_t)E * mix_plane_halves(c->g, c->mg), c->gpumem));
+    return 0;
+}
+
+// bulk table + contraction for E energies whose G0 window, T are on the
+// device.  planes_ready: the transform already wrote the mixed-precision
+// planes and scales (whole-chain path); otherwise they are made from G0w.
+int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos, bool planes_ready)
 {
     const Geom &g = c->g;
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, (size_t)E * g.os * g.os, c->gpumem));
@@ -238,12 +248,13 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
     RETCHK(launch_bulk(g, G0w, c->d_tiles, E, c->d_bulk, c->stream));
     stage_end(c, st);
     st = stage_begin(c, 4);
-    int r;
+    int r = 0;
     switch (c->precision) {
     case CQPI_PREC_MIXED:
-        RETCHK(dev_grow(c->d_mscale, c->c_mscale, (size_t)E, c->gpumem));
-        RETCHK(dev_grow(c->d_planes, c->c_planes, (size_t)E * mix_plane_halves(g, c->mg), c->gpumem));
-        r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale, c->d_planes, c->stream);
+        if (!planes_ready) {
+            RETCHK(mix_buffers(c, E));
+            r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale, c->d_planes, c->stream);
+        }
         if (!r)
             r = launch_contract_mixed(g, c->mg, c->d_planes, c->d_afrag, c->d_mscale, c->sA, T, c->d_bulk, E,
                                       ldos, c->stream);
@@ -415,7 +426,7 @@ int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, s
     HIPCHK(hipMemcpyAsync(c->d_G0w, c->h_win, nwin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_T, c->h_T, NP * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->copy_done, c->stream));
-    RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos));
+    RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos, false));
     c->nlayers = 1;
     c->last_chunk_e0 = 0;
     c->last_chunk_ec = 1;
@@ -583,6 +594,19 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
     RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
+    // mixed precision: the k->x transform writes the fp16 planes directly.  Their
+    // scale comes from |G0(R)_ab| <= ||G0(k)|| <= 1/eta, a bound for every R.
+    const bool mixed = c->precision == CQPI_PREC_MIXED;
+    double sG = 1.0;
+    if (mixed) {
+        RETCHK(mix_buffers(c, (int)Ec));
+        int ex;
+        std::frexp(1.0 / eta, &ex);
+        sG = std::ldexp(1.0, 13 - ex);
+        std::vector<double> sv(Ec, sG);
+        HIPCHK(hipMemcpyAsync(c->d_mscale, sv.data(), Ec * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     for (size_t e0 = 0; e0 < E; e0 += Ec) {
         int ec = (int)std::min(Ec, E - e0);
         int st = stage_begin(c, 1);
@@ -590,12 +614,13 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
                                    c->stream));
         stage_end(c, st);
         st = stage_begin(c, 2);
-        RETCHK(launch_transform_kx(g, c->d_F1, ec, c->d_tw, c->d_G0w, c->stream));
+        RETCHK(launch_transform_kx(g, mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, sG, c->d_G0w,
+                                   mixed ? c->d_planes : nullptr, c->stream));
         stage_end(c, st);
         st = stage_begin(c, 3);
         RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
         stage_end(c, st);
-        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW));
+        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, mixed));
         c->last_chunk_e0 = e0;
         c->last_chunk_ec = ec;
     }

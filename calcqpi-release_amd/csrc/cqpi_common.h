@@ -58,6 +58,8 @@ struct Geom {
     int W;        // map side os*n
 };
 
+struct MixGeom;
+
 // launchers (defined next to their kernels)
 namespace cqpi {
 
@@ -69,9 +71,10 @@ int launch_eigen(int B, const DevRGroup *groups, int ngroups, const DevHop *hops
 int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
                         const double *omegas, int E, double eta, const double2 *tw,
                         double2 *F1, hipStream_t s);
-// transform along kx, window extraction and 1/K^2: G0w[e][sx][sy][pair]
-int launch_transform_kx(const Geom &g, const double2 *F1, int E, const double2 *tw,
-                        double2 *G0w, hipStream_t s);
+// transform along kx, window extraction and 1/K^2: G0w[e][sx][sy][pair]; with
+// mg != NULL also the fp16 hi/lo planes of the mixed contraction (scale sG)
+int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,
+                        double sG, double2 *G0w, _Float16 *P, hipStream_t s);
 // T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular
 int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T,
                    int *status, hipStream_t s);

@@ -7,17 +7,17 @@
 // hi*hi + (hi*lo' + lo'*hi)/2^12 recover ~22 significant bits per product.
 // The T sandwich and the bulk table stay FP64.
 //
-// GEMM shape (per wave): one "tile" is CB consecutive cells along y at one
-// cell column cx and one sub-pixel column ii.  Its M = CB*os rows are the
-// pixels (c, jj); K runs over (dx, o, dy): the lattice columns cx+dx-w, the
-// orbitals o of the current spin block and the CB+2w lattice rows the tile
-// touches, in groups of 8 consecutive rows; N runs over (p, re/im) of
+// GEMM shape (per wave): one "tile" is CB consecutive cells along x at one
+// cell row cy and one sub-pixel row jj.  Its M = CB*os rows are the pixels
+// (c, ii); K runs over (dy, o, dx): the lattice rows cy+dy-w, the orbitals o
+// of the current spin block and the CB+2w lattice columns the tile touches,
+// in groups of 8 consecutive columns; N runs over (p, re/im) of
 // G0(pos+R)[o][p] (the a gather) and of G0(-pos-R)[p][o] (the b gather).
-// The tile matrix A[(c,jj)][(dx,o,dy)] = w_o(ii+(2w-dx)os, jj+(2w-dy+c)os)
-// (zero outside the band 0 <= dy-c <= 2w) is the same for every tile and is
-// precomputed on the host in MFMA fragment order.  G0 is repacked per energy
-// into "planes" (one per (o, p, re/im)) along y, so a lane's 8 K values of a
-// B fragment are 8 consecutive halves in memory.
+// The tile matrix A[(c,ii)][(dy,o,dx)] = w_o(ii+(2w-dx+c)os, jj+(2w-dy)os)
+// (zero outside the band 0 <= dx-c <= 2w) is the same for every tile and is
+// precomputed on the host in MFMA fragment order.  G0 is stored per energy as
+// "planes" (one per (o, p, re/im)) with rows along x, so a lane's 8 K values
+// of a B fragment are 8 consecutive halves in memory.
 //
 // v_mfma_f32_16x16x32_f16 lane layout: A: lane l holds row l%16, k 8*(l/16)..+7;
 // B: lane l holds column l%16, k 8*(l/16)..+7; C: lane l holds rows
@@ -64,24 +64,26 @@ __global__ void __launch_bounds__(1024) k_wscale(const double2 *__restrict__ G0w
     }
 }
 
-// planes P[e][ab][hl][(o*B+p)*2+ri][sx][Ly]: ab=0 holds G0(R)[o][p], ab=1 holds
-// G0(-R)[p][o], R = (sx-h, idx-OFF-h); entries outside the window are zero
+// planes P[e][ab][hl][(o*B+p)*2+ri][row sy][Ly], rows along x: ab=0 holds
+// G0(R)[o][p], ab=1 holds G0(-R)[p][o], R = (idx-OFF-h, sy-h); entries outside
+// the window are zero.  (The whole-chain path writes these planes from the
+// transform kernel; this kernel serves the GPUQPI drop-in path.)
 __global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
                        const double *__restrict__ scale, _Float16 *__restrict__ P)
 {
-    const int sx = blockIdx.x, e = blockIdx.z;
+    const int sy = blockIdx.x, e = blockIdx.z;
     const int B = g.B, NP = B * B, nw = g.nw;
     const int ab = blockIdx.y / mg.NPL, pl = blockIdx.y % mg.NPL;
     const int ri = pl & 1, op = pl >> 1, o = op / B, p = op % B;
     const double s = scale[e];
     const double2 *G = G0w + (size_t)e * nw * nw * NP;
     const size_t plane = (size_t)mg.Ly * nw;
-    _Float16 *hi = P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane + (size_t)sx * mg.Ly;
-    _Float16 *lo = P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane + (size_t)sx * mg.Ly;
+    _Float16 *hi = P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane + (size_t)sy * mg.Ly;
+    _Float16 *lo = P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane + (size_t)sy * mg.Ly;
     for (int idx = threadIdx.x; idx < mg.Ly; idx += blockDim.x) {
-        int sy = idx - mg.OFF;
+        int sx = idx - mg.OFF;
         double v = 0.0;
-        if (sy >= 0 && sy < nw) {
+        if (sx >= 0 && sx < nw) {
             double2 z = ab == 0 ? G[((size_t)sx * nw + sy) * NP + o * B + p]
                                 : G[((size_t)(2 * g.h - sx) * nw + (2 * g.h - sy)) * NP + p * B + o];
             v = (ri ? z.y : z.x) * s;
@@ -115,8 +117,11 @@ __device__ inline half8 load8(const _Float16 *p)
 }
 
 constexpr int kWaves = 4;
-constexpr int kIIB = 4;  // sub-pixel columns per accumulator block
+constexpr int kJB = 4;  // sub-pixel rows (jj) per accumulator block
 
+// one wave = one tile: CB cells along x starting at cx0, one cell row cy,
+// all sub-pixels.  Rows of the GEMM are the pixels (c, ii), c < CB, ii < os,
+// of one sub-pixel row jj; the jj loop runs in blocks of kJB accumulators.
 template <int NT, int LDW>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                     const half8 *__restrict__ Afrag,
@@ -126,32 +131,35 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                                                     double *__restrict__ ldos)
 {
     __shared__ double2 Ts[16 * 16];
-    __shared__ float Cs[kWaves][16][16 * NT + 1];
-    __shared__ double vals[kWaves][16][16];
+    __shared__ float Cs[kWaves][kJB][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int e = blockIdx.z;
     const int B = g.B, os = g.os, w = g.w, n = g.n, h = g.h, nw = g.nw;
     const int n2 = n >> 1;
-    const int cx_raw = blockIdx.x * kWaves + wave;
-    const bool cx_ok = cx_raw < n;
-    const int cx = cx_ok ? cx_raw : n - 1;
-    const int cy0 = blockIdx.y * mg.CB;
+    const int cy_raw = blockIdx.y * kWaves + wave;
+    const bool cy_ok = cy_raw < n;
+    const int cy = cy_ok ? cy_raw : n - 1;
+    const int cx0 = blockIdx.x * mg.CB;
     const double2 *Te = Tm + (size_t)e * B * B;
     for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
-    for (int i = lane; i < 16 * 16; i += 64) vals[wave][i / 16][i % 16] = 0.0;
     __syncthreads();
 
     const size_t plane = (size_t)mg.Ly * nw;
     const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
     const int col = lane & 15, kq = lane >> 4;
-    const int sy0 = cy0 - n2 - w + h + mg.OFF;
+    const int col0 = cx0 - n2 - w + h + mg.OFF;
+    const int rows = mg.CB * os;
+    double val[4] = {0.0, 0.0, 0.0, 0.0};
 
     for (int sg = 0; sg < mg.nsets; sg++) {
-        for (int ii0 = 0; ii0 < os; ii0 += kIIB) {
-            float4v hh[kIIB][NT], cr[kIIB][NT];
 #pragma unroll
-            for (int i = 0; i < kIIB; i++)
+        for (int jb = 0; jb < 4; jb++) {
+            const int jj0 = jb * kJB;
+            if (jj0 >= os) break;
+            float4v hh[kJB][NT], cr[kJB][NT];
+#pragma unroll
+            for (int i = 0; i < kJB; i++)
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
                     hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
@@ -161,15 +169,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                 const int kg = ks * 4 + kq;
                 half8 bh[NT], bl[NT];
                 const bool kok = kg < mg.KG;
-                int dx = 0, ol = 0, g8 = 0;
+                int dy = 0, ol = 0, g8 = 0;
                 if (kok) {
                     g8 = kg % mg.NUg;
                     int r = kg / mg.NUg;
                     ol = r % mg.nper;
-                    dx = r / mg.nper;
+                    dy = r / mg.nper;
                 }
                 const int o = sg * mg.nper + ol;
-                const int sx = cx - n2 + dx - w + h;
+                const int row = cy - n2 + dy - w + h;
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
                     const int nn = t * 16 + col;
@@ -183,7 +191,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                         pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
                     }
                     if (kok && ab >= 0) {
-                        const size_t off = (size_t)sx * mg.Ly + sy0 + g8 * 8;
+                        const size_t off = (size_t)row * mg.Ly + col0 + g8 * 8;
                         bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + off);
                         bl[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + off);
                     } else {
@@ -192,10 +200,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < kIIB; i++) {
-                    const int ii = ii0 + i;
-                    if (ii >= os) break;
-                    const size_t fa = ((((size_t)sg * os + ii) * mg.KS + ks) * 2) * 64;
+                for (int i = 0; i < kJB; i++) {
+                    const int jj = jj0 + i;
+                    if (jj >= os) break;
+                    const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64;
                     const half8 ah = Afrag[fa + lane];
                     const half8 al = Afrag[fa + 64 + lane];
 #pragma unroll
@@ -206,55 +214,214 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                     }
                 }
             }
-            // epilogue: Im sum_pq a_p T_pq b_q per pixel row, FP64
+            // epilogue (wave-local): C -> LDS, then one pixel per lane:
+            // Im sum_pq a_p T_pq b_q in FP64
 #pragma unroll
-            for (int i = 0; i < kIIB; i++) {
-                const int ii = ii0 + i;
-                if (ii >= os) break;
+            for (int i = 0; i < kJB; i++)
 #pragma unroll
                 for (int t = 0; t < NT; t++)
 #pragma unroll
                     for (int r = 0; r < 4; r++)
-                        Cs[wave][kq * 4 + r][t * 16 + col] = hh[i][t][r] + cr[i][t][r] * (1.0f / kLoScale);
-                __syncthreads();
-                const int m = lane >> 2, part = lane & 3;
-                double acc = 0.0;
-                for (int p = part; p < B; p += 4) {
-                    double2 tb = make_double2(0.0, 0.0);
-                    for (int q = 0; q < B; q++) {
-                        double2 bq = make_double2(Cs[wave][m][2 * B + 2 * q], Cs[wave][m][2 * B + 2 * q + 1]);
-                        tb = cadd(tb, cmul(Ts[p * B + q], bq));
+                        Cs[wave][i][kq * 4 + r][t * 16 + col] = hh[i][t][r] + cr[i][t][r] * (1.0f / kLoScale);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                const int i = lane >> 4, m = lane & 15;
+                if (jj0 + i < os && m < rows) {
+                    const float *Cr = Cs[wave][i][m];
+                    double acc = 0.0;
+                    for (int p = 0; p < B; p++) {
+                        double2 tb = make_double2(0.0, 0.0);
+                        for (int q = 0; q < B; q++)
+                            tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
+                        acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
                     }
-                    double ar = Cs[wave][m][2 * p], ai = Cs[wave][m][2 * p + 1];
-                    acc += ar * tb.y + ai * tb.x;
+                    val[jb] += acc;
                 }
-                acc += __shfl_xor(acc, 1);
-                acc += __shfl_xor(acc, 2);
-                if (part == 0) vals[wave][ii][m] += acc;
-                __syncthreads();
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     const double sc = sA * scale[e];
     const double unscale = 1.0 / (sc * sc);
-    const int W = g.W, rows = mg.CB * os;
+    const int W = g.W;
     double *out = ldos + (size_t)e * W * W;
-    for (int idx = lane; idx < os * 16; idx += 64) {
-        const int ii = idx >> 4, m = idx & 15;
-        if (m >= rows) continue;
-        const int c = m / os, jj = m - c * os, cy = cy0 + c;
-        if (!cx_ok || cy >= n) continue;
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) {
+        const int jj = jb * kJB + (lane >> 4), m = lane & 15;
+        if (jj >= os || m >= rows || !cy_ok) continue;
+        const int c = m / os, ii = m - c * os, cx = cx0 + c;
+        if (cx >= n) continue;
         const int X = cx * os + ii, Y = cy * os + jj;
-        out[(size_t)X + (size_t)Y * W] =
-            -(vals[wave][ii][m] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+        out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
     }
+}
+
+// Specialised kernel for small sub-pixel counts and K depths (compile-time OS,
+// KS): the A fragments of every (jj, ks) stay in registers and each wave walks
+// TPW tiles along y, so A is read from memory once per wave instead of once
+// per tile.  Non-spin models only (one orbital block).
+template <int NT, int LDW, int OS, int KS, int TPW>
+__global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
+                                                         const half8 *__restrict__ Afrag,
+                                                         const double *__restrict__ scale, double sA,
+                                                         const double2 *__restrict__ Tm,
+                                                         const double *__restrict__ bulk,
+                                                         double *__restrict__ ldos)
+{
+    __shared__ double2 Ts[16 * 16];
+    __shared__ float Cs[kWaves][OS][16][16 * NT + 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int e = blockIdx.z;
+    const int B = g.B, w = g.w, n = g.n, h = g.h, nw = g.nw;
+    const int n2 = n >> 1;
+    const int cx0 = blockIdx.x * mg.CB;
+    const double2 *Te = Tm + (size_t)e * B * B;
+    for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
+
+    half8 ah[OS][KS], al[OS][KS];
+#pragma unroll
+    for (int jj = 0; jj < OS; jj++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const size_t fa = (((size_t)jj * KS + ks) * 2) * 64;
+            ah[jj][ks] = Afrag[fa + lane];
+            al[jj][ks] = Afrag[fa + 64 + lane];
+        }
+    __syncthreads();
+
+    const size_t plane = (size_t)mg.Ly * nw;
+    const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
+    const int col = lane & 15, kq = lane >> 4;
+    const int col0 = cx0 - n2 - w + h + mg.OFF;
+    const int rows = mg.CB * OS;
+    const double sc = sA * scale[e];
+    const double unscale = 1.0 / (sc * sc);
+    const int W = g.W;
+    double *out = ldos + (size_t)e * W * W;
+
+    // per lane: the K group it loads and the plane offsets of its N columns
+    int kdy[KS], kg8[KS];
+    bool kok[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const int kg = ks * 4 + kq;
+        kok[ks] = kg < mg.KG;
+        const int g8 = kg % mg.NUg, r = kg / mg.NUg;
+        kg8[ks] = g8;
+        kdy[ks] = r / mg.nper;
+        const int ol = r % mg.nper;
+        kdy[ks] = kok[ks] ? kdy[ks] : 0;
+        kg8[ks] = kok[ks] ? g8 : 0;
+        // stash the orbital in the upper bits of kdy
+        kdy[ks] |= (kok[ks] ? ol : 0) << 16;
+    }
+
+    for (int tt = 0; tt < TPW; tt++) {
+        const int cy_raw = (blockIdx.y * TPW + tt) * kWaves + wave;
+        if (cy_raw >= n) break;  // uniform per wave; no block-wide barrier follows
+        const int cy = cy_raw;
+        float4v hh[OS][NT], cr[OS][NT];
+#pragma unroll
+        for (int i = 0; i < OS; i++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+                cr[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int dy = kdy[ks] & 0xffff, o = kdy[ks] >> 16;
+            const int row = cy - n2 + dy - w + h;
+            half8 bh[NT], bl[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int nn = t * 16 + col;
+                int ab = -1, pl = 0;
+                if (nn < 2 * B) {
+                    ab = 0;
+                    pl = (o * B + (nn >> 1)) * 2 + (nn & 1);
+                } else if (nn < 4 * B) {
+                    ab = 1;
+                    int m2 = nn - 2 * B;
+                    pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
+                }
+                if (kok[ks] && ab >= 0) {
+                    const size_t off = (size_t)row * mg.Ly + col0 + kg8[ks] * 8;
+                    bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + off);
+                    bl[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + off);
+                } else {
+                    bh[t] = half8{0, 0, 0, 0, 0, 0, 0, 0};
+                    bl[t] = bh[t];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < OS; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bh[t], hh[i][t], 0, 0, 0);
+                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bl[t], cr[i][t], 0, 0, 0);
+                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i][ks], bh[t], cr[i][t], 0, 0, 0);
+                }
+        }
+        // epilogue (wave-local)
+#pragma unroll
+        for (int i = 0; i < OS; i++)
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    Cs[wave][i][kq * 4 + r][t * 16 + col] = hh[i][t][r] + cr[i][t][r] * (1.0f / kLoScale);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int jb = 0; jb < (OS + 3) / 4; jb++) {
+            const int jj = jb * 4 + (lane >> 4), m = lane & 15;
+            if (jj < OS && m < rows) {
+                const float *Cr = Cs[wave][jj][m];
+                double acc = 0.0;
+                for (int p = 0; p < B; p++) {
+                    double2 tb = make_double2(0.0, 0.0);
+                    for (int q = 0; q < B; q++)
+                        tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
+                    acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
+                }
+                const int c = m / OS, ii = m - c * OS, cx = cx0 + c;
+                if (cx < n) {
+                    const int X = cx * OS + ii, Y = cy * OS + jj;
+                    out[(size_t)X + (size_t)Y * W] = -(acc * unscale + bulk[(size_t)e * OS * OS + ii * OS + jj]);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+constexpr int kTPW = 8;
+
+template <int NT, int LDW, int OS, int KS>
+bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
+              double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    if (g.os != OS || mg.KS != KS || mg.NT != NT || mg.nsets != 1 || mg.LDW != LDW) return false;
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * kTPW - 1) / (kWaves * kTPW), E);
+    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, kTPW>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A,
+                       scale, sA, T, bulk, ldos);
+    return true;
 }
 
 template <int NT>
 int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag, const double *scale,
               double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
-    dim3 grid((g.n + kWaves - 1) / kWaves, (g.n + mg.CB - 1) / mg.CB, E);
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves - 1) / kWaves, E);
     dim3 blk(64 * kWaves);
     const half8 *A = (const half8 *)Afrag;
     switch (mg.LDW) {
@@ -298,7 +465,7 @@ size_t mix_plane_halves(const Geom &g, const MixGeom &mg)
     return (size_t)4 * mg.NPL * g.nw * mg.Ly;
 }
 
-// A fragments [sg][ii][ks][hl][lane][8] (fp16) and the power-of-two scale sA
+// A fragments [sg][jj][ks][hl][lane][8] (fp16) and the power-of-two scale sA
 std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const double *tiles, double &sA)
 {
     double mx = 0.0;
@@ -313,24 +480,24 @@ std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const
     std::vector<_Float16> F((size_t)mg.nsets * g.os * mg.KS * 2 * 64 * 8);
     const int rows = mg.CB * g.os, T = g.T, w = g.w, os = g.os;
     for (int sg = 0; sg < mg.nsets; sg++)
-        for (int ii = 0; ii < os; ii++)
+        for (int jj = 0; jj < os; jj++)
             for (int ks = 0; ks < mg.KS; ks++)
                 for (int lane = 0; lane < 64; lane++)
                     for (int j = 0; j < 8; j++) {
                         int m = lane & 15, kg = ks * 4 + (lane >> 4);
                         double v = 0.0;
                         if (m < rows && kg < mg.KG) {
-                            int g8 = kg % mg.NUg, r = kg / mg.NUg, ol = r % mg.nper, dx = r / mg.nper;
-                            int dy = g8 * 8 + j, c = m / os, jj = m % os;
-                            if (dy < mg.NU && dy - c >= 0 && dy - c <= 2 * w) {
+                            int g8 = kg % mg.NUg, r = kg / mg.NUg, ol = r % mg.nper, dy = r / mg.nper;
+                            int dx = g8 * 8 + j, c = m / os, ii = m % os;
+                            if (dx < mg.NU && dx - c >= 0 && dx - c <= 2 * w) {
                                 int o = sg * mg.nper + ol;
-                                int x = ii + (2 * w - dx) * os, y = jj + (2 * w - dy + c) * os;
+                                int x = ii + (2 * w - dx + c) * os, y = jj + (2 * w - dy) * os;
                                 v = tiles[((size_t)o * T + x) * T + y] * sA;
                             }
                         }
                         _Float16 hi = (_Float16)v;
                         _Float16 lo = (_Float16)((v - (double)hi) * kLoScale);
-                        size_t base = ((((size_t)sg * os + ii) * mg.KS + ks) * 2) * 64 * 8;
+                        size_t base = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64 * 8;
                         F[base + (size_t)lane * 8 + j] = hi;
                         F[base + 64 * 8 + (size_t)lane * 8 + j] = lo;
                     }
@@ -350,6 +517,14 @@ int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, c
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
                           double *ldos, hipStream_t s)
 {
+    const half8 *A = (const half8 *)Afrag;
+    // register-resident tile fragments for the common small shapes:
+    // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital)
+    if (try_areg<1, 4, 4, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 4, 4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<2, 4, 4, 7>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 2, 8, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     switch (mg.NT) {
     case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
     case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
