@@ -138,7 +138,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", default="square_3orb")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-energies", type=int, default=2)
     args = ap.parse_args()
@@ -154,7 +154,7 @@ def main():
 
     if cqpi.device_count() < 1:
         raise SystemExit("no HIP device")
-    wl = synth.by_id(args.config)
+    wl = synth.workload(args.workload)
     E = len(wl.energies)
 
     qm = cqpi.engine_for(wl, "mixed")
@@ -185,7 +185,7 @@ def main():
         "data": "synthetic (seeded 3-orbital model, Gaussian x {xz,yz,xy} tiles)",
         "config": {"workload": "square lattice 3 orbitals, K=512, 51 energies in [-0.1,0.1], "
                                "eta=0.002, 128x128 cells, os=4, w=2 (512x512 pixels)"
-                   if args.config == 2 else "config %d" % args.config,
+                   if args.workload == "square_3orb" else "%s: %s" % (args.workload, wl.name),
                    "kpoints": wl.K, "energies": E, "cells": wl.n, "oversamp": wl.oversamp,
                    "window": wl.window, "bands": wl.bands},
         "fp64": {"value": wl.points / dt_6, "ms_per_step": dt_6 * 1e3,

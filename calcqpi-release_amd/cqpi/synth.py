@@ -237,7 +237,7 @@ def dorbital_L():
 # workloads
 # ----------------------------------------------------------------------------
 
-def config1(K=256, n=64, E=11) -> Workload:
+def square_1orb(K=256, n=64, E=11) -> Workload:
     """Single-orbital square lattice: t=-1, t'=0.3, on-site -mu with mu=0.5,
     9 R-vectors; V=1.0 on orbital 0; Gaussian tile sigma=0.35 at z=0.5,
     os=8, w=1; k-grid 256, 11 energies in [-0.5, 0.5], eta=0.01, map 64x64."""
@@ -254,11 +254,11 @@ def config1(K=256, n=64, E=11) -> Workload:
         energies=np.linspace(-0.5, 0.5, E), eta=0.01,
         V=np.array([[1.0 + 0j]]),
         tiles=gaussian_tiles(["s"], 0.35, 0.5, w, os_, normalise=False),
-        meta={"config": 1, "mu": mu, "t": t, "tp": tp},
+        meta={"mu": mu, "t": t, "tp": tp},
     )
 
 
-def config2(K=512, n=128, E=51, seed=1002, elo=-0.1, ehi=0.1) -> Workload:
+def square_3orb(K=512, n=128, E=51, seed=1002, elo=-0.1, ehi=0.1) -> Workload:
     """3-orbital square lattice, Hermitian real hoppings up to the 3rd
     neighbour (13 R) drawn N(0,0.3), on-site diagonal U(-0.2,0.2) (on-site
     off-diagonal N(0,0.3), Hermitised); V = 0.5 on every orbital; tiles
@@ -279,11 +279,11 @@ def config2(K=512, n=128, E=51, seed=1002, elo=-0.1, ehi=0.1) -> Workload:
         energies=np.linspace(elo, ehi, E), eta=0.002,
         V=np.diag([0.5] * b).astype(np.complex128),
         tiles=gaussian_tiles(["xz", "yz", "xy"], 0.4, 0.6, w, os_),
-        meta={"config": 2, "seed": seed},
+        meta={"seed": seed},
     )
 
 
-def config3(K=1024, n=256, E=101, seed=1003, lam=0.05) -> Workload:
+def hex_5d_spin(K=1024, n=256, E=101, seed=1003, lam=0.05) -> Workload:
     """Hexagonal lattice, 5 d orbitals x spin = 10 bands, complex hoppings
     N(0,0.2)+iN(0,0.02) up to the 4th shell (31 R), Hermitised, spin-diagonal,
     plus on-site spin-orbit lambda L.S with lambda=0.05; full 10x10 Hermitian
@@ -318,14 +318,14 @@ def config3(K=1024, n=256, E=101, seed=1003, lam=0.05) -> Workload:
         hops=hops_from_blocks(blocks, 2 * d), K=K, n=n, window=w, oversamp=os_,
         energies=np.linspace(-0.2, 0.2, E), eta=0.001, V=V,
         tiles=np.concatenate([tiles5, tiles5]), spin=True, lattice="hex",
-        meta={"config": 3, "seed": seed, "lambda": lam},
+        meta={"seed": seed, "lambda": lam},
     )
 
 
 ORB_SETS = {1: ["s"], 3: ["xz", "yz", "xy"], 5: ["xz", "yz", "xy", "x2y2", "z2"]}
 
 
-def config2_like(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=None):
+def square_norb(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=None):
     """Model and tiles generated as in config 2 for b in {1,3,5}."""
     rng = np.random.default_rng(seed)
     onsite = rng.normal(0, 0.3, (b, b))
@@ -348,7 +348,7 @@ def config2_like(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=N
     )
 
 
-def config4(count=128, seed=1004):
+def batch_uneven(count=128, seed=1004):
     """Batch of independent instances with uneven sizes: orbitals in {1,3,5},
     k-grid in {128,256,512}, 16-64 energies in [-0.3,0.3], V = U(0.1,2.0) x a
     random Hermitian unit-norm matrix, map in {32,64,128} cells."""
@@ -359,13 +359,13 @@ def config4(count=128, seed=1004):
         K = int(rng.choice([128, 256, 512]))
         n = int(rng.choice([32, 64, 128]))
         E = int(rng.integers(16, 65))
-        out.append(config2_like(b, K, n, E, seed=seed * 1000 + i))
+        out.append(square_norb(b, K, n, E, seed=seed * 1000 + i))
     return out
 
 
-def config5(K=512, n=64, E=21, seed=1005):
+def square_3orb_grad(K=512, n=64, E=21, seed=1005):
     """Config-2 shapes with 21 energies and a 64x64-cell map (gradient workload)."""
-    return config2(K=K, n=n, E=E, seed=seed)
+    return square_3orb(K=K, n=n, E=E, seed=seed)
 
 
 def perturbed(wl: Workload, seed=2005, dt=0.02, dmu=0.01, dV=0.1) -> Workload:
@@ -392,8 +392,16 @@ def perturbed(wl: Workload, seed=2005, dt=0.02, dmu=0.01, dV=0.1) -> Workload:
     return out
 
 
-def by_id(cid: int, **kw):
-    return {1: config1, 2: config2, 3: config3, 5: config5}[cid](**kw)
+WORKLOADS = {
+    "square_1orb": square_1orb,
+    "square_3orb": square_3orb,
+    "hex_5d_spin": hex_5d_spin,
+    "square_3orb_grad": square_3orb_grad,
+}
+
+
+def workload(name: str, **kw) -> Workload:
+    return WORKLOADS[name](**kw)
 
 
 # ----------------------------------------------------------------------------

@@ -1,0 +1,411 @@
+// calcqpi_amd -- calcqpi-compatible driver for the continuum-LDOS (wannier)
+// mode on one MI355X, on top of the C ABI of libcqpi_amd.so.
+//
+//   calcqpi_amd <inputfile>             run and write the qpifile (idl map)
+//   calcqpi_amd --print-config <input>  parse only, print the resolved job as JSON
+//
+// The input file and the output map follow the reference driver:
+//   * key=value; lines, '#' comments, (a,b,...) lists, "string" values,
+//     a warning for every unused key (loadconfig, include/parser.h:24-199);
+//   * defaults and keys of main(), src/calcqpi.cc:133-273, for output=wannier
+//     with green=normal: tbfile, qpifile, lattice, oversamp, window, layers,
+//     energies, eta, kpoints, fermi, scattering, phase, spin, scmodel,
+//     magscat, prearr, idlorbitalfile, logfile;
+//   * the orbital tiles come from the idl orbital file, duplicated for spin
+//     or padded with zero tiles as in src/calcqpi.cc:483-518;
+//   * energies of the layers are lb + (hb-lb) k/(L-1) (idl::getbias,
+//     src/idl.cc:969-973); the map is W x W x L with W = oversamp*lattice,
+//     header as in src/calcqpi.cc:640,655 and data written as float32, x
+//     fastest (idl::write, src/idl.cc:78-95).
+// Extension key: precision=fp64|mixed; (contraction precision, default fp64).
+// Modes other than output=wannier / green=normal and analytic or XSF orbitals
+// are reported as unsupported.
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cqpi_amd.h"
+
+using std::string;
+using std::vector;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// input file: key=value; (include/parser.h:29-54 semantics)
+// ---------------------------------------------------------------------------
+struct Config {
+    vector<string> name, field;
+    vector<int> used;
+    bool ok = false;
+
+    explicit Config(const char *path)
+    {
+        std::ifstream is(path);
+        if (!is) return;
+        ok = true;
+        string line;
+        size_t lineno = 0;
+        while (std::getline(is, line)) {
+            if (line.empty()) continue;
+            lineno++;
+            size_t hash = line.find('#');
+            if (hash != string::npos) line.erase(hash);
+            size_t eq = line.find('='), semi = line.find(';');
+            if (eq != string::npos && semi != string::npos) {
+                name.push_back(line.substr(0, eq));
+                field.push_back(line.substr(eq + 1, semi - eq - 1));
+            } else if (eq != string::npos || semi != string::npos) {
+                std::cerr << "Incorrect syntax, '=' or ';' missing.\nLine " << lineno << ": " << line << "\n";
+            }
+        }
+        used.assign(name.size(), 0);
+    }
+    bool probe(const string &key)
+    {
+        for (size_t i = 0; i < name.size(); i++)
+            if (name[i] == key) {
+                used[i]++;
+                return true;
+            }
+        return false;
+    }
+    string get(const string &key)
+    {
+        string v;
+        for (size_t i = 0; i < name.size(); i++)
+            if (name[i] == key) {
+                used[i]++;
+                v = field[i];
+            }
+        if (v.empty()) return v;
+        size_t a = v.find_first_not_of(" \t"), b = v.find_last_not_of(" \t");
+        if (a == string::npos) return "";
+        return v.substr(a, b - a + 1);
+    }
+    static string unquote(const string &v)
+    {
+        size_t a = v.find_first_of('"'), b = v.find_last_of('"');
+        if (a != string::npos && b != string::npos && b > a) return v.substr(a + 1, b - a - 1);
+        return "";
+    }
+    string str(const string &key) { return unquote(get(key)); }
+    double num(const string &key)
+    {
+        string v = get(key);
+        return v.empty() ? NAN : std::atof(v.c_str());
+    }
+    int integer(const string &key)
+    {
+        string v = get(key);
+        return v.empty() ? 0 : std::atoi(v.c_str());
+    }
+    vector<string> list(const string &key)
+    {
+        string v = get(key);
+        vector<string> out;
+        size_t first = v.find('('), last = v.rfind(')');
+        if (first == string::npos || last == string::npos || last < first) return out;
+        string inner = v.substr(first + 1, last - first - 1);
+        std::stringstream ss(inner);
+        string item;
+        while (std::getline(ss, item, ',')) {
+            size_t a = item.find_first_not_of(" \t"), b = item.find_last_not_of(" \t");
+            out.push_back(a == string::npos ? string() : item.substr(a, b - a + 1));
+        }
+        return out;
+    }
+    vector<double> vec(const string &key)
+    {
+        vector<double> out;
+        for (auto &s : list(key)) out.push_back(std::atof(s.c_str()));
+        return out;
+    }
+    void warn_unused(std::ostream &os)
+    {
+        for (size_t i = 0; i < name.size(); i++)
+            if (!used[i]) os << "Unused line " << i << ": " << name[i] << "\n";
+    }
+};
+
+// ---------------------------------------------------------------------------
+// idl files (src/idl.cc:50-95)
+// ---------------------------------------------------------------------------
+struct IdlMap {
+    string name, date;
+    size_t sx = 0, sy = 0, sz = 0;
+    double hdr[7] = {0, 0, 0, 0, 0, 0, 0};  // scan x,y; offset x,y; bias lo,hi; setpoint
+    vector<double> data;                   // x + y*sx + z*sx*sy
+};
+
+bool read_idl(const string &path, IdlMap &m)
+{
+    std::ifstream is(path, std::ios::binary);
+    if (!is) return false;
+    std::getline(is, m.name);
+    std::getline(is, m.date);
+    is >> m.sx >> m.sy >> m.sz;
+    for (double &v : m.hdr) is >> v;
+    if (!is) return false;
+    is.ignore(1, '\n');
+    size_t tot = m.sx * m.sy * m.sz;
+    vector<float> raw(tot);
+    is.read(reinterpret_cast<char *>(raw.data()), (std::streamsize)(tot * sizeof(float)));
+    if (!is) return false;
+    m.data.assign(raw.begin(), raw.end());
+    return true;
+}
+
+bool write_idl(const string &path, const IdlMap &m)
+{
+    std::ofstream os(path, std::ios::binary);
+    if (!os) return false;
+    os << m.name << "\n" << m.date << "\n" << m.sx << "\n" << m.sy << "\n" << m.sz << "\n";
+    for (double v : m.hdr) os << v << "\n";
+    vector<float> raw(m.data.begin(), m.data.end());
+    os.write(reinterpret_cast<const char *>(raw.data()), (std::streamsize)(raw.size() * sizeof(float)));
+    return (bool)os;
+}
+
+int check(int rc, const char *what)
+{
+    if (rc) std::cerr << what << " failed: " << cqpi_last_error() << " (code " << rc << ")\n";
+    return rc;
+}
+
+string json_escape(const string &s)
+{
+    string o;
+    for (char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        o += c;
+    }
+    return o;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    bool print_only = false;
+    int argi = 1;
+    if (argc > 2 && string(argv[1]) == "--print-config") {
+        print_only = true;
+        argi = 2;
+    }
+    if (argi >= argc) {
+        std::cerr << "Syntax: calcqpi_amd [--print-config] configfile\n"
+                  << "Continuum-LDOS (output=wannier, green=normal) on one MI355X; keys as calcqpi,\n"
+                  << "plus precision=fp64|mixed; for the contraction.\n";
+        return 1;
+    }
+    Config lf(argv[argi]);
+    if (!lf.ok) {
+        std::cerr << "Error while opening file " << argv[argi] << " for reading.\n";
+        return 1;
+    }
+    std::ofstream logfile;
+    std::ostream *log = &std::cout;
+    if (lf.probe("logfile")) {
+        logfile.open(lf.str("logfile"), std::ios::app);
+        if (logfile.is_open()) log = &logfile;
+    }
+    // defaults: src/calcqpi.cc:133-137
+    long n = 201, layers = 21, oversamp = 4, window = 2, kpoints = 0;
+    bool spin = false, scmodel = false;
+    double lenergy = -1.0, henergy = 1.0, eta = 0.005, efermi = 0.0, magscat = 0.0;
+    std::complex<double> scatphase(1.0, 0.0);
+    vector<std::complex<double>> scattering;
+    string tbfile, qpifile, precision = "fp64";
+
+    if (lf.probe("tbfile")) tbfile = lf.str("tbfile");
+    else {
+        std::cerr << "No tightbinding model specified ... exiting.\n";
+        return 1;
+    }
+    if (lf.probe("fermi")) efermi = lf.num("fermi");
+    if (lf.probe("green") && lf.get("green") != "normal") {
+        std::cerr << "green=" << lf.get("green") << " is not supported by this build (normal only).\n";
+        return 1;
+    }
+    if (lf.probe("spin")) spin = lf.get("spin") == "true";
+    if (lf.probe("scmodel")) scmodel = lf.get("scmodel") == "true";
+    if (lf.probe("output") && lf.get("output") != "wannier") {
+        std::cerr << "output=" << lf.get("output") << " is not supported by this build (wannier only).\n";
+        return 1;
+    }
+    if (lf.probe("threshold")) {
+        std::cerr << "threshold lists are not supported by this build.\n";
+        return 1;
+    }
+    if (lf.probe("qpifile")) qpifile = lf.str("qpifile");
+    if (lf.probe("lattice")) n = lf.integer("lattice");
+    if (lf.probe("oversamp")) oversamp = lf.integer("oversamp");
+    if (lf.probe("window")) window = lf.integer("window");
+    if (lf.probe("layers")) layers = lf.integer("layers");
+    if (lf.probe("energies")) {
+        vector<double> en = lf.vec("energies");
+        if (en.size() >= 2) {
+            lenergy = en[0];
+            henergy = en[1];
+        }
+    }
+    if (lf.probe("eta")) eta = lf.num("eta");
+    if (lf.probe("kpoints")) kpoints = lf.integer("kpoints");
+    if (!kpoints) kpoints = n;
+    if (lf.probe("phase")) {
+        vector<double> ph = lf.vec("phase");
+        if (ph.size() >= 2) scatphase = std::complex<double>(ph[0], ph[1]);
+    }
+    if (lf.probe("scattering"))
+        for (double v : lf.vec("scattering")) scattering.push_back(v * scatphase);
+    if (lf.probe("magscat")) magscat = lf.num("magscat");
+    if (lf.probe("precision")) precision = lf.get("precision");
+    if (precision != "fp64" && precision != "mixed") {
+        std::cerr << "precision must be fp64 or mixed.\n";
+        return 1;
+    }
+
+    // model
+    cqpi_hopping *hops = nullptr;
+    size_t nhops = 0, bands = 0;
+    char tbname[1024];
+    if (check(cqpi_read_hr(tbfile.c_str(), scmodel, &hops, &nhops, &bands, tbname, sizeof tbname), "reading the TB model")) {
+        std::cerr << "Error loading TB model from " << tbfile << ", exiting.\n";
+        return 1;
+    }
+    size_t maxbands = bands;
+    if (scmodel) {
+        maxbands = bands >> 1;
+        if (scattering.size() == maxbands)
+            for (size_t i = 0; i < maxbands; i++) scattering.push_back((-1.0 + 2.0 * magscat) * scattering[i]);
+    }
+    // V = diag(vscat): src/qpi.h:107-130, src/calcqpi.cc:393-395
+    vector<std::complex<double>> vscat(bands, scatphase);
+    if (!scattering.empty())
+        for (size_t i = 0; i < bands; i++) vscat[i] = i < scattering.size() ? scattering[i] : 0.0;
+
+    // orbitals from the idl orbital file: src/calcqpi.cc:483-518
+    const size_t T = (size_t)(2 * window + 1) * oversamp;
+    vector<double> tiles(maxbands * T * T, 0.0);
+    if (!lf.probe("idlorbitalfile")) {
+        std::cerr << "This build takes the Wannier tiles from an idlorbitalfile only.\n";
+        return 1;
+    }
+    {
+        string orbfile = lf.str("idlorbitalfile");
+        IdlMap orb;
+        if (!read_idl(orbfile, orb)) {
+            std::cerr << "Error loading orbitals from file " << orbfile << ", exiting.\n";
+            return 1;
+        }
+        if (orb.sx != orb.sy || orb.sx != T) {
+            std::cerr << "Orbital functions in " << orbfile << " have unsuitable dimensions.\n";
+            return 1;
+        }
+        vector<double> prearr;
+        if (lf.probe("prearr")) prearr = lf.vec("prearr");
+        prearr.resize(std::max(orb.sz, maxbands), 1.0);
+        vector<vector<double>> wfs;
+        for (size_t k = 0; k < orb.sz; k++) {
+            vector<double> t(T * T);
+            for (size_t i = 0; i < T; i++)
+                for (size_t j = 0; j < T; j++) t[i * T + j] = prearr[k] * orb.data[i + j * T + k * T * T];
+            wfs.push_back(t);
+        }
+        if ((maxbands >> 1) == orb.sz && spin) {
+            size_t orbs = wfs.size();
+            for (size_t k = 0; k < orbs; k++) wfs.push_back(wfs[k]);
+        }
+        for (size_t o = 0; o < maxbands && o < wfs.size(); o++)
+            std::copy(wfs[o].begin(), wfs[o].end(), tiles.begin() + o * T * T);
+        if (wfs.size() < maxbands)
+            *log << "Warning: Fewer number of orbitals in " << orbfile << " than there are bands, padding with zeros.\n";
+    }
+    vector<double> omegas(layers);
+    for (long k = 0; k < layers; k++)
+        omegas[k] = layers == 1 ? lenergy : (henergy - lenergy) * (double)k / ((double)layers - 1.0) + lenergy;
+    const long W = oversamp * n;
+
+    if (print_only) {
+        std::cout.precision(17);
+        std::cout << "{\"tbfile\": \"" << json_escape(tbfile) << "\", \"qpifile\": \"" << json_escape(qpifile)
+                  << "\", \"name\": \"" << json_escape(tbname) << "\", \"bands\": " << bands
+                  << ", \"maxbands\": " << maxbands << ", \"hoppings\": " << nhops << ", \"lattice\": " << n
+                  << ", \"kpoints\": " << kpoints << ", \"oversamp\": " << oversamp << ", \"window\": " << window
+                  << ", \"layers\": " << layers << ", \"energies\": [" << lenergy << ", " << henergy
+                  << "], \"eta\": " << eta << ", \"fermi\": " << efermi << ", \"spin\": " << (spin ? "true" : "false")
+                  << ", \"scmodel\": " << (scmodel ? "true" : "false") << ", \"precision\": \"" << precision
+                  << "\", \"vscat\": [";
+        for (size_t i = 0; i < vscat.size(); i++)
+            std::cout << (i ? ", " : "") << "[" << vscat[i].real() << ", " << vscat[i].imag() << "]";
+        double tsum = 0;
+        for (double v : tiles) tsum += v;
+        std::cout << "], \"tile_sum\": " << tsum << "}\n";
+        cqpi_free(hops);
+        lf.warn_unused(std::cerr);
+        return 0;
+    }
+
+    *log << "calcqpi_amd: continuum LDOS on MI355X (" << cqpi_version() << ")\n"
+         << "         Tight-binding model  : " << tbfile << " (" << tbname << ", " << bands << " bands)\n"
+         << "         Output file for QPI  : " << qpifile << "\n"
+         << "         Lattice              : " << n << "x" << n << ", kpoints " << kpoints << "x" << kpoints << "\n"
+         << "         energy range         : (" << lenergy << "," << henergy << "), " << layers << " layers\n"
+         << "         broadening parameter : " << eta << ", window " << window << ", oversamp " << oversamp << "\n"
+         << "         contraction precision: " << precision << "\n";
+
+    auto t0 = std::chrono::steady_clock::now();
+    cqpi_ctx *ctx = nullptr;
+    if (check(cqpi_create((size_t)W, (size_t)kpoints, (size_t)n, (size_t)window, (size_t)oversamp, bands, maxbands,
+                          spin, tiles.data(), &ctx),
+              "cqpi_create"))
+        return 1;
+    char info[2048];
+    if (!cqpi_printinfo(ctx, info, sizeof info)) *log << info;
+    vector<double> V(2 * bands * bands, 0.0);
+    for (size_t i = 0; i < bands; i++) {
+        V[2 * (i * bands + i)] = vscat[i].real();
+        V[2 * (i * bands + i) + 1] = vscat[i].imag();
+    }
+    int rc = check(cqpi_set_model(ctx, hops, nhops, efermi, scmodel), "cqpi_set_model");
+    cqpi_free(hops);
+    if (!rc) rc = check(cqpi_set_impurity(ctx, V.data()), "cqpi_set_impurity");
+    if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed" ? CQPI_PREC_MIXED : CQPI_PREC_FP64), "cqpi_set_precision");
+    IdlMap out;
+    out.data.assign((size_t)W * W * layers, 0.0);
+    if (!rc) rc = check(cqpi_run(ctx, omegas.data(), omegas.size(), eta, out.data.data()), "cqpi_run");
+    cqpi_destroy(ctx);
+    if (rc) return 1;
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    std::ostringstream comment;
+    comment << "CQPI calculation for " << tbname << " with window=" << window << " and oversamp=" << oversamp;
+    out.name = comment.str();
+    time_t now = time(nullptr);
+    out.date = ctime(&now);
+    if (!out.date.empty() && out.date.back() == '\n') out.date.pop_back();
+    out.sx = out.sy = (size_t)W;
+    out.sz = (size_t)layers;
+    // idl(W,W,L, 2.0,2.0, lb,hb, 0.0, -1.0,-1.0) in file order: scan, offset, bias, setpoint
+    double hdr[7] = {2.0, 2.0, -1.0, -1.0, lenergy, henergy, 0.0};
+    std::copy(hdr, hdr + 7, out.hdr);
+    *log << "Calculation finished in " << secs << " s (" << (double)W * W * layers / secs
+         << " pixel-energy points/s including set-up and copies).\nSaving to " << qpifile << "\n";
+    if (!write_idl(qpifile, out)) {
+        std::cerr << "Error writing " << qpifile << "\n";
+        return 1;
+    }
+    *log << "Done.\n";
+    lf.warn_unused(std::cerr);
+    return 0;
+}

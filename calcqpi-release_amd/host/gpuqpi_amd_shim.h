@@ -1,0 +1,106 @@
+// gpuqpi_amd_shim.h -- GPUQPI plug-in for calcQPI's tmatrix that forwards to
+// libcqpi_amd.so.  Compiled inside the calcQPI source tree (it uses the
+// reference's GSL matrices, wannierfunctions and idl classes), selected in
+// src/qpi.cc the way HipQPI is selected (src/qpi.cc:34-43, :290-321).
+//
+// Behaviour follows HipQPI (src/qpihipkernel.cc:395-704): one context per
+// tmatrix, created at the first energy and reused; wannierldos copies G0
+// before returning and runs the contraction asynchronously; retrieveResult
+// synchronises.  Errors print the library's message and exit with its code,
+// as gpuAssert does (src/qpihipkernel.cc:372-380).
+#ifndef GPUQPI_AMD_SHIM_H
+#define GPUQPI_AMD_SHIM_H
+
+#include <cstdlib>
+#include <iostream>
+#include <vector>
+
+#include "cqpi_amd.h"
+#include "gpuqpi.h"
+#include "idl.h"
+#include "wannierfunctions.h"
+
+class AmdQPI : public GPUQPI
+{
+    cqpi_ctx *ctx = nullptr;
+    size_t W, K, bands;
+    std::vector<const double *> table;  // g0[ix][iy]->data, built once (matrices are reused)
+    std::vector<double> scatbuf, ldosbuf;
+
+    static void check(int rc)
+    {
+        if (rc) {
+            std::cerr << "CPU HIP error: " << cqpi_last_error() << std::endl;
+            exit(rc);
+        }
+    }
+    static void unsupported(const char *what)
+    {
+        std::cerr << what << " is not provided by the MI355X plug-in." << std::endl;
+        exit(1);
+    }
+
+public:
+    AmdQPI(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands_,
+           size_t maxband, bool spin, std::vector<wannierfunctions> &wf)
+        : W(wanniern), K(kpoints), bands(bands_)
+    {
+        const size_t T = (2 * window + 1) * oversamp;
+        std::vector<double> tiles(maxband * T * T, 0.0);  // zero tiles for padded "z" orbitals
+        for (size_t o = 0; o < maxband && o < wf.size(); o++)
+            if (!wf[o].iszero())
+                for (size_t i = 0; i < T; i++)
+                    for (size_t j = 0; j < T; j++) tiles[(o * T + i) * T + j] = wf[o].getwave_cached(i, j);
+        check(cqpi_create(wanniern, kpoints, n, window, oversamp, bands, maxband, spin, tiles.data(), &ctx));
+        scatbuf.resize(2 * bands * bands);
+    }
+    void printinfo(std::ostream &os)
+    {
+        char buf[2048];
+        check(cqpi_printinfo(ctx, buf, sizeof buf));
+        os << buf;
+    }
+    void wannierldos(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    {
+        for (size_t i = 0; i < bands; i++)
+            for (size_t j = 0; j < bands; j++) {
+                scatbuf[2 * (i * bands + j)] = scat->data[2 * (i * scat->tda + j)];
+                scatbuf[2 * (i * bands + j) + 1] = scat->data[2 * (i * scat->tda + j) + 1];
+            }
+        if (table.empty()) {
+            table.resize(K * K);
+            for (size_t ix = 0; ix < K; ix++)
+                for (size_t iy = 0; iy < K; iy++) table[ix * K + iy] = g0[ix][iy]->data;
+        }
+        check(cqpi_wannierldos(ctx, scatbuf.data(), table.data(), g0[0][0]->tda));
+    }
+    void wannierjosephson(gsl_matrix_complex *, std::vector<std::vector<gsl_matrix_complex *> > &, gsl_complex)
+    {
+        unsupported("Josephson mode");
+    }
+    void wannierldoslist(gsl_matrix_complex *, std::vector<std::vector<gsl_matrix_complex *> > &)
+    {
+        unsupported("The threshold-list contraction");
+    }
+    void spf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("SPF mode"); }
+    void uspf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("USPF mode"); }
+    void retrieveResult(double *ldos) { check(cqpi_retrieve(ctx, ldos)); }
+    void retrieveResult(idl &map, size_t layer)
+    {
+        ldosbuf.resize(W * W);
+        check(cqpi_retrieve(ctx, ldosbuf.data()));
+        for (size_t i = 0; i < W; i++)
+            for (size_t j = 0; j < W; j++) map.set(i, j, layer, ldosbuf[i + j * W]);
+    }
+    void retrieveResult(std::vector<std::vector<double> > &ldos)
+    {
+        ldosbuf.resize(W * W);
+        check(cqpi_retrieve(ctx, ldosbuf.data()));
+        ldos.assign(W, std::vector<double>(W));
+        for (size_t i = 0; i < W; i++)
+            for (size_t j = 0; j < W; j++) ldos[i][j] = ldosbuf[i + j * W];
+    }
+    ~AmdQPI() { cqpi_destroy(ctx); }
+};
+
+#endif

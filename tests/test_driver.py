@@ -1,0 +1,93 @@
+"""The calcqpi-compatible driver: input-file parsing (CPU) and a whole run
+writing the idl map (GPU), compared with the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import refmath as rm
+from cqpi import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "calcqpi-release_amd", "calcqpi_amd")
+
+
+def _print_config(path):
+    # file names in the input resolve against the working directory, as in calcqpi
+    r = subprocess.run([DRIVER, "--print-config", path], capture_output=True, text=True, timeout=60,
+                       cwd=os.path.dirname(path))
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout), r.stderr
+
+
+def test_driver_defaults_and_keys(tmp_path):
+    wl = synth.square_3orb(K=16, n=8, E=5)
+    p = synth.write_inputs(str(tmp_path), wl, extra={"phase": "(0.0,1.0)", "precision": "mixed"})
+    cfg, err = _print_config(p)
+    assert cfg["bands"] == 3 and cfg["lattice"] == 8 and cfg["kpoints"] == 16
+    assert cfg["layers"] == 5 and cfg["window"] == 2 and cfg["oversamp"] == 4
+    assert cfg["energies"] == [-0.1, 0.1] and cfg["precision"] == "mixed"
+    # scattering (0.5,0.5,0.5) times phase (0,1): src/calcqpi.cc:254-272
+    assert cfg["vscat"] == [[0.0, 0.5]] * 3
+    assert abs(cfg["tile_sum"] - wl.tiles.sum()) < 1e-9 * max(1.0, abs(wl.tiles.sum()))
+    assert "Unused" not in err
+
+
+def test_driver_parser_details(tmp_path):
+    """Comments, blank lines, padding inside values, missing scattering
+    (every orbital gets the phase), kpoints defaulting to lattice, unused keys."""
+    wl = synth.square_3orb(K=8, n=8, E=3)
+    synth.write_inputs(str(tmp_path), wl)
+    (tmp_path / "b.in").write_text(
+        "# a comment line\n\n"
+        'tbfile= "model.dat" ;  # trailing comment\n'
+        'idlorbitalfile="orbitals.idl";\n'
+        "lattice=  8 ;\nwindow=2;\noversamp=4;\n"
+        "phase=(0.25, -0.5);\n"
+        "frobnicate=3;\n")
+    cfg, err = _print_config(str(tmp_path / "b.in"))
+    assert cfg["kpoints"] == 8 and cfg["layers"] == 21 and cfg["eta"] == 0.005
+    assert cfg["energies"] == [-1.0, 1.0]
+    assert cfg["vscat"] == [[0.25, -0.5]] * 3
+    assert "Unused line" in err and "frobnicate" in err
+
+
+def test_driver_spin_duplicates_tiles(tmp_path):
+    wl = synth.hex_5d_spin(K=8, n=4, E=2)
+    p = synth.write_inputs(str(tmp_path), synth.Workload(**{**wl.__dict__, "V": np.diag(np.diag(wl.V)).real + 0j}))
+    cfg, _ = _print_config(p)
+    assert cfg["bands"] == 10 and cfg["spin"] is True
+    assert abs(cfg["tile_sum"] - wl.tiles.sum()) < 1e-9 * max(1.0, abs(wl.tiles.sum()))
+
+
+def test_driver_rejects_wrong_tile_size(tmp_path):
+    wl = synth.square_3orb(K=8, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path), wl, extra={})
+    txt = open(p).read().replace("window=2;", "window=1;")
+    open(p, "w").write(txt)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert r.returncode != 0 and "unsuitable dimensions" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_end_to_end_matches_oracle(tmp_path, gpu):
+    wl = synth.square_3orb(K=32, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path), wl)
+    print("case: driver run, config-2 model, K=32, n=8, E=3", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    data, hdr = synth.read_idl(str(tmp_path / "qpi.idl"))
+    assert data.shape == (wl.W, wl.W, 3)
+    assert hdr["bias"] == [-0.1, 0.1] and hdr["scan_range"] == [2.0, 2.0] and hdr["offset"] == [-1.0, -1.0]
+    assert hdr["name"].startswith("CQPI calculation for ")
+    # the oracle reads the same files the driver read
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
+    wl.hops = hops
+    ref = oracle.cldos_run(wl)
+    for e in range(3):
+        got = data[:, :, e].T.ravel()  # x + y*W
+        # the idl file stores float32
+        assert rm.rel_errors(got, ref[e].astype(np.float32).astype(np.float64))[0] < 1e-6

@@ -46,8 +46,8 @@ def test_no_device_is_reported_not_faked():
 
 @pytest.mark.parametrize("cfg", [1, 2, 3])
 def test_hr_reader_matches_oracle(tmp_path, cfg):
-    wl = {1: synth.config1, 2: lambda: synth.config2(K=8, n=4),
-          3: lambda: synth.config3(K=8, n=4)}[cfg]()
+    wl = {1: synth.square_1orb, 2: lambda: synth.square_3orb(K=8, n=4),
+          3: lambda: synth.hex_5d_spin(K=8, n=4)}[cfg]()
     p = str(tmp_path / "m_hr.dat")
     synth.write_hr(p, wl)
     h1, b1, n1 = cqpi.read_hr(p)
@@ -115,7 +115,7 @@ def test_idl_roundtrip_between_writers(tmp_path):
 
 
 def test_write_inputs(tmp_path):
-    wl = synth.config2(K=16, n=8, E=5)
+    wl = synth.square_3orb(K=16, n=8, E=5)
     p = synth.write_inputs(str(tmp_path), wl)
     txt = open(p).read()
     assert "scattering=(0.5,0.5,0.5);" in txt and "window=2;" in txt

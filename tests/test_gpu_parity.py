@@ -28,43 +28,43 @@ def _assert_close(got, ref, tol=TOL64):
 
 
 def test_full_chain_3orb_small(gpu):
-    wl = synth.config2_like(3, K=32, n=8, E=3, seed=11, w=1, os_=2, elo=-0.1, ehi=0.1, eta=0.01)
+    wl = synth.square_norb(3, K=32, n=8, E=3, seed=11, w=1, os_=2, elo=-0.1, ehi=0.1, eta=0.01)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_full_chain_config1_reduced(gpu):
-    wl = synth.config1(K=64, n=16, E=3)
+    wl = synth.square_1orb(K=64, n=16, E=3)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_full_chain_config2_shapes_reduced(gpu):
     # config-2 model, tiles, window and oversampling with a smaller grid and map
-    wl = synth.config2(K=64, n=16, E=3)
+    wl = synth.square_3orb(K=64, n=16, E=3)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_full_chain_spin_hex_reduced(gpu):
     # config-3 model family (hexagonal, 10 bands, SOC, full Hermitian V, spin blocks)
-    wl = synth.config3(K=16, n=6, E=2)
+    wl = synth.hex_5d_spin(K=16, n=6, E=2)
     wl.window, wl.oversamp = 1, 2
     wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_non_power_of_two_kgrid(gpu):
-    wl = synth.config2_like(3, K=24, n=8, E=2, seed=12, w=1, os_=2, eta=0.02)
+    wl = synth.square_norb(3, K=24, n=8, E=2, seed=12, w=1, os_=2, eta=0.02)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_map_larger_than_kgrid_odd_n(gpu):
     # n > K: the map repeats periodic images; odd n exercises the window bounds
-    wl = synth.config2_like(1, K=8, n=11, E=2, seed=13, w=2, os_=2, eta=0.05)
+    wl = synth.square_norb(1, K=8, n=11, E=2, seed=13, w=2, os_=2, eta=0.05)
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
 def test_degenerate_bands(gpu):
     # two identical decoupled orbitals: every band is exactly two-fold degenerate
-    base = synth.config1(K=32, n=8, E=2)
+    base = synth.square_1orb(K=32, n=8, E=2)
     b1 = {(int(h["rx"]), int(h["ry"])): complex(h["tre"], h["tim"]) for h in base.hops}
     blocks = {R: np.diag([t, t]) for R, t in b1.items()}
     wl = synth.Workload(name="doubled", bands=2, hops=synth.hops_from_blocks(blocks, 2), K=32, n=8,
@@ -75,7 +75,7 @@ def test_degenerate_bands(gpu):
 
 
 def test_g0_window_and_tmatrix(gpu):
-    wl = synth.config2_like(3, K=32, n=8, E=2, seed=14, w=1, os_=2, eta=0.01)
+    wl = synth.square_norb(3, K=32, n=8, E=2, seed=14, w=1, os_=2, eta=0.01)
     with cqpi.engine_for(wl) as q:
         q.run(wl.energies, wl.eta)
         h = q.window_half()
@@ -93,7 +93,7 @@ def test_g0_window_and_tmatrix(gpu):
 
 def test_dropin_wannierldos_matches_oracle(gpu):
     """GPUQPI::wannierldos path: host G0(R) and T in, one map out."""
-    wl = synth.config2(K=32, n=12, E=1)
+    wl = synth.square_3orb(K=32, n=12, E=1)
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
     gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
     T = oracle.tmatrix(wl.V, gR[0, 0])
@@ -113,7 +113,7 @@ def test_dropin_wannierldos_matches_oracle(gpu):
 def test_dropin_repeated_calls_and_zero_tile(gpu):
     """Two energies through the drop-in path, with one orbital's tile zero
     (the reference pads missing orbitals with zero functions)."""
-    wl = synth.config2(K=32, n=8, E=2)
+    wl = synth.square_3orb(K=32, n=8, E=2)
     tiles = wl.tiles.copy()
     tiles[1] = 0.0
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
@@ -133,7 +133,7 @@ def test_dropin_repeated_calls_and_zero_tile(gpu):
 
 def test_no_impurity_gives_periodic_map(gpu):
     """V = 0: rho is the lattice-periodic bulk term in every cell."""
-    wl = synth.config2(K=32, n=8, E=1)
+    wl = synth.square_3orb(K=32, n=8, E=1)
     wl.V = np.zeros((3, 3), complex)
     rho = _run(wl)[0].reshape(wl.W, wl.W)  # [y, x]
     cell = rho[: wl.oversamp, : wl.oversamp]
@@ -143,7 +143,7 @@ def test_no_impurity_gives_periodic_map(gpu):
 
 def test_singular_tmatrix_reported(gpu):
     """1 - V G0(0) singular: reported as an error, not a silent result."""
-    wl = synth.config1(K=16, n=4, E=1)
+    wl = synth.square_1orb(K=16, n=4, E=1)
     ev, V = oracle.calchamiltonian(wl.hops, 1, 0.0, wl.K)
     g00 = oracle.realspace(oracle.greens_k(ev, V, 0.2, 0.05))[0, 0, 0, 0]
     wl.V = np.array([[1.0 / g00]])  # makes 1 - V G0(0) exactly zero up to rounding
@@ -168,17 +168,17 @@ TOLMIX = 1e-4
 @pytest.mark.parametrize("case", ["c2", "c1", "hexspin", "oddn", "b5"])
 def test_mixed_matches_oracle(gpu, case):
     if case == "c2":
-        wl = synth.config2(K=64, n=16, E=3)
+        wl = synth.square_3orb(K=64, n=16, E=3)
     elif case == "c1":
-        wl = synth.config1(K=64, n=16, E=3)
+        wl = synth.square_1orb(K=64, n=16, E=3)
     elif case == "hexspin":
-        wl = synth.config3(K=16, n=6, E=2)
+        wl = synth.hex_5d_spin(K=16, n=6, E=2)
         wl.window, wl.oversamp = 1, 2
         wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
     elif case == "oddn":
-        wl = synth.config2_like(3, K=16, n=13, E=2, seed=31, w=2, os_=4, eta=0.01)
+        wl = synth.square_norb(3, K=16, n=13, E=2, seed=31, w=2, os_=4, eta=0.01)
     else:
-        wl = synth.config2_like(5, K=32, n=10, E=2, seed=32, w=2, os_=4, eta=0.01)
+        wl = synth.square_norb(5, K=32, n=10, E=2, seed=32, w=2, os_=4, eta=0.01)
     got = _run(wl, "mixed")
     ref = oracle.cldos_run(wl)
     for e in range(ref.shape[0]):
@@ -188,7 +188,7 @@ def test_mixed_matches_oracle(gpu, case):
 
 
 def test_mixed_dropin_path(gpu):
-    wl = synth.config2(K=32, n=12, E=1)
+    wl = synth.square_3orb(K=32, n=12, E=1)
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
     gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
     T = oracle.tmatrix(wl.V, gR[0, 0])

@@ -14,14 +14,14 @@ from cqpi import synth
 
 @pytest.fixture(scope="module")
 def small3():
-    wl = synth.config2_like(3, K=8, n=8, E=1, seed=7, w=1, os_=2, eta=0.01)
+    wl = synth.square_norb(3, K=8, n=8, E=1, seed=7, w=1, os_=2, eta=0.01)
     wl.efermi = 0.1
     wl.V = np.array([[0.5, 0.1 - 0.2j, 0], [0.1 + 0.2j, 0.3, 0.05], [0, 0.05, 0.2]])
     return wl
 
 
 def test_setmatrix_closed_form_one_band():
-    wl = synth.config1()
+    wl = synth.square_1orb()
     for kx, ky in [(0.0, 0.0), (0.25, -0.125), (0.5, 0.5), (-0.3, 0.1)]:
         H = oracle.setmatrix(wl.hops, 1, 0.2, kx, ky)
         eps = -0.5 + 2 * (-1.0) * (np.cos(2 * np.pi * kx) + np.cos(2 * np.pi * ky)) \
@@ -79,7 +79,7 @@ def test_realspace_matches_numpy_ifft2(K):
 
 def test_realspace_sum_rule_and_one_band_closed_form():
     # sum_R G0(R) = G0(k=0); one band: G0(R) = 1/K^2 sum_k e^{2 pi i kR}/(z - eps_k)
-    wl = synth.config1(K=16)
+    wl = synth.square_1orb(K=16)
     ev, V = oracle.calchamiltonian(wl.hops, 1, 0.0, 16)
     gk = oracle.greens_k(ev, V, 0.1, 0.05)
     gR = oracle.realspace(gk)
@@ -117,7 +117,7 @@ def test_contraction_matches_factorised(small3):
 
 
 def test_contraction_spin_blocks_factorised():
-    wl = synth.config2_like(3, K=6, n=6, E=1, seed=21, w=1, os_=2)
+    wl = synth.square_norb(3, K=6, n=6, E=1, seed=21, w=1, os_=2)
     # 4 bands, spin: blocks {0,1} and {2,3}
     rng = np.random.default_rng(2)
     gR = rng.normal(size=(6, 6, 4, 4)) + 1j * rng.normal(size=(6, 6, 4, 4))
