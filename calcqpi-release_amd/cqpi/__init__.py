@@ -57,6 +57,8 @@ SIGNATURES = {
     "cqpi_read_hr": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_sz),
                                C.c_char_p, _sz]),
     "cqpi_free": (None, [_vp]),
+    "cqpi_set_keep_windows": (C.c_int, [_vp, C.c_int]),
+    "cqpi_gradient": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -137,6 +139,7 @@ class AmdQPI:
         _check(lib().cqpi_create(self.W, self.K, self.n, self.window, self.oversamp, self.bands,
                                  self.maxband, int(self.spin), _d(wf), C.byref(self._ctx)))
         self.nlayers = 0
+        self._nhops = 0
 
     # -- GPUQPI ------------------------------------------------------------
     def wannierldos(self, scat, g0):
@@ -170,6 +173,7 @@ class AmdQPI:
 
     def set_model(self, hops, efermi=0.0, scmodel=False):
         hops = np.ascontiguousarray(hops, dtype=synth.HOP_DTYPE)
+        self._nhops = len(hops)
         _check(lib().cqpi_set_model(self._ctx, hops.ctypes.data if len(hops) else None, len(hops),
                                     float(efermi), int(scmodel)))
 
@@ -220,6 +224,23 @@ class AmdQPI:
         n = (C.c_int64 * 5)()
         _check(lib().cqpi_stage_times(self._ctx, _d(ms), n))
         return {STAGES[i]: (float(ms[i]), int(n[i])) for i in range(5)}
+
+    def set_keep_windows(self, on=True):
+        _check(lib().cqpi_set_keep_windows(self._ctx, int(on)))
+
+    def gradient(self, rhobar):
+        """Adjoint of the last run (kept windows): rhobar = dL/drho, shape
+        (E, W*W).  Returns (g_hops (nhops, 2) = dL/d(Re t, Im t),
+        g_mu, g_V (b x b complex, dL = Re sum conj(g_V) dV))."""
+        rb = np.ascontiguousarray(rhobar, dtype=np.float64)
+        if rb.size != self.nlayers * self.W * self.W:
+            raise ValueError("rhobar must hold E*W*W values of the last run")
+        nh = self._nhops
+        gh = np.zeros((max(nh, 1), 2))
+        gmu = np.zeros(1)
+        gV = np.zeros((self.bands, self.bands), np.complex128)
+        _check(lib().cqpi_gradient(self._ctx, _d(rb), _d(gh), _d(gmu), _d(gV.view(np.float64))))
+        return gh[:nh], float(gmu[0]), gV
 
     def device_ldos(self):
         p = _vp()

@@ -1,0 +1,582 @@
+// Hand-written adjoint of the continuum-LDOS chain (FP64).
+//
+// Forward (per energy):  H(k) = sum_h t_h e^{+-2 pi i R_h.k} -/+ mu
+//   G(k) = (w + i eta - H(k))^{-1};  G(R) = K^-2 sum_k G(k) e^{+2 pi i k.R}
+//   T = (1 - V G(0))^{-1} V;  rho(x) = -Im sum_s a^T T b - bulk(x)
+// with a_p = sum w_o G(pos+R)_{op}, b_q = sum G(-pos-R)_{qo} w_o,
+// bulk = sum w_o1 w_o2 Im G(R2-R1)_{o2 o1}.
+// Given rhobar = dL/drho, this returns dL/dt_h (real and imaginary parts),
+// dL/dmu and dL/dV.  Complex adjoints use dL = Re sum conj(Zbar) dZ:
+//   sbar = -i rhobar;  abar_p = sbar conj((T b)_p);  bbar_q = sbar conj((a^T T)_q)
+//   Tbar += sbar conj(a b^T)
+//   Gbar(pos+R)_{op} += sum_x w_o abar_p;  Gbar(-pos-R)_{qo} += sum_x w_o bbar_q
+//   Gbar(R2-R1)_{o2 o1} += -i rhobar w_o1 w_o2            (periodic term)
+//   Gbar(0) += T^H Tbar T^H;  Vbar = M^-H Tbar (1 + G(0) T)^H, M = 1 - V G(0)
+//   Gbar(k) = K^-2 sum_R Gbar(R) e^{-2 pi i k.R}           (adjoint transform)
+//   Hbar(k) = G(k)^H Gbar(k) G(k)^H
+//   dL/dt_h = sum_k conj(Hbar_{o1 o2}(k)) phi_h(k);  dL/dmu = -/+ Re tr Hbar.
+// G(k) is never stored: the last pass rebuilds it per energy from the cached
+// eigen data.  The R-window of G and T come from the forward run.
+#include "cqpi_common.h"
+#include "cqpi_fft.h"
+
+namespace {
+
+using cqpi::brev;
+using cqpi::imod;
+
+constexpr int kCells = 8;
+constexpr int kWaves = 4;
+
+__device__ inline double wsum_reduce(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// abar, bbar per pixel and per spin block; partial Tbar per workgroup
+template <int B, int P>
+__global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *__restrict__ G0w,
+                                                       const double2 *__restrict__ Tm,
+                                                       const double *__restrict__ tiles,
+                                                       const double *__restrict__ rhobar,
+                                                       double2 *__restrict__ abar, double2 *__restrict__ bbar,
+                                                       double2 *__restrict__ Tpart)
+{
+    __shared__ double2 red[kWaves][B * B];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cx = blockIdx.x * kCells + (lane & (kCells - 1));
+    const int cy = blockIdx.y * kCells + (lane >> 3);
+    const int e = blockIdx.z;
+    const bool active = cx < g.n && cy < g.n;
+    const int os = g.os, w = g.w, T = g.T, nw = g.nw, h = g.h, W = g.W;
+    const int NP = B * B;
+    const int n2 = g.n >> 1;
+    const int px = (active ? cx : 0) - n2, py = (active ? cy : 0) - n2;
+    const int npix = os * os;
+    const int npg = (npix + P - 1) / P;
+    const int nsets = g.spin ? 2 : 1;
+    const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
+    const double2 *Te = Tm + (size_t)e * NP;
+    double2 tacc[B * B];
+#pragma unroll
+    for (int i = 0; i < NP; i++) tacc[i] = make_double2(0.0, 0.0);
+
+    for (int pg = wave; pg < npg; pg += kWaves) {
+        int ii[P], jj[P];
+        bool ok[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            int pix = pg * P + p;
+            ok[p] = active && pix < npix;
+            pix = pix < npix ? pix : 0;
+            ii[p] = pix / os;
+            jj[p] = pix - ii[p] * os;
+        }
+        for (int sg = 0; sg < nsets; sg++) {
+            const int olo = sg * nper, ohi = olo + nper;
+            double2 a[P][B], b[P][B];
+#pragma unroll
+            for (int p = 0; p < P; p++)
+#pragma unroll
+                for (int q = 0; q < B; q++) {
+                    a[p][q] = make_double2(0.0, 0.0);
+                    b[p][q] = make_double2(0.0, 0.0);
+                }
+            for (int rx = -w; rx <= w; rx++)
+                for (int ry = -w; ry <= w; ry++) {
+                    const double2 *GA = G0 + ((size_t)(px + rx + h) * nw + (py + ry + h)) * NP;
+                    const double2 *GB = G0 + ((size_t)(-px - rx + h) * nw + (-py - ry + h)) * NP;
+                    const int tx = (w - rx) * os, ty = (w - ry) * os;
+                    for (int o = olo; o < ohi; o++) {
+                        double wt[P];
+#pragma unroll
+                        for (int p = 0; p < P; p++) wt[p] = tiles[((size_t)o * T + ii[p] + tx) * T + jj[p] + ty];
+#pragma unroll
+                        for (int q = 0; q < B; q++) {
+                            double2 ga = GA[o * B + q], gb = GB[q * B + o];
+#pragma unroll
+                            for (int p = 0; p < P; p++) {
+                                a[p][q] = cadd(a[p][q], cscale(wt[p], ga));
+                                b[p][q] = cadd(b[p][q], cscale(wt[p], gb));
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                if (!ok[p]) continue;
+                const int X = cx * os + ii[p], Y = cy * os + jj[p];
+                const size_t pixid = (size_t)X + (size_t)Y * W;
+                const double rb = rhobar[(size_t)e * W * W + pixid];
+                const double2 sbar = make_double2(0.0, -rb);
+                double2 *ab = abar + (((size_t)e * nsets + sg) * W * W + pixid) * B;
+                double2 *bb = bbar + (((size_t)e * nsets + sg) * W * W + pixid) * B;
+#pragma unroll
+                for (int q1 = 0; q1 < B; q1++) {
+                    double2 tb = make_double2(0.0, 0.0), at = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int q2 = 0; q2 < B; q2++) {
+                        tb = cadd(tb, cmul(Te[q1 * B + q2], b[p][q2]));
+                        at = cadd(at, cmul(a[p][q2], Te[q2 * B + q1]));
+                    }
+                    ab[q1] = cmul(sbar, cconj(tb));
+                    bb[q1] = cmul(sbar, cconj(at));
+                }
+#pragma unroll
+                for (int q1 = 0; q1 < B; q1++)
+#pragma unroll
+                    for (int q2 = 0; q2 < B; q2++)
+                        tacc[q1 * B + q2] = cadd(tacc[q1 * B + q2], cmul(sbar, cconj(cmul(a[p][q1], b[p][q2]))));
+            }
+        }
+    }
+    // Tbar partial: wave reduction, then across waves
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+        double re = wsum_reduce(tacc[i].x), im = wsum_reduce(tacc[i].y);
+        if (lane == 0) red[wave][i] = make_double2(re, im);
+    }
+    __syncthreads();
+    if (threadIdx.x < NP) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int v = 0; v < kWaves; v++) s = cadd(s, red[v][threadIdx.x]);
+        size_t blk = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        Tpart[((size_t)e * gridDim.x * gridDim.y + blk) * NP + threadIdx.x] = s;
+    }
+}
+
+// Tbar[e] = sum of the per-workgroup partials; rhosum[e][pix] = sum over cells of rhobar
+__global__ void k_adj_reduce(Geom g, const double2 *__restrict__ Tpart, int nblk, const double *__restrict__ rhobar,
+                             double2 *__restrict__ Tbar, double *__restrict__ rhosum)
+{
+    __shared__ double r1[256], r2[256];
+    const int e = blockIdx.y, item = blockIdx.x;
+    const int NP = g.B * g.B, os = g.os, W = g.W;
+    double s1 = 0.0, s2 = 0.0;
+    if (item < NP) {
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+            double2 v = Tpart[((size_t)e * nblk + b) * NP + item];
+            s1 += v.x;
+            s2 += v.y;
+        }
+    } else {
+        const int pix = item - NP, ii = pix / os, jj = pix % os;
+        for (int c = threadIdx.x; c < g.n * g.n; c += blockDim.x) {
+            int cx = c % g.n, cy = c / g.n;
+            s1 += rhobar[(size_t)e * W * W + (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W];
+        }
+    }
+    r1[threadIdx.x] = s1;
+    r2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            r1[threadIdx.x] += r1[threadIdx.x + s];
+            r2[threadIdx.x] += r2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (item < NP)
+            Tbar[(size_t)e * NP + item] = make_double2(r1[0], r2[0]);
+        else
+            rhosum[(size_t)e * os * os + item - NP] = r1[0];
+    }
+}
+
+// Gbar on the R window: contraction transpose (a and b gathers) + periodic term
+__global__ void k_adj_window(Geom g, const double *__restrict__ tiles, const double2 *__restrict__ abar,
+                             const double2 *__restrict__ bbar, const double *__restrict__ rhosum,
+                             double2 *__restrict__ Gbar)
+{
+    const int B = g.B, NP = B * B, nw = g.nw, h = g.h, os = g.os, w = g.w, T = g.T, W = g.W, n = g.n;
+    const long total = (long)nw * nw * NP;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y;
+    if (idx >= total) return;
+    const int pr = (int)(idx % NP);
+    const long s = idx / NP;
+    const int sy = (int)(s % nw), sx = (int)(s / nw);
+    const int r = pr / B, c = pr % B;
+    const int Sx = sx - h, Sy = sy - h;
+    const int n2 = n >> 1;
+    const int nsets = g.spin ? 2 : 1;
+    const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    double2 acc = make_double2(0.0, 0.0);
+    // a gather: Gbar(pos+R)_{o=r, p=c} += sum_x w_r(x;R) abar_c(pos, x)
+    if (r < nsets * nper) {
+        const int sg = r / nper;
+        for (int rx = -w; rx <= w; rx++)
+            for (int ry = -w; ry <= w; ry++) {
+                const int cx = Sx - rx + n2, cy = Sy - ry + n2;
+                if (cx < 0 || cy < 0 || cx >= n || cy >= n) continue;
+                const double2 *ab = abar + ((size_t)e * nsets + sg) * W * W * B;
+                for (int ii = 0; ii < os; ii++)
+                    for (int jj = 0; jj < os; jj++) {
+                        double wt = tiles[((size_t)r * T + ii + (w - rx) * os) * T + jj + (w - ry) * os];
+                        if (wt == 0.0) continue;
+                        size_t pix = (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W;
+                        acc = cadd(acc, cscale(wt, ab[pix * B + c]));
+                    }
+            }
+    }
+    // b gather: Gbar(-pos-R)_{q=r, o=c} += sum_x w_c(x;R) bbar_r(pos, x)
+    if (c < nsets * nper) {
+        const int sg = c / nper;
+        for (int rx = -w; rx <= w; rx++)
+            for (int ry = -w; ry <= w; ry++) {
+                const int cx = -Sx - rx + n2, cy = -Sy - ry + n2;
+                if (cx < 0 || cy < 0 || cx >= n || cy >= n) continue;
+                const double2 *bb = bbar + ((size_t)e * nsets + sg) * W * W * B;
+                for (int ii = 0; ii < os; ii++)
+                    for (int jj = 0; jj < os; jj++) {
+                        double wt = tiles[((size_t)c * T + ii + (w - rx) * os) * T + jj + (w - ry) * os];
+                        if (wt == 0.0) continue;
+                        size_t pix = (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W;
+                        acc = cadd(acc, cscale(wt, bb[pix * B + r]));
+                    }
+            }
+    }
+    // periodic term: Gbar(R2-R1)_{o2=r, o1=c} += -i sum_x rhobar w_c(x;R1) w_r(x;R2), same spin block
+    if (r < nsets * nper && c < nsets * nper && r / nper == c / nper && abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
+        double re = 0.0;
+        for (int r1x = -w; r1x <= w; r1x++)
+            for (int r1y = -w; r1y <= w; r1y++) {
+                const int r2x = Sx + r1x, r2y = Sy + r1y;
+                if (r2x < -w || r2x > w || r2y < -w || r2y > w) continue;
+                for (int ii = 0; ii < os; ii++)
+                    for (int jj = 0; jj < os; jj++) {
+                        double w1 = tiles[((size_t)c * T + ii + (w - r1x) * os) * T + jj + (w - r1y) * os];
+                        double w2 = tiles[((size_t)r * T + ii + (w - r2x) * os) * T + jj + (w - r2y) * os];
+                        re += rhosum[(size_t)e * os * os + ii * os + jj] * w1 * w2;
+                    }
+            }
+        acc = cadd(acc, make_double2(0.0, -re));
+    }
+    Gbar[(size_t)e * total + idx] = acc;
+}
+
+constexpr int kMaxB = 16;
+
+// Gbar(0) += T^H Tbar T^H;  Vbar[e] = M^-H Tbar (1 + G(0) T)^H
+__global__ void k_adj_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
+                              const double2 *__restrict__ Tm, const double2 *__restrict__ Tbar, int E,
+                              double2 *__restrict__ Gbar, double2 *__restrict__ Vbar)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int B = g.B, NP = B * B;
+    const size_t off0 = ((size_t)e * g.nw * g.nw + (size_t)g.h * g.nw + g.h) * NP;
+    const double2 *G = G0w + off0;
+    const double2 *T = Tm + (size_t)e * NP;
+    const double2 *Tb = Tbar + (size_t)e * NP;
+    double2 M[kMaxB * kMaxB], Mi[kMaxB * kMaxB], X[kMaxB * kMaxB];
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) {
+            double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+            for (int k = 0; k < B; k++) acc = csub(acc, cmul(V[r * B + k], G[k * B + c]));
+            M[r * kMaxB + c] = acc;
+            Mi[r * kMaxB + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+        }
+    for (int j = 0; j < B; j++) {  // Gauss-Jordan, partial pivoting
+        int piv = j;
+        double best = cabs2(M[j * kMaxB + j]);
+        for (int i = j + 1; i < B; i++)
+            if (cabs2(M[i * kMaxB + j]) > best) { best = cabs2(M[i * kMaxB + j]); piv = i; }
+        if (best == 0.0) break;
+        if (piv != j)
+            for (int c = 0; c < B; c++) {
+                double2 t = M[j * kMaxB + c]; M[j * kMaxB + c] = M[piv * kMaxB + c]; M[piv * kMaxB + c] = t;
+                t = Mi[j * kMaxB + c]; Mi[j * kMaxB + c] = Mi[piv * kMaxB + c]; Mi[piv * kMaxB + c] = t;
+            }
+        double2 ip = cinv(M[j * kMaxB + j]);
+        for (int i = 0; i < B; i++) {
+            if (i == j) continue;
+            double2 l = cmul(M[i * kMaxB + j], ip);
+            for (int c = 0; c < B; c++) {
+                M[i * kMaxB + c] = csub(M[i * kMaxB + c], cmul(l, M[j * kMaxB + c]));
+                Mi[i * kMaxB + c] = csub(Mi[i * kMaxB + c], cmul(l, Mi[j * kMaxB + c]));
+            }
+        }
+        for (int c = 0; c < B; c++) {
+            M[j * kMaxB + c] = cmul(M[j * kMaxB + c], ip);
+            Mi[j * kMaxB + c] = cmul(Mi[j * kMaxB + c], ip);
+        }
+    }
+    // X = 1 + G T
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) {
+            double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+            for (int k = 0; k < B; k++) acc = cadd(acc, cmul(G[r * B + k], T[k * B + c]));
+            X[r * kMaxB + c] = acc;
+        }
+    // Vbar = Mi^H Tb X^H
+    double2 *Vb = Vbar + (size_t)e * NP;
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int k = 0; k < B; k++) {
+                double2 tx = make_double2(0.0, 0.0);  // (Tb X^H)_{kc}
+                for (int l = 0; l < B; l++) tx = cadd(tx, cmul(Tb[k * B + l], cconj(X[c * kMaxB + l])));
+                acc = cadd(acc, cmul(cconj(Mi[k * kMaxB + r]), tx));
+            }
+            Vb[r * B + c] = acc;
+        }
+    // Gbar(0) += T^H Tb T^H
+    double2 *G0b = Gbar + off0;
+    for (int r = 0; r < B; r++)
+        for (int c = 0; c < B; c++) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int k = 0; k < B; k++) {
+                double2 tt = make_double2(0.0, 0.0);  // (Tb T^H)_{kc}
+                for (int l = 0; l < B; l++) tt = cadd(tt, cmul(Tb[k * B + l], cconj(T[c * B + l])));
+                acc = cadd(acc, cmul(cconj(T[k * B + r]), tt));
+            }
+            G0b[r * B + c] = cadd(G0b[r * B + c], acc);
+        }
+}
+
+constexpr int kThreads = 256;
+
+// adjoint of the kx pass: F1bar[e][i][sy][pair] = K^-2 sum_sx Gbar[e][sx][sy][pair] e^{-2 pi i i (sx-h)/K}
+__global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__restrict__ Gbar,
+                                                   const double2 *__restrict__ twn, int np,
+                                                   double2 *__restrict__ F1bar)
+{
+    extern __shared__ double2 S[];
+    const int K = g.K, sy = blockIdx.x, e = blockIdx.y, NP = g.B * g.B, p0 = blockIdx.z * np;
+    const int npc = min(np, NP - p0);
+    const int nw = g.nw, h = g.h;
+    int lgK = 0;
+    while ((1 << lgK) < K) lgK++;
+    double2 *Tw = S + (size_t)np * K;
+    for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
+    const double2 *in = Gbar + (size_t)e * nw * nw * NP;
+    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
+        int m = t / npc, q = t - m * npc;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int sx = imod(m + h, K); sx < nw; sx += K) acc = cadd(acc, in[((size_t)sx * nw + sy) * NP + p0 + q]);
+        S[(size_t)q * K + brev(m, lgK)] = acc;
+    }
+    __syncthreads();
+    cqpi::fft_lines(S, npc, K, lgK, Tw, -1);
+    const double inv = 1.0 / ((double)K * (double)K);
+    double2 *out = F1bar + (size_t)e * K * nw * NP;
+    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
+        int i = t / npc, q = t - i * npc;
+        out[((size_t)i * nw + sy) * NP + p0 + q] = cscale(inv, S[(size_t)q * K + i]);
+    }
+}
+
+// adjoint of the ky pass fused with the resolvent adjoint, summed over energies:
+// Gbar(k) for one kx row, then Hacc(k) += G(k)^H Gbar(k) G(k)^H with G(k) rebuilt
+// from the eigen data.  One workgroup per row, all orbital pairs in LDS.
+template <int B>
+__global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__restrict__ eval,
+                                                   const double2 *__restrict__ evec,
+                                                   const double *__restrict__ omegas, int E, double eta,
+                                                   const double2 *__restrict__ F1bar,
+                                                   const double2 *__restrict__ twn, double2 *__restrict__ Hacc)
+{
+    extern __shared__ double2 S[];
+    constexpr int NP = B * B;
+    constexpr int JPT = 4;
+    const int K = g.K, i = blockIdx.x, nw = g.nw, h = g.h;
+    const long KK = (long)K * K;
+    int lgK = 0;
+    while ((1 << lgK) < K) lgK++;
+    double2 *Tw = S + (size_t)NP * K;
+    for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
+    double2 hacc[JPT][NP];
+#pragma unroll
+    for (int t = 0; t < JPT; t++)
+#pragma unroll
+        for (int p = 0; p < NP; p++) hacc[t][p] = make_double2(0.0, 0.0);
+    for (int e = 0; e < E; e++) {
+        __syncthreads();
+        const double2 *in = F1bar + ((size_t)e * K + i) * (size_t)nw * NP;
+        for (int t = threadIdx.x; t < K * NP; t += blockDim.x) {
+            int m = t / NP, q = t - m * NP;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int sy = imod(m + h, K); sy < nw; sy += K) acc = cadd(acc, in[(size_t)sy * NP + q]);
+            S[(size_t)q * K + brev(m, lgK)] = acc;
+        }
+        __syncthreads();
+        cqpi::fft_lines(S, NP, K, lgK, Tw, -1);
+        const double omega = omegas[e];
+#pragma unroll
+        for (int t = 0; t < JPT; t++) {
+            const int j = threadIdx.x + t * kThreads;
+            if (j >= K) break;
+            const long k = (long)i * K + j;
+            double2 f[B];
+#pragma unroll
+            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - eval[nn * KK + k], eta));
+            double2 G[NP], Gb[NP], X[NP];
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int nn = 0; nn < B; nn++)
+                        acc = cadd(acc, cmul(f[nn], cmulc(evec[(r * B + nn) * KK + k], evec[(c * B + nn) * KK + k])));
+                    G[r * B + c] = acc;
+                    Gb[r * B + c] = S[(size_t)(r * B + c) * K + j];
+                }
+            // X = Gb G^H, then Hbar = G^H X
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int l = 0; l < B; l++) acc = cadd(acc, cmulc(Gb[r * B + l], G[c * B + l]));
+                    X[r * B + c] = acc;
+                }
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int l = 0; l < B; l++) acc = cadd(acc, cmul(cconj(G[l * B + r]), X[l * B + c]));
+                    hacc[t][r * B + c] = cadd(hacc[t][r * B + c], acc);
+                }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < JPT; t++) {
+        const int j = threadIdx.x + t * kThreads;
+        if (j >= K) break;
+        const long k = (long)i * K + j;
+#pragma unroll
+        for (int p = 0; p < NP; p++) Hacc[k * NP + p] = hacc[t][p];
+    }
+}
+
+__device__ inline double kcoord(int i, int K)
+{
+    return (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
+}
+
+// SR[r][pair] = sum_k Hacc(k)[pair] e^{-2 pi i R_r.k}; row nR holds the traces for mu:
+// SR[nR][0].x = sum_k sum_i s_i Re Hacc_ii(k), s_i = -1 (i < scmax), +1 otherwise
+__global__ void k_adj_hsum(Geom g, const double2 *__restrict__ Hacc, const double2 *__restrict__ Rlist, int nR,
+                           int scmax, double2 *__restrict__ SR)
+{
+    __shared__ double r1[256], r2[256];
+    const int ridx = blockIdx.x, pr = blockIdx.y;
+    const int B = g.B, NP = B * B, K = g.K;
+    const long KK = (long)K * K;
+    double s1 = 0.0, s2 = 0.0;
+    if (ridx < nR) {
+        const double2 R = Rlist[ridx];
+        for (long k = threadIdx.x; k < KK; k += blockDim.x) {
+            int i = (int)(k / K), j = (int)(k % K);
+            double arg = -2.0 * M_PI * (R.x * kcoord(i, K) + R.y * kcoord(j, K));
+            double sn, cs;
+            sincos(arg, &sn, &cs);
+            double2 v = cmul(Hacc[k * NP + pr], make_double2(cs, sn));
+            s1 += v.x;
+            s2 += v.y;
+        }
+    } else if (pr == 0) {
+        for (long k = threadIdx.x; k < KK; k += blockDim.x)
+            for (int d = 0; d < B; d++) s1 += (d < scmax ? -1.0 : 1.0) * Hacc[k * NP + d * B + d].x;
+    }
+    r1[threadIdx.x] = s1;
+    r2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            r1[threadIdx.x] += r1[threadIdx.x + s];
+            r2[threadIdx.x] += r2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) SR[(size_t)ridx * NP + pr] = make_double2(r1[0], r2[0]);
+}
+
+template <int B>
+int launch_pix_t(const Geom &g, const double2 *G0w, const double2 *T, const double *tiles, const double *rhobar,
+                 int E, double2 *abar, double2 *bbar, double2 *Tpart, int &nblk, hipStream_t s)
+{
+    constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 6 ? 2 : 1));
+    int nb = (g.n + kCells - 1) / kCells;
+    nblk = nb * nb;
+    hipLaunchKernelGGL((k_adj_pix<B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, tiles, rhobar,
+                       abar, bbar, Tpart);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+template <int B>
+int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
+                const double2 *F1bar, const double2 *twn, double2 *Hacc, hipStream_t s)
+{
+    size_t lds = sizeof(double2) * ((size_t)B * B * g.K + (size_t)(g.K >> 1));
+    hipLaunchKernelGGL(k_adj_ky<B>, dim3(g.K), dim3(kThreads), lds, s, g, eval, evec, omegas, E, eta, F1bar, twn,
+                       Hacc);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+}  // namespace
+
+namespace cqpi {
+
+size_t adjoint_pix_blocks(const Geom &g)
+{
+    size_t nb = (g.n + kCells - 1) / kCells;
+    return nb * nb;
+}
+
+bool adjoint_supported(const Geom &g)
+{
+    bool pow2 = (g.K & (g.K - 1)) == 0;
+    size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
+    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024;
+}
+
+int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const double2 *V, const double *tiles,
+                   const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
+                   double eta, const double2 *twn, const double2 *Rlist, int nR, int scmax, double2 *abar,
+                   double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar, double2 *Vbar,
+                   double2 *F1bar, double2 *Hacc, double2 *SR, hipStream_t s)
+{
+    if (!adjoint_supported(g)) return CQPI_E_ARG;
+    int nblk = 0, r = CQPI_E_ARG;
+    switch (g.B) {
+    case 1: r = launch_pix_t<1>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
+    case 2: r = launch_pix_t<2>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
+    case 3: r = launch_pix_t<3>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
+    case 4: r = launch_pix_t<4>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
+    }
+    if (r) return r;
+    const int NP = g.B * g.B;
+    hipLaunchKernelGGL(k_adj_reduce, dim3(NP + g.os * g.os, E), dim3(256), 0, s, g, Tpart, nblk, rhobar, Tbar, rhosum);
+    long total = (long)g.nw * g.nw * NP;
+    hipLaunchKernelGGL(k_adj_window, dim3((unsigned)((total + 255) / 256), E), dim3(256), 0, s, g, tiles, abar, bbar,
+                       rhosum, Gbar);
+    hipLaunchKernelGGL(k_adj_tmatrix, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar);
+    // adjoint transform along kx (pairs chunked to the LDS budget), then ky + resolvent
+    int npmax = (int)((32 * 1024) / (sizeof(double2) * (size_t)g.K));
+    if (npmax < 1) npmax = 1;
+    int nch = (NP + npmax - 1) / npmax, np = (NP + nch - 1) / nch;
+    size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
+    hipLaunchKernelGGL(k_adj_kx, dim3(g.nw, E, nch), dim3(kThreads), lds, s, g, Gbar, twn, np, F1bar);
+    if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    switch (g.B) {
+    case 1: r = launch_ky_t<1>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+    case 2: r = launch_ky_t<2>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+    case 3: r = launch_ky_t<3>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+    case 4: r = launch_ky_t<4>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+    }
+    if (r) return r;
+    hipLaunchKernelGGL(k_adj_hsum, dim3(nR + 1, NP), dim3(256), 0, s, g, Hacc, Rlist, nR, scmax, SR);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+}  // namespace cqpi

@@ -138,6 +138,19 @@ struct cqpi_ctx {
     _Float16 *d_planes = nullptr;
     size_t c_planes = 0;
 
+    // gradient: kept forward state and adjoint buffers
+    std::vector<cqpi_hopping> h_hops;  // model as given to cqpi_set_model
+    bool keep_windows = false, windows_kept = false;
+    std::vector<double> last_omegas;
+    double last_eta = 0.0;
+    double2 *d_twn = nullptr;  // exp(-2 pi i m / K)
+    size_t c_twn = 0;
+    double2 *d_abar = nullptr, *d_bbar = nullptr, *d_Tpart = nullptr, *d_Tbar = nullptr, *d_Gbar = nullptr;
+    double2 *d_Vbar = nullptr, *d_Hacc = nullptr, *d_Rlist = nullptr, *d_SR = nullptr;
+    size_t c_abar = 0, c_bbar = 0, c_Tpart = 0, c_Tbar = 0, c_Gbar = 0, c_Vbar = 0, c_Hacc = 0, c_Rlist = 0, c_SR = 0;
+    double *d_rhobar = nullptr, *d_rhosum = nullptr;
+    size_t c_rhobar = 0, c_rhosum = 0;
+
     // host staging for the drop-in path (pinned)
     double2 *h_win = nullptr;
     size_t c_hwin = 0;
@@ -161,7 +174,9 @@ void release(cqpi_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw,
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
-                    c->d_afrag, c->d_mscale, c->d_planes};
+                    c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
+                    c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
+                    c->d_rhosum};
     for (void *p : bufs)
         if (p) hipFree(p);
     if (c->h_win) hipHostFree(c->h_win);
@@ -428,6 +443,7 @@ int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, s
     HIPCHK(hipEventRecord(c->copy_done, c->stream));
     RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos, false));
     c->nlayers = 1;
+    c->windows_kept = false;
     c->last_chunk_e0 = 0;
     c->last_chunk_ec = 1;
     // the caller may overwrite g0 and scat once the copies have landed
@@ -526,6 +542,8 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     c->efermi = efermi;
     c->scmax = scmodel ? (B >> 1) : B;
     c->has_model = true;
+    c->h_hops.assign(hops, hops + nhops);
+    c->windows_kept = false;
     c->eigen_valid = false;
     return 0;
 }
@@ -592,6 +610,13 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     // energies per chunk: intermediate of the two-pass transform kept under ~2 GiB
     size_t per_e = (size_t)g.K * g.nw * NP * sizeof(double2);
     size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
+    if (c->keep_windows) {
+        // one chunk: the window and T of every energy stay for cqpi_gradient
+        if (per_e * E > ((size_t)64 << 30))
+            return fail(CQPI_E_ARG, "keeping the windows of this many energies needs more than 64 GiB");
+        Ec = E;
+    }
+    c->windows_kept = false;
     RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
     // mixed precision: the k->x transform writes the fp16 planes directly.  Their
@@ -625,6 +650,9 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
         c->last_chunk_ec = ec;
     }
     c->nlayers = E;
+    c->windows_kept = c->keep_windows;
+    c->last_omegas.assign(omegas, omegas + E);
+    c->last_eta = eta;
     std::vector<int> status(E);
     HIPCHK(hipMemcpyAsync(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -633,6 +661,97 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
         if (status[e])
             return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
     if (ldos_host) HIPCHK(hipMemcpy(ldos_host, c->d_ldos, E * WW * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cqpi_set_keep_windows(cqpi_ctx *c, int on)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    c->keep_windows = on != 0;
+    return 0;
+}
+
+int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_mu, double *g_V)
+{
+    g_err.clear();
+    if (!c || !rhobar) return fail(CQPI_E_ARG, "NULL argument");
+    if (!c->windows_kept || c->nlayers == 0)
+        return fail(CQPI_E_STATE, "the last cqpi_run did not keep its windows (cqpi_set_keep_windows)");
+    const Geom &g = c->g;
+    if (!adjoint_supported(g))
+        return fail(CQPI_E_ARG, "the adjoint needs a power-of-two k-grid (<= 1024) and at most 4 bands");
+    const int B = g.B, NP = B * B;
+    const size_t E = c->nlayers, WW = (size_t)g.W * g.W;
+    const int nsets = g.spin ? 2 : 1;
+    // distinct lattice vectors of the hopping phases (sign folded for 'anti' terms)
+    std::vector<double2> R;
+    std::vector<int> hop_r(c->h_hops.size());
+    for (size_t i = 0; i < c->h_hops.size(); i++) {
+        const cqpi_hopping &h = c->h_hops[i];
+        double rx = h.anti ? -h.rx : h.rx, ry = h.anti ? -h.ry : h.ry;
+        rx += 0.0;
+        ry += 0.0;
+        int idx = -1;
+        for (size_t j = 0; j < R.size(); j++)
+            if (R[j].x == rx && R[j].y == ry) idx = (int)j;
+        if (idx < 0) {
+            idx = (int)R.size();
+            R.push_back(make_double2(rx, ry));
+        }
+        hop_r[i] = idx;
+    }
+    const int nR = (int)R.size();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->d_twn) {
+        std::vector<double2> tw(g.K);
+        for (int m = 0; m < g.K; m++) {
+            long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)g.K;
+            tw[m] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        RETCHK(dev_grow(c->d_twn, c->c_twn, (size_t)g.K, c->gpumem));
+        HIPCHK(hipMemcpy(c->d_twn, tw.data(), g.K * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    const size_t nblk = adjoint_pix_blocks(g);
+    RETCHK(dev_grow(c->d_rhobar, c->c_rhobar, E * WW, c->gpumem));
+    RETCHK(dev_grow(c->d_abar, c->c_abar, E * nsets * WW * B, c->gpumem));
+    RETCHK(dev_grow(c->d_bbar, c->c_bbar, E * nsets * WW * B, c->gpumem));
+    RETCHK(dev_grow(c->d_Tpart, c->c_Tpart, E * nblk * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_Tbar, c->c_Tbar, E * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_rhosum, c->c_rhosum, E * g.os * g.os, c->gpumem));
+    RETCHK(dev_grow(c->d_Gbar, c->c_Gbar, E * (size_t)g.nw * g.nw * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_Vbar, c->c_Vbar, E * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_F1, c->c_F1, E * (size_t)g.K * g.nw * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_Hacc, c->c_Hacc, (size_t)g.K * g.K * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_Rlist, c->c_Rlist, std::max<size_t>(R.size(), 1), c->gpumem));
+    RETCHK(dev_grow(c->d_SR, c->c_SR, (size_t)(nR + 1) * NP, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_rhobar, rhobar, E * WW * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (nR) HIPCHK(hipMemcpyAsync(c->d_Rlist, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_omega, c->last_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RETCHK(launch_adjoint(g, c->d_G0w, c->d_T, c->d_V, c->d_tiles, c->d_rhobar, c->d_eval, c->d_evec, c->d_omega,
+                          (int)E, c->last_eta, c->d_twn, c->d_Rlist, nR, c->scmax, c->d_abar, c->d_bbar, c->d_Tpart,
+                          c->d_Tbar, c->d_rhosum, c->d_Gbar, c->d_Vbar, c->d_F1, c->d_Hacc, c->d_SR, c->stream));
+    std::vector<double2> SR((size_t)(nR + 1) * NP), Vb(E * NP);
+    HIPCHK(hipMemcpyAsync(SR.data(), c->d_SR, SR.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(Vb.data(), c->d_Vbar, Vb.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // (the adjoint reused the F1 buffer; the kept windows and T stay valid)
+    if (g_hops)
+        for (size_t i = 0; i < c->h_hops.size(); i++) {
+            const double2 s = SR[(size_t)hop_r[i] * NP + c->h_hops[i].o1 * B + c->h_hops[i].o2];
+            g_hops[2 * i] = s.x;
+            g_hops[2 * i + 1] = s.y;
+        }
+    if (g_mu) *g_mu = SR[(size_t)nR * NP].x;
+    if (g_V)
+        for (int p = 0; p < NP; p++) {
+            double re = 0.0, im = 0.0;
+            for (size_t e = 0; e < E; e++) {
+                re += Vb[e * NP + p].x;
+                im += Vb[e * NP + p].y;
+            }
+            g_V[2 * p] = re;
+            g_V[2 * p + 1] = im;
+        }
     return 0;
 }
 

@@ -165,6 +165,27 @@ int cqpi_stage_times(cqpi_ctx *ctx, double *ms, int64_t *launches);
 int cqpi_set_timing(cqpi_ctx *ctx, int on);
 
 /* ---------------------------------------------------------------------- */
+/* Gradient (extension; the reference has no differentiation)              */
+/* ---------------------------------------------------------------------- */
+
+/* When on, the next cqpi_run keeps the G0(R) window and T of every energy
+ * (all energies in one chunk) so that cqpi_gradient can run; G0(k) itself is
+ * never stored.  Default off. */
+int cqpi_set_keep_windows(cqpi_ctx *ctx, int on);
+
+/* Hand-written adjoint of the whole chain for the maps of the last cqpi_run
+ * (which must have kept its windows), FP64.  rhobar = dL/drho for those maps
+ * (E*W*W doubles, layout of the maps).  Outputs:
+ *   g_hops[2*h], g_hops[2*h+1] = dL/dRe t_h, dL/dIm t_h for the hoppings in
+ *     the order given to cqpi_set_model (amplitudes as passed, i.e. after the
+ *     degeneracy division);
+ *   *g_mu = dL/defermi;
+ *   g_V (b x b complex) with dL = Re sum conj(g_V) dV, i.e. dL/dRe V_ij =
+ *     Re g_V_ij and dL/dIm V_ij = Im g_V_ij.
+ * Any output pointer may be NULL.  Needs a power-of-two k-grid, bands <= 4. */
+int cqpi_gradient(cqpi_ctx *ctx, const double *rhobar, double *g_hops, double *g_mu, double *g_V);
+
+/* ---------------------------------------------------------------------- */
 /* Input formats (host)                                                     */
 /* ---------------------------------------------------------------------- */
 
