@@ -59,7 +59,16 @@ SIGNATURES = {
     "cqpi_free": (None, [_vp]),
     "cqpi_set_keep_windows": (C.c_int, [_vp, C.c_int]),
     "cqpi_gradient": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "cqpi_gradient_device": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
 }
+
+
+class _DeviceArray:
+    """Zero-copy view of library device memory (float64) for torch.as_tensor."""
+
+    def __init__(self, ptr, shape):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f8", "data": (int(ptr), False),
+                                         "version": 3, "strides": None}
 
 
 class CqpiError(RuntimeError):
@@ -241,6 +250,25 @@ class AmdQPI:
         gV = np.zeros((self.bands, self.bands), np.complex128)
         _check(lib().cqpi_gradient(self._ctx, _d(rb), _d(gh), _d(gmu), _d(gV.view(np.float64))))
         return gh[:nh], float(gmu[0]), gV
+
+    def gradient_device(self, rhobar_ptr):
+        """As gradient() with dL/drho already in device memory (address of
+        E*W*W float64 values); the caller synchronises its writes first."""
+        nh = self._nhops
+        gh = np.zeros((max(nh, 1), 2))
+        gmu = np.zeros(1)
+        gV = np.zeros((self.bands, self.bands), np.complex128)
+        _check(lib().cqpi_gradient_device(self._ctx, C.c_void_p(int(rhobar_ptr)), _d(gh), _d(gmu),
+                                          _d(gV.view(np.float64))))
+        return gh[:nh], float(gmu[0]), gV
+
+    def ldos_tensor(self):
+        """The maps of the last run as a torch CUDA tensor (E, W*W), a
+        zero-copy view of the library buffer (valid until the next run)."""
+        import torch
+
+        ptr, cnt = self.device_ldos()
+        return torch.as_tensor(_DeviceArray(ptr, (self.nlayers, self.W * self.W)), device="cuda")
 
     def device_ldos(self):
         p = _vp()

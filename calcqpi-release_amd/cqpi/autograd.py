@@ -8,7 +8,8 @@ term); mu is the chemical potential (the `fermi` of the input file); Vre / Vim
 the impurity potential.  The forward runs the whole FP64 chain on the GPU
 (keeping the per-energy G0(R) windows, never G0(k)); backward calls the
 library's adjoint, which rebuilds G0(k) per energy from the cached eigen data.
-The returned maps have shape (E, W*W) in the library's x + y*W order.
+Maps and dL/drho stay in device memory.  The returned maps have shape
+(E, W*W) in the library's x + y*W order.
 """
 from __future__ import annotations
 
@@ -28,14 +29,17 @@ class CldosFunction(torch.autograd.Function):
         V = Vre.detach().cpu().double().numpy() + 1j * Vim.detach().cpu().double().numpy()
         engine.set_impurity(V)
         engine.set_keep_windows(True)
-        rho = engine.run(np.asarray(omegas, np.float64), float(eta))
+        engine.run(np.asarray(omegas, np.float64), float(eta), host=False)
+        rho = engine.ldos_tensor().clone()  # the library buffer is reused by the next run
         ctx.engine = engine
         ctx.device = tre.device
-        return torch.from_numpy(rho).to(tre.device)
+        return rho.to(tre.device)
 
     @staticmethod
     def backward(ctx, grad_rho):
-        gh, gmu, gV = ctx.engine.gradient(grad_rho.detach().cpu().double().numpy())
+        g = grad_rho.detach().to(device="cuda", dtype=torch.float64).contiguous()
+        torch.cuda.synchronize()
+        gh, gmu, gV = ctx.engine.gradient_device(g.data_ptr())
         dev = ctx.device
         return (torch.from_numpy(np.ascontiguousarray(gh[:, 0])).to(dev),
                 torch.from_numpy(np.ascontiguousarray(gh[:, 1])).to(dev),

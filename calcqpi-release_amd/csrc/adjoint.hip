@@ -112,8 +112,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *
                 const size_t pixid = (size_t)X + (size_t)Y * W;
                 const double rb = rhobar[(size_t)e * W * W + pixid];
                 const double2 sbar = make_double2(0.0, -rb);
-                double2 *ab = abar + (((size_t)e * nsets + sg) * W * W + pixid) * B;
-                double2 *bb = bbar + (((size_t)e * nsets + sg) * W * W + pixid) * B;
+                // planar [e][sg][p][pixel]
+                double2 *ab = abar + ((size_t)e * nsets + sg) * B * W * W + pixid;
+                double2 *bb = bbar + ((size_t)e * nsets + sg) * B * W * W + pixid;
 #pragma unroll
                 for (int q1 = 0; q1 < B; q1++) {
                     double2 tb = make_double2(0.0, 0.0), at = make_double2(0.0, 0.0);
@@ -122,8 +123,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *
                         tb = cadd(tb, cmul(Te[q1 * B + q2], b[p][q2]));
                         at = cadd(at, cmul(a[p][q2], Te[q2 * B + q1]));
                     }
-                    ab[q1] = cmul(sbar, cconj(tb));
-                    bb[q1] = cmul(sbar, cconj(at));
+                    ab[(size_t)q1 * W * W] = cmul(sbar, cconj(tb));
+                    bb[(size_t)q1 * W * W] = cmul(sbar, cconj(at));
                 }
 #pragma unroll
                 for (int q1 = 0; q1 < B; q1++)
@@ -187,60 +188,95 @@ __global__ void k_adj_reduce(Geom g, const double2 *__restrict__ Tpart, int nblk
     }
 }
 
-// Gbar on the R window: contraction transpose (a and b gathers) + periodic term
-__global__ void k_adj_window(Geom g, const double *__restrict__ tiles, const double2 *__restrict__ abar,
-                             const double2 *__restrict__ bbar, const double *__restrict__ rhosum,
-                             double2 *__restrict__ Gbar)
+// Window adjoint, step 1: contract the adjoints over the sub-pixels of each cell,
+//   Za[R][o][p][cell] = sum_x w_o(x;R) abar_p(cell, x)
+//   Zb[R][o][q][cell] = sum_x w_o(x;R) bbar_q(cell, x)
+// for o in the spin block of the adjoint used.  One thread per (cell, column)
+// and grid z = (a|b, spin block); with NPIX > 0 (os*os <= NPIX) the cell's
+// adjoints are held in registers, otherwise re-read through the cache.
+// Layout [e][R][o][col][cy][cx].
+template <int NPIX>
+__global__ void __launch_bounds__(128) k_adj_z(Geom g, const double *__restrict__ tiles,
+                                               const double2 *__restrict__ abar, const double2 *__restrict__ bbar,
+                                               double2 *__restrict__ Za, double2 *__restrict__ Zb)
 {
-    const int B = g.B, NP = B * B, nw = g.nw, h = g.h, os = g.os, w = g.w, T = g.T, W = g.W, n = g.n;
+    const int B = g.B, os = g.os, w = g.w, T = g.T, W = g.W, n = g.n;
+    const int side = 2 * w + 1, NR = side * side;
+    const long cells = (long)n * n;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y, ab = blockIdx.z & 1, sg = blockIdx.z >> 1;
+    if (idx >= cells * B) return;
+    const int cx = (int)(idx % n), cy = (int)((idx / n) % n), col = (int)(idx / cells);
+    const int nsets = g.spin ? 2 : 1;
+    const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    const double2 *s = (ab ? bbar : abar) + (((size_t)e * nsets + sg) * B + col) * W * W +
+                       (size_t)cx * os + (size_t)cy * os * W;
+    double2 *Z = (ab ? Zb : Za) + (size_t)e * NR * g.maxb * B * cells + (size_t)col * cells + (size_t)cy * n + cx;
+    const int npix = os * os;
+    double2 v[NPIX > 0 ? NPIX : 1];
+    if (NPIX > 0) {
+#pragma unroll
+        for (int pix = 0; pix < (NPIX > 0 ? NPIX : 1); pix++) {
+            const int ii = pix / os, jj = pix - ii * os;
+            v[pix] = pix < npix ? s[ii + (size_t)jj * W] : make_double2(0.0, 0.0);
+        }
+    }
+    for (int r = 0; r < NR; r++) {
+        // tile origin of R = (r/side - w, r%side - w): ((w - Rx) os, (w - Ry) os)
+        const int tx = (2 * w - r / side) * os, ty = (2 * w - r % side) * os;
+        for (int o = sg * nper; o < (sg + 1) * nper; o++) {
+            const double *wo = tiles + ((size_t)o * T + tx) * T + ty;
+            double2 acc = make_double2(0.0, 0.0);
+            if (NPIX > 0) {
+#pragma unroll
+                for (int pix = 0; pix < (NPIX > 0 ? NPIX : 1); pix++) {
+                    const int ii = pix / os, jj = pix - ii * os;
+                    if (pix < npix) acc = cadd(acc, cscale(wo[(size_t)ii * T + jj], v[pix]));
+                }
+            } else {
+                for (int ii = 0; ii < os; ii++)
+                    for (int jj = 0; jj < os; jj++) acc = cadd(acc, cscale(wo[(size_t)ii * T + jj], s[ii + (size_t)jj * W]));
+            }
+            Z[((size_t)r * g.maxb + o) * B * cells] = acc;
+        }
+    }
+}
+
+// Window adjoint, step 2: shifted sums over R plus the periodic term,
+//   Gbar(S)_{r,c} = sum_R Za[R][r][c][S-R] + sum_R Zb[R][c][r][-S-R]
+//                   - i sum_{R1} sum_x rhosum(x) w_c(x;R1) w_r(x;S+R1)   (|S| <= 2w, same spin block)
+// One thread per (slot, pair), lanes along the slot's x.
+__global__ void k_adj_shift(Geom g, const double *__restrict__ tiles, const double2 *__restrict__ Za,
+                            const double2 *__restrict__ Zb, const double *__restrict__ rhosum,
+                            double2 *__restrict__ Gbar)
+{
+    const int B = g.B, NP = B * B, nw = g.nw, h = g.h, os = g.os, w = g.w, T = g.T, n = g.n;
+    const int side = 2 * w + 1, NR = side * side, n2 = n >> 1;
     const long total = (long)nw * nw * NP;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int e = blockIdx.y;
     if (idx >= total) return;
-    const int pr = (int)(idx % NP);
-    const long s = idx / NP;
-    const int sy = (int)(s % nw), sx = (int)(s / nw);
+    const int sx = (int)(idx % nw), sy = (int)((idx / nw) % nw), pr = (int)(idx / ((long)nw * nw));
     const int r = pr / B, c = pr % B;
     const int Sx = sx - h, Sy = sy - h;
-    const int n2 = n >> 1;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    const long cells = (long)n * n;
+    const size_t zoff = (size_t)e * NR * g.maxb * B * cells;
     double2 acc = make_double2(0.0, 0.0);
-    // a gather: Gbar(pos+R)_{o=r, p=c} += sum_x w_r(x;R) abar_c(pos, x)
-    if (r < nsets * nper) {
-        const int sg = r / nper;
-        for (int rx = -w; rx <= w; rx++)
-            for (int ry = -w; ry <= w; ry++) {
-                const int cx = Sx - rx + n2, cy = Sy - ry + n2;
-                if (cx < 0 || cy < 0 || cx >= n || cy >= n) continue;
-                const double2 *ab = abar + ((size_t)e * nsets + sg) * W * W * B;
-                for (int ii = 0; ii < os; ii++)
-                    for (int jj = 0; jj < os; jj++) {
-                        double wt = tiles[((size_t)r * T + ii + (w - rx) * os) * T + jj + (w - ry) * os];
-                        if (wt == 0.0) continue;
-                        size_t pix = (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W;
-                        acc = cadd(acc, cscale(wt, ab[pix * B + c]));
-                    }
-            }
+    for (int rr = 0; rr < NR; rr++) {
+        const int rx = rr / side - w, ry = rr % side - w;
+        if (r < nsets * nper) {
+            const int cx = Sx - rx + n2, cy = Sy - ry + n2;
+            if (cx >= 0 && cy >= 0 && cx < n && cy < n)
+                acc = cadd(acc, Za[zoff + (((size_t)rr * g.maxb + r) * B + c) * cells + (size_t)cy * n + cx]);
+        }
+        if (c < nsets * nper) {
+            const int cx = -Sx - rx + n2, cy = -Sy - ry + n2;
+            if (cx >= 0 && cy >= 0 && cx < n && cy < n)
+                acc = cadd(acc, Zb[zoff + (((size_t)rr * g.maxb + c) * B + r) * cells + (size_t)cy * n + cx]);
+        }
     }
-    // b gather: Gbar(-pos-R)_{q=r, o=c} += sum_x w_c(x;R) bbar_r(pos, x)
-    if (c < nsets * nper) {
-        const int sg = c / nper;
-        for (int rx = -w; rx <= w; rx++)
-            for (int ry = -w; ry <= w; ry++) {
-                const int cx = -Sx - rx + n2, cy = -Sy - ry + n2;
-                if (cx < 0 || cy < 0 || cx >= n || cy >= n) continue;
-                const double2 *bb = bbar + ((size_t)e * nsets + sg) * W * W * B;
-                for (int ii = 0; ii < os; ii++)
-                    for (int jj = 0; jj < os; jj++) {
-                        double wt = tiles[((size_t)c * T + ii + (w - rx) * os) * T + jj + (w - ry) * os];
-                        if (wt == 0.0) continue;
-                        size_t pix = (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W;
-                        acc = cadd(acc, cscale(wt, bb[pix * B + r]));
-                    }
-            }
-    }
-    // periodic term: Gbar(R2-R1)_{o2=r, o1=c} += -i sum_x rhobar w_c(x;R1) w_r(x;R2), same spin block
     if (r < nsets * nper && c < nsets * nper && r / nper == c / nper && abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
         double re = 0.0;
         for (int r1x = -w; r1x <= w; r1x++)
@@ -256,7 +292,7 @@ __global__ void k_adj_window(Geom g, const double *__restrict__ tiles, const dou
             }
         acc = cadd(acc, make_double2(0.0, -re));
     }
-    Gbar[(size_t)e * total + idx] = acc;
+    Gbar[((size_t)e * nw * nw + (size_t)sx * nw + sy) * NP + pr] = acc;
 }
 
 constexpr int kMaxB = 16;
@@ -374,28 +410,40 @@ __global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__re
 // adjoint of the ky pass fused with the resolvent adjoint, summed over energies:
 // Gbar(k) for one kx row, then Hacc(k) += G(k)^H Gbar(k) G(k)^H with G(k) rebuilt
 // from the eigen data.  One workgroup per row, all orbital pairs in LDS.
-template <int B>
+template <int B, int JPT>
 __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__restrict__ eval,
                                                    const double2 *__restrict__ evec,
-                                                   const double *__restrict__ omegas, int E, double eta,
+                                                   const double *__restrict__ omegas, int E, int EG, double eta,
                                                    const double2 *__restrict__ F1bar,
                                                    const double2 *__restrict__ twn, double2 *__restrict__ Hacc)
 {
     extern __shared__ double2 S[];
     constexpr int NP = B * B;
-    constexpr int JPT = 4;
     const int K = g.K, i = blockIdx.x, nw = g.nw, h = g.h;
+    const int e0 = blockIdx.y * EG, e1 = min(E, e0 + EG);
     const long KK = (long)K * K;
     int lgK = 0;
     while ((1 << lgK) < K) lgK++;
     double2 *Tw = S + (size_t)NP * K;
     for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
+    // eigen data of the thread's k-points, kept for the energy loop
+    double ev[JPT][B];
+    double2 vv[JPT][NP];
+#pragma unroll
+    for (int t = 0; t < JPT; t++) {
+        const int j = threadIdx.x + t * kThreads;
+        const long k = (long)i * K + (j < K ? j : 0);
+#pragma unroll
+        for (int nn = 0; nn < B; nn++) ev[t][nn] = eval[nn * KK + k];
+#pragma unroll
+        for (int r = 0; r < NP; r++) vv[t][r] = evec[r * KK + k];
+    }
     double2 hacc[JPT][NP];
 #pragma unroll
     for (int t = 0; t < JPT; t++)
 #pragma unroll
         for (int p = 0; p < NP; p++) hacc[t][p] = make_double2(0.0, 0.0);
-    for (int e = 0; e < E; e++) {
+    for (int e = e0; e < e1; e++) {
         __syncthreads();
         const double2 *in = F1bar + ((size_t)e * K + i) * (size_t)nw * NP;
         for (int t = threadIdx.x; t < K * NP; t += blockDim.x) {
@@ -412,9 +460,10 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
             const int j = threadIdx.x + t * kThreads;
             if (j >= K) break;
             const long k = (long)i * K + j;
+            (void)k;
             double2 f[B];
 #pragma unroll
-            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - eval[nn * KK + k], eta));
+            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - ev[t][nn], eta));
             double2 G[NP], Gb[NP], X[NP];
 #pragma unroll
             for (int r = 0; r < B; r++)
@@ -423,7 +472,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
                     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
                     for (int nn = 0; nn < B; nn++)
-                        acc = cadd(acc, cmul(f[nn], cmulc(evec[(r * B + nn) * KK + k], evec[(c * B + nn) * KK + k])));
+                        acc = cadd(acc, cmul(f[nn], cmulc(vv[t][r * B + nn], vv[t][c * B + nn])));
                     G[r * B + c] = acc;
                     Gb[r * B + c] = S[(size_t)(r * B + c) * K + j];
                 }
@@ -454,7 +503,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
         if (j >= K) break;
         const long k = (long)i * K + j;
 #pragma unroll
-        for (int p = 0; p < NP; p++) Hacc[k * NP + p] = hacc[t][p];
+        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * KK + k) * NP + p] = hacc[t][p];
     }
 }
 
@@ -463,42 +512,87 @@ __device__ inline double kcoord(int i, int K)
     return (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
 }
 
-// SR[r][pair] = sum_k Hacc(k)[pair] e^{-2 pi i R_r.k}; row nR holds the traces for mu:
-// SR[nR][0].x = sum_k sum_i s_i Re Hacc_ii(k), s_i = -1 (i < scmax), +1 otherwise
-__global__ void k_adj_hsum(Geom g, const double2 *__restrict__ Hacc, const double2 *__restrict__ Rlist, int nR,
-                           int scmax, double2 *__restrict__ SR)
+constexpr int kMaxRy = 16;
+
+__device__ inline double2 block_sum2(double2 v, double *r1, double *r2)
 {
-    __shared__ double r1[256], r2[256];
-    const int ridx = blockIdx.x, pr = blockIdx.y;
-    const int B = g.B, NP = B * B, K = g.K;
-    const long KK = (long)K * K;
-    double s1 = 0.0, s2 = 0.0;
-    if (ridx < nR) {
-        const double2 R = Rlist[ridx];
-        for (long k = threadIdx.x; k < KK; k += blockDim.x) {
-            int i = (int)(k / K), j = (int)(k % K);
-            double arg = -2.0 * M_PI * (R.x * kcoord(i, K) + R.y * kcoord(j, K));
-            double sn, cs;
-            sincos(arg, &sn, &cs);
-            double2 v = cmul(Hacc[k * NP + pr], make_double2(cs, sn));
-            s1 += v.x;
-            s2 += v.y;
-        }
-    } else if (pr == 0) {
-        for (long k = threadIdx.x; k < KK; k += blockDim.x)
-            for (int d = 0; d < B; d++) s1 += (d < scmax ? -1.0 : 1.0) * Hacc[k * NP + d * B + d].x;
-    }
-    r1[threadIdx.x] = s1;
-    r2[threadIdx.x] = s2;
+    r1[threadIdx.x] = v.x;
+    r2[threadIdx.x] = v.y;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             r1[threadIdx.x] += r1[threadIdx.x + s];
             r2[threadIdx.x] += r2[threadIdx.x + s];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) SR[(size_t)ridx * NP + pr] = make_double2(r1[0], r2[0]);
+    double2 out = make_double2(r1[0], r2[0]);
+    __syncthreads();
+    return out;
+}
+
+// Hopping sums S(R) = sum_k Hbar(k) e^{-2 pi i R.k} factorise over kx and ky.
+// Stage 1, one workgroup per (kx row, orbital pair): summing the energy groups,
+//   U[kx][r][pair] = sum_ky Hbar(kx, ky) e^{-2 pi i ry_r ky}  for the distinct ry,
+//   U[kx][nry][pair] = sum_ky Hbar(kx, ky)                       (for the trace)
+__global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restrict__ Hacc, int NG,
+                                                  const double *__restrict__ ry, int nry, double2 *__restrict__ U)
+{
+    __shared__ double r1[256], r2[256];
+    const int kx = blockIdx.x, pr = blockIdx.y;
+    const int NP = g.B * g.B, K = g.K;
+    const long KK = (long)K * K;
+    double2 acc[kMaxRy + 1];
+#pragma unroll
+    for (int r = 0; r <= kMaxRy; r++) acc[r] = make_double2(0.0, 0.0);
+    for (int ky = threadIdx.x; ky < K; ky += blockDim.x) {
+        double2 hv = make_double2(0.0, 0.0);
+        for (int gi = 0; gi < NG; gi++) hv = cadd(hv, Hacc[((size_t)gi * KK + (long)kx * K + ky) * NP + pr]);
+        const double kyf = kcoord(ky, K);
+#pragma unroll
+        for (int r = 0; r < kMaxRy; r++) {
+            if (r >= nry) break;
+            double sn, cs;
+            sincos(-2.0 * M_PI * ry[r] * kyf, &sn, &cs);
+            acc[r] = cadd(acc[r], cmul(hv, make_double2(cs, sn)));
+        }
+        acc[kMaxRy] = cadd(acc[kMaxRy], hv);
+    }
+#pragma unroll
+    for (int r = 0; r <= kMaxRy; r++) {
+        if (r < nry || r == kMaxRy) {
+            double2 s = block_sum2(acc[r], r1, r2);
+            const int slot = r == kMaxRy ? nry : r;
+            if (threadIdx.x == 0) U[((size_t)kx * (nry + 1) + slot) * NP + pr] = s;
+        }
+    }
+}
+
+// Stage 2, one workgroup per (R, pair): SR[R][pair] = sum_kx U[kx][iry(R)][pair] e^{-2 pi i rx kx};
+// Rl[R] = (rx, iry); the last entry (0, nry) gives the plain sums used for the trace (mu).
+__global__ void __launch_bounds__(256) k_adj_hcol(Geom g, const double2 *__restrict__ U, int nry,
+                                                  const double2 *__restrict__ Rl, double2 *__restrict__ SR)
+{
+    __shared__ double r1[256], r2[256];
+    const int ridx = blockIdx.x, pr = blockIdx.y;
+    const int NP = g.B * g.B, K = g.K;
+    const double rx = Rl[ridx].x;
+    const int iry = (int)Rl[ridx].y;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kx = threadIdx.x; kx < K; kx += blockDim.x) {
+        double sn, cs;
+        sincos(-2.0 * M_PI * rx * kcoord(kx, K), &sn, &cs);
+        acc = cadd(acc, cmul(U[((size_t)kx * (nry + 1) + iry) * NP + pr], make_double2(cs, sn)));
+    }
+    double2 s = block_sum2(acc, r1, r2);
+    if (threadIdx.x == 0) SR[(size_t)ridx * NP + pr] = s;
+}
+
+// energy groups of the ky adjoint: enough workgroups to fill the chip
+int cqpi_energy_groups(const Geom &g, int E)
+{
+    int ng = 2048 / std::max(g.K, 1);
+    return std::max(1, std::min(E, ng));
 }
 
 template <int B>
@@ -518,8 +612,18 @@ int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const do
                 const double2 *F1bar, const double2 *twn, double2 *Hacc, hipStream_t s)
 {
     size_t lds = sizeof(double2) * ((size_t)B * B * g.K + (size_t)(g.K >> 1));
-    hipLaunchKernelGGL(k_adj_ky<B>, dim3(g.K), dim3(kThreads), lds, s, g, eval, evec, omegas, E, eta, F1bar, twn,
-                       Hacc);
+    const int NG = cqpi_energy_groups(g, E), EG = (E + NG - 1) / NG;
+    dim3 grid(g.K, NG);
+    const int jpt = (g.K + kThreads - 1) / kThreads;
+    if (jpt <= 1)
+        hipLaunchKernelGGL((k_adj_ky<B, 1>), grid, dim3(kThreads), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
+                           twn, Hacc);
+    else if (jpt == 2)
+        hipLaunchKernelGGL((k_adj_ky<B, 2>), grid, dim3(kThreads), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
+                           twn, Hacc);
+    else
+        hipLaunchKernelGGL((k_adj_ky<B, 4>), grid, dim3(kThreads), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
+                           twn, Hacc);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -537,16 +641,19 @@ bool adjoint_supported(const Geom &g)
 {
     bool pow2 = (g.K & (g.K - 1)) == 0;
     size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
-    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024;
+    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && g.os <= 16;
 }
+
+int adjoint_energy_groups(const Geom &g, int E) { return cqpi_energy_groups(g, E); }
 
 int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const double2 *V, const double *tiles,
                    const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
-                   double eta, const double2 *twn, const double2 *Rlist, int nR, int scmax, double2 *abar,
-                   double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar, double2 *Vbar,
-                   double2 *F1bar, double2 *Hacc, double2 *SR, hipStream_t s)
+                   double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
+                   double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
+                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double2 *Za, double2 *Zb,
+                   hipStream_t s)
 {
-    if (!adjoint_supported(g)) return CQPI_E_ARG;
+    if (!adjoint_supported(g) || nry > kMaxRy) return CQPI_E_ARG;
     int nblk = 0, r = CQPI_E_ARG;
     switch (g.B) {
     case 1: r = launch_pix_t<1>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
@@ -557,8 +664,14 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     if (r) return r;
     const int NP = g.B * g.B;
     hipLaunchKernelGGL(k_adj_reduce, dim3(NP + g.os * g.os, E), dim3(256), 0, s, g, Tpart, nblk, rhobar, Tbar, rhosum);
+    long zt = (long)g.n * g.n * g.B;
+    const dim3 zgrid((unsigned)((zt + 127) / 128), E, 2 * (g.spin ? 2 : 1));
+    if (g.os <= 4)
+        hipLaunchKernelGGL(k_adj_z<16>, zgrid, dim3(128), 0, s, g, tiles, abar, bbar, Za, Zb);
+    else
+        hipLaunchKernelGGL(k_adj_z<0>, zgrid, dim3(128), 0, s, g, tiles, abar, bbar, Za, Zb);
     long total = (long)g.nw * g.nw * NP;
-    hipLaunchKernelGGL(k_adj_window, dim3((unsigned)((total + 255) / 256), E), dim3(256), 0, s, g, tiles, abar, bbar,
+    hipLaunchKernelGGL(k_adj_shift, dim3((unsigned)((total + 255) / 256), E), dim3(256), 0, s, g, tiles, Za, Zb,
                        rhosum, Gbar);
     hipLaunchKernelGGL(k_adj_tmatrix, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar);
     // adjoint transform along kx (pairs chunked to the LDS budget), then ky + resolvent
@@ -575,7 +688,8 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case 4: r = launch_ky_t<4>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
     }
     if (r) return r;
-    hipLaunchKernelGGL(k_adj_hsum, dim3(nR + 1, NP), dim3(256), 0, s, g, Hacc, Rlist, nR, scmax, SR);
+    hipLaunchKernelGGL(k_adj_hrow, dim3(g.K, NP), dim3(256), 0, s, g, Hacc, cqpi_energy_groups(g, E), ryvals, nry, U);
+    hipLaunchKernelGGL(k_adj_hcol, dim3(nRl, NP), dim3(256), 0, s, g, U, nry, Rl, SR);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 

@@ -148,8 +148,10 @@ struct cqpi_ctx {
     double2 *d_abar = nullptr, *d_bbar = nullptr, *d_Tpart = nullptr, *d_Tbar = nullptr, *d_Gbar = nullptr;
     double2 *d_Vbar = nullptr, *d_Hacc = nullptr, *d_Rlist = nullptr, *d_SR = nullptr;
     size_t c_abar = 0, c_bbar = 0, c_Tpart = 0, c_Tbar = 0, c_Gbar = 0, c_Vbar = 0, c_Hacc = 0, c_Rlist = 0, c_SR = 0;
-    double *d_rhobar = nullptr, *d_rhosum = nullptr;
-    size_t c_rhobar = 0, c_rhosum = 0;
+    double *d_rhobar = nullptr, *d_rhosum = nullptr, *d_ry = nullptr;
+    size_t c_rhobar = 0, c_rhosum = 0, c_ry = 0;
+    double2 *d_U = nullptr, *d_Za = nullptr, *d_Zb = nullptr;
+    size_t c_U = 0, c_Za = 0, c_Zb = 0;
 
     // host staging for the drop-in path (pinned)
     double2 *h_win = nullptr;
@@ -176,7 +178,7 @@ void release(cqpi_ctx *c)
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
                     c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
-                    c->d_rhosum};
+                    c->d_rhosum, c->d_ry, c->d_U, c->d_Za, c->d_Zb};
     for (void *p : bufs)
         if (p) hipFree(p);
     if (c->h_win) hipHostFree(c->h_win);
@@ -671,7 +673,21 @@ int cqpi_set_keep_windows(cqpi_ctx *c, int on)
     return 0;
 }
 
+static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, double *g_hops, double *g_mu,
+                         double *g_V);
+
 int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_mu, double *g_V)
+{
+    return gradient_impl(c, rhobar, false, g_hops, g_mu, g_V);
+}
+
+int cqpi_gradient_device(cqpi_ctx *c, const double *d_rhobar, double *g_hops, double *g_mu, double *g_V)
+{
+    return gradient_impl(c, d_rhobar, true, g_hops, g_mu, g_V);
+}
+
+static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, double *g_hops, double *g_mu,
+                         double *g_V)
 {
     g_err.clear();
     if (!c || !rhobar) return fail(CQPI_E_ARG, "NULL argument");
@@ -683,24 +699,34 @@ int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_m
     const int B = g.B, NP = B * B;
     const size_t E = c->nlayers, WW = (size_t)g.W * g.W;
     const int nsets = g.spin ? 2 : 1;
-    // distinct lattice vectors of the hopping phases (sign folded for 'anti' terms)
+    // distinct lattice vectors of the hopping phases (sign folded for 'anti'
+    // terms), as (rx, index of ry among the distinct ry values)
+    std::vector<double> ryv;
     std::vector<double2> R;
     std::vector<int> hop_r(c->h_hops.size());
     for (size_t i = 0; i < c->h_hops.size(); i++) {
         const cqpi_hopping &h = c->h_hops[i];
-        double rx = h.anti ? -h.rx : h.rx, ry = h.anti ? -h.ry : h.ry;
-        rx += 0.0;
-        ry += 0.0;
+        double rx = (h.anti ? -h.rx : h.rx) + 0.0, ry = (h.anti ? -h.ry : h.ry) + 0.0;
+        int iy = -1;
+        for (size_t j = 0; j < ryv.size(); j++)
+            if (ryv[j] == ry) iy = (int)j;
+        if (iy < 0) {
+            iy = (int)ryv.size();
+            ryv.push_back(ry);
+        }
         int idx = -1;
         for (size_t j = 0; j < R.size(); j++)
-            if (R[j].x == rx && R[j].y == ry) idx = (int)j;
+            if (R[j].x == rx && R[j].y == (double)iy) idx = (int)j;
         if (idx < 0) {
             idx = (int)R.size();
-            R.push_back(make_double2(rx, ry));
+            R.push_back(make_double2(rx, (double)iy));
         }
         hop_r[i] = idx;
     }
-    const int nR = (int)R.size();
+    if (ryv.size() > 16) return fail(CQPI_E_ARG, "the adjoint handles at most 16 distinct hopping y offsets");
+    const int nR = (int)R.size(), nry = (int)ryv.size();
+    R.push_back(make_double2(0.0, (double)nry));  // plain sums, for the trace (mu)
+    if (ryv.empty()) ryv.push_back(0.0);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!c->d_twn) {
         std::vector<double2> tw(g.K);
@@ -712,7 +738,10 @@ int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_m
         HIPCHK(hipMemcpy(c->d_twn, tw.data(), g.K * sizeof(double2), hipMemcpyHostToDevice));
     }
     const size_t nblk = adjoint_pix_blocks(g);
-    RETCHK(dev_grow(c->d_rhobar, c->c_rhobar, E * WW, c->gpumem));
+    const size_t NG = (size_t)adjoint_energy_groups(g, (int)E);
+    if (!on_device) RETCHK(dev_grow(c->d_rhobar, c->c_rhobar, E * WW, c->gpumem));
+    RETCHK(dev_grow(c->d_ry, c->c_ry, ryv.size(), c->gpumem));
+    RETCHK(dev_grow(c->d_U, c->c_U, (size_t)g.K * (nry + 1) * NP, c->gpumem));
     RETCHK(dev_grow(c->d_abar, c->c_abar, E * nsets * WW * B, c->gpumem));
     RETCHK(dev_grow(c->d_bbar, c->c_bbar, E * nsets * WW * B, c->gpumem));
     RETCHK(dev_grow(c->d_Tpart, c->c_Tpart, E * nblk * NP, c->gpumem));
@@ -721,16 +750,25 @@ int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_m
     RETCHK(dev_grow(c->d_Gbar, c->c_Gbar, E * (size_t)g.nw * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Vbar, c->c_Vbar, E * NP, c->gpumem));
     RETCHK(dev_grow(c->d_F1, c->c_F1, E * (size_t)g.K * g.nw * NP, c->gpumem));
-    RETCHK(dev_grow(c->d_Hacc, c->c_Hacc, (size_t)g.K * g.K * NP, c->gpumem));
-    RETCHK(dev_grow(c->d_Rlist, c->c_Rlist, std::max<size_t>(R.size(), 1), c->gpumem));
-    RETCHK(dev_grow(c->d_SR, c->c_SR, (size_t)(nR + 1) * NP, c->gpumem));
-    HIPCHK(hipMemcpyAsync(c->d_rhobar, rhobar, E * WW * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (nR) HIPCHK(hipMemcpyAsync(c->d_Rlist, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    RETCHK(dev_grow(c->d_Hacc, c->c_Hacc, NG * g.K * g.K * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_Rlist, c->c_Rlist, R.size(), c->gpumem));
+    RETCHK(dev_grow(c->d_SR, c->c_SR, R.size() * NP, c->gpumem));
+    const size_t nz = E * (size_t)(2 * g.w + 1) * (2 * g.w + 1) * g.maxb * B * g.n * g.n;
+    RETCHK(dev_grow(c->d_Za, c->c_Za, nz, c->gpumem));
+    RETCHK(dev_grow(c->d_Zb, c->c_Zb, nz, c->gpumem));
+    const double *drb = rhobar;
+    if (!on_device) {
+        HIPCHK(hipMemcpyAsync(c->d_rhobar, rhobar, E * WW * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        drb = c->d_rhobar;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_Rlist, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_ry, ryv.data(), ryv.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_omega, c->last_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    RETCHK(launch_adjoint(g, c->d_G0w, c->d_T, c->d_V, c->d_tiles, c->d_rhobar, c->d_eval, c->d_evec, c->d_omega,
-                          (int)E, c->last_eta, c->d_twn, c->d_Rlist, nR, c->scmax, c->d_abar, c->d_bbar, c->d_Tpart,
-                          c->d_Tbar, c->d_rhosum, c->d_Gbar, c->d_Vbar, c->d_F1, c->d_Hacc, c->d_SR, c->stream));
-    std::vector<double2> SR((size_t)(nR + 1) * NP), Vb(E * NP);
+    RETCHK(launch_adjoint(g, c->d_G0w, c->d_T, c->d_V, c->d_tiles, drb, c->d_eval, c->d_evec, c->d_omega, (int)E,
+                          c->last_eta, c->d_twn, c->d_ry, nry, c->d_Rlist, (int)R.size(), c->d_abar, c->d_bbar,
+                          c->d_Tpart, c->d_Tbar, c->d_rhosum, c->d_Gbar, c->d_Vbar, c->d_F1, c->d_Hacc, c->d_U,
+                          c->d_SR, c->d_Za, c->d_Zb, c->stream));
+    std::vector<double2> SR(R.size() * NP), Vb(E * NP);
     HIPCHK(hipMemcpyAsync(SR.data(), c->d_SR, SR.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(Vb.data(), c->d_Vbar, Vb.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -741,7 +779,11 @@ int cqpi_gradient(cqpi_ctx *c, const double *rhobar, double *g_hops, double *g_m
             g_hops[2 * i] = s.x;
             g_hops[2 * i + 1] = s.y;
         }
-    if (g_mu) *g_mu = SR[(size_t)nR * NP].x;
+    if (g_mu) {  // dH/dmu = -1 on the particle block, +1 on the hole block
+        double s = 0.0;
+        for (int d = 0; d < B; d++) s += (d < c->scmax ? -1.0 : 1.0) * SR[(size_t)nR * NP + d * B + d].x;
+        *g_mu = s;
+    }
     if (g_V)
         for (int p = 0; p < NP; p++) {
             double re = 0.0, im = 0.0;

@@ -185,6 +185,11 @@ int cqpi_set_keep_windows(cqpi_ctx *ctx, int on);
  * Any output pointer may be NULL.  Needs a power-of-two k-grid, bands <= 4. */
 int cqpi_gradient(cqpi_ctx *ctx, const double *rhobar, double *g_hops, double *g_mu, double *g_V);
 
+/* As cqpi_gradient with rhobar in device memory (e.g. computed by the caller
+ * on the GPU from cqpi_device_ldos); the caller's writes to it must be
+ * complete (synchronised) before the call. */
+int cqpi_gradient_device(cqpi_ctx *ctx, const double *d_rhobar, double *g_hops, double *g_mu, double *g_V);
+
 /* ---------------------------------------------------------------------- */
 /* Input formats (host)                                                     */
 /* ---------------------------------------------------------------------- */
