@@ -188,111 +188,148 @@ __global__ void k_adj_reduce(Geom g, const double2 *__restrict__ Tpart, int nblk
     }
 }
 
-// Window adjoint, step 1: contract the adjoints over the sub-pixels of each cell,
-//   Za[R][o][p][cell] = sum_x w_o(x;R) abar_p(cell, x)
-//   Zb[R][o][q][cell] = sum_x w_o(x;R) bbar_q(cell, x)
-// for o in the spin block of the adjoint used.  One thread per (cell, column)
-// and grid z = (a|b, spin block); with NPIX > 0 (os*os <= NPIX) the cell's
-// adjoints are held in registers, otherwise re-read through the cache.
-// Layout [e][R][o][col][cy][cx].
-template <int NPIX>
-__global__ void __launch_bounds__(128) k_adj_z(Geom g, const double *__restrict__ tiles,
-                                               const double2 *__restrict__ abar, const double2 *__restrict__ bbar,
-                                               double2 *__restrict__ Za, double2 *__restrict__ Zb)
+// Weight overlaps of the periodic term, independent of energy:
+//   Q[r*b+c][S][x] = sum_{R1} w_c(x;R1) w_r(x;S+R1)   for |S| <= 2w (both R1, S+R1 in the window),
+// zero unless r and c carry Wannier functions in one spin block.
+// Layout [pair][Sx+2w][Sy+2w][pixel], pixel = ii*os+jj.
+__global__ void k_adj_q(Geom g, const double *__restrict__ tiles, double *__restrict__ Q)
 {
-    const int B = g.B, os = g.os, w = g.w, T = g.T, W = g.W, n = g.n;
-    const int side = 2 * w + 1, NR = side * side;
-    const long cells = (long)n * n;
+    const int B = g.B, os = g.os, w = g.w, T = g.T, D = 4 * w + 1, npix = os * os;
+    const long total = (long)B * B * D * D * npix;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = blockIdx.y, ab = blockIdx.z & 1, sg = blockIdx.z >> 1;
-    if (idx >= cells * B) return;
-    const int cx = (int)(idx % n), cy = (int)((idx / n) % n), col = (int)(idx / cells);
+    if (idx >= total) return;
+    const int pix = (int)(idx % npix);
+    const int sy = (int)((idx / npix) % D), sx = (int)((idx / ((long)npix * D)) % D);
+    const int pr = (int)(idx / ((long)npix * D * D));
+    const int r = pr / B, c = pr % B, ii = pix / os, jj = pix % os;
+    const int Sx = sx - 2 * w, Sy = sy - 2 * w;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
-    const double2 *s = (ab ? bbar : abar) + (((size_t)e * nsets + sg) * B + col) * W * W +
-                       (size_t)cx * os + (size_t)cy * os * W;
-    double2 *Z = (ab ? Zb : Za) + (size_t)e * NR * g.maxb * B * cells + (size_t)col * cells + (size_t)cy * n + cx;
-    const int npix = os * os;
-    double2 v[NPIX > 0 ? NPIX : 1];
-    if (NPIX > 0) {
-#pragma unroll
-        for (int pix = 0; pix < (NPIX > 0 ? NPIX : 1); pix++) {
-            const int ii = pix / os, jj = pix - ii * os;
-            v[pix] = pix < npix ? s[ii + (size_t)jj * W] : make_double2(0.0, 0.0);
-        }
+    double v = 0.0;
+    if (r < nsets * nper && c < nsets * nper && r / nper == c / nper) {
+        const size_t TT = (size_t)T * T;
+        for (int r1x = max(-w, -w - Sx); r1x <= min(w, w - Sx); r1x++)
+            for (int r1y = max(-w, -w - Sy); r1y <= min(w, w - Sy); r1y++)
+                v += tiles[c * TT + (size_t)(ii + (w - r1x) * os) * T + jj + (w - r1y) * os] *
+                     tiles[r * TT + (size_t)(ii + (w - Sx - r1x) * os) * T + jj + (w - Sy - r1y) * os];
     }
-    for (int r = 0; r < NR; r++) {
-        // tile origin of R = (r/side - w, r%side - w): ((w - Rx) os, (w - Ry) os)
-        const int tx = (2 * w - r / side) * os, ty = (2 * w - r % side) * os;
-        for (int o = sg * nper; o < (sg + 1) * nper; o++) {
-            const double *wo = tiles + ((size_t)o * T + tx) * T + ty;
-            double2 acc = make_double2(0.0, 0.0);
-            if (NPIX > 0) {
-#pragma unroll
-                for (int pix = 0; pix < (NPIX > 0 ? NPIX : 1); pix++) {
-                    const int ii = pix / os, jj = pix - ii * os;
-                    if (pix < npix) acc = cadd(acc, cscale(wo[(size_t)ii * T + jj], v[pix]));
-                }
-            } else {
-                for (int ii = 0; ii < os; ii++)
-                    for (int jj = 0; jj < os; jj++) acc = cadd(acc, cscale(wo[(size_t)ii * T + jj], s[ii + (size_t)jj * W]));
-            }
-            Z[((size_t)r * g.maxb + o) * B * cells] = acc;
-        }
-    }
+    Q[idx] = v;
 }
 
-// Window adjoint, step 2: shifted sums over R plus the periodic term,
-//   Gbar(S)_{r,c} = sum_R Za[R][r][c][S-R] + sum_R Zb[R][c][r][-S-R]
-//                   - i sum_{R1} sum_x rhosum(x) w_c(x;R1) w_r(x;S+R1)   (|S| <= 2w, same spin block)
-// One thread per (slot, pair), lanes along the slot's x.
-__global__ void k_adj_shift(Geom g, const double *__restrict__ tiles, const double2 *__restrict__ Za,
-                            const double2 *__restrict__ Zb, const double *__restrict__ rhosum,
-                            double2 *__restrict__ Gbar)
+constexpr int kWinTX = 16;  // slots per workgroup along x
+constexpr int kWinTY = 8;   // slots per workgroup along y
+constexpr int kWinPix = 4;  // sub-pixel rows staged per phase
+
+// Gbar on the slot window, one thread per slot holding all b x b entries:
+//   Gbar(S)_{o,p} += sum_R sum_x w_o(x;R) abar^{s(o)}_p(cell S-R, x)      (a gather transposed)
+//   Gbar(S)_{p,o} += sum_R sum_x w_o(x;R) bbar^{s(o)}_p(cell -S-R, x)     (b gather transposed)
+//   Gbar(S)_{r,c} += -i sum_x rhosum(x) Q[r,c][S][x]                      (periodic term)
+// with cells relative to the map centre (cell index = position + n/2) and
+// s(o) the spin block of orbital o.  A workgroup owns kWinTX x kWinTY slots;
+// the adjoints of the cells they reach (the tile plus a w halo, and its mirror
+// image for b) are staged in LDS one strip of kWinPix x 1 sub-pixels at a
+// time, so each adjoint value is read from memory about once per tile.
+// Writes every slot of the window (plain stores, no accumulation).
+template <int B>
+__global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double *__restrict__ tiles,
+                                                            const double2 *__restrict__ abar,
+                                                            const double2 *__restrict__ bbar,
+                                                            const double *__restrict__ rhosum,
+                                                            const double *__restrict__ Q,
+                                                            double2 *__restrict__ Gbar)
 {
-    const int B = g.B, NP = B * B, nw = g.nw, h = g.h, os = g.os, w = g.w, T = g.T, n = g.n;
-    const int side = 2 * w + 1, NR = side * side, n2 = n >> 1;
-    const long total = (long)nw * nw * NP;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = blockIdx.y;
-    if (idx >= total) return;
-    const int sx = (int)(idx % nw), sy = (int)((idx / nw) % nw), pr = (int)(idx / ((long)nw * nw));
-    const int r = pr / B, c = pr % B;
-    const int Sx = sx - h, Sy = sy - h;
+    extern __shared__ double2 stage[];  // [a|b][kWinPix][RY][RX], then weights [kWinPix][NR][B]
+    const int os = g.os, w = g.w, T = g.T, W = g.W, n = g.n, nw = g.nw, h = g.h, n2 = n >> 1;
+    const int RX = kWinTX + 2 * w, RY = kWinTY + 2 * w, RC = RX * RY;
+    const int side = 2 * w + 1, NR = side * side;
+    double *wl = reinterpret_cast<double *>(stage + 2 * kWinPix * RC);
+    const int tx = threadIdx.x % kWinTX, ty = threadIdx.x / kWinTX;
+    const int S0x = (int)blockIdx.x * kWinTX - h, S0y = (int)blockIdx.y * kWinTY - h;
+    const int Sx = S0x + tx, Sy = S0y + ty;
+    const int e = blockIdx.z;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
-    const long cells = (long)n * n;
-    const size_t zoff = (size_t)e * NR * g.maxb * B * cells;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int rr = 0; rr < NR; rr++) {
-        const int rx = rr / side - w, ry = rr % side - w;
-        if (r < nsets * nper) {
-            const int cx = Sx - rx + n2, cy = Sy - ry + n2;
-            if (cx >= 0 && cy >= 0 && cx < n && cy < n)
-                acc = cadd(acc, Za[zoff + (((size_t)rr * g.maxb + r) * B + c) * cells + (size_t)cy * n + cx]);
-        }
-        if (c < nsets * nper) {
-            const int cx = -Sx - rx + n2, cy = -Sy - ry + n2;
-            if (cx >= 0 && cy >= 0 && cx < n && cy < n)
-                acc = cadd(acc, Zb[zoff + (((size_t)rr * g.maxb + c) * B + r) * cells + (size_t)cy * n + cx]);
-        }
-    }
-    if (r < nsets * nper && c < nsets * nper && r / nper == c / nper && abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
-        double re = 0.0;
-        for (int r1x = -w; r1x <= w; r1x++)
-            for (int r1y = -w; r1y <= w; r1y++) {
-                const int r2x = Sx + r1x, r2y = Sy + r1y;
-                if (r2x < -w || r2x > w || r2y < -w || r2y > w) continue;
-                for (int ii = 0; ii < os; ii++)
-                    for (int jj = 0; jj < os; jj++) {
-                        double w1 = tiles[((size_t)c * T + ii + (w - r1x) * os) * T + jj + (w - r1y) * os];
-                        double w2 = tiles[((size_t)r * T + ii + (w - r2x) * os) * T + jj + (w - r2y) * os];
-                        re += rhosum[(size_t)e * os * os + ii * os + jj] * w1 * w2;
+    // cell origins of the staged regions: a reads cells S-R, b reads cells -S-R
+    const int ax0 = S0x - w + n2, ay0 = S0y - w + n2;
+    const int bx0 = -S0x - (kWinTX - 1) - w + n2, by0 = -S0y - (kWinTY - 1) - w + n2;
+    const double2 *la = stage, *lb = stage + kWinPix * RC;
+    // this thread's offsets inside the staged regions for R = 0
+    const int oa = (ty + w) * RX + tx + w;
+    const int ob = (kWinTY - 1 - ty + w) * RX + (kWinTX - 1 - tx) + w;
+    const size_t TT = (size_t)T * T;
+    double2 acc[B * B];
+#pragma unroll
+    for (int i = 0; i < B * B; i++) acc[i] = make_double2(0.0, 0.0);
+
+    for (int sg = 0; sg < nsets; sg++) {
+        const int olo = sg * nper, ohi = olo + nper;
+#pragma unroll
+        for (int p = 0; p < B; p++) {
+            const double2 *sa = abar + (((size_t)e * nsets + sg) * B + p) * W * W;
+            const double2 *sb = bbar + (((size_t)e * nsets + sg) * B + p) * W * W;
+            for (int jj = 0; jj < os; jj++)
+                for (int ii0 = 0; ii0 < os; ii0 += kWinPix) {
+                    const int npx = min(kWinPix, os - ii0);
+                    __syncthreads();
+                    for (int l = threadIdx.x; l < 2 * kWinPix * RC; l += kWinTX * kWinTY) {
+                        const int rx = l % RX;
+                        int rest = l / RX;
+                        const int i = rest % kWinPix;
+                        rest /= kWinPix;
+                        const int ry = rest % RY, ab = rest / RY;
+                        const int cx = (ab ? bx0 : ax0) + rx, cy = (ab ? by0 : ay0) + ry;
+                        double2 v = make_double2(0.0, 0.0);
+                        if (i < npx && cx >= 0 && cy >= 0 && cx < n && cy < n)
+                            v = (ab ? sb : sa)[(size_t)(cx * os + ii0 + i) + (size_t)(cy * os + jj) * W];
+                        stage[((size_t)(ab * kWinPix + i) * RY + ry) * RX + rx] = v;
                     }
-            }
-        acc = cadd(acc, make_double2(0.0, -re));
+                    // weights w_o(x;R) of this strip, zero outside the spin block
+                    for (int l = threadIdx.x; l < kWinPix * NR * B; l += kWinTX * kWinTY) {
+                        const int o = l % B, r = (l / B) % NR, i = l / (B * NR);
+                        double v = 0.0;
+                        if (i < npx && o >= olo && o < ohi)
+                            v = tiles[o * TT + (size_t)(ii0 + i + (2 * w - r / side) * os) * T + jj +
+                                      (2 * w - r % side) * os];
+                        wl[l] = v;
+                    }
+                    __syncthreads();
+                    for (int i = 0; i < npx; i++) {
+                        const double2 *pa = la + (size_t)i * RC + oa, *pb = lb + (size_t)i * RC + ob;
+                        const double *wi = wl + (size_t)i * NR * B;
+                        for (int rxi = -w; rxi <= w; rxi++) {
+                            const double *wr = wi + (size_t)(rxi + w) * side * B;
+#pragma unroll 4
+                            for (int ryi = -w; ryi <= w; ryi++) {
+                                const double2 va = pa[-ryi * RX - rxi], vb = pb[-ryi * RX - rxi];
+                                const double *wt = wr + (ryi + w) * B;
+#pragma unroll
+                                for (int o = 0; o < B; o++) {
+                                    const double wo = wt[o];
+                                    acc[o * B + p] = cadd(acc[o * B + p], cscale(wo, va));
+                                    acc[p * B + o] = cadd(acc[p * B + o], cscale(wo, vb));
+                                }
+                            }
+                        }
+                    }
+                }
+        }
     }
-    Gbar[((size_t)e * nw * nw + (size_t)sx * nw + sy) * NP + pr] = acc;
+    if (Sx > h || Sy > h) return;
+    if (abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
+        const int D = 4 * w + 1, npix = os * os;
+        const double *rs = rhosum + (size_t)e * npix;
+        const double *q = Q + ((size_t)(Sx + 2 * w) * D + (Sy + 2 * w)) * npix;
+#pragma unroll
+        for (int pr = 0; pr < B * B; pr++) {
+            const double *qp = q + (size_t)pr * D * D * npix;
+            double re = 0.0;
+            for (int pix = 0; pix < npix; pix++) re += rs[pix] * qp[pix];
+            acc[pr].y -= re;
+        }
+    }
+    double2 *out = Gbar + ((size_t)e * nw * nw + (size_t)(Sx + h) * nw + (Sy + h)) * (B * B);
+#pragma unroll
+    for (int i = 0; i < B * B; i++) out[i] = acc[i];
 }
 
 constexpr int kMaxB = 16;
@@ -607,6 +644,24 @@ int launch_pix_t(const Geom &g, const double2 *G0w, const double2 *T, const doub
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
+size_t win_lds(const Geom &g)
+{
+    return sizeof(double2) * 2 * kWinPix * (size_t)(kWinTX + 2 * g.w) * (kWinTY + 2 * g.w) +
+           sizeof(double) * kWinPix * (size_t)(2 * g.w + 1) * (2 * g.w + 1) * g.B;
+}
+
+template <int B>
+int launch_win_t(const Geom &g, const double *tiles, const double2 *abar, const double2 *bbar, const double *rhosum,
+                 double *Q, int E, double2 *Gbar, hipStream_t s)
+{
+    const long nq = (long)cqpi::adjoint_q_size(g);
+    hipLaunchKernelGGL(k_adj_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, tiles, Q);
+    dim3 grid((g.nw + kWinTX - 1) / kWinTX, (g.nw + kWinTY - 1) / kWinTY, E);
+    hipLaunchKernelGGL(k_adj_win<B>, grid, dim3(kWinTX * kWinTY), win_lds(g), s, g, tiles, abar, bbar, rhosum, Q,
+                       Gbar);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
 template <int B>
 int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                 const double2 *F1bar, const double2 *twn, double2 *Hacc, hipStream_t s)
@@ -631,6 +686,11 @@ int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const do
 
 namespace cqpi {
 
+size_t adjoint_q_size(const Geom &g)
+{
+    return (size_t)g.B * g.B * (4 * g.w + 1) * (4 * g.w + 1) * g.os * g.os;
+}
+
 size_t adjoint_pix_blocks(const Geom &g)
 {
     size_t nb = (g.n + kCells - 1) / kCells;
@@ -641,7 +701,7 @@ bool adjoint_supported(const Geom &g)
 {
     bool pow2 = (g.K & (g.K - 1)) == 0;
     size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
-    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && g.os <= 16;
+    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && win_lds(g) <= 64 * 1024;
 }
 
 int adjoint_energy_groups(const Geom &g, int E) { return cqpi_energy_groups(g, E); }
@@ -650,8 +710,7 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
                    const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
                    double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
                    double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
-                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double2 *Za, double2 *Zb,
-                   hipStream_t s)
+                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q, hipStream_t s)
 {
     if (!adjoint_supported(g) || nry > kMaxRy) return CQPI_E_ARG;
     int nblk = 0, r = CQPI_E_ARG;
@@ -664,15 +723,13 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     if (r) return r;
     const int NP = g.B * g.B;
     hipLaunchKernelGGL(k_adj_reduce, dim3(NP + g.os * g.os, E), dim3(256), 0, s, g, Tpart, nblk, rhobar, Tbar, rhosum);
-    long zt = (long)g.n * g.n * g.B;
-    const dim3 zgrid((unsigned)((zt + 127) / 128), E, 2 * (g.spin ? 2 : 1));
-    if (g.os <= 4)
-        hipLaunchKernelGGL(k_adj_z<16>, zgrid, dim3(128), 0, s, g, tiles, abar, bbar, Za, Zb);
-    else
-        hipLaunchKernelGGL(k_adj_z<0>, zgrid, dim3(128), 0, s, g, tiles, abar, bbar, Za, Zb);
-    long total = (long)g.nw * g.nw * NP;
-    hipLaunchKernelGGL(k_adj_shift, dim3((unsigned)((total + 255) / 256), E), dim3(256), 0, s, g, tiles, Za, Zb,
-                       rhosum, Gbar);
+    switch (g.B) {
+    case 1: r = launch_win_t<1>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
+    case 2: r = launch_win_t<2>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
+    case 3: r = launch_win_t<3>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
+    case 4: r = launch_win_t<4>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
+    }
+    if (r) return r;
     hipLaunchKernelGGL(k_adj_tmatrix, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar);
     // adjoint transform along kx (pairs chunked to the LDS budget), then ky + resolvent
     int npmax = (int)((32 * 1024) / (sizeof(double2) * (size_t)g.K));

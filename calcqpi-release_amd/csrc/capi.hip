@@ -148,10 +148,10 @@ struct cqpi_ctx {
     double2 *d_abar = nullptr, *d_bbar = nullptr, *d_Tpart = nullptr, *d_Tbar = nullptr, *d_Gbar = nullptr;
     double2 *d_Vbar = nullptr, *d_Hacc = nullptr, *d_Rlist = nullptr, *d_SR = nullptr;
     size_t c_abar = 0, c_bbar = 0, c_Tpart = 0, c_Tbar = 0, c_Gbar = 0, c_Vbar = 0, c_Hacc = 0, c_Rlist = 0, c_SR = 0;
-    double *d_rhobar = nullptr, *d_rhosum = nullptr, *d_ry = nullptr;
-    size_t c_rhobar = 0, c_rhosum = 0, c_ry = 0;
-    double2 *d_U = nullptr, *d_Za = nullptr, *d_Zb = nullptr;
-    size_t c_U = 0, c_Za = 0, c_Zb = 0;
+    double *d_rhobar = nullptr, *d_rhosum = nullptr, *d_ry = nullptr, *d_Q = nullptr;
+    size_t c_rhobar = 0, c_rhosum = 0, c_ry = 0, c_Q = 0;
+    double2 *d_U = nullptr;
+    size_t c_U = 0;
 
     // host staging for the drop-in path (pinned)
     double2 *h_win = nullptr;
@@ -178,7 +178,7 @@ void release(cqpi_ctx *c)
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
                     c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
-                    c->d_rhosum, c->d_ry, c->d_U, c->d_Za, c->d_Zb};
+                    c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
     for (void *p : bufs)
         if (p) hipFree(p);
     if (c->h_win) hipHostFree(c->h_win);
@@ -753,9 +753,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     RETCHK(dev_grow(c->d_Hacc, c->c_Hacc, NG * g.K * g.K * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Rlist, c->c_Rlist, R.size(), c->gpumem));
     RETCHK(dev_grow(c->d_SR, c->c_SR, R.size() * NP, c->gpumem));
-    const size_t nz = E * (size_t)(2 * g.w + 1) * (2 * g.w + 1) * g.maxb * B * g.n * g.n;
-    RETCHK(dev_grow(c->d_Za, c->c_Za, nz, c->gpumem));
-    RETCHK(dev_grow(c->d_Zb, c->c_Zb, nz, c->gpumem));
+    RETCHK(dev_grow(c->d_Q, c->c_Q, adjoint_q_size(g), c->gpumem));
     const double *drb = rhobar;
     if (!on_device) {
         HIPCHK(hipMemcpyAsync(c->d_rhobar, rhobar, E * WW * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -767,7 +765,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     RETCHK(launch_adjoint(g, c->d_G0w, c->d_T, c->d_V, c->d_tiles, drb, c->d_eval, c->d_evec, c->d_omega, (int)E,
                           c->last_eta, c->d_twn, c->d_ry, nry, c->d_Rlist, (int)R.size(), c->d_abar, c->d_bbar,
                           c->d_Tpart, c->d_Tbar, c->d_rhosum, c->d_Gbar, c->d_Vbar, c->d_F1, c->d_Hacc, c->d_U,
-                          c->d_SR, c->d_Za, c->d_Zb, c->stream));
+                          c->d_SR, c->d_Q, c->stream));
     std::vector<double2> SR(R.size() * NP), Vb(E * NP);
     HIPCHK(hipMemcpyAsync(SR.data(), c->d_SR, SR.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(Vb.data(), c->d_Vbar, Vb.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));

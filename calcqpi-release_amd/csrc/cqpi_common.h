@@ -83,6 +83,8 @@ int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, d
                 hipStream_t s);
 // adjoint of the whole chain (see adjoint.hip)
 size_t adjoint_pix_blocks(const Geom &g);
+// doubles of the energy-independent weight-overlap table of the periodic term
+size_t adjoint_q_size(const Geom &g);
 bool adjoint_supported(const Geom &g);
 int adjoint_energy_groups(const Geom &g, int E);
 // Rl[r] = (rx, index into ryvals) of each distinct hopping vector, plus a last
@@ -91,8 +93,7 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
                    const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
                    double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
                    double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
-                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double2 *Za,
-                   double2 *Zb, hipStream_t s);
+                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q, hipStream_t s);
 // contraction: ldos[e][x + y*W]
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                          const double *tiles, int E, double *ldos, hipStream_t s);
