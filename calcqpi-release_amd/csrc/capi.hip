@@ -127,6 +127,12 @@ struct cqpi_ctx {
     size_t nlayers = 0;
     size_t last_chunk_e0 = 0, last_chunk_ec = 0;
 
+    // FP32 contraction: tiles, narrowed windows and T
+    float *d_tilesf = nullptr;
+    size_t c_tilesf = 0;
+    float2 *d_G0f = nullptr, *d_Tf = nullptr;
+    size_t c_G0f = 0, c_Tf = 0;
+
     // mixed precision: tile fragments, per-energy scales, fp16 planes
     MixGeom mg{};
     bool mix_ready = false;
@@ -176,7 +182,7 @@ void release(cqpi_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw,
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
-                    c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
+                    c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
     for (void *p : bufs)
@@ -276,6 +282,11 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
             r = launch_contract_mixed(g, c->mg, c->d_planes, c->d_afrag, c->d_mscale, c->sA, T, c->d_bulk, E,
                                       ldos, c->stream);
         break;
+    case CQPI_PREC_FP32:
+        RETCHK(dev_grow(c->d_G0f, c->c_G0f, (size_t)E * g.nw * g.nw * g.B * g.B, c->gpumem));
+        RETCHK(dev_grow(c->d_Tf, c->c_Tf, (size_t)E * g.B * g.B, c->gpumem));
+        r = launch_contract_fp32(g, G0w, T, c->d_bulk, c->d_tilesf, E, ldos, c->d_G0f, c->d_Tf, c->stream);
+        break;
     default:
         r = launch_contract_fp64(g, G0w, T, c->d_bulk, c->d_tiles, E, ldos, c->stream);
         break;
@@ -372,8 +383,14 @@ void cqpi_destroy(cqpi_ctx *ctx) { release(ctx); }
 int cqpi_set_precision(cqpi_ctx *c, int precision)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
-    if (precision != CQPI_PREC_FP64 && precision != CQPI_PREC_MIXED)
+    if (precision != CQPI_PREC_FP64 && precision != CQPI_PREC_FP32 && precision != CQPI_PREC_MIXED)
         return fail(CQPI_E_ARG, "precision mode not available");
+    if (precision == CQPI_PREC_FP32 && !c->d_tilesf) {
+        std::vector<float> tf(c->h_tiles.begin(), c->h_tiles.end());
+        RETCHK(dev_grow(c->d_tilesf, c->c_tilesf, tf.size(), c->gpumem));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->d_tilesf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     if (precision == CQPI_PREC_MIXED && !c->mix_ready) {
         if (c->g.os > 16) return fail(CQPI_E_ARG, "mixed precision needs oversamp <= 16");
         c->mg = mix_geometry(c->g);

@@ -1,4 +1,5 @@
-// Continuum contraction (FP64): rho(r, w) for every sub-pixel of the map.
+// Continuum contraction (FP64, and the same loop nest in FP32): rho(r, w) for
+// every sub-pixel of the map.
 //
 // The reference evaluates, per cell pos and sub-pixel x (src/qpi.cc:889-945,
 // src/qpihipkernel.cc:60-100),
@@ -23,13 +24,27 @@ namespace {
 constexpr int kCells = 8;     // cells per block side
 constexpr int kWaves = 4;
 
-template <int B, int P>
-__global__ void __launch_bounds__(64 * kWaves) k_contract64(Geom g, const double2 *__restrict__ G0w,
-                                                          const double2 *__restrict__ Tm,
-                                                          const double *__restrict__ bulk,
-                                                          const double *__restrict__ tiles,
-                                                          double *__restrict__ ldos)
+template <typename Real>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    using type = double2;
+};
+template <>
+struct Cplx<float> {
+    using type = float2;
+};
+
+// Real = double: the FP64 contraction.  Real = float: the same loop nest in
+// FP32 (G0 window, T and tiles narrowed to FP32, FP32 accumulation; the
+// periodic table stays FP64).
+template <typename Real, int B, int P>
+__global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename Cplx<Real>::type *__restrict__ G0w,
+                                                        const typename Cplx<Real>::type *__restrict__ Tm,
+                                                        const double *__restrict__ bulk,
+                                                        const Real *__restrict__ tiles, double *__restrict__ ldos)
 {
+    using C = typename Cplx<Real>::type;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int cx = blockIdx.x * kCells + (lane & (kCells - 1));
@@ -44,8 +59,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract64(Geom g, const double
     const int npg = (npix + P - 1) / P;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
-    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
-    const double2 *Te = Tm + (size_t)e * NP;
+    const C *G0 = G0w + (size_t)e * nw * nw * NP;
+    const C *Te = Tm + (size_t)e * NP;
     double *out = ldos + (size_t)e * W * W;
 
     for (int pg = wave; pg < npg; pg += kWaves) {
@@ -59,33 +74,33 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract64(Geom g, const double
             ii[p] = pix / os;
             jj[p] = pix - ii[p] * os;
         }
-        double val[P];
+        Real val[P];
 #pragma unroll
-        for (int p = 0; p < P; p++) val[p] = 0.0;
+        for (int p = 0; p < P; p++) val[p] = 0;
         for (int sg = 0; sg < nsets; sg++) {
             const int olo = sg * nper, ohi = olo + nper;
-            double2 acca[P][B], accb[P][B];
+            C acca[P][B], accb[P][B];
 #pragma unroll
             for (int p = 0; p < P; p++)
 #pragma unroll
                 for (int q = 0; q < B; q++) {
-                    acca[p][q] = make_double2(0.0, 0.0);
-                    accb[p][q] = make_double2(0.0, 0.0);
+                    acca[p][q].x = acca[p][q].y = 0;
+                    accb[p][q].x = accb[p][q].y = 0;
                 }
             for (int rx = -w; rx <= w; rx++) {
                 for (int ry = -w; ry <= w; ry++) {
-                    const double2 *GA = G0 + ((size_t)(px + rx + h) * nw + (py + ry + h)) * NP;
-                    const double2 *GB = G0 + ((size_t)(-px - rx + h) * nw + (-py - ry + h)) * NP;
+                    const C *GA = G0 + ((size_t)(px + rx + h) * nw + (py + ry + h)) * NP;
+                    const C *GB = G0 + ((size_t)(-px - rx + h) * nw + (-py - ry + h)) * NP;
                     const int tx = (w - rx) * os, ty = (w - ry) * os;
                     for (int o = olo; o < ohi; o++) {
-                        double wt[P];
+                        Real wt[P];
 #pragma unroll
                         for (int p = 0; p < P; p++)
                             wt[p] = tiles[((size_t)o * T + ii[p] + tx) * T + jj[p] + ty];
 #pragma unroll
                         for (int q = 0; q < B; q++) {
-                            double2 ga = GA[o * B + q];
-                            double2 gb = GB[q * B + o];
+                            const C ga = GA[o * B + q];
+                            const C gb = GB[q * B + o];
 #pragma unroll
                             for (int p = 0; p < P; p++) {
                                 acca[p][q].x = fma(wt[p], ga.x, acca[p][q].x);
@@ -100,15 +115,18 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract64(Geom g, const double
             // Im sum_pq a_p T_pq b_q
 #pragma unroll
             for (int q1 = 0; q1 < B; q1++) {
-                double2 trow[B];
+                C trow[B];
 #pragma unroll
                 for (int q2 = 0; q2 < B; q2++) trow[q2] = Te[q1 * B + q2];
 #pragma unroll
                 for (int p = 0; p < P; p++) {
-                    double2 tb = make_double2(0.0, 0.0);
+                    Real tbx = 0, tby = 0;
 #pragma unroll
-                    for (int q2 = 0; q2 < B; q2++) tb = cadd(tb, cmul(trow[q2], accb[p][q2]));
-                    val[p] += acca[p][q1].x * tb.y + acca[p][q1].y * tb.x;
+                    for (int q2 = 0; q2 < B; q2++) {
+                        tbx += trow[q2].x * accb[p][q2].x - trow[q2].y * accb[p][q2].y;
+                        tby += trow[q2].x * accb[p][q2].y + trow[q2].y * accb[p][q2].x;
+                    }
+                    val[p] += acca[p][q1].x * tby + acca[p][q1].y * tbx;
                 }
             }
         }
@@ -116,9 +134,18 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract64(Geom g, const double
         for (int p = 0; p < P; p++) {
             if (!ok[p]) continue;
             int X = cx * os + ii[p], Y = cy * os + jj[p];
-            out[(size_t)X + (size_t)Y * W] = -(val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+            out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
         }
     }
+}
+
+// FP64 -> FP32 narrowing of the G0 windows and T of a chunk
+__global__ void k_narrow(const double2 *__restrict__ a, size_t na, const double2 *__restrict__ t, size_t nt,
+                         float2 *__restrict__ af, float2 *__restrict__ tf)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) af[i] = make_float2((float)a[i].x, (float)a[i].y);
+    if (i < nt) tf[i] = make_float2((float)t[i].x, (float)t[i].y);
 }
 
 template <int B>
@@ -127,7 +154,18 @@ int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double
 {
     constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 6 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
-    hipLaunchKernelGGL((k_contract64<B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
+    hipLaunchKernelGGL((k_contract<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
+                       tiles, ldos);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+template <int B>
+int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double *bulk, const float *tiles, int E,
+               double *ldos, hipStream_t s)
+{
+    constexpr int P = B <= 3 ? 8 : (B <= 6 ? 4 : 2);
+    int nb = (g.n + kCells - 1) / kCells;
+    hipLaunchKernelGGL((k_contract<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf, bulk,
                        tiles, ldos);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
@@ -142,6 +180,22 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
     switch (g.B) {
 #define CASE(b) \
     case b: return launch_c64<b>(g, G0w, T, bulk, tiles, E, ldos, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+        CASE(11) CASE(12) CASE(14) CASE(16)
+#undef CASE
+    default: return CQPI_E_ARG;
+    }
+}
+
+int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
+                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s)
+{
+    const size_t na = (size_t)E * g.nw * g.nw * g.B * g.B, nt = (size_t)E * g.B * g.B;
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, G0w, na, T, nt, G0f, Tf);
+    if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    switch (g.B) {
+#define CASE(b) \
+    case b: return launch_c32<b>(g, G0f, Tf, bulk, tilesf, E, ldos, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
         CASE(11) CASE(12) CASE(14) CASE(16)
 #undef CASE

@@ -97,4 +97,8 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
 // contraction: ldos[e][x + y*W]
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                          const double *tiles, int E, double *ldos, hipStream_t s);
+// FP32 contraction: narrows the E windows and T into G0f (E*nw*nw*b*b) and Tf
+// (E*b*b) first; tilesf are the tiles in FP32
+int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
+                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s);
 }  // namespace cqpi

@@ -17,7 +17,7 @@
 //     src/idl.cc:969-973); the map is W x W x L with W = oversamp*lattice,
 //     header as in src/calcqpi.cc:640,655 and data written as float32, x
 //     fastest (idl::write, src/idl.cc:78-95).
-// Extension key: precision=fp64|mixed; (contraction precision, default fp64).
+// Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
 // Modes other than output=wannier / green=normal and analytic or XSF orbitals
 // are reported as unsupported.
 #include <chrono>
@@ -205,7 +205,7 @@ int main(int argc, char **argv)
     if (argi >= argc) {
         std::cerr << "Syntax: calcqpi_amd [--print-config] configfile\n"
                   << "Continuum-LDOS (output=wannier, green=normal) on one MI355X; keys as calcqpi,\n"
-                  << "plus precision=fp64|mixed; for the contraction.\n";
+                  << "plus precision=fp64|fp32|mixed; for the contraction.\n";
         return 1;
     }
     Config lf(argv[argi]);
@@ -270,8 +270,8 @@ int main(int argc, char **argv)
         for (double v : lf.vec("scattering")) scattering.push_back(v * scatphase);
     if (lf.probe("magscat")) magscat = lf.num("magscat");
     if (lf.probe("precision")) precision = lf.get("precision");
-    if (precision != "fp64" && precision != "mixed") {
-        std::cerr << "precision must be fp64 or mixed.\n";
+    if (precision != "fp64" && precision != "fp32" && precision != "mixed") {
+        std::cerr << "precision must be fp64, fp32 or mixed.\n";
         return 1;
     }
 
@@ -380,7 +380,9 @@ int main(int argc, char **argv)
     int rc = check(cqpi_set_model(ctx, hops, nhops, efermi, scmodel), "cqpi_set_model");
     cqpi_free(hops);
     if (!rc) rc = check(cqpi_set_impurity(ctx, V.data()), "cqpi_set_impurity");
-    if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed" ? CQPI_PREC_MIXED : CQPI_PREC_FP64), "cqpi_set_precision");
+    if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed"  ? CQPI_PREC_MIXED
+                                                    : precision == "fp32" ? CQPI_PREC_FP32
+                                                                          : CQPI_PREC_FP64), "cqpi_set_precision");
     IdlMap out;
     out.data.assign((size_t)W * W * layers, 0.0);
     if (!rc) rc = check(cqpi_run(ctx, omegas.data(), omegas.size(), eta, out.data.data()), "cqpi_run");

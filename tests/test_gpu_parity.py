@@ -91,6 +91,28 @@ def test_g0_window_and_tmatrix(gpu):
             assert np.max(np.abs(T - Tref)) < 1e-12 * np.max(np.abs(Tref))
 
 
+@pytest.mark.parametrize("K", [256, 24])
+def test_transform_matches_rocfft(gpu, K):
+    """The in-house k->R transform against rocFFT (torch.fft on a ROCm device
+    runs hipFFT/rocFFT): G0(R) = K^-2 sum_k G0(k) e^{+2 pi i k.R} is ifft2 over
+    the k-grid.  G0(k) is built on the host from H(k) by a dense inverse."""
+    import torch
+    wl = synth.square_norb(3, K=K, n=8, E=2, seed=15, w=1, os_=2, eta=0.01)
+    kg = synth.kgrid(K)
+    H = synth.hk_numpy(wl.hops, wl.bands, wl.efermi, kg[:, None], kg[None, :])
+    print("case: rocFFT cross-check K=%d b=%d" % (K, wl.bands), flush=True)
+    with cqpi.engine_for(wl) as q:
+        q.run(wl.energies, wl.eta)
+        h = q.window_half()
+        idx = (np.arange(2 * h + 1) - h) % K
+        for e, om in enumerate(wl.energies):
+            Gk = np.linalg.inv((om + 1j * wl.eta) * np.eye(wl.bands) - H)
+            gR = torch.fft.ifft2(torch.from_numpy(Gk).to("cuda"), dim=(0, 1)).cpu().numpy()
+            ref = gR[np.ix_(idx, idx)]
+            win = q.g0_window(e)
+            assert np.max(np.abs(win - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
 def test_dropin_wannierldos_matches_oracle(gpu):
     """GPUQPI::wannierldos path: host G0(R) and T in, one map out."""
     wl = synth.square_3orb(K=32, n=12, E=1)
@@ -160,13 +182,12 @@ def test_singular_tmatrix_reported(gpu):
 
 
 # ---------------------------------------------------------------------------
-# mixed precision (split-fp16 MFMA contraction, FP64 elsewhere): 1e-4 relative
+# reduced-precision contractions (split-fp16 MFMA; FP32), FP64 elsewhere: 1e-4 relative
 # ---------------------------------------------------------------------------
 TOLMIX = 1e-4
 
 
-@pytest.mark.parametrize("case", ["c2", "c1", "hexspin", "oddn", "b5"])
-def test_mixed_matches_oracle(gpu, case):
+def _reduced_case(case):
     if case == "c2":
         wl = synth.square_3orb(K=64, n=16, E=3)
     elif case == "c1":
@@ -179,15 +200,23 @@ def test_mixed_matches_oracle(gpu, case):
         wl = synth.square_norb(3, K=16, n=13, E=2, seed=31, w=2, os_=4, eta=0.01)
     else:
         wl = synth.square_norb(5, K=32, n=10, E=2, seed=32, w=2, os_=4, eta=0.01)
-    got = _run(wl, "mixed")
+    return wl
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+@pytest.mark.parametrize("case", ["c2", "c1", "hexspin", "oddn", "b5"])
+def test_reduced_precision_matches_oracle(gpu, case, prec):
+    wl = _reduced_case(case)
+    got = _run(wl, prec)
     ref = oracle.cldos_run(wl)
     for e in range(ref.shape[0]):
         mx, rms = rm.rel_errors(got[e], ref[e])
-        print("mixed %s energy %d: max_rel %.3e rms_rel %.3e" % (case, e, mx, rms))
+        print("%s %s energy %d: max_rel %.3e rms_rel %.3e" % (prec, case, e, mx, rms))
         assert mx < TOLMIX, (case, e, mx, rms)
 
 
-def test_mixed_dropin_path(gpu):
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_reduced_precision_dropin_path(gpu, prec):
     wl = synth.square_3orb(K=32, n=12, E=1)
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
     gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
@@ -195,7 +224,7 @@ def test_mixed_dropin_path(gpu):
     ref = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp)
     q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
     try:
-        q.set_precision("mixed")
+        q.set_precision(prec)
         q.wannierldos(T, gR)
         got = q.retrieveResult()
     finally:
