@@ -67,7 +67,7 @@ int launch_eigen(int B, const DevRGroup *groups, int ngroups, const DevHop *hops
                  int scmax, int K, double *eval, double2 *evec, hipStream_t s);
 
 // resolvent G0(k,w) from eigen data fused with the transform along ky; writes
-// F1[e][i][slot][pair] for the Ry window.  omegas: device array of the chunk.
+// F1[e][slot][i][pair] for the Ry window.  omegas: device array of the chunk.
 int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
                         const double *omegas, int E, double eta, const double2 *tw,
                         double2 *F1, hipStream_t s);

@@ -70,4 +70,141 @@ __device__ inline double2 dft_point(const double2 *L, int K, int Rm, const doubl
     return acc;
 }
 
+
+// ---------------------------------------------------------------------------
+// Register-staged Stockham transform of one line per lane group.
+//
+// A line of N = 2^lg points (natural order, in LDS) is transformed in place
+// by a group of LN lanes of one wave: out[m] = sum_j in[j] w^(jm), w =
+// exp(sign 2 pi i / N).  Each pass of radix R (8 while possible, then 4 or 2)
+// loads R points per butterfly into registers (stride N/R: consecutive lanes
+// read consecutive addresses), applies the twiddles w_{LR}^{j r} from the
+// table tw[m] = w^m (m < N), does the radix-R DFT in registers and stores
+// the outputs in Stockham order, so the result is in natural order and no
+// bit reversal is needed.  Only the lanes of one wave touch a line, so the
+// passes need no workgroup barrier: a wavefront-scope fence orders the LDS
+// accesses between passes.
+// ---------------------------------------------------------------------------
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// multiply by sign * i
+__device__ inline double2 mul_si(double2 a, int sign)
+{
+    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R>
+__device__ inline void dft_small(double2 *x, int sign)
+{
+    if constexpr (R == 2) {
+        double2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+        double2 a2 = cadd(x[1], x[3]), a3 = mul_si(csub(x[1], x[3]), sign);
+        x[0] = cadd(a0, a2);
+        x[2] = csub(a0, a2);
+        x[1] = cadd(a1, a3);
+        x[3] = csub(a1, a3);
+    } else {
+        double2 e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+        dft_small<4>(e, sign);
+        dft_small<4>(o, sign);
+        constexpr double r2 = 0.70710678118654752440;
+        // w8^q o_q, w8 = (1 + sign i)/sqrt2
+        double2 o1 = cscale(r2, cadd(o[1], mul_si(o[1], sign)));
+        double2 o2 = mul_si(o[2], sign);
+        double2 o3 = cscale(r2, csub(mul_si(o[3], sign), o[3]));
+        x[0] = cadd(e[0], o[0]);
+        x[4] = csub(e[0], o[0]);
+        x[1] = cadd(e[1], o1);
+        x[5] = csub(e[1], o1);
+        x[2] = cadd(e[2], o2);
+        x[6] = csub(e[2], o2);
+        x[3] = cadd(e[3], o3);
+        x[7] = csub(e[3], o3);
+    }
+}
+
+// radix of the first pass for 2^lg remaining: 8 while possible, 4+4 for 16
+constexpr int stockham_radix(int lg) { return (lg >= 3 && lg != 4) ? 8 : (lg >= 2 ? 4 : 2); }
+constexpr int ilog2c(int r) { return r == 8 ? 3 : (r == 4 ? 2 : 1); }
+constexpr int ilog2n(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+// largest butterfly count of any pass = lanes that one line keeps busy (<= 64)
+constexpr int stockham_lanes(int N)
+{
+    int lg = ilog2n(N);
+    int mx = 1;
+    while (lg > 0) {
+        int r = stockham_radix(lg);
+        int nb = N / r;
+        if (nb > mx) mx = nb;
+        lg -= ilog2c(r);
+    }
+    return mx < 64 ? mx : 64;
+}
+
+template <int N, int L, int R, int LN>
+__device__ inline void stockham_pass(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+{
+    constexpr int NB = N / R;
+    constexpr int BPL = (NB + LN - 1) / LN;
+    double2 v[BPL][R];
+#pragma unroll
+    for (int b = 0; b < BPL; b++) {
+        const int k = ll + LN * b;
+        if (k < NB)
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = X[k + r * NB];
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < BPL; b++) {
+        const int k = ll + LN * b;
+        if (k < NB) {
+            const int j = k & (L - 1);
+            if constexpr (L > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw[j * r * (N / (L * R))]);
+            }
+            dft_small<R>(v[b], sign);
+            const int base = (k - j) * R + j;
+#pragma unroll
+            for (int q = 0; q < R; q++) X[base + q * L] = v[b][q];
+        }
+    }
+    wave_sync();
+}
+
+template <int N, int L, int LGREM, int LN>
+__device__ inline void stockham_rest(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+{
+    if constexpr (LGREM > 0) {
+        constexpr int R = stockham_radix(LGREM);
+        stockham_pass<N, L, R, LN>(X, tw, ll, sign);
+        stockham_rest<N, L * R, LGREM - ilog2c(R), LN>(X, tw, ll, sign);
+    }
+}
+
+// transform the line X (N points) with lanes ll in [0, stockham_lanes(N))
+template <int N>
+__device__ inline void wave_fft(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+{
+    constexpr int LN = stockham_lanes(N);
+    constexpr int lg = ilog2n(N);
+    stockham_rest<N, 1, lg, LN>(X, tw, ll, sign);
+}
+
 }  // namespace cqpi

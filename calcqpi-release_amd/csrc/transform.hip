@@ -8,16 +8,18 @@
 // contraction reads (src/qpihipkernel.cc:383-393).  Here the resolvent is
 // formed inside the first transform pass, in LDS, and each pass keeps only
 // the window R in [-h, h] (side nw = 2h+1):
-//   pass 1 (one workgroup per kx row i and batch of energies; the row's eigen
-//          data is read once for the whole batch):
-//       S[pair][ky] = G0(k)  ->  FFT along ky  ->  F1[e][i][Ry slot][pair]
-//   pass 2 (one workgroup per (Ry slot, energy, pair chunk)):
+//   pass 1 (one workgroup per kx row i, batch of energies and chunk of
+//          orbital pairs; the row's eigen data is read once for the batch):
+//       S[pair][ky] = G0(k)  ->  FFT along ky  ->  F1[e][Ry slot][i][pair]
+//   pass 2 (one workgroup per (Ry slot, energy, pair chunk), reading one
+//          contiguous block of F1):
 //       S[pair][kx] = F1     ->  FFT along kx  ->  G0w[e][Rx slot][Ry slot][pair] / K^2
 //       and, for the mixed-precision contraction, the fp16 hi/lo planes.
-// Power-of-two K use an in-LDS decimation-in-time transform (bit-reversed
-// load, radix-4 stages plus one radix-2 stage when log2 K is odd); other K
-// fall back to a direct DFT of the window outputs.  Twiddles
-// tw[m] = exp(+2 pi i m/K) come from a table computed on the host.
+// Power-of-two K from 64 to 1024 use the register-staged Stockham transform
+// (one line per lane group of one wave, radix 8/4/2 passes, no workgroup
+// barrier between passes); smaller powers of two use the in-LDS radix-4
+// transform; other K a direct DFT of the window outputs.  The twiddle table
+// tw[m] = exp(+2 pi i m/K), m < K, is computed on the host.
 #include "cqpi_common.h"
 #include "cqpi_fft.h"
 #include "mixed.h"
@@ -125,79 +127,31 @@ __global__ void __launch_bounds__(kThreads) k_resolvent_ky(Geom g, const double 
             }
         }
         __syncthreads();
-        double2 *out = F1 + ((size_t)e * K + i) * (size_t)nw * NP;
+        double2 *out = F1 + (size_t)e * nw * K * NP + (size_t)i * NP;
         if (pow2) {
             fft_lines(S, npc, K, lgK, Tw);
             for (int t = threadIdx.x; t < nw * npc; t += kThreads) {
                 int slot = t / npc, q = t - slot * npc;
-                out[(size_t)slot * NP + p0 + q] = S[(size_t)q * K + imod(slot - h, K)];
+                out[(size_t)slot * K * NP + p0 + q] = S[(size_t)q * K + imod(slot - h, K)];
             }
         } else {
             for (int t = threadIdx.x; t < nw * npc; t += kThreads) {
                 int slot = t / npc, q = t - slot * npc;
-                out[(size_t)slot * NP + p0 + q] = dft_point(S + (size_t)q * K, K, imod(slot - h, K), tw);
+                out[(size_t)slot * K * NP + p0 + q] = dft_point(S + (size_t)q * K, K, imod(slot - h, K), tw);
             }
         }
         __syncthreads();  // S is rewritten for the next energy
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, const double2 *__restrict__ F1,
-                                                         const double2 *__restrict__ tw, int np, double sG,
-                                                         double2 *__restrict__ G0w, _Float16 *__restrict__ P)
+// fp16 hi/lo planes for the mixed contraction (rows along x), from the
+// transformed lines S[q][m] (window point R at m = R mod K), scaled by s:
+//   a planes: row sy,      column sx  <- G0(sx, sy)[o][p]
+//   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
+__device__ inline void write_planes(const double2 *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
+                                    const MixGeom &mg, double s, _Float16 *__restrict__ P)
 {
-    extern __shared__ double2 S[];
-    const int K = g.K, sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
-    const int npc = min(np, NP - p0);
-    const int nw = g.nw, h = g.h;
-    const bool pow2 = (K & (K - 1)) == 0;
-    int lgK = 0;
-    while ((1 << lgK) < K) lgK++;
-    const double2 *in = F1 + (size_t)e * K * nw * NP;
-    double2 *Tw = S + (size_t)np * K;  // twiddles w^m, m < K/2, staged in LDS
-    if (pow2)
-        for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = tw[m];
-    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
-        int i = t / npc, q = t - i * npc;
-        int dst = pow2 ? brev(i, lgK) : i;
-        S[(size_t)q * K + dst] = in[((size_t)i * nw + sy) * NP + p0 + q];
-    }
-    __syncthreads();
-    const double inv = 1.0 / ((double)K * (double)K);
-    double2 *out = G0w + (size_t)e * nw * nw * NP;
-    if (pow2) {
-        fft_lines(S, npc, K, lgK, Tw);
-        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-            int sx = t / npc, q = t - sx * npc;
-            out[((size_t)sx * nw + sy) * NP + p0 + q] = cscale(inv, S[(size_t)q * K + imod(sx - h, K)]);
-        }
-    } else {
-        // direct DFT: transform the needed window points in place of the line
-        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-            int sx = t / npc, q = t - sx * npc;
-            out[((size_t)sx * nw + sy) * NP + p0 + q] =
-                cscale(inv, dft_point(S + (size_t)q * K, K, imod(sx - h, K), tw));
-        }
-        if (!P) return;
-        // the planes below read S at window points: make S hold them
-        __syncthreads();
-        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-            int sx = t / npc, q = t - sx * npc;
-            S[(size_t)q * K + imod(sx - h, K)] = make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-            int sx = t / npc, q = t - sx * npc;
-            S[(size_t)q * K + imod(sx - h, K)] = out[((size_t)sx * nw + sy) * NP + p0 + q];
-        }
-        __syncthreads();
-    }
-    if (!P) return;
-    // fp16 hi/lo planes for the mixed contraction (rows along x):
-    //   a planes: row sy,      column sx  <- G0(sx, sy)[o][p]
-    //   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
     const size_t plane = (size_t)mg.Ly * nw;
-    const double s = sG * (pow2 ? inv : 1.0);
     _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
     for (int t = threadIdx.x; t < 2 * npc * 2 * mg.Ly; t += blockDim.x) {
         int idx = t % mg.Ly;
@@ -233,6 +187,179 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, const double2 *__restrict__ F1,
+                                                         const double2 *__restrict__ tw, int np, double sG,
+                                                         double2 *__restrict__ G0w, _Float16 *__restrict__ P)
+{
+    extern __shared__ double2 S[];
+    const int K = g.K, sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
+    const int npc = min(np, NP - p0);
+    const int nw = g.nw, h = g.h;
+    const bool pow2 = (K & (K - 1)) == 0;
+    int lgK = 0;
+    while ((1 << lgK) < K) lgK++;
+    const double2 *in = F1 + ((size_t)e * nw + sy) * K * NP;
+    double2 *Tw = S + (size_t)np * K;  // twiddles w^m, m < K/2, staged in LDS
+    if (pow2)
+        for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = tw[m];
+    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
+        int i = t / npc, q = t - i * npc;
+        int dst = pow2 ? brev(i, lgK) : i;
+        S[(size_t)q * K + dst] = in[(size_t)i * NP + p0 + q];
+    }
+    __syncthreads();
+    const double inv = 1.0 / ((double)K * (double)K);
+    double2 *out = G0w + (size_t)e * nw * nw * NP;
+    if (pow2) {
+        fft_lines(S, npc, K, lgK, Tw);
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            int sx = t / npc, q = t - sx * npc;
+            out[((size_t)sx * nw + sy) * NP + p0 + q] = cscale(inv, S[(size_t)q * K + imod(sx - h, K)]);
+        }
+    } else {
+        // direct DFT: transform the needed window points in place of the line
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            int sx = t / npc, q = t - sx * npc;
+            out[((size_t)sx * nw + sy) * NP + p0 + q] =
+                cscale(inv, dft_point(S + (size_t)q * K, K, imod(sx - h, K), tw));
+        }
+        if (!P) return;
+        // the planes below read S at window points: make S hold them
+        __syncthreads();
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            int sx = t / npc, q = t - sx * npc;
+            S[(size_t)q * K + imod(sx - h, K)] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            int sx = t / npc, q = t - sx * npc;
+            S[(size_t)q * K + imod(sx - h, K)] = out[((size_t)sx * nw + sy) * NP + p0 + q];
+        }
+        __syncthreads();
+    }
+    if (!P) return;
+    write_planes(S, K, sy, e, p0, npc, B, nw, h, mg, sG * (pow2 ? inv : 1.0), P);
+}
+
+// ---------------------------------------------------------------------------
+// Stockham path (power-of-two K = N, 64 <= N <= 1024)
+// ---------------------------------------------------------------------------
+constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
+constexpr int kWaveMaxThreads = 512;
+
+// pass 1: block = lane groups of stockham_lanes(N) lanes, one line (orbital
+// pair of the chunk) per group.  B > 0: the eigen data of the thread's k-point
+// (N <= blockDim) stays in registers for the whole energy batch; B == 0: band
+// count g.B, eigen data read per energy.
+template <int B, int N>
+__global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const double *__restrict__ eval,
+                                                   const double2 *__restrict__ evec,
+                                                   const double *__restrict__ omegas, int E, int EB, double eta,
+                                                   const double2 *__restrict__ tw, int np, double2 *__restrict__ F1)
+{
+    extern __shared__ double2 S[];
+    constexpr int LN = cqpi::stockham_lanes(N);
+    constexpr bool kRegs = B > 0 && B <= 4;
+    const int nb = B > 0 ? B : g.B, NP = nb * nb;
+    const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
+    const int e0 = blockIdx.y * EB, e1 = min(E, e0 + EB);
+    const long KK = (long)N * N;
+    const int nw = g.nw, h = g.h;
+    const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
+    const bool regs = kRegs && N <= (int)blockDim.x;
+
+    double ev_r[kRegs ? B : 1];
+    double2 vv_r[kRegs ? B * B : 1];
+    if constexpr (kRegs) {
+        const int j = threadIdx.x;
+        if (regs && j < N) {
+            const long k = (long)i * N + j;
+#pragma unroll
+            for (int n = 0; n < B; n++) ev_r[n] = eval[n * KK + k];
+#pragma unroll
+            for (int r = 0; r < B * B; r++) vv_r[r] = evec[r * KK + k];
+        }
+    }
+    for (int e = e0; e < e1; e++) {
+        const double omega = omegas[e];
+        if constexpr (kRegs) {
+            const int j = threadIdx.x;
+            if (regs && j < N) {
+                double2 f[B];
+#pragma unroll
+                for (int n = 0; n < B; n++) f[n] = cinv(make_double2(omega - ev_r[n], eta));
+                for (int q = 0; q < npc; q++) {
+                    const int pr = p0 + q, r = pr / B, c = pr - r * B;
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int n = 0; n < B; n++) {
+                        double2 vr = vv_r[n], vc = vv_r[n];
+#pragma unroll
+                        for (int x = 0; x < B; x++) {
+                            if (x == r) vr = vv_r[x * B + n];
+                            if (x == c) vc = vv_r[x * B + n];
+                        }
+                        acc = cadd(acc, cmul(f[n], cmulc(vr, vc)));
+                    }
+                    S[(size_t)q * N + j] = acc;
+                }
+            }
+        }
+        if (!regs) {
+            for (int j = threadIdx.x; j < N; j += blockDim.x) {
+                const long k = (long)i * N + j;
+                for (int q = 0; q < npc; q++) {
+                    const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int n = 0; n < nb; n++) {
+                        const double2 f = cinv(make_double2(omega - eval[n * KK + k], eta));
+                        acc = cadd(acc, cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
+                    }
+                    S[(size_t)q * N + j] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * N, tw, ll, +1);
+        __syncthreads();
+        double2 *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            const int slot = t / npc, q = t - slot * npc;
+            out[(size_t)slot * N * NP + q] = S[(size_t)q * N + ((slot - h) & (N - 1))];
+        }
+        __syncthreads();  // S is rewritten for the next energy
+    }
+}
+
+// pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk)
+template <int N>
+__global__ void __launch_bounds__(kWaveMaxThreads) k_kx_s(Geom g, MixGeom mg, const double2 *__restrict__ F1,
+                                               const double2 *__restrict__ tw, int np, double sG,
+                                               double2 *__restrict__ G0w, _Float16 *__restrict__ P)
+{
+    extern __shared__ double2 S[];
+    constexpr int LN = cqpi::stockham_lanes(N);
+    const int sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
+    const int npc = min(np, NP - p0);
+    const int nw = g.nw, h = g.h;
+    const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
+    const double2 *in = F1 + ((size_t)e * nw + sy) * N * NP + p0;
+    for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
+        const int i = t / npc, q = t - i * npc;
+        S[(size_t)q * N + i] = in[(size_t)i * NP + q];
+    }
+    __syncthreads();
+    for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * N, tw, ll, +1);
+    __syncthreads();
+    const double inv = 1.0 / ((double)N * (double)N);
+    double2 *out = G0w + (size_t)e * nw * nw * NP;
+    for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+        const int sx = t / npc, q = t - sx * npc;
+        out[((size_t)sx * nw + sy) * NP + p0 + q] = cscale(inv, S[(size_t)q * N + ((sx - h) & (N - 1))]);
+    }
+    if (P) write_planes(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
+}
+
 constexpr size_t kLdsBudget = 80 * 1024;   // largest line buffer a workgroup may take
 constexpr size_t kLdsTarget = 32 * 1024;   // preferred: several workgroups per CU
 
@@ -245,6 +372,70 @@ int pairs_per_chunk(int K, int NP)
     int nch = (NP + npmax - 1) / npmax;
     return (NP + nch - 1) / nch;
 }
+
+// Stockham path: pairs per chunk from the LDS budget (one line of N doubles2
+// per pair), workgroup = enough lane groups for the chunk's lines and, on
+// pass 1, at least one thread per k-point (up to kWaveMaxThreads)
+constexpr size_t kWaveLds = 80 * 1024;
+
+int wave_pairs(int N, int NP)
+{
+    const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
+    const int lines = std::max(1, std::min(groups, (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
+    const int nch = (NP + lines - 1) / lines;
+    return (NP + nch - 1) / nch;
+}
+
+int wave_threads(int N, int np, bool pass1)
+{
+    int t = np * cqpi::stockham_lanes(N);
+    if (pass1) t = std::max(t, std::min(N, kWaveMaxThreads));
+    t = (t + 63) / 64 * 64;
+    return std::min(t, kWaveMaxThreads);
+}
+
+template <int B, int N>
+int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
+                 const double2 *tw, double2 *F1, hipStream_t s)
+{
+    const int NP = g.B * g.B, np = wave_pairs(N, NP), nch = (NP + np - 1) / np;
+    const int thr = wave_threads(N, np, true);
+    if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
+    // energies per workgroup: enough workgroups for ~2 per CU
+    const int rows = N * nch;
+    const int EB = std::max(1, std::min(E, (rows * E) / 512));
+    const int nb = (E + EB - 1) / EB;
+    const size_t lds = sizeof(double2) * (size_t)np * N;
+    hipLaunchKernelGGL((k_res_ky_s<B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB, eta,
+                       tw, np, F1);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+template <int N>
+int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
+                  const double2 *tw, double2 *F1, hipStream_t s)
+{
+    switch (g.B) {
+    case 1: return launch_res_s<1, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 2: return launch_res_s<2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 3: return launch_res_s<3, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 4: return launch_res_s<4, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    default: return launch_res_s<0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    }
+}
+
+template <int N>
+int launch_kx_s(const Geom &g, const MixGeom &m, const double2 *F1, int E, const double2 *tw, double sG,
+                double2 *G0w, _Float16 *P, hipStream_t s)
+{
+    const int NP = g.B * g.B, np = wave_pairs(N, NP), nch = (NP + np - 1) / np;
+    const int thr = wave_threads(N, np, false);
+    const size_t lds = sizeof(double2) * (size_t)np * N;
+    hipLaunchKernelGGL((k_kx_s<N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+bool wave_path(int K) { return (K & (K - 1)) == 0 && K >= kMinWaveK && K <= kMaxWaveK; }
 
 template <int B>
 int launch_res_t(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
@@ -279,6 +470,15 @@ namespace cqpi {
 int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, const double *omegas,
                         int E, double eta, const double2 *tw, double2 *F1, hipStream_t s)
 {
+    if (wave_path(g.K)) {
+        switch (g.K) {
+        case 64: return launch_res_sb<64>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 128: return launch_res_sb<128>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 256: return launch_res_sb<256>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 512: return launch_res_sb<512>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 1024: return launch_res_sb<1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        }
+    }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
     switch (g.B) {
 #define CASE(b) \
@@ -293,13 +493,23 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
 int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,
                         double sG, double2 *G0w, _Float16 *P, hipStream_t s)
 {
+    MixGeom m{};
+    if (mg) m = *mg;
+    _Float16 *Pm = mg ? P : nullptr;
+    if (wave_path(g.K)) {
+        switch (g.K) {
+        case 64: return launch_kx_s<64>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 128: return launch_kx_s<128>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 256: return launch_kx_s<256>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 512: return launch_kx_s<512>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 1024: return launch_kx_s<1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        }
+    }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
     int NP = g.B * g.B;
     int np = pairs_per_chunk(g.K, NP);
     int nch = (NP + np - 1) / np;
     size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
-    MixGeom m{};
-    if (mg) m = *mg;
     hipLaunchKernelGGL(k_transform_kx, dim3(g.nw, E, nch), dim3(kThreads), lds, s, g, m, F1, tw, np, sG,
                        G0w, mg ? P : nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
