@@ -60,7 +60,25 @@ SIGNATURES = {
     "cqpi_set_keep_windows": (C.c_int, [_vp, C.c_int]),
     "cqpi_gradient": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "cqpi_gradient_device": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
+    "cqpi_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "cqpi_get_tuning": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cqpi_autotune": (C.c_int, [_vp, _dp, _sz, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cqpi_batch_create": (C.c_int, [_vp, _sz, C.c_int, C.POINTER(_vp)]),
+    "cqpi_batch_autotune": (C.c_int, [_vp, C.c_int]),
+    "cqpi_batch_run": (C.c_int, [_vp]),
+    "cqpi_batch_size": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "cqpi_batch_context": (C.c_int, [_vp, _sz, C.POINTER(_vp)]),
+    "cqpi_batch_info": (C.c_int, [_vp, _dp, C.POINTER(_sz), C.POINTER(_sz)]),
+    "cqpi_batch_destroy": (None, [_vp]),
 }
+
+
+class _Instance(C.Structure):
+    """cqpi_instance of include/cqpi_amd.h."""
+    _fields_ = [("kpoints", _sz), ("n", _sz), ("window", _sz), ("oversamp", _sz), ("bands", _sz),
+                ("maxband", _sz), ("spin", C.c_int), ("scmodel", C.c_int), ("hops", _vp), ("nhops", _sz),
+                ("efermi", C.c_double), ("V", _dp), ("wf", _dp), ("omegas", _dp), ("E", _sz),
+                ("eta", C.c_double)]
 
 
 class _DeviceArray:
@@ -262,6 +280,23 @@ class AmdQPI:
                                           _d(gV.view(np.float64))))
         return gh[:nh], float(gmu[0]), gV
 
+    def set_tuning(self, tpw=0, eb=0):
+        _check(lib().cqpi_set_tuning(self._ctx, int(tpw), int(eb)))
+
+    def tuning(self):
+        t, e = C.c_int(), C.c_int()
+        _check(lib().cqpi_get_tuning(self._ctx, C.byref(t), C.byref(e)))
+        return t.value, e.value
+
+    def autotune(self, omegas, eta, reps=3):
+        """Times candidate launch tunings for this shape on the device and keeps
+        the fastest; returns (tpw, eb)."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        t, e = C.c_int(), C.c_int()
+        _check(lib().cqpi_autotune(self._ctx, _d(om), len(om), float(eta), int(reps), C.byref(t), C.byref(e)))
+        self.nlayers = len(om)
+        return t.value, e.value
+
     def ldos_tensor(self):
         """The maps of the last run as a torch CUDA tensor (E, W*W), a
         zero-copy view of the library buffer (valid until the next run)."""
@@ -301,3 +336,83 @@ def engine_for(wl: "synth.Workload", precision="fp64") -> AmdQPI:
     q.set_impurity(wl.V)
     q.set_precision(precision)
     return q
+
+
+class _Borrowed(AmdQPI):
+    """An instance context owned by a Batch (not destroyed by this object)."""
+
+    def __init__(self, ctx, wl, nlayers):
+        self._ctx = ctx
+        self.W, self.K, self.n = wl.W, wl.K, wl.n
+        self.window, self.oversamp, self.bands = wl.window, wl.oversamp, wl.bands
+        self.maxband, self.spin = wl.maxband, wl.spin
+        self.nlayers = nlayers
+        self._nhops = len(wl.hops)
+
+    def close(self):
+        self._ctx = _vp()
+
+
+class Batch:
+    """Many independent instances (synth.Workload each) run as one captured
+    HIP graph: cqpi_batch_* of include/cqpi_amd.h."""
+
+    def __init__(self, workloads, precision="fp64"):
+        self.workloads = list(workloads)
+        self._keep = []
+        arr = (_Instance * len(self.workloads))()
+        for i, wl in enumerate(self.workloads):
+            hops = np.ascontiguousarray(wl.hops, dtype=synth.HOP_DTYPE)
+            V = np.ascontiguousarray(wl.V, dtype=np.complex128)
+            wf = np.ascontiguousarray(wl.tiles, dtype=np.float64)
+            om = np.ascontiguousarray(wl.energies, dtype=np.float64)
+            self._keep += [hops, V, wf, om]
+            arr[i] = _Instance(wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, int(wl.spin),
+                               int(wl.scmodel), hops.ctypes.data if len(hops) else None, len(hops),
+                               float(wl.efermi), _d(V.view(np.float64)), _d(wf), _d(om), len(om), float(wl.eta))
+        self._b = _vp()
+        _check(lib().cqpi_batch_create(arr, len(self.workloads), PRECISIONS.get(precision, precision),
+                                       C.byref(self._b)))
+        self._keep = []
+
+    def autotune(self, reps=3):
+        _check(lib().cqpi_batch_autotune(self._b, int(reps)))
+
+    def run(self):
+        """Launches the batch graph once (captured on the first call); returns
+        the device milliseconds of the launch."""
+        _check(lib().cqpi_batch_run(self._b))
+        return self.info()["last_ms"]
+
+    def info(self):
+        ms = np.zeros(1)
+        nodes, mem = _sz(), _sz()
+        _check(lib().cqpi_batch_info(self._b, _d(ms), C.byref(nodes), C.byref(mem)))
+        return {"last_ms": float(ms[0]), "graph_nodes": nodes.value, "gpumem": mem.value}
+
+    def instance(self, i) -> AmdQPI:
+        """Context of instance i (valid while the batch lives)."""
+        c = _vp()
+        _check(lib().cqpi_batch_context(self._b, i, C.byref(c)))
+        return _Borrowed(c, self.workloads[i], len(self.workloads[i].energies))
+
+    def maps(self, i):
+        wl = self.workloads[i]
+        return self.instance(i).retrieve_layers(0, len(wl.energies))
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            lib().cqpi_batch_destroy(self._b)
+            self._b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

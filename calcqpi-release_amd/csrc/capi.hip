@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -293,6 +294,105 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
     }
     stage_end(c, st);
     if (r) return fail(r, "contraction launch failed");
+    return 0;
+}
+
+// energies per chunk and mixed-precision plane scale of one cqpi_run
+struct RunPlan {
+    size_t E = 0, Ec = 0;
+    double eta = 0.0, sG = 1.0;
+    bool mixed = false;
+};
+
+// allocations and uploads of a run (synchronous); no kernel launch
+int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan &plan)
+{
+    const Geom &g = c->g;
+    const int NP = g.B * g.B;
+    size_t WW = (size_t)g.W * g.W;
+    RETCHK(dev_grow(c->d_ldos, c->c_ldos, E * WW, c->gpumem));
+    RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
+    RETCHK(dev_grow(c->d_T, c->c_T, E * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_status, c->c_status, E, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_omega, omegas, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // energies per chunk: intermediate of the two-pass transform kept under ~2 GiB
+    size_t per_e = (size_t)g.K * g.nw * NP * sizeof(double2);
+    size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
+    if (c->keep_windows) {
+        // one chunk: the window and T of every energy stay for cqpi_gradient
+        if (per_e * E > ((size_t)64 << 30))
+            return fail(CQPI_E_ARG, "keeping the windows of this many energies needs more than 64 GiB");
+        Ec = E;
+    }
+    c->windows_kept = false;
+    RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_bulk, c->c_bulk, Ec * g.os * g.os, c->gpumem));
+    // mixed precision: the k->x transform writes the fp16 planes directly.  Their
+    // scale comes from |G0(R)_ab| <= ||G0(k)|| <= 1/eta, a bound for every R.
+    plan.mixed = c->precision == CQPI_PREC_MIXED;
+    plan.sG = 1.0;
+    if (plan.mixed) {
+        RETCHK(mix_buffers(c, (int)Ec));
+        int ex;
+        std::frexp(1.0 / eta, &ex);
+        plan.sG = std::ldexp(1.0, 13 - ex);
+        std::vector<double> sv(Ec, plan.sG);
+        HIPCHK(hipMemcpyAsync(c->d_mscale, sv.data(), Ec * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->precision == CQPI_PREC_FP32) {
+        RETCHK(dev_grow(c->d_G0f, c->c_G0f, Ec * g.nw * g.nw * NP, c->gpumem));
+        RETCHK(dev_grow(c->d_Tf, c->c_Tf, Ec * NP, c->gpumem));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    plan.E = E;
+    plan.Ec = Ec;
+    plan.eta = eta;
+    c->last_omegas.assign(omegas, omegas + E);
+    c->last_eta = eta;
+    return 0;
+}
+
+// device work of a prepared run on c->stream: resolvent, transforms, T-matrix
+// and contraction of every energy chunk.  Launches only (no allocation, copy
+// or synchronisation), so it can be captured into a graph.
+int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
+{
+    const Geom &g = c->g;
+    const int NP = g.B * g.B;
+    const size_t WW = (size_t)g.W * g.W, E = plan.E, Ec = plan.Ec;
+    for (size_t e0 = 0; e0 < E; e0 += Ec) {
+        int ec = (int)std::min(Ec, E - e0);
+        int st = stage_begin(c, 1);
+        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, c->d_F1,
+                                   c->stream));
+        stage_end(c, st);
+        st = stage_begin(c, 2);
+        RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, plan.sG, c->d_G0w,
+                                   plan.mixed ? c->d_planes : nullptr, c->stream));
+        stage_end(c, st);
+        st = stage_begin(c, 3);
+        RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
+        stage_end(c, st);
+        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed));
+        c->last_chunk_e0 = e0;
+        c->last_chunk_ec = ec;
+    }
+    return 0;
+}
+
+// waits for the run and reports a singular 1 - V G0(0)
+int finish_run(cqpi_ctx *c, const RunPlan &plan)
+{
+    const size_t E = plan.E;
+    c->nlayers = E;
+    c->windows_kept = c->keep_windows;
+    std::vector<int> status(E);
+    HIPCHK(hipMemcpyAsync(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t e = 0; e < E; e++)
+        if (status[e])
+            return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
     return 0;
 }
 
@@ -607,8 +707,6 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     if (!c || (!omegas && E)) return fail(CQPI_E_ARG, "NULL argument");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     if (E == 0) return fail(CQPI_E_ARG, "no energies");
-    const Geom &g = c->g;
-    const int NP = g.B * g.B;
     HIPCHK(hipStreamSynchronize(c->stream));
     stage_collect(c);
     // the timing window covers this run, plus a cqpi_compute_eigen called just before it
@@ -620,65 +718,12 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
         c->in_run = false;
         if (r) return r;
     }
-    size_t WW = (size_t)g.W * g.W;
-    RETCHK(dev_grow(c->d_ldos, c->c_ldos, E * WW, c->gpumem));
-    RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
-    RETCHK(dev_grow(c->d_T, c->c_T, E * NP, c->gpumem));
-    RETCHK(dev_grow(c->d_status, c->c_status, E, c->gpumem));
-    HIPCHK(hipMemcpyAsync(c->d_omega, omegas, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    // energies per chunk: intermediate of the two-pass transform kept under ~2 GiB
-    size_t per_e = (size_t)g.K * g.nw * NP * sizeof(double2);
-    size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
-    if (c->keep_windows) {
-        // one chunk: the window and T of every energy stay for cqpi_gradient
-        if (per_e * E > ((size_t)64 << 30))
-            return fail(CQPI_E_ARG, "keeping the windows of this many energies needs more than 64 GiB");
-        Ec = E;
-    }
-    c->windows_kept = false;
-    RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
-    RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
-    // mixed precision: the k->x transform writes the fp16 planes directly.  Their
-    // scale comes from |G0(R)_ab| <= ||G0(k)|| <= 1/eta, a bound for every R.
-    const bool mixed = c->precision == CQPI_PREC_MIXED;
-    double sG = 1.0;
-    if (mixed) {
-        RETCHK(mix_buffers(c, (int)Ec));
-        int ex;
-        std::frexp(1.0 / eta, &ex);
-        sG = std::ldexp(1.0, 13 - ex);
-        std::vector<double> sv(Ec, sG);
-        HIPCHK(hipMemcpyAsync(c->d_mscale, sv.data(), Ec * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
-    for (size_t e0 = 0; e0 < E; e0 += Ec) {
-        int ec = (int)std::min(Ec, E - e0);
-        int st = stage_begin(c, 1);
-        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, eta, c->d_tw, c->d_F1,
-                                   c->stream));
-        stage_end(c, st);
-        st = stage_begin(c, 2);
-        RETCHK(launch_transform_kx(g, mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, sG, c->d_G0w,
-                                   mixed ? c->d_planes : nullptr, c->stream));
-        stage_end(c, st);
-        st = stage_begin(c, 3);
-        RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
-        stage_end(c, st);
-        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, mixed));
-        c->last_chunk_e0 = e0;
-        c->last_chunk_ec = ec;
-    }
-    c->nlayers = E;
-    c->windows_kept = c->keep_windows;
-    c->last_omegas.assign(omegas, omegas + E);
-    c->last_eta = eta;
-    std::vector<int> status(E);
-    HIPCHK(hipMemcpyAsync(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    RunPlan plan;
+    RETCHK(prepare_run(c, omegas, E, eta, plan));
+    RETCHK(enqueue_chunks(c, plan));
+    RETCHK(finish_run(c, plan));
     stage_collect(c);
-    for (size_t e = 0; e < E; e++)
-        if (status[e])
-            return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
+    const size_t WW = (size_t)c->g.W * c->g.W;
     if (ldos_host) HIPCHK(hipMemcpy(ldos_host, c->d_ldos, E * WW * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -843,5 +888,372 @@ int cqpi_retrieve_tmatrix(cqpi_ctx *c, size_t e, double *out)
 }
 
 void cqpi_free(void *p) { free(p); }
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* Autotuning                                                                */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+
+// best launch tuning per problem shape, shared by every context of the process
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune;
+
+std::string tune_key(const cqpi_ctx *c, size_t E)
+{
+    const Geom &g = c->g;
+    char k[160];
+    snprintf(k, sizeof k, "K%d n%d w%d os%d b%d m%d s%d p%d E%zu", g.K, g.n, g.w, g.os, g.B, g.maxb, g.spin,
+             c->precision, E);
+    return k;
+}
+
+// device time of one prepared run (eigen data cached) with the context's tuning
+int time_run(cqpi_ctx *c, const RunPlan &plan, int reps, float &ms)
+{
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int r = enqueue_chunks(c, plan);  // warm-up
+    if (!r && hipEventRecord(a, c->stream) != hipSuccess) r = CQPI_E_HIP;
+    for (int i = 0; i < reps && !r; i++) r = enqueue_chunks(c, plan);
+    if (!r && hipEventRecord(b, c->stream) != hipSuccess) r = CQPI_E_HIP;
+    if (!r && hipEventSynchronize(b) != hipSuccess) r = CQPI_E_HIP;
+    if (!r && hipEventElapsedTime(&ms, a, b) != hipSuccess) r = CQPI_E_HIP;
+    ms /= std::max(reps, 1);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return r ? fail(r, "autotune timing failed") : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqpi_set_tuning(cqpi_ctx *c, int tpw, int eb)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (tpw != 0 && tpw != 2 && tpw != 4 && tpw != 8) return fail(CQPI_E_ARG, "tpw must be 0, 2, 4 or 8");
+    if (eb < 0) return fail(CQPI_E_ARG, "eb must be >= 0");
+    c->g.tpw = tpw;
+    c->g.eb = eb;
+    return 0;
+}
+
+int cqpi_get_tuning(cqpi_ctx *c, int *tpw, int *eb)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (tpw) *tpw = c->g.tpw;
+    if (eb) *eb = c->g.eb;
+    return 0;
+}
+
+int cqpi_autotune(cqpi_ctx *c, const double *omegas, size_t E, double eta, int reps, int *tpw, int *eb)
+{
+    g_err.clear();
+    if (!c || !omegas || !E) return fail(CQPI_E_ARG, "NULL argument or no energies");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    const std::string key = tune_key(c, E);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            c->g.tpw = it->second.first;
+            c->g.eb = it->second.second;
+            if (tpw) *tpw = c->g.tpw;
+            if (eb) *eb = c->g.eb;
+            return 0;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_collect(c);
+    const bool timing = c->timing;
+    c->timing = false;
+    int r = 0;
+    if (!c->eigen_valid) {
+        c->in_run = true;
+        r = cqpi_compute_eigen(c);
+        c->in_run = false;
+    }
+    RunPlan plan;
+    if (!r) r = prepare_run(c, omegas, E, eta, plan);
+    // energies per workgroup of the first transform pass, then (mixed) cell
+    // rows per wave of the contraction; each knob is timed with the other fixed
+    int best_eb = 0, best_tpw = 0;
+    float best = 1e30f;
+    const int ebs[] = {0, (int)plan.Ec, (int)((plan.Ec + 1) / 2), (int)((plan.Ec + 3) / 4), (int)((plan.Ec + 7) / 8)};
+    for (int cand : ebs) {
+        if (r) break;
+        c->g.eb = cand;
+        c->g.tpw = 0;
+        float ms = 0.f;
+        r = time_run(c, plan, reps, ms);
+        if (!r && ms < best) {
+            best = ms;
+            best_eb = cand;
+        }
+    }
+    if (!r && plan.mixed) {
+        const int tpws[] = {4, 2};
+        for (int cand : tpws) {
+            c->g.eb = best_eb;
+            c->g.tpw = cand;
+            float ms = 0.f;
+            r = time_run(c, plan, reps, ms);
+            if (r) break;
+            if (ms < best) {
+                best = ms;
+                best_tpw = cand;
+            }
+        }
+    }
+    c->timing = timing;
+    c->g.eb = best_eb;
+    c->g.tpw = best_tpw;
+    if (!r) r = finish_run(c, plan);
+    if (r) return r;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = {best_tpw, best_eb};
+    }
+    if (tpw) *tpw = best_tpw;
+    if (eb) *eb = best_eb;
+    return 0;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent instances in one captured graph               */
+/* ------------------------------------------------------------------------ */
+
+struct cqpi_batch {
+    std::vector<cqpi_ctx *> inst;
+    std::vector<std::vector<double>> omegas;
+    std::vector<double> eta;
+    std::vector<RunPlan> plans;
+    std::vector<int> branch_of;
+    hipStream_t main = nullptr;
+    std::vector<hipStream_t> branches;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t nodes = 0;
+    float last_ms = 0.f;
+};
+
+namespace {
+
+constexpr int kBatchBranches = 4;
+
+void batch_release(cqpi_batch *b)
+{
+    if (!b) return;
+    if (b->main) hipStreamSynchronize(b->main);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (hipStream_t s : b->branches)
+        if (s) hipStreamDestroy(s);
+    for (hipEvent_t e : b->joins)
+        if (e) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->t0) hipEventDestroy(b->t0);
+    if (b->t1) hipEventDestroy(b->t1);
+    if (b->main) hipStreamDestroy(b->main);
+    for (cqpi_ctx *c : b->inst) release(c);
+    delete b;
+}
+
+// relative device work of an instance: transforms ~ E K^2 b^2 log K, contraction ~ E W^2 (2w+1)^2 b^2
+double batch_cost(const cqpi_ctx *c, size_t E)
+{
+    const Geom &g = c->g;
+    double side = 2.0 * g.w + 1.0;
+    return (double)E * ((double)g.K * g.K * g.B * g.B * std::log2((double)g.K + 1.0) +
+                        (double)g.W * g.W * side * side * g.B * g.B);
+}
+
+// eager run of every instance (allocates its buffers, validates it), then
+// capture of all their launches into one graph: instances spread over
+// kBatchBranches parallel branches, heaviest first onto the least loaded one
+int batch_capture(cqpi_batch *b)
+{
+    const size_t n = b->inst.size();
+    b->plans.assign(n, RunPlan());
+    b->branch_of.assign(n, 0);
+    for (size_t i = 0; i < n; i++) {
+        cqpi_ctx *c = b->inst[i];
+        RETCHK(cqpi_run(c, b->omegas[i].data(), b->omegas[i].size(), b->eta[i], nullptr));
+        RETCHK(prepare_run(c, b->omegas[i].data(), b->omegas[i].size(), b->eta[i], b->plans[i]));
+    }
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::vector<double> cost(n);
+    for (size_t i = 0; i < n; i++) cost[i] = batch_cost(b->inst[i], b->omegas[i].size());
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    const int nb = (int)b->branches.size();
+    std::vector<double> load(nb, 0.0);
+    HIPCHK(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+    int r = 0;
+    if (hipEventRecord(b->fork, b->main) != hipSuccess) r = CQPI_E_HIP;
+    for (int k = 0; k < nb && !r; k++)
+        if (hipStreamWaitEvent(b->branches[k], b->fork, 0) != hipSuccess) r = CQPI_E_HIP;
+    for (size_t oi = 0; oi < n && !r; oi++) {
+        const size_t i = order[oi];
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += cost[i];
+        b->branch_of[i] = k;
+        cqpi_ctx *c = b->inst[i];
+        hipStream_t own = c->stream;
+        const bool timing = c->timing;
+        c->timing = false;
+        c->stream = b->branches[k];
+        const Geom &g = c->g;
+        r = launch_eigen(g.B, c->d_groups, c->ngroups, c->d_hops, c->efermi, c->scmax, g.K, c->d_eval, c->d_evec,
+                         c->stream);
+        if (!r) r = enqueue_chunks(c, b->plans[i]);
+        c->stream = own;
+        c->timing = timing;
+    }
+    for (int k = 0; k < nb && !r; k++) {
+        if (hipEventRecord(b->joins[k], b->branches[k]) != hipSuccess ||
+            hipStreamWaitEvent(b->main, b->joins[k], 0) != hipSuccess)
+            r = CQPI_E_HIP;
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->main, &graph);
+    if (r || ec != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(r ? r : CQPI_E_HIP, "batch graph capture failed");
+    }
+    b->graph = graph;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    size_t nodes = 0;
+    HIPCHK(hipGraphGetNodes(b->graph, nullptr, &nodes));
+    b->nodes = nodes;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqpi_batch_create(const cqpi_instance *in, size_t count, int precision, cqpi_batch **out)
+{
+    g_err.clear();
+    if (!out || (!in && count)) return fail(CQPI_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (!count) return fail(CQPI_E_ARG, "empty batch");
+    cqpi_batch *b = new cqpi_batch();
+    int r = 0;
+    for (size_t i = 0; i < count && !r; i++) {
+        const cqpi_instance &d = in[i];
+        if (!d.omegas || !d.E || !d.V) {
+            r = fail(CQPI_E_ARG, "instance " + std::to_string(i) + ": omegas, E and V are required");
+            break;
+        }
+        cqpi_ctx *c = nullptr;
+        r = cqpi_create(d.oversamp * d.n, d.kpoints, d.n, d.window, d.oversamp, d.bands, d.maxband, d.spin, d.wf,
+                        &c);
+        if (r) break;
+        b->inst.push_back(c);
+        r = cqpi_set_precision(c, precision);
+        if (!r) r = cqpi_set_model(c, d.hops, d.nhops, d.efermi, d.scmodel);
+        if (!r) r = cqpi_set_impurity(c, d.V);
+        if (!r) {
+            b->omegas.emplace_back(d.omegas, d.omegas + d.E);
+            b->eta.push_back(d.eta);
+        }
+    }
+    if (!r && hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) != hipSuccess) r = fail(CQPI_E_HIP, "stream");
+    for (int k = 0; k < kBatchBranches && !r; k++) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            r = fail(CQPI_E_HIP, "stream/event");
+        b->branches.push_back(s);
+        b->joins.push_back(e);
+    }
+    if (!r && (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess))
+        r = fail(CQPI_E_HIP, "event");
+    if (r) {
+        std::string msg = g_err;
+        batch_release(b);
+        g_err = msg;
+        return r;
+    }
+    *out = b;
+    return 0;
+}
+
+int cqpi_batch_autotune(cqpi_batch *b, int reps)
+{
+    g_err.clear();
+    if (!b) return fail(CQPI_E_ARG, "batch is NULL");
+    if (b->exec) return fail(CQPI_E_STATE, "the batch graph is already captured");
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        int tpw = 0, eb = 0;
+        RETCHK(cqpi_autotune(b->inst[i], b->omegas[i].data(), b->omegas[i].size(), b->eta[i], reps, &tpw, &eb));
+    }
+    return 0;
+}
+
+int cqpi_batch_run(cqpi_batch *b)
+{
+    g_err.clear();
+    if (!b) return fail(CQPI_E_ARG, "batch is NULL");
+    if (!b->exec) RETCHK(batch_capture(b));
+    HIPCHK(hipEventRecord(b->t0, b->main));
+    HIPCHK(hipGraphLaunch(b->exec, b->main));
+    HIPCHK(hipEventRecord(b->t1, b->main));
+    HIPCHK(hipStreamSynchronize(b->main));
+    HIPCHK(hipEventElapsedTime(&b->last_ms, b->t0, b->t1));
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        cqpi_ctx *c = b->inst[i];
+        const size_t E = b->omegas[i].size();
+        std::vector<int> status(E);
+        HIPCHK(hipMemcpy(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < E; e++)
+            if (status[e])
+                return fail(CQPI_E_SINGULAR, "instance " + std::to_string(i) + ": 1 - V G0(0) is singular at energy index " +
+                                                 std::to_string(e));
+        c->nlayers = E;
+    }
+    return 0;
+}
+
+int cqpi_batch_size(cqpi_batch *b, size_t *count)
+{
+    if (!b || !count) return fail(CQPI_E_ARG, "NULL argument");
+    *count = b->inst.size();
+    return 0;
+}
+
+int cqpi_batch_context(cqpi_batch *b, size_t i, cqpi_ctx **ctx)
+{
+    if (!b || !ctx || i >= b->inst.size()) return fail(CQPI_E_ARG, "bad batch index");
+    *ctx = b->inst[i];
+    return 0;
+}
+
+int cqpi_batch_info(cqpi_batch *b, double *last_ms, size_t *graph_nodes, size_t *gpumem)
+{
+    if (!b) return fail(CQPI_E_ARG, "batch is NULL");
+    if (last_ms) *last_ms = b->last_ms;
+    if (graph_nodes) *graph_nodes = b->nodes;
+    if (gpumem) {
+        size_t m = 0;
+        for (cqpi_ctx *c : b->inst) m += c->gpumem;
+        *gpumem = m;
+    }
+    return 0;
+}
+
+void cqpi_batch_destroy(cqpi_batch *b) { batch_release(b); }
 
 }  // extern "C"

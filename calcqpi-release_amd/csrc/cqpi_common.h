@@ -56,6 +56,9 @@ struct Geom {
     int nw;       // 2h+1
     int T;        // tile side (2w+1)*os
     int W;        // map side os*n
+    // launch tuning (0: built-in heuristic), chosen by the autotuner
+    int tpw;      // mixed contraction: cell rows per wave (2, 4, 8)
+    int eb;       // first transform pass: energies per workgroup
 };
 
 struct MixGeom;

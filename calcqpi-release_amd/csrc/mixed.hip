@@ -404,16 +404,27 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     }
 }
 
-constexpr int kTPW = 8;
+constexpr int kTPW = 8;  // default cell rows per wave
+
+template <int NT, int LDW, int OS, int KS, int TPW>
+void launch_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
+                 double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * TPW - 1) / (kWaves * TPW), E);
+    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A, scale, sA,
+                       T, bulk, ldos);
+}
 
 template <int NT, int LDW, int OS, int KS>
 bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
               double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
     if (g.os != OS || mg.KS != KS || mg.NT != NT || mg.nsets != 1 || mg.LDW != LDW) return false;
-    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * kTPW - 1) / (kWaves * kTPW), E);
-    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, kTPW>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A,
-                       scale, sA, T, bulk, ldos);
+    switch (g.tpw) {
+    case 2: launch_areg<NT, LDW, OS, KS, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 4: launch_areg<NT, LDW, OS, KS, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    default: launch_areg<NT, LDW, OS, KS, kTPW>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    }
     return true;
 }
 

@@ -403,7 +403,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     // energies per workgroup: enough workgroups for ~2 per CU
     const int rows = N * nch;
-    const int EB = std::max(1, std::min(E, (rows * E) / 512));
+    const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 512));
     const int nb = (E + EB - 1) / EB;
     const size_t lds = sizeof(double2) * (size_t)np * N;
     hipLaunchKernelGGL((k_res_ky_s<B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB, eta,
@@ -446,7 +446,7 @@ int launch_res_t(const Geom &g, const double *eval, const double2 *evec, const d
     int nch = (NP + np - 1) / np;
     // energies per workgroup: enough workgroups to fill the chip (>= 4 per CU)
     int rows = g.K * nch;
-    int EB = std::max(1, std::min(E, (rows * E) / 1024));
+    int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 1024));
     int nb = (E + EB - 1) / EB;
     size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
     dim3 grid(g.K, nb, nch), blk(kThreads);

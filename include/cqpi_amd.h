@@ -191,6 +191,66 @@ int cqpi_gradient(cqpi_ctx *ctx, const double *rhobar, double *g_hops, double *g
 int cqpi_gradient_device(cqpi_ctx *ctx, const double *d_rhobar, double *g_hops, double *g_mu, double *g_V);
 
 /* ---------------------------------------------------------------------- */
+/* Launch tuning (extension; the reference has fixed launch shapes)         */
+/* ---------------------------------------------------------------------- */
+
+/* Sets the launch tuning of later runs: tpw = cell rows per wave of the
+ * mixed-precision contraction (0 = default 8, or 2, 4, 8); eb = energies per
+ * workgroup of the first transform pass (0 = built-in heuristic).  Results
+ * do not depend on either. */
+int cqpi_set_tuning(cqpi_ctx *ctx, int tpw, int eb);
+int cqpi_get_tuning(cqpi_ctx *ctx, int *tpw, int *eb);
+
+/* Autotuner: times candidate tunings on the device for this context's shape
+ * and E energies (reps timed runs each, eigen data cached), keeps the fastest
+ * and remembers it for the process (contexts of the same shape then reuse it
+ * without timing).  The maps of the last timed run are left in the context.
+ * *tpw and *eb (may be NULL) receive the choice. */
+int cqpi_autotune(cqpi_ctx *ctx, const double *omegas, size_t E, double eta, int reps, int *tpw, int *eb);
+
+/* ---------------------------------------------------------------------- */
+/* Batches of independent instances in one graph (extension)                */
+/* ---------------------------------------------------------------------- */
+
+/* One instance of a batch: geometry as in cqpi_create (wanniern is
+ * oversamp*n), model as in cqpi_set_model, V as in cqpi_set_impurity, and its
+ * own energies.  The pointers are read during cqpi_batch_create only. */
+typedef struct {
+    size_t kpoints, n, window, oversamp, bands, maxband;
+    int spin, scmodel;
+    const cqpi_hopping *hops;
+    size_t nhops;
+    double efermi;
+    const double *V;      /* b x b complex */
+    const double *wf;     /* maxband x T x T tiles */
+    const double *omegas; /* E energies */
+    size_t E;
+    double eta;
+} cqpi_instance;
+
+typedef struct cqpi_batch cqpi_batch;
+
+/* Creates a context per instance (all with the given precision). */
+int cqpi_batch_create(const cqpi_instance *inst, size_t count, int precision, cqpi_batch **out);
+/* Runs cqpi_autotune for every instance (shapes seen before reuse their
+ * tuning).  Only before the first cqpi_batch_run. */
+int cqpi_batch_autotune(cqpi_batch *b, int reps);
+/* Whole batch on the device.  The first call runs every instance once
+ * (allocation, validation) and captures the launches of all instances -- per
+ * instance the eigensolve, resolvent, both transforms, T-matrix and
+ * contraction -- into one HIP graph with parallel branches; every call then
+ * launches that graph once and waits for it.  CQPI_E_SINGULAR names the first
+ * instance and energy with a singular 1 - V G0(0). */
+int cqpi_batch_run(cqpi_batch *b);
+int cqpi_batch_size(cqpi_batch *b, size_t *count);
+/* Context of instance i (owned by the batch): its maps are read with
+ * cqpi_retrieve_layers / cqpi_device_ldos after cqpi_batch_run. */
+int cqpi_batch_context(cqpi_batch *b, size_t i, cqpi_ctx **ctx);
+/* Device time of the last graph launch, graph node count, device memory. */
+int cqpi_batch_info(cqpi_batch *b, double *last_ms, size_t *graph_nodes, size_t *gpumem);
+void cqpi_batch_destroy(cqpi_batch *b);
+
+/* ---------------------------------------------------------------------- */
 /* Input formats (host)                                                     */
 /* ---------------------------------------------------------------------- */
 
