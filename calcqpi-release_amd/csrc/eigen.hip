@@ -73,12 +73,14 @@ __device__ inline bool converged(const double2 *A, int b)
 
 // Cyclic-by-row Jacobi.  Small B: everything unrolled so A and V stay in
 // registers.  Larger B: rolled loops (A and V live in private memory).
+constexpr int kUnrollB = 5;
+
 template <int B>
 __device__ void jacobi(double2 (&A)[B * B], double2 (&V)[B * B])
 {
     for (int i = 0; i < B * B; i++) V[i] = make_double2((i % (B + 1)) == 0 ? 1.0 : 0.0, 0.0);
     for (int i = 0; i < B; i++) A[i * B + i].y = 0.0;
-    if constexpr (B <= 4) {
+    if constexpr (B <= kUnrollB) {
         for (int sweep = 0; sweep < 60; sweep++) {
             bool done;
             {
@@ -134,7 +136,7 @@ __global__ void __launch_bounds__(128) k_eigen(const DevRGroup *__restrict__ gro
             DevHop hp = hops[m];
             double2 e = make_double2(cs, hp.anti ? -sn : sn);
             double2 v = cmul(make_double2(hp.tre, hp.tim), e);
-            if constexpr (B <= 4) {
+            if constexpr (B <= kUnrollB) {
                 // runtime index into a register array: select by compare to keep it in VGPRs
 #pragma unroll
                 for (int r = 0; r < B * B; r++)

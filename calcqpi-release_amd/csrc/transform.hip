@@ -245,12 +245,14 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
 // Stockham path (power-of-two K = N, 64 <= N <= 1024)
 // ---------------------------------------------------------------------------
 constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
+constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in registers
+constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
 
 // pass 1: block = lane groups of stockham_lanes(N) lanes, one line (orbital
-// pair of the chunk) per group.  B > 0: the eigen data of the thread's k-point
-// (N <= blockDim) stays in registers for the whole energy batch; B == 0: band
-// count g.B, eigen data read per energy.
+// pair of the chunk) per group.  0 < B <= kMaxRegB: the eigen data of the
+// thread's k-point (N <= blockDim) stays in registers for the whole energy
+// batch; otherwise (B == 0: band count g.B) it is read per energy.
 template <int B, int N>
 __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const double *__restrict__ eval,
                                                    const double2 *__restrict__ evec,
@@ -259,7 +261,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 {
     extern __shared__ double2 S[];
     constexpr int LN = cqpi::stockham_lanes(N);
-    constexpr bool kRegs = B > 0 && B <= 4;
+    constexpr bool kRegs = B > 0 && B <= kMaxRegB;
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
     const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
     const int e0 = blockIdx.y * EB, e1 = min(E, e0 + EB);
@@ -306,17 +308,24 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
             }
         }
         if (!regs) {
+            // one resolvent factor per band, accumulated into the chunk's pairs
             for (int j = threadIdx.x; j < N; j += blockDim.x) {
                 const long k = (long)i * N + j;
-                for (int q = 0; q < npc; q++) {
-                    const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
-                    double2 acc = make_double2(0.0, 0.0);
-                    for (int n = 0; n < nb; n++) {
-                        const double2 f = cinv(make_double2(omega - eval[n * KK + k], eta));
-                        acc = cadd(acc, cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
+                double2 acc[kMaxLines];
+#pragma unroll
+                for (int q = 0; q < kMaxLines; q++) acc[q] = make_double2(0.0, 0.0);
+                for (int n = 0; n < nb; n++) {
+                    const double2 f = cinv(make_double2(omega - eval[n * KK + k], eta));
+#pragma unroll
+                    for (int q = 0; q < kMaxLines; q++) {
+                        if (q >= npc) break;
+                        const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
+                        acc[q] = cadd(acc[q], cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
                     }
-                    S[(size_t)q * N + j] = acc;
                 }
+#pragma unroll
+                for (int q = 0; q < kMaxLines; q++)
+                    if (q < npc) S[(size_t)q * N + j] = acc[q];
             }
         }
         __syncthreads();
@@ -381,7 +390,7 @@ constexpr size_t kWaveLds = 80 * 1024;
 int wave_pairs(int N, int NP)
 {
     const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
-    const int lines = std::max(1, std::min(groups, (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
+    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -420,6 +429,7 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
     case 2: return launch_res_s<2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     case 3: return launch_res_s<3, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     case 4: return launch_res_s<4, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 5: return launch_res_s<5, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     default: return launch_res_s<0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     }
 }
