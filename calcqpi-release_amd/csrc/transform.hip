@@ -261,20 +261,19 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 {
     extern __shared__ double2 S[];
     constexpr int LN = cqpi::stockham_lanes(N);
-    constexpr bool kRegs = B > 0 && B <= kMaxRegB;
+    constexpr bool kRegs = B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads;  // blockDim >= N (launcher)
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
     const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
     const int e0 = blockIdx.y * EB, e1 = min(E, e0 + EB);
     const long KK = (long)N * N;
     const int nw = g.nw, h = g.h;
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
-    const bool regs = kRegs && N <= (int)blockDim.x;
 
     double ev_r[kRegs ? B : 1];
     double2 vv_r[kRegs ? B * B : 1];
     if constexpr (kRegs) {
         const int j = threadIdx.x;
-        if (regs && j < N) {
+        if (j < N) {
             const long k = (long)i * N + j;
 #pragma unroll
             for (int n = 0; n < B; n++) ev_r[n] = eval[n * KK + k];
@@ -286,7 +285,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
         const double omega = omegas[e];
         if constexpr (kRegs) {
             const int j = threadIdx.x;
-            if (regs && j < N) {
+            if (j < N) {
                 double2 f[B];
 #pragma unroll
                 for (int n = 0; n < B; n++) f[n] = cinv(make_double2(omega - ev_r[n], eta));
@@ -307,7 +306,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
                 }
             }
         }
-        if (!regs) {
+        if constexpr (!kRegs) {
             // one resolvent factor per band, accumulated into the chunk's pairs
             for (int j = threadIdx.x; j < N; j += blockDim.x) {
                 const long k = (long)i * N + j;
@@ -318,9 +317,11 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
                     const double2 f = cinv(make_double2(omega - eval[n * KK + k], eta));
 #pragma unroll
                     for (int q = 0; q < kMaxLines; q++) {
-                        if (q >= npc) break;
-                        const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
-                        acc[q] = cadd(acc[q], cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
+                        if (q < npc) {
+                            const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
+                            acc[q] = cadd(acc[q],
+                                          cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
+                        }
                     }
                 }
 #pragma unroll
@@ -424,6 +425,7 @@ template <int N>
 int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                   const double2 *tw, double2 *F1, hipStream_t s)
 {
+    if (N > kWaveMaxThreads) return launch_res_s<0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     switch (g.B) {
     case 1: return launch_res_s<1, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     case 2: return launch_res_s<2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
