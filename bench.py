@@ -1,4 +1,4 @@
-"""Continuum-LDOS throughput on one MI355X: mixed precision vs FP64.
+"""Continuum-LDOS throughput on one MI355X: mixed precision vs FP32 vs FP64.
 
 A step is one whole pass of the device chain over the workload: per-k
 eigensolve of H(k), then for every energy the resolvent, the k->R transform,
@@ -9,11 +9,16 @@ fixed seed.
 
 `value` is the mixed-precision path (split-fp16 MFMA contraction, FP64
 eigensolve / resolvent / transform / T-matrix / periodic term).  The same
-line reports the FP64 path, their ratio, the accuracy of both against each
-other and against the FP64 CPU oracle, the roofline of the dominant kernel
-and the CPU baseline.  Prints one JSON line.
+line reports the FP32 and FP64 paths, their ratios, the accuracy against the
+FP64 path and the FP64 CPU oracle, the roofline of the dominant kernel, the
+batched workload (128 uneven instances in one HIP graph), the gradient
+workload and the CPU baseline.  Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
+                  [--workload NAME] [--energies N]
+  python bench.py --probe        one mixed and one FP64 step after a 256 MiB
+                                 device copy (the traffic calibration), for
+                                 rocprofv3 --pmc runs
 """
 import argparse
 import json
@@ -28,7 +33,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "LDOS pixel-energy points/s, mixed vs FP64 on 1 GPU at <=1e-4 rel err; grad time"
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}   # FP64 vector/matrix; dense fp16 MFMA
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "mixed": 2500.0}   # FP64, FP32 vector; dense fp16 MFMA
 
 
 def flops_per_point(wl):
@@ -145,6 +150,27 @@ def gradient_timing(cqpi, synth, torch, reps=3):
                     "adjoint; maps and dL/drho stay in HBM"}
 
 
+TRAFFIC_MD = os.path.join(ROOT, "profiles", "pmc_traffic_square3orb_K512_E51.md")
+
+
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` (name prefix) in the `precision` step,
+    from the committed rocprofv3 --pmc summary of this workload (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        txt = open(TRAFFIC_MD).read()
+    except OSError:
+        return None
+    if "```json" not in txt:
+        return None
+    table = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+    for name, row in table.get("kernels", {}).items():
+        label = name.rsplit(" (", 1)[1].rstrip(")") if " (" in name else None
+        if name.startswith(kernel) and label in (None, precision):
+            return row.get("bytes_per_launch")
+    return None
+
+
 def roofline(wl, q, st, precision):
     names = list(st)
     dom = max(range(len(names)), key=lambda i: st[names[i]][0])
@@ -152,23 +178,83 @@ def roofline(wl, q, st, precision):
     avg_s = ms / max(nl, 1) / 1e3
     E = len(wl.energies)
     nw = 2 * q.window_half() + 1
+    kern = {1: "k_res_ky_s", 2: "k_kx_s", 4: "k_cmix_areg" if precision == "mixed" else "k_contract"}.get(dom, "")
     if dom == 4:
         per_launch = wl.W * wl.W * E / max(nl, 1) * flops_per_point(wl)
         achieved = per_launch / avg_s / 1e12
         peak = PEAK_TFLOPS[precision]
         roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kern, precision)}
     elif dom in (1, 2):
         per_launch = stage_bytes(wl, nw, E)[dom] / max(nl, 1)
         achieved = per_launch / avg_s / 1e9
         roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern, precision),
+                "algorithmic_bytes_per_launch": per_launch}
     else:
         roof = {"kernel": names[dom], "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": None, "traffic": None}
     roof["avg_launch_ms"] = avg_s * 1e3
     roof["launches_per_step"] = nl
+    roof["timing"] = "HIP events on the library stream around the launch, inside the timed run"
     return roof
+
+
+def batch_timing(cqpi, synth, torch, count=128, reps=5):
+    """Batched workload: `count` independent instances with uneven shapes
+    (orbitals {1,3,5}, k-grids {128,256,512}, 16-64 energies, maps
+    {32,64,128} cells), autotuned, all run as one captured HIP graph.  Error:
+    every map of every instance, mixed vs the FP64 batch, on the device."""
+    wls = synth.batch_uneven(count)
+    pts = sum(wl.W * wl.W * len(wl.energies) for wl in wls)
+    out = {}
+    maps = {}
+    for prec in ("mixed", "fp64"):
+        b = cqpi.Batch(wls, prec)
+        t0 = time.perf_counter()
+        if prec == "mixed":
+            b.autotune(reps=2)
+        t_tune = time.perf_counter() - t0
+        b.run()  # eager pass, capture, first launch
+        ms = min(b.run() for _ in range(reps))
+        info = b.info()
+        out[prec] = {"graph_ms": ms, "points_per_s": pts / (ms / 1e3), "graph_nodes": info["graph_nodes"],
+                     "device_memory_GB": info["gpumem"] / 1e9}
+        if prec == "mixed":
+            out[prec]["autotune_s"] = t_tune
+        maps[prec] = b
+    worst = 0.0
+    for i in range(count):
+        a = maps["mixed"].instance(i).ldos_tensor()
+        r = maps["fp64"].instance(i).ldos_tensor()
+        m = torch.amax(torch.abs(r), dim=1, keepdim=True)
+        den = torch.maximum(torch.abs(r), 1e-3 * m)
+        worst = max(worst, float(torch.max(torch.abs(a - r) / den)))
+    for b in maps.values():
+        b.close()
+    return {"workload": "%d instances, seed 1004: orbitals {1,3,5}, k-grid {128,256,512}^2, 16-64 energies "
+                        "in [-0.3,0.3], maps {32,64,128}^2 cells, w=2, os=4, V = U(0.1,2) x Hermitian unit" % count,
+            "points": pts, "mixed": out["mixed"], "fp64": out["fp64"],
+            "speedup_mixed_vs_fp64": out["fp64"]["graph_ms"] / out["mixed"]["graph_ms"],
+            "max_rel_mixed_vs_fp64": worst,
+            "note": "one hipGraphLaunch per run (graph captured once; 4 parallel branches); tiles autotuned "
+                    "per shape before capture"}
+
+
+def probe(cqpi, synth, torch, workload):
+    """One mixed and one FP64 step after a 256 MiB device copy: the run that
+    the rocprofv3 --pmc traffic passes profile."""
+    src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+    dst = src.clone()
+    torch.cuda.synchronize()
+    del src, dst
+    wl = synth.workload(workload)
+    for prec in ("mixed", "fp64"):
+        with cqpi.engine_for(wl, prec) as q:
+            q.compute_eigen()
+            q.run(wl.energies, wl.eta, host=False)
+    torch.cuda.synchronize()
+    print("probe done: copy 256 MiB, %s one mixed and one fp64 step" % workload, flush=True)
 
 
 def main():
@@ -177,7 +263,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="square_3orb")
+    ap.add_argument("--energies", type=int, default=0, help="use only the first N energies of the workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--probe", action="store_true")
     ap.add_argument("--cpu-energies", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,22 +282,35 @@ def main():
 
     if cqpi.device_count() < 1:
         raise SystemExit("no HIP device")
+    if args.probe:
+        probe(cqpi, synth, torch, args.workload)
+        return
     wl = synth.workload(args.workload)
+    if args.energies:
+        wl.energies = wl.energies[:args.energies]
     E = len(wl.energies)
 
-    qm = cqpi.engine_for(wl, "mixed")
-    dt_m, st_m = time_steps(qm, wl, args.steps, args.warmup, torch)
-    q6 = cqpi.engine_for(wl, "fp64")
-    dt_6, st_6 = time_steps(q6, wl, args.steps, args.warmup, torch)
+    timed = {}
+    for prec in ("mixed", "fp32", "fp64"):
+        q = cqpi.engine_for(wl, prec)
+        dt, st = time_steps(q, wl, args.steps, args.warmup, torch)
+        timed[prec] = (q, dt, st)
+    qm, dt_m, st_m = timed["mixed"]
+    q6, dt_6, st_6 = timed["fp64"]
+    q3, dt_3, st_3 = timed["fp32"]
 
-    # accuracy: every energy of the mixed path against the FP64 device path
-    rm_ = qm.run(wl.energies, wl.eta, host=True)
+    # accuracy: every energy of the reduced-precision paths against the FP64 device path
     r6 = q6.run(wl.energies, wl.eta, host=True)
+    rm_ = qm.run(wl.energies, wl.eta, host=True)
+    r3 = q3.run(wl.energies, wl.eta, host=True)
     errs = [rel_errors(rm_[e], r6[e]) for e in range(E)]
+    errs3 = [rel_errors(r3[e], r6[e]) for e in range(E)]
     acc = {"mixed_vs_fp64_gpu_max_rel": max(e[0] for e in errs),
            "mixed_vs_fp64_gpu_rms_rel": max(e[1] for e in errs),
+           "fp32_vs_fp64_gpu_max_rel": max(e[0] for e in errs3),
            "floor": "max|d| / max(|ref|, 1e-3 max|ref|) per energy, max over energies"}
 
+    default = args.workload == "square_3orb" and not args.energies
     out = {
         "metric": METRIC,
         "value": wl.points / dt_m,
@@ -220,23 +323,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "mixed: fp16x2 split MFMA contraction (f32 accumulate), f64 elsewhere",
-        "data": "synthetic (seeded 3-orbital model, Gaussian x {xz,yz,xy} tiles)",
+        "data": "synthetic (seeded model, Gaussian x angular-factor tiles)",
         "config": {"workload": "square lattice 3 orbitals, K=512, 51 energies in [-0.1,0.1], "
                                "eta=0.002, 128x128 cells, os=4, w=2 (512x512 pixels)"
-                   if args.workload == "square_3orb" else "%s: %s" % (args.workload, wl.name),
+                   if default else "%s: %s, %d energies" % (args.workload, wl.name, E),
                    "kpoints": wl.K, "energies": E, "cells": wl.n, "oversamp": wl.oversamp,
                    "window": wl.window, "bands": wl.bands},
-        "fp64": {"value": wl.points / dt_6, "ms_per_step": dt_6 * 1e3,
+        "fp64": {"value": wl.W * wl.W * E / dt_6, "ms_per_step": dt_6 * 1e3,
                  "stages_ms": {k: v[0] for k, v in st_6.items()},
                  "roofline": roofline(wl, q6, st_6, "fp64")},
+        "fp32": {"value": wl.W * wl.W * E / dt_3, "ms_per_step": dt_3 * 1e3,
+                 "stages_ms": {k: v[0] for k, v in st_3.items()}},
         "speedup_mixed_vs_fp64": dt_6 / dt_m,
+        "speedup_fp32_vs_fp64": dt_6 / dt_3,
         "stages_ms": {k: v[0] for k, v in st_m.items()},
         "accuracy": acc,
         "roofline": roofline(wl, qm, st_m, "mixed"),
     }
-    qm.close()
-    q6.close()
-    out["gradient"] = gradient_timing(cqpi, synth, torch)
+    for q, _, _ in timed.values():
+        q.close()
+    if not args.no_batch:
+        out["batch"] = batch_timing(cqpi, synth, torch)
+    if not args.no_grad:
+        out["gradient"] = gradient_timing(cqpi, synth, torch)
     if not args.no_cpu:
         info, maps = cpu_baseline(wl, args.cpu_energies)
         out["cpu_baseline"] = info

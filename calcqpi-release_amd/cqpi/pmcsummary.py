@@ -1,0 +1,107 @@
+"""Summarise rocprofv3 --pmc runs of `bench.py --probe` into HBM traffic and
+MFMA figures per kernel (markdown with a JSON block that bench.py reads).
+
+    python -m cqpi.pmcsummary FETCH.csv WRITE.csv [MFMA.csv] > summary.md
+
+The probe first copies 256 MiB with a plain device copy (the runtime's
+copyBuffer kernel), then runs one mixed-precision and one FP64 step.
+FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by the
+factor that makes the copy read and write exactly 256 MiB.  Kernels with two
+dispatches are labelled by the probe's order (mixed first, then fp64).
+MFMA busy is SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / XCDs * SIMDs).
+"""
+from __future__ import annotations
+
+import csv
+import json
+import re
+import sys
+from collections import OrderedDict
+
+COPY_BYTES = 256 << 20
+XCDS, SIMDS = 8, 1024   # MI355X: 8 XCDs, 256 CUs x 4 SIMDs
+
+
+def short_name(name: str) -> str:
+    m = re.search(r"::(k_\w+(?:<[^>(]*>)?)\(", name)
+    if m:
+        return m.group(1)
+    m = re.search(r"_GLOBAL__N_1\d+(k_\w+?)(I.*?E)?E", name)
+    if m:
+        args = re.findall(r"Li(\d+)E", m.group(2) or "")
+        return m.group(1) + ("<%s>" % ", ".join(args) if args else "")
+    return name[:40]
+
+
+def read(path, counter):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                rows.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"]),
+                             int(r["Dispatch_Id"])))
+    rows.sort(key=lambda x: x[3])
+    return rows
+
+
+def per_kernel(rows):
+    """OrderedDict label -> value, labels 'name' or 'name (mixed|fp64)'."""
+    groups = OrderedDict()
+    for name, grid, val, _ in rows:
+        if "rocclr" in name or "at::native" in name:
+            continue
+        groups.setdefault(short_name(name), []).append(val)
+    out = OrderedDict()
+    for k, v in groups.items():
+        if len(v) == 2:
+            out[k + " (mixed)"] = v[0]
+            out[k + " (fp64)"] = v[1]
+        else:
+            out[k] = sum(v) / len(v)
+    return out
+
+
+def calibration(rows, expected_kib):
+    copies = [val for name, grid, val, _ in rows if "copyBuffer" in name]
+    big = max(copies) if copies else 0.0
+    return expected_kib / big if big else None
+
+
+def main(argv):
+    fetch = read(argv[0], "FETCH_SIZE")
+    write = read(argv[1], "WRITE_SIZE")
+    ff = calibration(fetch, COPY_BYTES / 1024)
+    fw = calibration(write, COPY_BYTES / 1024)
+    F, Wr = per_kernel(fetch), per_kernel(write)
+    mf = {}
+    if len(argv) > 2:
+        busy = per_kernel(read(argv[2], "SQ_VALU_MFMA_BUSY_CYCLES"))
+        act = per_kernel(read(argv[2], "GRBM_GUI_ACTIVE"))
+        mops = per_kernel(read(argv[2], "SQ_INSTS_VALU_MFMA_MOPS_F16"))
+        for k in busy:
+            if busy[k] > 0 and act.get(k):
+                mf[k] = {"mfma_busy": busy[k] / (act[k] / XCDS * SIMDS),
+                         "mfma_f16_flops": mops.get(k, 0.0) * 512}
+    table = OrderedDict()
+    for k in F:
+        fb = F[k] * 1024 * ff
+        wb = Wr.get(k, 0.0) * 1024 * fw
+        table[k] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb}
+        table[k].update(mf.get(k, {}))
+    print("| kernel | HBM read MB | HBM write MB | MFMA busy | MFMA f16 GFLOP |")
+    print("|---|---|---|---|---|")
+    for k, r in table.items():
+        print("| `%s` | %.1f | %.1f | %s | %s |" % (
+            k, r["fetch_bytes"] / 1e6, r["write_bytes"] / 1e6,
+            "%.1f%%" % (100 * r["mfma_busy"]) if "mfma_busy" in r else "",
+            "%.1f" % (r["mfma_f16_flops"] / 1e9) if "mfma_f16_flops" in r else ""))
+    print()
+    print("Copy calibration: FETCH_SIZE x %.3f, WRITE_SIZE x %.3f (256 MiB copy read and written)." % (ff, fw))
+    print()
+    print("```json")
+    print(json.dumps({"calibration": {"fetch": ff, "write": fw}, "kernels": table}, indent=1))
+    print("```")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -61,6 +61,26 @@ struct Geom {
     int eb;       // first transform pass: energies per workgroup
 };
 
+// Workgroup coordinates of a 3-D grid renumbered for the 8 XCDs of the chip.
+// The dispatcher hands consecutive workgroup ids to the XCDs round-robin, so
+// id L runs on XCD L % 8.  The renumbering v = (ids before XCD L % 8) + L / 8
+// is a bijection of [0, total) that gives each XCD a contiguous range of v;
+// decoding v with z slowest keeps the workgroups of one z slice (one energy)
+// on few XCDs, so that slice's data stays in those XCDs' L2.
+__device__ inline void xcd_block(int &bx, int &by, int &bz)
+{
+    const unsigned nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+    const unsigned total = nx * ny * nz;
+    const unsigned L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const unsigned x = L % 8u, q = total / 8u, r = total % 8u;
+    const unsigned v = x * q + (x < r ? x : r) + L / 8u;
+    const unsigned plane = nx * ny;
+    bz = (int)(v / plane);
+    const unsigned rem = v - (unsigned)bz * plane;
+    by = (int)(rem / nx);
+    bx = (int)(rem - (unsigned)by * nx);
+}
+
 struct MixGeom;
 
 // launchers (defined next to their kernels)

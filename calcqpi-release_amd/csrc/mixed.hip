@@ -134,13 +134,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
     __shared__ float Cs[kWaves][kJB][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int e = blockIdx.z;
+    int bx, by, e;
+    xcd_block(bx, by, e);
     const int B = g.B, os = g.os, w = g.w, n = g.n, h = g.h, nw = g.nw;
     const int n2 = n >> 1;
-    const int cy_raw = blockIdx.y * kWaves + wave;
+    const int cy_raw = by * kWaves + wave;
     const bool cy_ok = cy_raw < n;
     const int cy = cy_ok ? cy_raw : n - 1;
-    const int cx0 = blockIdx.x * mg.CB;
+    const int cx0 = bx * mg.CB;
     const double2 *Te = Tm + (size_t)e * B * B;
     for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
     __syncthreads();
@@ -276,10 +277,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     __shared__ float Cs[kWaves][OS][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int e = blockIdx.z;
+    int bx, by, e;
+    xcd_block(bx, by, e);
     const int B = g.B, w = g.w, n = g.n, h = g.h, nw = g.nw;
     const int n2 = n >> 1;
-    const int cx0 = blockIdx.x * mg.CB;
+    const int cx0 = bx * mg.CB;
     const double2 *Te = Tm + (size_t)e * B * B;
     for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
 
@@ -322,7 +324,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     }
 
     for (int tt = 0; tt < TPW; tt++) {
-        const int cy_raw = (blockIdx.y * TPW + tt) * kWaves + wave;
+        const int cy_raw = (by * TPW + tt) * kWaves + wave;
         if (cy_raw >= n) break;  // uniform per wave; no block-wide barrier follows
         const int cy = cy_raw;
         float4v hh[OS][NT], cr[OS][NT];

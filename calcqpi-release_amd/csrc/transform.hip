@@ -144,46 +144,47 @@ __global__ void __launch_bounds__(kThreads) k_resolvent_ky(Geom g, const double 
     }
 }
 
+typedef _Float16 planes_h8 __attribute__((ext_vector_type(8)));
+
 // fp16 hi/lo planes for the mixed contraction (rows along x), from the
 // transformed lines S[q][m] (window point R at m = R mod K), scaled by s:
 //   a planes: row sy,      column sx  <- G0(sx, sy)[o][p]
 //   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
+// Each item fills 8 consecutive columns (row length Ly is a multiple of 8)
+// and stores them as one 16-byte hi and one 16-byte lo vector.
 __device__ inline void write_planes(const double2 *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
                                     const MixGeom &mg, double s, _Float16 *__restrict__ P)
 {
     const size_t plane = (size_t)mg.Ly * nw;
     _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
-    for (int t = threadIdx.x; t < 2 * npc * 2 * mg.Ly; t += blockDim.x) {
-        int idx = t % mg.Ly;
-        int rest = t / mg.Ly;
-        int ri = rest & 1;
+    const int ng = mg.Ly >> 3;
+    for (int t = threadIdx.x; t < 2 * npc * 2 * ng; t += blockDim.x) {
+        const int gi = t % ng;
+        int rest = t / ng;
+        const int ri = rest & 1;
         rest >>= 1;
-        int q = rest % npc, ab = rest / npc;
-        int pr = p0 + q, r = pr / B, c = pr - r * B;
-        int sx = idx - mg.OFF;
-        double v = 0.0;
-        int row, pl;
-        if (ab == 0) {
-            row = sy;
-            pl = (r * B + c) * 2 + ri;  // o = r, p = c
+        const int q = rest % npc, ab = rest / npc;
+        const int pr = p0 + q, r = pr / B, c = pr - r * B;
+        // a: o = r, p = c;  b: G0[r][c] = G0(-R)^T[c][r], o = c, p = r
+        const int row = ab ? 2 * h - sy : sy;
+        const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
+        const double2 *line = S + (size_t)q * K;
+        planes_h8 hv, lv;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int sx = gi * 8 + u - mg.OFF;
+            double v = 0.0;
             if (sx >= 0 && sx < nw) {
-                double2 z = S[(size_t)q * K + imod(sx - h, K)];
+                const double2 z = line[imod(ab ? h - sx : sx - h, K)];
                 v = (ri ? z.y : z.x) * s;
             }
-        } else {
-            row = 2 * h - sy;
-            pl = (c * B + r) * 2 + ri;  // G0[r][c] = G0(-R)^T[c][r]: o = c, p = r
-            if (sx >= 0 && sx < nw) {
-                double2 z = S[(size_t)q * K + imod(h - sx, K)];
-                v = (ri ? z.y : z.x) * s;
-            }
+            const _Float16 hi = (_Float16)v;
+            hv[u] = hi;
+            lv[u] = (_Float16)((v - (double)hi) * 4096.0);
         }
-        _Float16 hi = (_Float16)v;
-        _Float16 lo = (_Float16)((v - (double)hi) * 4096.0);
-        size_t o0 = ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + (size_t)row * mg.Ly + idx;
-        size_t o1 = ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + (size_t)row * mg.Ly + idx;
-        Pe[o0] = hi;
-        Pe[o1] = lo;
+        const size_t o = (size_t)row * mg.Ly + (size_t)gi * 8;
+        *reinterpret_cast<planes_h8 *>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + o) = hv;
+        *reinterpret_cast<planes_h8 *>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + o) = lv;
     }
 }
 
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 
 // pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk)
 template <int N>
-__global__ void __launch_bounds__(kWaveMaxThreads) k_kx_s(Geom g, MixGeom mg, const double2 *__restrict__ F1,
+__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const double2 *__restrict__ F1,
                                                const double2 *__restrict__ tw, int np, double sG,
                                                double2 *__restrict__ G0w, _Float16 *__restrict__ P)
 {
@@ -436,12 +437,21 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
     }
 }
 
+// pass 2 reads whole [kx][pair] rows of F1: one chunk of all pairs whenever
+// they fit the LDS budget (a split would fetch each row's cache lines twice)
+int kx_pairs(int N, int NP)
+{
+    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
+    const int nch = (NP + lines - 1) / lines;
+    return (NP + nch - 1) / nch;
+}
+
 template <int N>
 int launch_kx_s(const Geom &g, const MixGeom &m, const double2 *F1, int E, const double2 *tw, double sG,
                 double2 *G0w, _Float16 *P, hipStream_t s)
 {
-    const int NP = g.B * g.B, np = wave_pairs(N, NP), nch = (NP + np - 1) / np;
-    const int thr = wave_threads(N, np, false);
+    const int NP = g.B * g.B, np = kx_pairs(N, NP), nch = (NP + np - 1) / np;
+    const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(double2) * (size_t)np * N;
     hipLaunchKernelGGL((k_kx_s<N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
