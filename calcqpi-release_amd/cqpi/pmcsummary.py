@@ -2,6 +2,7 @@
 MFMA figures per kernel (markdown with a JSON block that bench.py reads).
 
     python -m cqpi.pmcsummary FETCH.csv WRITE.csv [MFMA.csv] > summary.md
+    python -m cqpi.pmcsummary --stats KERNEL_STATS.csv > summary.md
 
 The probe first copies 256 MiB with a plain device copy (the runtime's
 copyBuffer kernel), then runs one mixed-precision and one FP64 step.
@@ -67,7 +68,24 @@ def calibration(rows, expected_kib):
     return expected_kib / big if big else None
 
 
+def stats(path):
+    """Per-kernel time table of a rocprofv3 --kernel-trace --stats run."""
+    with open(path) as f:
+        rows = list(csv.DictReader(f))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    print("| kernel | calls | avg us | min us | max us | share % |")
+    print("|---|---|---|---|---|---|")
+    for r in rows:
+        print("| `%s` | %s | %.1f | %.1f | %.1f | %.1f |" % (
+            short_name(r["Name"]) if "k_" in r["Name"] else r["Name"][:40], r["Calls"],
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3,
+            100 * float(r["TotalDurationNs"]) / total))
+
+
 def main(argv):
+    if argv and argv[0] == "--stats":
+        stats(argv[1])
+        return
     fetch = read(argv[0], "FETCH_SIZE")
     write = read(argv[1], "WRITE_SIZE")
     ff = calibration(fetch, COPY_BYTES / 1024)
