@@ -108,6 +108,8 @@ struct cqpi_ctx {
     // transform
     double2 *d_tw = nullptr;
     size_t c_tw = 0;
+    float2 *d_tw32 = nullptr;  // the same table rounded to FP32 (reduced-precision transform)
+    size_t c_tw32 = 0;
 
     // per-chunk buffers
     double2 *d_F1 = nullptr;
@@ -181,7 +183,7 @@ void release(cqpi_ctx *c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw,
+    void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
@@ -363,13 +365,25 @@ int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
     const size_t WW = (size_t)g.W * g.W, E = plan.E, Ec = plan.Ec;
     for (size_t e0 = 0; e0 < E; e0 += Ec) {
         int ec = (int)std::min(Ec, E - e0);
+        // reduced-precision modes run the resolvent and both transform passes in
+        // FP32 where the Stockham path applies (F1 then holds float2)
+        const bool t32 = c->precision != CQPI_PREC_FP64 && transform32_supported(g);
+        float2 *F1f = reinterpret_cast<float2 *>(c->d_F1);
         int st = stage_begin(c, 1);
-        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, c->d_F1,
-                                   c->stream));
+        if (t32)
+            RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f,
+                                         c->stream));
+        else
+            RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, c->d_F1,
+                                       c->stream));
         stage_end(c, st);
         st = stage_begin(c, 2);
-        RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, plan.sG, c->d_G0w,
-                                   plan.mixed ? c->d_planes : nullptr, c->stream));
+        if (t32)
+            RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, c->d_G0w,
+                                         plan.mixed ? c->d_planes : nullptr, c->stream));
+        else
+            RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, plan.sG, c->d_G0w,
+                                       plan.mixed ? c->d_planes : nullptr, c->stream));
         stage_end(c, st);
         st = stage_begin(c, 3);
         RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
@@ -466,6 +480,11 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
         }
         r = dev_grow(c->d_tw, c->c_tw, kpoints, c->gpumem);
         if (!r && hipMemcpy(c->d_tw, tw.data(), kpoints * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+            r = fail(CQPI_E_HIP, "twiddle upload failed");
+        std::vector<float2> tw32(kpoints);
+        for (size_t m = 0; m < kpoints; m++) tw32[m] = make_float2((float)tw[m].x, (float)tw[m].y);
+        if (!r) r = dev_grow(c->d_tw32, c->c_tw32, kpoints, c->gpumem);
+        if (!r && hipMemcpy(c->d_tw32, tw32.data(), kpoints * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
             r = fail(CQPI_E_HIP, "twiddle upload failed");
     }
     if (!r) r = upload_identity_V(c);

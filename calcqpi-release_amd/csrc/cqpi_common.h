@@ -30,6 +30,19 @@ __host__ __device__ inline double2 cinv(double2 z)
 }
 __host__ __device__ inline double2 cdiv(double2 a, double2 b) { return cmul(a, cinv(b)); }
 
+// complex FP32 as float2 (the reduced-precision transform)
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ inline float2 cscale(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+
 // Hopping terms grouped by lattice vector: every term of group g shares
 // (rx, ry, rz) and the phase exp(+-2 pi i R.k) is evaluated once per group.
 struct DevHop {
@@ -98,6 +111,13 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
 // mg != NULL also the fp16 hi/lo planes of the mixed contraction (scale sG)
 int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,
                         double sG, double2 *G0w, _Float16 *P, hipStream_t s);
+// the same two passes with FP32 lines, F1 and twiddles (power-of-two K in
+// [64, 1024]); the resolvent factor is formed in FP64, the window written in FP64
+bool transform32_supported(const Geom &g);
+int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
+                          double eta, const float2 *tw, float2 *F1, hipStream_t s);
+int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
+                          double2 *G0w, _Float16 *P, hipStream_t s);
 // T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular
 int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T,
                    int *status, hipStream_t s);

@@ -94,34 +94,40 @@ __device__ inline void wave_sync()
 }
 
 // multiply by sign * i
-__device__ inline double2 mul_si(double2 a, int sign)
+template <typename C>
+__device__ inline C mul_si(C a, int sign)
 {
-    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+    C r;
+    r.x = sign > 0 ? -a.y : a.y;
+    r.y = sign > 0 ? a.x : -a.x;
+    return r;
 }
 
-template <int R>
-__device__ inline void dft_small(double2 *x, int sign)
+// in-register DFT of R points (C = double2 or float2)
+template <int R, typename C>
+__device__ inline void dft_small(C *x, int sign)
 {
     if constexpr (R == 2) {
-        double2 a = x[0], b = x[1];
+        C a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
     } else if constexpr (R == 4) {
-        double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
-        double2 a2 = cadd(x[1], x[3]), a3 = mul_si(csub(x[1], x[3]), sign);
+        C a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+        C a2 = cadd(x[1], x[3]), a3 = mul_si(csub(x[1], x[3]), sign);
         x[0] = cadd(a0, a2);
         x[2] = csub(a0, a2);
         x[1] = cadd(a1, a3);
         x[3] = csub(a1, a3);
     } else {
-        double2 e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+        C e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
         dft_small<4>(e, sign);
         dft_small<4>(o, sign);
-        constexpr double r2 = 0.70710678118654752440;
+        using Re = decltype(x[0].x);
+        constexpr Re r2 = (Re)0.70710678118654752440;
         // w8^q o_q, w8 = (1 + sign i)/sqrt2
-        double2 o1 = cscale(r2, cadd(o[1], mul_si(o[1], sign)));
-        double2 o2 = mul_si(o[2], sign);
-        double2 o3 = cscale(r2, csub(mul_si(o[3], sign), o[3]));
+        C o1 = cscale(r2, cadd(o[1], mul_si(o[1], sign)));
+        C o2 = mul_si(o[2], sign);
+        C o3 = cscale(r2, csub(mul_si(o[3], sign), o[3]));
         x[0] = cadd(e[0], o[0]);
         x[4] = csub(e[0], o[0]);
         x[1] = cadd(e[1], o1);
@@ -156,12 +162,12 @@ constexpr int stockham_lanes(int N)
     return mx < 64 ? mx : 64;
 }
 
-template <int N, int L, int R, int LN>
-__device__ inline void stockham_pass(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+template <int N, int L, int R, int LN, typename C>
+__device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     constexpr int NB = N / R;
     constexpr int BPL = (NB + LN - 1) / LN;
-    double2 v[BPL][R];
+    C v[BPL][R];
 #pragma unroll
     for (int b = 0; b < BPL; b++) {
         const int k = ll + LN * b;
@@ -188,8 +194,8 @@ __device__ inline void stockham_pass(double2 *X, const double2 *__restrict__ tw,
     wave_sync();
 }
 
-template <int N, int L, int LGREM, int LN>
-__device__ inline void stockham_rest(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+template <int N, int L, int LGREM, int LN, typename C>
+__device__ inline void stockham_rest(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     if constexpr (LGREM > 0) {
         constexpr int R = stockham_radix(LGREM);
@@ -198,9 +204,10 @@ __device__ inline void stockham_rest(double2 *X, const double2 *__restrict__ tw,
     }
 }
 
-// transform the line X (N points) with lanes ll in [0, stockham_lanes(N))
-template <int N>
-__device__ inline void wave_fft(double2 *X, const double2 *__restrict__ tw, int ll, int sign)
+// transform the line X (N points, C = double2 or float2, twiddle table of the
+// same type) with lanes ll in [0, stockham_lanes(N))
+template <int N, typename C>
+__device__ inline void wave_fft(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     constexpr int LN = stockham_lanes(N);
     constexpr int lg = ilog2n(N);

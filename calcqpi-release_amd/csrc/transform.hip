@@ -152,7 +152,8 @@ typedef _Float16 planes_h8 __attribute__((ext_vector_type(8)));
 //   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
 // Each item fills 8 consecutive columns (row length Ly is a multiple of 8)
 // and stores them as one 16-byte hi and one 16-byte lo vector.
-__device__ inline void write_planes(const double2 *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
+template <typename C>
+__device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
                                     const MixGeom &mg, double s, _Float16 *__restrict__ P)
 {
     const size_t plane = (size_t)mg.Ly * nw;
@@ -168,15 +169,15 @@ __device__ inline void write_planes(const double2 *S, int K, int sy, int e, int 
         // a: o = r, p = c;  b: G0[r][c] = G0(-R)^T[c][r], o = c, p = r
         const int row = ab ? 2 * h - sy : sy;
         const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
-        const double2 *line = S + (size_t)q * K;
+        const C *line = S + (size_t)q * K;
         planes_h8 hv, lv;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int sx = gi * 8 + u - mg.OFF;
             double v = 0.0;
             if (sx >= 0 && sx < nw) {
-                const double2 z = line[imod(ab ? h - sx : sx - h, K)];
-                v = (ri ? z.y : z.x) * s;
+                const C z = line[imod(ab ? h - sx : sx - h, K)];
+                v = (double)(ri ? z.y : z.x) * s;
             }
             const _Float16 hi = (_Float16)v;
             hv[u] = hi;
@@ -243,24 +244,38 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
 }
 
 // ---------------------------------------------------------------------------
-// Stockham path (power-of-two K = N, 64 <= N <= 1024)
+// Stockham path (power-of-two K = N, 64 <= N <= 1024).  C is the complex type
+// of the lines, F1 and the twiddles: double2 (FP64 mode) or float2 (the
+// reduced-precision modes: the resolvent factor 1/(w - e + i eta) is formed in
+// FP64 from the FP64 eigenvalues and then rounded).
 // ---------------------------------------------------------------------------
 constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
 constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in registers
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
 
+template <typename C>
+__device__ inline C to_c(double2 z)
+{
+    C r;
+    r.x = z.x;
+    r.y = z.y;
+    return r;
+}
+
 // pass 1: block = lane groups of stockham_lanes(N) lanes, one line (orbital
 // pair of the chunk) per group.  0 < B <= kMaxRegB: the eigen data of the
 // thread's k-point (N <= blockDim) stays in registers for the whole energy
 // batch; otherwise (B == 0: band count g.B) it is read per energy.
-template <int B, int N>
+template <typename C, int B, int N>
 __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const double *__restrict__ eval,
-                                                   const double2 *__restrict__ evec,
-                                                   const double *__restrict__ omegas, int E, int EB, double eta,
-                                                   const double2 *__restrict__ tw, int np, double2 *__restrict__ F1)
+                                                              const double2 *__restrict__ evec,
+                                                              const double *__restrict__ omegas, int E, int EB,
+                                                              double eta, const C *__restrict__ tw, int np,
+                                                              C *__restrict__ F1)
 {
-    extern __shared__ double2 S[];
+    extern __shared__ __align__(16) unsigned char smem_s[];
+    C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
     constexpr bool kRegs = B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads;  // blockDim >= N (launcher)
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
@@ -271,7 +286,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
 
     double ev_r[kRegs ? B : 1];
-    double2 vv_r[kRegs ? B * B : 1];
+    C vv_r[kRegs ? B * B : 1];
     if constexpr (kRegs) {
         const int j = threadIdx.x;
         if (j < N) {
@@ -279,7 +294,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 #pragma unroll
             for (int n = 0; n < B; n++) ev_r[n] = eval[n * KK + k];
 #pragma unroll
-            for (int r = 0; r < B * B; r++) vv_r[r] = evec[r * KK + k];
+            for (int r = 0; r < B * B; r++) vv_r[r] = to_c<C>(evec[r * KK + k]);
         }
     }
     for (int e = e0; e < e1; e++) {
@@ -287,15 +302,15 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
         if constexpr (kRegs) {
             const int j = threadIdx.x;
             if (j < N) {
-                double2 f[B];
+                C f[B];
 #pragma unroll
-                for (int n = 0; n < B; n++) f[n] = cinv(make_double2(omega - ev_r[n], eta));
+                for (int n = 0; n < B; n++) f[n] = to_c<C>(cinv(make_double2(omega - ev_r[n], eta)));
                 for (int q = 0; q < npc; q++) {
                     const int pr = p0 + q, r = pr / B, c = pr - r * B;
-                    double2 acc = make_double2(0.0, 0.0);
+                    C acc = to_c<C>(make_double2(0.0, 0.0));
 #pragma unroll
                     for (int n = 0; n < B; n++) {
-                        double2 vr = vv_r[n], vc = vv_r[n];
+                        C vr = vv_r[n], vc = vv_r[n];
 #pragma unroll
                         for (int x = 0; x < B; x++) {
                             if (x == r) vr = vv_r[x * B + n];
@@ -311,17 +326,17 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
             // one resolvent factor per band, accumulated into the chunk's pairs
             for (int j = threadIdx.x; j < N; j += blockDim.x) {
                 const long k = (long)i * N + j;
-                double2 acc[kMaxLines];
+                C acc[kMaxLines];
 #pragma unroll
-                for (int q = 0; q < kMaxLines; q++) acc[q] = make_double2(0.0, 0.0);
+                for (int q = 0; q < kMaxLines; q++) acc[q] = to_c<C>(make_double2(0.0, 0.0));
                 for (int n = 0; n < nb; n++) {
-                    const double2 f = cinv(make_double2(omega - eval[n * KK + k], eta));
+                    const C f = to_c<C>(cinv(make_double2(omega - eval[n * KK + k], eta)));
 #pragma unroll
                     for (int q = 0; q < kMaxLines; q++) {
                         if (q < npc) {
                             const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
-                            acc[q] = cadd(acc[q],
-                                          cmul(f, cmulc(evec[(r * nb + n) * KK + k], evec[(c * nb + n) * KK + k])));
+                            acc[q] = cadd(acc[q], cmul(f, cmulc(to_c<C>(evec[(r * nb + n) * KK + k]),
+                                                                to_c<C>(evec[(c * nb + n) * KK + k]))));
                         }
                     }
                 }
@@ -333,7 +348,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
         __syncthreads();
         for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * N, tw, ll, +1);
         __syncthreads();
-        double2 *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
+        C *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int slot = t / npc, q = t - slot * npc;
             out[(size_t)slot * N * NP + q] = S[(size_t)q * N + ((slot - h) & (N - 1))];
@@ -342,19 +357,21 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
     }
 }
 
-// pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk)
-template <int N>
-__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const double2 *__restrict__ F1,
-                                               const double2 *__restrict__ tw, int np, double sG,
-                                               double2 *__restrict__ G0w, _Float16 *__restrict__ P)
+// pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk);
+// the window is written in FP64 whatever C is
+template <typename C, int N>
+__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
+                                                                 const C *__restrict__ tw, int np, double sG,
+                                                                 double2 *__restrict__ G0w, _Float16 *__restrict__ P)
 {
-    extern __shared__ double2 S[];
+    extern __shared__ __align__(16) unsigned char smem_s[];
+    C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
     const int sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
     const int npc = min(np, NP - p0);
     const int nw = g.nw, h = g.h;
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
-    const double2 *in = F1 + ((size_t)e * nw + sy) * N * NP + p0;
+    const C *in = F1 + ((size_t)e * nw + sy) * N * NP + p0;
     for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
         const int i = t / npc, q = t - i * npc;
         S[(size_t)q * N + i] = in[(size_t)i * NP + q];
@@ -366,7 +383,8 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     double2 *out = G0w + (size_t)e * nw * nw * NP;
     for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
         const int sx = t / npc, q = t - sx * npc;
-        out[((size_t)sx * nw + sy) * NP + p0 + q] = cscale(inv, S[(size_t)q * N + ((sx - h) & (N - 1))]);
+        const C z = S[(size_t)q * N + ((sx - h) & (N - 1))];
+        out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
     }
     if (P) write_planes(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
 }
@@ -384,15 +402,15 @@ int pairs_per_chunk(int K, int NP)
     return (NP + nch - 1) / nch;
 }
 
-// Stockham path: pairs per chunk from the LDS budget (one line of N doubles2
-// per pair), workgroup = enough lane groups for the chunk's lines and, on
-// pass 1, at least one thread per k-point (up to kWaveMaxThreads)
+// Stockham path: pairs per chunk from the LDS budget (one line of N complex
+// values per pair), workgroup = enough lane groups for the chunk's lines and,
+// on pass 1, at least one thread per k-point (up to kWaveMaxThreads)
 constexpr size_t kWaveLds = 80 * 1024;
 
-int wave_pairs(int N, int NP)
+int wave_pairs(int N, int NP, size_t csize)
 {
     const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
-    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
+    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)N))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -405,55 +423,55 @@ int wave_threads(int N, int np, bool pass1)
     return std::min(t, kWaveMaxThreads);
 }
 
-template <int B, int N>
+template <typename C, int B, int N>
 int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
-                 const double2 *tw, double2 *F1, hipStream_t s)
+                 const C *tw, C *F1, hipStream_t s)
 {
-    const int NP = g.B * g.B, np = wave_pairs(N, NP), nch = (NP + np - 1) / np;
+    const int NP = g.B * g.B, np = wave_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int thr = wave_threads(N, np, true);
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     // energies per workgroup: enough workgroups for ~2 per CU
     const int rows = N * nch;
     const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 512));
     const int nb = (E + EB - 1) / EB;
-    const size_t lds = sizeof(double2) * (size_t)np * N;
-    hipLaunchKernelGGL((k_res_ky_s<B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB, eta,
-                       tw, np, F1);
+    const size_t lds = sizeof(C) * (size_t)np * N;
+    hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
+                       eta, tw, np, F1);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
-template <int N>
+template <typename C, int N>
 int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
-                  const double2 *tw, double2 *F1, hipStream_t s)
+                  const C *tw, C *F1, hipStream_t s)
 {
-    if (N > kWaveMaxThreads) return launch_res_s<0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    if (N > kWaveMaxThreads) return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     switch (g.B) {
-    case 1: return launch_res_s<1, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 2: return launch_res_s<2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 3: return launch_res_s<3, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 4: return launch_res_s<4, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 5: return launch_res_s<5, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    default: return launch_res_s<0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 1: return launch_res_s<C, 1, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 2: return launch_res_s<C, 2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 3: return launch_res_s<C, 3, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 4: return launch_res_s<C, 4, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 5: return launch_res_s<C, 5, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    default: return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
     }
 }
 
 // pass 2 reads whole [kx][pair] rows of F1: one chunk of all pairs whenever
 // they fit the LDS budget (a split would fetch each row's cache lines twice)
-int kx_pairs(int N, int NP)
+int kx_pairs(int N, int NP, size_t csize)
 {
-    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (sizeof(double2) * (size_t)N))));
+    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)N))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
 
-template <int N>
-int launch_kx_s(const Geom &g, const MixGeom &m, const double2 *F1, int E, const double2 *tw, double sG,
-                double2 *G0w, _Float16 *P, hipStream_t s)
+template <typename C, int N>
+int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw, double sG, double2 *G0w,
+                _Float16 *P, hipStream_t s)
 {
-    const int NP = g.B * g.B, np = kx_pairs(N, NP), nch = (NP + np - 1) / np;
+    const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
-    const size_t lds = sizeof(double2) * (size_t)np * N;
-    hipLaunchKernelGGL((k_kx_s<N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
+    const size_t lds = sizeof(C) * (size_t)np * N;
+    hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -494,11 +512,11 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
 {
     if (wave_path(g.K)) {
         switch (g.K) {
-        case 64: return launch_res_sb<64>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 128: return launch_res_sb<128>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 256: return launch_res_sb<256>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 512: return launch_res_sb<512>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 1024: return launch_res_sb<1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 64: return launch_res_sb<double2, 64>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 128: return launch_res_sb<double2, 128>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 256: return launch_res_sb<double2, 256>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 512: return launch_res_sb<double2, 512>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 1024: return launch_res_sb<double2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
         }
     }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
@@ -512,6 +530,37 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
     }
 }
 
+bool transform32_supported(const Geom &g) { return wave_path(g.K); }
+
+int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
+                          double eta, const float2 *tw, float2 *F1, hipStream_t s)
+{
+    switch (g.K) {
+    case 64: return launch_res_sb<float2, 64>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 128: return launch_res_sb<float2, 128>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 256: return launch_res_sb<float2, 256>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 512: return launch_res_sb<float2, 512>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 1024: return launch_res_sb<float2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    default: return CQPI_E_ARG;
+    }
+}
+
+int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
+                          double2 *G0w, _Float16 *P, hipStream_t s)
+{
+    MixGeom m{};
+    if (mg) m = *mg;
+    _Float16 *Pm = mg ? P : nullptr;
+    switch (g.K) {
+    case 64: return launch_kx_s<float2, 64>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    case 128: return launch_kx_s<float2, 128>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    case 256: return launch_kx_s<float2, 256>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    case 512: return launch_kx_s<float2, 512>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    case 1024: return launch_kx_s<float2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    default: return CQPI_E_ARG;
+    }
+}
+
 int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,
                         double sG, double2 *G0w, _Float16 *P, hipStream_t s)
 {
@@ -520,11 +569,11 @@ int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int
     _Float16 *Pm = mg ? P : nullptr;
     if (wave_path(g.K)) {
         switch (g.K) {
-        case 64: return launch_kx_s<64>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 128: return launch_kx_s<128>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 256: return launch_kx_s<256>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 512: return launch_kx_s<512>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 1024: return launch_kx_s<1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 64: return launch_kx_s<double2, 64>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 128: return launch_kx_s<double2, 128>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 256: return launch_kx_s<double2, 256>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 512: return launch_kx_s<double2, 512>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 1024: return launch_kx_s<double2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
         }
     }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
