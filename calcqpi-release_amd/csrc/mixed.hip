@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
 // KS): the A fragments of every (jj, ks) stay in registers and each wave walks
 // TPW tiles along y, so A is read from memory once per wave instead of once
 // per tile.  Non-spin models only (one orbital block).
-template <int NT, int LDW, int OS, int KS, int TPW>
+template <int NT, int LDW, int OS, int KS, int TPW, int BT>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                          const half8 *__restrict__ Afrag,
                                                          const double *__restrict__ scale, double sA,
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
                                                          const double *__restrict__ bulk,
                                                          double *__restrict__ ldos)
 {
-    __shared__ double2 Ts[16 * 16];
+    __shared__ float2 Tf[BT * BT];  // T in FP32 for the epilogue
     __shared__ float Cs[kWaves][OS][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     const int n2 = n >> 1;
     const int cx0 = bx * mg.CB;
     const double2 *Te = Tm + (size_t)e * B * B;
-    for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
+    for (int i = threadIdx.x; i < BT * BT; i += blockDim.x) Tf[i] = make_float2((float)Te[i].x, (float)Te[i].y);
 
     half8 ah[OS][KS], al[OS][KS];
 #pragma unroll
@@ -385,13 +385,23 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
         for (int jb = 0; jb < (OS + 3) / 4; jb++) {
             const int jj = jb * 4 + (lane >> 4), m = lane & 15;
             if (jj < OS && m < rows) {
+                // Im sum_pq a_p T_pq b_q: (T b)_p in FP32, the sum over p in FP64
                 const float *Cr = Cs[wave][jj][m];
+                float cv[4 * BT];
+#pragma unroll
+                for (int k = 0; k < 4 * BT; k++) cv[k] = Cr[k];
                 double acc = 0.0;
-                for (int p = 0; p < B; p++) {
-                    double2 tb = make_double2(0.0, 0.0);
-                    for (int q = 0; q < B; q++)
-                        tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
-                    acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
+#pragma unroll
+                for (int p = 0; p < BT; p++) {
+                    float tbx = 0.f, tby = 0.f;
+#pragma unroll
+                    for (int q = 0; q < BT; q++) {
+                        const float2 t = Tf[p * BT + q];
+                        const float bx = cv[2 * BT + 2 * q], by = cv[2 * BT + 2 * q + 1];
+                        tbx += t.x * bx - t.y * by;
+                        tby += t.x * by + t.y * bx;
+                    }
+                    acc += (double)cv[2 * p] * (double)tby + (double)cv[2 * p + 1] * (double)tbx;
                 }
                 const int c = m / OS, ii = m - c * OS, cx = cx0 + c;
                 if (cx < n) {
@@ -408,24 +418,24 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
 
 constexpr int kTPW = 8;  // default cell rows per wave
 
-template <int NT, int LDW, int OS, int KS, int TPW>
+template <int NT, int LDW, int OS, int KS, int TPW, int BT>
 void launch_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
                  double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * TPW - 1) / (kWaves * TPW), E);
-    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A, scale, sA,
+    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW, BT>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A, scale, sA,
                        T, bulk, ldos);
 }
 
-template <int NT, int LDW, int OS, int KS>
+template <int NT, int LDW, int OS, int KS, int BT>
 bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
               double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
-    if (g.os != OS || mg.KS != KS || mg.NT != NT || mg.nsets != 1 || mg.LDW != LDW) return false;
+    if (g.os != OS || g.B != BT || mg.KS != KS || mg.NT != NT || mg.nsets != 1 || mg.LDW != LDW) return false;
     switch (g.tpw) {
-    case 2: launch_areg<NT, LDW, OS, KS, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-    case 4: launch_areg<NT, LDW, OS, KS, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-    default: launch_areg<NT, LDW, OS, KS, kTPW>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 2: launch_areg<NT, LDW, OS, KS, 2, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 4: launch_areg<NT, LDW, OS, KS, 4, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    default: launch_areg<NT, LDW, OS, KS, kTPW, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
     }
     return true;
 }
@@ -533,10 +543,10 @@ int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, c
     const half8 *A = (const half8 *)Afrag;
     // register-resident tile fragments for the common small shapes:
     // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital)
-    if (try_areg<1, 4, 4, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<1, 4, 4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<2, 4, 4, 7>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<1, 2, 8, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
+    if (try_areg<1, 4, 4, 4, 3>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 4, 4, 2, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<2, 4, 4, 7, 5>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 2, 8, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     switch (mg.NT) {
     case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);

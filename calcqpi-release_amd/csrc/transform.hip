@@ -159,6 +159,7 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
     const size_t plane = (size_t)mg.Ly * nw;
     _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
     const int ng = mg.Ly >> 3;
+    const bool pow2 = (K & (K - 1)) == 0;
     for (int t = threadIdx.x; t < 2 * npc * 2 * ng; t += blockDim.x) {
         const int gi = t % ng;
         int rest = t / ng;
@@ -176,7 +177,8 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
             const int sx = gi * 8 + u - mg.OFF;
             double v = 0.0;
             if (sx >= 0 && sx < nw) {
-                const C z = line[imod(ab ? h - sx : sx - h, K)];
+                const int rr = ab ? h - sx : sx - h;
+                const C z = line[pow2 ? (rr & (K - 1)) : imod(rr, K)];
                 v = (double)(ri ? z.y : z.x) * s;
             }
             const _Float16 hi = (_Float16)v;
