@@ -162,6 +162,21 @@ constexpr int stockham_lanes(int N)
     return mx < 64 ? mx : 64;
 }
 
+// FP64 lines are stored padded: point i at pidx(i) = i + i/8 + i/64, so that
+// the strided Stockham stores of 16-byte points (stride R, then blocks of 8 at
+// stride 64) spread over the LDS banks; a padded line of N points takes
+// padded_len(N) slots.  FP32 lines (8-byte points) measured faster unpadded.
+template <typename C>
+__host__ __device__ constexpr int pidx(int i)
+{
+    return sizeof(C) == 16 ? i + (i >> 3) + (i >> 6) : i;
+}
+template <typename C>
+__host__ __device__ constexpr int padded_len(int N)
+{
+    return sizeof(C) == 16 ? N + (N >> 3) + (N >> 6) : N;
+}
+
 template <int N, int L, int R, int LN, typename C>
 __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int sign)
 {
@@ -173,7 +188,7 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
         const int k = ll + LN * b;
         if (k < NB)
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = X[k + r * NB];
+            for (int r = 0; r < R; r++) v[b][r] = X[pidx<C>(k + r * NB)];
     }
     wave_sync();
 #pragma unroll
@@ -188,7 +203,7 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
             dft_small<R>(v[b], sign);
             const int base = (k - j) * R + j;
 #pragma unroll
-            for (int q = 0; q < R; q++) X[base + q * L] = v[b][q];
+            for (int q = 0; q < R; q++) X[pidx<C>(base + q * L)] = v[b][q];
         }
     }
     wave_sync();
@@ -204,8 +219,8 @@ __device__ inline void stockham_rest(C *X, const C *__restrict__ tw, int ll, int
     }
 }
 
-// transform the line X (N points, C = double2 or float2, twiddle table of the
-// same type) with lanes ll in [0, stockham_lanes(N))
+// transform the padded line X (N points, C = double2 or float2, twiddle table
+// of the same type) with lanes ll in [0, stockham_lanes(N))
 template <int N, typename C>
 __device__ inline void wave_fft(C *X, const C *__restrict__ tw, int ll, int sign)
 {

@@ -152,7 +152,7 @@ typedef _Float16 planes_h8 __attribute__((ext_vector_type(8)));
 //   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
 // Each item fills 8 consecutive columns (row length Ly is a multiple of 8)
 // and stores them as one 16-byte hi and one 16-byte lo vector.
-template <typename C>
+template <typename C, bool PAD>
 __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
                                     const MixGeom &mg, double s, _Float16 *__restrict__ P)
 {
@@ -170,7 +170,7 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
         // a: o = r, p = c;  b: G0[r][c] = G0(-R)^T[c][r], o = c, p = r
         const int row = ab ? 2 * h - sy : sy;
         const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
-        const C *line = S + (size_t)q * K;
+        const C *line = S + (size_t)q * (PAD ? cqpi::padded_len<C>(K) : K);
         planes_h8 hv, lv;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -178,7 +178,8 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
             double v = 0.0;
             if (sx >= 0 && sx < nw) {
                 const int rr = ab ? h - sx : sx - h;
-                const C z = line[pow2 ? (rr & (K - 1)) : imod(rr, K)];
+                const int m = pow2 ? (rr & (K - 1)) : imod(rr, K);
+                const C z = line[PAD ? cqpi::pidx<C>(m) : m];
                 v = (double)(ri ? z.y : z.x) * s;
             }
             const _Float16 hi = (_Float16)v;
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
         __syncthreads();
     }
     if (!P) return;
-    write_planes(S, K, sy, e, p0, npc, B, nw, h, mg, sG * (pow2 ? inv : 1.0), P);
+    write_planes<double2, false>(S, K, sy, e, p0, npc, B, nw, h, mg, sG * (pow2 ? inv : 1.0), P);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,6 +280,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
+    constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
     constexpr bool kRegs = B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads;  // blockDim >= N (launcher)
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
     const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
@@ -320,7 +322,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
                         }
                         acc = cadd(acc, cmul(f[n], cmulc(vr, vc)));
                     }
-                    S[(size_t)q * N + j] = acc;
+                    S[(size_t)q * NL + cqpi::pidx<C>(j)] = acc;
                 }
             }
         }
@@ -344,16 +346,16 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
                 }
 #pragma unroll
                 for (int q = 0; q < kMaxLines; q++)
-                    if (q < npc) S[(size_t)q * N + j] = acc[q];
+                    if (q < npc) S[(size_t)q * NL + cqpi::pidx<C>(j)] = acc[q];
             }
         }
         __syncthreads();
-        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * N, tw, ll, +1);
+        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
         __syncthreads();
         C *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int slot = t / npc, q = t - slot * npc;
-            out[(size_t)slot * N * NP + q] = S[(size_t)q * N + ((slot - h) & (N - 1))];
+            out[(size_t)slot * N * NP + q] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
         }
         __syncthreads();  // S is rewritten for the next energy
     }
@@ -369,6 +371,7 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
+    constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
     const int sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
     const int npc = min(np, NP - p0);
     const int nw = g.nw, h = g.h;
@@ -376,19 +379,19 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     const C *in = F1 + ((size_t)e * nw + sy) * N * NP + p0;
     for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
         const int i = t / npc, q = t - i * npc;
-        S[(size_t)q * N + i] = in[(size_t)i * NP + q];
+        S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * NP + q];
     }
     __syncthreads();
-    for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * N, tw, ll, +1);
+    for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
     __syncthreads();
     const double inv = 1.0 / ((double)N * (double)N);
     double2 *out = G0w + (size_t)e * nw * nw * NP;
     for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
         const int sx = t / npc, q = t - sx * npc;
-        const C z = S[(size_t)q * N + ((sx - h) & (N - 1))];
+        const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
         out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
     }
-    if (P) write_planes(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
+    if (P) write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
 }
 
 constexpr size_t kLdsBudget = 80 * 1024;   // largest line buffer a workgroup may take
@@ -407,12 +410,12 @@ int pairs_per_chunk(int K, int NP)
 // Stockham path: pairs per chunk from the LDS budget (one line of N complex
 // values per pair), workgroup = enough lane groups for the chunk's lines and,
 // on pass 1, at least one thread per k-point (up to kWaveMaxThreads)
-constexpr size_t kWaveLds = 80 * 1024;
+constexpr size_t kWaveLds = 96 * 1024;
 
 int wave_pairs(int N, int NP, size_t csize)
 {
     const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
-    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)N))));
+    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : N)))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -436,7 +439,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int rows = N * nch;
     const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 512));
     const int nb = (E + EB - 1) / EB;
-    const size_t lds = sizeof(C) * (size_t)np * N;
+    const size_t lds = sizeof(C) * (size_t)np * cqpi::padded_len<C>(N);
     hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
                        eta, tw, np, F1);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -461,7 +464,7 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
 // they fit the LDS budget (a split would fetch each row's cache lines twice)
 int kx_pairs(int N, int NP, size_t csize)
 {
-    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)N))));
+    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : N)))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -472,7 +475,7 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
 {
     const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
-    const size_t lds = sizeof(C) * (size_t)np * N;
+    const size_t lds = sizeof(C) * (size_t)np * cqpi::padded_len<C>(N);
     hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
