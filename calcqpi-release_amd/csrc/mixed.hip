@@ -306,6 +306,25 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     const int W = g.W;
     double *out = ldos + (size_t)e * W * W;
 
+    // Stage the plane rows this workgroup's tiles read: rows of the cells
+    // cy_lo .. cy_lo + TPW*kWaves - 1 plus the w halo, the NUg groups of 8
+    // columns from col0, all 4*NPL planes.  Layout [plane][row (RP)][NUg*8],
+    // RP odd so that the 16 planes one MFMA column group reads fall on
+    // different banks.
+    extern __shared__ __align__(16) unsigned char smem_m[];
+    _Float16 *Pl = reinterpret_cast<_Float16 *>(smem_m);
+    const int RW = TPW * kWaves + 2 * w, RP = RW | 1, seg = mg.NUg;
+    const int cy_lo = by * TPW * kWaves;
+    const int row_lo = cy_lo - n2 - w + h;
+    for (int l = threadIdx.x; l < 4 * mg.NPL * RW * seg; l += blockDim.x) {
+        const int g8 = l % seg, rest = l / seg, r = rest % RW, pid = rest / RW;
+        const int row = row_lo + r;
+        half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (row < nw) v = load8<LDW>(Pe + (size_t)pid * plane + (size_t)row * mg.Ly + col0 + g8 * 8);
+        *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
+    }
+    __syncthreads();
+
     // per lane: the K group it loads and the plane offsets of its N columns
     int kdy[KS], kg8[KS];
     bool kok[KS];
@@ -338,7 +357,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int dy = kdy[ks] & 0xffff, o = kdy[ks] >> 16;
-            const int row = cy - n2 + dy - w + h;
+            const int rrow = cy - cy_lo + dy;  // staged row
             half8 bh[NT], bl[NT];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
@@ -353,9 +372,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
                     pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
                 }
                 if (kok[ks] && ab >= 0) {
-                    const size_t off = (size_t)row * mg.Ly + col0 + kg8[ks] * 8;
-                    bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + off);
-                    bl[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + off);
+                    const int pid = (ab * 2) * mg.NPL + pl;
+                    const _Float16 *q0 = Pl + ((size_t)pid * RP + rrow) * seg * 8 + kg8[ks] * 8;
+                    const _Float16 *q1 = q0 + (size_t)mg.NPL * RP * seg * 8;  // lo plane of the same (o, p)
+                    bh[t] = *reinterpret_cast<const half8 *>(q0);
+                    bl[t] = *reinterpret_cast<const half8 *>(q1);
                 } else {
                     bh[t] = half8{0, 0, 0, 0, 0, 0, 0, 0};
                     bl[t] = bh[t];
@@ -423,7 +444,9 @@ void launch_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half
                  double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * TPW - 1) / (kWaves * TPW), E);
-    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW, BT>), grid, dim3(64 * kWaves), 0, s, g, mg, P, A, scale, sA,
+    const int RP = (TPW * kWaves + 2 * g.w) | 1;
+    const size_t lds = (size_t)4 * mg.NPL * RP * mg.NUg * 8 * sizeof(_Float16);
+    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW, BT>), grid, dim3(64 * kWaves), lds, s, g, mg, P, A, scale, sA,
                        T, bulk, ldos);
 }
 
