@@ -74,6 +74,17 @@ struct Geom {
     int eb;       // first transform pass: energies per workgroup
 };
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads also waits
+// for the workgroup's outstanding global stores; where a barrier only guards
+// the reuse of an LDS buffer this one lets those stores drain in the
+// background.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup coordinates of a 3-D grid renumbered for the 8 XCDs of the chip.
 // The dispatcher hands consecutive workgroup ids to the XCDs round-robin, so
 // id L runs on XCD L % 8.  The renumbering v = (ids before XCD L % 8) + L / 8

@@ -349,15 +349,15 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
                     if (q < npc) S[(size_t)q * NL + cqpi::pidx<C>(j)] = acc[q];
             }
         }
-        __syncthreads();
+        lds_barrier();
         for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
-        __syncthreads();
+        lds_barrier();
         C *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int slot = t / npc, q = t - slot * npc;
             out[(size_t)slot * N * NP + q] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
         }
-        __syncthreads();  // S is rewritten for the next energy
+        lds_barrier();  // S is rewritten for the next energy
     }
 }
 
@@ -381,9 +381,9 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
         const int i = t / npc, q = t - i * npc;
         S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * NP + q];
     }
-    __syncthreads();
+    lds_barrier();
     for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
-    __syncthreads();
+    lds_barrier();
     const double inv = 1.0 / ((double)N * (double)N);
     double2 *out = G0w + (size_t)e * nw * nw * NP;
     for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
