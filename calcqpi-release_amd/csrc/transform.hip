@@ -257,6 +257,27 @@ constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in reg
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
 
+// resolvent factor 1/(d + i eta), d = w - e formed in FP64 by the caller
+// (it cancels); FP64 result for double2 lines, FP32 (one reciprocal) for float2
+template <typename C>
+__device__ inline C res_factor(double d, double eta)
+{
+    if constexpr (sizeof(C) == 16) {
+        const double2 f = cinv(make_double2(d, eta));
+        C r;
+        r.x = f.x;
+        r.y = f.y;
+        return r;
+    } else {
+        const float x = (float)d, y = (float)eta;
+        const float inv = __builtin_amdgcn_rcpf(x * x + y * y);
+        C r;
+        r.x = x * inv;
+        r.y = -y * inv;
+        return r;
+    }
+}
+
 template <typename C>
 __device__ inline C to_c(double2 z)
 {
@@ -281,6 +302,12 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
     C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
     constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
+    // FP32 lines read the twiddles from an LDS copy; FP64 ones (which measured
+    // slower that way) from the cached global table
+    C *Twl = S + (size_t)np * NL;
+    if constexpr (sizeof(C) == 8)
+        for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
+    const C *Tw = sizeof(C) == 8 ? Twl : tw;
     constexpr bool kRegs = B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads;  // blockDim >= N (launcher)
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
     const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
@@ -308,7 +335,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
             if (j < N) {
                 C f[B];
 #pragma unroll
-                for (int n = 0; n < B; n++) f[n] = to_c<C>(cinv(make_double2(omega - ev_r[n], eta)));
+                for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
                 for (int q = 0; q < npc; q++) {
                     const int pr = p0 + q, r = pr / B, c = pr - r * B;
                     C acc = to_c<C>(make_double2(0.0, 0.0));
@@ -334,7 +361,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 #pragma unroll
                 for (int q = 0; q < kMaxLines; q++) acc[q] = to_c<C>(make_double2(0.0, 0.0));
                 for (int n = 0; n < nb; n++) {
-                    const C f = to_c<C>(cinv(make_double2(omega - eval[n * KK + k], eta)));
+                    const C f = res_factor<C>(omega - eval[n * KK + k], eta);
 #pragma unroll
                     for (int q = 0; q < kMaxLines; q++) {
                         if (q < npc) {
@@ -350,7 +377,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
             }
         }
         lds_barrier();
-        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
+        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
         lds_barrier();
         C *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
@@ -372,6 +399,12 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
     constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
+    // FP32 lines read the twiddles from an LDS copy; FP64 ones (which measured
+    // slower that way) from the cached global table
+    C *Twl = S + (size_t)np * NL;
+    if constexpr (sizeof(C) == 8)
+        for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
+    const C *Tw = sizeof(C) == 8 ? Twl : tw;
     const int sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
     const int npc = min(np, NP - p0);
     const int nw = g.nw, h = g.h;
@@ -382,7 +415,7 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
         S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * NP + q];
     }
     lds_barrier();
-    for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, tw, ll, +1);
+    for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
     lds_barrier();
     const double inv = 1.0 / ((double)N * (double)N);
     double2 *out = G0w + (size_t)e * nw * nw * NP;
@@ -439,7 +472,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int rows = N * nch;
     const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 512));
     const int nb = (E + EB - 1) / EB;
-    const size_t lds = sizeof(C) * (size_t)np * cqpi::padded_len<C>(N);
+    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
                        eta, tw, np, F1);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -475,7 +508,7 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
 {
     const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
-    const size_t lds = sizeof(C) * (size_t)np * cqpi::padded_len<C>(N);
+    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
