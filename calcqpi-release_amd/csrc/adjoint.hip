@@ -540,8 +540,96 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
         if (j >= K) break;
         const long k = (long)i * K + j;
 #pragma unroll
-        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * KK + k) * NP + p] = hacc[t][p];
+        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hacc[t][p];
     }
+}
+
+// The same adjoint on the Stockham path (power-of-two K = N <= 512, b <= 3):
+// one thread per k-point of the row (eigen data and the Hbar accumulators in
+// registers), one lane group per orbital-pair line, padded FP64 lines.
+constexpr int kAdjWaveMaxThreads = 512;
+
+template <int B, int N>
+__global__ void __launch_bounds__(kAdjWaveMaxThreads) k_adj_ky_s(Geom g, const double *__restrict__ eval,
+                                                                 const double2 *__restrict__ evec,
+                                                                 const double *__restrict__ omegas, int E, int EG,
+                                                                 double eta, const double2 *__restrict__ F1bar,
+                                                                 const double2 *__restrict__ twn,
+                                                                 double2 *__restrict__ Hacc)
+{
+    extern __shared__ __align__(16) unsigned char smem_a[];
+    double2 *S = reinterpret_cast<double2 *>(smem_a);
+    constexpr int NP = B * B;
+    constexpr int LN = cqpi::stockham_lanes(N);
+    constexpr int NL = cqpi::padded_len<double2>(N);
+    const int i = blockIdx.x, nw = g.nw, h = g.h;
+    const int e0 = blockIdx.y * EG, e1 = min(E, e0 + EG);
+    const long KK = (long)N * N;
+    const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
+    const int j = threadIdx.x;  // this thread's k-point (blockDim >= N)
+    const bool has_k = j < N;
+    const long k = (long)i * N + (has_k ? j : 0);
+    double ev[B];
+    double2 vv[NP], hacc[NP];
+#pragma unroll
+    for (int nn = 0; nn < B; nn++) ev[nn] = eval[nn * KK + k];
+#pragma unroll
+    for (int r = 0; r < NP; r++) {
+        vv[r] = evec[r * KK + k];
+        hacc[r] = make_double2(0.0, 0.0);
+    }
+    for (int e = e0; e < e1; e++) {
+        const double2 *in = F1bar + ((size_t)e * N + i) * (size_t)nw * NP;
+        for (int t = threadIdx.x; t < N * NP; t += blockDim.x) {
+            const int m = t / NP, q = t - m * NP;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int sy = (m + h) & (N - 1); sy < nw; sy += N) acc = cadd(acc, in[(size_t)sy * NP + q]);
+            S[(size_t)q * NL + cqpi::pidx<double2>(m)] = acc;
+        }
+        lds_barrier();
+        for (int q = grp; q < NP; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, twn, ll, -1);
+        lds_barrier();
+        if (has_k) {
+            const double omega = omegas[e];
+            double2 f[B];
+#pragma unroll
+            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - ev[nn], eta));
+            double2 G[NP];
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int nn = 0; nn < B; nn++) acc = cadd(acc, cmul(f[nn], cmulc(vv[r * B + nn], vv[c * B + nn])));
+                    G[r * B + c] = acc;
+                }
+            // Hbar = G^H (Gb G^H), one column of X = Gb G^H at a time; Gb read from LDS
+            const double2 *gb = S + cqpi::pidx<double2>(j);
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                double2 X[B];
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int l = 0; l < B; l++) acc = cadd(acc, cmulc(gb[(size_t)(r * B + l) * NL], G[c * B + l]));
+                    X[r] = acc;
+                }
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int l = 0; l < B; l++) acc = cadd(acc, cmul(cconj(G[l * B + r]), X[l]));
+                    hacc[r * B + c] = cadd(hacc[r * B + c], acc);
+                }
+            }
+        }
+        lds_barrier();  // S is rewritten for the next energy
+    }
+    if (has_k)
+#pragma unroll
+        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hacc[p];
 }
 
 __device__ inline double kcoord(int i, int K)
@@ -568,12 +656,26 @@ __device__ inline double2 block_sum2(double2 v, double *r1, double *r2)
     return out;
 }
 
+// e^{-2 pi i x kf} for a hopping coordinate x and the grid coordinate kf of
+// index k: from the table twn[m] = e^{-2 pi i m/K} when x is an integer (then
+// kf = k/K and (k-K)/K give the same phase), else by sincos
+__device__ inline double2 hop_phase(double x, int k, int K, const double2 *__restrict__ twn)
+{
+    const double xr = rint(x);
+    if (xr == x && fabs(xr) < 1e6) return twn[imod((int)(((long)xr * k) % K), K)];
+    double sn, cs;
+    sincos(-2.0 * M_PI * x * kcoord(k, K), &sn, &cs);
+    return make_double2(cs, sn);
+}
+
 // Hopping sums S(R) = sum_k Hbar(k) e^{-2 pi i R.k} factorise over kx and ky.
 // Stage 1, one workgroup per (kx row, orbital pair): summing the energy groups,
 //   U[kx][r][pair] = sum_ky Hbar(kx, ky) e^{-2 pi i ry_r ky}  for the distinct ry,
 //   U[kx][nry][pair] = sum_ky Hbar(kx, ky)                       (for the trace)
+// Hacc is [group][pair][k], so a workgroup reads contiguous rows.
 __global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restrict__ Hacc, int NG,
-                                                  const double *__restrict__ ry, int nry, double2 *__restrict__ U)
+                                                  const double *__restrict__ ry, int nry,
+                                                  const double2 *__restrict__ twn, double2 *__restrict__ U)
 {
     __shared__ double r1[256], r2[256];
     const int kx = blockIdx.x, pr = blockIdx.y;
@@ -584,15 +686,10 @@ __global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restr
     for (int r = 0; r <= kMaxRy; r++) acc[r] = make_double2(0.0, 0.0);
     for (int ky = threadIdx.x; ky < K; ky += blockDim.x) {
         double2 hv = make_double2(0.0, 0.0);
-        for (int gi = 0; gi < NG; gi++) hv = cadd(hv, Hacc[((size_t)gi * KK + (long)kx * K + ky) * NP + pr]);
-        const double kyf = kcoord(ky, K);
+        for (int gi = 0; gi < NG; gi++) hv = cadd(hv, Hacc[((size_t)gi * NP + pr) * KK + (long)kx * K + ky]);
 #pragma unroll
-        for (int r = 0; r < kMaxRy; r++) {
-            if (r >= nry) break;
-            double sn, cs;
-            sincos(-2.0 * M_PI * ry[r] * kyf, &sn, &cs);
-            acc[r] = cadd(acc[r], cmul(hv, make_double2(cs, sn)));
-        }
+        for (int r = 0; r < kMaxRy; r++)
+            if (r < nry) acc[r] = cadd(acc[r], cmul(hv, hop_phase(ry[r], ky, K, twn)));
         acc[kMaxRy] = cadd(acc[kMaxRy], hv);
     }
 #pragma unroll
@@ -608,7 +705,8 @@ __global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restr
 // Stage 2, one workgroup per (R, pair): SR[R][pair] = sum_kx U[kx][iry(R)][pair] e^{-2 pi i rx kx};
 // Rl[R] = (rx, iry); the last entry (0, nry) gives the plain sums used for the trace (mu).
 __global__ void __launch_bounds__(256) k_adj_hcol(Geom g, const double2 *__restrict__ U, int nry,
-                                                  const double2 *__restrict__ Rl, double2 *__restrict__ SR)
+                                                  const double2 *__restrict__ Rl, const double2 *__restrict__ twn,
+                                                  double2 *__restrict__ SR)
 {
     __shared__ double r1[256], r2[256];
     const int ridx = blockIdx.x, pr = blockIdx.y;
@@ -616,11 +714,8 @@ __global__ void __launch_bounds__(256) k_adj_hcol(Geom g, const double2 *__restr
     const double rx = Rl[ridx].x;
     const int iry = (int)Rl[ridx].y;
     double2 acc = make_double2(0.0, 0.0);
-    for (int kx = threadIdx.x; kx < K; kx += blockDim.x) {
-        double sn, cs;
-        sincos(-2.0 * M_PI * rx * kcoord(kx, K), &sn, &cs);
-        acc = cadd(acc, cmul(U[((size_t)kx * (nry + 1) + iry) * NP + pr], make_double2(cs, sn)));
-    }
+    for (int kx = threadIdx.x; kx < K; kx += blockDim.x)
+        acc = cadd(acc, cmul(U[((size_t)kx * (nry + 1) + iry) * NP + pr], hop_phase(rx, kx, K, twn)));
     double2 s = block_sum2(acc, r1, r2);
     if (threadIdx.x == 0) SR[(size_t)ridx * NP + pr] = s;
 }
@@ -662,10 +757,33 @@ int launch_win_t(const Geom &g, const double *tiles, const double2 *abar, const 
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
+template <int B, int N>
+int launch_ky_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
+                const double2 *F1bar, const double2 *twn, double2 *Hacc, hipStream_t s)
+{
+    constexpr int NP = B * B;
+    const int NG = cqpi_energy_groups(g, E), EG = (E + NG - 1) / NG;
+    int thr = std::max(NP * cqpi::stockham_lanes(N), N);
+    thr = std::min((thr + 63) / 64 * 64, kAdjWaveMaxThreads);  // >= N: one thread per k-point
+    const size_t lds = sizeof(double2) * (size_t)NP * cqpi::padded_len<double2>(N);
+    hipLaunchKernelGGL((k_adj_ky_s<B, N>), dim3(N, NG), dim3(thr), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
+                       twn, Hacc);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
 template <int B>
 int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                 const double2 *F1bar, const double2 *twn, double2 *Hacc, hipStream_t s)
 {
+    if constexpr (B <= 3) {
+        switch (g.K) {
+        case 64: return launch_ky_s<B, 64>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s);
+        case 128: return launch_ky_s<B, 128>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s);
+        case 256: return launch_ky_s<B, 256>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s);
+        case 512: return launch_ky_s<B, 512>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s);
+        default: break;
+        }
+    }
     size_t lds = sizeof(double2) * ((size_t)B * B * g.K + (size_t)(g.K >> 1));
     const int NG = cqpi_energy_groups(g, E), EG = (E + NG - 1) / NG;
     dim3 grid(g.K, NG);
@@ -745,8 +863,9 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case 4: r = launch_ky_t<4>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
     }
     if (r) return r;
-    hipLaunchKernelGGL(k_adj_hrow, dim3(g.K, NP), dim3(256), 0, s, g, Hacc, cqpi_energy_groups(g, E), ryvals, nry, U);
-    hipLaunchKernelGGL(k_adj_hcol, dim3(nRl, NP), dim3(256), 0, s, g, U, nry, Rl, SR);
+    hipLaunchKernelGGL(k_adj_hrow, dim3(g.K, NP), dim3(256), 0, s, g, Hacc, cqpi_energy_groups(g, E), ryvals, nry,
+                       twn, U);
+    hipLaunchKernelGGL(k_adj_hcol, dim3(nRl, NP), dim3(256), 0, s, g, U, nry, Rl, twn, SR);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
