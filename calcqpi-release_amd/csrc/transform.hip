@@ -171,20 +171,24 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
         const int row = ab ? 2 * h - sy : sy;
         const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
         const C *line = S + (size_t)q * (PAD ? cqpi::padded_len<C>(K) : K);
+        // split in the precision of the lines: for FP32 lines (s a power of
+        // two) v - hi is exact in FP32, so hi + lo/2^12 keeps 22 bits of v
+        using Re = decltype(line[0].x);
+        const Re sr = (Re)s;
         planes_h8 hv, lv;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int sx = gi * 8 + u - mg.OFF;
-            double v = 0.0;
+            Re v = 0;
             if (sx >= 0 && sx < nw) {
                 const int rr = ab ? h - sx : sx - h;
                 const int m = pow2 ? (rr & (K - 1)) : imod(rr, K);
                 const C z = line[PAD ? cqpi::pidx<C>(m) : m];
-                v = (double)(ri ? z.y : z.x) * s;
+                v = (ri ? z.y : z.x) * sr;
             }
             const _Float16 hi = (_Float16)v;
             hv[u] = hi;
-            lv[u] = (_Float16)((v - (double)hi) * 4096.0);
+            lv[u] = (_Float16)((v - (Re)hi) * (Re)4096);
         }
         const size_t o = (size_t)row * mg.Ly + (size_t)gi * 8;
         *reinterpret_cast<planes_h8 *>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + o) = hv;
