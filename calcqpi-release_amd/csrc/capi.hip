@@ -162,6 +162,12 @@ struct cqpi_ctx {
     double2 *d_U = nullptr;
     size_t c_U = 0;
 
+    // uploads of the last prepared run (skipped when unchanged) and the pinned
+    // status read-back buffer
+    std::vector<double> up_omegas, up_scale;
+    int *h_status = nullptr;
+    size_t c_hstatus = 0;
+
     // host staging for the drop-in path (pinned)
     double2 *h_win = nullptr;
     size_t c_hwin = 0;
@@ -192,6 +198,7 @@ void release(cqpi_ctx *c)
         if (p) hipFree(p);
     if (c->h_win) hipHostFree(c->h_win);
     if (c->h_T) hipHostFree(c->h_T);
+    if (c->h_status) hipHostFree(c->h_status);
     for (auto &p : c->pending) {
         hipEventDestroy(p.second.first);
         hipEventDestroy(p.second.second);
@@ -280,6 +287,7 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
         if (!planes_ready) {
             RETCHK(mix_buffers(c, E));
             r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale, c->d_planes, c->stream);
+            c->up_scale.clear();  // the device scales now differ from the last upload
         }
         if (!r)
             r = launch_contract_mixed(g, c->mg, c->d_planes, c->d_afrag, c->d_mscale, c->sA, T, c->d_bulk, E,
@@ -313,10 +321,17 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     const int NP = g.B * g.B;
     size_t WW = (size_t)g.W * g.W;
     RETCHK(dev_grow(c->d_ldos, c->c_ldos, E * WW, c->gpumem));
-    RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
     RETCHK(dev_grow(c->d_T, c->c_T, E * NP, c->gpumem));
     RETCHK(dev_grow(c->d_status, c->c_status, E, c->gpumem));
-    HIPCHK(hipMemcpyAsync(c->d_omega, omegas, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const bool grew = c->c_omega < E || !c->d_omega;
+    RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
+    // energies: uploaded from a member copy (it outlives the asynchronous
+    // copy), and only when they changed
+    if (grew || c->up_omegas.size() != E || !std::equal(omegas, omegas + E, c->up_omegas.begin())) {
+        c->up_omegas.assign(omegas, omegas + E);
+        HIPCHK(hipMemcpyAsync(c->d_omega, c->up_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    }
     // energies per chunk: intermediate of the two-pass transform kept under ~2 GiB
     size_t per_e = (size_t)g.K * g.nw * NP * sizeof(double2);
     size_t Ec = std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / std::max<size_t>(per_e, 1)));
@@ -335,18 +350,21 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     plan.mixed = c->precision == CQPI_PREC_MIXED;
     plan.sG = 1.0;
     if (plan.mixed) {
+        const bool sgrew = c->c_mscale < Ec || !c->d_mscale;
         RETCHK(mix_buffers(c, (int)Ec));
         int ex;
         std::frexp(1.0 / eta, &ex);
         plan.sG = std::ldexp(1.0, 13 - ex);
-        std::vector<double> sv(Ec, plan.sG);
-        HIPCHK(hipMemcpyAsync(c->d_mscale, sv.data(), Ec * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (sgrew || c->up_scale.size() != Ec || c->up_scale[0] != plan.sG) {
+            c->up_scale.assign(Ec, plan.sG);
+            HIPCHK(hipMemcpyAsync(c->d_mscale, c->up_scale.data(), Ec * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+        }
     }
     if (c->precision == CQPI_PREC_FP32) {
         RETCHK(dev_grow(c->d_G0f, c->c_G0f, Ec * g.nw * g.nw * NP, c->gpumem));
         RETCHK(dev_grow(c->d_Tf, c->c_Tf, Ec * NP, c->gpumem));
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
     plan.E = E;
     plan.Ec = Ec;
     plan.eta = eta;
@@ -401,11 +419,17 @@ int finish_run(cqpi_ctx *c, const RunPlan &plan)
     const size_t E = plan.E;
     c->nlayers = E;
     c->windows_kept = c->keep_windows;
-    std::vector<int> status(E);
-    HIPCHK(hipMemcpyAsync(status.data(), c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (c->c_hstatus < E) {
+        if (c->h_status) hipHostFree(c->h_status);
+        c->h_status = nullptr;
+        c->c_hstatus = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_status, E * sizeof(int), hipHostMallocDefault));
+        c->c_hstatus = E;
+    }
+    HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (size_t e = 0; e < E; e++)
-        if (status[e])
+        if (c->h_status[e])
             return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
     return 0;
 }

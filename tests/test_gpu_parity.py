@@ -215,6 +215,25 @@ def test_reduced_precision_matches_oracle(gpu, case, prec):
         assert mx < TOLMIX, (case, e, mx, rms)
 
 
+def test_mixed_run_after_dropin_call_on_one_context(gpu):
+    """The drop-in call rewrites the per-energy plane scales; a later whole-chain
+    run on the same context must not reuse stale ones (and repeats exactly)."""
+    wl = synth.square_3orb(K=32, n=8, E=2)
+    print("case: mixed run / drop-in / run on one context", flush=True)
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    with cqpi.engine_for(wl, "mixed") as q:
+        first = q.run(wl.energies, wl.eta)
+        q.wannierldos(T, gR)
+        q.retrieveResult()
+        again = q.run(wl.energies, wl.eta)
+        assert np.array_equal(first, again)
+        ref = oracle.cldos_run(wl)
+        for e in range(ref.shape[0]):
+            assert rm.rel_errors(again[e], ref[e])[0] < TOLMIX
+
+
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_reduced_precision_dropin_path(gpu, prec):
     wl = synth.square_3orb(K=32, n=12, E=1)
