@@ -152,7 +152,7 @@ template <int B>
 int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                const double *tiles, int E, double *ldos, hipStream_t s)
 {
-    constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 6 ? 2 : 1));
+    constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 12 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
     hipLaunchKernelGGL((k_contract<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
                        tiles, ldos);
