@@ -24,13 +24,43 @@ XCDS, SIMDS = 8, 1024   # MI355X: 8 XCDs, 256 CUs x 4 SIMDs
 
 
 def short_name(name: str) -> str:
-    m = re.search(r"::(k_\w+(?:<[^>(]*>)?)\(", name)
+    """k_name<args> from a demangled or mangled kernel name; the complex line
+    type of the transform kernels is shown as f32 / f64."""
+    name = name.replace("HIP_vector_type<float, 2u>", "f32").replace("HIP_vector_type<double, 2u>", "f64")
+    i = name.find("::k_")
+    if i >= 0:
+        j = i + 2
+        k = j
+        while k < len(name) and (name[k].isalnum() or name[k] == "_"):
+            k += 1
+        out = name[j:k]
+        if k < len(name) and name[k] == "<":
+            depth, m = 0, k
+            while m < len(name):
+                depth += name[m] == "<"
+                depth -= name[m] == ">"
+                m += 1
+                if depth == 0:
+                    break
+            out += name[k:m]
+        return out
+    m = re.search(r"_GLOBAL__N_1(\d+)", name)
     if m:
-        return m.group(1)
-    m = re.search(r"_GLOBAL__N_1\d+(k_\w+?)(I.*?E)?E", name)
-    if m:
-        args = re.findall(r"Li(\d+)E", m.group(2) or "")
-        return m.group(1) + ("<%s>" % ", ".join(args) if args else "")
+        # anonymous namespace "_GLOBAL__N_1", then <length><identifier>
+        rest = name[m.end():]
+        ln = int(m.group(1))
+        ident, tail = rest[:ln], rest[ln:]
+        if not ident.startswith("k_"):
+            return name[:40]
+        args = []
+        if tail.startswith("I"):
+            end = tail.find("EEv")  # template arguments end before the void return type
+            for t in re.findall(r"HIP_vector_typeI([fd])Lj2EE|Li(\d+)E", tail[: end + 1 if end > 0 else None]):
+                if t[0]:
+                    args.append("f32" if t[0] == "f" else "f64")
+                elif t[1]:
+                    args.append(t[1])
+        return ident + ("<%s>" % ", ".join(args) if args else "")
     return name[:40]
 
 
