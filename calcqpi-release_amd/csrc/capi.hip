@@ -79,6 +79,12 @@ struct cqpi_ctx {
     int precision = CQPI_PREC_FP64;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (created on first use): the two halves of an energy chunk
+    // run concurrently in eager runs; batch instances keep to one stream (the
+    // batch graph has its own parallel branches)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool single_stream = false;
     size_t gpumem = 0;
 
     // tiles
@@ -205,6 +211,9 @@ void release(cqpi_ctx *c)
     }
     if (c->copy_done) hipEventDestroy(c->copy_done);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream2) hipStreamDestroy(c->stream2);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -273,33 +282,42 @@ int mix_buffers(cqpi_ctx *c, int E)
 // bulk table + contraction for E energies whose G0 window, T are on the
 // device.  planes_ready: the transform already wrote the mixed-precision
 // planes and scales (whole-chain path); otherwise they are made from G0w.
-int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos, bool planes_ready)
+int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos, bool planes_ready,
+             size_t eo = 0, hipStream_t s = nullptr)
 {
+    // eo: offset of these energies in the chunk buffers (periodic term, planes,
+    // scales, FP32 copies); s: stream (default: the context stream)
     const Geom &g = c->g;
-    RETCHK(dev_grow(c->d_bulk, c->c_bulk, (size_t)E * g.os * g.os, c->gpumem));
+    if (!s) s = c->stream;
+    const int NP = g.B * g.B;
+    RETCHK(dev_grow(c->d_bulk, c->c_bulk, (eo + E) * g.os * g.os, c->gpumem));
+    double *bulk = c->d_bulk + eo * g.os * g.os;
     int st = stage_begin(c, 3);
-    RETCHK(launch_bulk(g, G0w, c->d_tiles, E, c->d_bulk, c->stream));
+    RETCHK(launch_bulk(g, G0w, c->d_tiles, E, bulk, s));
     stage_end(c, st);
     st = stage_begin(c, 4);
     int r = 0;
     switch (c->precision) {
-    case CQPI_PREC_MIXED:
+    case CQPI_PREC_MIXED: {
         if (!planes_ready) {
-            RETCHK(mix_buffers(c, E));
-            r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale, c->d_planes, c->stream);
+            RETCHK(mix_buffers(c, (int)(eo + E)));
+            r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale + eo,
+                                   c->d_planes + eo * mix_plane_halves(g, c->mg), s);
             c->up_scale.clear();  // the device scales now differ from the last upload
         }
         if (!r)
-            r = launch_contract_mixed(g, c->mg, c->d_planes, c->d_afrag, c->d_mscale, c->sA, T, c->d_bulk, E,
-                                      ldos, c->stream);
+            r = launch_contract_mixed(g, c->mg, c->d_planes + eo * mix_plane_halves(g, c->mg), c->d_afrag,
+                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s);
         break;
+    }
     case CQPI_PREC_FP32:
-        RETCHK(dev_grow(c->d_G0f, c->c_G0f, (size_t)E * g.nw * g.nw * g.B * g.B, c->gpumem));
-        RETCHK(dev_grow(c->d_Tf, c->c_Tf, (size_t)E * g.B * g.B, c->gpumem));
-        r = launch_contract_fp32(g, G0w, T, c->d_bulk, c->d_tilesf, E, ldos, c->d_G0f, c->d_Tf, c->stream);
+        RETCHK(dev_grow(c->d_G0f, c->c_G0f, (eo + E) * g.nw * g.nw * NP, c->gpumem));
+        RETCHK(dev_grow(c->d_Tf, c->c_Tf, (eo + E) * NP, c->gpumem));
+        r = launch_contract_fp32(g, G0w, T, bulk, c->d_tilesf, E, ldos, c->d_G0f + eo * g.nw * g.nw * NP,
+                                 c->d_Tf + eo * NP, s);
         break;
     default:
-        r = launch_contract_fp64(g, G0w, T, c->d_bulk, c->d_tiles, E, ldos, c->stream);
+        r = launch_contract_fp64(g, G0w, T, bulk, c->d_tiles, E, ldos, s);
         break;
     }
     stage_end(c, st);
@@ -376,37 +394,70 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
 // device work of a prepared run on c->stream: resolvent, transforms, T-matrix
 // and contraction of every energy chunk.  Launches only (no allocation, copy
 // or synchronisation), so it can be captured into a graph.
-int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
+// energies [e0, e0 + ec) of the current chunk, at offset eo in the chunk
+// buffers, on stream s
+int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo, hipStream_t s)
 {
     const Geom &g = c->g;
     const int NP = g.B * g.B;
-    const size_t WW = (size_t)g.W * g.W, E = plan.E, Ec = plan.Ec;
+    const size_t WW = (size_t)g.W * g.W;
+    const size_t f1e = (size_t)g.K * g.nw * NP, we = (size_t)g.nw * g.nw * NP;
+    double2 *G0w = c->d_G0w + eo * we;
+    _Float16 *planes = plan.mixed ? c->d_planes + eo * mix_plane_halves(g, c->mg) : nullptr;
+    // reduced-precision modes run the resolvent and both transform passes in
+    // FP32 where the Stockham path applies (F1 then holds float2)
+    const bool t32 = c->precision != CQPI_PREC_FP64 && transform32_supported(g);
+    float2 *F1f = reinterpret_cast<float2 *>(c->d_F1) + eo * f1e;
+    double2 *F1d = c->d_F1 + eo * f1e;
+    int st = stage_begin(c, 1);
+    if (t32)
+        RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f, s));
+    else
+        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, F1d, s));
+    stage_end(c, st);
+    st = stage_begin(c, 2);
+    if (t32)
+        RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes, s));
+    else
+        RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
+    stage_end(c, st);
+    st = stage_begin(c, 3);
+    RETCHK(launch_tmatrix(g, G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, s));
+    stage_end(c, st);
+    RETCHK(contract(c, G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed, eo, s));
+    return 0;
+}
+
+// device work of a prepared run: resolvent, transforms, T-matrix and
+// contraction of every energy chunk.  In eager runs a chunk of 8 or more
+// energies is split in two halves that run concurrently on the context's two
+// streams (a latency-bound transform pass of one half overlaps the contraction
+// of the other); with stage timing on, and for batch instances, everything
+// runs in order on the context stream.  Launches only (no
+// allocation, copy or synchronisation, apart from creating the second stream
+// on first use outside a capture), so it can be captured into a graph.
+int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
+{
+    const size_t E = plan.E, Ec = plan.Ec;
     for (size_t e0 = 0; e0 < E; e0 += Ec) {
-        int ec = (int)std::min(Ec, E - e0);
-        // reduced-precision modes run the resolvent and both transform passes in
-        // FP32 where the Stockham path applies (F1 then holds float2)
-        const bool t32 = c->precision != CQPI_PREC_FP64 && transform32_supported(g);
-        float2 *F1f = reinterpret_cast<float2 *>(c->d_F1);
-        int st = stage_begin(c, 1);
-        if (t32)
-            RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f,
-                                         c->stream));
-        else
-            RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, c->d_F1,
-                                       c->stream));
-        stage_end(c, st);
-        st = stage_begin(c, 2);
-        if (t32)
-            RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, c->d_G0w,
-                                         plan.mixed ? c->d_planes : nullptr, c->stream));
-        else
-            RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, c->d_F1, ec, c->d_tw, plan.sG, c->d_G0w,
-                                       plan.mixed ? c->d_planes : nullptr, c->stream));
-        stage_end(c, st);
-        st = stage_begin(c, 3);
-        RETCHK(launch_tmatrix(g, c->d_G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->stream));
-        stage_end(c, st);
-        RETCHK(contract(c, c->d_G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed));
+        const int ec = (int)std::min(Ec, E - e0);
+        if (ec >= 8 && !c->timing && !c->single_stream && !c->stream2) {
+            if (hipStreamCreate(&c->stream2) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+                return fail(CQPI_E_HIP, "second stream/events could not be created");
+        }
+        if (ec >= 8 && !c->timing && !c->single_stream) {
+            const int h1 = (ec + 1) / 2, h2 = ec - h1;
+            HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+            HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+            RETCHK(enqueue_range(c, plan, e0, h1, 0, c->stream));
+            RETCHK(enqueue_range(c, plan, e0 + h1, h2, (size_t)h1, c->stream2));
+            HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        } else {
+            RETCHK(enqueue_range(c, plan, e0, ec, 0, c->stream));
+        }
         c->last_chunk_e0 = e0;
         c->last_chunk_ec = ec;
     }
@@ -1203,6 +1254,7 @@ int cqpi_batch_create(const cqpi_instance *in, size_t count, int precision, cqpi
                         &c);
         if (r) break;
         b->inst.push_back(c);
+        c->single_stream = true;
         r = cqpi_set_precision(c, precision);
         if (!r) r = cqpi_set_model(c, d.hops, d.nhops, d.efermi, d.scmodel);
         if (!r) r = cqpi_set_impurity(c, d.V);

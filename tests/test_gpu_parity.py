@@ -51,6 +51,14 @@ def test_full_chain_spin_hex_reduced(gpu):
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_energy_chunk_split_over_two_streams(gpu, prec):
+    # 9 energies: the run splits them 5 + 4 over the context's two streams
+    wl = synth.square_norb(3, K=64, n=8, E=9, seed=16, w=2, os_=4, eta=0.01)
+    got = _run(wl, prec)
+    _assert_close(got, oracle.cldos_run(wl), TOL64 if prec == "fp64" else TOLMIX)
+
+
 def test_non_power_of_two_kgrid(gpu):
     wl = synth.square_norb(3, K=24, n=8, E=2, seed=12, w=1, os_=2, eta=0.02)
     _assert_close(_run(wl), oracle.cldos_run(wl))

@@ -18,8 +18,8 @@ import refmath as rm
 from cqpi import synth
 
 
-def _job():
-    wl = synth.square_norb(3, K=32, n=8, E=3, seed=41, w=2, os_=4, elo=-0.05, ehi=0.05, eta=0.01)
+def _job(E=3):
+    wl = synth.square_norb(3, K=32, n=8, E=E, seed=41, w=2, os_=4, elo=-0.05, ehi=0.05, eta=0.01)
     wl.efermi = 0.05
     wl.V = np.array([[0.5, 0.1 - 0.05j, 0.0], [0.1 + 0.05j, 0.4, 0.02j], [0.0, -0.02j, 0.3]])
     target = synth.perturbed(wl)
@@ -84,9 +84,11 @@ def _gpu_loss_and_grad(q, wl, tgt):
 
 
 @pytest.mark.gpu
-def test_gradient_matches_torch_autograd(gpu):
-    wl, target = _job()
-    print("case: gradient, 3-orbital square lattice K=32 n=8 w=2 os=4 E=3", flush=True)
+@pytest.mark.parametrize("E", [3, 9])
+def test_gradient_matches_torch_autograd(gpu, E):
+    # E = 9: the forward splits the energies over two streams
+    wl, target = _job(E)
+    print("case: gradient, 3-orbital square lattice K=32 n=8 w=2 os=4 E=%d" % E, flush=True)
     tgt = oracle.cldos_run(target)
     with cqpi.engine_for(wl) as q:
         _, (gh, gmu, gV) = _gpu_loss_and_grad(q, wl, tgt)
