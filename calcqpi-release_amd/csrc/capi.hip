@@ -96,8 +96,8 @@ struct cqpi_ctx {
     bool has_model = false;
     DevRGroup *d_groups = nullptr;
     size_t c_groups = 0;
-    DevHop *d_hops = nullptr;
-    size_t c_hops = 0;
+    double2 *d_tmat = nullptr;  // [group][o1][o2] summed amplitudes
+    size_t c_tmat = 0;
     int ngroups = 0;
     double efermi = 0.0;
     int scmax = 0;
@@ -195,7 +195,7 @@ void release(cqpi_ctx *c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->d_tiles, c->d_groups, c->d_hops, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
+    void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
                     c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
@@ -712,45 +712,43 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
     if (nhops && !hops) return fail(CQPI_E_ARG, "hops is NULL");
     const int B = c->g.B;
-    // group by lattice vector (rx, ry); kz = 0 so rz drops out of the phase
-    std::map<std::pair<double, double>, std::vector<DevHop>> groups;
-    std::vector<std::pair<double, double>> order;
+    // group by lattice vector (rx, ry), 'anti' terms folded onto -R; kz = 0
+    // so rz drops out of the phase.  Each group sums its amplitudes into a
+    // dense B x B block.
+    const int K = c->g.K;
+    std::map<std::pair<double, double>, size_t> index;
+    std::vector<DevRGroup> G;
+    std::vector<double2> T;
     for (size_t i = 0; i < nhops; i++) {
         const cqpi_hopping &h = hops[i];
         if (h.o1 < 0 || h.o2 < 0 || h.o1 >= B || h.o2 >= B)
             return fail(CQPI_E_ARG, "hopping orbital index out of range at term " + std::to_string(i));
-        auto key = std::make_pair(h.rx + 0.0, h.ry + 0.0);
-        auto it = groups.find(key);
-        if (it == groups.end()) {
-            order.push_back(key);
-            it = groups.emplace(key, std::vector<DevHop>()).first;
+        const double rx = (h.anti ? -h.rx : h.rx) + 0.0, ry = (h.anti ? -h.ry : h.ry) + 0.0;
+        auto key = std::make_pair(rx, ry);
+        auto it = index.find(key);
+        if (it == index.end()) {
+            DevRGroup r;
+            r.rx = rx;
+            r.ry = ry;
+            const bool integral = rx == std::floor(rx) && ry == std::floor(ry) && std::fabs(rx) < 1e9 &&
+                                  std::fabs(ry) < 1e9 && K <= 32768;
+            r.integral = integral ? 1 : 0;
+            r.irx = integral ? (int)(((long long)rx % K + K) % K) : 0;
+            r.iry = integral ? (int)(((long long)ry % K + K) % K) : 0;
+            r.pad = 0;
+            it = index.emplace(key, G.size()).first;
+            G.push_back(r);
+            T.resize(T.size() + (size_t)B * B, make_double2(0.0, 0.0));
         }
-        DevHop d;
-        d.o1 = h.o1;
-        d.o2 = h.o2;
-        d.tre = h.tre;
-        d.tim = h.tim;
-        d.anti = h.anti ? 1 : 0;
-        d.pad = 0;
-        it->second.push_back(d);
-    }
-    std::vector<DevRGroup> G;
-    std::vector<DevHop> H;
-    for (auto &key : order) {
-        auto &v = groups[key];
-        DevRGroup r;
-        r.rx = key.first;
-        r.ry = key.second;
-        r.first = (int)H.size();
-        r.count = (int)v.size();
-        G.push_back(r);
-        H.insert(H.end(), v.begin(), v.end());
+        double2 &t = T[it->second * B * B + (size_t)h.o1 * B + h.o2];
+        t.x += h.tre;
+        t.y += h.tim;
     }
     RETCHK(dev_grow(c->d_groups, c->c_groups, std::max<size_t>(G.size(), 1), c->gpumem));
-    RETCHK(dev_grow(c->d_hops, c->c_hops, std::max<size_t>(H.size(), 1), c->gpumem));
+    RETCHK(dev_grow(c->d_tmat, c->c_tmat, std::max<size_t>(T.size(), 1), c->gpumem));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!G.empty()) HIPCHK(hipMemcpy(c->d_groups, G.data(), G.size() * sizeof(DevRGroup), hipMemcpyHostToDevice));
-    if (!H.empty()) HIPCHK(hipMemcpy(c->d_hops, H.data(), H.size() * sizeof(DevHop), hipMemcpyHostToDevice));
+    if (!T.empty()) HIPCHK(hipMemcpy(c->d_tmat, T.data(), T.size() * sizeof(double2), hipMemcpyHostToDevice));
     c->ngroups = (int)G.size();
     c->efermi = efermi;
     c->scmax = scmodel ? (B >> 1) : B;
@@ -788,7 +786,7 @@ int cqpi_compute_eigen(cqpi_ctx *c)
     RETCHK(dev_grow(c->d_eval, c->c_eval, KK * g.B, c->gpumem));
     RETCHK(dev_grow(c->d_evec, c->c_evec, KK * g.B * g.B, c->gpumem));
     int st = stage_begin(c, 0);
-    RETCHK(launch_eigen(g.B, c->d_groups, c->ngroups, c->d_hops, c->efermi, c->scmax, g.K, c->d_eval,
+    RETCHK(launch_eigen(g.B, c->d_groups, c->ngroups, c->d_tmat, c->d_tw, c->efermi, c->scmax, g.K, c->d_eval,
                         c->d_evec, c->stream));
     stage_end(c, st);
     c->eigen_valid = true;
@@ -1206,7 +1204,7 @@ int batch_capture(cqpi_batch *b)
         c->timing = false;
         c->stream = b->branches[k];
         const Geom &g = c->g;
-        r = launch_eigen(g.B, c->d_groups, c->ngroups, c->d_hops, c->efermi, c->scmax, g.K, c->d_eval, c->d_evec,
+        r = launch_eigen(g.B, c->d_groups, c->ngroups, c->d_tmat, c->d_tw, c->efermi, c->scmax, g.K, c->d_eval, c->d_evec,
                          c->stream);
         if (!r) r = enqueue_chunks(c, b->plans[i]);
         c->stream = own;

@@ -45,15 +45,16 @@ __host__ __device__ inline float2 cscale(float s, float2 a) { return make_float2
 
 // Hopping terms grouped by lattice vector: every term of group g shares
 // (rx, ry, rz) and the phase exp(+-2 pi i R.k) is evaluated once per group.
-struct DevHop {
-    int o1, o2;
-    double tre, tim;
-    int anti;
-    int pad;
-};
+// Hopping terms grouped by lattice vector R, 'anti' terms (phase
+// exp(-2 pi i R.k)) folded onto -R, each group's amplitudes summed into one
+// dense B x B block.  For integral R (kept mod K in irx, iry) the phase
+// exp(+2 pi i R.k) at grid point (i, j) is the twiddle exp(+2 pi i m/K),
+// m = (irx i + iry j) mod K.
 struct DevRGroup {
     double rx, ry;
-    int first, count;
+    int irx, iry;
+    int integral;
+    int pad;
 };
 
 // Geometry shared by the transform, T-matrix and contraction kernels.
@@ -85,6 +86,18 @@ __device__ inline void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// t / d by a multiply-high with m = udiv_magic(d) = ceil(2^32 / d).  With
+// m d = 2^32 + r, 0 <= r < d: t m / 2^32 = t/d + t r / (d 2^32), and the
+// second term stays below 1/d (so the floor is exact) for 0 <= t < 2^32 / d.
+__host__ __device__ inline unsigned long long udiv_magic(unsigned d)
+{
+    return ((1ull << 32) + d - 1) / d;
+}
+__device__ inline int udiv(int t, unsigned long long m)
+{
+    return (int)(((unsigned long long)(unsigned)t * m) >> 32);
+}
+
 // Workgroup coordinates of a 3-D grid renumbered for the 8 XCDs of the chip.
 // The dispatcher hands consecutive workgroup ids to the XCDs round-robin, so
 // id L runs on XCD L % 8.  The renumbering v = (ids before XCD L % 8) + L / 8
@@ -110,8 +123,8 @@ struct MixGeom;
 // launchers (defined next to their kernels)
 namespace cqpi {
 
-int launch_eigen(int B, const DevRGroup *groups, int ngroups, const DevHop *hops, double efermi,
-                 int scmax, int K, double *eval, double2 *evec, hipStream_t s);
+int launch_eigen(int B, const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw,
+                 double efermi, int scmax, int K, double *eval, double2 *evec, hipStream_t s);
 
 // resolvent G0(k,w) from eigen data fused with the transform along ky; writes
 // F1[e][slot][i][pair] for the Ry window.  omegas: device array of the chunk.

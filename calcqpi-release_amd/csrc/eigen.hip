@@ -1,7 +1,8 @@
 // Batched per-k Hermitian eigensolve of H(k) on the K x K grid (FP64).
 //
 // One thread per k-point.  H(k) = sum_R t(R) exp(+-2 pi i R.k) - efermi
-// (src/tightbinding.cc:210-219, src/tightbinding.h:128-135) on the grid
+// (src/tightbinding.cc:210-219, src/tightbinding.h:128-135), accumulated as
+// one dense B x B block per lattice vector, on the grid
 // k_i = i/K or (i-K)/K for i > K/2 (src/qpi.cc:465-466), diagonalised by
 // cyclic complex Jacobi rotations.  Eigen data is written structure-of-arrays
 // so that consecutive threads (consecutive k) store consecutive addresses:
@@ -112,8 +113,8 @@ __device__ void jacobi(double2 (&A)[B * B], double2 (&V)[B * B])
 
 template <int B>
 __global__ void __launch_bounds__(128) k_eigen(const DevRGroup *__restrict__ groups, int ngroups,
-                                               const DevHop *__restrict__ hops, double efermi,
-                                               int scmax, int K, double *__restrict__ eval,
+                                               const double2 *__restrict__ tmat, const double2 *__restrict__ tw,
+                                               double efermi, int scmax, int K, double *__restrict__ eval,
                                                double2 *__restrict__ evec)
 {
     const long KK = (long)K * K;
@@ -127,23 +128,25 @@ __global__ void __launch_bounds__(128) k_eigen(const DevRGroup *__restrict__ gro
     for (int r = 0; r < B * B; r++) A[r] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int r = 0; r < B; r++) A[r * B + r].x = (r < scmax) ? -efermi : efermi;
+    const unsigned kmask = (K & (K - 1)) == 0 ? (unsigned)K - 1u : 0u;
     for (int g = 0; g < ngroups; g++) {
-        DevRGroup G = groups[g];
-        double arg = 2.0 * M_PI * (G.rx * kx + G.ry * ky);
-        double sn, cs;
-        sincos(arg, &sn, &cs);
-        for (int m = G.first; m < G.first + G.count; m++) {
-            DevHop hp = hops[m];
-            double2 e = make_double2(cs, hp.anti ? -sn : sn);
-            double2 v = cmul(make_double2(hp.tre, hp.tim), e);
-            if constexpr (B <= kUnrollB) {
-                // runtime index into a register array: select by compare to keep it in VGPRs
+        const DevRGroup G = groups[g];
+        double2 e;
+        if (G.integral) {
+            // irx, iry < K <= 32768: the sum stays below 2^31
+            const unsigned m = (unsigned)G.irx * (unsigned)i + (unsigned)G.iry * (unsigned)j;
+            e = tw[kmask ? (m & kmask) : (m % (unsigned)K)];
+        } else {
+            double sn, cs;
+            sincos(2.0 * M_PI * (G.rx * kx + G.ry * ky), &sn, &cs);
+            e = make_double2(cs, sn);
+        }
+        const double2 *T = tmat + (size_t)g * B * B;
+        if constexpr (B <= kUnrollB) {
 #pragma unroll
-                for (int r = 0; r < B * B; r++)
-                    if (r == hp.o1 * B + hp.o2) A[r] = cadd(A[r], v);
-            } else {
-                A[hp.o1 * B + hp.o2] = cadd(A[hp.o1 * B + hp.o2], v);
-            }
+            for (int r = 0; r < B * B; r++) A[r] = cadd(A[r], cmul(T[r], e));
+        } else {
+            for (int r = 0; r < B * B; r++) A[r] = cadd(A[r], cmul(T[r], e));
         }
     }
     jacobi<B>(A, V);
@@ -154,14 +157,14 @@ __global__ void __launch_bounds__(128) k_eigen(const DevRGroup *__restrict__ gro
 }
 
 template <int B>
-int launch_eigen_t(const DevRGroup *groups, int ngroups, const DevHop *hops, double efermi,
+int launch_eigen_t(const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw, double efermi,
                    int scmax, int K, double *eval, double2 *evec, hipStream_t s)
 {
     long KK = (long)K * K;
     int threads = 128;
     long blocks = (KK + threads - 1) / threads;
     hipLaunchKernelGGL(k_eigen<B>, dim3((unsigned)blocks), dim3(threads), 0, s, groups, ngroups,
-                       hops, efermi, scmax, K, eval, evec);
+                       tmat, tw, efermi, scmax, K, eval, evec);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -169,12 +172,12 @@ int launch_eigen_t(const DevRGroup *groups, int ngroups, const DevHop *hops, dou
 
 namespace cqpi {
 
-int launch_eigen(int B, const DevRGroup *groups, int ngroups, const DevHop *hops, double efermi,
-                 int scmax, int K, double *eval, double2 *evec, hipStream_t s)
+int launch_eigen(int B, const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw,
+                 double efermi, int scmax, int K, double *eval, double2 *evec, hipStream_t s)
 {
     switch (B) {
 #define CASE(b) \
-    case b: return launch_eigen_t<b>(groups, ngroups, hops, efermi, scmax, K, eval, evec, s);
+    case b: return launch_eigen_t<b>(groups, ngroups, tmat, tw, efermi, scmax, K, eval, evec, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
         CASE(11) CASE(12) CASE(14) CASE(16)
 #undef CASE

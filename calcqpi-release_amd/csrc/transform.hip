@@ -10,11 +10,17 @@
 // the window R in [-h, h] (side nw = 2h+1):
 //   pass 1 (one workgroup per kx row i, batch of energies and chunk of
 //          orbital pairs; the row's eigen data is read once for the batch):
-//       S[pair][ky] = G0(k)  ->  FFT along ky  ->  F1[e][Ry slot][i][pair]
-//   pass 2 (one workgroup per (Ry slot, energy, pair chunk), reading one
-//          contiguous block of F1):
+//       S[pair][ky] = G0(k)  ->  FFT along ky  ->  F1
+//   pass 2 (one workgroup per (Ry slot, energy, pair chunk)):
 //       S[pair][kx] = F1     ->  FFT along kx  ->  G0w[e][Rx slot][Ry slot][pair] / K^2
 //       and, for the mixed-precision contraction, the fp16 hi/lo planes.
+// F1 layout: the direct-DFT and radix-4 paths use [e][Ry slot][i][pair], which
+// pass 2 reads as one contiguous block.  The Stockham path uses one block of
+// nw * pairs values per (e, kx row i), holding pass 1's chunk c (pairs
+// [p0, p0 + npc)) at p0 * nw as [Ry slot][pair - p0], so that each pass-1
+// workgroup writes one contiguous run per energy (the slot-major layout
+// scattered its writes in 40-byte pieces, 3.7x slower in isolation); pass 2
+// gathers a slot's pairs from npc-point pieces that neighbouring slots share.
 // Power-of-two K from 64 to 1024 use the register-staged Stockham transform
 // (one line per lane group of one wave, radix 8/4/2 passes, no workgroup
 // barrier between passes); smaller powers of two use the in-LDS radix-4
@@ -260,6 +266,10 @@ constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
 constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in registers
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
+// FP32 pass-1 workgroups are held to 80 VGPRs a lane, so that 6 waves share
+// a SIMD (three workgroups of 512 per CU): measured 20% faster than the
+// compiler's 94 VGPRs (5 waves, two workgroups); FP64 ones keep theirs
+constexpr int kResWaves32 = 6;
 
 // resolvent factor 1/(d + i eta), d = w - e formed in FP64 by the caller
 // (it cancels); FP64 result for double2 lines, FP32 (one reciprocal) for float2
@@ -295,8 +305,11 @@ __device__ inline C to_c(double2 z)
 // pair of the chunk) per group.  0 < B <= kMaxRegB: the eigen data of the
 // thread's k-point (N <= blockDim) stays in registers for the whole energy
 // batch; otherwise (B == 0: band count g.B) it is read per energy.
+template <int B, int N>
+constexpr bool res_regs() { return B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads; }
+
 template <typename C, int B, int N>
-__global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const double *__restrict__ eval,
+__global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 : 1) k_res_ky_s(Geom g, const double *__restrict__ eval,
                                                               const double2 *__restrict__ evec,
                                                               const double *__restrict__ omegas, int E, int EB,
                                                               double eta, const C *__restrict__ tw, int np,
@@ -312,13 +325,14 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
     if constexpr (sizeof(C) == 8)
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
     const C *Tw = sizeof(C) == 8 ? Twl : tw;
-    constexpr bool kRegs = B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads;  // blockDim >= N (launcher)
+    constexpr bool kRegs = res_regs<B, N>();  // blockDim >= N (launcher)
     const int nb = B > 0 ? B : g.B, NP = nb * nb;
     const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
     const int e0 = blockIdx.y * EB, e1 = min(E, e0 + EB);
     const long KK = (long)N * N;
     const int nw = g.nw, h = g.h;
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
+    const unsigned long long dm = udiv_magic(npc);
 
     double ev_r[kRegs ? B : 1];
     C vv_r[kRegs ? B * B : 1];
@@ -383,10 +397,11 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
         lds_barrier();
         for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
         lds_barrier();
-        C *out = F1 + (size_t)e * nw * N * NP + (size_t)i * NP + p0;
+        // the chunk's part of the (e, kx row) block: one contiguous run
+        C *out = F1 + ((size_t)e * N + i) * nw * NP + (size_t)p0 * nw;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-            const int slot = t / npc, q = t - slot * npc;
-            out[(size_t)slot * N * NP + q] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
+            const int slot = udiv(t, dm), q = t - slot * npc;
+            out[t] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
         }
         lds_barrier();  // S is rewritten for the next energy
     }
@@ -396,7 +411,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads) k_res_ky_s(Geom g, const doub
 // the window is written in FP64 whatever C is
 template <typename C, int N>
 __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
-                                                                 const C *__restrict__ tw, int np, double sG,
+                                                                 const C *__restrict__ tw, int np, int np1, double sG,
                                                                  double2 *__restrict__ G0w, _Float16 *__restrict__ P)
 {
     extern __shared__ __align__(16) unsigned char smem_s[];
@@ -409,14 +424,23 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     if constexpr (sizeof(C) == 8)
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
     const C *Tw = sizeof(C) == 8 ? Twl : tw;
-    const int sy = blockIdx.x, e = blockIdx.y, B = g.B, NP = B * B, p0 = blockIdx.z * np;
+    // neighbouring Ry slots read the same cache lines of F1: keep them on one XCD
+    int sy, e, z;
+    xcd_block(sy, e, z);
+    const int B = g.B, NP = B * B, p0 = z * np;
     const int npc = min(np, NP - p0);
     const int nw = g.nw, h = g.h;
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
-    const C *in = F1 + ((size_t)e * nw + sy) * N * NP + p0;
+    // pair p sits in pass-1 chunk c = p / np1, at c np1 nw + slot npc1(c) + p - c np1
+    // of its (e, kx row) block
+    const C *in = F1 + (size_t)e * N * nw * NP;
+    const unsigned long long dm = udiv_magic(npc);
     for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
-        const int i = t / npc, q = t - i * npc;
-        S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * NP + q];
+        const int i = udiv(t, dm), q = t - i * npc, p = p0 + q;
+        int c0 = 0;
+        while (c0 + np1 <= p) c0 += np1;
+        const int npc1 = min(np1, NP - c0);
+        S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * nw * NP + (size_t)c0 * nw + sy * npc1 + (p - c0)];
     }
     lds_barrier();
     for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
@@ -452,7 +476,7 @@ constexpr size_t kWaveLds = 96 * 1024;
 int wave_pairs(int N, int NP, size_t csize)
 {
     const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
-    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : N)))));
+    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -469,12 +493,16 @@ template <typename C, int B, int N>
 int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                  const C *tw, C *F1, hipStream_t s)
 {
-    const int NP = g.B * g.B, np = wave_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
+    const int NP = g.B * g.B;
+    const int np = wave_pairs(N, NP, sizeof(C));
+    const int nch = (NP + np - 1) / np;
     const int thr = wave_threads(N, np, true);
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
-    // energies per workgroup: enough workgroups for ~2 per CU
+    // energies per workgroup: FP32 lines, with several workgroups resident
+    // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2
     const int rows = N * nch;
-    const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / 512));
+    const int wg_target = sizeof(C) == 8 ? 4096 : 512;
+    const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / wg_target));
     const int nb = (E + EB - 1) / EB;
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
@@ -501,7 +529,7 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
 // they fit the LDS budget (a split would fetch each row's cache lines twice)
 int kx_pairs(int N, int NP, size_t csize)
 {
-    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : N)))));
+    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -511,9 +539,10 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
                 _Float16 *P, hipStream_t s)
 {
     const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
+    const int np1 = wave_pairs(N, NP, sizeof(C));  // pass 1's chunks fix the F1 layout
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
-    hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, sG, G0w, P);
+    hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, sG, G0w, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
