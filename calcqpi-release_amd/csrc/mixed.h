@@ -20,6 +20,15 @@ struct MixGeom {
     int Ly;     // plane row length (halves)
 };
 
+// Element (row, col) of one fp16 plane of nw rows and Ly columns.  Columns
+// come in groups of 8 and each group keeps its rows contiguous,
+// [col / 8][row][col % 8], so that the contraction's staging of a strip of
+// columns over a run of rows reads contiguous memory.
+__host__ __device__ inline size_t plane_off(int row, int col, int nw)
+{
+    return ((size_t)(col >> 3) * nw + row) * 8 + (col & 7);
+}
+
 namespace cqpi {
 MixGeom mix_geometry(const Geom &g);
 size_t mix_plane_halves(const Geom &g, const MixGeom &mg);

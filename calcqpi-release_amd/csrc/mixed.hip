@@ -64,10 +64,11 @@ __global__ void __launch_bounds__(1024) k_wscale(const double2 *__restrict__ G0w
     }
 }
 
-// planes P[e][ab][hl][(o*B+p)*2+ri][row sy][Ly], rows along x: ab=0 holds
-// G0(R)[o][p], ab=1 holds G0(-R)[p][o], R = (idx-OFF-h, sy-h); entries outside
-// the window are zero.  (The whole-chain path writes these planes from the
-// transform kernel; this kernel serves the GPUQPI drop-in path.)
+// planes P[e][ab][hl][(o*B+p)*2+ri] of nw rows sy and Ly columns idx (laid
+// out by plane_off), rows along x: ab=0 holds G0(R)[o][p], ab=1 holds
+// G0(-R)[p][o], R = (idx-OFF-h, sy-h); entries outside the window are zero.
+// (The whole-chain path writes these planes from the transform kernel; this
+// kernel serves the GPUQPI drop-in path.)
 __global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
                        const double *__restrict__ scale, _Float16 *__restrict__ P)
 {
@@ -78,8 +79,8 @@ __global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
     const double s = scale[e];
     const double2 *G = G0w + (size_t)e * nw * nw * NP;
     const size_t plane = (size_t)mg.Ly * nw;
-    _Float16 *hi = P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane + (size_t)sy * mg.Ly;
-    _Float16 *lo = P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane + (size_t)sy * mg.Ly;
+    _Float16 *hi = P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane;
+    _Float16 *lo = P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane;
     for (int idx = threadIdx.x; idx < mg.Ly; idx += blockDim.x) {
         int sx = idx - mg.OFF;
         double v = 0.0;
@@ -89,29 +90,33 @@ __global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
             v = (ri ? z.y : z.x) * s;
         }
         _Float16 h = (_Float16)v;
-        hi[idx] = h;
-        lo[idx] = (_Float16)((v - (double)h) * kLoScale);
+        hi[plane_off(sy, idx, nw)] = h;
+        lo[plane_off(sy, idx, nw)] = (_Float16)((v - (double)h) * kLoScale);
     }
 }
 
+// the 8 halves at columns col .. col+7 of row `row` of plane pl (col a
+// multiple of LDW; the run may span two column groups of plane_off)
 template <int LDW>
-__device__ inline half8 load8(const _Float16 *p)
+__device__ inline half8 load8(const _Float16 *pl, int row, int col, int nw)
 {
     half8 r;
     if constexpr (LDW == 8) {
-        uint4 u = *(const uint4 *)p;
+        uint4 u = *(const uint4 *)(pl + plane_off(row, col, nw));
         __builtin_memcpy(&r, &u, 16);
     } else if constexpr (LDW == 4) {
-        uint2 u0 = *(const uint2 *)p, u1 = *(const uint2 *)(p + 4);
+        uint2 u0 = *(const uint2 *)(pl + plane_off(row, col, nw)), u1 = *(const uint2 *)(pl + plane_off(row, col + 4, nw));
         uint4 u = make_uint4(u0.x, u0.y, u1.x, u1.y);
         __builtin_memcpy(&r, &u, 16);
     } else if constexpr (LDW == 2) {
-        const unsigned *q = (const unsigned *)p;
-        uint4 u = make_uint4(q[0], q[1], q[2], q[3]);
+        uint4 u = make_uint4(*(const unsigned *)(pl + plane_off(row, col, nw)),
+                             *(const unsigned *)(pl + plane_off(row, col + 2, nw)),
+                             *(const unsigned *)(pl + plane_off(row, col + 4, nw)),
+                             *(const unsigned *)(pl + plane_off(row, col + 6, nw)));
         __builtin_memcpy(&r, &u, 16);
     } else {
 #pragma unroll
-        for (int j = 0; j < 8; j++) r[j] = p[j];
+        for (int j = 0; j < 8; j++) r[j] = pl[plane_off(row, col + j, nw)];
     }
     return r;
 }
@@ -192,9 +197,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                         pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
                     }
                     if (kok && ab >= 0) {
-                        const size_t off = (size_t)row * mg.Ly + col0 + g8 * 8;
-                        bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + off);
-                        bl[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + off);
+                        bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
+                        bl[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
                     } else {
                         bh[t] = half8{0, 0, 0, 0, 0, 0, 0, 0};
                         bl[t] = bh[t];
@@ -265,7 +269,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
 // KS): the A fragments of every (jj, ks) stay in registers and each wave walks
 // TPW tiles along y, so A is read from memory once per wave instead of once
 // per tile.  Non-spin models only (one orbital block).
-template <int NT, int LDW, int OS, int KS, int TPW, int BT>
+template <int NT, int LDW, int OS, int KS, int TPW, int BT, int JS>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                          const half8 *__restrict__ Afrag,
                                                          const double *__restrict__ scale, double sA,
@@ -273,10 +277,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
                                                          const double *__restrict__ bulk,
                                                          double *__restrict__ ldos)
 {
+    // JS waves share a cell row, wave w taking the sub-pixel rows
+    // jj in [j0, j0 + OSW): fewer tile fragments per wave, more waves per SIMD
+    constexpr int OSW = OS / JS, CPS = kWaves / JS;  // cell rows per step
     __shared__ float2 Tf[BT * BT];  // T in FP32 for the epilogue
-    __shared__ float Cs[kWaves][OS][16][16 * NT + 1];
+    __shared__ float Cs[kWaves][OSW][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int j0 = (wave % JS) * OSW, wrow = wave / JS;
     int bx, by, e;
     xcd_block(bx, by, e);
     const int B = g.B, w = g.w, n = g.n, h = g.h, nw = g.nw;
@@ -285,12 +293,12 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     const double2 *Te = Tm + (size_t)e * B * B;
     for (int i = threadIdx.x; i < BT * BT; i += blockDim.x) Tf[i] = make_float2((float)Te[i].x, (float)Te[i].y);
 
-    half8 ah[OS][KS], al[OS][KS];
+    half8 ah[OSW][KS], al[OSW][KS];
 #pragma unroll
-    for (int jj = 0; jj < OS; jj++)
+    for (int jj = 0; jj < OSW; jj++)
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const size_t fa = (((size_t)jj * KS + ks) * 2) * 64;
+            const size_t fa = (((size_t)(j0 + jj) * KS + ks) * 2) * 64;
             ah[jj][ks] = Afrag[fa + lane];
             al[jj][ks] = Afrag[fa + 64 + lane];
         }
@@ -313,42 +321,76 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
     // different banks.
     extern __shared__ __align__(16) unsigned char smem_m[];
     _Float16 *Pl = reinterpret_cast<_Float16 *>(smem_m);
-    const int RW = TPW * kWaves + 2 * w, RP = RW | 1, seg = mg.NUg;
-    const int cy_lo = by * TPW * kWaves;
+    const int RW = TPW * CPS + 2 * w, RP = RW | 1, seg = mg.NUg;
+    const int cy_lo = by * TPW * CPS;
     const int row_lo = cy_lo - n2 - w + h;
+    // (consecutive threads take consecutive rows: contiguous in the planes)
     for (int l = threadIdx.x; l < 4 * mg.NPL * RW * seg; l += blockDim.x) {
-        const int g8 = l % seg, rest = l / seg, r = rest % RW, pid = rest / RW;
+        const int r = l % RW, rest = l / RW, g8 = rest % seg, pid = rest / seg;
         const int row = row_lo + r;
         half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
-        if (row < nw) v = load8<LDW>(Pe + (size_t)pid * plane + (size_t)row * mg.Ly + col0 + g8 * 8);
+        if (row < nw) v = load8<LDW>(Pe + (size_t)pid * plane, row, col0 + g8 * 8, nw);
         *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
     }
+    if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + 4 * mg.NPL * RP * seg * 8) = half8{0, 0, 0, 0, 0, 0, 0, 0};
     __syncthreads();
 
-    // per lane: the K group it loads and the plane offsets of its N columns
-    int kdy[KS], kg8[KS];
-    bool kok[KS];
+    // per lane and K step: the staged offset (halves) of its B fragment in the
+    // hi plane at tile row 0, or -1 where the lane's K group or N column is
+    // padding (then it reads the zero slot after the staged planes)
+    const int LOFF = mg.NPL * RP * seg * 8;   // hi -> lo plane of the same (o, p)
+    const int ZOFF = 4 * mg.NPL * RP * seg * 8;
+    int qoff[KS][NT];
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
         const int kg = ks * 4 + kq;
-        kok[ks] = kg < mg.KG;
+        const bool kok = kg < mg.KG;
         const int g8 = kg % mg.NUg, r = kg / mg.NUg;
-        kg8[ks] = g8;
-        kdy[ks] = r / mg.nper;
-        const int ol = r % mg.nper;
-        kdy[ks] = kok[ks] ? kdy[ks] : 0;
-        kg8[ks] = kok[ks] ? g8 : 0;
-        // stash the orbital in the upper bits of kdy
-        kdy[ks] |= (kok[ks] ? ol : 0) << 16;
+        const int dy = r / mg.nper, o = r % mg.nper;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int nn = t * 16 + col;
+            int ab = -1, pl = 0;
+            if (nn < 2 * B) {
+                ab = 0;
+                pl = (o * B + (nn >> 1)) * 2 + (nn & 1);
+            } else if (nn < 4 * B) {
+                ab = 1;
+                int m2 = nn - 2 * B;
+                pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
+            }
+            const int pid = (ab * 2) * mg.NPL + pl;
+            qoff[ks][t] = (kok && ab >= 0) ? (pid * RP + dy) * seg * 8 + g8 * 8 : -1;
+        }
+    }
+    // the periodic (bulk) term of each pixel this lane finishes
+    constexpr int NJB = (OSW + 3) / 4;
+    double bk[NJB];
+#pragma unroll
+    for (int jb = 0; jb < NJB; jb++) {
+        const int jl = jb * 4 + (lane >> 4), ii = (lane & 15) % OS;
+        bk[jb] = (jl < OSW) ? bulk[(size_t)e * OS * OS + ii * OS + j0 + jl] : 0.0;
     }
 
     for (int tt = 0; tt < TPW; tt++) {
-        const int cy_raw = (by * TPW + tt) * kWaves + wave;
+        const int cy_raw = (by * TPW + tt) * CPS + wrow;
         if (cy_raw >= n) break;  // uniform per wave; no block-wide barrier follows
         const int cy = cy_raw;
-        float4v hh[OS][NT], cr[OS][NT];
+        const int trow = (cy - cy_lo) * seg * 8;
+        // all B fragments of the tile first (unconditional LDS reads), then the MFMAs
+        half8 bh[KS][NT], bl[KS][NT];
 #pragma unroll
-        for (int i = 0; i < OS; i++)
+        for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int q = qoff[ks][t];
+                const int a0 = q >= 0 ? q + trow : ZOFF, a1 = q >= 0 ? q + trow + LOFF : ZOFF;
+                bh[ks][t] = *reinterpret_cast<const half8 *>(Pl + a0);
+                bl[ks][t] = *reinterpret_cast<const half8 *>(Pl + a1);
+            }
+        float4v hh[OSW][NT], cr[OSW][NT];
+#pragma unroll
+        for (int i = 0; i < OSW; i++)
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
@@ -356,44 +398,18 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
             }
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const int dy = kdy[ks] & 0xffff, o = kdy[ks] >> 16;
-            const int rrow = cy - cy_lo + dy;  // staged row
-            half8 bh[NT], bl[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const int nn = t * 16 + col;
-                int ab = -1, pl = 0;
-                if (nn < 2 * B) {
-                    ab = 0;
-                    pl = (o * B + (nn >> 1)) * 2 + (nn & 1);
-                } else if (nn < 4 * B) {
-                    ab = 1;
-                    int m2 = nn - 2 * B;
-                    pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
-                }
-                if (kok[ks] && ab >= 0) {
-                    const int pid = (ab * 2) * mg.NPL + pl;
-                    const _Float16 *q0 = Pl + ((size_t)pid * RP + rrow) * seg * 8 + kg8[ks] * 8;
-                    const _Float16 *q1 = q0 + (size_t)mg.NPL * RP * seg * 8;  // lo plane of the same (o, p)
-                    bh[t] = *reinterpret_cast<const half8 *>(q0);
-                    bl[t] = *reinterpret_cast<const half8 *>(q1);
-                } else {
-                    bh[t] = half8{0, 0, 0, 0, 0, 0, 0, 0};
-                    bl[t] = bh[t];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < OS; i++)
+            for (int i = 0; i < OSW; i++)
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
-                    hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bh[t], hh[i][t], 0, 0, 0);
-                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bl[t], cr[i][t], 0, 0, 0);
-                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i][ks], bh[t], cr[i][t], 0, 0, 0);
+                    hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bh[ks][t], hh[i][t], 0, 0, 0);
+                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i][ks], bl[ks][t], cr[i][t], 0, 0, 0);
+                    cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i][ks], bh[ks][t], cr[i][t], 0, 0, 0);
                 }
         }
         // epilogue (wave-local)
 #pragma unroll
-        for (int i = 0; i < OS; i++)
+        for (int i = 0; i < OSW; i++)
 #pragma unroll
             for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -403,11 +419,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int jb = 0; jb < (OS + 3) / 4; jb++) {
-            const int jj = jb * 4 + (lane >> 4), m = lane & 15;
-            if (jj < OS && m < rows) {
+        for (int jb = 0; jb < (OSW + 3) / 4; jb++) {
+            const int jl = jb * 4 + (lane >> 4), jj = j0 + jl, m = lane & 15;
+            if (jl < OSW && m < rows) {
                 // Im sum_pq a_p T_pq b_q: (T b)_p in FP32, the sum over p in FP64
-                const float *Cr = Cs[wave][jj][m];
+                const float *Cr = Cs[wave][jl][m];
                 float cv[4 * BT];
 #pragma unroll
                 for (int k = 0; k < 4 * BT; k++) cv[k] = Cr[k];
@@ -427,7 +443,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
                 const int c = m / OS, ii = m - c * OS, cx = cx0 + c;
                 if (cx < n) {
                     const int X = cx * OS + ii, Y = cy * OS + jj;
-                    out[(size_t)X + (size_t)Y * W] = -(acc * unscale + bulk[(size_t)e * OS * OS + ii * OS + jj]);
+                    out[(size_t)X + (size_t)Y * W] = -(acc * unscale + bk[jb]);
                 }
             }
         }
@@ -439,26 +455,27 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, c
 
 constexpr int kTPW = 8;  // default cell rows per wave
 
-template <int NT, int LDW, int OS, int KS, int TPW, int BT>
+template <int NT, int LDW, int OS, int KS, int TPW, int BT, int JS>
 void launch_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
                  double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
-    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves * TPW - 1) / (kWaves * TPW), E);
-    const int RP = (TPW * kWaves + 2 * g.w) | 1;
-    const size_t lds = (size_t)4 * mg.NPL * RP * mg.NUg * 8 * sizeof(_Float16);
-    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW, BT>), grid, dim3(64 * kWaves), lds, s, g, mg, P, A, scale, sA,
-                       T, bulk, ldos);
+    constexpr int CPS = kWaves / JS;
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + CPS * TPW - 1) / (CPS * TPW), E);
+    const int RP = (TPW * CPS + 2 * g.w) | 1;
+    const size_t lds = ((size_t)4 * mg.NPL * RP * mg.NUg * 8 + 8) * sizeof(_Float16);  // + the zero slot
+    hipLaunchKernelGGL((k_cmix_areg<NT, LDW, OS, KS, TPW, BT, JS>), grid, dim3(64 * kWaves), lds, s, g, mg, P, A, scale,
+                       sA, T, bulk, ldos);
 }
 
-template <int NT, int LDW, int OS, int KS, int BT>
+template <int NT, int LDW, int OS, int KS, int BT, int JS>
 bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale,
               double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
 {
     if (g.os != OS || g.B != BT || mg.KS != KS || mg.NT != NT || mg.nsets != 1 || mg.LDW != LDW) return false;
     switch (g.tpw) {
-    case 2: launch_areg<NT, LDW, OS, KS, 2, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-    case 4: launch_areg<NT, LDW, OS, KS, 4, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-    default: launch_areg<NT, LDW, OS, KS, kTPW, BT>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 2: launch_areg<NT, LDW, OS, KS, 2, BT, JS>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 4: launch_areg<NT, LDW, OS, KS, 4, BT, JS>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+    default: launch_areg<NT, LDW, OS, KS, kTPW * JS, BT, JS>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
     }
     return true;
 }
@@ -565,11 +582,13 @@ int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, c
 {
     const half8 *A = (const half8 *)Afrag;
     // register-resident tile fragments for the common small shapes:
-    // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital)
-    if (try_areg<1, 4, 4, 4, 3>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<1, 4, 4, 2, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<2, 4, 4, 7, 5>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<1, 2, 8, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
+    // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital).
+    // 3 orbitals: two waves share a cell row (half the fragments each, three
+    // waves a SIMD instead of two: 4% faster)
+    if (try_areg<1, 4, 4, 4, 3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 4, 4, 2, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<2, 4, 4, 7, 5, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+        try_areg<1, 2, 8, 1, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     switch (mg.NT) {
     case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);

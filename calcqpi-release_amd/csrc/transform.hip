@@ -156,8 +156,9 @@ typedef _Float16 planes_h8 __attribute__((ext_vector_type(8)));
 // transformed lines S[q][m] (window point R at m = R mod K), scaled by s:
 //   a planes: row sy,      column sx  <- G0(sx, sy)[o][p]
 //   b planes: row 2h - sy, column sx  <- G0(2h - sx, sy)[p][o]   (= G0(-R)^T)
-// Each item fills 8 consecutive columns (row length Ly is a multiple of 8)
-// and stores them as one 16-byte hi and one 16-byte lo vector.
+// Each item fills 8 consecutive columns, one column group of plane_off (Ly
+// is a multiple of 8), and stores them as one 16-byte hi and one 16-byte lo
+// vector.
 template <typename C, bool PAD>
 __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
                                     const MixGeom &mg, double s, _Float16 *__restrict__ P)
@@ -196,7 +197,7 @@ __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, in
             hv[u] = hi;
             lv[u] = (_Float16)((v - (Re)hi) * (Re)4096);
         }
-        const size_t o = (size_t)row * mg.Ly + (size_t)gi * 8;
+        const size_t o = plane_off(row, gi * 8, nw);
         *reinterpret_cast<planes_h8 *>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane + o) = hv;
         *reinterpret_cast<planes_h8 *>(Pe + ((size_t)(ab * 2 + 1) * mg.NPL + pl) * plane + o) = lv;
     }
@@ -499,10 +500,12 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int thr = wave_threads(N, np, true);
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     // energies per workgroup: FP32 lines, with several workgroups resident
-    // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2
+    // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2.  The
+    // energy blocks are made even (51 energies: 4 blocks of 13, not 12,12,12,12,3)
     const int rows = N * nch;
     const int wg_target = sizeof(C) == 8 ? 4096 : 512;
-    const int EB = g.eb > 0 ? std::min(E, g.eb) : std::max(1, std::min(E, (rows * E) / wg_target));
+    const int nb_want = std::max(1, std::min(E, (wg_target + rows / 2) / rows));
+    const int EB = g.eb > 0 ? std::min(E, g.eb) : (E + nb_want - 1) / nb_want;
     const int nb = (E + EB - 1) / EB;
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
