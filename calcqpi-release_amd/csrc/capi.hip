@@ -120,6 +120,8 @@ struct cqpi_ctx {
     // per-chunk buffers
     double2 *d_F1 = nullptr;
     size_t c_F1 = 0;
+    double2 *d_gkw = nullptr;  // G(k) workspace of the many-band resolvent path
+    size_t c_gkw = 0;
     double2 *d_G0w = nullptr;
     size_t c_G0w = 0;
     double *d_bulk = nullptr;
@@ -196,7 +198,7 @@ void release(cqpi_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
-                    c->d_F1, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
+                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
@@ -330,6 +332,7 @@ struct RunPlan {
     size_t E = 0, Ec = 0;
     double eta = 0.0, sG = 1.0;
     bool mixed = false;
+    size_t work_bytes = 0;  // G(k) workspace of the resolvent pass (0: not used)
 };
 
 // allocations and uploads of a run (synchronous); no kernel launch
@@ -363,6 +366,9 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, Ec * g.os * g.os, c->gpumem));
+    plan.work_bytes = resolvent_work_bytes(g, (int)Ec, c->precision != CQPI_PREC_FP64 && transform32_supported(g));
+    if (plan.work_bytes)
+        RETCHK(dev_grow(c->d_gkw, c->c_gkw, (plan.work_bytes + sizeof(double2) - 1) / sizeof(double2), c->gpumem));
     // mixed precision: the k->x transform writes the fp16 planes directly.  Their
     // scale comes from |G0(R)_ab| <= ||G0(k)|| <= 1/eta, a bound for every R.
     plan.mixed = c->precision == CQPI_PREC_MIXED;
@@ -411,9 +417,11 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     double2 *F1d = c->d_F1 + eo * f1e;
     int st = stage_begin(c, 1);
     if (t32)
-        RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f, s));
+        RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f, c->d_gkw,
+                                     plan.work_bytes, s));
     else
-        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, F1d, s));
+        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, F1d, c->d_gkw,
+                                   plan.work_bytes, s));
     stage_end(c, st);
     st = stage_begin(c, 2);
     if (t32)
@@ -441,13 +449,15 @@ int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
     const size_t E = plan.E, Ec = plan.Ec;
     for (size_t e0 = 0; e0 < E; e0 += Ec) {
         const int ec = (int)std::min(Ec, E - e0);
-        if (ec >= 8 && !c->timing && !c->single_stream && !c->stream2) {
+        // (the G(k) workspace path has one workspace: no split)
+        const bool split = ec >= 8 && !c->timing && !c->single_stream && !plan.work_bytes;
+        if (split && !c->stream2) {
             if (hipStreamCreate(&c->stream2) != hipSuccess ||
                 hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
                 return fail(CQPI_E_HIP, "second stream/events could not be created");
         }
-        if (ec >= 8 && !c->timing && !c->single_stream) {
+        if (split) {
             const int h1 = (ec + 1) / 2, h2 = ec - h1;
             HIPCHK(hipEventRecord(c->ev_fork, c->stream));
             HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));

@@ -127,10 +127,14 @@ int launch_eigen(int B, const DevRGroup *groups, int ngroups, const double2 *tma
                  double efermi, int scmax, int K, double *eval, double2 *evec, hipStream_t s);
 
 // resolvent G0(k,w) from eigen data fused with the transform along ky; writes
-// F1[e][slot][i][pair] for the Ry window.  omegas: device array of the chunk.
+// F1 (the Ry window of every kx row).  omegas: device array of the chunk.
+// work: workspace of resolvent_work_bytes(g, E, ...) bytes (G(k) of a block
+// of energies, for band counts or grids beyond the register path; may be
+// NULL when that is 0).
 int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
                         const double *omegas, int E, double eta, const double2 *tw,
-                        double2 *F1, hipStream_t s);
+                        double2 *F1, void *work, size_t work_bytes, hipStream_t s);
+size_t resolvent_work_bytes(const Geom &g, int E, bool fp32);
 // transform along kx, window extraction and 1/K^2: G0w[e][sx][sy][pair]; with
 // mg != NULL also the fp16 hi/lo planes of the mixed contraction (scale sG)
 int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,
@@ -139,7 +143,7 @@ int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int
 // [64, 1024]); the resolvent factor is formed in FP64, the window written in FP64
 bool transform32_supported(const Geom &g);
 int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
-                          double eta, const float2 *tw, float2 *F1, hipStream_t s);
+                          double eta, const float2 *tw, float2 *F1, void *work, size_t work_bytes, hipStream_t s);
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
                           double2 *G0w, _Float16 *P, hipStream_t s);
 // T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular

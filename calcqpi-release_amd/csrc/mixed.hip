@@ -122,6 +122,10 @@ __device__ inline half8 load8(const _Float16 *pl, int row, int col, int nw)
 }
 
 constexpr int kWaves = 4;
+// waves per SIMD the register-resident contraction is compiled for: three for
+// up to 3 orbitals (the LDS of three workgroups fits a CU; the compiler then
+// keeps the accumulators in VGPRs instead of AGPRs), one for more
+constexpr int areg_waves(int bt) { return bt <= 3 ? 3 : 1; }
 constexpr int kJB = 4;  // sub-pixel rows (jj) per accumulator block
 
 // one wave = one tile: CB cells along x starting at cx0, one cell row cy,
@@ -270,7 +274,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
 // TPW tiles along y, so A is read from memory once per wave instead of once
 // per tile.  Non-spin models only (one orbital block).
 template <int NT, int LDW, int OS, int KS, int TPW, int BT, int JS>
-__global__ void __launch_bounds__(64 * kWaves) k_cmix_areg(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(areg_waves(BT), areg_waves(BT)))) k_cmix_areg(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                          const half8 *__restrict__ Afrag,
                                                          const double *__restrict__ scale, double sA,
                                                          const double2 *__restrict__ Tm,

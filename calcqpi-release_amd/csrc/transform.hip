@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
                                                               const double2 *__restrict__ evec,
                                                               const double *__restrict__ omegas, int E, int EB,
                                                               double eta, const C *__restrict__ tw, int np,
-                                                              C *__restrict__ F1)
+                                                              C *__restrict__ F1, const C *__restrict__ Gk)
 {
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
@@ -373,26 +373,12 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
             }
         }
         if constexpr (!kRegs) {
-            // one resolvent factor per band, accumulated into the chunk's pairs
-            for (int j = threadIdx.x; j < N; j += blockDim.x) {
-                const long k = (long)i * N + j;
-                C acc[kMaxLines];
-#pragma unroll
-                for (int q = 0; q < kMaxLines; q++) acc[q] = to_c<C>(make_double2(0.0, 0.0));
-                for (int n = 0; n < nb; n++) {
-                    const C f = res_factor<C>(omega - eval[n * KK + k], eta);
-#pragma unroll
-                    for (int q = 0; q < kMaxLines; q++) {
-                        if (q < npc) {
-                            const int pr = p0 + q, r = pr / nb, c = pr - r * nb;
-                            acc[q] = cadd(acc[q], cmul(f, cmulc(to_c<C>(evec[(r * nb + n) * KK + k]),
-                                                                to_c<C>(evec[(c * nb + n) * KK + k]))));
-                        }
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kMaxLines; q++)
-                    if (q < npc) S[(size_t)q * NL + cqpi::pidx<C>(j)] = acc[q];
+            // G(k) of this energy and the chunk's pairs from the workspace
+            // (k_gk), one line per pair, read along ky
+            const C *gk = Gk + ((size_t)e * NP + p0) * KK + (size_t)i * N;
+            for (int t = threadIdx.x; t < npc * N; t += blockDim.x) {
+                const int q = t / N, j = t - q * N;
+                S[(size_t)q * NL + cqpi::pidx<C>(j)] = gk[(size_t)q * KK + j];
             }
         }
         lds_barrier();
@@ -405,6 +391,83 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
             out[t] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
         }
         lds_barrier();  // S is rewritten for the next energy
+    }
+}
+
+// G(k) of every orbital pair for band counts and grids that the register
+// path of pass 1 does not cover: one workgroup per TK consecutive k-points of
+// one kx row, their eigen data staged in LDS once for a block of energies,
+// thread (kk, r) forms row r, G_rc = sum_n f_n v_rn conj(v_cn) (f_n the
+// resolvent factor, formed in FP64 and rounded as in pass 1).  Writes
+// Gk[e][pair][k], contiguous along ky for pass 1's line loads.
+template <typename C, int B>
+constexpr int gk_tile()
+{
+    // k-points per workgroup: the staged vectors within 48 KB, 8 to 64
+    int t = 64;
+    while (t > 8 && (size_t)t * B * B * sizeof(C) + (size_t)t * B * sizeof(double) > 48 * 1024) t >>= 1;
+    return t;
+}
+
+template <typename C, int B>
+__global__ void __launch_bounds__((gk_tile<C, B>() * B)) k_gk(int N, const double *__restrict__ eval,
+                                                            const double2 *__restrict__ evec,
+                                                            const double *__restrict__ omegas, int E, double eta,
+                                                            C *__restrict__ Gk)
+{
+    constexpr int TK = gk_tile<C, B>();
+    __shared__ C Vs[B * B][TK];
+    __shared__ double Es[B][TK];
+    const long KK = (long)N * N;
+    const long k0 = (long)blockIdx.y * N + (long)blockIdx.x * TK;
+    for (int t = threadIdx.x; t < B * B * TK; t += blockDim.x) {
+        const int kk = t % TK, rn = t / TK;
+        Vs[rn][kk] = to_c<C>(evec[rn * KK + k0 + kk]);
+    }
+    for (int t = threadIdx.x; t < B * TK; t += blockDim.x) {
+        const int kk = t % TK, n = t / TK;
+        Es[n][kk] = eval[n * KK + k0 + kk];
+    }
+    __syncthreads();
+    const int kk = threadIdx.x % TK, r = threadIdx.x / TK;
+    for (int e = 0; e < E; e++) {
+        const double omega = omegas[e];
+        C acc[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) acc[c] = to_c<C>(make_double2(0.0, 0.0));
+#pragma unroll 1
+        for (int n = 0; n < B; n++) {
+            const C a = cmul(res_factor<C>(omega - Es[n][kk], eta), Vs[r * B + n][kk]);
+#pragma unroll
+            for (int c = 0; c < B; c++) acc[c] = cadd(acc[c], cmulc(a, Vs[c * B + n][kk]));
+        }
+        C *out = Gk + ((size_t)e * B * B + (size_t)r * B) * KK + k0 + kk;
+#pragma unroll
+        for (int c = 0; c < B; c++) out[(size_t)c * KK] = acc[c];
+    }
+}
+
+template <typename C, int B>
+int launch_gk_t(int N, const double *eval, const double2 *evec, const double *omegas, int E, double eta, C *Gk,
+                hipStream_t s)
+{
+    constexpr int TK = gk_tile<C, B>();
+    if (N % TK) return CQPI_E_ARG;
+    hipLaunchKernelGGL((k_gk<C, B>), dim3(N / TK, N), dim3(TK * B), 0, s, N, eval, evec, omegas, E, eta, Gk);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+template <typename C>
+int launch_gk(int B, int N, const double *eval, const double2 *evec, const double *omegas, int E, double eta, C *Gk,
+              hipStream_t s)
+{
+    switch (B) {
+#define CASE(b) \
+    case b: return launch_gk_t<C, b>(N, eval, evec, omegas, E, eta, Gk, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+        CASE(11) CASE(12) CASE(14) CASE(16)
+#undef CASE
+    default: return CQPI_E_ARG;
     }
 }
 
@@ -490,15 +553,40 @@ int wave_threads(int N, int np, bool pass1)
     return std::min(t, kWaveMaxThreads);
 }
 
+// energies per G(k) block of the workspace path
+constexpr size_t kGkBudget = (size_t)4 << 30;
+int gk_block(const Geom &g, int E, size_t csize)
+{
+    const size_t per_e = (size_t)g.B * g.B * g.K * g.K * csize;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)E, kGkBudget / per_e));
+}
+
 template <typename C, int B, int N>
 int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
-                 const C *tw, C *F1, hipStream_t s)
+                 const C *tw, C *F1, void *work, size_t work_bytes, hipStream_t s)
 {
     const int NP = g.B * g.B;
     const int np = wave_pairs(N, NP, sizeof(C));
     const int nch = (NP + np - 1) / np;
     const int thr = wave_threads(N, np, true);
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
+    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
+    if constexpr (!res_regs<B, N>()) {
+        // G(k) through the workspace, a block of energies at a time
+        const size_t per_e = (size_t)NP * N * N * sizeof(C);
+        if (!work || work_bytes < per_e) return CQPI_E_ARG;
+        const int EBa = (int)std::min<size_t>((size_t)E, work_bytes / per_e);
+        C *Gk = static_cast<C *>(work);
+        for (int ea = 0; ea < E; ea += EBa) {
+            const int eb = std::min(EBa, E - ea);
+            const int r = launch_gk<C>(g.B, N, eval, evec, omegas + ea, eb, eta, Gk, s);
+            if (r) return r;
+            hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, eb, nch), dim3(thr), lds, s, g, eval, evec,
+                               omegas + ea, eb, 1, eta, tw, np, F1 + (size_t)ea * N * g.nw * NP, (const C *)Gk);
+            if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+        }
+        return 0;
+    }
     // energies per workgroup: FP32 lines, with several workgroups resident
     // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2.  The
     // energy blocks are made even (51 energies: 4 blocks of 13, not 12,12,12,12,3)
@@ -507,24 +595,23 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int nb_want = std::max(1, std::min(E, (wg_target + rows / 2) / rows));
     const int EB = g.eb > 0 ? std::min(E, g.eb) : (E + nb_want - 1) / nb_want;
     const int nb = (E + EB - 1) / EB;
-    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
-                       eta, tw, np, F1);
+                       eta, tw, np, F1, (const C *)nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
 template <typename C, int N>
 int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
-                  const C *tw, C *F1, hipStream_t s)
+                  const C *tw, C *F1, void *work, size_t work_bytes, hipStream_t s)
 {
-    if (N > kWaveMaxThreads) return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    if (N > kWaveMaxThreads) return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     switch (g.B) {
-    case 1: return launch_res_s<C, 1, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 2: return launch_res_s<C, 2, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 3: return launch_res_s<C, 3, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 4: return launch_res_s<C, 4, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 5: return launch_res_s<C, 5, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    default: return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 1: return launch_res_s<C, 1, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 2: return launch_res_s<C, 2, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 3: return launch_res_s<C, 3, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 4: return launch_res_s<C, 4, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 5: return launch_res_s<C, 5, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    default: return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     }
 }
 
@@ -582,15 +669,16 @@ int launch_res_t(const Geom &g, const double *eval, const double2 *evec, const d
 namespace cqpi {
 
 int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, const double *omegas,
-                        int E, double eta, const double2 *tw, double2 *F1, hipStream_t s)
+                        int E, double eta, const double2 *tw, double2 *F1, void *work, size_t work_bytes,
+                        hipStream_t s)
 {
     if (wave_path(g.K)) {
         switch (g.K) {
-        case 64: return launch_res_sb<double2, 64>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 128: return launch_res_sb<double2, 128>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 256: return launch_res_sb<double2, 256>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 512: return launch_res_sb<double2, 512>(g, eval, evec, omegas, E, eta, tw, F1, s);
-        case 1024: return launch_res_sb<double2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
+        case 64: return launch_res_sb<double2, 64>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+        case 128: return launch_res_sb<double2, 128>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+        case 256: return launch_res_sb<double2, 256>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+        case 512: return launch_res_sb<double2, 512>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+        case 1024: return launch_res_sb<double2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
         }
     }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
@@ -607,16 +695,23 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
 bool transform32_supported(const Geom &g) { return wave_path(g.K); }
 
 int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
-                          double eta, const float2 *tw, float2 *F1, hipStream_t s)
+                          double eta, const float2 *tw, float2 *F1, void *work, size_t work_bytes, hipStream_t s)
 {
     switch (g.K) {
-    case 64: return launch_res_sb<float2, 64>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 128: return launch_res_sb<float2, 128>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 256: return launch_res_sb<float2, 256>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 512: return launch_res_sb<float2, 512>(g, eval, evec, omegas, E, eta, tw, F1, s);
-    case 1024: return launch_res_sb<float2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, s);
+    case 64: return launch_res_sb<float2, 64>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 128: return launch_res_sb<float2, 128>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 256: return launch_res_sb<float2, 256>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 512: return launch_res_sb<float2, 512>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 1024: return launch_res_sb<float2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     default: return CQPI_E_ARG;
     }
+}
+
+size_t resolvent_work_bytes(const Geom &g, int E, bool fp32)
+{
+    if (!wave_path(g.K) || (g.B <= kMaxRegB && g.K <= kWaveMaxThreads)) return 0;
+    const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+    return (size_t)gk_block(g, E, cs) * g.B * g.B * g.K * g.K * cs;
 }
 
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,

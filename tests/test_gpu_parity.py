@@ -52,6 +52,30 @@ def test_full_chain_spin_hex_reduced(gpu):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_full_chain_spin_hex_gk_path(gpu, prec):
+    # 10 bands on a 64^2 grid: G(k) is formed for all 100 pairs in a workspace
+    # and read back by the line transform (bands beyond the register path)
+    wl = synth.hex_5d_spin(K=64, n=6, E=2)
+    wl.window, wl.oversamp = 1, 2
+    wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
+    _assert_close(_run(wl, prec), oracle.cldos_run(wl), TOL64 if prec == "fp64" else TOLMIX)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gk_path_energy_blocks(gpu, prec):
+    """10 bands on a 1024^2 grid: the G(k) workspace holds 5 (FP32) or 2 (FP64)
+    energies, so a run of 6 energies goes through several blocks; each
+    energy's map must equal the map of a run of that energy alone."""
+    wl = synth.hex_5d_spin(K=1024, n=4, E=6)
+    print("case: G(k) workspace blocks K=1024 b=10 E=6 prec=%s" % prec, flush=True)
+    with cqpi.engine_for(wl, prec) as q:
+        allE = q.run(wl.energies, wl.eta)
+        for e in range(len(wl.energies)):
+            one = q.run(wl.energies[e:e + 1], wl.eta)
+            assert np.array_equal(one[0], allE[e]), "energy %d differs between block and single runs" % e
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_energy_chunk_split_over_two_streams(gpu, prec):
     # 9 energies: the run splits them 5 + 4 over the context's two streams
     wl = synth.square_norb(3, K=64, n=8, E=9, seed=16, w=2, os_=4, eta=0.01)
