@@ -158,6 +158,7 @@ struct cqpi_ctx {
     // gradient: kept forward state and adjoint buffers
     std::vector<cqpi_hopping> h_hops;  // model as given to cqpi_set_model
     bool keep_windows = false, windows_kept = false;
+    bool window_whole = true;  // the last run wrote the whole G0 window
     std::vector<double> last_omegas;
     double last_eta = 0.0;
     double2 *d_twn = nullptr;  // exp(-2 pi i m / K)
@@ -333,6 +334,7 @@ struct RunPlan {
     double eta = 0.0, sG = 1.0;
     bool mixed = false;
     size_t work_bytes = 0;  // G(k) workspace of the resolvent pass (0: not used)
+    int rwin = 0;           // half-width of the G0 window part that is written
 };
 
 // allocations and uploads of a run (synchronous); no kernel launch
@@ -366,6 +368,10 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, Ec * g.os * g.os, c->gpumem));
+    // mixed precision needs the FP64 window only for |R| <= 2w (T-matrix and
+    // bulk term) unless it is kept for the gradient
+    plan.rwin = (c->precision == CQPI_PREC_MIXED && !c->keep_windows) ? std::min(g.h, 2 * g.w) : g.h;
+    c->window_whole = plan.rwin >= g.h;
     plan.work_bytes = resolvent_work_bytes(g, (int)Ec, c->precision != CQPI_PREC_FP64 && transform32_supported(g));
     if (plan.work_bytes)
         RETCHK(dev_grow(c->d_gkw, c->c_gkw, (plan.work_bytes + sizeof(double2) - 1) / sizeof(double2), c->gpumem));
@@ -425,7 +431,8 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     stage_end(c, st);
     st = stage_begin(c, 2);
     if (t32)
-        RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes, s));
+        RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes,
+                                     plan.rwin, s));
     else
         RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
     stage_end(c, st);
@@ -667,6 +674,7 @@ int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, s
     RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos, false));
     c->nlayers = 1;
     c->windows_kept = false;
+    c->window_whole = true;
     c->last_chunk_e0 = 0;
     c->last_chunk_ec = 1;
     // the caller may overwrite g0 and scat once the copies have landed
@@ -972,6 +980,9 @@ int cqpi_retrieve_g0_window(cqpi_ctx *c, size_t e, double *out)
     if (!c || !out) return fail(CQPI_E_ARG, "NULL argument");
     if (e < c->last_chunk_e0 || e >= c->last_chunk_e0 + c->last_chunk_ec)
         return fail(CQPI_E_STATE, "G0 window is kept for the last energy chunk only");
+    if (!c->window_whole)
+        return fail(CQPI_E_STATE, "a mixed-precision run writes only |R| <= 2w of the G0 window; "
+                                  "call cqpi_set_keep_windows(ctx, 1) before the run to keep it whole");
     size_t nwin = (size_t)c->g.nw * c->g.nw * c->g.B * c->g.B;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(out, c->d_G0w + (e - c->last_chunk_e0) * nwin, nwin * sizeof(double2), hipMemcpyDeviceToHost));

@@ -144,8 +144,10 @@ int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int
 bool transform32_supported(const Geom &g);
 int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
                           double eta, const float2 *tw, float2 *F1, void *work, size_t work_bytes, hipStream_t s);
+// rwin: half-width of the part of the window written to G0w (g.h: all of it;
+// the mixed contraction needs only |R| <= 2w, for the T-matrix and the bulk term)
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
-                          double2 *G0w, _Float16 *P, hipStream_t s);
+                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s);
 // T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular
 int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T,
                    int *status, hipStream_t s);

@@ -475,8 +475,9 @@ int launch_gk(int B, int N, const double *eval, const double2 *evec, const doubl
 // the window is written in FP64 whatever C is
 template <typename C, int N>
 __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
-                                                                 const C *__restrict__ tw, int np, int np1, double sG,
-                                                                 double2 *__restrict__ G0w, _Float16 *__restrict__ P)
+                                                                 const C *__restrict__ tw, int np, int np1, int rwin,
+                                                                 double sG, double2 *__restrict__ G0w,
+                                                                 _Float16 *__restrict__ P)
 {
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
@@ -511,10 +512,14 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     lds_barrier();
     const double inv = 1.0 / ((double)N * (double)N);
     double2 *out = G0w + (size_t)e * nw * nw * NP;
-    for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
-        const int sx = t / npc, q = t - sx * npc;
-        const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
-        out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
+    // the window in FP64: whole, or (rwin < h) only |R| <= rwin
+    if (abs(sy - h) <= rwin) {
+        const int sx0 = h - rwin, nx = 2 * rwin + 1;
+        for (int t = threadIdx.x; t < nx * npc; t += blockDim.x) {
+            const int sx = sx0 + t / npc, q = t - (sx - sx0) * npc;
+            const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
+            out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
+        }
     }
     if (P) write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
 }
@@ -626,13 +631,15 @@ int kx_pairs(int N, int NP, size_t csize)
 
 template <typename C, int N>
 int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw, double sG, double2 *G0w,
-                _Float16 *P, hipStream_t s)
+                _Float16 *P, int rwin, hipStream_t s)
 {
     const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int np1 = wave_pairs(N, NP, sizeof(C));  // pass 1's chunks fix the F1 layout
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
-    hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, sG, G0w, P);
+    rwin = std::min(std::max(rwin, 0), g.h);
+    hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, rwin, sG, G0w,
+                       P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -715,17 +722,17 @@ size_t resolvent_work_bytes(const Geom &g, int E, bool fp32)
 }
 
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
-                          double2 *G0w, _Float16 *P, hipStream_t s)
+                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s)
 {
     MixGeom m{};
     if (mg) m = *mg;
     _Float16 *Pm = mg ? P : nullptr;
     switch (g.K) {
-    case 64: return launch_kx_s<float2, 64>(g, m, F1, E, tw, sG, G0w, Pm, s);
-    case 128: return launch_kx_s<float2, 128>(g, m, F1, E, tw, sG, G0w, Pm, s);
-    case 256: return launch_kx_s<float2, 256>(g, m, F1, E, tw, sG, G0w, Pm, s);
-    case 512: return launch_kx_s<float2, 512>(g, m, F1, E, tw, sG, G0w, Pm, s);
-    case 1024: return launch_kx_s<float2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
+    case 64: return launch_kx_s<float2, 64>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
+    case 128: return launch_kx_s<float2, 128>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
+    case 256: return launch_kx_s<float2, 256>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
+    case 512: return launch_kx_s<float2, 512>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
+    case 1024: return launch_kx_s<float2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
     default: return CQPI_E_ARG;
     }
 }
@@ -738,11 +745,11 @@ int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int
     _Float16 *Pm = mg ? P : nullptr;
     if (wave_path(g.K)) {
         switch (g.K) {
-        case 64: return launch_kx_s<double2, 64>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 128: return launch_kx_s<double2, 128>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 256: return launch_kx_s<double2, 256>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 512: return launch_kx_s<double2, 512>(g, m, F1, E, tw, sG, G0w, Pm, s);
-        case 1024: return launch_kx_s<double2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, s);
+        case 64: return launch_kx_s<double2, 64>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
+        case 128: return launch_kx_s<double2, 128>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
+        case 256: return launch_kx_s<double2, 256>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
+        case 512: return launch_kx_s<double2, 512>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
+        case 1024: return launch_kx_s<double2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
         }
     }
     if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;

@@ -150,6 +150,27 @@ def test_transform_matches_rocfft(gpu, K, prec):
             assert np.max(np.abs(win - ref)) < tol * np.max(np.abs(ref))
 
 
+def test_mixed_run_keeps_central_window_only(gpu):
+    """A mixed-precision run writes the FP64 window only for |R| <= 2w (the
+    T-matrix and the bulk term read no more): retrieving the whole window is
+    refused, and with kept windows it is whole and matches the FP64 run's to
+    the FP32 transform's accuracy."""
+    wl = synth.square_norb(3, K=64, n=16, E=2, seed=17, w=1, os_=2, eta=0.01)
+    print("case: mixed-mode window retrieval K=64", flush=True)
+    with cqpi.engine_for(wl, "fp64") as q:
+        q.run(wl.energies, wl.eta)
+        ref = [q.g0_window(e) for e in range(2)]
+    with cqpi.engine_for(wl, "mixed") as q:
+        q.run(wl.energies, wl.eta)
+        with pytest.raises(cqpi.CqpiError):
+            q.g0_window(0)
+        q.set_keep_windows(True)
+        q.run(wl.energies, wl.eta)
+        for e in range(2):
+            win = q.g0_window(e)
+            assert np.max(np.abs(win - ref[e])) < 2e-6 * np.max(np.abs(ref[e]))
+
+
 def test_dropin_wannierldos_matches_oracle(gpu):
     """GPUQPI::wannierldos path: host G0(R) and T in, one map out."""
     wl = synth.square_3orb(K=32, n=12, E=1)
