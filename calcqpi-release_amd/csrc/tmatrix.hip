@@ -14,49 +14,81 @@ namespace {
 
 constexpr int kMaxB = 16;
 
-__global__ void k_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
+// one thread per energy.  MB: compile-time matrix side (the arrays stay in
+// registers for small MB: the row exchange of the pivoting is done by compares
+// so that every index is static); MB = kMaxB serves any B <= kMaxB
+template <int MB>
+__global__ void __launch_bounds__(64) k_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
                           int E, double2 *__restrict__ Tout, int *__restrict__ status)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const int B = g.B, NP = B * B;
+    const int B = MB < kMaxB ? MB : g.B, NP = B * B;
     const double2 *G = G0w + ((size_t)e * g.nw * g.nw + (size_t)g.h * g.nw + g.h) * NP;
-    double2 M[kMaxB * kMaxB], X[kMaxB * kMaxB];
-    for (int r = 0; r < B; r++)
-        for (int c = 0; c < B; c++) {
-            double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
-            for (int k = 0; k < B; k++) acc = csub(acc, cmul(V[r * B + k], G[k * B + c]));
-            M[r * kMaxB + c] = acc;
-            X[r * kMaxB + c] = V[r * B + c];
+    double2 M[MB * MB], X[MB * MB];
+#pragma unroll
+    for (int r = 0; r < MB; r++)
+#pragma unroll
+        for (int c = 0; c < MB; c++) {
+            if (r < B && c < B) {
+                double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+                for (int k = 0; k < B; k++) acc = csub(acc, cmul(V[r * B + k], G[k * B + c]));
+                M[r * MB + c] = acc;
+                X[r * MB + c] = V[r * B + c];
+            }
         }
     int singular = 0;
-    for (int j = 0; j < B && !singular; j++) {
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+        if (j >= B || singular) break;
         int piv = j;
-        double best = cabs2(M[j * kMaxB + j]);
-        for (int i = j + 1; i < B; i++) {
-            double v = cabs2(M[i * kMaxB + j]);
-            if (v > best) { best = v; piv = i; }
+        double best = cabs2(M[j * MB + j]);
+#pragma unroll
+        for (int i = j + 1; i < MB; i++) {
+            if (i < B) {
+                double v = cabs2(M[i * MB + j]);
+                if (v > best) { best = v; piv = i; }
+            }
         }
         if (best == 0.0) { singular = 1; break; }
-        if (piv != j)
-            for (int c = 0; c < B; c++) {
-                double2 t = M[j * kMaxB + c]; M[j * kMaxB + c] = M[piv * kMaxB + c]; M[piv * kMaxB + c] = t;
-                t = X[j * kMaxB + c]; X[j * kMaxB + c] = X[piv * kMaxB + c]; X[piv * kMaxB + c] = t;
+#pragma unroll
+        for (int i = j + 1; i < MB; i++) {
+            if (i < B && i == piv) {
+#pragma unroll
+                for (int c = 0; c < MB; c++) {
+                    if (c < B) {
+                        double2 t = M[j * MB + c]; M[j * MB + c] = M[i * MB + c]; M[i * MB + c] = t;
+                        t = X[j * MB + c]; X[j * MB + c] = X[i * MB + c]; X[i * MB + c] = t;
+                    }
+                }
             }
-        double2 invp = cinv(M[j * kMaxB + j]);
-        for (int i = 0; i < B; i++) {
-            if (i == j) continue;
-            double2 l = cmul(M[i * kMaxB + j], invp);
-            if (l.x == 0.0 && l.y == 0.0) continue;
-            for (int c = j; c < B; c++) M[i * kMaxB + c] = csub(M[i * kMaxB + c], cmul(l, M[j * kMaxB + c]));
-            for (int c = 0; c < B; c++) X[i * kMaxB + c] = csub(X[i * kMaxB + c], cmul(l, X[j * kMaxB + c]));
         }
-        for (int c = 0; c < B; c++) X[j * kMaxB + c] = cmul(X[j * kMaxB + c], invp);
-        for (int c = j; c < B; c++) M[j * kMaxB + c] = cmul(M[j * kMaxB + c], invp);
+        double2 invp = cinv(M[j * MB + j]);
+#pragma unroll
+        for (int i = 0; i < MB; i++) {
+            if (i == j || i >= B) continue;
+            double2 l = cmul(M[i * MB + j], invp);
+            if (l.x == 0.0 && l.y == 0.0) continue;
+#pragma unroll
+            for (int c = 0; c < MB; c++)
+                if (c >= j && c < B) M[i * MB + c] = csub(M[i * MB + c], cmul(l, M[j * MB + c]));
+#pragma unroll
+            for (int c = 0; c < MB; c++)
+                if (c < B) X[i * MB + c] = csub(X[i * MB + c], cmul(l, X[j * MB + c]));
+        }
+#pragma unroll
+        for (int c = 0; c < MB; c++)
+            if (c < B) X[j * MB + c] = cmul(X[j * MB + c], invp);
+#pragma unroll
+        for (int c = 0; c < MB; c++)
+            if (c >= j && c < B) M[j * MB + c] = cmul(M[j * MB + c], invp);
     }
     double2 *T = Tout + (size_t)e * NP;
-    for (int r = 0; r < B; r++)
-        for (int c = 0; c < B; c++) T[r * B + c] = X[r * kMaxB + c];
+#pragma unroll
+    for (int r = 0; r < MB; r++)
+#pragma unroll
+        for (int c = 0; c < MB; c++)
+            if (r < B && c < B) T[r * B + c] = X[r * MB + c];
     status[e] = singular;
 }
 
@@ -112,8 +144,15 @@ int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, d
 {
     if (g.B > kMaxB) return CQPI_E_ARG;
     int threads = 64;
-    hipLaunchKernelGGL(k_tmatrix, dim3((E + threads - 1) / threads), dim3(threads), 0, s, g, G0w, V, E,
-                       T, status);
+    const dim3 grid((E + threads - 1) / threads);
+    switch (g.B) {
+    case 1: hipLaunchKernelGGL(k_tmatrix<1>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    case 2: hipLaunchKernelGGL(k_tmatrix<2>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    case 3: hipLaunchKernelGGL(k_tmatrix<3>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    case 4: hipLaunchKernelGGL(k_tmatrix<4>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    case 5: hipLaunchKernelGGL(k_tmatrix<5>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    default: hipLaunchKernelGGL(k_tmatrix<kMaxB>, grid, dim3(threads), 0, s, g, G0w, V, E, T, status); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
