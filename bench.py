@@ -270,7 +270,9 @@ def batch_timing(cqpi, synth, torch, count=128, reps=5):
 
 def probe(cqpi, synth, torch, workload):
     """One mixed and one FP64 step after a 256 MiB device copy: the run that
-    the rocprofv3 --pmc traffic passes profile."""
+    the rocprofv3 --pmc traffic passes profile.  Stage timing is on, which
+    keeps each run on one stream, so every stage is one launch over all
+    energies (the timed steps split the energies over two streams)."""
     src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
     dst = src.clone()
     torch.cuda.synchronize()
@@ -278,6 +280,7 @@ def probe(cqpi, synth, torch, workload):
     wl = synth.workload(workload)
     for prec in ("mixed", "fp64"):
         with cqpi.engine_for(wl, prec) as q:
+            q.set_timing(True)
             q.compute_eigen()
             q.run(wl.energies, wl.eta, host=False)
     torch.cuda.synchronize()

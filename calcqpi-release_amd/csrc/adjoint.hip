@@ -569,13 +569,16 @@ __global__ void __launch_bounds__(kAdjWaveMaxThreads) k_adj_ky_s(Geom g, const d
     const int j = threadIdx.x;  // this thread's k-point (blockDim >= N)
     const bool has_k = j < N;
     const long k = (long)i * N + (has_k ? j : 0);
+    // the row's eigenvectors stay in LDS after the lines (the workgroup holds
+    // the CU's LDS either way), which keeps the thread's registers unspilled
+    double2 *Vl = S + (size_t)NP * NL;
     double ev[B];
-    double2 vv[NP], hacc[NP];
+    double2 hacc[NP];
 #pragma unroll
     for (int nn = 0; nn < B; nn++) ev[nn] = eval[nn * KK + k];
 #pragma unroll
     for (int r = 0; r < NP; r++) {
-        vv[r] = evec[r * KK + k];
+        if (has_k) Vl[(size_t)r * N + j] = evec[r * KK + k];
         hacc[r] = make_double2(0.0, 0.0);
     }
     for (int e = e0; e < e1; e++) {
@@ -601,7 +604,8 @@ __global__ void __launch_bounds__(kAdjWaveMaxThreads) k_adj_ky_s(Geom g, const d
                 for (int c = 0; c < B; c++) {
                     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-                    for (int nn = 0; nn < B; nn++) acc = cadd(acc, cmul(f[nn], cmulc(vv[r * B + nn], vv[c * B + nn])));
+                    for (int nn = 0; nn < B; nn++)
+                        acc = cadd(acc, cmul(f[nn], cmulc(Vl[(size_t)(r * B + nn) * N + j], Vl[(size_t)(c * B + nn) * N + j])));
                     G[r * B + c] = acc;
                 }
             // Hbar = G^H (Gb G^H), one column of X = Gb G^H at a time; Gb read from LDS
@@ -677,7 +681,7 @@ __global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restr
                                                   const double *__restrict__ ry, int nry,
                                                   const double2 *__restrict__ twn, double2 *__restrict__ U)
 {
-    __shared__ double r1[256], r2[256];
+    __shared__ double2 part[4][kMaxRy + 1];  // per-wave sums (256 threads)
     const int kx = blockIdx.x, pr = blockIdx.y;
     const int NP = g.B * g.B, K = g.K;
     const long KK = (long)K * K;
@@ -692,13 +696,22 @@ __global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restr
             if (r < nry) acc[r] = cadd(acc[r], cmul(hv, hop_phase(ry[r], ky, K, twn)));
         acc[kMaxRy] = cadd(acc[kMaxRy], hv);
     }
+    // wave sums by shuffles, then one barrier and the sum over the 4 waves
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int r = 0; r <= kMaxRy; r++) {
         if (r < nry || r == kMaxRy) {
-            double2 s = block_sum2(acc[r], r1, r2);
-            const int slot = r == kMaxRy ? nry : r;
-            if (threadIdx.x == 0) U[((size_t)kx * (nry + 1) + slot) * NP + pr] = s;
+            const double re = wsum_reduce(acc[r].x), im = wsum_reduce(acc[r].y);
+            if (lane == 0) part[wave][r] = make_double2(re, im);
         }
+    }
+    __syncthreads();
+    const int r = threadIdx.x;
+    if (r <= kMaxRy && (r < nry || r == kMaxRy)) {
+        double2 s = part[0][r];
+        for (int wv = 1; wv < 4; wv++) s = cadd(s, part[wv][r]);
+        const int slot = r == kMaxRy ? nry : r;
+        U[((size_t)kx * (nry + 1) + slot) * NP + pr] = s;
     }
 }
 
@@ -765,7 +778,7 @@ int launch_ky_s(const Geom &g, const double *eval, const double2 *evec, const do
     const int NG = cqpi_energy_groups(g, E), EG = (E + NG - 1) / NG;
     int thr = std::max(NP * cqpi::stockham_lanes(N), N);
     thr = std::min((thr + 63) / 64 * 64, kAdjWaveMaxThreads);  // >= N: one thread per k-point
-    const size_t lds = sizeof(double2) * (size_t)NP * cqpi::padded_len<double2>(N);
+    const size_t lds = sizeof(double2) * (size_t)NP * (cqpi::padded_len<double2>(N) + N);
     hipLaunchKernelGGL((k_adj_ky_s<B, N>), dim3(N, NG), dim3(thr), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
                        twn, Hacc);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
