@@ -105,11 +105,16 @@ def cpu_baseline(wl, nsample):
     return info, np.array(maps)
 
 
-def time_steps(q, wl, steps, warmup, torch):
+def time_steps(q, wl, steps, warmup, torch, stage_events=False):
+    """Wall time per step (eigensolve + all energies) over `steps` steps after
+    `warmup`, then the stage times of one more step.  stage_events: every
+    step records stage events, which keeps each run on one stream (one launch
+    per stage: what a kernel-trace profile of the stages should see)."""
     def step():
         q.compute_eigen()
         q.run(wl.energies, wl.eta, host=False)
 
+    q.set_timing(stage_events)
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -121,7 +126,7 @@ def time_steps(q, wl, steps, warmup, torch):
     q.set_timing(True)
     step()
     st = q.stage_times()
-    q.set_timing(False)
+    q.set_timing(stage_events)   # (with stage_events the later accuracy runs stay on one stream too)
     return dt, st
 
 
@@ -298,6 +303,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--stage-events", action="store_true",
+                    help="record stage events in every step (one stream, one launch per stage)")
     ap.add_argument("--cpu-energies", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -323,7 +330,7 @@ def main():
     timed = {}
     for prec in ("fp64", "fp32", "mixed"):
         q = cqpi.engine_for(wl, prec)
-        dt, st = time_steps(q, wl, args.steps, args.warmup, torch)
+        dt, st = time_steps(q, wl, args.steps, args.warmup, torch, args.stage_events)
         timed[prec] = (q, dt, st)
     qm, dt_m, st_m = timed["mixed"]
     q6, dt_6, st_6 = timed["fp64"]

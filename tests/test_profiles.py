@@ -28,9 +28,15 @@ def test_short_name(raw, short):
 
 
 def test_bench_reads_measured_traffic():
+    import json
     import bench
     f32 = bench.measured_traffic("k_res_ky_s", "mixed")
     f64 = bench.measured_traffic("k_res_ky_s", "fp64")
-    assert f32 and f64 and f32 < f64          # FP32 rows move fewer bytes
+    assert f32 and f64 and f32 != f64         # the line type selects the row
+    txt = open(bench.TRAFFIC_MD).read()
+    rows = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])["kernels"]
+    w32 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f32")]
+    w64 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f64")]
+    assert w32 and w64 and w32[0] < 0.6 * w64[0]   # FP32 rows: half the bytes written
     assert bench.measured_traffic("k_cmix_areg", "mixed") > 0
     assert bench.measured_traffic("no_such_kernel", "mixed") is None
