@@ -149,7 +149,10 @@ int cqpi_device_ldos(cqpi_ctx *ctx, const double **dptr, size_t *count);
 
 /* Copies G0(R) of energy index e of the last cqpi_run for the window of R
  * used by the contraction: side nw = 2h+1 with R = slot - h, h as reported
- * by cqpi_window_half; out is nw*nw*b*b complex, [(sx*nw+sy)*b*b + r*b + c]. */
+ * by cqpi_window_half; out is nw*nw*b*b complex, [(sx*nw+sy)*b*b + r*b + c].
+ * A mixed-precision run keeps only |R| <= 2w of the window in FP64 (all the
+ * T-matrix and the periodic term read); after such a run this returns
+ * CQPI_E_STATE unless cqpi_set_keep_windows(ctx, 1) was set for the run. */
 int cqpi_window_half(cqpi_ctx *ctx, size_t *h);
 int cqpi_retrieve_g0_window(cqpi_ctx *ctx, size_t e, double *out);
 /* Copies T (b x b complex) of energy index e of the last cqpi_run. */
