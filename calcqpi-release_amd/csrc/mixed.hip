@@ -28,6 +28,9 @@
 #include "cqpi_common.h"
 #include "mixed.h"
 
+#ifndef CMIX_JB3
+#define CMIX_JB3 2
+#endif
 namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -131,7 +134,7 @@ constexpr int kJB = 4;  // sub-pixel rows (jj) per accumulator block
 // one wave = one tile: CB cells along x starting at cx0, one cell row cy,
 // all sub-pixels.  Rows of the GEMM are the pixels (c, ii), c < CB, ii < os,
 // of one sub-pixel row jj; the jj loop runs in blocks of kJB accumulators.
-template <int NT, int LDW>
+template <int NT, int LDW, int JB>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                     const half8 *__restrict__ Afrag,
                                                     const double *__restrict__ scale, double sA,
@@ -140,7 +143,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                                                     double *__restrict__ ldos)
 {
     __shared__ double2 Ts[16 * 16];
-    __shared__ float Cs[kWaves][kJB][16][16 * NT + 1];
+    constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
+    __shared__ float Cs[kWaves][JB][16][16 * NT + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     int bx, by, e;
@@ -160,16 +164,18 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
     const int col = lane & 15, kq = lane >> 4;
     const int col0 = cx0 - n2 - w + h + mg.OFF;
     const int rows = mg.CB * os;
-    double val[4] = {0.0, 0.0, 0.0, 0.0};
+    double val[NB];
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++) val[jb] = 0.0;
 
     for (int sg = 0; sg < mg.nsets; sg++) {
 #pragma unroll
-        for (int jb = 0; jb < 4; jb++) {
-            const int jj0 = jb * kJB;
+        for (int jb = 0; jb < NB; jb++) {
+            const int jj0 = jb * JB;
             if (jj0 >= os) break;
-            float4v hh[kJB][NT], cr[kJB][NT];
+            float4v hh[JB][NT], cr[JB][NT];
 #pragma unroll
-            for (int i = 0; i < kJB; i++)
+            for (int i = 0; i < JB; i++)
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
                     hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
@@ -209,7 +215,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < kJB; i++) {
+                for (int i = 0; i < JB; i++) {
                     const int jj = jj0 + i;
                     if (jj >= os) break;
                     const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64;
@@ -226,7 +232,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
             // epilogue (wave-local): C -> LDS, then one pixel per lane:
             // Im sum_pq a_p T_pq b_q in FP64
 #pragma unroll
-            for (int i = 0; i < kJB; i++)
+            for (int i = 0; i < JB; i++)
 #pragma unroll
                 for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             {
                 const int i = lane >> 4, m = lane & 15;
-                if (jj0 + i < os && m < rows) {
+                if (i < JB && jj0 + i < os && m < rows) {
                     const float *Cr = Cs[wave][i][m];
                     double acc = 0.0;
                     for (int p = 0; p < B; p++) {
@@ -259,9 +265,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
     const int W = g.W;
     double *out = ldos + (size_t)e * W * W;
 #pragma unroll
-    for (int jb = 0; jb < 4; jb++) {
-        const int jj = jb * kJB + (lane >> 4), m = lane & 15;
-        if (jj >= os || m >= rows || !cy_ok) continue;
+    for (int jb = 0; jb < NB; jb++) {
+        const int i = lane >> 4, jj = jb * JB + i, m = lane & 15;
+        if (i >= JB || jj >= os || m >= rows || !cy_ok) continue;
         const int c = m / os, ii = m - c * os, cx = cx0 + c;
         if (cx >= n) continue;
         const int X = cx * os + ii, Y = cy * os + jj;
@@ -484,6 +490,190 @@ bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *
     return true;
 }
 
+// Generic shapes with the B operand staged in LDS (spin blocks, many bands,
+// any os): one workgroup of kStWaves waves covers kStWaves consecutive cell
+// rows (one per wave) and one tile of CB cells; for each spin block it copies
+// the plane rows those tiles reach (kStWaves + 2w rows, the spin block's
+// orbitals only) into LDS, then every wave reads its B fragments from there,
+// padding lanes from a zero slot.  A fragments come from global memory (L2),
+// as in k_cmix.
+constexpr int kStWaves = 8;
+
+size_t cmix_st_lds(const MixGeom &mg, const Geom &g, int NT, int JB)
+{
+    const int RP = (kStWaves + 2 * g.w) | 1;
+    const size_t planes = (size_t)4 * mg.nper * g.B * 2;  // (a|b, hi|lo) x the block's (o, p, re/im)
+    return (planes * RP * mg.NUg * 8 + 8) * sizeof(_Float16) + (size_t)kStWaves * JB * 16 * (16 * NT + 1) * sizeof(float);
+}
+
+template <int NT, int LDW, int JB>
+__global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
+                                                          const half8 *__restrict__ Afrag,
+                                                          const double *__restrict__ scale, double sA,
+                                                          const double2 *__restrict__ Tm,
+                                                          const double *__restrict__ bulk,
+                                                          double *__restrict__ ldos)
+{
+    constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
+    __shared__ double2 Ts[16 * 16];
+    extern __shared__ __align__(16) unsigned char smem_st[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int bx, by, e;
+    xcd_block(bx, by, e);
+    const int B = g.B, os = g.os, w = g.w, n = g.n, h = g.h, nw = g.nw;
+    const int n2 = n >> 1;
+    const int cy_lo = by * kStWaves;
+    const int cy_raw = cy_lo + wave;
+    const bool cy_ok = cy_raw < n;
+    const int cy = cy_ok ? cy_raw : n - 1;
+    const int cx0 = bx * mg.CB;
+    const double2 *Te = Tm + (size_t)e * B * B;
+    for (int i = threadIdx.x; i < B * B; i += blockDim.x) Ts[i] = Te[i];
+
+    const size_t plane = (size_t)mg.Ly * nw;
+    const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
+    const int col = lane & 15, kq = lane >> 4;
+    const int col0 = cx0 - n2 - w + h + mg.OFF;
+    const int rows = mg.CB * os;
+    const int RW = kStWaves + 2 * w, RP = RW | 1, seg = mg.NUg;
+    const int row_lo = cy_lo - n2 - w + h;
+    const int NPS = mg.nper * B * 2;                    // planes of one spin block per (a|b, hi|lo)
+    _Float16 *Pl = reinterpret_cast<_Float16 *>(smem_st);
+    const int LOFF = NPS * RP * seg * 8;                // hi -> lo plane of the same (o, p)
+    const int ZOFF = 4 * NPS * RP * seg * 8;            // 16-byte zero slot
+    float *Csw = reinterpret_cast<float *>(Pl + ZOFF + 8) + (size_t)wave * JB * 16 * (16 * NT + 1);
+    const int trow = (cy - cy_lo) * seg * 8;
+    double val[NB];
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++) val[jb] = 0.0;
+
+    for (int sg = 0; sg < mg.nsets; sg++) {
+        __syncthreads();  // the previous spin block's fragments are read
+        // (consecutive threads take consecutive rows: contiguous in the planes)
+        for (int l = threadIdx.x; l < 4 * NPS * RW * seg; l += blockDim.x) {
+            const int r = l % RW, rest = l / RW, g8 = rest % seg, pid = rest / seg;
+            const int abhl = pid / NPS, pl = sg * NPS + (pid - abhl * NPS);
+            const int row = row_lo + r;
+            half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (row < nw) v = load8<LDW>(Pe + ((size_t)abhl * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
+            *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
+        }
+        if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + ZOFF) = half8{0, 0, 0, 0, 0, 0, 0, 0};
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) {
+            const int jj0 = jb * JB;
+            if (jj0 >= os) break;
+            float4v hh[JB][NT], cr[JB][NT];
+#pragma unroll
+            for (int i = 0; i < JB; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+                    cr[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+                }
+            for (int ks = 0; ks < mg.KS; ks++) {
+                const int kg = ks * 4 + kq;
+                const bool kok = kg < mg.KG;
+                const int g8 = kg % mg.NUg, r = kg / mg.NUg;
+                const int ol = r % mg.nper, dy = r / mg.nper;
+                half8 bh[NT], bl[NT];
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const int nn = t * 16 + col;
+                    int ab = -1, pl = 0;
+                    if (nn < 2 * B) {
+                        ab = 0;
+                        pl = (ol * B + (nn >> 1)) * 2 + (nn & 1);
+                    } else if (nn < 4 * B) {
+                        ab = 1;
+                        int m2 = nn - 2 * B;
+                        pl = (ol * B + (m2 >> 1)) * 2 + (m2 & 1);
+                    }
+                    const bool ok = kok && ab >= 0;
+                    const int q = (((ab * 2) * NPS + pl) * RP + dy) * seg * 8 + g8 * 8 + trow;
+                    bh[t] = *reinterpret_cast<const half8 *>(Pl + (ok ? q : ZOFF));
+                    bl[t] = *reinterpret_cast<const half8 *>(Pl + (ok ? q + LOFF : ZOFF));
+                }
+#pragma unroll
+                for (int i = 0; i < JB; i++) {
+                    const int jj = jj0 + i;
+                    if (jj >= os) break;
+                    const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64;
+                    const half8 ah = Afrag[fa + lane];
+                    const half8 al = Afrag[fa + 64 + lane];
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[i][t], 0, 0, 0);
+                        cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], cr[i][t], 0, 0, 0);
+                        cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], cr[i][t], 0, 0, 0);
+                    }
+                }
+            }
+            // epilogue (wave-local): C -> LDS, then one pixel per lane:
+            // Im sum_pq a_p T_pq b_q in FP64
+#pragma unroll
+            for (int i = 0; i < JB; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        Csw[((size_t)i * 16 + kq * 4 + r) * (16 * NT + 1) + t * 16 + col] =
+                            hh[i][t][r] + cr[i][t][r] * (1.0f / kLoScale);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                const int i = lane >> 4, m = lane & 15;
+                if (i < JB && jj0 + i < os && m < rows) {
+                    const float *Cr = Csw + ((size_t)i * 16 + m) * (16 * NT + 1);
+                    double acc = 0.0;
+                    for (int p = 0; p < B; p++) {
+                        double2 tb = make_double2(0.0, 0.0);
+                        for (int q = 0; q < B; q++)
+                            tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
+                        acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
+                    }
+                    val[jb] += acc;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    const double sc = sA * scale[e];
+    const double unscale = 1.0 / (sc * sc);
+    const int W = g.W;
+    double *out = ldos + (size_t)e * W * W;
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++) {
+        const int i = lane >> 4, jj = jb * JB + i, m = lane & 15;
+        if (i >= JB || jj >= os || m >= rows || !cy_ok) continue;
+        const int c = m / os, ii = m - c * os, cx = cx0 + c;
+        if (cx >= n) continue;
+        const int X = cx * os + ii, Y = cy * os + jj;
+        out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+    }
+}
+
+template <int NT, int JB>
+int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
+              const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kStWaves - 1) / kStWaves, E);
+    dim3 blk(64 * kStWaves);
+    const size_t lds = cmix_st_lds(mg, g, NT, JB);
+    switch (mg.LDW) {
+    case 8: hipLaunchKernelGGL((k_cmix_st<NT, 8, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 4: hipLaunchKernelGGL((k_cmix_st<NT, 4, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 2: hipLaunchKernelGGL((k_cmix_st<NT, 2, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    default: hipLaunchKernelGGL((k_cmix_st<NT, 1, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
 template <int NT>
 int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag, const double *scale,
               double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
@@ -491,11 +681,14 @@ int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *A
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves - 1) / kWaves, E);
     dim3 blk(64 * kWaves);
     const half8 *A = (const half8 *)Afrag;
+    // accumulator blocks of JB sub-pixel rows: fewer for wide N, so that the
+    // accumulators leave room for more than one wave per SIMD
+    constexpr int JB = NT >= 3 ? CMIX_JB3 : 4;
     switch (mg.LDW) {
-    case 8: hipLaunchKernelGGL((k_cmix<NT, 8>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 4: hipLaunchKernelGGL((k_cmix<NT, 4>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 2: hipLaunchKernelGGL((k_cmix<NT, 2>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    default: hipLaunchKernelGGL((k_cmix<NT, 1>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 8: hipLaunchKernelGGL((k_cmix<NT, 8, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 4: hipLaunchKernelGGL((k_cmix<NT, 4, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 2: hipLaunchKernelGGL((k_cmix<NT, 2, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    default: hipLaunchKernelGGL((k_cmix<NT, 1, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
@@ -594,6 +787,17 @@ int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, c
         try_areg<2, 4, 4, 7, 5, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
         try_areg<1, 2, 8, 1, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    // staged B operand when the spin block's plane rows fit the LDS
+    constexpr size_t kStLdsMax = 150 * 1024;
+    if (g.os <= 16 && g.B <= 16) {
+        switch (mg.NT) {
+        case 1: if (cmix_st_lds(mg, g, 1, 4) <= kStLdsMax) return launch_st<1, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+        case 2: if (cmix_st_lds(mg, g, 2, 4) <= kStLdsMax) return launch_st<2, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+        case 3: if (cmix_st_lds(mg, g, 3, 2) <= kStLdsMax) return launch_st<3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+        case 4: if (cmix_st_lds(mg, g, 4, 2) <= kStLdsMax) return launch_st<4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+        default: break;
+        }
+    }
     switch (mg.NT) {
     case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
     case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
