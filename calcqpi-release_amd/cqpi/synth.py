@@ -322,11 +322,13 @@ def hex_5d_spin(K=1024, n=256, E=101, seed=1003, lam=0.05) -> Workload:
     )
 
 
-ORB_SETS = {1: ["s"], 3: ["xz", "yz", "xy"], 5: ["xz", "yz", "xy", "x2y2", "z2"]}
+ORB_SETS = {1: ["s"], 3: ["xz", "yz", "xy"], 5: ["xz", "yz", "xy", "x2y2", "z2"],
+            6: ["s", "xz", "yz", "xy", "x2y2", "z2"], 7: ["s", "xz", "yz", "xy", "x2y2", "z2", "s"]}
 
 
 def square_norb(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=None):
-    """Model and tiles generated as in config 2 for b in {1,3,5}."""
+    """Model and tiles generated as in config 2 for b in {1,3,5} (6 and 7 for
+    the lane-group eigensolve tests)."""
     rng = np.random.default_rng(seed)
     onsite = rng.normal(0, 0.3, (b, b))
     onsite = 0.5 * (onsite + onsite.T)

@@ -72,8 +72,9 @@ __device__ inline bool converged(const double2 *A, int b)
     return off == 0.0 || off < 1e-34 * diag;
 }
 
-// Cyclic-by-row Jacobi.  Small B: everything unrolled so A and V stay in
-// registers.  Larger B: rolled loops (A and V live in private memory).
+// Cyclic-by-row Jacobi, one thread per k-point.  Small B: everything unrolled
+// so A and V stay in registers.  B = 14, 16: rolled loops (A and V live in
+// private memory).  6 <= B <= 12 use the lane-group kernel k_eigen_g below.
 constexpr int kUnrollB = 5;
 
 template <int B>
@@ -156,11 +157,185 @@ __global__ void __launch_bounds__(128) k_eigen(const DevRGroup *__restrict__ gro
     for (int r = 0; r < B * B; r++) evec[r * KK + idx] = V[r];
 }
 
+// ---------------------------------------------------------------------------
+// 6 <= b <= 12: one lane group of 16 lanes per k-point, a parallel cyclic
+// Jacobi.  Lane r holds row r of A = V^H H V and row r of V (b rounded up to
+// an even M with a decoupled zero row).  Each round rotates the M/2 disjoint
+// pairs (r, M-1-r) at once; between rounds the positions 1..M-1 move round a
+// circle (position 0 fixed), so that M-1 rounds meet every pair once and
+// return every index to its place (a sweep).  Rows move between lanes by
+// shuffles, columns by a fixed permutation of each lane's registers, so every
+// register index is static.  The rotation of a pair is the one of the
+// per-thread kernel (explicit zero for A_pq, diagonal from t).
+// ---------------------------------------------------------------------------
+constexpr int kGroup = 16;
+
+__device__ inline double2 shfl2(double2 v, int src)
+{
+    return make_double2(__shfl(v.x, src, kGroup), __shfl(v.y, src, kGroup));
+}
+
+// source position of new position i after one turn of the circle
+__host__ __device__ constexpr int circle_src(int i, int M) { return i == 0 ? 0 : (i == 1 ? M - 1 : i - 1); }
+
+template <int B>
+__global__ void __launch_bounds__(256) k_eigen_g(const DevRGroup *__restrict__ groups, int ngroups,
+                                                 const double2 *__restrict__ tmat, const double2 *__restrict__ tw,
+                                                 double efermi, int scmax, int K, double *__restrict__ eval,
+                                                 double2 *__restrict__ evec)
+{
+    constexpr int M = B + (B & 1);
+    static_assert(M <= kGroup, "lane group too small");
+    const long KK = (long)K * K;
+    const int r = threadIdx.x & (kGroup - 1);
+    const long gid = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kGroup;  // k-point of the group
+    const bool kok = gid < KK;   // uniform over the group
+    const long k = kok ? gid : 0;
+    const int i = (int)(k / K), j = (int)(k - (long)i * K);
+    const bool act = r < M;
+    const int s = act ? M - 1 - r : r;          // partner position
+    const bool isp = r < M / 2;
+    const int pl = isp ? r : s, ql = isp ? s : r;  // p and q lanes of my pair
+
+    double2 Ar[M], Vr[M];
+#pragma unroll
+    for (int c = 0; c < M; c++) {
+        Ar[c] = make_double2(0.0, 0.0);
+        Vr[c] = make_double2(c == r ? 1.0 : 0.0, 0.0);
+    }
+    if (r < B) {
+        const double dia = r < scmax ? -efermi : efermi;
+#pragma unroll
+        for (int c = 0; c < M; c++)
+            if (c == r) Ar[c].x = dia;
+        const double kx = (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
+        const double ky = (j > (K >> 1)) ? ((double)j - (double)K) / (double)K : (double)j / (double)K;
+        const unsigned kmask = (K & (K - 1)) == 0 ? (unsigned)K - 1u : 0u;
+        for (int g = 0; g < ngroups; g++) {
+            const DevRGroup G = groups[g];
+            double2 e;
+            if (G.integral) {
+                const unsigned m = (unsigned)G.irx * (unsigned)i + (unsigned)G.iry * (unsigned)j;
+                e = tw[kmask ? (m & kmask) : (m % (unsigned)K)];
+            } else {
+                double sn, cs;
+                sincos(2.0 * M_PI * (G.rx * kx + G.ry * ky), &sn, &cs);
+                e = make_double2(cs, sn);
+            }
+            const double2 *T = tmat + ((size_t)g * B + r) * B;
+#pragma unroll
+            for (int c = 0; c < B; c++) Ar[c] = cadd(Ar[c], cmul(T[c], e));
+        }
+#pragma unroll
+        for (int c = 0; c < M; c++)
+            if (c == r) Ar[c].y = 0.0;
+    }
+
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, dg = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; c++) {
+            if (c == r) dg += Ar[c].x * Ar[c].x;
+            else off += cabs2(Ar[c]);
+        }
+        for (int o = 1; o < kGroup; o <<= 1) {
+            off += __shfl_xor(off, o, kGroup);
+            dg += __shfl_xor(dg, o, kGroup);
+        }
+        if (off == 0.0 || off < 1e-34 * dg) break;  // uniform over the group
+        for (int rnd = 0; rnd < M - 1; rnd++) {
+            // my diagonal and my pair element
+            double2 d = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < M; c++) {
+                if (c == r) d = Ar[c];
+                if (c == s) x = Ar[c];
+            }
+            // the pair's rotation, from the p lane's A_pp, A_pq and the q lane's A_qq
+            const double app = __shfl(d.x, pl, kGroup), aqq = __shfl(d.x, ql, kGroup);
+            const double2 apq = shfl2(x, pl);
+            double cr = 1.0, sr = 0.0, tr = 0.0;
+            double2 ph = make_double2(1.0, 0.0);
+            const double rr = sqrt(cabs2(apq));
+            if (act && rr != 0.0) {
+                ph = make_double2(apq.x / rr, apq.y / rr);
+                const double theta = (aqq - app) / (2.0 * rr);
+                double t;
+                if (fabs(theta) > 1e150)
+                    t = 0.5 / theta;
+                else
+                    t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                cr = 1.0 / sqrt(t * t + 1.0);
+                sr = t * cr;
+                tr = t * rr;
+            }
+            // columns p = jp, q = M-1-jp of every row (A and V), with pair jp's rotation
+#pragma unroll
+            for (int jp = 0; jp < M / 2; jp++) {
+                const double cj = __shfl(cr, jp, kGroup), sj = __shfl(sr, jp, kGroup);
+                const double2 phj = shfl2(ph, jp);
+                const double2 sphc = cscale(sj, cconj(phj)), cphc = cscale(cj, cconj(phj));
+                const int p = jp, q = M - 1 - jp;
+                double2 a = Ar[p], b = Ar[q];
+                Ar[p] = csub(cscale(cj, a), cmul(sphc, b));
+                Ar[q] = cadd(cscale(sj, a), cmul(cphc, b));
+                a = Vr[p];
+                b = Vr[q];
+                Vr[p] = csub(cscale(cj, a), cmul(sphc, b));
+                Vr[q] = cadd(cscale(sj, a), cmul(cphc, b));
+            }
+            // rows p and q: A_p <- c A_p - s ph A_q,  A_q <- s A_p + c ph A_q
+            {
+                const double2 sph = cscale(sr, ph), cph = cscale(cr, ph);
+#pragma unroll
+                for (int c = 0; c < M; c++) {
+                    const double2 o = shfl2(Ar[c], s);
+                    Ar[c] = isp ? csub(cscale(cr, Ar[c]), cmul(sph, o)) : cadd(cscale(sr, o), cmul(cph, Ar[c]));
+                }
+            }
+            // exact zero for the rotated pair, diagonal from t
+#pragma unroll
+            for (int c = 0; c < M; c++) {
+                if (c == s && act) Ar[c] = make_double2(0.0, 0.0);
+                if (c == r && act) Ar[c] = make_double2(isp ? app - tr : aqq + tr, 0.0);
+            }
+            // one turn of the circle: rows by shuffle, columns by a fixed permutation
+            {
+                const int src = act ? circle_src(r, M) : r;
+                double2 t[M];
+#pragma unroll
+                for (int c = 0; c < M; c++) t[c] = shfl2(Ar[c], src);
+#pragma unroll
+                for (int c = 0; c < M; c++) Ar[c] = t[circle_src(c, M)];
+#pragma unroll
+                for (int c = 0; c < M; c++) t[c] = Vr[c];
+#pragma unroll
+                for (int c = 0; c < M; c++) Vr[c] = t[circle_src(c, M)];
+            }
+        }
+    }
+    if (kok && r < B) {
+        double dv = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; c++)
+            if (c == r) dv = Ar[c].x;
+        eval[(long)r * KK + k] = dv;
+#pragma unroll
+        for (int n = 0; n < B; n++) evec[((long)r * B + n) * KK + k] = Vr[n];
+    }
+}
+
 template <int B>
 int launch_eigen_t(const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw, double efermi,
                    int scmax, int K, double *eval, double2 *evec, hipStream_t s)
 {
     long KK = (long)K * K;
+    if constexpr (B >= 6 && B <= 12) {
+        const long blocks = (KK * kGroup + 255) / 256;
+        hipLaunchKernelGGL(k_eigen_g<B>, dim3((unsigned)blocks), dim3(256), 0, s, groups, ngroups, tmat, tw, efermi,
+                           scmax, K, eval, evec);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     int threads = 128;
     long blocks = (KK + threads - 1) / threads;
     hipLaunchKernelGGL(k_eigen<B>, dim3((unsigned)blocks), dim3(threads), 0, s, groups, ngroups,

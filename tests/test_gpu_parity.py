@@ -51,6 +51,13 @@ def test_full_chain_spin_hex_reduced(gpu):
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
+@pytest.mark.parametrize("b", [6, 7])
+def test_full_chain_lane_group_eigensolve(gpu, b):
+    # 6..12 bands take the lane-group Jacobi (odd counts with a decoupled padding row)
+    wl = synth.square_norb(b, K=16, n=6, E=2, seed=20 + b, w=1, os_=2, eta=0.02)
+    _assert_close(_run(wl), oracle.cldos_run(wl))
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_full_chain_spin_hex_gk_path(gpu, prec):
     # 10 bands on a 64^2 grid: G(k) is formed for all 100 pairs in a workspace
