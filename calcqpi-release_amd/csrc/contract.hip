@@ -16,7 +16,10 @@
 // Mapping: a workgroup covers an 8 x 8 block of cells (one cell per lane) and
 // 4 waves; each wave takes groups of P sub-pixels, so the tile value
 // w_o(x-R) is uniform across a wave while G0 rows vary per lane.  Each lane
-// accumulates a and b for its P sub-pixels in registers.
+// accumulates a and b for its P sub-pixels in registers.  k_contract_st stages
+// the workgroup's window rows in LDS and is the one launched; k_contract,
+// which gathers them from memory per lane, serves windows whose staged rows
+// exceed the LDS.
 #include "cqpi_common.h"
 
 namespace {
@@ -139,18 +142,19 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
     }
 }
 
-// Many bands (b >= 6): the same loop nest with the window rows staged in LDS.
-// Per spin block and orbital o, the workgroup copies G0(pos+R)_{o,q} and
-// G0(-pos-R)_{q,o} for its 8 x 8 cells and every |R| <= w (two
-// (8+2w)^2 x b blocks) into LDS; the lanes then read them from there instead
-// of gathering a b^2 slot block per lane from memory.  The copies are redone
-// for each group of sub-pixels the waves take.
-template <int B, int P>
-__global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const double2 *__restrict__ G0w,
-                                                           const double2 *__restrict__ Tm,
+// The same loop nest with the window rows staged in LDS.  Per spin block and
+// orbital o, the workgroup copies G0(pos+R)_{o,q} and G0(-pos-R)_{q,o} for its
+// 8 x 8 cells and every |R| <= w (two (8+2w)^2 x b blocks) into LDS; the lanes
+// then read them from there instead of gathering a b^2 slot block per lane
+// from memory, where neighbouring cells re-read each other's slots.  The
+// copies are redone for each group of sub-pixels the waves take.
+template <typename Real, int B, int P>
+__global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typename Cplx<Real>::type *__restrict__ G0w,
+                                                           const typename Cplx<Real>::type *__restrict__ Tm,
                                                            const double *__restrict__ bulk,
-                                                           const double *__restrict__ tiles, double *__restrict__ ldos)
+                                                           const Real *__restrict__ tiles, double *__restrict__ ldos)
 {
+    using C = typename Cplx<Real>::type;
     extern __shared__ __align__(16) unsigned char smem_c[];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -163,14 +167,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const doubl
     constexpr int NP = B * B;
     const int n2 = g.n >> 1;
     const int SA = kCells + 2 * w;
-    double2 *GAs = reinterpret_cast<double2 *>(smem_c);   // [lx][ly][q]
-    double2 *GBs = GAs + (size_t)SA * SA * B;              // [mx][my][q]
+    C *GAs = reinterpret_cast<C *>(smem_c);   // [lx][ly][q]
+    C *GBs = GAs + (size_t)SA * SA * B;       // [mx][my][q]
     const int npix = os * os;
     const int npg = (npix + P - 1) / P;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
-    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
-    const double2 *Te = Tm + (size_t)e * NP;
+    const C *G0 = G0w + (size_t)e * nw * nw * NP;
+    const C *Te = Tm + (size_t)e * NP;
     double *out = ldos + (size_t)e * W * W;
     // slot origins of the staged blocks: GA slot (ax0 + lx, ay0 + ly), GB slot (bx0 + mx, by0 + my)
     const int ax0 = cx0 - n2 - w + h, ay0 = cy0 - n2 - w + h;
@@ -189,25 +193,25 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const doubl
             ii[p] = pix / os;
             jj[p] = pix - ii[p] * os;
         }
-        double val[P];
+        Real val[P];
 #pragma unroll
-        for (int p = 0; p < P; p++) val[p] = 0.0;
+        for (int p = 0; p < P; p++) val[p] = 0;
         for (int sg = 0; sg < nsets; sg++) {
             const int olo = sg * nper, ohi = olo + nper;
-            double2 acca[P][B], accb[P][B];
+            C acca[P][B], accb[P][B];
 #pragma unroll
             for (int p = 0; p < P; p++)
 #pragma unroll
                 for (int q = 0; q < B; q++) {
-                    acca[p][q] = make_double2(0.0, 0.0);
-                    accb[p][q] = make_double2(0.0, 0.0);
+                    acca[p][q] = C{0, 0};
+                    accb[p][q] = C{0, 0};
                 }
             for (int o = olo; o < ohi; o++) {
                 __syncthreads();  // the previous orbital's rows are read
                 for (int t = threadIdx.x; t < SA * SA * B; t += blockDim.x) {
                     const int q = t % B, sl = t / B, ly = sl % SA, lx = sl / SA;
                     const int axs = ax0 + lx, ays = ay0 + ly, bxs = bx0 + lx, bys = by0 + ly;
-                    double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+                    C va = C{0, 0}, vb = C{0, 0};
                     if (axs >= 0 && axs < nw && ays >= 0 && ays < nw)
                         va = G0[((size_t)axs * nw + ays) * NP + o * B + q];
                     if (bxs >= 0 && bxs < nw && bys >= 0 && bys < nw)
@@ -220,17 +224,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const doubl
                     for (int rx = -w; rx <= w; rx++) {
                         for (int ry = -w; ry <= w; ry++) {
                             // GA slot (pos + R), GB slot (-pos - R), in block coordinates
-                            const double2 *ga = GAs + ((size_t)(lxo + rx + w) * SA + (lyo + ry + w)) * B;
-                            const double2 *gb = GBs + ((size_t)(kCells - 1 - lxo - rx + w) * SA +
+                            const C *ga = GAs + ((size_t)(lxo + rx + w) * SA + (lyo + ry + w)) * B;
+                            const C *gb = GBs + ((size_t)(kCells - 1 - lxo - rx + w) * SA +
                                                        (kCells - 1 - lyo - ry + w)) * B;
                             const int tx = (w - rx) * os, ty = (w - ry) * os;
-                            double wt[P];
+                            Real wt[P];
 #pragma unroll
                             for (int p = 0; p < P; p++)
                                 wt[p] = tiles[((size_t)o * T + ii[p] + tx) * T + jj[p] + ty];
 #pragma unroll
                             for (int q = 0; q < B; q++) {
-                                const double2 a = ga[q], b = gb[q];
+                                const C a = ga[q], b = gb[q];
 #pragma unroll
                                 for (int p = 0; p < P; p++) {
                                     acca[p][q].x = fma(wt[p], a.x, acca[p][q].x);
@@ -248,10 +252,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const doubl
             for (int q1 = 0; q1 < B; q1++) {
 #pragma unroll
                 for (int p = 0; p < P; p++) {
-                    double tbx = 0.0, tby = 0.0;
+                    Real tbx = 0, tby = 0;
 #pragma unroll
                     for (int q2 = 0; q2 < B; q2++) {
-                        const double2 tq = Te[q1 * B + q2];
+                        const C tq = Te[q1 * B + q2];
                         tbx += tq.x * accb[p][q2].x - tq.y * accb[p][q2].y;
                         tby += tq.x * accb[p][q2].y + tq.y * accb[p][q2].x;
                     }
@@ -264,13 +268,16 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const doubl
             for (int p = 0; p < P; p++) {
                 if (!ok[p]) continue;
                 const int X = cx * os + ii[p], Y = cy * os + jj[p];
-                out[(size_t)X + (size_t)Y * W] = -(val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+                out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
             }
         }
     }
 }
 
-size_t contract_st_lds(const Geom &g) { return (size_t)2 * (kCells + 2 * g.w) * (kCells + 2 * g.w) * g.B * sizeof(double2); }
+size_t contract_st_lds(const Geom &g, size_t csize)
+{
+    return (size_t)2 * (kCells + 2 * g.w) * (kCells + 2 * g.w) * g.B * csize;
+}
 
 // FP64 -> FP32 narrowing of the G0 windows and T of a chunk
 __global__ void k_narrow(const double2 *__restrict__ a, size_t na, const double2 *__restrict__ t, size_t nt,
@@ -287,9 +294,10 @@ int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double
 {
     constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 12 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
-    if (B >= 6 && contract_st_lds(g) <= 150 * 1024) {
-        hipLaunchKernelGGL((k_contract_st<B, P>), dim3(nb, nb, E), dim3(64 * kWaves), contract_st_lds(g), s, g, G0w,
-                           T, bulk, tiles, ldos);
+    const size_t lds = contract_st_lds(g, sizeof(double2));
+    if (lds <= 150 * 1024) {
+        hipLaunchKernelGGL((k_contract_st<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w, T,
+                           bulk, tiles, ldos);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     hipLaunchKernelGGL((k_contract<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
@@ -301,8 +309,15 @@ template <int B>
 int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double *bulk, const float *tiles, int E,
                double *ldos, hipStream_t s)
 {
-    constexpr int P = B <= 3 ? 8 : (B <= 6 ? 4 : 2);
+    // four sub-pixel groups of 4 for os = 4 (one per wave, the rows staged once)
+    constexpr int P = B <= 6 ? 4 : 2;
     int nb = (g.n + kCells - 1) / kCells;
+    const size_t lds = contract_st_lds(g, sizeof(float2));
+    if (lds <= 150 * 1024) {
+        hipLaunchKernelGGL((k_contract_st<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0f, Tf,
+                           bulk, tiles, ldos);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     hipLaunchKernelGGL((k_contract<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf, bulk,
                        tiles, ldos);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;

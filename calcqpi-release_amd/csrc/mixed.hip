@@ -28,9 +28,6 @@
 #include "cqpi_common.h"
 #include "mixed.h"
 
-#ifndef CMIX_JB3
-#define CMIX_JB3 2
-#endif
 namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -683,7 +680,7 @@ int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *A
     const half8 *A = (const half8 *)Afrag;
     // accumulator blocks of JB sub-pixel rows: fewer for wide N, so that the
     // accumulators leave room for more than one wave per SIMD
-    constexpr int JB = NT >= 3 ? CMIX_JB3 : 4;
+    constexpr int JB = NT >= 3 ? 2 : 4;
     switch (mg.LDW) {
     case 8: hipLaunchKernelGGL((k_cmix<NT, 8, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
     case 4: hipLaunchKernelGGL((k_cmix<NT, 4, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;

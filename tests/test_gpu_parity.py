@@ -58,6 +58,15 @@ def test_full_chain_lane_group_eigensolve(gpu, b):
     _assert_close(_run(wl), oracle.cldos_run(wl))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_full_chain_wide_window(gpu, prec):
+    # w = 31: the FP64 window rows of an 8 x 8-cell block ((8+2w)^2 slots, two
+    # blocks) exceed the LDS, so the FP64 contraction gathers them per lane;
+    # the FP32 rows still fit and are staged
+    wl = synth.square_norb(1, K=128, n=4, E=1, seed=41, w=31, os_=1, eta=0.02)
+    _assert_close(_run(wl, prec), oracle.cldos_run(wl), TOL64 if prec == "fp64" else TOLMIX)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_full_chain_spin_hex_gk_path(gpu, prec):
     # 10 bands on a 64^2 grid: G(k) is formed for all 100 pairs in a workspace
