@@ -34,7 +34,11 @@ __device__ inline double wsum_reduce(double v)
     return v;
 }
 
-// abar, bbar per pixel and per spin block; partial Tbar per workgroup
+// abar, bbar per pixel and per spin block; partial Tbar per workgroup.  The
+// gathers a, b are the forward contraction's, with each orbital's window rows
+// for the workgroup's 8 x 8 cells staged in LDS (stage_window_rows; at most
+// 2 (8+2w)^2 b complex values, 33 KB for the largest window and band count
+// the adjoint accepts).
 template <int B, int P>
 __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *__restrict__ G0w,
                                                        const double2 *__restrict__ Tm,
@@ -44,27 +48,38 @@ __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *
                                                        double2 *__restrict__ Tpart)
 {
     __shared__ double2 red[kWaves][B * B];
+    extern __shared__ __align__(16) unsigned char smem_p[];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int cx = blockIdx.x * kCells + (lane & (kCells - 1));
-    const int cy = blockIdx.y * kCells + (lane >> 3);
+    const int cx0 = blockIdx.x * kCells, cy0 = blockIdx.y * kCells;
+    const int cx = cx0 + (lane & (kCells - 1));
+    const int cy = cy0 + (lane >> 3);
     const int e = blockIdx.z;
     const bool active = cx < g.n && cy < g.n;
     const int os = g.os, w = g.w, T = g.T, nw = g.nw, h = g.h, W = g.W;
     const int NP = B * B;
     const int n2 = g.n >> 1;
-    const int px = (active ? cx : 0) - n2, py = (active ? cy : 0) - n2;
     const int npix = os * os;
     const int npg = (npix + P - 1) / P;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
     const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
     const double2 *Te = Tm + (size_t)e * NP;
+    // staged blocks: slot origins as in the forward contraction
+    const int SA = kCells + 2 * w;
+    double2 *GAs = reinterpret_cast<double2 *>(smem_p);
+    double2 *GBs = GAs + (size_t)SA * SA * B;
+    const int ax0 = cx0 - n2 - w + h, ay0 = cy0 - n2 - w + h;
+    const int bx0 = h - (cx0 - n2) - (kCells - 1) - w, by0 = h - (cy0 - n2) - (kCells - 1) - w;
+    const int lxo = cx - cx0, lyo = cy - cy0;
     double2 tacc[B * B];
 #pragma unroll
     for (int i = 0; i < NP; i++) tacc[i] = make_double2(0.0, 0.0);
 
-    for (int pg = wave; pg < npg; pg += kWaves) {
+    // every thread runs the same number of iterations (the staging barriers)
+    for (int it = 0; it * kWaves < npg; it++) {
+        const int pg = it * kWaves + wave;
+        const bool run = pg < npg;
         int ii[P], jj[P];
         bool ok[P];
 #pragma unroll
@@ -85,26 +100,32 @@ __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *
                     a[p][q] = make_double2(0.0, 0.0);
                     b[p][q] = make_double2(0.0, 0.0);
                 }
-            for (int rx = -w; rx <= w; rx++)
-                for (int ry = -w; ry <= w; ry++) {
-                    const double2 *GA = G0 + ((size_t)(px + rx + h) * nw + (py + ry + h)) * NP;
-                    const double2 *GB = G0 + ((size_t)(-px - rx + h) * nw + (-py - ry + h)) * NP;
-                    const int tx = (w - rx) * os, ty = (w - ry) * os;
-                    for (int o = olo; o < ohi; o++) {
+            for (int o = olo; o < ohi; o++) {
+                __syncthreads();  // the previous orbital's rows are read
+                stage_window_rows<double2, B>(G0, nw, o, SA, ax0, ay0, bx0, by0, GAs, GBs);
+                __syncthreads();
+                if (!run) continue;
+                for (int rx = -w; rx <= w; rx++)
+                    for (int ry = -w; ry <= w; ry++) {
+                        const double2 *ga = GAs + ((size_t)(lxo + rx + w) * SA + (lyo + ry + w)) * B;
+                        const double2 *gb =
+                            GBs + ((size_t)(kCells - 1 - lxo - rx + w) * SA + (kCells - 1 - lyo - ry + w)) * B;
+                        const int tx = (w - rx) * os, ty = (w - ry) * os;
                         double wt[P];
 #pragma unroll
-                        for (int p = 0; p < P; p++) wt[p] = tiles[((size_t)o * T + ii[p] + tx) * T + jj[p] + ty];
+                        for (int p = 0; p < P; p++)
+                            wt[p] = tiles[((size_t)o * T + ii[p] + tx) * T + jj[p] + ty];
 #pragma unroll
                         for (int q = 0; q < B; q++) {
-                            double2 ga = GA[o * B + q], gb = GB[q * B + o];
+                            const double2 va = ga[q], vb = gb[q];
 #pragma unroll
                             for (int p = 0; p < P; p++) {
-                                a[p][q] = cadd(a[p][q], cscale(wt[p], ga));
-                                b[p][q] = cadd(b[p][q], cscale(wt[p], gb));
+                                a[p][q] = cadd(a[p][q], cscale(wt[p], va));
+                                b[p][q] = cadd(b[p][q], cscale(wt[p], vb));
                             }
                         }
                     }
-                }
+            }
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 if (!ok[p]) continue;
@@ -747,7 +768,9 @@ int launch_pix_t(const Geom &g, const double2 *G0w, const double2 *T, const doub
     constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 6 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
     nblk = nb * nb;
-    hipLaunchKernelGGL((k_adj_pix<B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, tiles, rhobar,
+    const size_t lds = (size_t)2 * (kCells + 2 * g.w) * (kCells + 2 * g.w) * B * sizeof(double2);
+    if (lds > 64 * 1024) return CQPI_E_ARG;  // beyond the windows adjoint_supported accepts
+    hipLaunchKernelGGL((k_adj_pix<B, P>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w, T, tiles, rhobar,
                        abar, bbar, Tpart);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }

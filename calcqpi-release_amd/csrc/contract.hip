@@ -208,17 +208,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typen
                 }
             for (int o = olo; o < ohi; o++) {
                 __syncthreads();  // the previous orbital's rows are read
-                for (int t = threadIdx.x; t < SA * SA * B; t += blockDim.x) {
-                    const int q = t % B, sl = t / B, ly = sl % SA, lx = sl / SA;
-                    const int axs = ax0 + lx, ays = ay0 + ly, bxs = bx0 + lx, bys = by0 + ly;
-                    C va = C{0, 0}, vb = C{0, 0};
-                    if (axs >= 0 && axs < nw && ays >= 0 && ays < nw)
-                        va = G0[((size_t)axs * nw + ays) * NP + o * B + q];
-                    if (bxs >= 0 && bxs < nw && bys >= 0 && bys < nw)
-                        vb = G0[((size_t)bxs * nw + bys) * NP + q * B + o];
-                    GAs[t] = va;
-                    GBs[t] = vb;
-                }
+                stage_window_rows<C, B>(G0, nw, o, SA, ax0, ay0, bx0, by0, GAs, GBs);
                 __syncthreads();
                 if (pg < npg && cell_ok) {
                     for (int rx = -w; rx <= w; rx++) {

@@ -118,6 +118,28 @@ __device__ inline void xcd_block(int &bx, int &by, int &bz)
     bx = (int)(rem - (unsigned)by * nx);
 }
 
+// Window rows of one orbital o for a square block of cells, staged in LDS by
+// the whole workgroup.  Slot coordinates are window slots (R + h).  For a
+// block whose cells need G0(pos+R) at slots (ax0 + lx, ay0 + ly) and
+// G0(-pos-R) at slots (bx0 + lx, by0 + ly), 0 <= lx, ly < SA:
+//   GAs[(lx SA + ly) B + q] = G0(slot a)_{o,q},  GBs[...] = G0(slot b)_{q,o},
+// zero where a slot falls outside the nw x nw window.  G0 points at one
+// energy's window, [slot x][slot y][pair].
+template <typename C, int B>
+__device__ inline void stage_window_rows(const C *__restrict__ G0, int nw, int o, int SA, int ax0, int ay0, int bx0,
+                                         int by0, C *GAs, C *GBs)
+{
+    for (int t = threadIdx.x; t < SA * SA * B; t += blockDim.x) {
+        const int q = t % B, sl = t / B, ly = sl % SA, lx = sl / SA;
+        const int axs = ax0 + lx, ays = ay0 + ly, bxs = bx0 + lx, bys = by0 + ly;
+        C va = C{0, 0}, vb = C{0, 0};
+        if (axs >= 0 && axs < nw && ays >= 0 && ays < nw) va = G0[((size_t)axs * nw + ays) * (B * B) + o * B + q];
+        if (bxs >= 0 && bxs < nw && bys >= 0 && bys < nw) vb = G0[((size_t)bxs * nw + bys) * (B * B) + q * B + o];
+        GAs[t] = va;
+        GBs[t] = vb;
+    }
+}
+
 struct MixGeom;
 
 // launchers (defined next to their kernels)
