@@ -240,6 +240,7 @@ __global__ void k_adj_q(Geom g, const double *__restrict__ tiles, double *__rest
 constexpr int kWinTX = 16;  // slots per workgroup along x
 constexpr int kWinTY = 8;   // slots per workgroup along y
 constexpr int kWinPix = 4;  // sub-pixel rows staged per phase
+constexpr int kStageCols = kWinTX * kWinTY / kWinPix;  // cell columns staged at once
 
 // Gbar on the slot window, one thread per slot holding all b x b entries:
 //   Gbar(S)_{o,p} += sum_R sum_x w_o(x;R) abar^{s(o)}_p(cell S-R, x)      (a gather transposed)
@@ -259,10 +260,12 @@ __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double
                                                             const double *__restrict__ Q,
                                                             double2 *__restrict__ Gbar)
 {
-    extern __shared__ double2 stage[];  // [a|b][kWinPix][RY][RX], then weights [kWinPix][NR][B]
+    extern __shared__ double2 stage[];  // [a|b][kWinPix][RY][RX] (planes padded to RC), weights [R][kWinPix][B]
     const int os = g.os, w = g.w, T = g.T, W = g.W, n = g.n, nw = g.nw, h = g.h, n2 = n >> 1;
-    const int RX = kWinTX + 2 * w, RY = kWinTY + 2 * w, RC = RX * RY;
-    const int side = 2 * w + 1, NR = side * side;
+    // RC: stride of the staged sub-pixel planes, padded by two values so the
+    // planes of one cell start on different LDS banks
+    const int RX = kWinTX + 2 * w, RY = kWinTY + 2 * w, RC = RX * RY + 2;
+    const int side = 2 * w + 1;
     double *wl = reinterpret_cast<double *>(stage + 2 * kWinPix * RC);
     const int tx = threadIdx.x % kWinTX, ty = threadIdx.x / kWinTX;
     const int S0x = (int)blockIdx.x * kWinTX - h, S0y = (int)blockIdx.y * kWinTY - h;
@@ -278,6 +281,13 @@ __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double
     const int oa = (ty + w) * RX + tx + w;
     const int ob = (kWinTY - 1 - ty + w) * RX + (kWinTX - 1 - tx) + w;
     const size_t TT = (size_t)T * T;
+    // staging lanes: sub-pixel row of the strip, first cell column; for the
+    // weights (row wi, orbital wo) and the first R row wr0
+    const int si = threadIdx.x % kWinPix, scx = threadIdx.x / kWinPix;
+    constexpr int kWgtStep = kWinTX * kWinTY / (kWinPix * B);
+    const bool wact = threadIdx.x < kWgtStep * kWinPix * B;
+    const int wio = threadIdx.x % (kWinPix * B), wr0 = threadIdx.x / (kWinPix * B);
+    const int wsi = wio / B, wso = wio % B;
     double2 acc[B * B];
 #pragma unroll
     for (int i = 0; i < B * B; i++) acc[i] = make_double2(0.0, 0.0);
@@ -292,42 +302,59 @@ __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double
                 for (int ii0 = 0; ii0 < os; ii0 += kWinPix) {
                     const int npx = min(kWinPix, os - ii0);
                     __syncthreads();
-                    for (int l = threadIdx.x; l < 2 * kWinPix * RC; l += kWinTX * kWinTY) {
-                        const int rx = l % RX;
-                        int rest = l / RX;
-                        const int i = rest % kWinPix;
-                        rest /= kWinPix;
-                        const int ry = rest % RY, ab = rest / RY;
-                        const int cx = (ab ? bx0 : ax0) + rx, cy = (ab ? by0 : ay0) + ry;
-                        double2 v = make_double2(0.0, 0.0);
-                        if (i < npx && cx >= 0 && cy >= 0 && cx < n && cy < n)
-                            v = (ab ? sb : sa)[(size_t)(cx * os + ii0 + i) + (size_t)(cy * os + jj) * W];
-                        stage[((size_t)(ab * kWinPix + i) * RY + ry) * RX + rx] = v;
+                    // lanes: the strip's sub-pixel row si fastest, then the cell
+                    // column, so a wave reads consecutive pixels of a map row
+#pragma unroll
+                    for (int ab = 0; ab < 2; ab++) {
+                        const double2 *src = ab ? sb : sa;
+                        const int x0 = ab ? bx0 : ax0, y0 = ab ? by0 : ay0;
+                        double2 *dst = stage + (size_t)(ab * kWinPix + si) * RC;
+                        for (int ry = 0; ry < RY; ry++) {
+                            const int cy = y0 + ry;
+                            const bool rowok = si < npx && cy >= 0 && cy < n;
+                            for (int rx = scx; rx < RX; rx += kStageCols) {
+                                const int cx = x0 + rx;
+                                double2 v = make_double2(0.0, 0.0);
+                                if (rowok && cx >= 0 && cx < n)
+                                    v = src[(size_t)(cx * os + ii0 + si) + (size_t)(cy * os + jj) * W];
+                                dst[ry * RX + rx] = v;
+                            }
+                        }
                     }
-                    // weights w_o(x;R) of this strip, zero outside the spin block
-                    for (int l = threadIdx.x; l < kWinPix * NR * B; l += kWinTX * kWinTY) {
-                        const int o = l % B, r = (l / B) % NR, i = l / (B * NR);
-                        double v = 0.0;
-                        if (i < npx && o >= olo && o < ohi)
-                            v = tiles[o * TT + (size_t)(ii0 + i + (2 * w - r / side) * os) * T + jj +
-                                      (2 * w - r % side) * os];
-                        wl[l] = v;
-                    }
+                    // weights w_o(x;R) of this strip as wl[R][row][o], zero outside
+                    // the spin block and past the strip's last sub-pixel; lanes
+                    // (row, o) fastest, then the R column
+                    if (wact)
+                        for (int rxw = 0; rxw < side; rxw++)
+                            for (int ryw = wr0; ryw < side; ryw += kWgtStep) {
+                                double v = 0.0;
+                                if (wsi < npx && wso >= olo && wso < ohi)
+                                    v = tiles[(size_t)wso * TT + (size_t)(ii0 + wsi + (2 * w - rxw) * os) * T + jj +
+                                              (2 * w - ryw) * os];
+                                wl[(rxw * side + ryw) * (kWinPix * B) + wio] = v;
+                            }
                     __syncthreads();
-                    for (int i = 0; i < npx; i++) {
-                        const double2 *pa = la + (size_t)i * RC + oa, *pb = lb + (size_t)i * RC + ob;
-                        const double *wi = wl + (size_t)i * NR * B;
-                        for (int rxi = -w; rxi <= w; rxi++) {
-                            const double *wr = wi + (size_t)(rxi + w) * side * B;
-#pragma unroll 4
-                            for (int ryi = -w; ryi <= w; ryi++) {
-                                const double2 va = pa[-ryi * RX - rxi], vb = pb[-ryi * RX - rxi];
-                                const double *wt = wr + (ryi + w) * B;
+                    // all kWinPix rows of the strip per R, so that each wait on
+                    // LDS covers 4 kWinPix B FMAs; rows past the strip's last
+                    // sub-pixel are staged as zeros with zero weights
+                    for (int rxi = -w; rxi <= w; rxi++) {
+                        for (int ryi = -w; ryi <= w; ryi++) {
+                            const int d = -ryi * RX - rxi;
+                            double2 va[kWinPix], vb[kWinPix];
+#pragma unroll
+                            for (int i = 0; i < kWinPix; i++) {
+                                va[i] = la[i * RC + oa + d];
+                                vb[i] = lb[i * RC + ob + d];
+                            }
+                            // the kWinPix x B weights of this R, contiguous
+                            const double *wr = wl + ((rxi + w) * side + (ryi + w)) * (kWinPix * B);
+#pragma unroll
+                            for (int i = 0; i < kWinPix; i++) {
 #pragma unroll
                                 for (int o = 0; o < B; o++) {
-                                    const double wo = wt[o];
-                                    acc[o * B + p] = cadd(acc[o * B + p], cscale(wo, va));
-                                    acc[p * B + o] = cadd(acc[p * B + o], cscale(wo, vb));
+                                    const double wo = wr[i * B + o];
+                                    acc[o * B + p] = cadd(acc[o * B + p], cscale(wo, va[i]));
+                                    acc[p * B + o] = cadd(acc[p * B + o], cscale(wo, vb[i]));
                                 }
                             }
                         }
@@ -353,79 +380,113 @@ __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double
     for (int i = 0; i < B * B; i++) out[i] = acc[i];
 }
 
-constexpr int kMaxB = 16;
-
-// Gbar(0) += T^H Tbar T^H;  Vbar[e] = M^-H Tbar (1 + G(0) T)^H
-__global__ void k_adj_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
-                              const double2 *__restrict__ Tm, const double2 *__restrict__ Tbar, int E,
-                              double2 *__restrict__ Gbar, double2 *__restrict__ Vbar)
+// Gbar(0) += T^H Tbar T^H;  Vbar[e] = M^-H Tbar (1 + G(0) T)^H.  One thread
+// per energy; B is a compile-time constant and every loop unrolled, so the
+// matrices stay in registers (the pivot row exchange goes by compares).
+template <int B>
+__global__ void __launch_bounds__(64) k_adj_tmatrix(Geom g, const double2 *__restrict__ G0w,
+                                                    const double2 *__restrict__ V, const double2 *__restrict__ Tm,
+                                                    const double2 *__restrict__ Tbar, int E,
+                                                    double2 *__restrict__ Gbar, double2 *__restrict__ Vbar)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const int B = g.B, NP = B * B;
+    constexpr int NP = B * B;
     const size_t off0 = ((size_t)e * g.nw * g.nw + (size_t)g.h * g.nw + g.h) * NP;
     const double2 *G = G0w + off0;
     const double2 *T = Tm + (size_t)e * NP;
     const double2 *Tb = Tbar + (size_t)e * NP;
-    double2 M[kMaxB * kMaxB], Mi[kMaxB * kMaxB], X[kMaxB * kMaxB];
+    double2 M[NP], Mi[NP], X[NP];
+#pragma unroll
     for (int r = 0; r < B; r++)
+#pragma unroll
         for (int c = 0; c < B; c++) {
             double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+#pragma unroll
             for (int k = 0; k < B; k++) acc = csub(acc, cmul(V[r * B + k], G[k * B + c]));
-            M[r * kMaxB + c] = acc;
-            Mi[r * kMaxB + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+            M[r * B + c] = acc;
+            Mi[r * B + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
         }
+    bool sing = false;
+#pragma unroll
     for (int j = 0; j < B; j++) {  // Gauss-Jordan, partial pivoting
         int piv = j;
-        double best = cabs2(M[j * kMaxB + j]);
+        double best = cabs2(M[j * B + j]);
+#pragma unroll
         for (int i = j + 1; i < B; i++)
-            if (cabs2(M[i * kMaxB + j]) > best) { best = cabs2(M[i * kMaxB + j]); piv = i; }
-        if (best == 0.0) break;
-        if (piv != j)
-            for (int c = 0; c < B; c++) {
-                double2 t = M[j * kMaxB + c]; M[j * kMaxB + c] = M[piv * kMaxB + c]; M[piv * kMaxB + c] = t;
-                t = Mi[j * kMaxB + c]; Mi[j * kMaxB + c] = Mi[piv * kMaxB + c]; Mi[piv * kMaxB + c] = t;
+            if (cabs2(M[i * B + j]) > best) {
+                best = cabs2(M[i * B + j]);
+                piv = i;
             }
-        double2 ip = cinv(M[j * kMaxB + j]);
+        // a zero pivot (an energy the forward flagged singular) ends the elimination
+        sing = sing || best == 0.0;
+        if (sing) continue;
+#pragma unroll
+        for (int i = j + 1; i < B; i++)
+            if (i == piv)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 t = M[j * B + c];
+                    M[j * B + c] = M[i * B + c];
+                    M[i * B + c] = t;
+                    t = Mi[j * B + c];
+                    Mi[j * B + c] = Mi[i * B + c];
+                    Mi[i * B + c] = t;
+                }
+        const double2 ip = cinv(M[j * B + j]);
+#pragma unroll
         for (int i = 0; i < B; i++) {
             if (i == j) continue;
-            double2 l = cmul(M[i * kMaxB + j], ip);
+            const double2 l = cmul(M[i * B + j], ip);
+#pragma unroll
             for (int c = 0; c < B; c++) {
-                M[i * kMaxB + c] = csub(M[i * kMaxB + c], cmul(l, M[j * kMaxB + c]));
-                Mi[i * kMaxB + c] = csub(Mi[i * kMaxB + c], cmul(l, Mi[j * kMaxB + c]));
+                M[i * B + c] = csub(M[i * B + c], cmul(l, M[j * B + c]));
+                Mi[i * B + c] = csub(Mi[i * B + c], cmul(l, Mi[j * B + c]));
             }
         }
+#pragma unroll
         for (int c = 0; c < B; c++) {
-            M[j * kMaxB + c] = cmul(M[j * kMaxB + c], ip);
-            Mi[j * kMaxB + c] = cmul(Mi[j * kMaxB + c], ip);
+            M[j * B + c] = cmul(M[j * B + c], ip);
+            Mi[j * B + c] = cmul(Mi[j * B + c], ip);
         }
     }
     // X = 1 + G T
+#pragma unroll
     for (int r = 0; r < B; r++)
+#pragma unroll
         for (int c = 0; c < B; c++) {
             double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
+#pragma unroll
             for (int k = 0; k < B; k++) acc = cadd(acc, cmul(G[r * B + k], T[k * B + c]));
-            X[r * kMaxB + c] = acc;
+            X[r * B + c] = acc;
         }
     // Vbar = Mi^H Tb X^H
     double2 *Vb = Vbar + (size_t)e * NP;
+#pragma unroll
     for (int r = 0; r < B; r++)
+#pragma unroll
         for (int c = 0; c < B; c++) {
             double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
             for (int k = 0; k < B; k++) {
                 double2 tx = make_double2(0.0, 0.0);  // (Tb X^H)_{kc}
-                for (int l = 0; l < B; l++) tx = cadd(tx, cmul(Tb[k * B + l], cconj(X[c * kMaxB + l])));
-                acc = cadd(acc, cmul(cconj(Mi[k * kMaxB + r]), tx));
+#pragma unroll
+                for (int l = 0; l < B; l++) tx = cadd(tx, cmul(Tb[k * B + l], cconj(X[c * B + l])));
+                acc = cadd(acc, cmul(cconj(Mi[k * B + r]), tx));
             }
             Vb[r * B + c] = acc;
         }
     // Gbar(0) += T^H Tb T^H
     double2 *G0b = Gbar + off0;
+#pragma unroll
     for (int r = 0; r < B; r++)
+#pragma unroll
         for (int c = 0; c < B; c++) {
             double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
             for (int k = 0; k < B; k++) {
                 double2 tt = make_double2(0.0, 0.0);  // (Tb T^H)_{kc}
+#pragma unroll
                 for (int l = 0; l < B; l++) tt = cadd(tt, cmul(Tb[k * B + l], cconj(T[c * B + l])));
                 acc = cadd(acc, cmul(cconj(T[k * B + r]), tt));
             }
@@ -777,7 +838,7 @@ int launch_pix_t(const Geom &g, const double2 *G0w, const double2 *T, const doub
 
 size_t win_lds(const Geom &g)
 {
-    return sizeof(double2) * 2 * kWinPix * (size_t)(kWinTX + 2 * g.w) * (kWinTY + 2 * g.w) +
+    return sizeof(double2) * 2 * kWinPix * ((size_t)(kWinTX + 2 * g.w) * (kWinTY + 2 * g.w) + 2) +
            sizeof(double) * kWinPix * (size_t)(2 * g.w + 1) * (2 * g.w + 1) * g.B;
 }
 
@@ -855,7 +916,7 @@ bool adjoint_supported(const Geom &g)
 {
     bool pow2 = (g.K & (g.K - 1)) == 0;
     size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
-    return pow2 && g.B <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && win_lds(g) <= 64 * 1024;
+    return pow2 && g.B <= 4 && g.w <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && win_lds(g) <= 64 * 1024;
 }
 
 int adjoint_energy_groups(const Geom &g, int E) { return cqpi_energy_groups(g, E); }
@@ -884,7 +945,15 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case 4: r = launch_win_t<4>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
     }
     if (r) return r;
-    hipLaunchKernelGGL(k_adj_tmatrix, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar);
+    switch (g.B) {
+#define CASE(b)                                                                                                   \
+    case b:                                                                                                       \
+        hipLaunchKernelGGL(k_adj_tmatrix<b>, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar); \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4)
+#undef CASE
+    default: return CQPI_E_ARG;
+    }
     // adjoint transform along kx (pairs chunked to the LDS budget), then ky + resolvent
     int npmax = (int)((32 * 1024) / (sizeof(double2) * (size_t)g.K));
     if (npmax < 1) npmax = 1;
