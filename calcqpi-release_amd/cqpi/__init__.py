@@ -39,6 +39,7 @@ SIGNATURES = {
     "cqpi_device_count": (C.c_int, []),
     "cqpi_create": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _sz, C.c_int, _dp, C.POINTER(_vp)]),
     "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
+    "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_retrieve": (C.c_int, [_vp, _dp]),
     "cqpi_printinfo": (C.c_int, [_vp, C.c_char_p, _sz]),
     "cqpi_destroy": (None, [_vp]),
@@ -169,9 +170,7 @@ class AmdQPI:
         self._nhops = 0
 
     # -- GPUQPI ------------------------------------------------------------
-    def wannierldos(self, scat, g0):
-        """scat: T (b x b complex); g0: G0(R) on the k-grid, shape (K, K, b, b)
-        complex, g0[ix, iy] = G0(R = (ix, iy) mod K)."""
+    def _dropin_args(self, scat, g0):
         b = self.bands
         scat = np.ascontiguousarray(scat, dtype=np.complex128)
         g0 = np.ascontiguousarray(g0, dtype=np.complex128)
@@ -179,8 +178,24 @@ class AmdQPI:
             raise ValueError("shape mismatch")
         base = g0.ctypes.data
         ptrs = (base + np.arange(self.K * self.K, dtype=np.uint64) * np.uint64(b * b * 16)).astype(np.uint64)
+        return scat, g0, ptrs
+
+    def wannierldos(self, scat, g0):
+        """scat: T (b x b complex); g0: G0(R) on the k-grid, shape (K, K, b, b)
+        complex, g0[ix, iy] = G0(R = (ix, iy) mod K)."""
+        scat, g0, ptrs = self._dropin_args(scat, g0)
         _check(lib().cqpi_wannierldos(self._ctx, _d(scat.view(np.float64)),
-                                      ptrs.ctypes.data_as(C.POINTER(_vp)), b))
+                                      ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
+        self.nlayers = 1
+
+    def wannierjosephson(self, scat, g0, tip):
+        """As wannierldos, for the Josephson map of a superconducting model
+        (particle-hole block, columns o + maxband) with the complex tip factor
+        (src/qpihipkernel.cc:600-622)."""
+        scat, g0, ptrs = self._dropin_args(scat, g0)
+        tip = complex(tip)
+        _check(lib().cqpi_wannierjosephson(self._ctx, _d(scat.view(np.float64)),
+                                           ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands, tip.real, tip.imag))
         self.nlayers = 1
 
     def retrieveResult(self, ldos=None):

@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_adj_pix(Geom g, const double2 *
                 }
             for (int o = olo; o < ohi; o++) {
                 __syncthreads();  // the previous orbital's rows are read
-                stage_window_rows<double2, B>(G0, nw, o, SA, ax0, ay0, bx0, by0, GAs, GBs);
+                stage_window_rows<double2, B>(G0, nw, o, o, SA, ax0, ay0, bx0, by0, GAs, GBs);
                 __syncthreads();
                 if (!run) continue;
                 for (int rx = -w; rx <= w; rx++)

@@ -133,6 +133,8 @@ struct cqpi_ctx {
     // per-run buffers
     double2 *d_T = nullptr;
     size_t c_T = 0;
+    double2 *d_tips = nullptr;  // tip factor of each energy of a Josephson map
+    size_t c_tips = 0;
     int *d_status = nullptr;
     size_t c_status = 0;
     double *d_omega = nullptr;
@@ -203,7 +205,7 @@ void release(cqpi_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
-                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_status, c->d_omega, c->d_ldos,
+                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
@@ -292,17 +294,25 @@ int mix_buffers(cqpi_ctx *c, int E)
 // device.  planes_ready: the transform already wrote the mixed-precision
 // planes and scales (whole-chain path); otherwise they are made from G0w.
 int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos, bool planes_ready,
-             size_t eo = 0, hipStream_t s = nullptr)
+             size_t eo = 0, hipStream_t s = nullptr, const double2 *tips = nullptr)
 {
     // eo: offset of these energies in the chunk buffers (periodic term, planes,
-    // scales, FP32 copies); s: stream (default: the context stream)
+    // scales, FP32 copies); s: stream (default: the context stream); tips:
+    // device tip factors of the E energies for the Josephson map (NULL: LDOS)
     const Geom &g = c->g;
     if (!s) s = c->stream;
     const int NP = g.B * g.B;
-    RETCHK(dev_grow(c->d_bulk, c->c_bulk, (eo + E) * g.os * g.os, c->gpumem));
-    double *bulk = c->d_bulk + eo * g.os * g.os;
+    if (tips && c->precision == CQPI_PREC_MIXED)
+        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
+    if (tips && 2 * g.maxb > g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
+    const size_t per = tips ? 2 : 1;  // doubles per periodic-table entry
+    RETCHK(dev_grow(c->d_bulk, c->c_bulk, per * (eo + E) * g.os * g.os, c->gpumem));
+    double *bulk = c->d_bulk + per * eo * g.os * g.os;
     int st = stage_begin(c, 3);
-    RETCHK(launch_bulk(g, G0w, c->d_tiles, E, bulk, s));
+    if (tips)
+        RETCHK(launch_bulk_josephson(g, G0w, c->d_tiles, E, reinterpret_cast<double2 *>(bulk), s));
+    else
+        RETCHK(launch_bulk(g, G0w, c->d_tiles, E, bulk, s));
     stage_end(c, st);
     st = stage_begin(c, 4);
     int r = 0;
@@ -323,10 +333,10 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
         RETCHK(dev_grow(c->d_G0f, c->c_G0f, (eo + E) * g.nw * g.nw * NP, c->gpumem));
         RETCHK(dev_grow(c->d_Tf, c->c_Tf, (eo + E) * NP, c->gpumem));
         r = launch_contract_fp32(g, G0w, T, bulk, c->d_tilesf, E, ldos, c->d_G0f + eo * g.nw * g.nw * NP,
-                                 c->d_Tf + eo * NP, s);
+                                 c->d_Tf + eo * NP, s, tips);
         break;
     default:
-        r = launch_contract_fp64(g, G0w, T, bulk, c->d_tiles, E, ldos, s);
+        r = launch_contract_fp64(g, G0w, T, bulk, c->d_tiles, E, ldos, s, tips);
         break;
     }
     stage_end(c, st);
@@ -519,6 +529,66 @@ int finish_run(cqpi_ctx *c, const RunPlan &plan)
 
 }  // namespace
 
+// One energy of the drop-in path: the G0(R) window and T handed over by the
+// caller, then the LDOS map (tip == NULL) or the Josephson map with the tip
+// factor tip[0] + i tip[1].
+static int dropin_map(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda, const double *tip)
+{
+    g_err.clear();
+    if (!c || !scat || !g0) return fail(CQPI_E_ARG, "NULL argument");
+    const Geom &g = c->g;
+    const int B = g.B, NP = B * B, K = g.K;
+    if (tda < (size_t)B) return fail(CQPI_E_ARG, "tda < bands");
+    size_t nwin = (size_t)g.nw * g.nw * NP;
+    // previous work must be done before the pinned staging and window buffer are reused
+    HIPCHK(hipStreamSynchronize(c->stream));
+    reset_stages(c);
+    if (nwin > c->c_hwin) {
+        if (c->h_win) hipHostFree(c->h_win);
+        c->h_win = nullptr;
+        c->c_hwin = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_win, nwin * sizeof(double2)));
+        c->c_hwin = nwin;
+    }
+    // T (up to 16 x 16) and, after it, the tip factor
+    if (!c->h_T) HIPCHK(hipHostMalloc((void **)&c->h_T, (16 * 16 + 1) * sizeof(double2)));
+    // window of G0(R), R = slot - h wrapped onto the k-grid (src/qpihipkernel.cc:383-393)
+    for (int sx = 0; sx < g.nw; sx++) {
+        int ix = ((sx - g.h) % K + K) % K;
+        for (int sy = 0; sy < g.nw; sy++) {
+            int iy = ((sy - g.h) % K + K) % K;
+            const double *m = g0[(size_t)ix * K + iy];
+            if (!m) return fail(CQPI_E_ARG, "NULL matrix in g0 table");
+            double2 *dst = c->h_win + ((size_t)sx * g.nw + sy) * NP;
+            for (int r = 0; r < B; r++)
+                for (int cc = 0; cc < B; cc++)
+                    dst[r * B + cc] = make_double2(m[2 * (r * tda + cc)], m[2 * (r * tda + cc) + 1]);
+        }
+    }
+    for (int i = 0; i < NP; i++) c->h_T[i] = make_double2(scat[2 * i], scat[2 * i + 1]);
+    if (tip) {
+        c->h_T[16 * 16] = make_double2(tip[0], tip[1]);
+        RETCHK(dev_grow(c->d_tips, c->c_tips, 1, c->gpumem));
+    }
+    RETCHK(dev_grow(c->d_G0w, c->c_G0w, nwin, c->gpumem));
+    RETCHK(dev_grow(c->d_T, c->c_T, (size_t)NP, c->gpumem));
+    RETCHK(dev_grow(c->d_ldos, c->c_ldos, (size_t)g.W * g.W, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_G0w, c->h_win, nwin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_T, c->h_T, NP * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    if (tip)
+        HIPCHK(hipMemcpyAsync(c->d_tips, c->h_T + 16 * 16, sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->copy_done, c->stream));
+    RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos, false, 0, nullptr, tip ? c->d_tips : nullptr));
+    c->nlayers = 1;
+    c->windows_kept = false;
+    c->window_whole = true;
+    c->last_chunk_e0 = 0;
+    c->last_chunk_ec = 1;
+    // the caller may overwrite g0 and scat once the copies have landed
+    HIPCHK(hipEventSynchronize(c->copy_done));
+    return 0;
+}
+
 extern "C" {
 
 const char *cqpi_last_error(void) { return g_err.c_str(); }
@@ -649,52 +719,14 @@ int cqpi_stage_times(cqpi_ctx *c, double *ms, int64_t *launches)
 
 int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda)
 {
-    g_err.clear();
-    if (!c || !scat || !g0) return fail(CQPI_E_ARG, "NULL argument");
-    const Geom &g = c->g;
-    const int B = g.B, NP = B * B, K = g.K;
-    if (tda < (size_t)B) return fail(CQPI_E_ARG, "tda < bands");
-    size_t nwin = (size_t)g.nw * g.nw * NP;
-    // previous work must be done before the pinned staging and window buffer are reused
-    HIPCHK(hipStreamSynchronize(c->stream));
-    reset_stages(c);
-    if (nwin > c->c_hwin) {
-        if (c->h_win) hipHostFree(c->h_win);
-        c->h_win = nullptr;
-        c->c_hwin = 0;
-        HIPCHK(hipHostMalloc((void **)&c->h_win, nwin * sizeof(double2)));
-        c->c_hwin = nwin;
-    }
-    if (!c->h_T) HIPCHK(hipHostMalloc((void **)&c->h_T, 16 * 16 * sizeof(double2)));
-    // window of G0(R), R = slot - h wrapped onto the k-grid (src/qpihipkernel.cc:383-393)
-    for (int sx = 0; sx < g.nw; sx++) {
-        int ix = ((sx - g.h) % K + K) % K;
-        for (int sy = 0; sy < g.nw; sy++) {
-            int iy = ((sy - g.h) % K + K) % K;
-            const double *m = g0[(size_t)ix * K + iy];
-            if (!m) return fail(CQPI_E_ARG, "NULL matrix in g0 table");
-            double2 *dst = c->h_win + ((size_t)sx * g.nw + sy) * NP;
-            for (int r = 0; r < B; r++)
-                for (int cc = 0; cc < B; cc++)
-                    dst[r * B + cc] = make_double2(m[2 * (r * tda + cc)], m[2 * (r * tda + cc) + 1]);
-        }
-    }
-    for (int i = 0; i < NP; i++) c->h_T[i] = make_double2(scat[2 * i], scat[2 * i + 1]);
-    RETCHK(dev_grow(c->d_G0w, c->c_G0w, nwin, c->gpumem));
-    RETCHK(dev_grow(c->d_T, c->c_T, (size_t)NP, c->gpumem));
-    RETCHK(dev_grow(c->d_ldos, c->c_ldos, (size_t)g.W * g.W, c->gpumem));
-    HIPCHK(hipMemcpyAsync(c->d_G0w, c->h_win, nwin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_T, c->h_T, NP * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->copy_done, c->stream));
-    RETCHK(contract(c, c->d_G0w, c->d_T, 1, c->d_ldos, false));
-    c->nlayers = 1;
-    c->windows_kept = false;
-    c->window_whole = true;
-    c->last_chunk_e0 = 0;
-    c->last_chunk_ec = 1;
-    // the caller may overwrite g0 and scat once the copies have landed
-    HIPCHK(hipEventSynchronize(c->copy_done));
-    return 0;
+    return dropin_map(c, scat, g0, tda, nullptr);
+}
+
+int cqpi_wannierjosephson(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda, double tip_re,
+                          double tip_im)
+{
+    const double tip[2] = {tip_re, tip_im};
+    return dropin_map(c, scat, g0, tda, tip);
 }
 
 int cqpi_retrieve(cqpi_ctx *c, double *ldos) { return cqpi_retrieve_layers(c, 0, 1, ldos); }

@@ -148,11 +148,19 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
 // then read them from there instead of gathering a b^2 slot block per lane
 // from memory, where neighbouring cells re-read each other's slots.  The
 // copies are redone for each group of sub-pixels the waves take.
-template <typename Real, int B, int P>
+//
+// J: the Josephson map of a superconducting model (src/qpihipkernel.cc:171-223,
+// spin :225-271; CPU src/qpi.cc:953-1033).  The b gather reads the
+// particle-hole column o + maxband, and per energy
+//   rho = - sum_s Im[ tip (a^T T b' + bulk') ]
+// with the complex periodic table bulk' (bulk holds E x os^2 double2) and the
+// tip factor tips[e].
+template <typename Real, int B, int P, bool J>
 __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typename Cplx<Real>::type *__restrict__ G0w,
                                                            const typename Cplx<Real>::type *__restrict__ Tm,
                                                            const double *__restrict__ bulk,
-                                                           const Real *__restrict__ tiles, double *__restrict__ ldos)
+                                                           const Real *__restrict__ tiles, double *__restrict__ ldos,
+                                                           const double2 *__restrict__ tips)
 {
     using C = typename Cplx<Real>::type;
     extern __shared__ __align__(16) unsigned char smem_c[];
@@ -193,9 +201,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typen
             ii[p] = pix / os;
             jj[p] = pix - ii[p] * os;
         }
-        Real val[P];
+        Real val[P], valr[P];  // Im and (J) Re of sum_s a^T T b
 #pragma unroll
-        for (int p = 0; p < P; p++) val[p] = 0;
+        for (int p = 0; p < P; p++) val[p] = valr[p] = 0;
         for (int sg = 0; sg < nsets; sg++) {
             const int olo = sg * nper, ohi = olo + nper;
             C acca[P][B], accb[P][B];
@@ -208,7 +216,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typen
                 }
             for (int o = olo; o < ohi; o++) {
                 __syncthreads();  // the previous orbital's rows are read
-                stage_window_rows<C, B>(G0, nw, o, SA, ax0, ay0, bx0, by0, GAs, GBs);
+                stage_window_rows<C, B>(G0, nw, o, J ? o + g.maxb : o, SA, ax0, ay0, bx0, by0, GAs, GBs);
                 __syncthreads();
                 if (pg < npg && cell_ok) {
                     for (int rx = -w; rx <= w; rx++) {
@@ -250,6 +258,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typen
                         tby += tq.x * accb[p][q2].y + tq.y * accb[p][q2].x;
                     }
                     val[p] += acca[p][q1].x * tby + acca[p][q1].y * tbx;
+                    if (J) valr[p] += acca[p][q1].x * tbx - acca[p][q1].y * tby;
                 }
             }
         }
@@ -258,7 +267,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_st(Geom g, const typen
             for (int p = 0; p < P; p++) {
                 if (!ok[p]) continue;
                 const int X = cx * os + ii[p], Y = cy * os + jj[p];
-                out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+                if (J) {
+                    // -Im[tip z], z = sum_s a^T T b' + bulk'
+                    const double2 bz = reinterpret_cast<const double2 *>(bulk)[(size_t)e * npix + ii[p] * os + jj[p]];
+                    const double2 tp = tips[e];
+                    const double zr = (double)valr[p] + bz.x, zi = (double)val[p] + bz.y;
+                    out[(size_t)X + (size_t)Y * W] = -(tp.x * zi + tp.y * zr);
+                } else {
+                    out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+                }
             }
         }
     }
@@ -280,14 +297,20 @@ __global__ void k_narrow(const double2 *__restrict__ a, size_t na, const double2
 
 template <int B>
 int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-               const double *tiles, int E, double *ldos, hipStream_t s)
+               const double *tiles, int E, double *ldos, const double2 *tips, hipStream_t s)
 {
     constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 12 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
     const size_t lds = contract_st_lds(g, sizeof(double2));
+    if (tips) {
+        if (lds > 150 * 1024) return CQPI_E_ARG;  // the Josephson map has no per-lane gather variant
+        hipLaunchKernelGGL((k_contract_st<double, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w,
+                           T, bulk, tiles, ldos, tips);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     if (lds <= 150 * 1024) {
-        hipLaunchKernelGGL((k_contract_st<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w, T,
-                           bulk, tiles, ldos);
+        hipLaunchKernelGGL((k_contract_st<double, B, P, false>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w,
+                           T, bulk, tiles, ldos, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     hipLaunchKernelGGL((k_contract<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
@@ -297,15 +320,21 @@ int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double
 
 template <int B>
 int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double *bulk, const float *tiles, int E,
-               double *ldos, hipStream_t s)
+               double *ldos, const double2 *tips, hipStream_t s)
 {
     // four sub-pixel groups of 4 for os = 4 (one per wave, the rows staged once)
     constexpr int P = B <= 6 ? 4 : 2;
     int nb = (g.n + kCells - 1) / kCells;
     const size_t lds = contract_st_lds(g, sizeof(float2));
+    if (tips) {
+        if (lds > 150 * 1024) return CQPI_E_ARG;
+        hipLaunchKernelGGL((k_contract_st<float, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0f, Tf,
+                           bulk, tiles, ldos, tips);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     if (lds <= 150 * 1024) {
-        hipLaunchKernelGGL((k_contract_st<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0f, Tf,
-                           bulk, tiles, ldos);
+        hipLaunchKernelGGL((k_contract_st<float, B, P, false>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0f,
+                           Tf, bulk, tiles, ldos, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     hipLaunchKernelGGL((k_contract<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf, bulk,
@@ -318,11 +347,12 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
 namespace cqpi {
 
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                         const double *tiles, int E, double *ldos, hipStream_t s)
+                         const double *tiles, int E, double *ldos, hipStream_t s, const double2 *tips)
 {
+    if (tips && 2 * g.maxb > g.B) return CQPI_E_ARG;
     switch (g.B) {
 #define CASE(b) \
-    case b: return launch_c64<b>(g, G0w, T, bulk, tiles, E, ldos, s);
+    case b: return launch_c64<b>(g, G0w, T, bulk, tiles, E, ldos, tips, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
         CASE(11) CASE(12) CASE(14) CASE(16)
 #undef CASE
@@ -331,14 +361,16 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
 }
 
 int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s)
+                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s,
+                         const double2 *tips)
 {
+    if (tips && 2 * g.maxb > g.B) return CQPI_E_ARG;
     const size_t na = (size_t)E * g.nw * g.nw * g.B * g.B, nt = (size_t)E * g.B * g.B;
     hipLaunchKernelGGL(k_narrow, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, G0w, na, T, nt, G0f, Tf);
     if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
     switch (g.B) {
 #define CASE(b) \
-    case b: return launch_c32<b>(g, G0f, Tf, bulk, tilesf, E, ldos, s);
+    case b: return launch_c32<b>(g, G0f, Tf, bulk, tilesf, E, ldos, tips, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
         CASE(11) CASE(12) CASE(14) CASE(16)
 #undef CASE

@@ -43,8 +43,6 @@ __host__ __device__ inline float2 cmulc(float2 a, float2 b)
 }
 __host__ __device__ inline float2 cscale(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
 
-// Hopping terms grouped by lattice vector: every term of group g shares
-// (rx, ry, rz) and the phase exp(+-2 pi i R.k) is evaluated once per group.
 // Hopping terms grouped by lattice vector R, 'anti' terms (phase
 // exp(-2 pi i R.k)) folded onto -R, each group's amplitudes summed into one
 // dense B x B block.  For integral R (kept mod K in irx, iry) the phase
@@ -122,19 +120,20 @@ __device__ inline void xcd_block(int &bx, int &by, int &bz)
 // the whole workgroup.  Slot coordinates are window slots (R + h).  For a
 // block whose cells need G0(pos+R) at slots (ax0 + lx, ay0 + ly) and
 // G0(-pos-R) at slots (bx0 + lx, by0 + ly), 0 <= lx, ly < SA:
-//   GAs[(lx SA + ly) B + q] = G0(slot a)_{o,q},  GBs[...] = G0(slot b)_{q,o},
-// zero where a slot falls outside the nw x nw window.  G0 points at one
-// energy's window, [slot x][slot y][pair].
+//   GAs[(lx SA + ly) B + q] = G0(slot a)_{o,q},  GBs[...] = G0(slot b)_{q,ob},
+// zero where a slot falls outside the nw x nw window (ob = o for the LDOS,
+// o + maxband for the particle-hole column of the Josephson map).  G0 points
+// at one energy's window, [slot x][slot y][pair].
 template <typename C, int B>
-__device__ inline void stage_window_rows(const C *__restrict__ G0, int nw, int o, int SA, int ax0, int ay0, int bx0,
-                                         int by0, C *GAs, C *GBs)
+__device__ inline void stage_window_rows(const C *__restrict__ G0, int nw, int o, int ob, int SA, int ax0, int ay0,
+                                         int bx0, int by0, C *GAs, C *GBs)
 {
     for (int t = threadIdx.x; t < SA * SA * B; t += blockDim.x) {
         const int q = t % B, sl = t / B, ly = sl % SA, lx = sl / SA;
         const int axs = ax0 + lx, ays = ay0 + ly, bxs = bx0 + lx, bys = by0 + ly;
         C va = C{0, 0}, vb = C{0, 0};
         if (axs >= 0 && axs < nw && ays >= 0 && ays < nw) va = G0[((size_t)axs * nw + ays) * (B * B) + o * B + q];
-        if (bxs >= 0 && bxs < nw && bys >= 0 && bys < nw) vb = G0[((size_t)bxs * nw + bys) * (B * B) + q * B + o];
+        if (bxs >= 0 && bxs < nw && bys >= 0 && bys < nw) vb = G0[((size_t)bxs * nw + bys) * (B * B) + q * B + ob];
         GAs[t] = va;
         GBs[t] = vb;
     }
@@ -176,6 +175,10 @@ int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, d
 // periodic (bulk) term per sub-pixel: bulk[e][ii*os+jj]
 int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, double *bulk,
                 hipStream_t s);
+// the Josephson map's complex periodic table: bulk[e][ii*os+jj] =
+// sum w_o1 w_o2 G0(R2-R1)_{o2, o1+maxband}
+int launch_bulk_josephson(const Geom &g, const double2 *G0w, const double *tiles, int E, double2 *bulk,
+                          hipStream_t s);
 // adjoint of the whole chain (see adjoint.hip)
 size_t adjoint_pix_blocks(const Geom &g);
 // doubles of the energy-independent weight-overlap table of the periodic term
@@ -189,11 +192,14 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
                    double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
                    double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
                    double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q, hipStream_t s);
-// contraction: ldos[e][x + y*W]
+// contraction: ldos[e][x + y*W].  With tips != NULL (device, one per energy)
+// the Josephson map instead: particle-hole column o + maxband, bulk the
+// complex periodic table of launch_bulk_josephson (needs 2 maxband <= b)
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                         const double *tiles, int E, double *ldos, hipStream_t s);
+                         const double *tiles, int E, double *ldos, hipStream_t s, const double2 *tips = nullptr);
 // FP32 contraction: narrows the E windows and T into G0f (E*nw*nw*b*b) and Tf
 // (E*b*b) first; tilesf are the tiles in FP32
 int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
-                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s);
+                         const float *tilesf, int E, double *ldos, float2 *G0f, float2 *Tf, hipStream_t s,
+                         const double2 *tips = nullptr);
 }  // namespace cqpi

@@ -94,11 +94,13 @@ __global__ void __launch_bounds__(64) k_tmatrix(Geom g, const double2 *__restric
 
 constexpr int kBulkThreads = 256;
 
+// J = false: bulk[e][pix] = sum w_o1 w_o2 Im G0(R2-R1)_{o2,o1} (double).
+// J = true: the Josephson map's complex table, column o1 + maxband (double2).
+template <bool J>
 __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__restrict__ G0w,
-                                                      const double *__restrict__ tiles,
-                                                      double *__restrict__ bulk)
+                                                      const double *__restrict__ tiles, void *__restrict__ out)
 {
-    __shared__ double red[kBulkThreads];
+    __shared__ double red[kBulkThreads], redr[J ? kBulkThreads : 1];
     const int pix = blockIdx.x, e = blockIdx.y;
     const int os = g.os, w = g.w, T = g.T, B = g.B, NP = B * B, nw = g.nw, h = g.h;
     const int ii = pix / os, jj = pix - ii * os;
@@ -106,8 +108,9 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__
     const int side = 2 * w + 1, NW = side * side;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
+    const int cofs = J ? g.maxb : 0;
     const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
-    double acc = 0.0;
+    double acc = 0.0, accr = 0.0;
     for (int t = threadIdx.x; t < NW * NW; t += blockDim.x) {
         int a1 = t / NW, a2 = t - a1 * NW;
         int r1x = a1 / side - w, r1y = a1 % side - w;
@@ -121,18 +124,29 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__
                 if (f1 == 0.0) continue;
                 for (int o2 = off; o2 < off + nper; o2++) {
                     double f2 = tiles[((size_t)o2 * T + x2) * T + y2];
-                    acc += f1 * f2 * G[o2 * B + o1].y;
+                    const double2 gv = G[o2 * B + o1 + cofs];
+                    acc += f1 * f2 * gv.y;
+                    if (J) accr += f1 * f2 * gv.x;
                 }
             }
         }
     }
     red[threadIdx.x] = acc;
+    if (J) redr[threadIdx.x] = accr;
     __syncthreads();
     for (int s = kBulkThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if (threadIdx.x < s) {
+            red[threadIdx.x] += red[threadIdx.x + s];
+            if (J) redr[threadIdx.x] += redr[threadIdx.x + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) bulk[(size_t)e * os * os + pix] = red[0];
+    if (threadIdx.x == 0) {
+        if (J)
+            static_cast<double2 *>(out)[(size_t)e * os * os + pix] = make_double2(redr[0], red[0]);
+        else
+            static_cast<double *>(out)[(size_t)e * os * os + pix] = red[0];
+    }
 }
 
 }  // namespace
@@ -159,7 +173,15 @@ int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, d
 int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, double *bulk,
                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bulk, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
+    hipLaunchKernelGGL(k_bulk<false>, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+int launch_bulk_josephson(const Geom &g, const double2 *G0w, const double *tiles, int E, double2 *bulk,
+                          hipStream_t s)
+{
+    if (2 * g.maxb > g.B) return CQPI_E_ARG;
+    hipLaunchKernelGGL(k_bulk<true>, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 

@@ -60,7 +60,8 @@ public:
         check(cqpi_printinfo(ctx, buf, sizeof buf));
         os << buf;
     }
-    void wannierldos(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    // T packed with ld = bands, and the pointer table of g0 (built once)
+    void pack(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
     {
         for (size_t i = 0; i < bands; i++)
             for (size_t j = 0; j < bands; j++) {
@@ -72,11 +73,17 @@ public:
             for (size_t ix = 0; ix < K; ix++)
                 for (size_t iy = 0; iy < K; iy++) table[ix * K + iy] = g0[ix][iy]->data;
         }
+    }
+    void wannierldos(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    {
+        pack(scat, g0);
         check(cqpi_wannierldos(ctx, scatbuf.data(), table.data(), g0[0][0]->tda));
     }
-    void wannierjosephson(gsl_matrix_complex *, std::vector<std::vector<gsl_matrix_complex *> > &, gsl_complex)
+    void wannierjosephson(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0,
+                          gsl_complex tip)
     {
-        unsupported("Josephson mode");
+        pack(scat, g0);
+        check(cqpi_wannierjosephson(ctx, scatbuf.data(), table.data(), g0[0][0]->tda, GSL_REAL(tip), GSL_IMAG(tip)));
     }
     void wannierldoslist(gsl_matrix_complex *, std::vector<std::vector<gsl_matrix_complex *> > &)
     {

@@ -98,6 +98,20 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
  * the contraction runs asynchronously. */
 int cqpi_wannierldos(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda);
 
+/* Replaces HipQPI::wannierjosephson(scat, g0, tip) (include/gpuqpi.h:24,
+ * src/qpihipkernel.cc:600-622; kernels :171-271, CPU src/qpi.cc:953-1033):
+ * the Josephson map of a superconducting model,
+ *   rho(x) = - sum Im[ tip (G0(pos+R2) T G0(-pos-R1) + G0(R2-R1))_{o2, o1+maxband} ]
+ *            w_o1(x-R1) w_o2(x-R2)
+ * over o1, o2 < maxband (one spin block at a time with spin), i.e. the
+ * particle-hole block.  The context must have been created with
+ * 2*maxband <= bands (the reference passes maxband = bands/2).  Arguments as
+ * cqpi_wannierldos; tip = tip_re + i tip_im (the driver's energy-dependent
+ * tip factor, src/calcqpi.cc:705-709).  FP64 or FP32 precision; retrieve the
+ * map with cqpi_retrieve. */
+int cqpi_wannierjosephson(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda, double tip_re,
+                          double tip_im);
+
 /* Replaces HipQPI::retrieveResult(double *ldos) (src/qpihipkernel.cc:690-704):
  * waits for the device and copies the W x W map of the last wannierldos call
  * (or layer 0 of the last cqpi_run) to ldos. */

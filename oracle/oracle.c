@@ -478,6 +478,78 @@ void or_wannierldos(size_t K, size_t n, size_t window, size_t oversamp, size_t b
 }
 
 /* ------------------------------------------------------------------------- */
+/* Josephson map, reference loop nest: tmatrix::calcwannierjosephson          */
+/* src/qpi.cc:953-1033 (CPU branch; GPU kernels src/qpihipkernel.cc:171-271).  */
+/* maxband orbitals carry tiles (the reference uses bands/2); the particle-   */
+/* hole column o1 + maxband is read and the sum is multiplied by tip before   */
+/* the imaginary part is taken.  maxband = 0 means bands/2.                   */
+/* ------------------------------------------------------------------------- */
+void or_wannierjosephson(size_t K, size_t n, size_t window, size_t oversamp, size_t b, size_t maxband,
+                         int spin, const double *Tin, const double *g0_in, const double *wf,
+                         const int *wfzero, double tip_re, double tip_im, double *ldos)
+{
+    const cplx *T = (const cplx *)Tin, *g0 = (const cplx *)g0_in;
+    const cplx tip = tip_re + I * tip_im;
+    size_t W = oversamp * n, TS = (2 * window + 1) * oversamp, bb = b * b;
+    if (!maxband) maxband = b >> 1;
+    size_t n2 = n >> 1, spinbands = maxband >> 1;
+    int w = (int)window;
+    for (size_t i = 0; i < W * W; i++) ldos[i] = 0.0;
+#define G0AT(ix, iy) (g0 + ((size_t)(ix) * K + (size_t)(iy)) * bb)
+#define WF(o, x, y) wf[((size_t)(o) * TS + (size_t)(x)) * TS + (size_t)(y)]
+#pragma omp parallel
+    {
+        cplx *buffer = (cplx *)malloc(bb * sizeof(cplx)), *cl = (cplx *)malloc(bb * sizeof(cplx));
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = 0; j < n; j++) {
+                long ipos = (long)i - (long)n2, jpos = (long)j - (long)n2;
+                for (int nni1 = -w; nni1 <= w; nni1++)
+                    for (int nnj1 = -w; nnj1 <= w; nnj1++) {
+                        size_t ax = (size_t)((2 * (long)K - (ipos + nni1)) % (long)K);
+                        size_t ay = (size_t)((2 * (long)K - (jpos + nnj1)) % (long)K);
+                        zgemm_small(b, T, G0AT(ax, ay), buffer);
+                        for (int nni2 = -w; nni2 <= w; nni2++)
+                            for (int nnj2 = -w; nnj2 <= w; nnj2++) {
+                                size_t bx = (size_t)(((long)K + ipos + nni2) % (long)K);
+                                size_t by = (size_t)(((long)K + jpos + nnj2) % (long)K);
+                                zgemm_small(b, G0AT(bx, by), buffer, cl);
+                                size_t cx = (size_t)(((long)K - nni1 + nni2) % (long)K);
+                                size_t cy = (size_t)(((long)K - nnj1 + nnj2) % (long)K);
+                                const cplx *gb = G0AT(cx, cy);
+                                for (size_t ii = 0; ii < oversamp; ii++)
+                                    for (size_t jj = 0; jj < oversamp; jj++) {
+                                        size_t x = ii + window * oversamp, y = jj + window * oversamp;
+                                        double *out = &ldos[((i * oversamp + ii) % W) + ((j * oversamp + jj) % W) * W];
+                                        size_t x1 = x - nni1 * oversamp, y1 = y - nnj1 * oversamp;
+                                        size_t x2 = x - nni2 * oversamp, y2 = y - nnj2 * oversamp;
+                                        int nsets = spin ? 2 : 1;
+                                        size_t nper = spin ? spinbands : maxband;
+                                        for (int sg = 0; sg < nsets; sg++) {
+                                            size_t off = sg ? spinbands : 0;
+                                            for (size_t o1 = off; o1 < off + nper; o1++) {
+                                                if (wfzero && wfzero[o1]) continue;
+                                                for (size_t o2 = off; o2 < off + nper; o2++) {
+                                                    if (wfzero && wfzero[o2]) continue;
+                                                    double factor = WF(o1, x1, y1) * WF(o2, x2, y2);
+                                                    size_t col = o1 + maxband;
+                                                    cplx val = (cl[o2 * b + col] + gb[o2 * b + col]) * tip;
+                                                    *out -= cimag(val) * factor;
+                                                }
+                                            }
+                                        }
+                                    }
+                            }
+                    }
+            }
+        free(buffer);
+        free(cl);
+    }
+#undef G0AT
+#undef WF
+}
+
+/* ------------------------------------------------------------------------- */
 /* Whole path for a list of energies (the energy loop of src/calcqpi.cc:671-760 */
 /* with green=normal and output=wannier).  ldos_out: E x W x W, x + y*W.       */
 /* Returns nonzero if a T-matrix inversion failed.                            */

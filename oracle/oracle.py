@@ -43,6 +43,8 @@ def lib():
         L.or_tmatrix.argtypes = [sz, _dp, _dp, _dp]
         L.or_tmatrix.restype = C.c_int
         L.or_wannierldos.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip, _dp]
+        L.or_wannierjosephson.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip,
+                                          C.c_double, C.c_double, _dp]
         L.or_cldos_run.argtypes = [C.c_void_p, sz, sz, C.c_double, C.c_int, sz, sz, sz, sz, sz,
                                    C.c_int, C.c_double, _dp, sz, _dp, _dp, _ip, _dp]
         L.or_cldos_run.restype = C.c_int
@@ -146,6 +148,26 @@ def wannierldos(g0R, T, tiles, n, window, oversamp, maxband=0, spin=False, wfzer
     lib().or_wannierldos(K, n, window, oversamp, b, maxband, int(spin),
                          _d(_c(T, np.complex128).view(np.float64)),
                          _d(_c(g0R, np.complex128).view(np.float64)), _d(tiles), wz, _d(out))
+    return out
+
+
+def wannierjosephson(g0R, T, tiles, n, window, oversamp, tip, maxband=0, spin=False, wfzero=None):
+    """Josephson map (src/qpi.cc:953-1033): tiles for maxband orbitals (0:
+    bands/2), particle-hole column o1 + maxband, complex tip factor."""
+    K, _, b, _ = g0R.shape
+    W = n * oversamp
+    out = np.zeros(W * W)
+    tiles = _c(tiles, np.float64)
+    if not maxband:
+        maxband = b // 2
+    wz = None
+    if wfzero is not None:
+        wz = _c(wfzero, np.int32).ctypes.data_as(_ip)
+    tip = complex(tip)
+    lib().or_wannierjosephson(K, n, window, oversamp, b, maxband, int(spin),
+                              _d(_c(T, np.complex128).view(np.float64)),
+                              _d(_c(g0R, np.complex128).view(np.float64)), _d(tiles), wz,
+                              tip.real, tip.imag, _d(out))
     return out
 
 

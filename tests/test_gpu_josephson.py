@@ -1,0 +1,119 @@
+"""GPU parity of the Josephson map (GPUQPI::wannierjosephson, the drop-in path
+of src/qpihipkernel.cc:600-622) against the FP64 oracle restatement of
+src/qpi.cc:953-1033, called through the C ABI.
+
+G0(R) comes from a Bogoliubov-de Gennes model built from a seeded normal-state
+model: H_BdG(k) = [[H(k), D], [D^H, -H(-k)^*]] with constant s-wave pairing D,
+inverted per k and transformed by the pinned oracle transform, so the
+particle-hole block the map reads is nonzero."""
+import numpy as np
+import pytest
+
+import cqpi
+import oracle
+import refmath as rm
+from cqpi import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-4
+
+
+def _bdg(nb, K=16, n=8, seed=51, delta=0.08, omega=0.03):
+    wl = synth.square_norb(nb, K=K, n=n, E=1, seed=seed, w=1, os_=2, eta=0.02)
+    k = synth.kgrid(K)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    H = synth.hk_numpy(wl.hops, nb, wl.efermi, kx, ky)
+    Hm = synth.hk_numpy(wl.hops, nb, wl.efermi, -kx, -ky)
+    b = 2 * nb
+    Hb = np.zeros((K, K, b, b), np.complex128)
+    Hb[..., :nb, :nb] = H
+    Hb[..., nb:, nb:] = -np.conj(Hm)
+    D = delta * np.eye(nb)
+    Hb[..., :nb, nb:] = D
+    Hb[..., nb:, :nb] = D.conj().T
+    g0k = np.linalg.inv((omega + 1j * wl.eta) * np.eye(b) - Hb)
+    gR = oracle.realspace(g0k)
+    V = np.zeros((b, b), np.complex128)
+    V[:nb, :nb] = wl.V
+    V[nb:, nb:] = -np.conj(wl.V)
+    T = oracle.tmatrix(V, gR[0, 0])
+    return wl, gR, T
+
+
+def _tip(omega=0.03, deltat=0.01, etat=0.02):
+    # src/calcqpi.cc:705-709
+    oc = complex(omega, etat)
+    return complex(0.0, -np.copysign(1.0, omega) * np.pi * deltat) / np.sqrt(oc * oc - deltat * deltat)
+
+
+def _run(wl, gR, T, tip, spin, prec="fp64"):
+    b = gR.shape[2]
+    print("case: josephson drop-in b=%d spin=%s K=%d n=%d prec=%s" % (b, spin, wl.K, wl.n, prec), flush=True)
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, b, b // 2, spin, wl.tiles)
+    try:
+        if prec != "fp64":
+            q.set_precision(prec)
+        q.wannierjosephson(T, gR, tip)
+        return q.retrieveResult()
+    finally:
+        q.close()
+
+
+@pytest.mark.parametrize("nb,spin", [(1, False), (3, False), (6, True)])
+def test_dropin_josephson_matches_oracle(gpu, nb, spin):
+    wl, gR, T = _bdg(nb)
+    tip = _tip()
+    ref = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip, spin=spin)
+    got = _run(wl, gR, T, tip, spin)
+    assert np.abs(ref).max() > 0
+    assert rm.rel_errors(got, ref)[0] < TOL64
+
+
+def test_dropin_josephson_fp32(gpu):
+    wl, gR, T = _bdg(3)
+    tip = _tip(omega=-0.02)
+    ref = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip)
+    got = _run(wl, gR, T, tip, False, "fp32")
+    assert rm.rel_errors(got, ref)[0] < TOL32
+
+
+def test_dropin_josephson_then_ldos(gpu):
+    """The two maps share one context: a Josephson call followed by an LDOS
+    call gives each map its own periodic table."""
+    wl, gR, T = _bdg(3)
+    b = gR.shape[2]
+    tip = _tip()
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, b, b // 2, False, wl.tiles)
+    try:
+        q.wannierjosephson(T, gR, tip)
+        gj = q.retrieveResult()
+        q.wannierldos(T, gR)
+        gl = q.retrieveResult()
+    finally:
+        q.close()
+    rj = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip)
+    rl = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, maxband=b // 2)
+    assert rm.rel_errors(gj, rj)[0] < TOL64
+    assert rm.rel_errors(gl, rl)[0] < TOL64
+
+
+def test_josephson_rejects_what_it_cannot_compute(gpu):
+    wl, gR, T = _bdg(1)
+    b = gR.shape[2]
+    # every band carries a tile: there is no particle-hole column o + maxband
+    tiles = np.concatenate([wl.tiles, wl.tiles])
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, b, b, False, tiles)
+    try:
+        with pytest.raises(cqpi.CqpiError):
+            q.wannierjosephson(T, gR, 1.0)
+    finally:
+        q.close()
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, b, b // 2, False, wl.tiles)
+    try:
+        q.set_precision("mixed")
+        with pytest.raises(cqpi.CqpiError):
+            q.wannierjosephson(T, gR, 1.0)
+    finally:
+        q.close()

@@ -128,6 +128,52 @@ def test_contraction_spin_blocks_factorised():
     assert rm.rel_errors(got, ref)[0] < 1e-12
 
 
+def _josephson_inputs(b, seed):
+    rng = np.random.default_rng(seed)
+    K, n, w, os_ = 6, 6, 1, 2
+    gR = rng.normal(size=(K, K, b, b)) + 1j * rng.normal(size=(K, K, b, b))
+    T = rng.normal(size=(b, b)) + 1j * rng.normal(size=(b, b))
+    tiles = rng.normal(size=(b // 2, (2 * w + 1) * os_, (2 * w + 1) * os_))
+    return gR, T, tiles, n, w, os_
+
+
+def _hole_swap(b):
+    """Column permutation o <-> o + b/2 of the particle and hole blocks."""
+    mb = b // 2
+    return np.array([o + mb if o < mb else o - mb for o in range(b)])
+
+
+@pytest.mark.parametrize("b,spin", [(4, False), (6, False), (8, True)])
+def test_josephson_is_ldos_of_the_hole_block(b, spin):
+    """With tip = 1 the Josephson map (column o1 + maxband) equals the LDOS map
+    of G0 Pi with T' = Pi^T T, Pi swapping the particle and hole columns:
+    (G0 Pi T' G0 Pi)_{o2,o1} = (G0 T G0)_{o2,o1+maxband} and the periodic term
+    G0 Pi(R2-R1)_{o2,o1} = G0(R2-R1)_{o2,o1+maxband}.  The LDOS oracle is
+    pinned by the supercell inverse; a wrong column, a dropped periodic term or
+    a transposed product fails here."""
+    gR, T, tiles, n, w, os_ = _josephson_inputs(b, 30 + b)
+    pi = _hole_swap(b)
+    got = oracle.wannierjosephson(gR, T, tiles, n, w, os_, 1.0, spin=spin)
+    ref = oracle.wannierldos(gR[..., pi], T[pi, :], tiles, n, w, os_, maxband=b // 2, spin=spin)
+    assert rm.rel_errors(got, ref)[0] < 1e-12
+
+
+@pytest.mark.parametrize("phi", [0.7, 2.0, -1.3])
+def test_josephson_tip_phase(phi):
+    """z = (G0 T G0 + G0)_{o2,o1+maxband} turns into e^{i phi} z under
+    G0 -> e^{i phi} G0, T -> e^{-i phi} T, so the map with tip = 1 on the
+    rotated inputs equals the map with tip = e^{i phi} on the original ones;
+    a conjugated tip or Re in place of Im fails this."""
+    gR, T, tiles, n, w, os_ = _josephson_inputs(4, 41)
+    ph = np.exp(1j * phi)
+    got = oracle.wannierjosephson(gR, T, tiles, n, w, os_, ph)
+    ref = oracle.wannierjosephson(gR * ph, T / ph, tiles, n, w, os_, 1.0)
+    assert rm.rel_errors(got, ref)[0] < 1e-12
+    # real scaling of the tip scales the map
+    got2 = oracle.wannierjosephson(gR, T, tiles, n, w, os_, 2.5 * ph)
+    assert np.allclose(got2, 2.5 * got, rtol=1e-13, atol=1e-13 * np.abs(got).max())
+
+
 def test_whole_path_matches_supercell_bruteforce(small3):
     """Dense inverse of the K x K periodic supercell with the impurity on site 0
     pins eigensolve, resolvent, transform, T-matrix and contraction together."""
