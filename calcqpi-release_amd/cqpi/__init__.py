@@ -40,6 +40,7 @@ SIGNATURES = {
     "cqpi_create": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _sz, C.c_int, _dp, C.POINTER(_vp)]),
     "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
+    "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_retrieve": (C.c_int, [_vp, _dp]),
     "cqpi_printinfo": (C.c_int, [_vp, C.c_char_p, _sz]),
     "cqpi_destroy": (None, [_vp]),
@@ -209,6 +210,11 @@ class AmdQPI:
         return buf.value.decode()
 
     # -- whole chain --------------------------------------------------------
+    def set_josephson(self, on=True, deltat=0.001, etat=float("nan")):
+        """cqpi_run computes the Josephson map (tip gap deltat, tip broadening
+        etat; NaN: the run's eta) instead of the LDOS map."""
+        _check(lib().cqpi_set_josephson(self._ctx, int(bool(on)), float(deltat), float(etat)))
+
     def set_precision(self, mode):
         mode = PRECISIONS.get(mode, mode)
         _check(lib().cqpi_set_precision(self._ctx, int(mode)))

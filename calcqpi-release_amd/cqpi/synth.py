@@ -350,6 +350,40 @@ def square_norb(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=No
     )
 
 
+def bdg_square(nb, K, n, E, seed, delta=0.08, w=1, os_=2, eta=0.01, efermi=0.05, elo=-0.06, ehi=0.06):
+    """A superconducting (scmodel) square-lattice model for the Josephson map:
+    the normal-state blocks t(R) of square_norb, the hole block -t(R)^T with
+    the 'anti' phase (so H_hh(k) = -H(-k)^*), constant on-site s-wave pairing
+    delta; V = diag(V_n, -V_n^*); tiles for the nb electron orbitals."""
+    base = square_norb(nb, K, n, E, seed, elo=elo, ehi=ehi, w=w, os_=os_, eta=eta)
+    rng = np.random.default_rng(seed + 500)
+    onsite = rng.normal(0, 0.3, (nb, nb))
+    onsite = 0.5 * (onsite + onsite.T)
+    np.fill_diagonal(onsite, rng.uniform(-0.2, 0.2, nb))
+    normal = hermitian_blocks(square_shells(2), nb, rng,
+                              lambda sh: rng.normal(0, 0.3, sh).astype(np.complex128),
+                              onsite.astype(np.complex128))
+    b = 2 * nb
+    blocks = {}
+    for R, t in normal.items():
+        m = np.zeros((b, b), np.complex128)
+        m[:nb, :nb] = t
+        m[nb:, nb:] = -t.T
+        if R == (0, 0):
+            m[:nb, nb:] = delta * np.eye(nb)
+            m[nb:, :nb] = np.conj(delta) * np.eye(nb)
+        blocks[R] = m
+    V = np.zeros((b, b), np.complex128)
+    V[:nb, :nb] = base.V
+    V[nb:, nb:] = -np.conj(base.V)
+    return Workload(
+        name="BdG square %d-orbital seed %d" % (nb, seed), bands=b,
+        hops=hops_from_blocks(blocks, b, scmodel=True), K=K, n=n, window=w, oversamp=os_,
+        energies=base.energies, eta=eta, V=V, tiles=base.tiles, efermi=efermi, scmodel=True,
+        meta={"seed": seed, "delta": delta},
+    )
+
+
 def batch_uneven(count=128, seed=1004):
     """Batch of independent instances with uneven sizes: orbitals in {1,3,5},
     k-grid in {128,256,512}, 16-64 energies in [-0.3,0.3], V = U(0.1,2.0) x a

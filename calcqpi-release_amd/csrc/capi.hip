@@ -2,6 +2,7 @@
 // and the whole-chain device path.  See include/cqpi_amd.h for the contract.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -164,6 +165,10 @@ struct cqpi_ctx {
     // gradient: kept forward state and adjoint buffers
     std::vector<cqpi_hopping> h_hops;  // model as given to cqpi_set_model
     bool keep_windows = false, windows_kept = false;
+    // Josephson maps from cqpi_run (tip gap and broadening; etat < 0: the run's eta)
+    bool josephson = false, maps_josephson = false;
+    double jos_deltat = 0.001, jos_etat = -1.0;
+    std::vector<double2> up_tips;
     bool window_whole = true;  // the last run wrote the whole G0 window
     std::vector<double> last_omegas;
     double last_eta = 0.0;
@@ -203,28 +208,28 @@ namespace {
 void release(cqpi_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
                     c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
     for (void *p : bufs)
-        if (p) hipFree(p);
-    if (c->h_win) hipHostFree(c->h_win);
-    if (c->h_T) hipHostFree(c->h_T);
-    if (c->h_status) hipHostFree(c->h_status);
+        if (p) (void)hipFree(p);
+    if (c->h_win) (void)hipHostFree(c->h_win);
+    if (c->h_T) (void)hipHostFree(c->h_T);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     for (auto &p : c->pending) {
-        hipEventDestroy(p.second.first);
-        hipEventDestroy(p.second.second);
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
     }
-    if (c->copy_done) hipEventDestroy(c->copy_done);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->copy_done) (void)hipEventDestroy(c->copy_done);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     for (int i = 0; i < kSplit - 1; i++) {
-        if (c->xstream[i]) hipStreamDestroy(c->xstream[i]);
-        if (c->ev_join[i]) hipEventDestroy(c->ev_join[i]);
+        if (c->xstream[i]) (void)hipStreamDestroy(c->xstream[i]);
+        if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
     }
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     delete c;
 }
 
@@ -235,10 +240,10 @@ int stage_begin(cqpi_ctx *c, int stage)
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return -1;
     if (hipEventCreate(&b) != hipSuccess) {
-        hipEventDestroy(a);
+        (void)hipEventDestroy(a);
         return -1;
     }
-    hipEventRecord(a, c->stream);
+    (void)hipEventRecord(a, c->stream);
     c->pending.push_back({stage, {a, b}});
     return (int)c->pending.size() - 1;
 }
@@ -246,7 +251,7 @@ int stage_begin(cqpi_ctx *c, int stage)
 void stage_end(cqpi_ctx *c, int idx)
 {
     if (idx < 0) return;
-    hipEventRecord(c->pending[idx].second.second, c->stream);
+    (void)hipEventRecord(c->pending[idx].second.second, c->stream);
 }
 
 void stage_collect(cqpi_ctx *c)
@@ -258,8 +263,8 @@ void stage_collect(cqpi_ctx *c)
             c->stage_ms[p.first] += ms;
             c->stage_n[p.first] += 1;
         }
-        hipEventDestroy(p.second.first);
-        hipEventDestroy(p.second.second);
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
     }
     c->pending.clear();
 }
@@ -383,7 +388,29 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     c->windows_kept = false;
     RETCHK(dev_grow(c->d_F1, c->c_F1, Ec * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_G0w, c->c_G0w, Ec * (size_t)g.nw * g.nw * NP, c->gpumem));
-    RETCHK(dev_grow(c->d_bulk, c->c_bulk, Ec * g.os * g.os, c->gpumem));
+    RETCHK(dev_grow(c->d_bulk, c->c_bulk, (c->josephson ? 2 : 1) * Ec * g.os * g.os, c->gpumem));
+    if (c->josephson) {
+        // tip factor of each energy (src/calcqpi.cc:705-709):
+        // i (-sign(w) pi deltat) / sqrt((w + i etat)^2 - deltat^2), principal root
+        const double dt = c->jos_deltat, et = c->jos_etat < 0.0 ? eta : c->jos_etat;
+        std::vector<double2> tips(E);
+        for (size_t e = 0; e < E; e++) {
+            const std::complex<double> oc(omegas[e], et);
+            const std::complex<double> t =
+                std::complex<double>(0.0, -std::copysign(1.0, omegas[e]) * M_PI * dt) / std::sqrt(oc * oc - dt * dt);
+            tips[e] = make_double2(t.real(), t.imag());
+        }
+        const bool tgrew = c->c_tips < E || !c->d_tips;
+        RETCHK(dev_grow(c->d_tips, c->c_tips, E, c->gpumem));
+        if (tgrew || c->up_tips.size() != E ||
+            !std::equal(tips.begin(), tips.end(), c->up_tips.begin(),
+                        [](double2 a, double2 b) { return a.x == b.x && a.y == b.y; })) {
+            c->up_tips = tips;
+            HIPCHK(hipMemcpyAsync(c->d_tips, c->up_tips.data(), E * sizeof(double2), hipMemcpyHostToDevice,
+                                  c->stream));
+        }
+    }
+    c->maps_josephson = c->josephson;
     // mixed precision needs the FP64 window only for |R| <= 2w (T-matrix and
     // bulk term) unless it is kept for the gradient
     plan.rwin = (c->precision == CQPI_PREC_MIXED && !c->keep_windows) ? std::min(g.h, 2 * g.w) : g.h;
@@ -455,7 +482,8 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     st = stage_begin(c, 3);
     RETCHK(launch_tmatrix(g, G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, s));
     stage_end(c, st);
-    RETCHK(contract(c, G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed, eo, s));
+    RETCHK(contract(c, G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed, eo, s,
+                    c->josephson ? c->d_tips + e0 : nullptr));
     return 0;
 }
 
@@ -513,7 +541,7 @@ int finish_run(cqpi_ctx *c, const RunPlan &plan)
     c->nlayers = E;
     c->windows_kept = c->keep_windows;
     if (c->c_hstatus < E) {
-        if (c->h_status) hipHostFree(c->h_status);
+        if (c->h_status) (void)hipHostFree(c->h_status);
         c->h_status = nullptr;
         c->c_hstatus = 0;
         HIPCHK(hipHostMalloc((void **)&c->h_status, E * sizeof(int), hipHostMallocDefault));
@@ -544,7 +572,7 @@ static int dropin_map(cqpi_ctx *c, const double *scat, const double *const *g0, 
     HIPCHK(hipStreamSynchronize(c->stream));
     reset_stages(c);
     if (nwin > c->c_hwin) {
-        if (c->h_win) hipHostFree(c->h_win);
+        if (c->h_win) (void)hipHostFree(c->h_win);
         c->h_win = nullptr;
         c->c_hwin = 0;
         HIPCHK(hipHostMalloc((void **)&c->h_win, nwin * sizeof(double2)));
@@ -634,7 +662,7 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
     c->wanniern = wanniern;
     c->kpoints = kpoints;
     c->scmax = g.B;
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         release(c);
         return fail(CQPI_E_HIP, "hipStreamCreate failed");
@@ -681,6 +709,8 @@ int cqpi_set_precision(cqpi_ctx *c, int precision)
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
     if (precision != CQPI_PREC_FP64 && precision != CQPI_PREC_FP32 && precision != CQPI_PREC_MIXED)
         return fail(CQPI_E_ARG, "precision mode not available");
+    if (precision == CQPI_PREC_MIXED && c->josephson)
+        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
     if (precision == CQPI_PREC_FP32 && !c->d_tilesf) {
         std::vector<float> tf(c->h_tiles.begin(), c->h_tiles.end());
         RETCHK(dev_grow(c->d_tilesf, c->c_tilesf, tf.size(), c->gpumem));
@@ -885,6 +915,18 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     return 0;
 }
 
+int cqpi_set_josephson(cqpi_ctx *c, int on, double deltat, double etat)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (on && 2 * c->g.maxb > c->g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
+    if (on && c->precision == CQPI_PREC_MIXED)
+        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
+    c->josephson = on != 0;
+    c->jos_deltat = deltat;
+    c->jos_etat = std::isnan(etat) ? -1.0 : etat;
+    return 0;
+}
+
 int cqpi_set_keep_windows(cqpi_ctx *c, int on)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
@@ -912,6 +954,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     if (!c || !rhobar) return fail(CQPI_E_ARG, "NULL argument");
     if (!c->windows_kept || c->nlayers == 0)
         return fail(CQPI_E_STATE, "the last cqpi_run did not keep its windows (cqpi_set_keep_windows)");
+    if (c->maps_josephson) return fail(CQPI_E_ARG, "the gradient is of the LDOS map, not the Josephson map");
     const Geom &g = c->g;
     if (!adjoint_supported(g))
         return fail(CQPI_E_ARG, "the adjoint needs a power-of-two k-grid (<= 1024) and at most 4 bands");
@@ -1083,8 +1126,8 @@ int time_run(cqpi_ctx *c, const RunPlan &plan, int reps, float &ms)
     if (!r && hipEventSynchronize(b) != hipSuccess) r = CQPI_E_HIP;
     if (!r && hipEventElapsedTime(&ms, a, b) != hipSuccess) r = CQPI_E_HIP;
     ms /= std::max(reps, 1);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     return r ? fail(r, "autotune timing failed") : 0;
 }
 
@@ -1212,17 +1255,17 @@ constexpr int kBatchBranches = 4;
 void batch_release(cqpi_batch *b)
 {
     if (!b) return;
-    if (b->main) hipStreamSynchronize(b->main);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->main) (void)hipStreamSynchronize(b->main);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipStream_t s : b->branches)
-        if (s) hipStreamDestroy(s);
+        if (s) (void)hipStreamDestroy(s);
     for (hipEvent_t e : b->joins)
-        if (e) hipEventDestroy(e);
-    if (b->fork) hipEventDestroy(b->fork);
-    if (b->t0) hipEventDestroy(b->t0);
-    if (b->t1) hipEventDestroy(b->t1);
-    if (b->main) hipStreamDestroy(b->main);
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    if (b->main) (void)hipStreamDestroy(b->main);
     for (cqpi_ctx *c : b->inst) release(c);
     delete b;
 }
@@ -1286,7 +1329,7 @@ int batch_capture(cqpi_batch *b)
     hipGraph_t graph = nullptr;
     hipError_t ec = hipStreamEndCapture(b->main, &graph);
     if (r || ec != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return fail(r ? r : CQPI_E_HIP, "batch graph capture failed");
     }
     b->graph = graph;

@@ -181,6 +181,16 @@ int cqpi_stage_times(cqpi_ctx *ctx, double *ms, int64_t *launches);
 /* Enables/disables the per-stage event timing (default off). */
 int cqpi_set_timing(cqpi_ctx *ctx, int on);
 
+/* Selects the map cqpi_run computes: on = 0 the LDOS map (default), on = 1
+ * the Josephson map of a superconducting model (output=josephson in the
+ * reference driver, src/calcqpi.cc:705-709 and src/qpi.cc:953-1033), with
+ * the tip factor of energy w
+ *   tip(w) = i (-sign(w) pi deltat) / sqrt((w + i etat)^2 - deltat^2)
+ * (principal root; etat NaN: the run's eta, as the driver's default).  Needs
+ * 2*maxband <= bands and FP64 or FP32 precision; a run with it on cannot feed
+ * cqpi_gradient. */
+int cqpi_set_josephson(cqpi_ctx *ctx, int on, double deltat, double etat);
+
 /* ---------------------------------------------------------------------- */
 /* Gradient (extension; the reference has no differentiation)              */
 /* ---------------------------------------------------------------------- */

@@ -117,3 +117,45 @@ def test_josephson_rejects_what_it_cannot_compute(gpu):
             q.wannierjosephson(T, gR, 1.0)
     finally:
         q.close()
+
+
+def _oracle_chain_josephson(wl, deltat, etat=None):
+    """Per energy: oracle eigensolve, resolvent, transform and T-matrix of the
+    superconducting model, then the Josephson map with the driver's tip
+    factor (src/calcqpi.cc:705-709)."""
+    ev, vec = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K, scmodel=True)
+    out = []
+    for om in wl.energies:
+        gR = oracle.realspace(oracle.greens_k(ev, vec, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        tip = _tip(om, deltat, wl.eta if etat is None else etat)
+        out.append(oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("prec,etat", [("fp64", None), ("fp64", 0.004), ("fp32", None)])
+def test_full_chain_josephson(gpu, prec, etat):
+    """cqpi_run with the Josephson map selected: eigensolve, resolvent,
+    transforms and T-matrix on the device, tip factor per energy."""
+    wl = synth.bdg_square(3, K=32, n=8, E=4, seed=61)
+    ref = _oracle_chain_josephson(wl, 0.01, etat)
+    print("case: josephson full chain b=6 K=32 n=8 E=4 prec=%s etat=%s" % (prec, etat), flush=True)
+    with cqpi.engine_for(wl, prec) as q:
+        q.set_josephson(True, deltat=0.01, etat=float("nan") if etat is None else etat)
+        got = q.run(wl.energies, wl.eta)
+    tol = TOL64 if prec == "fp64" else TOL32
+    for e in range(len(wl.energies)):
+        assert np.abs(ref[e]).max() > 0
+        assert rm.rel_errors(got[e], ref[e])[0] < tol, e
+
+
+def test_full_chain_josephson_not_differentiated(gpu):
+    wl = synth.bdg_square(1, K=16, n=4, E=2, seed=62)
+    with cqpi.engine_for(wl, "fp64") as q:
+        q.set_keep_windows(True)
+        q.set_josephson(True, deltat=0.01)
+        maps = q.run(wl.energies, wl.eta)
+        with pytest.raises(cqpi.CqpiError):
+            q.gradient(np.ones_like(maps))
+        with pytest.raises(cqpi.CqpiError):
+            q.set_precision("mixed")

@@ -205,3 +205,20 @@ def test_getbias():
     assert oracle.getbias(-0.1, 0.1, 51, 0) == -0.1
     assert abs(oracle.getbias(-0.1, 0.1, 51, 50) - 0.1) < 1e-17
     assert oracle.getbias(0.3, 0.5, 1, 0) == 0.3
+
+
+def test_bdg_workload_is_particle_hole_symmetric():
+    """The superconducting test model: Hermitian H(k), hole block -H(-k)^*
+    through the 'anti' phase and the +mu diagonal, pairing in the off-diagonal
+    blocks; the oracle's H(k) (scmodel) agrees with the numpy one."""
+    wl = synth.bdg_square(3, K=8, n=4, E=1, seed=63)
+    k = synth.kgrid(8)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    H = synth.hk_numpy(wl.hops, 6, wl.efermi, kx, ky, scmodel=True)
+    Hm = synth.hk_numpy(wl.hops, 6, wl.efermi, -kx, -ky, scmodel=True)
+    assert np.abs(H - np.conj(np.swapaxes(H, -1, -2))).max() < 1e-14
+    assert np.abs(H[..., 3:, 3:] + np.conj(Hm[..., :3, :3])).max() < 1e-14
+    assert np.abs(H[..., :3, 3:] - wl.meta["delta"] * np.eye(3)).max() < 1e-14
+    ev, vec = oracle.calchamiltonian(wl.hops, 6, wl.efermi, 8, scmodel=True)
+    Hr = np.einsum("...in,...n,...jn->...ij", vec, ev, np.conj(vec))
+    assert np.abs(Hr - H).max() < 1e-12
