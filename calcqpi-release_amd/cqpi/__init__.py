@@ -26,6 +26,9 @@ _LIB = None
 
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED}
+
+# cqpi_flagentry: {i, j, o1, o2 (uint32), factor (double)}, 24 bytes
+FLAG_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("o1", "<u4"), ("o2", "<u4"), ("factor", "<f8")])
 STAGES = ("eigensolve", "resolvent+transform_ky", "transform_kx", "tmatrix+bulk", "contraction")
 
 _sz = C.c_size_t
@@ -41,6 +44,10 @@ SIGNATURES = {
     "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
+    "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
+    "cqpi_wannierldoslist": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
+    "cqpi_make_threshold_list": (C.c_int, [_dp, _sz, _sz, _sz, C.c_double, _sz, C.c_int, _vp, _sz,
+                                           C.POINTER(_sz), C.POINTER(_sz)]),
     "cqpi_retrieve": (C.c_int, [_vp, _dp]),
     "cqpi_printinfo": (C.c_int, [_vp, C.c_char_p, _sz]),
     "cqpi_destroy": (None, [_vp]),
@@ -187,6 +194,34 @@ class AmdQPI:
         scat, g0, ptrs = self._dropin_args(scat, g0)
         _check(lib().cqpi_wannierldos(self._ctx, _d(scat.view(np.float64)),
                                       ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
+        self.nlayers = 1
+
+    @classmethod
+    def from_list(cls, wanniern, kpoints, n, window, oversamp, bands, flags, counts):
+        """A threshold-list context (cqpi_create_list): flags FLAG_DTYPE entries,
+        counts the (2w+1)^4 list lengths in IDX4CC order."""
+        self = cls.__new__(cls)
+        self.W, self.K, self.n = int(wanniern), int(kpoints), int(n)
+        self.window, self.oversamp, self.bands = int(window), int(oversamp), int(bands)
+        self.maxband, self.spin = self.bands, False
+        self._flags = np.ascontiguousarray(flags, FLAG_DTYPE)
+        self._counts = np.ascontiguousarray(counts, np.uintp)
+        if self._counts.size != (2 * self.window + 1) ** 4 or int(self._counts.sum()) != self._flags.size:
+            raise ValueError("counts must hold (2w+1)^4 list lengths summing to len(flags)")
+        self._wf = None
+        self._ctx = _vp()
+        _check(lib().cqpi_create_list(self.W, self.K, self.n, self.window, self.oversamp, self.bands,
+                                      self._flags.ctypes.data, self._counts.ctypes.data_as(C.POINTER(_sz)),
+                                      C.byref(self._ctx)))
+        self.nlayers = 0
+        self._nhops = 0
+        return self
+
+    def wannierldoslist(self, scat, g0):
+        """GPUQPI::wannierldoslist for a context made by from_list."""
+        scat, g0, ptrs = self._dropin_args(scat, g0)
+        _check(lib().cqpi_wannierldoslist(self._ctx, _d(scat.view(np.float64)),
+                                          ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
         self.nlayers = 1
 
     def wannierjosephson(self, scat, g0, tip):
@@ -348,6 +383,22 @@ class AmdQPI:
 
     def __exit__(self, *a):
         self.close()
+
+
+def make_threshold_list(tiles, window, oversamp, threshold, maxband=0, spin=False):
+    """cqpi_make_threshold_list (src/qpi.cc:139-211): (flags FLAG_DTYPE,
+    counts per window pair in IDX4CC order)."""
+    tiles = np.ascontiguousarray(tiles, np.float64)
+    norb = tiles.shape[0]
+    counts = np.zeros((2 * window + 1) ** 4, np.uintp)
+    total = _sz(0)
+    _check(lib().cqpi_make_threshold_list(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
+                                          None, 0, counts.ctypes.data_as(C.POINTER(_sz)), C.byref(total)))
+    flags = np.zeros(max(total.value, 1), FLAG_DTYPE)
+    _check(lib().cqpi_make_threshold_list(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
+                                          flags.ctypes.data, total.value, counts.ctypes.data_as(C.POINTER(_sz)),
+                                          C.byref(total)))
+    return flags[:total.value], counts
 
 
 def engine_for(wl: "synth.Workload", precision="fp64") -> AmdQPI:

@@ -136,6 +136,11 @@ struct cqpi_ctx {
     size_t c_T = 0;
     double2 *d_tips = nullptr;  // tip factor of each energy of a Josephson map
     size_t c_tips = 0;
+    // threshold list (cqpi_create_list): per-pair factor table and pair index
+    bool list_mode = false;
+    double *d_lcoef = nullptr;
+    int *d_lr1off = nullptr, *d_lpair = nullptr;
+    int lnact = 0;
     int *d_status = nullptr;
     size_t c_status = 0;
     double *d_omega = nullptr;
@@ -210,7 +215,8 @@ void release(cqpi_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
-                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_status, c->d_omega, c->d_ldos,
+                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_lcoef, c->d_lr1off,
+                    c->d_lpair, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
@@ -313,6 +319,14 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
     const size_t per = tips ? 2 : 1;  // doubles per periodic-table entry
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, per * (eo + E) * g.os * g.os, c->gpumem));
     double *bulk = c->d_bulk + per * eo * g.os * g.os;
+    if (c->list_mode) {
+        // the periodic part and the map from the threshold list (FP64)
+        int st = stage_begin(c, 4);
+        int r = launch_contract_list(g, G0w, T, c->d_lcoef, c->d_lr1off, c->d_lpair, c->lnact, E, bulk, ldos, s);
+        stage_end(c, st);
+        if (r) return fail(r, "threshold-list contraction launch failed");
+        return 0;
+    }
     int st = stage_begin(c, 3);
     if (tips)
         RETCHK(launch_bulk_josephson(g, G0w, c->d_tiles, E, reinterpret_cast<double2 *>(bulk), s));
@@ -704,6 +718,109 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
 
 void cqpi_destroy(cqpi_ctx *ctx) { release(ctx); }
 
+int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands,
+                     const cqpi_flagentry *flags, const size_t *counts, cqpi_ctx **out)
+{
+    g_err.clear();
+    if (!out || !counts) return fail(CQPI_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (bands > 5) return fail(CQPI_E_ARG, "the threshold-list contraction supports up to 5 bands");
+    const size_t side = 2 * window + 1, NR = side * side, npair = NR * NR;
+    const size_t NP = bands * bands, npix = oversamp * oversamp;
+    size_t total = 0;
+    for (size_t p = 0; p < npair; p++) total += counts[p];
+    if (total && !flags) return fail(CQPI_E_ARG, "flags is NULL");
+    // dense factor table of the active pairs, [k][o2][o1][pix], duplicates summed
+    std::vector<int> r1off(NR + 1, 0), lpair;
+    std::vector<double> coef;
+    size_t pos = 0;
+    for (size_t p = 0; p < npair; p++) {
+        const size_t cnt = counts[p];
+        if (!cnt) continue;
+        const size_t k = lpair.size();
+        lpair.push_back((int)p);  // p = r1 * NR + r2 in IDX4CC order
+        r1off[p / NR + 1]++;
+        coef.resize((k + 1) * NP * npix, 0.0);
+        for (size_t m = 0; m < cnt; m++, pos++) {
+            const cqpi_flagentry &f = flags[pos];
+            if (f.i >= oversamp || f.j >= oversamp || f.o1 >= bands || f.o2 >= bands)
+                return fail(CQPI_E_ARG, "threshold-list entry out of range (i, j < oversamp; o1, o2 < bands)");
+            coef[(k * NP + (size_t)f.o2 * bands + f.o1) * npix + (size_t)f.i * oversamp + f.j] += f.factor;
+        }
+    }
+    for (size_t r = 0; r < NR; r++) r1off[r + 1] += r1off[r];
+    const size_t T = side * oversamp;
+    std::vector<double> zero_tiles(bands * T * T, 0.0);  // the list carries the factors
+    RETCHK(cqpi_create(wanniern, kpoints, n, window, oversamp, bands, bands, 0, zero_tiles.data(), out));
+    cqpi_ctx *c = *out;
+    c->list_mode = true;
+    c->lnact = (int)lpair.size();
+    size_t cap_coef = 0, cap_off = 0, cap_pair = 0;
+    int r = dev_grow(c->d_lcoef, cap_coef, std::max<size_t>(coef.size(), 1), c->gpumem);
+    if (!r) r = dev_grow(c->d_lr1off, cap_off, NR + 1, c->gpumem);
+    if (!r) r = dev_grow(c->d_lpair, cap_pair, std::max<size_t>(lpair.size(), 1), c->gpumem);
+    if (!r && !coef.empty() &&
+        hipMemcpy(c->d_lcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
+    if (!r && hipMemcpy(c->d_lr1off, r1off.data(), (NR + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
+    if (!r && !lpair.empty() &&
+        hipMemcpy(c->d_lpair, lpair.data(), lpair.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
+    if (r) {
+        release(c);
+        *out = nullptr;
+        return r;
+    }
+    return 0;
+}
+
+int cqpi_make_threshold_list(const double *wf, size_t norb, size_t window, size_t oversamp, double threshold,
+                             size_t maxband, int spin, cqpi_flagentry *out, size_t cap, size_t *counts,
+                             size_t *total)
+{
+    g_err.clear();
+    if (!wf || !counts || !total) return fail(CQPI_E_ARG, "NULL argument");
+    if (!maxband) maxband = norb;
+    if (maxband > norb) return fail(CQPI_E_ARG, "maxband > number of tiles");
+    const size_t T = (2 * window + 1) * oversamp, wins = 2 * window + 1, sb = maxband >> 1;
+    auto wv = [&](size_t o, size_t x, size_t y) { return wf[(o * T + x) * T + y]; };
+    double maxval = 0.0;
+    for (size_t l = 0; l < norb; l++)
+        for (size_t i = 0; i < T; i++)
+            for (size_t j = 0; j < T; j++) maxval = std::max(maxval, wv(l, i, j) * wv(l, i, j));
+    const double cut = threshold * maxval;
+    const size_t np = spin ? sb : maxband;
+    size_t cnt = 0;
+    for (size_t xn1 = 0; xn1 < wins; xn1++)
+        for (size_t yn1 = 0; yn1 < wins; yn1++)
+            for (size_t xn2 = 0; xn2 < wins; xn2++)
+                for (size_t yn2 = 0; yn2 < wins; yn2++) {
+                    const size_t first = cnt;
+                    for (size_t o1 = 0; o1 < np; o1++)
+                        for (size_t o2 = 0; o2 < np; o2++)
+                            for (size_t i = 0; i < oversamp; i++)
+                                for (size_t j = 0; j < oversamp; j++) {
+                                    const size_t x = i + (wins - 1) * oversamp, y = j + (wins - 1) * oversamp;
+                                    for (int part = 0; part < (spin ? 2 : 1); part++) {
+                                        const size_t a = part ? sb + o1 : o1, b = part ? sb + o2 : o2;
+                                        const double f = wv(a, x - xn1 * oversamp, y - yn1 * oversamp) *
+                                                         wv(b, x - xn2 * oversamp, y - yn2 * oversamp);
+                                        if (std::fabs(f) > cut) {
+                                            if (out && cnt < cap)
+                                                out[cnt] = cqpi_flagentry{(uint32_t)i, (uint32_t)j, (uint32_t)a,
+                                                                          (uint32_t)b, f};
+                                            cnt++;
+                                        }
+                                    }
+                                }
+                    counts[((xn1 * wins + yn1) * wins + xn2) * wins + yn2] = cnt - first;
+                }
+    *total = cnt;
+    if (out && cnt > cap) return fail(CQPI_E_ARG, "cap is smaller than the list; *total holds its length");
+    return 0;
+}
+
 int cqpi_set_precision(cqpi_ctx *c, int precision)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
@@ -711,6 +828,8 @@ int cqpi_set_precision(cqpi_ctx *c, int precision)
         return fail(CQPI_E_ARG, "precision mode not available");
     if (precision == CQPI_PREC_MIXED && c->josephson)
         return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
+    if (precision != CQPI_PREC_FP64 && c->list_mode)
+        return fail(CQPI_E_ARG, "the threshold-list contraction is computed in FP64 only");
     if (precision == CQPI_PREC_FP32 && !c->d_tilesf) {
         std::vector<float> tf(c->h_tiles.begin(), c->h_tiles.end());
         RETCHK(dev_grow(c->d_tilesf, c->c_tilesf, tf.size(), c->gpumem));
@@ -749,6 +868,14 @@ int cqpi_stage_times(cqpi_ctx *c, double *ms, int64_t *launches)
 
 int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda)
 {
+    if (c && c->list_mode)
+        return fail(CQPI_E_STATE, "this context holds a threshold list: use cqpi_wannierldoslist");
+    return dropin_map(c, scat, g0, tda, nullptr);
+}
+
+int cqpi_wannierldoslist(cqpi_ctx *c, const double *scat, const double *const *g0, size_t tda)
+{
+    if (c && !c->list_mode) return fail(CQPI_E_STATE, "this context was not created by cqpi_create_list");
     return dropin_map(c, scat, g0, tda, nullptr);
 }
 
@@ -918,6 +1045,7 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
 int cqpi_set_josephson(cqpi_ctx *c, int on, double deltat, double etat)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (on && c->list_mode) return fail(CQPI_E_ARG, "a threshold-list context computes the LDOS map only");
     if (on && 2 * c->g.maxb > c->g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
     if (on && c->precision == CQPI_PREC_MIXED)
         return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
@@ -955,6 +1083,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     if (!c->windows_kept || c->nlayers == 0)
         return fail(CQPI_E_STATE, "the last cqpi_run did not keep its windows (cqpi_set_keep_windows)");
     if (c->maps_josephson) return fail(CQPI_E_ARG, "the gradient is of the LDOS map, not the Josephson map");
+    if (c->list_mode) return fail(CQPI_E_ARG, "the gradient is of the dense LDOS map, not a threshold-list map");
     const Geom &g = c->g;
     if (!adjoint_supported(g))
         return fail(CQPI_E_ARG, "the adjoint needs a power-of-two k-grid (<= 1024) and at most 4 bands");

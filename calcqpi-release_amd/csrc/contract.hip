@@ -342,6 +342,137 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
+// ---------------------------------------------------------------------------
+// Threshold-list contraction (src/qpihipkernel.cc:273-315, CPU src/qpi.cc:1088-1135):
+//   rho(pos, x) = - sum_(R1,R2) sum_(o1,o2) F[R1,R2,o2,o1,x]
+//                   Im[ (G0(pos+R2) T G0(-pos-R1))_{o2,o1} + G0(R2-R1)_{o2,o1} ]
+// with F the list's factors summed into a dense table per active window pair
+// (coef[k][o2][o1][pix], pair k = (r1, r2) = lpair[k], pairs of one r1
+// contiguous from r1off[r1]).  The list selects arbitrary (R1, o1, R2, o2)
+// pairs, so the sum does not factorise as the dense map does: per pair the
+// lane forms C = G0(pos+R2) Y with Y = T G0(-pos-R1) kept from its R1.
+// Lanes are the 8 x 8 cells, the waves split the R1 values, kListPx
+// sub-pixels are accumulated per pass and summed over the waves in LDS.
+// The periodic part (position independent) comes from k_bulk_list.
+// ---------------------------------------------------------------------------
+constexpr int kListPx = 16;
+
+template <int B>
+__global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const double2 *__restrict__ G0w,
+                                                             const double2 *__restrict__ Tm,
+                                                             const double *__restrict__ bulk,
+                                                             const double *__restrict__ coef,
+                                                             const int *__restrict__ r1off,
+                                                             const int *__restrict__ lpair, double *__restrict__ ldos)
+{
+    __shared__ double red[kWaves - 1][kListPx][64];
+    constexpr int NP = B * B;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cx = blockIdx.x * kCells + (lane & (kCells - 1));
+    const int cy = blockIdx.y * kCells + (lane >> 3);
+    const int e = blockIdx.z;
+    const bool cell_ok = cx < g.n && cy < g.n;
+    const int os = g.os, w = g.w, nw = g.nw, h = g.h, W = g.W;
+    const int side = 2 * w + 1, NR = side * side, npix = os * os;
+    const int n2 = g.n >> 1;
+    // cells past the map edge compute with cell 0 (in the window) and store nothing
+    const int px = (cell_ok ? cx : 0) - n2, py = (cell_ok ? cy : 0) - n2;
+    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
+    const double2 *Te = Tm + (size_t)e * NP;
+    double *out = ldos + (size_t)e * W * W;
+
+    for (int pc = 0; pc < npix; pc += kListPx) {
+        double val[kListPx];
+#pragma unroll
+        for (int q = 0; q < kListPx; q++) val[q] = 0.0;
+        for (int r1 = wave; r1 < NR; r1 += kWaves) {
+            const int rx1 = r1 / side - w, ry1 = r1 % side - w;
+            const double2 *GB = G0 + ((size_t)(-px - rx1 + h) * nw + (-py - ry1 + h)) * NP;
+            double2 Y[NP];  // T G0(-pos-R1)
+#pragma unroll
+            for (int q = 0; q < B; q++)
+#pragma unroll
+                for (int o1 = 0; o1 < B; o1++) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int p = 0; p < B; p++) acc = cadd(acc, cmul(Te[q * B + p], GB[p * B + o1]));
+                    Y[q * B + o1] = acc;
+                }
+            const int k1 = r1off[r1 + 1];
+            for (int k = r1off[r1]; k < k1; k++) {
+                const int r2 = lpair[k] - r1 * NR;
+                const int rx2 = r2 / side - w, ry2 = r2 % side - w;
+                const double2 *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
+                const double *cf = coef + (size_t)k * NP * npix + pc;
+#pragma unroll
+                for (int o2 = 0; o2 < B; o2++) {
+                    double2 arow[B];
+#pragma unroll
+                    for (int q = 0; q < B; q++) arow[q] = GA[o2 * B + q];
+#pragma unroll
+                    for (int o1 = 0; o1 < B; o1++) {
+                        double cim = 0.0;  // Im (G0(pos+R2) Y)_{o2,o1}
+#pragma unroll
+                        for (int q = 0; q < B; q++)
+                            cim += arow[q].x * Y[q * B + o1].y + arow[q].y * Y[q * B + o1].x;
+                        const double *cp = cf + (size_t)(o2 * B + o1) * npix;
+#pragma unroll
+                        for (int q = 0; q < kListPx; q++)
+                            if (pc + q < npix) val[q] += cp[q] * cim;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the previous pass's partial sums are read
+        if (wave > 0)
+#pragma unroll
+            for (int q = 0; q < kListPx; q++) red[wave - 1][q][lane] = val[q];
+        __syncthreads();
+        if (wave == 0 && cell_ok) {
+#pragma unroll
+            for (int q = 0; q < kListPx; q++) {
+                const int pix = pc + q;
+                if (pix >= npix) break;
+                double v = val[q];
+#pragma unroll
+                for (int v2 = 0; v2 < kWaves - 1; v2++) v += red[v2][q][lane];
+                const int ii = pix / os, jj = pix - ii * os;
+                out[(size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W] = -(v + bulk[(size_t)e * npix + pix]);
+            }
+        }
+    }
+}
+
+// periodic part of the list map: bulk[e][pix] = sum_(pairs, o1, o2) F Im G0(R2-R1)_{o2,o1}
+constexpr int kListBulkThreads = 256;
+
+__global__ void __launch_bounds__(kListBulkThreads) k_bulk_list(Geom g, const double2 *__restrict__ G0w,
+                                                              const double *__restrict__ coef,
+                                                              const int *__restrict__ lpair, int nact,
+                                                              double *__restrict__ bulk)
+{
+    __shared__ double red[kListBulkThreads];
+    const int pix = blockIdx.x, e = blockIdx.y;
+    const int B = g.B, NP = B * B, nw = g.nw, h = g.h, w = g.w;
+    const int side = 2 * w + 1, NR = side * side, npix = g.os * g.os;
+    const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < nact * NP; t += blockDim.x) {
+        const int k = t / NP, oo = t - k * NP;
+        const int r1 = lpair[k] / NR, r2 = lpair[k] - r1 * NR;
+        const int dx = (r2 / side) - (r1 / side), dy = (r2 % side) - (r1 % side);  // R2 - R1
+        acc += coef[((size_t)k * NP + oo) * npix + pix] * G0[((size_t)(dx + h) * nw + (dy + h)) * NP + oo].y;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kListBulkThreads / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bulk[(size_t)e * npix + pix] = red[0];
+}
+
 }  // namespace
 
 namespace cqpi {
@@ -358,6 +489,27 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
 #undef CASE
     default: return CQPI_E_ARG;
     }
+}
+
+int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
+                         const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bulk_list, dim3(g.os * g.os, E), dim3(kListBulkThreads), 0, s, g, G0w, coef, lpair, nact,
+                       bulk);
+    if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    const int nb = (g.n + kCells - 1) / kCells;
+    switch (g.B) {
+#define CASE(b)                                                                                                    \
+    case b:                                                                                                        \
+        hipLaunchKernelGGL(k_contract_list<b>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk, coef,    \
+                           r1off, lpair, ldos);                                                                    \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#undef CASE
+    default: return CQPI_E_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
 int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,

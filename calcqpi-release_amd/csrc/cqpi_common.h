@@ -197,6 +197,12 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
 // complex periodic table of launch_bulk_josephson (needs 2 maxband <= b)
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                          const double *tiles, int E, double *ldos, hipStream_t s, const double2 *tips = nullptr);
+// threshold-list contraction (b <= 5): coef[k][o2][o1][pix] per active window
+// pair k = lpair[k] = r1*(2w+1)^2 + r2, pairs of one r1 contiguous from
+// r1off[r1]; writes the periodic part to bulk (E x os^2) and the maps to ldos
+int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
+                         const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
+                         hipStream_t s);
 // FP32 contraction: narrows the E windows and T into G0f (E*nw*nw*b*b) and Tf
 // (E*b*b) first; tilesf are the tiles in FP32
 int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,

@@ -18,6 +18,7 @@
 #include "cqpi_amd.h"
 #include "gpuqpi.h"
 #include "idl.h"
+#include "qpi.h"  // flaglist (src/qpi.h:33-39), as qpihipkernel.h includes it
 #include "wannierfunctions.h"
 
 class AmdQPI : public GPUQPI
@@ -54,6 +55,26 @@ public:
         check(cqpi_create(wanniern, kpoints, n, window, oversamp, bands, maxband, spin, tiles.data(), &ctx));
         scatbuf.resize(2 * bands * bands);
     }
+    // the threshold-list constructor (src/qpihipkernel.cc:431-493): the lists
+    // flattened in IDX4CC order, as HipQPI flattens them
+    AmdQPI(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands_, flaglist &flist)
+        : W(wanniern), K(kpoints), bands(bands_)
+    {
+        const size_t winn = 2 * window + 1;
+        std::vector<cqpi_flagentry> flat;
+        std::vector<size_t> counts;
+        for (size_t i = 0; i < winn; i++)
+            for (size_t j = 0; j < winn; j++)
+                for (size_t k = 0; k < winn; k++)
+                    for (size_t l = 0; l < winn; l++) {
+                        counts.push_back(flist[i][j][k][l].size());
+                        for (const flaglistentry &f : flist[i][j][k][l])
+                            flat.push_back(cqpi_flagentry{(uint32_t)f.i, (uint32_t)f.j, (uint32_t)f.o1, (uint32_t)f.o2,
+                                                          f.factor});
+                    }
+        check(cqpi_create_list(wanniern, kpoints, n, window, oversamp, bands, flat.data(), counts.data(), &ctx));
+        scatbuf.resize(2 * bands * bands);
+    }
     void printinfo(std::ostream &os)
     {
         char buf[2048];
@@ -85,9 +106,10 @@ public:
         pack(scat, g0);
         check(cqpi_wannierjosephson(ctx, scatbuf.data(), table.data(), g0[0][0]->tda, GSL_REAL(tip), GSL_IMAG(tip)));
     }
-    void wannierldoslist(gsl_matrix_complex *, std::vector<std::vector<gsl_matrix_complex *> > &)
+    void wannierldoslist(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
     {
-        unsupported("The threshold-list contraction");
+        pack(scat, g0);
+        check(cqpi_wannierldoslist(ctx, scatbuf.data(), table.data(), g0[0][0]->tda));
     }
     void spf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("SPF mode"); }
     void uspf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("USPF mode"); }

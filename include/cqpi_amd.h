@@ -65,6 +65,14 @@ typedef struct {
 
 typedef struct cqpi_ctx cqpi_ctx;
 
+/* One entry of a threshold list: sub-pixel (i, j) of the cell, orbitals
+ * o1, o2 and the tile product factor; the layout of the reference's
+ * qpigpuflaglist (src/qpihipkernel.h:47-50) with 32-bit indices. */
+typedef struct {
+    uint32_t i, j, o1, o2;
+    double factor;
+} cqpi_flagentry;
+
 /* ---------------------------------------------------------------------- */
 /* Status                                                                  */
 /* ---------------------------------------------------------------------- */
@@ -111,6 +119,37 @@ int cqpi_wannierldos(cqpi_ctx *ctx, const double *scat, const double *const *g0,
  * map with cqpi_retrieve. */
 int cqpi_wannierjosephson(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda, double tip_re,
                           double tip_im);
+
+/* Replaces HipQPI::HipQPI(wanniern, kpoints, n, window, oversamp, bands,
+ * flaglist) (src/qpihipkernel.h:76, src/qpihipkernel.cc:431-493): a context
+ * for the threshold-list contraction.  counts holds the (2w+1)^4 list
+ * lengths of the window pairs (xn1, yn1, xn2, yn2), R = xn - w, in the order
+ * ((xn1*(2w+1) + yn1)*(2w+1) + xn2)*(2w+1) + yn2 (IDX4CC); flags holds the
+ * entries of all lists one after the other, as the reference flattens them
+ * (src/qpihipkernel.cc:452-480).  Needs bands <= 5; FP64 only.  The context
+ * serves cqpi_wannierldoslist and cqpi_run (maps from the list). */
+int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands,
+                     const cqpi_flagentry *flags, const size_t *counts, cqpi_ctx **out);
+
+/* Replaces HipQPI::wannierldoslist(scat, g0) (include/gpuqpi.h:25,
+ * src/qpihipkernel.cc:624-636; kernel :273-315, CPU src/qpi.cc:1088-1135):
+ *   rho(x) = - sum over the list entries of (R1, R2) with sub-pixel x of
+ *            factor Im[ (G0(pos+R2) T G0(-pos-R1) + G0(R2-R1))_{o2,o1} ].
+ * Arguments as cqpi_wannierldos; the context must come from
+ * cqpi_create_list.  Retrieve the map with cqpi_retrieve. */
+int cqpi_wannierldoslist(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda);
+
+/* The driver's list builder mkthresholdlist (src/qpi.cc:139-211; called at
+ * src/calcqpi.cc:620-634): entries whose tile product exceeds
+ * threshold * max(w^2) in magnitude, over all window pairs, orbitals
+ * o1, o2 < maxband (0: norb; with spin both in one half) and sub-pixels, in
+ * the reference's order.  wf: norb tiles, layout as cqpi_create.  counts
+ * receives the (2w+1)^4 list lengths, *total the entry count; out (may be
+ * NULL) receives up to cap entries (CQPI_E_ARG if the list is longer; call
+ * with out = NULL first to size it). */
+int cqpi_make_threshold_list(const double *wf, size_t norb, size_t window, size_t oversamp, double threshold,
+                             size_t maxband, int spin, cqpi_flagentry *out, size_t cap, size_t *counts,
+                             size_t *total);
 
 /* Replaces HipQPI::retrieveResult(double *ldos) (src/qpihipkernel.cc:690-704):
  * waits for the device and copies the W x W map of the last wannierldos call

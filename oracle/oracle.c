@@ -550,6 +550,118 @@ void or_wannierjosephson(size_t K, size_t n, size_t window, size_t oversamp, siz
 }
 
 /* ------------------------------------------------------------------------- */
+/* Threshold list: mkthresholdlist src/qpi.cc:139-211.  Entries {i, j, o1, o2, */
+/* factor} for every window pair (xn1, yn1, xn2, yn2) in [0, 2w]^4 whose tile */
+/* product exceeds threshold * max(w^2) in magnitude, in the reference's     */
+/* order (spin: the (o1, o2) entry, then the (sb+o1, sb+o2) entry).  counts   */
+/* receives the (2w+1)^4 list lengths in IDX4CC order; out (may be NULL)      */
+/* receives up to cap entries.  Returns the total count.                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t i, j, o1, o2;
+    double factor;
+} or_flag;
+
+size_t or_mkthresholdlist(const double *wf, size_t norb, size_t window, size_t oversamp, double threshold,
+                          size_t maxband, int spin, or_flag *out, size_t cap, size_t *counts)
+{
+    size_t TS = (2 * window + 1) * oversamp, wins = 2 * window + 1, count = 0;
+    double maxval = 0.0;
+    if (!maxband) maxband = norb;
+    size_t spinbands = maxband >> 1;
+#define WF(o, x, y) wf[((size_t)(o) * TS + (size_t)(x)) * TS + (size_t)(y)]
+    for (size_t l = 0; l < norb; l++)
+        for (size_t i = 0; i < TS; i++)
+            for (size_t j = 0; j < TS; j++) {
+                double v = WF(l, i, j) * WF(l, i, j);
+                if (v > maxval) maxval = v;
+            }
+    size_t np = spin ? spinbands : maxband;
+    for (size_t xn1 = 0; xn1 < wins; xn1++)
+        for (size_t yn1 = 0; yn1 < wins; yn1++)
+            for (size_t xn2 = 0; xn2 < wins; xn2++)
+                for (size_t yn2 = 0; yn2 < wins; yn2++) {
+                    size_t before = count;
+                    for (size_t o1 = 0; o1 < np; o1++)
+                        for (size_t o2 = 0; o2 < np; o2++)
+                            for (size_t i = 0; i < oversamp; i++)
+                                for (size_t j = 0; j < oversamp; j++) {
+                                    size_t x = i + (wins - 1) * oversamp, y = j + (wins - 1) * oversamp;
+                                    for (int part = 0; part < (spin ? 2 : 1); part++) {
+                                        size_t a = part ? spinbands + o1 : o1, c = part ? spinbands + o2 : o2;
+                                        double f = WF(a, x - xn1 * oversamp, y - yn1 * oversamp) *
+                                                   WF(c, x - xn2 * oversamp, y - yn2 * oversamp);
+                                        if (fabs(f) > threshold * maxval) {
+                                            if (out && count < cap) {
+                                                or_flag e = {(uint32_t)i, (uint32_t)j, (uint32_t)a, (uint32_t)c, f};
+                                                out[count] = e;
+                                            }
+                                            count++;
+                                        }
+                                    }
+                                }
+                    if (counts) counts[((xn1 * wins + yn1) * wins + xn2) * wins + yn2] = count - before;
+                }
+#undef WF
+    return count;
+}
+
+/* ------------------------------------------------------------------------- */
+/* LDOS from a threshold list: tmatrix::calcwannierldos(oversamp, window,      */
+/* flaglist) src/qpi.cc:1088-1135 (GPU kernel src/qpihipkernel.cc:273-315).   */
+/* flags/counts as produced by or_mkthresholdlist (any list of that layout).  */
+/* ------------------------------------------------------------------------- */
+void or_wannierldoslist(size_t K, size_t n, size_t window, size_t oversamp, size_t b, const double *Tin,
+                        const double *g0_in, const or_flag *flags, const size_t *counts, double *ldos)
+{
+    const cplx *T = (const cplx *)Tin, *g0 = (const cplx *)g0_in;
+    size_t W = oversamp * n, bb = b * b, wins = 2 * window + 1, npair = wins * wins * wins * wins;
+    size_t n2 = n >> 1;
+    int w = (int)window;
+    size_t *offs = (size_t *)malloc((npair + 1) * sizeof(size_t));
+    offs[0] = 0;
+    for (size_t p = 0; p < npair; p++) offs[p + 1] = offs[p] + counts[p];
+    for (size_t i = 0; i < W * W; i++) ldos[i] = 0.0;
+#define G0AT(ix, iy) (g0 + ((size_t)(ix) * K + (size_t)(iy)) * bb)
+#pragma omp parallel
+    {
+        cplx *buffer = (cplx *)malloc(bb * sizeof(cplx)), *cl = (cplx *)malloc(bb * sizeof(cplx));
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = 0; j < n; j++) {
+                long ipos = (long)i - (long)n2, jpos = (long)j - (long)n2;
+                for (int nni1 = -w; nni1 <= w; nni1++)
+                    for (int nnj1 = -w; nnj1 <= w; nnj1++) {
+                        size_t ax = (size_t)((2 * (long)K - (ipos + nni1)) % (long)K);
+                        size_t ay = (size_t)((2 * (long)K - (jpos + nnj1)) % (long)K);
+                        zgemm_small(b, T, G0AT(ax, ay), buffer);
+                        for (int nni2 = -w; nni2 <= w; nni2++)
+                            for (int nnj2 = -w; nnj2 <= w; nnj2++) {
+                                size_t bx = (size_t)(((long)K + ipos + nni2) % (long)K);
+                                size_t by = (size_t)(((long)K + jpos + nnj2) % (long)K);
+                                zgemm_small(b, G0AT(bx, by), buffer, cl);
+                                size_t cx = (size_t)(((long)K - nni1 + nni2) % (long)K);
+                                size_t cy = (size_t)(((long)K - nnj1 + nnj2) % (long)K);
+                                const cplx *gb = G0AT(cx, cy);
+                                size_t p = (((size_t)(nni1 + w) * wins + (size_t)(nnj1 + w)) * wins +
+                                            (size_t)(nni2 + w)) * wins + (size_t)(nnj2 + w);
+                                for (size_t ind = offs[p]; ind < offs[p + 1]; ind++) {
+                                    const or_flag *f = &flags[ind];
+                                    double *out = &ldos[((i * oversamp + f->i) % W) + ((j * oversamp + f->j) % W) * W];
+                                    *out -= cimag(cl[f->o2 * b + f->o1]) * f->factor;
+                                    *out -= cimag(gb[f->o2 * b + f->o1]) * f->factor;
+                                }
+                            }
+                    }
+            }
+        free(buffer);
+        free(cl);
+    }
+#undef G0AT
+    free(offs);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Whole path for a list of energies (the energy loop of src/calcqpi.cc:671-760 */
 /* with green=normal and output=wannier).  ldos_out: E x W x W, x + y*W.       */
 /* Returns nonzero if a T-matrix inversion failed.                            */

@@ -45,6 +45,10 @@ def lib():
         L.or_wannierldos.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip, _dp]
         L.or_wannierjosephson.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip,
                                           C.c_double, C.c_double, _dp]
+        L.or_mkthresholdlist.argtypes = [_dp, sz, sz, sz, C.c_double, sz, C.c_int, C.c_void_p, sz,
+                                         C.POINTER(sz)]
+        L.or_mkthresholdlist.restype = sz
+        L.or_wannierldoslist.argtypes = [sz, sz, sz, sz, sz, _dp, _dp, C.c_void_p, C.POINTER(sz), _dp]
         L.or_cldos_run.argtypes = [C.c_void_p, sz, sz, C.c_double, C.c_int, sz, sz, sz, sz, sz,
                                    C.c_int, C.c_double, _dp, sz, _dp, _dp, _ip, _dp]
         L.or_cldos_run.restype = C.c_int
@@ -168,6 +172,39 @@ def wannierjosephson(g0R, T, tiles, n, window, oversamp, tip, maxband=0, spin=Fa
                               _d(_c(T, np.complex128).view(np.float64)),
                               _d(_c(g0R, np.complex128).view(np.float64)), _d(tiles), wz,
                               tip.real, tip.imag, _d(out))
+    return out
+
+
+# {i, j, o1, o2, factor}: the layout of the reference's qpigpuflaglist
+# (src/qpihipkernel.h:47-50) and of cqpi_flagentry
+FLAG_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("o1", "<u4"), ("o2", "<u4"), ("factor", "<f8")])
+
+
+def mkthresholdlist(tiles, window, oversamp, threshold, maxband=0, spin=False):
+    """src/qpi.cc:139-211: (entries FLAG_DTYPE, counts per window pair in
+    IDX4CC order)."""
+    tiles = _c(tiles, np.float64)
+    norb = tiles.shape[0]
+    wins = 2 * window + 1
+    counts = np.zeros(wins ** 4, np.uintp)
+    total = lib().or_mkthresholdlist(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
+                                     None, 0, counts.ctypes.data_as(C.POINTER(C.c_size_t)))
+    out = np.zeros(max(total, 1), FLAG_DTYPE)
+    lib().or_mkthresholdlist(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
+                             out.ctypes.data, total, counts.ctypes.data_as(C.POINTER(C.c_size_t)))
+    return out[:total], counts
+
+
+def wannierldoslist(g0R, T, flags, counts, n, window, oversamp):
+    """src/qpi.cc:1088-1135: the map from a threshold list."""
+    K, _, b, _ = g0R.shape
+    W = n * oversamp
+    out = np.zeros(W * W)
+    flags = np.ascontiguousarray(flags, FLAG_DTYPE)
+    counts = np.ascontiguousarray(counts, np.uintp)
+    lib().or_wannierldoslist(K, n, window, oversamp, b, _d(_c(T, np.complex128).view(np.float64)),
+                             _d(_c(g0R, np.complex128).view(np.float64)), flags.ctypes.data,
+                             counts.ctypes.data_as(C.POINTER(C.c_size_t)), _d(out))
     return out
 
 

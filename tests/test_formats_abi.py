@@ -121,3 +121,31 @@ def test_write_inputs(tmp_path):
     assert "scattering=(0.5,0.5,0.5);" in txt and "window=2;" in txt
     tiles, _ = synth.read_idl(str(tmp_path / "orbitals.idl"))
     assert np.array_equal(np.transpose(tiles, (2, 0, 1)), wl.tiles)
+
+
+def test_flag_entry_layout():
+    assert cqpi.FLAG_DTYPE.itemsize == 24 and cqpi.FLAG_DTYPE == oracle.FLAG_DTYPE
+
+
+@pytest.mark.parametrize("threshold,spin", [(-1.0, False), (0.05, False), (0.02, True)])
+def test_threshold_list_builder_matches_oracle(threshold, spin):
+    """cqpi_make_threshold_list (host code of the library) builds the same
+    entries, in the same order, as the oracle's mkthresholdlist."""
+    wl = synth.square_norb(5 if spin else 3, K=8, n=4, E=1, seed=71, w=2, os_=4)
+    tiles = wl.tiles[:4] if spin else wl.tiles
+    got, gc = cqpi.make_threshold_list(tiles, 2, 4, threshold, spin=spin)
+    ref, rc = oracle.mkthresholdlist(tiles, 2, 4, threshold, spin=spin)
+    assert np.array_equal(gc, rc)
+    assert np.array_equal(got, ref)
+    assert len(got) > 0
+
+
+def test_threshold_list_builder_rejects_short_buffer():
+    wl = synth.square_norb(3, K=8, n=4, E=1, seed=71, w=1, os_=2)
+    counts = np.zeros(81, np.uintp)
+    total = C.c_size_t(0)
+    flags = np.zeros(3, cqpi.FLAG_DTYPE)
+    rc = cqpi.lib().cqpi_make_threshold_list(
+        wl.tiles.ctypes.data_as(C.POINTER(C.c_double)), 3, 1, 2, 0.0, 0, 0, flags.ctypes.data, 3,
+        counts.ctypes.data_as(C.POINTER(C.c_size_t)), C.byref(total))
+    assert rc == 1 and total.value > 3

@@ -222,3 +222,58 @@ def test_bdg_workload_is_particle_hole_symmetric():
     ev, vec = oracle.calchamiltonian(wl.hops, 6, wl.efermi, 8, scmodel=True)
     Hr = np.einsum("...in,...n,...jn->...ij", vec, ev, np.conj(vec))
     assert np.abs(Hr - H).max() < 1e-12
+
+
+@pytest.mark.parametrize("spin", [False, True])
+def test_full_threshold_list_is_the_dense_map(spin):
+    """A threshold below zero keeps every (R1, R2, o1, o2, sub-pixel) term, so
+    the list contraction must equal the dense contraction (pinned by the
+    supercell inverse); this also pins the builder's enumeration and factors."""
+    b = 4 if spin else 3
+    rng = np.random.default_rng(70 + b)
+    K, n, w, os_ = 6, 6, 1, 2
+    gR = rng.normal(size=(K, K, b, b)) + 1j * rng.normal(size=(K, K, b, b))
+    T = rng.normal(size=(b, b)) + 1j * rng.normal(size=(b, b))
+    tiles = rng.normal(size=(b, 6, 6))
+    flags, counts = oracle.mkthresholdlist(tiles, w, os_, -1.0, spin=spin)
+    assert len(flags) == (2 * w + 1) ** 4 * os_ ** 2 * (b * b if not spin else 2 * (b // 2) ** 2)
+    got = oracle.wannierldoslist(gR, T, flags, counts, n, w, os_)
+    ref = oracle.wannierldos(gR, T, tiles, n, w, os_, maxband=b, spin=spin)
+    assert rm.rel_errors(got, ref)[0] < 1e-12
+
+
+def test_threshold_list_selects_by_magnitude():
+    """The list at threshold t is exactly the full list's entries with
+    |factor| > t * max(w^2), pair by pair, in the same order."""
+    wl = synth.square_norb(3, K=8, n=6, E=1, seed=7, w=1, os_=2)
+    full, cfull = oracle.mkthresholdlist(wl.tiles, 1, 2, -1.0)
+    maxval = np.max(wl.tiles ** 2)
+    offs = np.concatenate([[0], np.cumsum(cfull)]).astype(int)
+    for t in (0.01, 0.05, 0.2):
+        sel, csel = oracle.mkthresholdlist(wl.tiles, 1, 2, t)
+        keep = np.abs(full["factor"]) > t * maxval
+        assert np.array_equal(sel, full[keep])
+        assert np.array_equal(csel, [keep[offs[p]:offs[p + 1]].sum() for p in range(len(cfull))])
+        assert 0 < len(sel) < len(full)
+
+
+def test_threshold_list_contraction_is_linear_in_the_list():
+    """Splitting every pair's list in two gives two maps that sum to the map of
+    the whole list (each entry adds its own term)."""
+    wl = synth.square_norb(3, K=8, n=6, E=1, seed=7, w=1, os_=2, eta=0.01)
+    ev, V = oracle.calchamiltonian(wl.hops, 3, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, 0.05, wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    flags, counts = oracle.mkthresholdlist(wl.tiles, 1, 2, 0.02)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    parts = [([], []), ([], [])]
+    for p in range(len(counts)):
+        seg = flags[offs[p]:offs[p + 1]]
+        h = len(seg) // 2
+        for k, piece in enumerate((seg[:h], seg[h:])):
+            parts[k][0].append(piece)
+            parts[k][1].append(len(piece))
+    whole = oracle.wannierldoslist(gR, T, flags, counts, 6, 1, 2)
+    split = sum(oracle.wannierldoslist(gR, T, np.concatenate(f), np.array(c, np.uintp), 6, 1, 2)
+                for f, c in parts)
+    assert rm.rel_errors(split, whole)[0] < 1e-12
