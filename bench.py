@@ -130,6 +130,46 @@ def time_steps(q, wl, steps, warmup, torch, stage_events=False):
     return dt, st
 
 
+def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
+    """The reference's other two contraction kinds at the 3-orbital geometry
+    (K=512, 51 energies, 128x128 cells, os=4, w=2), whole chain on the device:
+    the Josephson map of a superconducting model (3 orbitals x particle-hole,
+    6 bands; FP64 and FP32) and the threshold-list map (FP64) of the 3-orbital
+    workload with the driver's list rule at the given threshold."""
+    out = {}
+    wlj = synth.bdg_square(3, K=512, n=128, E=51, seed=1006, w=2, os_=4, eta=0.002, elo=-0.1, ehi=0.1)
+    jos = {"workload": "superconducting square lattice, 3 orbitals x particle-hole (6 bands), K=512, "
+                       "51 energies in [-0.1,0.1], eta=0.002, 128x128 cells, os=4, w=2; tip deltat=0.001, "
+                       "etat=eta"}
+    maps = {}
+    for prec in ("fp64", "fp32"):
+        q = cqpi.engine_for(wlj, prec)
+        q.set_josephson(True, deltat=0.001)
+        dt, st = time_steps(q, wlj, steps, warmup, torch)
+        jos[prec] = {"ms_per_step": dt * 1e3, "points_per_s": wlj.points / dt,
+                     "stages_ms": {k: v[0] for k, v in st.items()}}
+        maps[prec] = q.run(wlj.energies, wlj.eta, host=True)
+        q.close()
+    jos["fp32_vs_fp64_gpu_max_rel"] = max(rel_errors(maps["fp32"][e], maps["fp64"][e])[0]
+                                          for e in range(len(wlj.energies)))
+    out["josephson"] = jos
+    wl = synth.workload("square_3orb")
+    flags, counts = cqpi.make_threshold_list(wl.tiles, wl.window, wl.oversamp, threshold)
+    dense = (2 * wl.window + 1) ** 4 * wl.oversamp ** 2 * wl.bands ** 2
+    q = cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, flags, counts)
+    q.set_model(wl.hops, wl.efermi, wl.scmodel)
+    q.set_impurity(wl.V)
+    dt, st = time_steps(q, wl, steps, warmup, torch)
+    q.close()
+    out["threshold_list"] = {
+        "workload": "square lattice 3 orbitals, K=512, 51 energies, 128x128 cells, os=4, w=2 (the headline "
+                    "workload) with the threshold list",
+        "threshold": threshold, "entries": int(len(flags)), "entries_dense": dense,
+        "fp64": {"ms_per_step": dt * 1e3, "points_per_s": wl.points / dt,
+                 "stages_ms": {k: v[0] for k, v in st.items()}}}
+    return out
+
+
 def gradient_timing(cqpi, synth, torch, reps=3):
     """Gradient workload (3-orbital model, K=512, 21 energies, 64x64 cells):
     FP64 forward (keeping the per-energy windows) and the hand-written
@@ -302,6 +342,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the Josephson and threshold-list maps")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--stage-events", action="store_true",
                     help="record stage events in every step (one stream, one launch per stage)")
@@ -384,6 +425,8 @@ def main():
         out["batch"] = batch_timing(cqpi, synth, torch)
     if not args.no_grad:
         out["gradient"] = gradient_timing(cqpi, synth, torch)
+    if not args.no_variants:
+        out["variants"] = variants_timing(cqpi, synth, torch)
     if not args.no_cpu:
         info, maps = cpu_baseline(wl, args.cpu_energies)
         out["cpu_baseline"] = info
