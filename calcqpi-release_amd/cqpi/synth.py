@@ -494,7 +494,7 @@ def read_idl(path):
                   "offset": vals[2:4], "bias": vals[4:6], "setpoint": vals[6]}
 
 
-def write_inputs(dirpath, wl: Workload, qpifile="qpi.idl", extra=None):
+def write_inputs(dirpath, wl: Workload, qpifile="qpi.idl", extra=None, output="wannier"):
     """Writes model.dat, orbitals.idl and calcqpi.in into dirpath; returns the
     input-file path.  The impurity must be diagonal (the input format only
     carries per-orbital scattering, src/calcqpi.cc:254-272)."""
@@ -512,7 +512,7 @@ def write_inputs(dirpath, wl: Workload, qpifile="qpi.idl", extra=None):
     scat = ",".join("%.17g" % v.real for v in np.diag(wl.V))
     E = len(wl.energies)
     lines = [
-        'tbfile="model.dat";', 'qpifile="%s";' % qpifile, "output=wannier;",
+        'tbfile="model.dat";', 'qpifile="%s";' % qpifile, "output=%s;" % output,
         "lattice=%d;" % wl.n, "kpoints=%d;" % wl.K, "oversamp=%d;" % wl.oversamp,
         "window=%d;" % wl.window, "layers=%d;" % E,
         "energies=(%.17g,%.17g);" % (wl.energies[0], wl.energies[-1]),
@@ -520,6 +520,8 @@ def write_inputs(dirpath, wl: Workload, qpifile="qpi.idl", extra=None):
         "scattering=(%s);" % scat, 'idlorbitalfile="orbitals.idl";',
         "spin=%s;" % ("true" if wl.spin else "false"),
     ]
+    if wl.scmodel:
+        lines.append("scmodel=true;")
     if extra:
         lines += ["%s=%s;" % (k, v) for k, v in extra.items()]
     p = os.path.join(dirpath, "calcqpi.in")

@@ -17,9 +17,13 @@
 //     src/idl.cc:969-973); the map is W x W x L with W = oversamp*lattice,
 //     header as in src/calcqpi.cc:640,655 and data written as float32, x
 //     fastest (idl::write, src/idl.cc:78-95).
+//   * output=josephson (scmodel models only) with deltat and etat, the tip
+//     factor of src/calcqpi.cc:705-709, header name of src/calcqpi.cc:643-645;
+//   * threshold=t builds the threshold list (src/calcqpi.cc:620-634) and
+//     computes the wannier map from it (src/calcqpi.cc:699-702).
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
-// Modes other than output=wannier / green=normal and analytic or XSF orbitals
-// are reported as unsupported.
+// Other output and green modes, spinarr, and analytic or XSF orbitals are
+// reported as unsupported.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -204,7 +208,7 @@ int main(int argc, char **argv)
     }
     if (argi >= argc) {
         std::cerr << "Syntax: calcqpi_amd [--print-config] configfile\n"
-                  << "Continuum-LDOS (output=wannier, green=normal) on one MI355X; keys as calcqpi,\n"
+                  << "Continuum-LDOS (output=wannier|josephson, green=normal) on one MI355X; keys as calcqpi,\n"
                   << "plus precision=fp64|fp32|mixed; for the contraction.\n";
         return 1;
     }
@@ -223,6 +227,8 @@ int main(int argc, char **argv)
     long n = 201, layers = 21, oversamp = 4, window = 2, kpoints = 0;
     bool spin = false, scmodel = false;
     double lenergy = -1.0, henergy = 1.0, eta = 0.005, efermi = 0.0, magscat = 0.0;
+    double threshold = NAN, deltat = 0.001, etat = NAN;
+    bool josephson = false;
     std::complex<double> scatphase(1.0, 0.0);
     vector<std::complex<double>> scattering;
     string tbfile, qpifile, precision = "fp64";
@@ -239,12 +245,24 @@ int main(int argc, char **argv)
     }
     if (lf.probe("spin")) spin = lf.get("spin") == "true";
     if (lf.probe("scmodel")) scmodel = lf.get("scmodel") == "true";
-    if (lf.probe("output") && lf.get("output") != "wannier") {
-        std::cerr << "output=" << lf.get("output") << " is not supported by this build (wannier only).\n";
+    if (lf.probe("output")) {
+        const string mode = lf.get("output");
+        if (mode == "josephson") josephson = true;
+        else if (mode != "wannier") {
+            std::cerr << "output=" << mode << " is not supported by this build (wannier, josephson).\n";
+            return 1;
+        }
+    }
+    if (lf.probe("threshold")) threshold = lf.num("threshold");
+    if (lf.probe("spinarr")) {
+        std::cerr << "spinarr is not supported by this build.\n";
         return 1;
     }
-    if (lf.probe("threshold")) {
-        std::cerr << "threshold lists are not supported by this build.\n";
+    if (lf.probe("deltat")) deltat = lf.num("deltat");
+    if (lf.probe("etat")) etat = lf.num("etat");
+    if (josephson && !scmodel) {
+        // src/calcqpi.cc:458-461
+        std::cerr << "Error: Calculation in josephson-mode with scmodel=false makes no sense - exiting.\n";
         return 1;
     }
     if (lf.probe("qpifile")) qpifile = lf.str("qpifile");
@@ -260,6 +278,7 @@ int main(int argc, char **argv)
         }
     }
     if (lf.probe("eta")) eta = lf.num("eta");
+    if (std::isnan(etat)) etat = eta;  // src/calcqpi.cc:249-250
     if (lf.probe("kpoints")) kpoints = lf.integer("kpoints");
     if (!kpoints) kpoints = n;
     if (lf.probe("phase")) {
@@ -345,7 +364,10 @@ int main(int argc, char **argv)
                   << ", \"layers\": " << layers << ", \"energies\": [" << lenergy << ", " << henergy
                   << "], \"eta\": " << eta << ", \"fermi\": " << efermi << ", \"spin\": " << (spin ? "true" : "false")
                   << ", \"scmodel\": " << (scmodel ? "true" : "false") << ", \"precision\": \"" << precision
-                  << "\", \"vscat\": [";
+                  << "\", \"output\": \"" << (josephson ? "josephson" : "wannier") << "\", \"threshold\": ";
+        if (std::isnan(threshold)) std::cout << "null";
+        else std::cout << threshold;
+        std::cout << ", \"deltat\": " << deltat << ", \"etat\": " << etat << ", \"vscat\": [";
         for (size_t i = 0; i < vscat.size(); i++)
             std::cout << (i ? ", " : "") << "[" << vscat[i].real() << ", " << vscat[i].imag() << "]";
         double tsum = 0;
@@ -366,9 +388,30 @@ int main(int argc, char **argv)
 
     auto t0 = std::chrono::steady_clock::now();
     cqpi_ctx *ctx = nullptr;
-    if (check(cqpi_create((size_t)W, (size_t)kpoints, (size_t)n, (size_t)window, (size_t)oversamp, bands, maxbands,
-                          spin, tiles.data(), &ctx),
-              "cqpi_create"))
+    if (!std::isnan(threshold) && !josephson) {
+        // threshold list over the maxbands tiles (src/calcqpi.cc:620-634)
+        const size_t wins = (size_t)(2 * window + 1);
+        vector<size_t> counts(wins * wins * wins * wins);
+        size_t total = 0;
+        if (check(cqpi_make_threshold_list(tiles.data(), maxbands, (size_t)window, (size_t)oversamp, threshold,
+                                           maxbands, spin, nullptr, 0, counts.data(), &total),
+                  "cqpi_make_threshold_list"))
+            return 1;
+        vector<cqpi_flagentry> flags(std::max<size_t>(total, 1));
+        if (check(cqpi_make_threshold_list(tiles.data(), maxbands, (size_t)window, (size_t)oversamp, threshold,
+                                           maxbands, spin, flags.data(), total, counts.data(), &total),
+                  "cqpi_make_threshold_list"))
+            return 1;
+        const size_t entries = wins * wins * wins * wins * (size_t)(oversamp * oversamp) * maxbands * maxbands;
+        *log << "Threshold list has " << total << "/" << entries << " entries (" << 100.0 * total / entries
+             << "%), requiring " << total * sizeof(cqpi_flagentry) / 1024.0 / 1024.0 / 1024.0 << "GB of memory.\n";
+        if (check(cqpi_create_list((size_t)W, (size_t)kpoints, (size_t)n, (size_t)window, (size_t)oversamp, bands,
+                                   flags.data(), counts.data(), &ctx),
+                  "cqpi_create_list"))
+            return 1;
+    } else if (check(cqpi_create((size_t)W, (size_t)kpoints, (size_t)n, (size_t)window, (size_t)oversamp, bands,
+                                 maxbands, spin, tiles.data(), &ctx),
+                     "cqpi_create"))
         return 1;
     char info[2048];
     if (!cqpi_printinfo(ctx, info, sizeof info)) *log << info;
@@ -383,6 +426,7 @@ int main(int argc, char **argv)
     if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed"  ? CQPI_PREC_MIXED
                                                     : precision == "fp32" ? CQPI_PREC_FP32
                                                                           : CQPI_PREC_FP64), "cqpi_set_precision");
+    if (!rc && josephson) rc = check(cqpi_set_josephson(ctx, 1, deltat, etat), "cqpi_set_josephson");
     IdlMap out;
     out.data.assign((size_t)W * W * layers, 0.0);
     if (!rc) rc = check(cqpi_run(ctx, omegas.data(), omegas.size(), eta, out.data.data()), "cqpi_run");
@@ -391,7 +435,8 @@ int main(int argc, char **argv)
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     std::ostringstream comment;
-    comment << "CQPI calculation for " << tbname << " with window=" << window << " and oversamp=" << oversamp;
+    comment << (josephson ? "Josephson calculation for " : "CQPI calculation for ") << tbname
+            << " with window=" << window << " and oversamp=" << oversamp;
     out.name = comment.str();
     time_t now = time(nullptr);
     out.date = ctime(&now);

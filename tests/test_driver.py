@@ -91,3 +91,74 @@ def test_driver_end_to_end_matches_oracle(tmp_path, gpu):
         got = data[:, :, e].T.ravel()  # x + y*W
         # the idl file stores float32
         assert rm.rel_errors(got, ref[e].astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+def _bdg_diag(E=4, K=32, n=8):
+    """The superconducting test model with a diagonal impurity (the input
+    format carries per-orbital scattering only).  An even number of energies
+    keeps w = 0 off the grid: there the Josephson map of this particle-hole
+    symmetric model vanishes (both sides are rounding noise)."""
+    wl = synth.bdg_square(3, K=K, n=n, E=E, seed=91, w=1, os_=2, eta=0.01)
+    wl.V = np.diag([0.5, 0.3, 0.2, -0.5, -0.3, -0.2]).astype(np.complex128)
+    return wl
+
+
+def test_driver_josephson_and_threshold_keys(tmp_path):
+    wl = _bdg_diag()
+    p = synth.write_inputs(str(tmp_path / "j"), wl, output="josephson", extra={"deltat": "0.02"})
+    cfg, err = _print_config(p)
+    assert cfg["output"] == "josephson" and cfg["deltat"] == 0.02 and cfg["etat"] == wl.eta
+    assert cfg["scmodel"] is True and cfg["maxbands"] == 3 and cfg["threshold"] is None
+    assert "Unused" not in err
+    wl3 = synth.square_3orb(K=8, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path / "t"), wl3, extra={"threshold": "0.05", "etat": "0.003"})
+    cfg, _ = _print_config(p)
+    assert cfg["output"] == "wannier" and cfg["threshold"] == 0.05 and cfg["etat"] == 0.003
+
+
+def test_driver_josephson_needs_scmodel(tmp_path):
+    wl = synth.square_3orb(K=8, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path), wl, output="josephson")
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert r.returncode != 0 and "scmodel=false" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_josephson_matches_oracle(tmp_path, gpu):
+    wl = _bdg_diag()
+    p = synth.write_inputs(str(tmp_path), wl, output="josephson", extra={"deltat": "0.01"})
+    print("case: driver run, Josephson map, BdG 6 bands, K=32, n=8, E=4", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    data, hdr = synth.read_idl(str(tmp_path / "qpi.idl"))
+    assert hdr["name"].startswith("Josephson calculation for ")
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"), scmodel=True)
+    ev, vec = oracle.calchamiltonian(hops, b, wl.efermi, wl.K, scmodel=True)
+    for e, om in enumerate(wl.energies):
+        gR = oracle.realspace(oracle.greens_k(ev, vec, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        oc = complex(om, wl.eta)
+        tip = complex(0.0, -np.copysign(1.0, om) * np.pi * 0.01) / np.sqrt(oc * oc - 0.01 ** 2)
+        ref = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip)
+        got = data[:, :, e].T.ravel()
+        assert rm.rel_errors(got, ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_threshold_matches_oracle(tmp_path, gpu):
+    wl = synth.square_3orb(K=32, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path), wl, extra={"threshold": "0.03"})
+    print("case: driver run, threshold list 0.03, config-2 model, K=32, n=8, E=3", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Threshold list has" in r.stdout
+    data, _ = synth.read_idl(str(tmp_path / "qpi.idl"))
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
+    flags, counts = oracle.mkthresholdlist(wl.tiles, wl.window, wl.oversamp, 0.03)
+    ev, vec = oracle.calchamiltonian(hops, b, wl.efermi, wl.K)
+    for e, om in enumerate(wl.energies):
+        gR = oracle.realspace(oracle.greens_k(ev, vec, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, wl.window, wl.oversamp)
+        got = data[:, :, e].T.ravel()
+        assert rm.rel_errors(got, ref.astype(np.float32).astype(np.float64))[0] < 1e-6
