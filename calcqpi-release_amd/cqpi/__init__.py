@@ -45,6 +45,11 @@ SIGNATURES = {
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
+    "cqpi_create_spectral": (C.c_int, [_sz, _sz, _sz, _sz, C.c_int, _dp, _sz, _dp, _sz, C.POINTER(_vp)]),
+    "cqpi_spf": (C.c_int, [_vp, C.POINTER(_vp), _sz]),
+    "cqpi_uspf": (C.c_int, [_vp, C.POINTER(_vp), _sz]),
+    "cqpi_set_spectral": (C.c_int, [_vp, C.c_int]),
+    "cqpi_dos": (C.c_int, [_vp, _dp, _sz, C.c_double, _dp]),
     "cqpi_wannierldoslist": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_make_threshold_list": (C.c_int, [_dp, _sz, _sz, _sz, C.c_double, _sz, C.c_int, _vp, _sz,
                                            C.POINTER(_sz), C.POINTER(_sz)]),
@@ -216,6 +221,58 @@ class AmdQPI:
         self.nlayers = 0
         self._nhops = 0
         return self
+
+    @classmethod
+    def spectral(cls, kpoints, n, bands, maxbands=0, spin=False, pos=None, prearr=None):
+        """A spectral-function context (cqpi_create_spectral): maps n x n; pos
+        (maxbands x 2) and prearr for the unfolded spectral function."""
+        self = cls.__new__(cls)
+        self.K, self.n, self.W = int(kpoints), int(n), int(n)
+        self.window, self.oversamp, self.bands = 0, 1, int(bands)
+        self.maxband, self.spin = int(maxbands) if maxbands else int(bands), bool(spin)
+        self._pos = np.ascontiguousarray(np.zeros((0, 2)) if pos is None else np.asarray(pos)[:, :2], np.float64)
+        self._pre = np.ascontiguousarray(np.zeros(0) if prearr is None else prearr, np.float64)
+        self._wf = None
+        self._ctx = _vp()
+        _check(lib().cqpi_create_spectral(self.K, self.n, self.bands, self.maxband, int(self.spin),
+                                          _d(self._pos) if len(self._pos) else None, len(self._pos),
+                                          _d(self._pre) if len(self._pre) else None, len(self._pre),
+                                          C.byref(self._ctx)))
+        self.nlayers = 0
+        self._nhops = 0
+        return self
+
+    def _g0k_table(self, g0k):
+        b = self.bands
+        g0k = np.ascontiguousarray(g0k, dtype=np.complex128)
+        if g0k.shape != (self.K, self.K, b, b):
+            raise ValueError("g0 must have shape (K, K, b, b)")
+        ptrs = (g0k.ctypes.data + np.arange(self.K * self.K, dtype=np.uint64) * np.uint64(b * b * 16)).astype(np.uint64)
+        return g0k, ptrs
+
+    def spf(self, g0k):
+        """GPUQPI::spf: g0k = G0(k) on the grid, shape (K, K, b, b)."""
+        g0k, ptrs = self._g0k_table(g0k)
+        _check(lib().cqpi_spf(self._ctx, ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
+        self.nlayers = 1
+
+    def uspf(self, g0k):
+        """GPUQPI::uspf: g0k = G0(k) on the grid, shape (K, K, b, b)."""
+        g0k, ptrs = self._g0k_table(g0k)
+        _check(lib().cqpi_uspf(self._ctx, ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
+        self.nlayers = 1
+
+    def set_spectral(self, mode):
+        """cqpi_run computes SPF ("spf"), USPF ("uspf") maps, or none (None)."""
+        m = {None: 0, "spf": 1, "uspf": 2}[mode]
+        _check(lib().cqpi_set_spectral(self._ctx, m))
+
+    def dos(self, omegas, eta):
+        """Density of states -sum Im G0(R=0)_oo per energy (cqpi_dos)."""
+        om = np.ascontiguousarray(omegas, np.float64)
+        out = np.zeros(len(om))
+        _check(lib().cqpi_dos(self._ctx, _d(om), len(om), float(eta), _d(out)))
+        return out
 
     def wannierldoslist(self, scat, g0):
         """GPUQPI::wannierldoslist for a context made by from_list."""

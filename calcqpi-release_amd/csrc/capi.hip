@@ -136,6 +136,16 @@ struct cqpi_ctx {
     size_t c_T = 0;
     double2 *d_tips = nullptr;  // tip factor of each energy of a Josephson map
     size_t c_tips = 0;
+    // spectral-function context (cqpi_create_spectral): positions and
+    // prefactors of the maxband orbitals, G0(k) of the drop-in calls, the
+    // map cqpi_run makes (0: none, 1: SPF, 2: USPF), DOS buffers
+    bool spectral = false;
+    int spec_mode = 0;
+    double *d_spos = nullptr;
+    double2 *d_g0k = nullptr, *h_g0k = nullptr;
+    size_t c_g0k = 0, c_hg0k = 0, c_spos = 0;
+    double *d_dos = nullptr, *d_dosp = nullptr, *d_domega = nullptr;
+    size_t c_dos = 0, c_dosp = 0, c_domega = 0;
     // threshold list (cqpi_create_list): per-pair factor table and pair index
     bool list_mode = false;
     double *d_lcoef = nullptr;
@@ -216,7 +226,8 @@ void release(cqpi_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
                     c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_lcoef, c->d_lr1off,
-                    c->d_lpair, c->d_status, c->d_omega, c->d_ldos,
+                    c->d_lpair, c->d_spos, c->d_g0k, c->d_dos, c->d_dosp, c->d_domega,
+                    c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
@@ -224,6 +235,7 @@ void release(cqpi_ctx *c)
         if (p) (void)hipFree(p);
     if (c->h_win) (void)hipHostFree(c->h_win);
     if (c->h_T) (void)hipHostFree(c->h_T);
+    if (c->h_g0k) (void)hipHostFree(c->h_g0k);
     if (c->h_status) (void)hipHostFree(c->h_status);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
@@ -718,6 +730,119 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
 
 void cqpi_destroy(cqpi_ctx *ctx) { release(ctx); }
 
+int cqpi_create_spectral(size_t kpoints, size_t n, size_t bands, size_t maxbands, int spin, const double *pos,
+                         size_t npos, const double *prearr, size_t nprearr, cqpi_ctx **out)
+{
+    g_err.clear();
+    if (!out) return fail(CQPI_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (!maxbands) maxbands = bands;
+    if (n > kpoints) return fail(CQPI_E_ARG, "the map side n must not exceed kpoints");
+    if (maxbands > bands) return fail(CQPI_E_ARG, "maxbands > bands");
+    if (npos && npos < maxbands) return fail(CQPI_E_ARG, "pos must hold maxbands positions (or none)");
+    if ((npos && !pos) || (nprearr && !prearr)) return fail(CQPI_E_ARG, "NULL pos or prearr");
+    // orbital x, y and prefactor (the reference's d_mpos, src/qpihipkernel.cc:511-521)
+    std::vector<double> sp(3 * maxbands, 0.0);
+    for (size_t i = 0; i < maxbands; i++) {
+        if (npos) {
+            sp[3 * i] = pos[2 * i];
+            sp[3 * i + 1] = pos[2 * i + 1];
+        }
+        sp[3 * i + 2] = nprearr ? prearr[i % nprearr] : 1.0;
+    }
+    std::vector<double> zero_tiles(maxbands, 0.0);  // window 0, oversampling 1: map n x n
+    RETCHK(cqpi_create(n, kpoints, n, 0, 1, bands, maxbands, spin, zero_tiles.data(), out));
+    cqpi_ctx *c = *out;
+    c->spectral = true;
+    int r = dev_grow(c->d_spos, c->c_spos, sp.size(), c->gpumem);
+    if (!r && hipMemcpy(c->d_spos, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "position upload failed");
+    if (r) {
+        release(c);
+        *out = nullptr;
+    }
+    return r;
+}
+
+// one spectral map of the drop-in path from the caller's G0(k) on the whole grid
+static int dropin_spectral(cqpi_ctx *c, const double *const *g0k, size_t tda, int mode)
+{
+    g_err.clear();
+    if (!c || !g0k) return fail(CQPI_E_ARG, "NULL argument");
+    if (!c->spectral) return fail(CQPI_E_STATE, "this context was not created by cqpi_create_spectral");
+    const Geom &g = c->g;
+    const size_t B = g.B, NP = B * B, KK = (size_t)g.K * g.K, n = KK * NP;
+    if (tda < B) return fail(CQPI_E_ARG, "tda < bands");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    reset_stages(c);
+    if (n > c->c_hg0k) {
+        if (c->h_g0k) (void)hipHostFree(c->h_g0k);
+        c->h_g0k = nullptr;
+        c->c_hg0k = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_g0k, n * sizeof(double2)));
+        c->c_hg0k = n;
+    }
+    for (size_t kk = 0; kk < KK; kk++) {
+        const double *m = g0k[kk];
+        if (!m) return fail(CQPI_E_ARG, "NULL matrix in g0 table");
+        for (size_t r = 0; r < B; r++)
+            for (size_t cc = 0; cc < B; cc++)
+                c->h_g0k[kk * NP + r * B + cc] = make_double2(m[2 * (r * tda + cc)], m[2 * (r * tda + cc) + 1]);
+    }
+    RETCHK(dev_grow(c->d_g0k, c->c_g0k, n, c->gpumem));
+    RETCHK(dev_grow(c->d_ldos, c->c_ldos, (size_t)g.W * g.W, c->gpumem));
+    HIPCHK(hipMemcpyAsync(c->d_g0k, c->h_g0k, n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->copy_done, c->stream));
+    int st = stage_begin(c, 4);
+    int r = launch_spectral(g.K, g.n, g.B, g.maxb, g.spin, mode, c->d_g0k, nullptr, nullptr, nullptr, 1, 0.0,
+                            c->d_spos, c->d_ldos, c->stream);
+    stage_end(c, st);
+    if (r) return fail(r, "spectral-function launch failed");
+    c->nlayers = 1;
+    c->windows_kept = false;
+    HIPCHK(hipEventSynchronize(c->copy_done));  // the caller may overwrite g0 now
+    return 0;
+}
+
+int cqpi_spf(cqpi_ctx *c, const double *const *g0k, size_t tda) { return dropin_spectral(c, g0k, tda, 1); }
+
+int cqpi_uspf(cqpi_ctx *c, const double *const *g0k, size_t tda) { return dropin_spectral(c, g0k, tda, 2); }
+
+int cqpi_set_spectral(cqpi_ctx *c, int mode)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (mode < 0 || mode > 2) return fail(CQPI_E_ARG, "mode must be 0 (none), 1 (SPF) or 2 (USPF)");
+    if (mode && !c->spectral) return fail(CQPI_E_STATE, "this context was not created by cqpi_create_spectral");
+    c->spec_mode = mode;
+    return 0;
+}
+
+int cqpi_dos(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *dos)
+{
+    g_err.clear();
+    if (!c || !omegas || !dos) return fail(CQPI_E_ARG, "NULL argument");
+    if (!E) return fail(CQPI_E_ARG, "no energies");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->eigen_valid) {
+        c->in_run = true;
+        int r = cqpi_compute_eigen(c);
+        c->in_run = false;
+        if (r) return r;
+    }
+    const Geom &g = c->g;
+    RETCHK(dev_grow(c->d_domega, c->c_domega, E, c->gpumem));
+    RETCHK(dev_grow(c->d_dos, c->c_dos, E, c->gpumem));
+    RETCHK(dev_grow(c->d_dosp, c->c_dosp, E * (size_t)dos_chunks(g.K), c->gpumem));
+    HIPCHK(hipMemcpy(c->d_domega, omegas, E * sizeof(double), hipMemcpyHostToDevice));
+    int r = launch_dos(g.K, g.B, g.maxb, c->d_eval, c->d_evec, c->d_domega, (int)E, eta, c->d_dosp, c->d_dos,
+                       c->stream);
+    if (r) return fail(r, "density-of-states launch failed");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(dos, c->d_dos, E * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands,
                      const cqpi_flagentry *flags, const size_t *counts, cqpi_ctx **out)
 {
@@ -870,6 +995,7 @@ int cqpi_wannierldos(cqpi_ctx *c, const double *scat, const double *const *g0, s
 {
     if (c && c->list_mode)
         return fail(CQPI_E_STATE, "this context holds a threshold list: use cqpi_wannierldoslist");
+    if (c && c->spectral) return fail(CQPI_E_STATE, "this context makes spectral-function maps (cqpi_spf, cqpi_uspf)");
     return dropin_map(c, scat, g0, tda, nullptr);
 }
 
@@ -883,6 +1009,8 @@ int cqpi_wannierjosephson(cqpi_ctx *c, const double *scat, const double *const *
                           double tip_im)
 {
     const double tip[2] = {tip_re, tip_im};
+    if (c && (c->list_mode || c->spectral))
+        return fail(CQPI_E_STATE, "the Josephson map needs a context made by cqpi_create");
     return dropin_map(c, scat, g0, tda, tip);
 }
 
@@ -1032,6 +1160,29 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
         c->in_run = false;
         if (r) return r;
     }
+    if (c->spectral && !c->spec_mode)
+        return fail(CQPI_E_STATE, "a spectral-function context: select SPF or USPF with cqpi_set_spectral");
+    if (c->spec_mode) {
+        // SPF / USPF maps from the eigen data, n x n per energy
+        const Geom &g = c->g;
+        RETCHK(dev_grow(c->d_ldos, c->c_ldos, E * (size_t)g.W * g.W, c->gpumem));
+        RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
+        c->up_omegas.assign(omegas, omegas + E);
+        HIPCHK(hipMemcpyAsync(c->d_omega, c->up_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        int st = stage_begin(c, 4);
+        int r = launch_spectral(g.K, g.n, g.B, g.maxb, g.spin, c->spec_mode, nullptr, c->d_eval, c->d_evec,
+                                c->d_omega, (int)E, eta, c->d_spos, c->d_ldos, c->stream);
+        stage_end(c, st);
+        if (r) return fail(r, "spectral-function launch failed");
+        c->nlayers = E;
+        c->windows_kept = false;
+        c->last_omegas.assign(omegas, omegas + E);
+        c->last_eta = eta;
+        stage_collect(c);
+        if (ldos_host)
+            HIPCHK(hipMemcpy(ldos_host, c->d_ldos, E * (size_t)g.W * g.W * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    }
     RunPlan plan;
     RETCHK(prepare_run(c, omegas, E, eta, plan));
     RETCHK(enqueue_chunks(c, plan));
@@ -1045,7 +1196,8 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
 int cqpi_set_josephson(cqpi_ctx *c, int on, double deltat, double etat)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
-    if (on && c->list_mode) return fail(CQPI_E_ARG, "a threshold-list context computes the LDOS map only");
+    if (on && (c->list_mode || c->spectral))
+        return fail(CQPI_E_ARG, "the Josephson map needs a context made by cqpi_create");
     if (on && 2 * c->g.maxb > c->g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
     if (on && c->precision == CQPI_PREC_MIXED)
         return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");

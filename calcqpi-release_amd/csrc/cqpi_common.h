@@ -203,6 +203,16 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
 int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
                          const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
                          hipStream_t s);
+// spectral-function maps n x n per energy (mode 1: SPF, 2: USPF; pos: mb x 3 =
+// x, y, prefactor): from G0(k) blocks g0k[K*K][b*b] (one energy) or, with
+// g0k == NULL, from the eigen data for the E energies omegas (device)
+int launch_spectral(int K, int n, int B, int mb, int spin, int mode, const double2 *g0k, const double *eval,
+                    const double2 *evec, const double *omegas, int E, double eta, const double *pos, double *out,
+                    hipStream_t s);
+// density of states -sum_{o<mb} Im G0(R=0)_oo per energy; partial: E * dos_chunks(K) doubles
+int dos_chunks(int K);
+int launch_dos(int K, int B, int mb, const double *eval, const double2 *evec, const double *omegas, int E,
+               double eta, double *partial, double *dos, hipStream_t s);
 // FP32 contraction: narrows the E windows and T into G0f (E*nw*nw*b*b) and Tf
 // (E*b*b) first; tilesf are the tiles in FP32
 int launch_contract_fp32(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,

@@ -35,11 +35,6 @@ class AmdQPI : public GPUQPI
             exit(rc);
         }
     }
-    static void unsupported(const char *what)
-    {
-        std::cerr << what << " is not provided by the MI355X plug-in." << std::endl;
-        exit(1);
-    }
 
 public:
     AmdQPI(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands_,
@@ -75,13 +70,36 @@ public:
         check(cqpi_create_list(wanniern, kpoints, n, window, oversamp, bands, flat.data(), counts.data(), &ctx));
         scatbuf.resize(2 * bands * bands);
     }
+    // the spectral-function constructor (src/qpihipkernel.cc:498-533): maps n x n
+    AmdQPI(size_t kpoints, size_t n, size_t bands_, size_t maxbands, bool spin, std::vector<std::vector<double> > &pos,
+           std::vector<double> &prearr)
+        : W(n), K(kpoints), bands(bands_)
+    {
+        std::vector<double> xy;
+        for (const std::vector<double> &p : pos) {
+            xy.push_back(p.size() > 0 ? p[0] : 0.0);
+            xy.push_back(p.size() > 1 ? p[1] : 0.0);
+        }
+        check(cqpi_create_spectral(kpoints, n, bands, maxbands, spin, xy.empty() ? nullptr : xy.data(), pos.size(),
+                                   prearr.empty() ? nullptr : prearr.data(), prearr.size(), &ctx));
+        scatbuf.resize(2 * bands * bands);
+    }
     void printinfo(std::ostream &os)
     {
         char buf[2048];
         check(cqpi_printinfo(ctx, buf, sizeof buf));
         os << buf;
     }
-    // T packed with ld = bands, and the pointer table of g0 (built once)
+    // the pointer table of g0 (built once: the matrices are reused)
+    void table_of(std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    {
+        if (table.empty()) {
+            table.resize(K * K);
+            for (size_t ix = 0; ix < K; ix++)
+                for (size_t iy = 0; iy < K; iy++) table[ix * K + iy] = g0[ix][iy]->data;
+        }
+    }
+    // T packed with ld = bands, and the pointer table
     void pack(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
     {
         for (size_t i = 0; i < bands; i++)
@@ -89,11 +107,7 @@ public:
                 scatbuf[2 * (i * bands + j)] = scat->data[2 * (i * scat->tda + j)];
                 scatbuf[2 * (i * bands + j) + 1] = scat->data[2 * (i * scat->tda + j) + 1];
             }
-        if (table.empty()) {
-            table.resize(K * K);
-            for (size_t ix = 0; ix < K; ix++)
-                for (size_t iy = 0; iy < K; iy++) table[ix * K + iy] = g0[ix][iy]->data;
-        }
+        table_of(g0);
     }
     void wannierldos(gsl_matrix_complex *scat, std::vector<std::vector<gsl_matrix_complex *> > &g0)
     {
@@ -111,8 +125,17 @@ public:
         pack(scat, g0);
         check(cqpi_wannierldoslist(ctx, scatbuf.data(), table.data(), g0[0][0]->tda));
     }
-    void spf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("SPF mode"); }
-    void uspf(std::vector<std::vector<gsl_matrix_complex *> > &) { unsupported("USPF mode"); }
+    // g0 holds G0(k) in these two modes (the reference skips the transform)
+    void spf(std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    {
+        table_of(g0);
+        check(cqpi_spf(ctx, table.data(), g0[0][0]->tda));
+    }
+    void uspf(std::vector<std::vector<gsl_matrix_complex *> > &g0)
+    {
+        table_of(g0);
+        check(cqpi_uspf(ctx, table.data(), g0[0][0]->tda));
+    }
     void retrieveResult(double *ldos) { check(cqpi_retrieve(ctx, ldos)); }
     void retrieveResult(idl &map, size_t layer)
     {

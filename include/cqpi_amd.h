@@ -151,6 +151,33 @@ int cqpi_make_threshold_list(const double *wf, size_t norb, size_t window, size_
                              size_t maxband, int spin, cqpi_flagentry *out, size_t cap, size_t *counts,
                              size_t *total);
 
+/* Replaces HipQPI::HipQPI(kpoints, n, bands, maxbands, spin, pos, prearr)
+ * (src/qpihipkernel.h:77, src/qpihipkernel.cc:498-533): a context for the
+ * spectral-function maps, n x n (n <= kpoints; the driver sets n = kpoints
+ * for SPF, src/calcqpi.cc:333-334).  pos: npos x 2 orbital positions x, y
+ * (npos = 0: all at the origin, else npos >= maxbands); prearr: nprearr
+ * prefactors, orbital i uses prearr[i % nprearr] (nprearr = 0: all 1).
+ * maxbands = 0 means bands.  The maps are retrieved with cqpi_retrieve. */
+int cqpi_create_spectral(size_t kpoints, size_t n, size_t bands, size_t maxbands, int spin, const double *pos,
+                         size_t npos, const double *prearr, size_t nprearr, cqpi_ctx **out);
+
+/* Replaces HipQPI::spf(g0) (include/gpuqpi.h:26, src/qpihipkernel.cc:638-643;
+ * kernel :317-329, CPU src/qpi.cc:657-681): the spectral function
+ *   A(k) = -sum_{o<maxbands} Im G0(k)_oo
+ * of grid point (i, j), i, j < n, at pixel ((i - n/2) mod n, (j - n/2) mod n).
+ * g0: the kpoints*kpoints pointer table of cqpi_wannierldos, here holding
+ * G0(k) (the reference does not transform it for this mode); copied before
+ * the call returns. */
+int cqpi_spf(cqpi_ctx *ctx, const double *const *g0, size_t tda);
+
+/* Replaces HipQPI::uspf(g0) (include/gpuqpi.h:27, src/qpihipkernel.cc:645-653;
+ * kernels :331-370, CPU src/qpi.cc:731-775): the unfolded spectral function
+ * at pixel (i, j), q = ((i - n/2)/kpoints, (j - n/2)/kpoints),
+ *   -sum_{k,l} z_k z_l Im[ G0(k_g)_{l,k} exp(2 pi i q.(x_k - x_l)) ]
+ * over maxbands orbitals (with spin: within each half), k_g the grid point
+ * ((i + n/2) mod n, (j + n/2) mod n) mod kpoints.  Arguments as cqpi_spf. */
+int cqpi_uspf(cqpi_ctx *ctx, const double *const *g0, size_t tda);
+
 /* Replaces HipQPI::retrieveResult(double *ldos) (src/qpihipkernel.cc:690-704):
  * waits for the device and copies the W x W map of the last wannierldos call
  * (or layer 0 of the last cqpi_run) to ldos. */
@@ -229,6 +256,18 @@ int cqpi_set_timing(cqpi_ctx *ctx, int on);
  * 2*maxband <= bands and FP64 or FP32 precision; a run with it on cannot feed
  * cqpi_gradient. */
 int cqpi_set_josephson(cqpi_ctx *ctx, int on, double deltat, double etat);
+
+/* For a context of cqpi_create_spectral: mode 1 (SPF) or 2 (USPF) makes
+ * cqpi_run compute those maps (n x n per energy) from the device's eigen
+ * data instead of host G0(k) (output=spf|uspf of the reference driver,
+ * src/calcqpi.cc:712-717); 0 selects none. */
+int cqpi_set_spectral(cqpi_ctx *ctx, int mode);
+
+/* Density of states -sum_{o<maxband} Im G0(R=0)_oo (tmatrix::calcdos,
+ * src/qpi.cc:856-862; the driver's dosfile, src/calcqpi.cc:336-391) for E
+ * energies, G0(R=0) = kpoints^-2 sum_k G0(k) from the device's eigen data;
+ * dos: E doubles on the host.  Needs cqpi_set_model. */
+int cqpi_dos(cqpi_ctx *ctx, const double *omegas, size_t E, double eta, double *dos);
 
 /* ---------------------------------------------------------------------- */
 /* Gradient (extension; the reference has no differentiation)              */

@@ -662,6 +662,58 @@ void or_wannierldoslist(size_t K, size_t n, size_t window, size_t oversamp, size
 }
 
 /* ------------------------------------------------------------------------- */
+/* Spectral function maps from G0(k) on the K x K grid (block (i, j) at        */
+/* ((i*K + j)*b*b), row-major b x b).  Map n x n, x + y*n.                     */
+/* SPF: gpucalcspf src/qpihipkernel.cc:317-329 (CPU calcspf src/qpi.cc:657-681 */
+/* with shift, n = K): k index (i, j), i, j < n, at pixel ((i-n/2) mod n, ...) */
+/* of -sum_{o<maxband} Im G0(k)_oo.                                            */
+/* USPF: calcuspf src/qpi.cc:731-775 (GPU :331-370): pixel (i, j), k = ((i -   */
+/* n/2)/K, (j - n/2)/K) in the phase, grid index ((i + n/2) mod n) mod K,      */
+/* -sum_{k,l} z_k z_l Im[ G0_{l,k} e^{2 pi i k.(x_k - x_l)} ] over maxband      */
+/* orbitals (spin: each half).  pos: maxband x 2 (x, y), z: maxband.           */
+/* ------------------------------------------------------------------------- */
+void or_spf(size_t K, size_t n, size_t b, size_t maxband, const double *g0k_in, double *out)
+{
+    const cplx *g0 = (const cplx *)g0k_in;
+    size_t n2 = n >> 1;
+    if (!maxband) maxband = b;
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < n; j++) {
+            const cplx *G = g0 + (i * K + j) * b * b;
+            double tr = 0.0;
+            for (size_t o = 0; o < maxband; o++) tr -= cimag(G[o * b + o]);
+            out[((n + i - n2) % n) + ((n + j - n2) % n) * n] = tr;
+        }
+}
+
+void or_uspf(size_t K, size_t n, size_t b, size_t maxband, int spin, const double *pos, const double *z,
+             const double *g0k_in, double *out)
+{
+    const cplx *g0 = (const cplx *)g0k_in;
+    size_t n2 = n >> 1;
+    if (!maxband) maxband = b;
+    size_t sb = maxband >> 1;
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < n; j++) {
+            double kx = ((double)i - (double)n2) / (double)K, ky = ((double)j - (double)n2) / (double)K;
+            size_t ki = ((i + n2) % n) % K, kj = ((j + n2) % n) % K;
+            const cplx *G = g0 + (ki * K + kj) * b * b;
+            double tr = 0.0;
+            int nsets = spin ? 2 : 1;
+            size_t np = spin ? sb : maxband;
+            for (int sg = 0; sg < nsets; sg++) {
+                size_t off = sg ? sb : 0;
+                for (size_t k = off; k < off + np; k++)
+                    for (size_t l = off; l < off + np; l++) {
+                        double ph = 2.0 * M_PI * (kx * (pos[2 * k] - pos[2 * l]) + ky * (pos[2 * k + 1] - pos[2 * l + 1]));
+                        tr -= z[k] * z[l] * cimag(G[l * b + k] * cexp(I * ph));
+                    }
+            }
+            out[i + j * n] = tr;
+        }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Whole path for a list of energies (the energy loop of src/calcqpi.cc:671-760 */
 /* with green=normal and output=wannier).  ldos_out: E x W x W, x + y*W.       */
 /* Returns nonzero if a T-matrix inversion failed.                            */

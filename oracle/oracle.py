@@ -49,6 +49,8 @@ def lib():
                                          C.POINTER(sz)]
         L.or_mkthresholdlist.restype = sz
         L.or_wannierldoslist.argtypes = [sz, sz, sz, sz, sz, _dp, _dp, C.c_void_p, C.POINTER(sz), _dp]
+        L.or_spf.argtypes = [sz, sz, sz, sz, _dp, _dp]
+        L.or_uspf.argtypes = [sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _dp]
         L.or_cldos_run.argtypes = [C.c_void_p, sz, sz, C.c_double, C.c_int, sz, sz, sz, sz, sz,
                                    C.c_int, C.c_double, _dp, sz, _dp, _dp, _ip, _dp]
         L.or_cldos_run.restype = C.c_int
@@ -206,6 +208,32 @@ def wannierldoslist(g0R, T, flags, counts, n, window, oversamp):
                              _d(_c(g0R, np.complex128).view(np.float64)), flags.ctypes.data,
                              counts.ctypes.data_as(C.POINTER(C.c_size_t)), _d(out))
     return out
+
+
+def spf(g0k, n, maxband=0):
+    """Spectral function map (src/qpihipkernel.cc:317-329): n x n, x + y*n."""
+    K, _, b, _ = g0k.shape
+    out = np.zeros(n * n)
+    lib().or_spf(K, n, b, maxband, _d(_c(g0k, np.complex128).view(np.float64)), _d(out))
+    return out
+
+
+def uspf(g0k, n, pos, prearr, maxband=0, spin=False):
+    """Unfolded spectral function map (src/qpi.cc:731-775): pos maxband x 2."""
+    K, _, b, _ = g0k.shape
+    mb = maxband or b
+    out = np.zeros(n * n)
+    lib().or_uspf(K, n, b, mb, int(spin), _d(_c(np.asarray(pos)[:mb, :2], np.float64)),
+                  _d(_c(np.asarray(prearr)[:mb], np.float64)), _d(_c(g0k, np.complex128).view(np.float64)),
+                  _d(out))
+    return out
+
+
+def dos(g0R, maxband=0):
+    """tmatrix::calcdos (src/qpi.cc:856-862): -sum_{i<maxband} Im G0(R=0)_ii."""
+    b = g0R.shape[2]
+    mb = maxband or b
+    return -float(np.sum(np.imag(np.diag(g0R[0, 0])[:mb])))
 
 
 def cldos_run(wl, energies=None):

@@ -277,3 +277,66 @@ def test_threshold_list_contraction_is_linear_in_the_list():
     split = sum(oracle.wannierldoslist(gR, T, np.concatenate(f), np.array(c, np.uintp), 6, 1, 2)
                 for f, c in parts)
     assert rm.rel_errors(split, whole)[0] < 1e-12
+
+
+def test_spf_one_band_closed_form_and_centering():
+    """One band: A(k) = eta / ((w - e(k))^2 + eta^2), shown with k = 0 at
+    pixel (n/2, n/2) (the map is the k grid rolled by n/2)."""
+    wl = synth.square_1orb(K=16, n=16, E=1)
+    om, eta = 0.3, 0.05
+    ev, V = oracle.calchamiltonian(wl.hops, 1, wl.efermi, 16)
+    g0k = oracle.greens_k(ev, V, om, eta)
+    got = oracle.spf(g0k, 16).reshape(16, 16).T        # [x, y]
+    k = synth.kgrid(16)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    e = synth.hk_numpy(wl.hops, 1, wl.efermi, kx, ky)[..., 0, 0].real
+    A = eta / ((om - e) ** 2 + eta ** 2)
+    assert np.allclose(got, np.roll(np.roll(A, 8, 0), 8, 1), rtol=1e-12, atol=0)
+    assert abs(got[8, 8] - A[0, 0]) < 1e-12 * A[0, 0]
+
+
+def test_dos_is_the_mean_spectral_weight():
+    """G0(R=0) is the k average of G0(k): the DOS equals the mean of the SPF map
+    over the whole grid (n = K)."""
+    wl = synth.square_norb(3, K=16, n=16, E=1, seed=9)
+    ev, V = oracle.calchamiltonian(wl.hops, 3, wl.efermi, 16)
+    g0k = oracle.greens_k(ev, V, 0.1, 0.02)
+    for mb in (3, 2):
+        assert abs(oracle.dos(oracle.realspace(g0k), mb) - oracle.spf(g0k, 16, mb).mean()) < 1e-12
+
+
+def test_uspf_one_band_at_the_origin_is_the_spf():
+    wl = synth.square_1orb(K=16, n=16, E=1)
+    ev, V = oracle.calchamiltonian(wl.hops, 1, wl.efermi, 16)
+    g0k = oracle.greens_k(ev, V, -0.2, 0.03)
+    u = oracle.uspf(g0k, 16, np.zeros((1, 2)), np.ones(1))
+    assert np.allclose(u, oracle.spf(g0k, 16), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("spin", [False, True])
+def test_uspf_position_phase_is_a_gauge_shift(spin):
+    """Orbital positions x_o enter as e^{2 pi i k.(x_k - x_l)}: the map of model A
+    with positions x equals the map at zero positions of model B whose hops
+    (o1, o2, R) move to R + x_o2 - x_o1 (H_B = U^H H_A U, U = diag e^{2 pi i k.x}).
+    Pixels on the zone edge (index 0) pick k = -1/2 in the phase but +1/2 on
+    the grid, so they are left out."""
+    b = 4 if spin else 2
+    rng = np.random.default_rng(5 + b)
+    onsite = rng.normal(0, 0.3, (b, b))
+    onsite = 0.5 * (onsite + onsite.T)
+    blocks = synth.hermitian_blocks(synth.square_shells(2), b, rng,
+                                    lambda sh: rng.normal(0, 0.3, sh) + 1j * rng.normal(0, 0.1, sh),
+                                    onsite.astype(np.complex128))
+    hA = synth.hops_from_blocks(blocks, b)
+    pos = rng.uniform(-0.5, 0.5, (b, 2))
+    z = rng.uniform(0.5, 1.5, b)
+    hB = hA.copy()
+    hB["rx"] += pos[hA["o2"], 0] - pos[hA["o1"], 0]
+    hB["ry"] += pos[hA["o2"], 1] - pos[hA["o1"], 1]
+    K = 16
+    gA = oracle.greens_k(*oracle.calchamiltonian(hA, b, 0.05, K), 0.1, 0.02)
+    gB = oracle.greens_k(*oracle.calchamiltonian(hB, b, 0.05, K), 0.1, 0.02)
+    uA = oracle.uspf(gA, K, pos, z, spin=spin).reshape(K, K)
+    uB = oracle.uspf(gB, K, np.zeros((b, 2)), z, spin=spin).reshape(K, K)
+    assert np.allclose(uA[1:, 1:], uB[1:, 1:], rtol=1e-11, atol=1e-12 * np.abs(uA).max())
+    assert not np.allclose(uA, oracle.uspf(gA, K, np.zeros((b, 2)), z, spin=spin).reshape(K, K))
