@@ -20,7 +20,12 @@
 //   * output=josephson (scmodel models only) with deltat and etat, the tip
 //     factor of src/calcqpi.cc:705-709, header name of src/calcqpi.cc:643-645;
 //   * threshold=t builds the threshold list (src/calcqpi.cc:620-634) and
-//     computes the wannier map from it (src/calcqpi.cc:699-702).
+//     computes the wannier map from it (src/calcqpi.cc:699-702);
+//   * output=spf|uspf: spectral-function maps n x n (spf sets kpoints = n,
+//     src/calcqpi.cc:333-334; uspf takes pos[i] and prearr, :464-482),
+//     header names of :647-652; output=nomode makes no map;
+//   * dosfile, doslayers, dosenergies: the density of states as text lines
+//     "energy dos" (src/calcqpi.cc:336-391).
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
 // Other output and green modes, spinarr, and analytic or XSF orbitals are
 // reported as unsupported.
@@ -218,7 +223,7 @@ int main(int argc, char **argv)
         return 1;
     }
     std::ofstream logfile;
-    std::ostream *log = &std::cout;
+    std::ostream *log = print_only ? &std::cerr : &std::cout;  // --print-config keeps stdout for the JSON
     if (lf.probe("logfile")) {
         logfile.open(lf.str("logfile"), std::ios::app);
         if (logfile.is_open()) log = &logfile;
@@ -229,6 +234,8 @@ int main(int argc, char **argv)
     double lenergy = -1.0, henergy = 1.0, eta = 0.005, efermi = 0.0, magscat = 0.0;
     double threshold = NAN, deltat = 0.001, etat = NAN;
     bool josephson = false;
+    int spectral = 0;     // 1: spf, 2: uspf
+    bool nomap = false;   // output=nomode
     std::complex<double> scatphase(1.0, 0.0);
     vector<std::complex<double>> scattering;
     string tbfile, qpifile, precision = "fp64";
@@ -248,8 +255,12 @@ int main(int argc, char **argv)
     if (lf.probe("output")) {
         const string mode = lf.get("output");
         if (mode == "josephson") josephson = true;
+        else if (mode == "spf") spectral = 1;
+        else if (mode == "uspf") spectral = 2;
+        else if (mode == "nomode") nomap = true;
         else if (mode != "wannier") {
-            std::cerr << "output=" << mode << " is not supported by this build (wannier, josephson).\n";
+            std::cerr << "output=" << mode << " is not supported by this build (wannier, josephson, spf, uspf, "
+                      << "nomode).\n";
             return 1;
         }
     }
@@ -281,6 +292,22 @@ int main(int argc, char **argv)
     if (std::isnan(etat)) etat = eta;  // src/calcqpi.cc:249-250
     if (lf.probe("kpoints")) kpoints = lf.integer("kpoints");
     if (!kpoints) kpoints = n;
+    if (spectral == 1) kpoints = n;  // src/calcqpi.cc:333-334
+    // density of states: src/calcqpi.cc:336-391
+    string dosfile;
+    long doslayers = 101;
+    double doslenergy = -0.1, doshenergy = 0.1;
+    if (lf.probe("dosfile")) {
+        dosfile = lf.str("dosfile");
+        if (lf.probe("doslayers")) doslayers = lf.integer("doslayers");
+        if (lf.probe("dosenergies")) {
+            vector<double> en = lf.vec("dosenergies");
+            if (en.size() >= 2) {
+                doslenergy = en[0];
+                doshenergy = en[1];
+            }
+        }
+    }
     if (lf.probe("phase")) {
         vector<double> ph = lf.vec("phase");
         if (ph.size() >= 2) scatphase = std::complex<double>(ph[0], ph[1]);
@@ -313,10 +340,26 @@ int main(int argc, char **argv)
     if (!scattering.empty())
         for (size_t i = 0; i < bands; i++) vscat[i] = i < scattering.size() ? scattering[i] : 0.0;
 
+    // unfolded spectral function: orbital positions and prefactors (src/calcqpi.cc:464-482)
+    vector<double> posxy(2 * maxbands, 0.0), sprearr;
+    if (spectral == 2) {
+        for (size_t i = 0; i < maxbands; i++) {
+            const string key = "pos[" + std::to_string(i) + "]";
+            if (lf.probe(key)) {
+                vector<double> p = lf.vec(key);
+                for (size_t c = 0; c < 2 && c < p.size(); c++) posxy[2 * i + c] = p[c];
+            } else
+                *log << "No position found for band " << i << ", using (0,0,0).\n";
+        }
+        if (lf.probe("prearr")) sprearr = lf.vec("prearr");
+        sprearr.resize(maxbands, 1.0);
+    }
     // orbitals from the idl orbital file: src/calcqpi.cc:483-518
     const size_t T = (size_t)(2 * window + 1) * oversamp;
     vector<double> tiles(maxbands * T * T, 0.0);
-    if (!lf.probe("idlorbitalfile")) {
+    if (spectral || nomap) {
+        // the spectral maps and the DOS use no Wannier tiles
+    } else if (!lf.probe("idlorbitalfile")) {
         std::cerr << "This build takes the Wannier tiles from an idlorbitalfile only.\n";
         return 1;
     }
@@ -353,7 +396,7 @@ int main(int argc, char **argv)
     vector<double> omegas(layers);
     for (long k = 0; k < layers; k++)
         omegas[k] = layers == 1 ? lenergy : (henergy - lenergy) * (double)k / ((double)layers - 1.0) + lenergy;
-    const long W = oversamp * n;
+    const long W = spectral || nomap ? n : oversamp * n;
 
     if (print_only) {
         std::cout.precision(17);
@@ -364,10 +407,17 @@ int main(int argc, char **argv)
                   << ", \"layers\": " << layers << ", \"energies\": [" << lenergy << ", " << henergy
                   << "], \"eta\": " << eta << ", \"fermi\": " << efermi << ", \"spin\": " << (spin ? "true" : "false")
                   << ", \"scmodel\": " << (scmodel ? "true" : "false") << ", \"precision\": \"" << precision
-                  << "\", \"output\": \"" << (josephson ? "josephson" : "wannier") << "\", \"threshold\": ";
+                  << "\", \"output\": \""
+                  << (josephson ? "josephson" : spectral == 1 ? "spf" : spectral == 2 ? "uspf" : nomap ? "nomode" : "wannier")
+                  << "\", \"threshold\": ";
         if (std::isnan(threshold)) std::cout << "null";
         else std::cout << threshold;
-        std::cout << ", \"deltat\": " << deltat << ", \"etat\": " << etat << ", \"vscat\": [";
+        std::cout << ", \"deltat\": " << deltat << ", \"etat\": " << etat << ", \"spectral\": " << spectral
+                  << ", \"dosfile\": \"" << json_escape(dosfile) << "\", \"doslayers\": " << doslayers
+                  << ", \"dosenergies\": [" << doslenergy << ", " << doshenergy << "], \"pos\": [";
+        for (size_t i = 0; i < maxbands; i++)
+            std::cout << (i ? ", " : "") << "[" << posxy[2 * i] << ", " << posxy[2 * i + 1] << "]";
+        std::cout << "], \"vscat\": [";
         for (size_t i = 0; i < vscat.size(); i++)
             std::cout << (i ? ", " : "") << "[" << vscat[i].real() << ", " << vscat[i].imag() << "]";
         double tsum = 0;
@@ -388,7 +438,13 @@ int main(int argc, char **argv)
 
     auto t0 = std::chrono::steady_clock::now();
     cqpi_ctx *ctx = nullptr;
-    if (!std::isnan(threshold) && !josephson) {
+    if (spectral || nomap) {
+        // spectral maps (and the DOS) from a spectral-function context
+        if (check(cqpi_create_spectral((size_t)kpoints, (size_t)n, bands, maxbands, spin, posxy.data(), maxbands,
+                                       spectral == 2 ? sprearr.data() : nullptr, spectral == 2 ? maxbands : 0, &ctx),
+                  "cqpi_create_spectral"))
+            return 1;
+    } else if (!std::isnan(threshold) && !josephson) {
         // threshold list over the maxbands tiles (src/calcqpi.cc:620-634)
         const size_t wins = (size_t)(2 * window + 1);
         vector<size_t> counts(wins * wins * wins * wins);
@@ -422,11 +478,35 @@ int main(int argc, char **argv)
     }
     int rc = check(cqpi_set_model(ctx, hops, nhops, efermi, scmodel), "cqpi_set_model");
     cqpi_free(hops);
-    if (!rc) rc = check(cqpi_set_impurity(ctx, V.data()), "cqpi_set_impurity");
-    if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed"  ? CQPI_PREC_MIXED
-                                                    : precision == "fp32" ? CQPI_PREC_FP32
-                                                                          : CQPI_PREC_FP64), "cqpi_set_precision");
-    if (!rc && josephson) rc = check(cqpi_set_josephson(ctx, 1, deltat, etat), "cqpi_set_josephson");
+    if (!rc && !dosfile.empty()) {
+        // energies lb + k (hb - lb)/layers, k < layers (src/calcqpi.cc:348-356)
+        const double step = (doshenergy - doslenergy) / (double)doslayers;
+        vector<double> dosw(doslayers), dos(doslayers);
+        for (long k = 0; k < doslayers; k++) dosw[k] = doslenergy + (double)k * step;
+        *log << "Calculating density of states in energy range (" << doslenergy << "," << doshenergy << ") with "
+             << doslayers << " layers.\n";
+        rc = check(cqpi_dos(ctx, dosw.data(), dosw.size(), eta, dos.data()), "cqpi_dos");
+        if (!rc) {
+            std::ofstream dosout(dosfile);
+            for (long k = 0; k < doslayers; k++) dosout << dosw[k] << " " << dos[k] << "\n";
+            *log << "... done and saved to " << dosfile << ".\n";
+        }
+    }
+    if (nomap) {
+        cqpi_destroy(ctx);
+        if (rc) return 1;
+        lf.warn_unused(std::cerr);
+        return 0;
+    }
+    if (spectral) {
+        if (!rc) rc = check(cqpi_set_spectral(ctx, spectral), "cqpi_set_spectral");
+    } else {
+        if (!rc) rc = check(cqpi_set_impurity(ctx, V.data()), "cqpi_set_impurity");
+        if (!rc) rc = check(cqpi_set_precision(ctx, precision == "mixed"  ? CQPI_PREC_MIXED
+                                                        : precision == "fp32" ? CQPI_PREC_FP32
+                                                                              : CQPI_PREC_FP64), "cqpi_set_precision");
+        if (!rc && josephson) rc = check(cqpi_set_josephson(ctx, 1, deltat, etat), "cqpi_set_josephson");
+    }
     IdlMap out;
     out.data.assign((size_t)W * W * layers, 0.0);
     if (!rc) rc = check(cqpi_run(ctx, omegas.data(), omegas.size(), eta, out.data.data()), "cqpi_run");
@@ -434,9 +514,13 @@ int main(int argc, char **argv)
     if (rc) return 1;
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
+    // src/calcqpi.cc:639-652
     std::ostringstream comment;
-    comment << (josephson ? "Josephson calculation for " : "CQPI calculation for ") << tbname
-            << " with window=" << window << " and oversamp=" << oversamp;
+    if (spectral == 1) comment << "SPF calculation for " << tbname;
+    else if (spectral == 2) comment << "USPF calculation for " << tbname << " with kpoints=" << kpoints;
+    else
+        comment << (josephson ? "Josephson calculation for " : "CQPI calculation for ") << tbname
+                << " with window=" << window << " and oversamp=" << oversamp;
     out.name = comment.str();
     time_t now = time(nullptr);
     out.date = ctime(&now);

@@ -162,3 +162,52 @@ def test_driver_threshold_matches_oracle(tmp_path, gpu):
         ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, wl.window, wl.oversamp)
         got = data[:, :, e].T.ravel()
         assert rm.rel_errors(got, ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+def test_driver_spectral_and_dos_keys(tmp_path):
+    wl = synth.square_3orb(K=16, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path / "s"), wl, output="spf", extra={"dosfile": '"dos.dat"', "doslayers": "7"})
+    cfg, err = _print_config(p)
+    assert cfg["spectral"] == 1 and cfg["kpoints"] == 8   # spf: kpoints = lattice
+    assert cfg["dosfile"] == "dos.dat" and cfg["doslayers"] == 7 and cfg["dosenergies"] == [-0.1, 0.1]
+    p = synth.write_inputs(str(tmp_path / "u"), wl, output="uspf",
+                           extra={"pos[0]": "(0.0,0.0)", "pos[1]": "(0.5,0.25,1.0)", "prearr": "(1.0,0.5,2.0)"})
+    cfg, err = _print_config(p)
+    assert cfg["spectral"] == 2 and cfg["kpoints"] == 16
+    assert cfg["pos"] == [[0.0, 0.0], [0.5, 0.25], [0.0, 0.0]]
+    assert "Unused" not in err
+
+
+@pytest.mark.gpu
+def test_driver_spectral_and_dos_match_oracle(tmp_path, gpu):
+    wl = synth.square_3orb(K=16, n=16, E=3)
+    ev, vec = oracle.calchamiltonian(wl.hops, 3, wl.efermi, 16)
+    # SPF, with a DOS file
+    p = synth.write_inputs(str(tmp_path / "s"), wl, output="spf",
+                           extra={"dosfile": '"dos.dat"', "doslayers": "5", "dosenergies": "(-0.2,0.2)"})
+    print("case: driver run, spf and dos, K=n=16, E=3", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path / "s"))
+    assert r.returncode == 0, r.stderr + r.stdout
+    data, hdr = synth.read_idl(str(tmp_path / "s" / "qpi.idl"))
+    assert data.shape == (16, 16, 3) and hdr["name"].startswith("SPF calculation for ")
+    for e, om in enumerate(wl.energies):
+        ref = oracle.spf(oracle.greens_k(ev, vec, om, wl.eta), 16)
+        assert rm.rel_errors(data[:, :, e].T.ravel(), ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+    rows = np.loadtxt(str(tmp_path / "s" / "dos.dat"))
+    assert rows.shape == (5, 2) and np.allclose(rows[:, 0], -0.2 + 0.08 * np.arange(5))
+    for w, d in rows:
+        ref = oracle.dos(oracle.realspace(oracle.greens_k(ev, vec, w, wl.eta)))
+        assert abs(d - ref) < 1e-5 * abs(ref)   # the file holds 6 significant digits
+    # USPF with orbital positions and prefactors
+    pos = np.array([[0.0, 0.0], [0.5, 0.25], [-0.25, 0.5]])
+    pre = np.array([1.0, 0.5, 2.0])
+    extra = {"pos[%d]" % i: "(%g,%g)" % tuple(pos[i]) for i in range(3)}
+    extra["prearr"] = "(1.0,0.5,2.0)"
+    p = synth.write_inputs(str(tmp_path / "u"), wl, output="uspf", extra=extra)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path / "u"))
+    assert r.returncode == 0, r.stderr + r.stdout
+    data, hdr = synth.read_idl(str(tmp_path / "u" / "qpi.idl"))
+    assert hdr["name"] == "USPF calculation for %s with kpoints=16" % hdr["name"].split(" for ")[1].split(" with")[0]
+    for e, om in enumerate(wl.energies):
+        ref = oracle.uspf(oracle.greens_k(ev, vec, om, wl.eta), 16, pos, pre)
+        assert rm.rel_errors(data[:, :, e].T.ravel(), ref.astype(np.float32).astype(np.float64))[0] < 1e-6
