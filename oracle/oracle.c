@@ -714,6 +714,134 @@ void or_uspf(size_t K, size_t n, size_t b, size_t maxband, int spin, const doubl
 }
 
 /* ------------------------------------------------------------------------- */
+/* Surface / bulk Green's function by decimation:                             */
+/* tmatrix::calcsurfacegreensfunction src/qpi.cc:488-554 with                 */
+/* tightbind::setmatrix(k, H, kzdep) src/tightbinding.cc:184-208 (in-plane:   */
+/* hops with rz == 0 plus the -/+ efermi diagonal; out-of-plane: hops with    */
+/* rz < 0, no diagonal), matnorm src/qpi.cc:55-61 (Frobenius norm) and        */
+/* calcgreensfunctioninv src/tightbinding.h:75-84 ((w + i eta - H)^-1).       */
+/* shops (may be NULL): the surface layer's model for eps_s.  Writes G0(k) of */
+/* the K x K grid; returns the largest iteration count used (or -1 if maxit   */
+/* was reached somewhere).                                                    */
+/* ------------------------------------------------------------------------- */
+static void or_setmatrix_kz(const or_hop *hops, size_t nh, size_t b, double efermi, size_t scmax, double kx,
+                            double ky, int outofplane, cplx *H)
+{
+    for (size_t i = 0; i < b * b; i++) H[i] = 0;
+    if (!outofplane)
+        for (size_t i = 0; i < b; i++) H[i * b + i] = (i < scmax) ? -efermi : efermi;
+    for (size_t m = 0; m < nh; m++) {
+        const or_hop *h = &hops[m];
+        if (outofplane ? !(h->rz < 0.0) : !(h->rz == 0.0)) continue;
+        double arg = 2.0 * M_PI * (h->rx * kx + h->ry * ky);
+        if (h->anti) arg = -arg;
+        H[h->o1 * b + h->o2] += (h->tre + I * h->tim) * (cos(arg) + I * sin(arg));
+    }
+}
+
+/* (w + i eta - H)^-1 by Gauss-Jordan with partial pivoting */
+static void or_resinv(size_t b, const cplx *H, double omega, double eta, cplx *G)
+{
+    cplx *M = (cplx *)malloc(b * b * sizeof(cplx));
+    for (size_t r = 0; r < b; r++)
+        for (size_t c = 0; c < b; c++) {
+            M[r * b + c] = (r == c ? omega + I * eta : 0) - H[r * b + c];
+            G[r * b + c] = r == c ? 1 : 0;
+        }
+    for (size_t j = 0; j < b; j++) {
+        size_t piv = j;
+        for (size_t i = j + 1; i < b; i++)
+            if (cabs(M[i * b + j]) > cabs(M[piv * b + j])) piv = i;
+        if (piv != j)
+            for (size_t c = 0; c < b; c++) {
+                cplx t = M[j * b + c]; M[j * b + c] = M[piv * b + c]; M[piv * b + c] = t;
+                t = G[j * b + c]; G[j * b + c] = G[piv * b + c]; G[piv * b + c] = t;
+            }
+        cplx ip = 1.0 / M[j * b + j];
+        for (size_t c = 0; c < b; c++) {
+            M[j * b + c] *= ip;
+            G[j * b + c] *= ip;
+        }
+        for (size_t i = 0; i < b; i++) {
+            if (i == j) continue;
+            cplx l = M[i * b + j];
+            for (size_t c = 0; c < b; c++) {
+                M[i * b + c] -= l * M[j * b + c];
+                G[i * b + c] -= l * G[j * b + c];
+            }
+        }
+    }
+    free(M);
+}
+
+int or_surfacegreen(const or_hop *hops, size_t nh, const or_hop *shops, size_t nsh, size_t b, double efermi,
+                    size_t scmax, size_t K, double omega, double eta, double epserr, int surface, int maxit,
+                    double *g0k_out)
+{
+    cplx *g0 = (cplx *)g0k_out;
+    size_t bb = b * b;
+    int worst = 0;
+#pragma omp parallel
+    {
+        cplx *es = (cplx *)malloc(bb * sizeof(cplx)), *eb = (cplx *)malloc(bb * sizeof(cplx)),
+             *al = (cplx *)malloc(bb * sizeof(cplx)), *be = (cplx *)malloc(bb * sizeof(cplx)),
+             *g = (cplx *)malloc(bb * sizeof(cplx)), *t1 = (cplx *)malloc(bb * sizeof(cplx)),
+             *t2 = (cplx *)malloc(bb * sizeof(cplx));
+        int local = 0;
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < K; i++)
+            for (size_t j = 0; j < K; j++) {
+                double kx = or_kcoord(i, K), ky = or_kcoord(j, K);
+                if (shops) or_setmatrix_kz(shops, nsh, b, efermi, scmax, kx, ky, 0, es);
+                else or_setmatrix_kz(hops, nh, b, efermi, scmax, kx, ky, 0, es);
+                or_setmatrix_kz(hops, nh, b, efermi, scmax, kx, ky, 0, eb);
+                or_setmatrix_kz(hops, nh, b, efermi, scmax, kx, ky, 1, al);
+                for (size_t r = 0; r < b; r++)
+                    for (size_t c = 0; c < b; c++) be[r * b + c] = conj(al[c * b + r]);
+                int it = 0;
+                for (;;) {
+                    or_resinv(b, eb, omega, eta, g);
+                    zgemm_small(b, g, be, t1);           /* g beta */
+                    zgemm_small(b, al, t1, t2);          /* alpha g beta */
+                    for (size_t m = 0; m < bb; m++) {
+                        es[m] += t2[m];
+                        eb[m] += t2[m];
+                    }
+                    zgemm_small(b, g, al, t1);           /* g alpha */
+                    zgemm_small(b, be, t1, t2);          /* beta g alpha */
+                    for (size_t m = 0; m < bb; m++) eb[m] += t2[m];
+                    zgemm_small(b, al, t1, t2);          /* alpha g alpha */
+                    memcpy(al, t2, bb * sizeof(cplx));
+                    zgemm_small(b, g, be, t1);
+                    zgemm_small(b, be, t1, t2);          /* beta g beta */
+                    memcpy(be, t2, bb * sizeof(cplx));
+                    it++;
+                    double na = 0, nb2 = 0;
+                    for (size_t m = 0; m < bb; m++) {
+                        na += creal(al[m]) * creal(al[m]) + cimag(al[m]) * cimag(al[m]);
+                        nb2 += creal(be[m]) * creal(be[m]) + cimag(be[m]) * cimag(be[m]);
+                    }
+                    if (!(sqrt(na) > epserr || sqrt(nb2) > epserr)) break;
+                    if (it >= maxit) {
+                        it = -1;
+                        break;
+                    }
+                }
+                or_resinv(b, surface ? es : eb, omega, eta, g0 + (i * K + j) * bb);
+                if (it < 0) local = -1;
+                else if (local >= 0 && it > local) local = it;
+            }
+#pragma omp critical
+        {
+            if (local < 0 || worst < 0) worst = -1;
+            else if (local > worst) worst = local;
+        }
+        free(es); free(eb); free(al); free(be); free(g); free(t1); free(t2);
+    }
+    return worst;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Whole path for a list of energies (the energy loop of src/calcqpi.cc:671-760 */
 /* with green=normal and output=wannier).  ldos_out: E x W x W, x + y*W.       */
 /* Returns nonzero if a T-matrix inversion failed.                            */

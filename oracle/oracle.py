@@ -50,6 +50,9 @@ def lib():
         L.or_mkthresholdlist.restype = sz
         L.or_wannierldoslist.argtypes = [sz, sz, sz, sz, sz, _dp, _dp, C.c_void_p, C.POINTER(sz), _dp]
         L.or_spf.argtypes = [sz, sz, sz, sz, _dp, _dp]
+        L.or_surfacegreen.argtypes = [C.c_void_p, sz, C.c_void_p, sz, sz, C.c_double, sz, sz, C.c_double,
+                                      C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+        L.or_surfacegreen.restype = C.c_int
         L.or_uspf.argtypes = [sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _dp]
         L.or_cldos_run.argtypes = [C.c_void_p, sz, sz, C.c_double, C.c_int, sz, sz, sz, sz, sz,
                                    C.c_int, C.c_double, _dp, sz, _dp, _dp, _ip, _dp]
@@ -208,6 +211,21 @@ def wannierldoslist(g0R, T, flags, counts, n, window, oversamp):
                              _d(_c(g0R, np.complex128).view(np.float64)), flags.ctypes.data,
                              counts.ctypes.data_as(C.POINTER(C.c_size_t)), _d(out))
     return out
+
+
+def surfacegreen(hops, b, efermi, K, omega, eta, epserr=1e-5, surface=True, shops=None, scmodel=False,
+                 maxit=500):
+    """src/qpi.cc:488-554: G0(k) (K, K, b, b) of the surface (or bulk) of a
+    layered model (in-plane hops rz == 0, coupling to the next layer rz < 0),
+    and the largest iteration count (-1 if maxit was reached)."""
+    hops = _c(hops, HOP_DTYPE)
+    sh = None if shops is None else _c(shops, HOP_DTYPE)
+    g = np.zeros((K, K, b, b), np.complex128)
+    scmax = b >> 1 if scmodel else b
+    it = lib().or_surfacegreen(hops.ctypes.data, len(hops), None if sh is None else sh.ctypes.data,
+                               0 if sh is None else len(sh), b, efermi, scmax, K, omega, eta, epserr,
+                               int(surface), maxit, _d(g.view(np.float64)))
+    return g, it
 
 
 def spf(g0k, n, maxband=0):

@@ -340,3 +340,47 @@ def test_uspf_position_phase_is_a_gauge_shift(spin):
     uB = oracle.uspf(gB, K, np.zeros((b, 2)), z, spin=spin).reshape(K, K)
     assert np.allclose(uA[1:, 1:], uB[1:, 1:], rtol=1e-11, atol=1e-12 * np.abs(uA).max())
     assert not np.allclose(uA, oracle.uspf(gA, K, np.zeros((b, 2)), z, spin=spin).reshape(K, K))
+
+
+def _layered_1band(t_perp, rzsign=-1):
+    """Square-lattice one-band layers (rz = 0) coupled along z by t_perp at
+    R = (0, 0, -1) and its conjugate at (0, 0, +1)."""
+    wl = synth.square_1orb(K=8, n=8, E=1)
+    h = list(wl.hops)
+    extra = np.zeros(2, synth.HOP_DTYPE)
+    extra["rz"] = [rzsign, -rzsign]
+    extra["tre"] = [np.real(t_perp), np.real(t_perp)]
+    extra["tim"] = [np.imag(t_perp), -np.imag(t_perp)]
+    return wl, np.concatenate([np.array(h, synth.HOP_DTYPE), extra])
+
+
+def test_surface_green_without_coupling_is_the_plain_resolvent():
+    wl = synth.square_norb(3, K=8, n=8, E=1, seed=12)
+    for surface in (True, False):
+        g, it = oracle.surfacegreen(wl.hops, 3, 0.05, 8, 0.1, 0.03, surface=surface)
+        ref = oracle.greens_k(*oracle.calchamiltonian(wl.hops, 3, 0.05, 8), 0.1, 0.03)
+        assert it == 1 and np.allclose(g, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("surface", [True, False])
+def test_surface_green_semi_infinite_chain_closed_form(surface):
+    """One band per layer, interlayer hopping t: per k the layers form a
+    semi-infinite chain with on-site e(k), whose surface and bulk Green's
+    functions are (z-e -/+ sqrt((z-e)^2 - 4|t|^2)) / (2|t|^2) and
+    1/sqrt((z-e)^2 - 4|t|^2) on the retarded branch (Im g < 0)."""
+    t = 0.4 * np.exp(0.3j)
+    wl, hops = _layered_1band(t)
+    om, eta = 0.2, 0.05
+    g, it = oracle.surfacegreen(hops, 1, wl.efermi, 8, om, eta, epserr=1e-14, surface=surface)
+    assert it > 1
+    k = synth.kgrid(8)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    e = synth.hk_numpy(wl.hops, 1, wl.efermi, kx, ky)[..., 0, 0]
+    z = om + 1j * eta - e
+    root = np.sqrt(z * z - 4 * abs(t) ** 2)
+    if surface:
+        cands = [(z - root) / (2 * abs(t) ** 2), (z + root) / (2 * abs(t) ** 2)]
+    else:
+        cands = [1 / root, -1 / root]
+    ref = np.where(np.imag(cands[0]) < 0, cands[0], cands[1])
+    assert np.allclose(g[..., 0, 0], ref, rtol=1e-10, atol=0)
