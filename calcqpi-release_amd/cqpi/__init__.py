@@ -26,6 +26,8 @@ _LIB = None
 
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED}
+# cqpi_set_surface modes (green= of the reference driver)
+GREENS = {"normal": 0, "surface": 1, "bulk": 2}
 
 # cqpi_flagentry: {i, j, o1, o2 (uint32), factor (double)}, 24 bytes
 FLAG_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("o1", "<u4"), ("o2", "<u4"), ("factor", "<f8")])
@@ -44,6 +46,7 @@ SIGNATURES = {
     "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
+    "cqpi_set_surface": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _sz]),
     "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
     "cqpi_create_spectral": (C.c_int, [_sz, _sz, _sz, _sz, C.c_int, _dp, _sz, _dp, _sz, C.POINTER(_vp)]),
     "cqpi_spf": (C.c_int, [_vp, C.POINTER(_vp), _sz]),
@@ -306,6 +309,15 @@ class AmdQPI:
         """cqpi_run computes the Josephson map (tip gap deltat, tip broadening
         etat; NaN: the run's eta) instead of the LDOS map."""
         _check(lib().cqpi_set_josephson(self._ctx, int(bool(on)), float(deltat), float(etat)))
+
+    def set_surface(self, mode="surface", epserr=1e-5, shops=None):
+        """cqpi_run uses the surface ("surface") or bulk ("bulk") Green's
+        function of the layered model, or G0(k) of the eigen data ("normal");
+        shops: in-plane hoppings of the surface layer (default: the model's)."""
+        mode = GREENS.get(mode, mode)
+        sh = np.ascontiguousarray(shops if shops is not None else np.zeros(0, synth.HOP_DTYPE), dtype=synth.HOP_DTYPE)
+        _check(lib().cqpi_set_surface(self._ctx, int(mode), float(epserr), sh.ctypes.data if len(sh) else None,
+                                      len(sh)))
 
     def set_precision(self, mode):
         mode = PRECISIONS.get(mode, mode)

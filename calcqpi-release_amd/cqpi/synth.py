@@ -25,6 +25,7 @@ max |w| = 1; the 5-orbital d-set is (xz, yz, xy, x^2-y^2, 3z^2-r^2).
 from __future__ import annotations
 
 import math
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -324,6 +325,26 @@ def hex_5d_spin(K=1024, n=256, E=101, seed=1003, lam=0.05) -> Workload:
 
 ORB_SETS = {1: ["s"], 3: ["xz", "yz", "xy"], 5: ["xz", "yz", "xy", "x2y2", "z2"],
             6: ["s", "xz", "yz", "xy", "x2y2", "z2"], 7: ["s", "xz", "yz", "xy", "x2y2", "z2", "s"]}
+
+
+def layered(b, K, n, E, seed, tz=0.12, **kw):
+    """square_norb's model as one layer (its terms at rz = 0) of a stack along
+    z: coupling blocks to the next layer at R = (0, 0, -1) and (1, 0, -1)
+    (random, scale tz) and their conjugates at rz = +1.  Returns the Workload
+    with the stacked hoppings."""
+    wl = square_norb(b, K=K, n=n, E=E, seed=seed, **kw)
+    rng = np.random.default_rng(seed + 1000)
+    extra = []
+    for rx in (0.0, 1.0):
+        M = tz * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b))) / (2.0 if rx else 1.0)
+        for o1 in range(b):
+            for o2 in range(b):
+                # <o1, R| H |o2, 0> = M[o1, o2] at R = (rx, 0, -1); the
+                # hermitian partner conj(M[o1, o2]) at (-rx, 0, +1) as (o2, o1)
+                extra.append((rx, 0.0, -1.0, o1, o2, M[o1, o2].real, M[o1, o2].imag, 0, 0))
+                extra.append((-rx, 0.0, 1.0, o2, o1, M[o1, o2].real, -M[o1, o2].imag, 0, 0))
+    hops = np.concatenate([np.asarray(wl.hops, HOP_DTYPE), np.array(extra, HOP_DTYPE)])
+    return dataclasses.replace(wl, name="layered %d-orbital" % b, hops=hops)
 
 
 def square_norb(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=None):

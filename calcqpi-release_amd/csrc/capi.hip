@@ -151,6 +151,21 @@ struct cqpi_ctx {
     double *d_lcoef = nullptr;
     int *d_lr1off = nullptr, *d_lpair = nullptr;
     int lnact = 0;
+    // surface / bulk Green's function of a layered model (cqpi_set_surface):
+    // 0 off (G0(k) from the eigen data), 1 surface layer, 2 bulk.  Hopping
+    // groups of the bulk in-plane, surface in-plane and coupling blocks;
+    // d_sstat: decimation steps per energy (surf_maxit + 1: not converged)
+    int surf_mode = 0;
+    double surf_eps = 1e-5;
+    int surf_maxit = 500;
+    std::vector<cqpi_hopping> h_shops;
+    DevRGroup *d_sgroups = nullptr;
+    double2 *d_stmat = nullptr;
+    size_t c_sgroups = 0, c_stmat = 0;
+    int sg_n[3] = {0, 0, 0};
+    size_t sg_goff[3] = {0, 0, 0}, sg_toff[3] = {0, 0, 0};
+    int *d_sstat = nullptr;
+    size_t c_sstat = 0;
     int *d_status = nullptr;
     size_t c_status = 0;
     double *d_omega = nullptr;
@@ -227,7 +242,7 @@ void release(cqpi_ctx *c)
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
                     c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_lcoef, c->d_lr1off,
                     c->d_lpair, c->d_spos, c->d_g0k, c->d_dos, c->d_dosp, c->d_domega,
-                    c->d_status, c->d_omega, c->d_ldos,
+                    c->d_sgroups, c->d_stmat, c->d_sstat, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
@@ -442,6 +457,13 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     plan.rwin = (c->precision == CQPI_PREC_MIXED && !c->keep_windows) ? std::min(g.h, 2 * g.w) : g.h;
     c->window_whole = plan.rwin >= g.h;
     plan.work_bytes = resolvent_work_bytes(g, (int)Ec, c->precision != CQPI_PREC_FP64 && transform32_supported(g));
+    if (c->surf_mode) {
+        // G(k) of the decimation: a block of energies at a time (under ~2 GiB)
+        const size_t cs = c->precision != CQPI_PREC_FP64 ? sizeof(float2) : sizeof(double2);
+        const size_t per_gk = (size_t)NP * g.K * g.K * cs;
+        plan.work_bytes = std::max<size_t>(1, std::min<size_t>(Ec, ((size_t)2 << 30) / per_gk)) * per_gk;
+        RETCHK(dev_grow(c->d_sstat, c->c_sstat, E, c->gpumem));
+    }
     if (plan.work_bytes)
         RETCHK(dev_grow(c->d_gkw, c->c_gkw, (plan.work_bytes + sizeof(double2) - 1) / sizeof(double2), c->gpumem));
     // mixed precision: the k->x transform writes the fp16 planes directly.  Their
@@ -491,7 +513,28 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     float2 *F1f = reinterpret_cast<float2 *>(c->d_F1) + eo * f1e;
     double2 *F1d = c->d_F1 + eo * f1e;
     int st = stage_begin(c, 1);
-    if (t32)
+    if (c->surf_mode) {
+        // G(k) by decimation into the workspace, then the ky pass from it
+        const size_t per_gk = (size_t)NP * g.K * g.K * (t32 ? sizeof(float2) : sizeof(double2));
+        const int EBa = (int)std::max<size_t>(1, std::min<size_t>((size_t)ec, plan.work_bytes / per_gk));
+        const DevRGroup *sg[3];
+        const double2 *stm[3];
+        for (int q = 0; q < 3; q++) {
+            sg[q] = c->d_sgroups + c->sg_goff[q];
+            stm[q] = c->d_stmat + c->sg_toff[q];
+        }
+        for (int ea = 0; ea < ec; ea += EBa) {
+            const int eb = std::min(EBa, ec - ea);
+            const double *om = c->d_omega + e0 + ea;
+            RETCHK(launch_surface_gk(g.B, g.K, sg, stm, c->sg_n, c->d_tw, c->efermi, c->scmax, om, eb, plan.eta,
+                                     c->surf_eps, c->surf_maxit, c->surf_mode == CQPI_GREEN_SURFACE, c->d_gkw, t32,
+                                     c->d_sstat + e0 + ea, s));
+            if (t32)
+                RETCHK(launch_resolvent_ky_gk(g, c->d_gkw, om, eb, c->d_tw32, F1f + (size_t)ea * f1e, true, s));
+            else
+                RETCHK(launch_resolvent_ky_gk(g, c->d_gkw, om, eb, c->d_tw, F1d + (size_t)ea * f1e, false, s));
+        }
+    } else if (t32)
         RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f, c->d_gkw,
                                      plan.work_bytes, s));
     else
@@ -578,6 +621,14 @@ int finish_run(cqpi_ctx *c, const RunPlan &plan)
     for (size_t e = 0; e < E; e++)
         if (c->h_status[e])
             return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
+    if (c->surf_mode) {
+        HIPCHK(hipMemcpy(c->h_status, c->d_sstat, E * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < E; e++)
+            if (c->h_status[e] > c->surf_maxit)
+                return fail(CQPI_E_CONVERGE, "the surface Green's function did not converge within " +
+                                                 std::to_string(c->surf_maxit) + " steps at energy index " +
+                                                 std::to_string(e));
+    }
     return 0;
 }
 
@@ -1056,23 +1107,24 @@ int cqpi_printinfo(cqpi_ctx *c, char *buf, size_t len)
     return 0;
 }
 
-int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double efermi, int scmodel)
+// Hopping terms grouped by lattice vector (rx, ry), 'anti' terms folded onto
+// -R; each group sums its amplitudes into a dense B x B block.  layer < 0:
+// every term (kz = 0, so rz drops out of the phase); 0: the in-plane terms
+// (rz == 0); 1: the coupling to the next layer (rz < 0), as
+// tightbind::setmatrix with kzdepinplane / kzdepoutofplane
+// (src/tightbinding.cc:184-208).  Returns the index of a term whose orbital
+// is out of range, else -1.
+static long group_hoppings(const cqpi_hopping *hops, size_t nhops, int B, int K, int layer, std::vector<DevRGroup> &G,
+                           std::vector<double2> &T)
 {
-    g_err.clear();
-    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
-    if (nhops && !hops) return fail(CQPI_E_ARG, "hops is NULL");
-    const int B = c->g.B;
-    // group by lattice vector (rx, ry), 'anti' terms folded onto -R; kz = 0
-    // so rz drops out of the phase.  Each group sums its amplitudes into a
-    // dense B x B block.
-    const int K = c->g.K;
     std::map<std::pair<double, double>, size_t> index;
-    std::vector<DevRGroup> G;
-    std::vector<double2> T;
+    G.clear();
+    T.clear();
     for (size_t i = 0; i < nhops; i++) {
         const cqpi_hopping &h = hops[i];
-        if (h.o1 < 0 || h.o2 < 0 || h.o1 >= B || h.o2 >= B)
-            return fail(CQPI_E_ARG, "hopping orbital index out of range at term " + std::to_string(i));
+        if (h.o1 < 0 || h.o2 < 0 || h.o1 >= B || h.o2 >= B) return (long)i;
+        if (layer == 0 && !(h.rz == 0.0)) continue;
+        if (layer == 1 && !(h.rz < 0.0)) continue;
         const double rx = (h.anti ? -h.rx : h.rx) + 0.0, ry = (h.anti ? -h.ry : h.ry) + 0.0;
         auto key = std::make_pair(rx, ry);
         auto it = index.find(key);
@@ -1094,6 +1146,47 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
         t.x += h.tre;
         t.y += h.tim;
     }
+    return -1;
+}
+
+// uploads the three hopping-group sets of the decimation (bulk in-plane,
+// surface in-plane, coupling) from the context's models
+static int upload_surface_groups(cqpi_ctx *c)
+{
+    const int B = c->g.B, K = c->g.K;
+    std::vector<DevRGroup> Gall, G;
+    std::vector<double2> Tall, T;
+    for (int q = 0; q < 3; q++) {
+        const std::vector<cqpi_hopping> &src = (q == 1 && !c->h_shops.empty()) ? c->h_shops : c->h_hops;
+        const long bad = group_hoppings(src.data(), src.size(), B, K, q == 2 ? 1 : 0, G, T);
+        if (bad >= 0)
+            return fail(CQPI_E_ARG, std::string(q == 1 && !c->h_shops.empty() ? "surface " : "") +
+                                        "hopping orbital index out of range at term " + std::to_string(bad));
+        c->sg_goff[q] = Gall.size();
+        c->sg_toff[q] = Tall.size();
+        c->sg_n[q] = (int)G.size();
+        Gall.insert(Gall.end(), G.begin(), G.end());
+        Tall.insert(Tall.end(), T.begin(), T.end());
+    }
+    RETCHK(dev_grow(c->d_sgroups, c->c_sgroups, std::max<size_t>(Gall.size(), 1), c->gpumem));
+    RETCHK(dev_grow(c->d_stmat, c->c_stmat, std::max<size_t>(Tall.size(), 1), c->gpumem));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!Gall.empty())
+        HIPCHK(hipMemcpy(c->d_sgroups, Gall.data(), Gall.size() * sizeof(DevRGroup), hipMemcpyHostToDevice));
+    if (!Tall.empty()) HIPCHK(hipMemcpy(c->d_stmat, Tall.data(), Tall.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double efermi, int scmodel)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (nhops && !hops) return fail(CQPI_E_ARG, "hops is NULL");
+    const int B = c->g.B;
+    std::vector<DevRGroup> G;
+    std::vector<double2> T;
+    const long bad = group_hoppings(hops, nhops, B, c->g.K, -1, G, T);
+    if (bad >= 0) return fail(CQPI_E_ARG, "hopping orbital index out of range at term " + std::to_string(bad));
     RETCHK(dev_grow(c->d_groups, c->c_groups, std::max<size_t>(G.size(), 1), c->gpumem));
     RETCHK(dev_grow(c->d_tmat, c->c_tmat, std::max<size_t>(T.size(), 1), c->gpumem));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1106,6 +1199,39 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     c->h_hops.assign(hops, hops + nhops);
     c->windows_kept = false;
     c->eigen_valid = false;
+    if (c->surf_mode) RETCHK(upload_surface_groups(c));
+    return 0;
+}
+
+int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *shops, size_t nshops)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (mode < CQPI_GREEN_NORMAL || mode > CQPI_GREEN_BULK) return fail(CQPI_E_ARG, "unknown Green's function mode");
+    if (mode == CQPI_GREEN_NORMAL) {
+        c->surf_mode = 0;
+        c->h_shops.clear();
+        return 0;
+    }
+    if (c->list_mode || c->spectral)
+        return fail(CQPI_E_ARG, "the surface Green's function needs a context made by cqpi_create");
+    if (!(epserr > 0.0)) return fail(CQPI_E_ARG, "epserr must be positive");
+    if (nshops && !shops) return fail(CQPI_E_ARG, "shops is NULL");
+    if (c->g.B > 5) return fail(CQPI_E_ARG, "the surface Green's function supports up to 5 bands");
+    const int K = c->g.K;
+    if ((K & (K - 1)) != 0 || K < 64 || K > 1024)
+        return fail(CQPI_E_ARG, "the surface Green's function needs a power-of-two k grid in [64, 1024]");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    c->h_shops.assign(shops, shops + nshops);
+    c->surf_mode = mode;
+    c->surf_eps = epserr;
+    const int r = upload_surface_groups(c);
+    if (r) {
+        c->surf_mode = 0;
+        c->h_shops.clear();
+        return r;
+    }
+    c->windows_kept = false;
     return 0;
 }
 
@@ -1154,7 +1280,8 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     // the timing window covers this run, plus a cqpi_compute_eigen called just before it
     if (!c->eigen_timed) reset_stages(c);
     c->eigen_timed = false;
-    if (!c->eigen_valid) {
+    // the surface Green's function does not use the eigen data
+    if (!c->eigen_valid && !(c->surf_mode && !c->spec_mode)) {
         c->in_run = true;
         int r = cqpi_compute_eigen(c);
         c->in_run = false;
@@ -1235,6 +1362,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     if (!c->windows_kept || c->nlayers == 0)
         return fail(CQPI_E_STATE, "the last cqpi_run did not keep its windows (cqpi_set_keep_windows)");
     if (c->maps_josephson) return fail(CQPI_E_ARG, "the gradient is of the LDOS map, not the Josephson map");
+    if (c->surf_mode) return fail(CQPI_E_ARG, "the gradient needs G0(k) from the eigen data, not the surface Green's function");
     if (c->list_mode) return fail(CQPI_E_ARG, "the gradient is of the dense LDOS map, not a threshold-list map");
     const Geom &g = c->g;
     if (!adjoint_supported(g))

@@ -156,6 +156,17 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec,
                         const double *omegas, int E, double eta, const double2 *tw,
                         double2 *F1, void *work, size_t work_bytes, hipStream_t s);
 size_t resolvent_work_bytes(const Geom &g, int E, bool fp32);
+// the ky pass from G(k) of E energies already in the workspace layout
+// Gk[e][pair][kx][ky] (double2, or float2 with fp32; power-of-two K in [64, 1024])
+int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, int E, const void *tw, void *F1,
+                           bool fp32, hipStream_t s);
+// surface (surface != 0) or bulk Green's function of a layered model by
+// decimation into that layout; groups g/t/n: bulk in-plane, surface
+// in-plane, coupling (rz < 0).  stat[e]: the most decimation steps any k
+// needed (maxit + 1: not converged).  B <= 5.
+int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *const t[3], const int n[3],
+                      const double2 *tw, double efermi, int scmax, const double *omegas, int E, double eta,
+                      double epserr, int maxit, int surface, void *Gk, bool fp32, int *stat, hipStream_t s);
 // transform along kx, window extraction and 1/K^2: G0w[e][sx][sy][pair]; with
 // mg != NULL also the fp16 hi/lo planes of the mixed contraction (scale sG)
 int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int E, const double2 *tw,

@@ -1,0 +1,257 @@
+// Surface / bulk Green's function of a layered model by decimation
+// (tmatrix::calcsurfacegreensfunction, src/qpi.cc:488-554), FP64.
+//
+// Per k (fractional k of the grid, kz = 0) the layers form a semi-infinite
+// chain with in-plane blocks eps_s (surface layer), eps_b (bulk layer) and
+// the coupling alpha to the next layer, beta = alpha^H
+// (tightbind::setmatrix with kzdepinplane / kzdepoutofplane,
+// src/tightbinding.cc:184-208: in-plane = hops with rz == 0 plus the -/+ efermi
+// diagonal, coupling = hops with rz < 0).  Each step, with
+// g = (w + i eta - eps_b)^-1:
+//   eps_s += alpha g beta;  eps_b += alpha g beta + beta g alpha;
+//   alpha <- alpha g alpha; beta <- beta g beta;
+// until both Frobenius norms are <= epserr (src/qpi.cc:541); then
+// G(k) = (w + i eta - eps_s)^-1 (surface) or with eps_b (bulk).  One thread per
+// (k, energy) with the b x b blocks in registers; the result goes to the G(k)
+// workspace layout Gk[e][pair][kx][ky] that the first transform pass reads.
+// A run stops a k-point after maxit steps and reports it in stat.
+#include "cqpi_common.h"
+
+namespace {
+
+struct SurfSets {
+    const DevRGroup *g[3];  // 0: bulk in-plane, 1: surface in-plane, 2: coupling (rz < 0)
+    const double2 *t[3];
+    int n[3];
+};
+
+template <int B>
+__device__ inline void build_h(const DevRGroup *__restrict__ groups, int ngroups, const double2 *__restrict__ tmat,
+                               const double2 *__restrict__ tw, int i, int j, int K, bool diag, double efermi,
+                               int scmax, double2 *H)
+{
+#pragma unroll
+    for (int r = 0; r < B * B; r++) H[r] = make_double2(0.0, 0.0);
+    if (diag)
+#pragma unroll
+        for (int r = 0; r < B; r++) H[r * B + r].x = r < scmax ? -efermi : efermi;
+    const double kx = (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
+    const double ky = (j > (K >> 1)) ? ((double)j - (double)K) / (double)K : (double)j / (double)K;
+    const unsigned kmask = (K & (K - 1)) == 0 ? (unsigned)K - 1u : 0u;
+    for (int gi = 0; gi < ngroups; gi++) {
+        const DevRGroup G = groups[gi];
+        double2 e;
+        if (G.integral) {
+            const unsigned m = (unsigned)G.irx * (unsigned)i + (unsigned)G.iry * (unsigned)j;
+            e = tw[kmask ? (m & kmask) : (m % (unsigned)K)];
+        } else {
+            double sn, cs;
+            sincos(2.0 * M_PI * (G.rx * kx + G.ry * ky), &sn, &cs);
+            e = make_double2(cs, sn);
+        }
+        const double2 *T = tmat + (size_t)gi * B * B;
+#pragma unroll
+        for (int r = 0; r < B * B; r++) H[r] = cadd(H[r], cmul(T[r], e));
+    }
+}
+
+// G = (w + i eta - H)^-1, Gauss-Jordan with partial pivoting; every index
+// static (the row exchange by compares)
+template <int B>
+__device__ inline void resinv(const double2 *H, double om, double eta, double2 *G)
+{
+    double2 M[B * B];
+#pragma unroll
+    for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            M[r * B + c] = make_double2((r == c ? om : 0.0) - H[r * B + c].x, (r == c ? eta : 0.0) - H[r * B + c].y);
+            G[r * B + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+        }
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+        int piv = j;
+        double best = cabs2(M[j * B + j]);
+#pragma unroll
+        for (int i = j + 1; i < B; i++)
+            if (cabs2(M[i * B + j]) > best) {
+                best = cabs2(M[i * B + j]);
+                piv = i;
+            }
+#pragma unroll
+        for (int i = j + 1; i < B; i++)
+            if (i == piv)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    double2 t = M[j * B + c];
+                    M[j * B + c] = M[i * B + c];
+                    M[i * B + c] = t;
+                    t = G[j * B + c];
+                    G[j * B + c] = G[i * B + c];
+                    G[i * B + c] = t;
+                }
+        const double2 ip = cinv(M[j * B + j]);
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            M[j * B + c] = cmul(M[j * B + c], ip);
+            G[j * B + c] = cmul(G[j * B + c], ip);
+        }
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            if (i == j) continue;
+            const double2 l = M[i * B + j];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                M[i * B + c] = csub(M[i * B + c], cmul(l, M[j * B + c]));
+                G[i * B + c] = csub(G[i * B + c], cmul(l, G[j * B + c]));
+            }
+        }
+    }
+}
+
+// P = X Y (B x B)
+template <int B>
+__device__ inline void mm(const double2 *X, const double2 *Y, double2 *P)
+{
+#pragma unroll
+    for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            double2 a = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < B; k++) a = cadd(a, cmul(X[r * B + k], Y[k * B + c]));
+            P[r * B + c] = a;
+        }
+}
+
+// X <- X Y, row by row (row r of X Y needs only row r of X)
+template <int B>
+__device__ inline void mm_left_inplace(double2 *X, const double2 *Y)
+{
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        double2 row[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            double2 a = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < B; k++) a = cadd(a, cmul(X[r * B + k], Y[k * B + c]));
+            row[c] = a;
+        }
+#pragma unroll
+        for (int c = 0; c < B; c++) X[r * B + c] = row[c];
+    }
+}
+
+__device__ inline void store_c(double2 *p, double2 v) { *p = v; }
+__device__ inline void store_c(float2 *p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+template <typename C, int B>
+__global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const double2 *__restrict__ tw, double efermi,
+                                                   int scmax, const double *__restrict__ omegas, double eta,
+                                                   double epserr, int maxit, int surface, C *__restrict__ Gk,
+                                                   int *__restrict__ stat)
+{
+    constexpr int NP = B * B;
+    const long KK = (long)K * K;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y;
+    if (k >= KK) return;
+    const int i = (int)(k / K), j = (int)(k - (long)i * K);
+    double2 es[NP], eb[NP], al[NP], be[NP], g[NP], t1[NP];
+    build_h<B>(sets.g[1], sets.n[1], sets.t[1], tw, i, j, K, true, efermi, scmax, es);
+    build_h<B>(sets.g[0], sets.n[0], sets.t[0], tw, i, j, K, true, efermi, scmax, eb);
+    build_h<B>(sets.g[2], sets.n[2], sets.t[2], tw, i, j, K, false, efermi, scmax, al);
+#pragma unroll
+    for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = 0; c < B; c++) be[r * B + c] = cconj(al[c * B + r]);
+    const double om = omegas[e];
+    const double eps2 = epserr * epserr;
+    int it = 0;
+    for (;;) {
+        resinv<B>(eb, om, eta, g);
+        mm<B>(g, be, t1);  // t1 = g beta
+#pragma unroll
+        for (int r = 0; r < B; r++)
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                double2 a = make_double2(0.0, 0.0);  // (alpha g beta)_rc
+#pragma unroll
+                for (int q = 0; q < B; q++) a = cadd(a, cmul(al[r * B + q], t1[q * B + c]));
+                es[r * B + c] = cadd(es[r * B + c], a);
+                eb[r * B + c] = cadd(eb[r * B + c], a);
+            }
+        mm_left_inplace<B>(g, al);  // g <- g alpha
+#pragma unroll
+        for (int r = 0; r < B; r++)
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                double2 a = make_double2(0.0, 0.0);  // (beta g alpha)_rc
+#pragma unroll
+                for (int q = 0; q < B; q++) a = cadd(a, cmul(be[r * B + q], g[q * B + c]));
+                eb[r * B + c] = cadd(eb[r * B + c], a);
+            }
+        mm_left_inplace<B>(be, t1);  // beta <- beta g beta
+        mm_left_inplace<B>(al, g);   // alpha <- alpha g alpha
+        it++;
+        double na = 0.0, nb = 0.0;
+#pragma unroll
+        for (int r = 0; r < NP; r++) {
+            na += cabs2(al[r]);
+            nb += cabs2(be[r]);
+        }
+        if (!(na > eps2 || nb > eps2)) break;  // both Frobenius norms <= epserr
+        if (it >= maxit) {
+            it = maxit + 1;  // not converged
+            break;
+        }
+    }
+    resinv<B>(surface ? es : eb, om, eta, g);
+#pragma unroll
+    for (int r = 0; r < NP; r++) store_c(Gk + ((size_t)e * NP + r) * KK + k, g[r]);
+    atomicMax(stat + e, it);
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+namespace cqpi {
+
+int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *const t[3], const int n[3],
+                      const double2 *tw, double efermi, int scmax, const double *omegas, int E, double eta,
+                      double epserr, int maxit, int surface, void *Gk, bool fp32, int *stat, hipStream_t s)
+{
+    SurfSets sets;
+    for (int q = 0; q < 3; q++) {
+        sets.g[q] = g[q];
+        sets.t[q] = t[q];
+        sets.n[q] = n[q];
+    }
+    if (E < 1 || K < 1) return CQPI_E_ARG;
+    hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((E + 63) / 64)), dim3(64), 0, s, stat, E);
+    const long KK = (long)K * K;
+    const dim3 grid((unsigned)((KK + 63) / 64), (unsigned)E);
+    switch (B) {
+#define CASE(b)                                                                                                    \
+    case b:                                                                                                        \
+        if (fp32)                                                                                                  \
+            hipLaunchKernelGGL((k_surface_gk<float2, b>), grid, dim3(64), 0, s, K, sets, tw, efermi, scmax, omegas, \
+                               eta, epserr, maxit, surface, static_cast<float2 *>(Gk), stat);                      \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_surface_gk<double2, b>), grid, dim3(64), 0, s, K, sets, tw, efermi, scmax,        \
+                               omegas, eta, epserr, maxit, surface, static_cast<double2 *>(Gk), stat);             \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#undef CASE
+    default: return CQPI_E_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+}  // namespace cqpi

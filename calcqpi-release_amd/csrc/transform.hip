@@ -605,6 +605,22 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
+// pass 1 alone, from G(k) of every energy already in the workspace layout
+// Gk[e][pair][kx][ky] (the surface Green's function writes it)
+template <typename C, int N>
+int launch_res_gk(const Geom &g, const C *Gk, const double *omegas, int E, const C *tw, C *F1, hipStream_t s)
+{
+    const int NP = g.B * g.B;
+    const int np = wave_pairs(N, NP, sizeof(C));
+    const int nch = (NP + np - 1) / np;
+    const int thr = wave_threads(N, np, true);
+    if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
+    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
+    hipLaunchKernelGGL((k_res_ky_s<C, 0, N>), dim3(N, E, nch), dim3(thr), lds, s, g, (const double *)nullptr,
+                       (const double2 *)nullptr, omegas, E, 1, 0.0, tw, np, F1, Gk);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
 template <typename C, int N>
 int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                   const C *tw, C *F1, void *work, size_t work_bytes, hipStream_t s)
@@ -697,6 +713,23 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
 #undef CASE
     default: return CQPI_E_ARG;
     }
+}
+
+int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, int E, const void *tw, void *F1,
+                           bool fp32, hipStream_t s)
+{
+    if (!wave_path(g.K)) return CQPI_E_ARG;
+#define GK_CASE(n)                                                                                             \
+    case n:                                                                                                    \
+        return fp32 ? launch_res_gk<float2, n>(g, static_cast<const float2 *>(Gk), omegas, E,                 \
+                                               static_cast<const float2 *>(tw), static_cast<float2 *>(F1), s) \
+                    : launch_res_gk<double2, n>(g, static_cast<const double2 *>(Gk), omegas, E,               \
+                                                static_cast<const double2 *>(tw), static_cast<double2 *>(F1), s);
+    switch (g.K) {
+        GK_CASE(64) GK_CASE(128) GK_CASE(256) GK_CASE(512) GK_CASE(1024)
+    }
+#undef GK_CASE
+    return CQPI_E_ARG;
 }
 
 bool transform32_supported(const Geom &g) { return wave_path(g.K); }

@@ -43,7 +43,8 @@ enum {
     CQPI_E_STATE = 3,    /* call out of order (e.g. run before set_model) */
     CQPI_E_SINGULAR = 4, /* 1 - V G0(0) singular at some energy */
     CQPI_E_IO = 5,       /* file could not be read or parsed */
-    CQPI_E_NODEV = 6     /* no HIP device */
+    CQPI_E_NODEV = 6,    /* no HIP device */
+    CQPI_E_CONVERGE = 7  /* surface Green's function not converged */
 };
 
 /* precision of the continuum contraction (the other stages are always FP64) */
@@ -256,6 +257,25 @@ int cqpi_set_timing(cqpi_ctx *ctx, int on);
  * 2*maxband <= bands and FP64 or FP32 precision; a run with it on cannot feed
  * cqpi_gradient. */
 int cqpi_set_josephson(cqpi_ctx *ctx, int on, double deltat, double etat);
+
+/* Green's function the LDOS runs use (green= in the reference driver,
+ * src/calcqpi.cc:129, :207-208, :367-375). */
+enum {
+    CQPI_GREEN_NORMAL = 0,  /* G0(k) of the bulk model from its eigen data (default) */
+    CQPI_GREEN_SURFACE = 1, /* surface layer of a semi-infinite stack of layers */
+    CQPI_GREEN_BULK = 2     /* a layer deep inside that stack */
+};
+/* Replaces G0(k) of cqpi_run by the surface or bulk Green's function of the
+ * layered model (tmatrix::calcsurfacegreensfunction, src/qpi.cc:488-554,
+ * at kz = 0): per k the in-plane block (hoppings with rz == 0 plus the
+ * -/+ efermi diagonal) and the coupling to the next layer (rz < 0) are
+ * decimated until the Frobenius norms of both couplings are <= epserr; shops
+ * (NULL/0: the model's own hoppings) gives the surface layer's in-plane block
+ * (stbfile=).  Call after cqpi_set_model.  bands <= 5, power-of-two k grid
+ * in [64, 1024], a context made by cqpi_create; cqpi_run returns
+ * CQPI_E_CONVERGE if some k needs more than 500 steps.  A run with it cannot
+ * feed cqpi_gradient.  mode CQPI_GREEN_NORMAL switches it off. */
+int cqpi_set_surface(cqpi_ctx *ctx, int mode, double epserr, const cqpi_hopping *shops, size_t nshops);
 
 /* For a context of cqpi_create_spectral: mode 1 (SPF) or 2 (USPF) makes
  * cqpi_run compute those maps (n x n per energy) from the device's eigen

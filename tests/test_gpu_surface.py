@@ -1,0 +1,109 @@
+"""GPU parity of the surface / bulk Green's function (green=surface|bulk,
+tmatrix::calcsurfacegreensfunction, src/qpi.cc:488-554) against the FP64
+oracle chain surfacegreen -> realspace -> tmatrix -> wannierldos, called
+through the C ABI (cqpi_set_surface + cqpi_run)."""
+import numpy as np
+import pytest
+
+import cqpi
+import oracle
+import refmath as rm
+from cqpi import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL_RED = 1e-4
+
+
+def _oracle_maps(wl, surface, epserr, shops=None):
+    out = []
+    for om in wl.energies:
+        g, it = oracle.surfacegreen(wl.hops, wl.bands, wl.efermi, wl.K, om, wl.eta, epserr=epserr,
+                                    surface=surface, shops=shops, scmodel=wl.scmodel)
+        assert it > 1
+        gR = oracle.realspace(g)
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        out.append(oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, wl.maxband, wl.spin))
+    return np.array(out)
+
+
+def _run(wl, mode, epserr, precision="fp64", shops=None):
+    print("case: %s %s K=%d n=%d b=%d E=%d prec=%s shops=%s" % (
+        wl.name, mode, wl.K, wl.n, wl.bands, len(wl.energies), precision, shops is not None), flush=True)
+    with cqpi.engine_for(wl, precision) as q:
+        q.set_surface(mode, epserr, shops)
+        return q.run(wl.energies, wl.eta)
+
+
+def _assert_close(got, ref, tol):
+    for e in range(ref.shape[0]):
+        mx, rms = rm.rel_errors(got[e], ref[e])
+        assert mx < tol, "energy %d: max_rel %.3e rms_rel %.3e" % (e, mx, rms)
+
+
+@pytest.mark.parametrize("mode", ["surface", "bulk"])
+@pytest.mark.parametrize("b", [1, 3])
+def test_surface_chain_matches_oracle(gpu, mode, b):
+    wl = synth.layered(b, K=64, n=16, E=3, seed=40 + b, w=1, os_=2, eta=0.02)
+    _assert_close(_run(wl, mode, 1e-10), _oracle_maps(wl, mode == "surface", 1e-10), TOL64)
+
+
+def test_surface_layer_model(gpu):
+    """stbfile=: the surface layer's own in-plane terms (here the on-site
+    energies shifted) for eps_s; the bulk and coupling blocks from the model."""
+    wl = synth.layered(3, K=64, n=16, E=2, seed=47, w=1, os_=2, eta=0.02)
+    shops = np.array(wl.hops[wl.hops["rz"] == 0], synth.HOP_DTYPE)
+    on = (shops["rx"] == 0) & (shops["ry"] == 0) & (shops["o1"] == shops["o2"])
+    shops["tre"][on] += 0.07
+    _assert_close(_run(wl, "surface", 1e-10, shops=shops),
+                  _oracle_maps(wl, True, 1e-10, shops=shops), TOL64)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_surface_reduced_precision(gpu, precision):
+    wl = synth.layered(3, K=128, n=16, E=2, seed=48, w=1, os_=2, eta=0.02)
+    _assert_close(_run(wl, "surface", 1e-10, precision), _oracle_maps(wl, True, 1e-10), TOL_RED)
+
+
+def test_surface_default_epserr(gpu):
+    """epserr = 1e-5 (the driver's default): the decimation stops at the same
+    step as the oracle's, so the maps agree to rounding."""
+    wl = synth.layered(3, K=64, n=16, E=2, seed=49, w=1, os_=2, eta=0.03)
+    _assert_close(_run(wl, "bulk", 1e-5), _oracle_maps(wl, False, 1e-5), 1e-8)
+
+
+def test_normal_mode_restores_the_eigen_path(gpu):
+    wl = synth.layered(3, K=64, n=16, E=2, seed=50, w=1, os_=2, eta=0.02)
+    with cqpi.engine_for(wl) as q:
+        q.set_surface("surface", 1e-10)
+        a = q.run(wl.energies, wl.eta)
+        q.set_surface("normal")
+        b = q.run(wl.energies, wl.eta)
+    _assert_close(a, _oracle_maps(wl, True, 1e-10), TOL64)
+    _assert_close(b, oracle.cldos_run(wl), TOL64)
+
+
+def test_surface_rejections(gpu):
+    wl = synth.layered(3, K=64, n=16, E=1, seed=51, w=1, os_=2, eta=0.02)
+    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
+    try:
+        with pytest.raises(cqpi.CqpiError):
+            q.set_surface("surface")  # before set_model
+        q.set_model(wl.hops, wl.efermi, False)
+        with pytest.raises(cqpi.CqpiError):
+            q.set_surface("surface", 0.0)
+        with pytest.raises(cqpi.CqpiError):
+            q.set_surface(7)
+    finally:
+        q.close()
+    wl24 = synth.layered(3, K=24, n=8, E=1, seed=52, w=1, os_=2, eta=0.02)
+    with cqpi.engine_for(wl24) as q:
+        with pytest.raises(cqpi.CqpiError):
+            q.set_surface("surface")  # k grid not a power of two in [64, 1024]
+    with cqpi.engine_for(wl) as q:
+        q.set_keep_windows(True)
+        q.set_surface("bulk", 1e-10)
+        q.run(wl.energies, wl.eta)
+        with pytest.raises(cqpi.CqpiError):
+            q.gradient(np.ones((1, wl.W * wl.W)))  # the adjoint differentiates the eigen path only
