@@ -337,12 +337,14 @@ def layered(b, K, n, E, seed, tz=0.12, **kw):
     extra = []
     for rx in (0.0, 1.0):
         M = tz * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b))) / (2.0 if rx else 1.0)
+        # <o1, R| H |o2, 0> = M[o1, o2] at R = (rx, 0, -1); the hermitian
+        # partner at (-rx, 0, +1) is M^H (one block per R, as an _hr file groups them)
         for o1 in range(b):
             for o2 in range(b):
-                # <o1, R| H |o2, 0> = M[o1, o2] at R = (rx, 0, -1); the
-                # hermitian partner conj(M[o1, o2]) at (-rx, 0, +1) as (o2, o1)
                 extra.append((rx, 0.0, -1.0, o1, o2, M[o1, o2].real, M[o1, o2].imag, 0, 0))
-                extra.append((-rx, 0.0, 1.0, o2, o1, M[o1, o2].real, -M[o1, o2].imag, 0, 0))
+        for o1 in range(b):
+            for o2 in range(b):
+                extra.append((-rx, 0.0, 1.0, o1, o2, M[o2, o1].real, -M[o2, o1].imag, 0, 0))
     hops = np.concatenate([np.asarray(wl.hops, HOP_DTYPE), np.array(extra, HOP_DTYPE)])
     return dataclasses.replace(wl, name="layered %d-orbital" % b, hops=hops)
 

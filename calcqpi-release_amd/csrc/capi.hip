@@ -459,7 +459,8 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     plan.work_bytes = resolvent_work_bytes(g, (int)Ec, c->precision != CQPI_PREC_FP64 && transform32_supported(g));
     if (c->surf_mode) {
         // G(k) of the decimation: a block of energies at a time (under ~2 GiB)
-        const size_t cs = c->precision != CQPI_PREC_FP64 ? sizeof(float2) : sizeof(double2);
+        const size_t cs =
+            c->precision != CQPI_PREC_FP64 && transform32_supported(g) ? sizeof(float2) : sizeof(double2);
         const size_t per_gk = (size_t)NP * g.K * g.K * cs;
         plan.work_bytes = std::max<size_t>(1, std::min<size_t>(Ec, ((size_t)2 << 30) / per_gk)) * per_gk;
         RETCHK(dev_grow(c->d_sstat, c->c_sstat, E, c->gpumem));
@@ -603,6 +604,12 @@ int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
     return 0;
 }
 
+}  // namespace
+
+static int check_surface_steps(cqpi_ctx *c, size_t E);
+
+namespace {
+
 // waits for the run and reports a singular 1 - V G0(0)
 int finish_run(cqpi_ctx *c, const RunPlan &plan)
 {
@@ -621,14 +628,7 @@ int finish_run(cqpi_ctx *c, const RunPlan &plan)
     for (size_t e = 0; e < E; e++)
         if (c->h_status[e])
             return fail(CQPI_E_SINGULAR, "1 - V G0(0) is singular at energy index " + std::to_string(e));
-    if (c->surf_mode) {
-        HIPCHK(hipMemcpy(c->h_status, c->d_sstat, E * sizeof(int), hipMemcpyDeviceToHost));
-        for (size_t e = 0; e < E; e++)
-            if (c->h_status[e] > c->surf_maxit)
-                return fail(CQPI_E_CONVERGE, "the surface Green's function did not converge within " +
-                                                 std::to_string(c->surf_maxit) + " steps at energy index " +
-                                                 std::to_string(e));
-    }
+    if (c->surf_mode) RETCHK(check_surface_steps(c, E));
     return 0;
 }
 
@@ -868,6 +868,45 @@ int cqpi_set_spectral(cqpi_ctx *c, int mode)
     return 0;
 }
 
+// energies per G(k) block of the surface Green's function (FP64 workspace
+// kept under ~2 GiB); grows the workspace and the step counters
+static int surface_blocks(cqpi_ctx *c, size_t E, int &EB)
+{
+    const Geom &g = c->g;
+    const size_t per_gk = (size_t)g.B * g.B * g.K * g.K * sizeof(double2);
+    EB = (int)std::max<size_t>(1, std::min<size_t>(E, ((size_t)2 << 30) / per_gk));
+    RETCHK(dev_grow(c->d_gkw, c->c_gkw, (size_t)EB * per_gk / sizeof(double2), c->gpumem));
+    RETCHK(dev_grow(c->d_sstat, c->c_sstat, E, c->gpumem));
+    return 0;
+}
+
+// FP64 G(k) of the surface Green's function of energies d_om[0, E) into the workspace
+static int surface_gk(cqpi_ctx *c, const double *d_om, int E, double eta, int *stat)
+{
+    const DevRGroup *sg[3];
+    const double2 *stm[3];
+    for (int q = 0; q < 3; q++) {
+        sg[q] = c->d_sgroups + c->sg_goff[q];
+        stm[q] = c->d_stmat + c->sg_toff[q];
+    }
+    return launch_surface_gk(c->g.B, c->g.K, sg, stm, c->sg_n, c->d_tw, c->efermi, c->scmax, d_om, E, eta,
+                             c->surf_eps, c->surf_maxit, c->surf_mode == CQPI_GREEN_SURFACE, c->d_gkw, false, stat,
+                             c->stream);
+}
+
+// after the stream is done: CQPI_E_CONVERGE if a k-point of energy e < E ran out of steps
+static int check_surface_steps(cqpi_ctx *c, size_t E)
+{
+    std::vector<int> st(E);
+    HIPCHK(hipMemcpy(st.data(), c->d_sstat, E * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < E; e++)
+        if (st[e] > c->surf_maxit)
+            return fail(CQPI_E_CONVERGE, "the surface Green's function did not converge within " +
+                                             std::to_string(c->surf_maxit) + " steps at energy index " +
+                                             std::to_string(e));
+    return 0;
+}
+
 int cqpi_dos(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *dos)
 {
     g_err.clear();
@@ -875,21 +914,35 @@ int cqpi_dos(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *do
     if (!E) return fail(CQPI_E_ARG, "no energies");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (!c->eigen_valid) {
+    if (!c->eigen_valid && !c->surf_mode) {
         c->in_run = true;
         int r = cqpi_compute_eigen(c);
         c->in_run = false;
         if (r) return r;
     }
     const Geom &g = c->g;
+    const int nc = dos_chunks(g.K);
     RETCHK(dev_grow(c->d_domega, c->c_domega, E, c->gpumem));
     RETCHK(dev_grow(c->d_dos, c->c_dos, E, c->gpumem));
-    RETCHK(dev_grow(c->d_dosp, c->c_dosp, E * (size_t)dos_chunks(g.K), c->gpumem));
+    RETCHK(dev_grow(c->d_dosp, c->c_dosp, E * (size_t)nc, c->gpumem));
     HIPCHK(hipMemcpy(c->d_domega, omegas, E * sizeof(double), hipMemcpyHostToDevice));
-    int r = launch_dos(g.K, g.B, g.maxb, c->d_eval, c->d_evec, c->d_domega, (int)E, eta, c->d_dosp, c->d_dos,
-                       c->stream);
-    if (r) return fail(r, "density-of-states launch failed");
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->surf_mode) {
+        int EB = 1;
+        RETCHK(surface_blocks(c, E, EB));
+        for (size_t ea = 0; ea < E; ea += EB) {
+            const int eb = (int)std::min<size_t>(EB, E - ea);
+            int r = surface_gk(c, c->d_domega + ea, eb, eta, c->d_sstat + ea);
+            if (!r) r = launch_dos_gk(g.K, g.B, g.maxb, c->d_gkw, eb, c->d_dosp + ea * nc, c->d_dos + ea, c->stream);
+            if (r) return fail(r, "density-of-states launch failed");
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RETCHK(check_surface_steps(c, E));
+    } else {
+        int r = launch_dos(g.K, g.B, g.maxb, c->d_eval, c->d_evec, c->d_domega, (int)E, eta, c->d_dosp, c->d_dos,
+                           c->stream);
+        if (r) return fail(r, "density-of-states launch failed");
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     HIPCHK(hipMemcpy(dos, c->d_dos, E * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -1213,14 +1266,9 @@ int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *s
         c->h_shops.clear();
         return 0;
     }
-    if (c->list_mode || c->spectral)
-        return fail(CQPI_E_ARG, "the surface Green's function needs a context made by cqpi_create");
     if (!(epserr > 0.0)) return fail(CQPI_E_ARG, "epserr must be positive");
     if (nshops && !shops) return fail(CQPI_E_ARG, "shops is NULL");
     if (c->g.B > 5) return fail(CQPI_E_ARG, "the surface Green's function supports up to 5 bands");
-    const int K = c->g.K;
-    if ((K & (K - 1)) != 0 || K < 64 || K > 1024)
-        return fail(CQPI_E_ARG, "the surface Green's function needs a power-of-two k grid in [64, 1024]");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     c->h_shops.assign(shops, shops + nshops);
     c->surf_mode = mode;
@@ -1297,10 +1345,27 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
         c->up_omegas.assign(omegas, omegas + E);
         HIPCHK(hipMemcpyAsync(c->d_omega, c->up_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice, c->stream));
         int st = stage_begin(c, 4);
-        int r = launch_spectral(g.K, g.n, g.B, g.maxb, g.spin, c->spec_mode, nullptr, c->d_eval, c->d_evec,
+        int r = 0;
+        if (c->surf_mode) {
+            int EB = 1;
+            RETCHK(surface_blocks(c, E, EB));
+            for (size_t ea = 0; ea < E && !r; ea += EB) {
+                const int eb = (int)std::min<size_t>(EB, E - ea);
+                r = surface_gk(c, c->d_omega + ea, eb, eta, c->d_sstat + ea);
+                if (!r)
+                    r = launch_spectral_gk(g.K, g.n, g.B, g.maxb, g.spin, c->spec_mode, c->d_gkw, eb, c->d_spos,
+                                           c->d_ldos + ea * (size_t)g.W * g.W, c->stream);
+            }
+        } else {
+            r = launch_spectral(g.K, g.n, g.B, g.maxb, g.spin, c->spec_mode, nullptr, c->d_eval, c->d_evec,
                                 c->d_omega, (int)E, eta, c->d_spos, c->d_ldos, c->stream);
+        }
         stage_end(c, st);
         if (r) return fail(r, "spectral-function launch failed");
+        if (c->surf_mode) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            RETCHK(check_surface_steps(c, E));
+        }
         c->nlayers = E;
         c->windows_kept = false;
         c->last_omegas.assign(omegas, omegas + E);

@@ -222,6 +222,11 @@ int launch_spectral(int K, int n, int B, int mb, int spin, int mode, const doubl
                     hipStream_t s);
 // density of states -sum_{o<mb} Im G0(R=0)_oo per energy; partial: E * dos_chunks(K) doubles
 int dos_chunks(int K);
+// the SPF / USPF maps and the DOS from the G(k) workspace Gk[e][pair][k] of E
+// energies (the surface Green's function); partial: E * dos_chunks(K) doubles
+int launch_spectral_gk(int K, int n, int B, int mb, int spin, int mode, const double2 *Gk, int E, const double *pos,
+                       double *out, hipStream_t s);
+int launch_dos_gk(int K, int B, int mb, const double2 *Gk, int E, double *partial, double *dos, hipStream_t s);
 int launch_dos(int K, int B, int mb, const double *eval, const double2 *evec, const double *omegas, int E,
                double eta, double *partial, double *dos, hipStream_t s);
 // FP32 contraction: narrows the E windows and T into G0f (E*nw*nw*b*b) and Tf

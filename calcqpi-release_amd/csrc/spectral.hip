@@ -10,8 +10,9 @@
 //        k_g = ((i + n/2) mod n) mod K (per spin half with spin).
 //   DOS  (tmatrix::calcdos, src/qpi.cc:856-862): -sum_{o<maxband} Im G0(R=0)_oo,
 //        with G0(R=0) = K^-2 sum_k G0(k).
-// G0(k) comes either from the host (the drop-in calls hand over the k grid)
-// or from the cached eigen data, G0(k)_{l,m} = sum_n v_ln conj(v_mn) / (w + i eta - e_n).
+// G0(k) comes from the host (the drop-in calls hand over the k grid), from
+// the cached eigen data, G0(k)_{l,m} = sum_n v_ln conj(v_mn) / (w + i eta - e_n),
+// or from the G(k) workspace Gk[e][pair][k] that the surface Green's function fills.
 #include <algorithm>
 
 #include "cqpi_common.h"
@@ -35,9 +36,12 @@ __device__ inline double2 g_eig(const double *__restrict__ eval, const double2 *
     return acc;
 }
 
-// EIG = false: G0(k) blocks g0k[kk][l*B + m] (drop-in, one energy);
-// EIG = true: from the eigen data for energy omegas[blockIdx.y]
-template <bool EIG>
+enum { kSrcHost = 0, kSrcEigen = 1, kSrcWork = 2 };
+
+// SRC kSrcHost: G0(k) blocks g0k[kk][l*B + m] (drop-in, one energy);
+// kSrcEigen: from the eigen data for energy omegas[blockIdx.y];
+// kSrcWork: g0k is the workspace Gk[e][l*B + m][kk]
+template <int SRC>
 __global__ void __launch_bounds__(256) k_spectral(SpecArgs a, const double2 *__restrict__ g0k,
                                                   const double *__restrict__ eval, const double2 *__restrict__ evec,
                                                   const double *__restrict__ omegas, double eta,
@@ -48,9 +52,10 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs a, const double2 *__r
     const int n = a.n, K = a.K, B = a.B, n2 = n >> 1;
     if (p >= n * n) return;
     const long KK = (long)K * K;
-    const double om = EIG ? omegas[e] : 0.0;
+    const double om = SRC == kSrcEigen ? omegas[e] : 0.0;
     auto G = [&](long kk, int l, int m) -> double2 {
-        if (EIG) return g_eig(eval, evec, KK, kk, B, l, m, om, eta);
+        if (SRC == kSrcEigen) return g_eig(eval, evec, KK, kk, B, l, m, om, eta);
+        if (SRC == kSrcWork) return g0k[((long)e * B * B + l * B + m) * KK + kk];
         return g0k[kk * B * B + l * B + m];
     };
     double tr = 0.0;
@@ -110,6 +115,28 @@ __global__ void __launch_bounds__(kDosThreads) k_dos_partial(int K, int B, int m
     if (threadIdx.x == 0) partial[(size_t)e * nc + c] = red[0];
 }
 
+// the same partial sums from the G(k) workspace Gk[e][pair][k]
+__global__ void __launch_bounds__(kDosThreads) k_dos_partial_gk(int K, int B, int mb, const double2 *__restrict__ Gk,
+                                                                double *__restrict__ partial)
+{
+    __shared__ double red[kDosThreads];
+    const long KK = (long)K * K;
+    const int e = blockIdx.y, c = blockIdx.x, nc = gridDim.x;
+    const long k0 = KK * c / nc, k1 = KK * (c + 1) / nc;
+    double acc = 0.0;
+    for (int o = 0; o < mb; o++) {
+        const double2 *row = Gk + ((size_t)e * B * B + (size_t)o * B + o) * KK;
+        for (long kk = k0 + threadIdx.x; kk < k1; kk += blockDim.x) acc -= row[kk].y;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kDosThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)e * nc + c] = red[0];
+}
+
 __global__ void k_dos_final(int E, int nc, double scale, const double *__restrict__ partial, double *__restrict__ dos)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,9 +158,31 @@ int launch_spectral(int K, int n, int B, int mb, int spin, int mode, const doubl
     const SpecArgs a{K, n, B, mb, spin, mode};
     const dim3 grid((unsigned)((n * n + 255) / 256), (unsigned)E);
     if (g0k)
-        hipLaunchKernelGGL(k_spectral<false>, grid, dim3(256), 0, s, a, g0k, eval, evec, omegas, eta, pos, out);
+        hipLaunchKernelGGL(k_spectral<kSrcHost>, grid, dim3(256), 0, s, a, g0k, eval, evec, omegas, eta, pos, out);
     else
-        hipLaunchKernelGGL(k_spectral<true>, grid, dim3(256), 0, s, a, g0k, eval, evec, omegas, eta, pos, out);
+        hipLaunchKernelGGL(k_spectral<kSrcEigen>, grid, dim3(256), 0, s, a, g0k, eval, evec, omegas, eta, pos, out);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+int launch_spectral_gk(int K, int n, int B, int mb, int spin, int mode, const double2 *Gk, int E, const double *pos,
+                       double *out, hipStream_t s)
+{
+    if ((mode != 1 && mode != 2) || n > K || mb > B || E < 1) return CQPI_E_ARG;
+    const SpecArgs a{K, n, B, mb, spin, mode};
+    const dim3 grid((unsigned)((n * n + 255) / 256), (unsigned)E);
+    hipLaunchKernelGGL(k_spectral<kSrcWork>, grid, dim3(256), 0, s, a, Gk, nullptr, nullptr, nullptr, 0.0, pos, out);
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+int launch_dos_gk(int K, int B, int mb, const double2 *Gk, int E, double *partial, double *dos, hipStream_t s)
+{
+    if (mb > B || E < 1) return CQPI_E_ARG;
+    const int nc = dos_chunks(K);
+    hipLaunchKernelGGL(k_dos_partial_gk, dim3((unsigned)nc, (unsigned)E), dim3(kDosThreads), 0, s, K, B, mb, Gk,
+                       partial);
+    if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    hipLaunchKernelGGL(k_dos_final, dim3((unsigned)((E + 63) / 64)), dim3(64), 0, s, E, nc,
+                       1.0 / ((double)K * (double)K), partial, dos);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 

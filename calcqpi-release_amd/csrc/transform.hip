@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(kThreads) k_resolvent_ky(Geom g, const double 
                                                          const double2 *__restrict__ evec,
                                                          const double *__restrict__ omegas, int E,
                                                          int EB, double eta, const double2 *__restrict__ tw,
-                                                         int np, double2 *__restrict__ F1)
+                                                         int np, double2 *__restrict__ F1,
+                                                         const double2 *__restrict__ Gk)
 {
     extern __shared__ double2 S[];
     constexpr int NP = B * B;
@@ -86,7 +87,14 @@ __global__ void __launch_bounds__(kThreads) k_resolvent_ky(Geom g, const double 
     }
     for (int e = e0; e < e1; e++) {
         const double omega = omegas[e];
-        if (regs) {
+        if (Gk) {
+            // G(k) of energy e from the workspace Gk[e][pair][kx][ky]
+            const double2 *gk = Gk + ((size_t)e * NP + p0) * KK + (size_t)i * K;
+            for (int t = threadIdx.x; t < npc * K; t += kThreads) {
+                const int q = t / K, j = t - q * K;
+                S[(size_t)q * K + (pow2 ? brev(j, lgK) : j)] = gk[(size_t)q * KK + j];
+            }
+        } else if (regs) {
             if constexpr (kRegs) {
 #pragma unroll
                 for (int jj = 0; jj < kMaxJ; jj++) {
@@ -663,7 +671,7 @@ bool wave_path(int K) { return (K & (K - 1)) == 0 && K >= kMinWaveK && K <= kMax
 
 template <int B>
 int launch_res_t(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
-                 double eta, const double2 *tw, double2 *F1, hipStream_t s)
+                 double eta, const double2 *tw, double2 *F1, hipStream_t s, const double2 *Gk = nullptr)
 {
     int NP = B * B;
     int np = pairs_per_chunk(g.K, NP);
@@ -676,14 +684,16 @@ int launch_res_t(const Geom &g, const double *eval, const double2 *evec, const d
     dim3 grid(g.K, nb, nch), blk(kThreads);
     // k-points per thread whose eigen data stays in registers (small B only)
     int jpt = (g.K + kThreads - 1) / kThreads;
-    if (B <= 4 && jpt == 1)
-        hipLaunchKernelGGL((k_resolvent_ky<B, 1>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1);
+    if (Gk)
+        hipLaunchKernelGGL((k_resolvent_ky<B, 0>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1, Gk);
+    else if (B <= 4 && jpt == 1)
+        hipLaunchKernelGGL((k_resolvent_ky<B, 1>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1, Gk);
     else if (B <= 4 && jpt == 2)
-        hipLaunchKernelGGL((k_resolvent_ky<B, 2>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1);
+        hipLaunchKernelGGL((k_resolvent_ky<B, 2>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1, Gk);
     else if (B <= 4 && jpt <= 4)
-        hipLaunchKernelGGL((k_resolvent_ky<B, 4>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1);
+        hipLaunchKernelGGL((k_resolvent_ky<B, 4>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1, Gk);
     else
-        hipLaunchKernelGGL((k_resolvent_ky<B, 0>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1);
+        hipLaunchKernelGGL((k_resolvent_ky<B, 0>), grid, blk, lds, s, g, eval, evec, omegas, E, EB, eta, tw, np, F1, Gk);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -718,7 +728,21 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
 int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, int E, const void *tw, void *F1,
                            bool fp32, hipStream_t s)
 {
-    if (!wave_path(g.K)) return CQPI_E_ARG;
+    if (!wave_path(g.K)) {
+        // other grids: the LDS-line pass (FP64 only)
+        if (fp32 || (size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
+        const double2 *gk = static_cast<const double2 *>(Gk);
+        const double2 *t = static_cast<const double2 *>(tw);
+        double2 *f = static_cast<double2 *>(F1);
+        switch (g.B) {
+        case 1: return launch_res_t<1>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+        case 2: return launch_res_t<2>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+        case 3: return launch_res_t<3>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+        case 4: return launch_res_t<4>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+        case 5: return launch_res_t<5>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+        default: return CQPI_E_ARG;
+        }
+    }
 #define GK_CASE(n)                                                                                             \
     case n:                                                                                                    \
         return fp32 ? launch_res_gk<float2, n>(g, static_cast<const float2 *>(Gk), omegas, E,                 \

@@ -7,8 +7,8 @@
 // The input file and the output map follow the reference driver:
 //   * key=value; lines, '#' comments, (a,b,...) lists, "string" values,
 //     a warning for every unused key (loadconfig, include/parser.h:24-199);
-//   * defaults and keys of main(), src/calcqpi.cc:133-273, for output=wannier
-//     with green=normal: tbfile, qpifile, lattice, oversamp, window, layers,
+//   * defaults and keys of main(), src/calcqpi.cc:133-273, for output=wannier:
+//     tbfile, qpifile, lattice, oversamp, window, layers,
 //     energies, eta, kpoints, fermi, scattering, phase, spin, scmodel,
 //     magscat, prearr, idlorbitalfile, logfile;
 //   * the orbital tiles come from the idl orbital file, duplicated for spin
@@ -26,9 +26,12 @@
 //     header names of :647-652; output=nomode makes no map;
 //   * dosfile, doslayers, dosenergies: the density of states as text lines
 //     "energy dos" (src/calcqpi.cc:336-391).
+//   * green=surface|bulk with stbfile and epserr: the surface or bulk
+//     Green's function of a layered model for every output (src/calcqpi.cc:
+//     203-208, :293-307, :367-375); the principal-layer warning as there.
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
-// Other output and green modes, spinarr, and analytic or XSF orbitals are
-// reported as unsupported.
+// Other output modes, spinarr, and analytic or XSF orbitals are reported as
+// unsupported.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -246,9 +249,15 @@ int main(int argc, char **argv)
         return 1;
     }
     if (lf.probe("fermi")) efermi = lf.num("fermi");
-    if (lf.probe("green") && lf.get("green") != "normal") {
-        std::cerr << "green=" << lf.get("green") << " is not supported by this build (normal only).\n";
-        return 1;
+    // Green's function: src/calcqpi.cc:129, :203-208 (anything else is normal)
+    int green = CQPI_GREEN_NORMAL;
+    double epserr = 1.0e-5;
+    string stbfile;
+    if (lf.probe("stbfile")) stbfile = lf.str("stbfile");
+    if (lf.probe("epserr")) epserr = lf.num("epserr");
+    if (lf.probe("green")) {
+        if (lf.get("green") == "surface") green = CQPI_GREEN_SURFACE;
+        else if (lf.get("green") == "bulk") green = CQPI_GREEN_BULK;
     }
     if (lf.probe("spin")) spin = lf.get("spin") == "true";
     if (lf.probe("scmodel")) scmodel = lf.get("scmodel") == "true";
@@ -328,6 +337,34 @@ int main(int argc, char **argv)
     if (check(cqpi_read_hr(tbfile.c_str(), scmodel, &hops, &nhops, &bands, tbname, sizeof tbname), "reading the TB model")) {
         std::cerr << "Error loading TB model from " << tbfile << ", exiting.\n";
         return 1;
+    }
+    // surface layer model and the principal-layer check (src/calcqpi.cc:293-307)
+    cqpi_hopping *shops = nullptr;
+    size_t nshops = 0;
+    auto principal = [](const cqpi_hopping *h, size_t nh) {
+        for (size_t i = 0; i < nh; i++)
+            if (std::fabs(h[i].rz) > 1 && (h[i].tre != 0.0 || h[i].tim != 0.0)) return false;
+        return true;
+    };
+    if (!stbfile.empty()) {
+        size_t sbands = 0;
+        char sname[1024];
+        *log << "Loading surface TB model " << stbfile << "\n";
+        if (check(cqpi_read_hr(stbfile.c_str(), scmodel, &shops, &nshops, &sbands, sname, sizeof sname),
+                  "reading the surface TB model") ||
+            sbands != bands) {
+            std::cerr << "Error loading surface TB model from " << stbfile << ", exiting.\n";
+            cqpi_free(hops);
+            cqpi_free(shops);
+            return 1;
+        }
+    }
+    if (green != CQPI_GREEN_NORMAL) {
+        if (!principal(hops, nhops))
+            *log << "Warning: Tight-binding model in file " << tbfile << " is not in principal layer form.\n";
+        if (shops && !principal(shops, nshops))
+            *log << "Warning: Surface tight-binding model in file " << stbfile
+                 << " is not in principal layer form.\n";
     }
     size_t maxbands = bands;
     if (scmodel) {
@@ -412,6 +449,10 @@ int main(int argc, char **argv)
                   << "\", \"threshold\": ";
         if (std::isnan(threshold)) std::cout << "null";
         else std::cout << threshold;
+        std::cout << ", \"green\": \""
+                  << (green == CQPI_GREEN_SURFACE ? "surface" : green == CQPI_GREEN_BULK ? "bulk" : "normal")
+                  << "\", \"epserr\": " << epserr << ", \"stbfile\": \"" << json_escape(stbfile)
+                  << "\", \"surface_hoppings\": " << nshops;
         std::cout << ", \"deltat\": " << deltat << ", \"etat\": " << etat << ", \"spectral\": " << spectral
                   << ", \"dosfile\": \"" << json_escape(dosfile) << "\", \"doslayers\": " << doslayers
                   << ", \"dosenergies\": [" << doslenergy << ", " << doshenergy << "], \"pos\": [";
@@ -424,6 +465,7 @@ int main(int argc, char **argv)
         for (double v : tiles) tsum += v;
         std::cout << "], \"tile_sum\": " << tsum << "}\n";
         cqpi_free(hops);
+        cqpi_free(shops);
         lf.warn_unused(std::cerr);
         return 0;
     }
@@ -435,6 +477,10 @@ int main(int argc, char **argv)
          << "         energy range         : (" << lenergy << "," << henergy << "), " << layers << " layers\n"
          << "         broadening parameter : " << eta << ", window " << window << ", oversamp " << oversamp << "\n"
          << "         contraction precision: " << precision << "\n";
+    if (green != CQPI_GREEN_NORMAL)
+        *log << "         Surface tight-binding model: " << stbfile << "\n"
+             << "         epserr               : " << epserr << "\n"
+             << "         Green's function     : " << (green == CQPI_GREEN_SURFACE ? "surface" : "bulk") << "\n";
 
     auto t0 = std::chrono::steady_clock::now();
     cqpi_ctx *ctx = nullptr;
@@ -477,7 +523,10 @@ int main(int argc, char **argv)
         V[2 * (i * bands + i) + 1] = vscat[i].imag();
     }
     int rc = check(cqpi_set_model(ctx, hops, nhops, efermi, scmodel), "cqpi_set_model");
+    if (!rc && green != CQPI_GREEN_NORMAL)
+        rc = check(cqpi_set_surface(ctx, green, epserr, shops, nshops), "cqpi_set_surface");
     cqpi_free(hops);
+    cqpi_free(shops);
     if (!rc && !dosfile.empty()) {
         // energies lb + k (hb - lb)/layers, k < layers (src/calcqpi.cc:348-356)
         const double step = (doshenergy - doslenergy) / (double)doslayers;

@@ -271,8 +271,9 @@ enum {
  * -/+ efermi diagonal) and the coupling to the next layer (rz < 0) are
  * decimated until the Frobenius norms of both couplings are <= epserr; shops
  * (NULL/0: the model's own hoppings) gives the surface layer's in-plane block
- * (stbfile=).  Call after cqpi_set_model.  bands <= 5, power-of-two k grid
- * in [64, 1024], a context made by cqpi_create; cqpi_run returns
+ * (stbfile=).  Call after cqpi_set_model; bands <= 5.  It applies to every
+ * output of the context: LDOS and Josephson maps, threshold-list maps, the
+ * SPF / USPF maps and cqpi_dos.  cqpi_run and cqpi_dos return
  * CQPI_E_CONVERGE if some k needs more than 500 steps.  A run with it cannot
  * feed cqpi_gradient.  mode CQPI_GREEN_NORMAL switches it off. */
 int cqpi_set_surface(cqpi_ctx *ctx, int mode, double epserr, const cqpi_hopping *shops, size_t nshops);

@@ -211,3 +211,60 @@ def test_driver_spectral_and_dos_match_oracle(tmp_path, gpu):
     for e, om in enumerate(wl.energies):
         ref = oracle.uspf(oracle.greens_k(ev, vec, om, wl.eta), 16, pos, pre)
         assert rm.rel_errors(data[:, :, e].T.ravel(), ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+def _layered_diag(E=2, K=48, n=8):
+    wl = synth.layered(3, K=K, n=n, E=E, seed=61, w=1, os_=2, eta=0.02)
+    wl.V = np.diag(np.diag(wl.V).real).astype(complex)
+    return wl
+
+
+def test_driver_green_keys(tmp_path):
+    """green / stbfile / epserr (src/calcqpi.cc:203-208) and the
+    principal-layer warning for terms beyond the next layer (:301-307)."""
+    wl = _layered_diag()
+    shops = np.array(wl.hops[wl.hops["rz"] == 0], synth.HOP_DTYPE)
+    far = np.array(wl.hops[:1], synth.HOP_DTYPE)
+    far["rz"] = -2.0
+    synth.write_hr(str(tmp_path / "surf.dat"), synth.Workload(
+        name="surface layer", bands=3, hops=np.concatenate([shops, far]), K=wl.K, n=wl.n, window=1, oversamp=2,
+        energies=wl.energies, eta=wl.eta, V=wl.V, tiles=wl.tiles))
+    p = synth.write_inputs(str(tmp_path), wl, extra={"green": "surface", "epserr": "1e-8",
+                                                     "stbfile": '"surf.dat"'})
+    cfg, err = _print_config(p)
+    assert cfg["green"] == "surface" and cfg["epserr"] == 1e-8 and cfg["stbfile"] == "surf.dat"
+    assert cfg["surface_hoppings"] == len(shops) + 1
+    assert "Surface tight-binding model in file surf.dat is not in principal layer form" in err
+    assert "Tight-binding model in file model.dat is not" not in err
+    assert "Unused" not in err
+    p = synth.write_inputs(str(tmp_path / "b"), wl, extra={"green": "bulk"})
+    cfg, err = _print_config(p)
+    assert cfg["green"] == "bulk" and cfg["epserr"] == 1e-5 and cfg["stbfile"] == ""
+    p = synth.write_inputs(str(tmp_path / "n"), wl, extra={"green": "other"})
+    cfg, err = _print_config(p)
+    assert cfg["green"] == "normal"
+
+
+@pytest.mark.gpu
+def test_driver_surface_green_matches_oracle(tmp_path, gpu):
+    wl = _layered_diag()
+    p = synth.write_inputs(str(tmp_path), wl, extra={"green": "bulk", "epserr": "1e-10", "dosfile": '"dos.dat"',
+                                                     "doslayers": "4", "dosenergies": "(-0.2,0.2)"})
+    print("case: driver run, green=bulk, layered 3-orbital model, K=48, n=8, E=2", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Green's function     : bulk" in r.stdout
+    data, hdr = synth.read_idl(str(tmp_path / "qpi.idl"))
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
+    for e in range(2):
+        om = oracle.getbias(wl.energies[0], wl.energies[-1], 2, e)
+        g, _ = oracle.surfacegreen(hops, 3, wl.efermi, wl.K, om, wl.eta, epserr=1e-10, surface=False)
+        gR = oracle.realspace(g)
+        ref = oracle.wannierldos(gR, oracle.tmatrix(wl.V, gR[0, 0]), wl.tiles, wl.n, 1, 2)
+        got = data[:, :, e].T.ravel()
+        assert rm.rel_errors(got, ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+    rows = np.loadtxt(str(tmp_path / "dos.dat"))
+    for w, d in rows:
+        g, _ = oracle.surfacegreen(hops, 3, wl.efermi, wl.K, w, wl.eta, epserr=1e-10, surface=False)
+        ref = oracle.dos(oracle.realspace(g))
+        assert abs(d - ref) < 1e-5 * abs(ref)

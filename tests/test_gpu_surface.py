@@ -1,7 +1,8 @@
 """GPU parity of the surface / bulk Green's function (green=surface|bulk,
 tmatrix::calcsurfacegreensfunction, src/qpi.cc:488-554) against the FP64
-oracle chain surfacegreen -> realspace -> tmatrix -> wannierldos, called
-through the C ABI (cqpi_set_surface + cqpi_run)."""
+oracle chain surfacegreen -> realspace -> tmatrix -> wannierldos (and the
+threshold-list, SPF / USPF and DOS outputs that use it), called through the
+C ABI (cqpi_set_surface + cqpi_run / cqpi_dos)."""
 import numpy as np
 import pytest
 
@@ -97,13 +98,70 @@ def test_surface_rejections(gpu):
             q.set_surface(7)
     finally:
         q.close()
-    wl24 = synth.layered(3, K=24, n=8, E=1, seed=52, w=1, os_=2, eta=0.02)
-    with cqpi.engine_for(wl24) as q:
+    wl6 = synth.square_norb(6, K=16, n=4, E=1, seed=52, w=1, os_=2)
+    with cqpi.engine_for(wl6) as q:
         with pytest.raises(cqpi.CqpiError):
-            q.set_surface("surface")  # k grid not a power of two in [64, 1024]
+            q.set_surface("surface")  # more than 5 bands
     with cqpi.engine_for(wl) as q:
         q.set_keep_windows(True)
         q.set_surface("bulk", 1e-10)
         q.run(wl.energies, wl.eta)
         with pytest.raises(cqpi.CqpiError):
             q.gradient(np.ones((1, wl.W * wl.W)))  # the adjoint differentiates the eigen path only
+
+
+def test_surface_general_grid(gpu):
+    """A k grid outside the power-of-two Stockham range (K = 48): the LDS-line
+    ky pass reads G(k) from the decimation workspace."""
+    wl = synth.layered(3, K=48, n=12, E=2, seed=53, w=1, os_=2, eta=0.02)
+    _assert_close(_run(wl, "surface", 1e-10), _oracle_maps(wl, True, 1e-10), TOL64)
+
+
+def test_surface_threshold_list(gpu):
+    wl = synth.layered(3, K=64, n=8, E=2, seed=54, w=1, os_=2, eta=0.02)
+    flags, counts = cqpi.make_threshold_list(wl.tiles, 1, 2, 0.05)
+    q = cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, flags, counts)
+    try:
+        q.set_model(wl.hops, wl.efermi, wl.scmodel)
+        q.set_impurity(wl.V)
+        q.set_surface("bulk", 1e-10)
+        got = q.run(wl.energies, wl.eta)
+    finally:
+        q.close()
+    for e, om in enumerate(wl.energies):
+        g, _ = oracle.surfacegreen(wl.hops, 3, wl.efermi, wl.K, om, wl.eta, epserr=1e-10, surface=False)
+        gR = oracle.realspace(g)
+        ref = oracle.wannierldoslist(gR, oracle.tmatrix(wl.V, gR[0, 0]), flags, counts, wl.n, 1, 2)
+        assert rm.rel_errors(got[e], ref)[0] < TOL64, e
+
+
+@pytest.mark.parametrize("mode", ["spf", "uspf"])
+def test_surface_spectral_maps(gpu, mode):
+    wl = synth.layered(3, K=32, n=32, E=2, seed=55, w=1, os_=1, eta=0.02)
+    pos = np.array([[0.0, 0.0], [0.5, 0.0], [0.0, 0.5]])
+    pre = np.array([1.0, 0.8, 1.2])
+    q = cqpi.AmdQPI.spectral(32, 32, 3, 3, False, pos, pre)
+    try:
+        q.set_model(wl.hops, wl.efermi, False)
+        q.set_surface("surface", 1e-10)
+        q.set_spectral(mode)
+        got = q.run(wl.energies, wl.eta)
+    finally:
+        q.close()
+    for e, om in enumerate(wl.energies):
+        g, _ = oracle.surfacegreen(wl.hops, 3, wl.efermi, 32, om, wl.eta, epserr=1e-10, surface=True)
+        ref = oracle.spf(g, 32) if mode == "spf" else oracle.uspf(g, 32, pos, pre)
+        assert rm.rel_errors(got[e], ref)[0] < TOL64, e
+
+
+@pytest.mark.parametrize("mode", ["surface", "bulk"])
+def test_surface_dos(gpu, mode):
+    wl = synth.layered(3, K=40, n=8, E=1, seed=56, w=1, os_=2, eta=0.03)
+    om = np.linspace(-0.4, 0.4, 5)
+    with cqpi.engine_for(wl) as q:
+        q.set_surface(mode, 1e-10)
+        got = q.dos(om, 0.03)
+    ref = np.array([oracle.dos(oracle.realspace(oracle.surfacegreen(wl.hops, 3, wl.efermi, 40, w, 0.03,
+                                                                     epserr=1e-10, surface=mode == "surface")[0]))
+                    for w in om])
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
