@@ -105,13 +105,16 @@ def cpu_baseline(wl, nsample):
     return info, np.array(maps)
 
 
-def time_steps(q, wl, steps, warmup, torch, stage_events=False):
+def time_steps(q, wl, steps, warmup, torch, stage_events=False, eigen=True):
     """Wall time per step (eigensolve + all energies) over `steps` steps after
     `warmup`, then the stage times of one more step.  stage_events: every
     step records stage events, which keeps each run on one stream (one launch
-    per stage: what a kernel-trace profile of the stages should see)."""
+    per stage: what a kernel-trace profile of the stages should see).
+    eigen=False: no eigensolve in the step (the surface Green's function does
+    not use it)."""
     def step():
-        q.compute_eigen()
+        if eigen:
+            q.compute_eigen()
         q.run(wl.energies, wl.eta, host=False)
 
     q.set_timing(stage_events)
@@ -135,7 +138,8 @@ def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
     (K=512, 51 energies, 128x128 cells, os=4, w=2), whole chain on the device:
     the Josephson map of a superconducting model (3 orbitals x particle-hole,
     6 bands; FP64 and FP32) and the threshold-list map (FP64) of the 3-orbital
-    workload with the driver's list rule at the given threshold."""
+    workload with the driver's list rule at the given threshold; and the LDOS
+    map from the surface Green's function of a layered 3-orbital model."""
     out = {}
     wlj = synth.bdg_square(3, K=512, n=128, E=51, seed=1006, w=2, os_=4, eta=0.002, elo=-0.1, ehi=0.1)
     jos = {"workload": "superconducting square lattice, 3 orbitals x particle-hole (6 bands), K=512, "
@@ -167,6 +171,19 @@ def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
         "threshold": threshold, "entries": int(len(flags)), "entries_dense": dense,
         "fp64": {"ms_per_step": dt * 1e3, "points_per_s": wl.points / dt,
                  "stages_ms": {k: v[0] for k, v in st.items()}}}
+    # surface Green's function of a layered model (green=surface, epserr 1e-5)
+    wls = synth.layered(3, K=512, n=128, E=51, seed=1007, w=2, os_=4, eta=0.002, elo=-0.1, ehi=0.1)
+    surf = {"workload": "3-orbital square-lattice layers coupled along z (blocks at R=(0,0,-1) and (1,0,-1)), "
+                        "K=512, 51 energies in [-0.1,0.1], eta=0.002, 128x128 cells, os=4, w=2; green=surface, "
+                        "epserr=1e-5"}
+    for prec in ("fp64", "mixed"):
+        q = cqpi.engine_for(wls, prec)
+        q.set_surface("surface", 1e-5)
+        dt, st = time_steps(q, wls, steps, warmup, torch, eigen=False)
+        surf[prec] = {"ms_per_step": dt * 1e3, "points_per_s": wls.points / dt,
+                      "stages_ms": {k: v[0] for k, v in st.items()}}
+        q.close()
+    out["surface_green"] = surf
     return out
 
 
@@ -342,7 +359,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
-    ap.add_argument("--no-variants", action="store_true", help="skip the Josephson and threshold-list maps")
+    ap.add_argument("--no-variants", action="store_true", help="skip the Josephson, threshold-list and surface Green's function maps")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--stage-events", action="store_true",
                     help="record stage events in every step (one stream, one launch per stage)")
