@@ -47,6 +47,9 @@ SIGNATURES = {
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_set_surface": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _sz]),
+    "cqpi_orbital_tile": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _sz, _sz, _dp]),
+    "cqpi_orbital_range": (C.c_double, [C.c_double, C.c_double]),
     "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
     "cqpi_create_spectral": (C.c_int, [_sz, _sz, _sz, _sz, C.c_int, _dp, _sz, _dp, _sz, C.POINTER(_vp)]),
     "cqpi_spf": (C.c_int, [_vp, C.POINTER(_vp), _sz]),
@@ -452,6 +455,17 @@ class AmdQPI:
 
     def __exit__(self, *a):
         self.close()
+
+
+def orbital_tile(name, radius=0.5, height=5.0, xpos=0.0, ypos=0.0, theta=0.0, prefact=1.0, window=2, oversamp=4):
+    """An analytic orbital sampled on the Wannier tile grid (cqpi_orbital_tile):
+    T x T, T = (2 window + 1) oversamp; theta in radians."""
+    T = (2 * window + 1) * oversamp
+    out = np.zeros((T, T))
+    _check(lib().cqpi_orbital_tile(name.encode(), float(radius), float(height), float(xpos), float(ypos),
+                                   float(theta), float(prefact), int(window), int(oversamp),
+                                   out.ctypes.data_as(_dp)))
+    return out
 
 
 def make_threshold_list(tiles, window, oversamp, threshold, maxband=0, spin=False):

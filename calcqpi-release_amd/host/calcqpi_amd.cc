@@ -26,12 +26,16 @@
 //     header names of :647-652; output=nomode makes no map;
 //   * dosfile, doslayers, dosenergies: the density of states as text lines
 //     "energy dos" (src/calcqpi.cc:336-391).
+//   * orbitals=(s,px,...) with radius, angle, anglearr, zheight, pos[i] and
+//     prearr: analytic orbitals sampled on the tile grid (src/calcqpi.cc:
+//     519-575); window=0 sizes the window from the radius; wffile writes
+//     the tiles (:602-618);
 //   * green=surface|bulk with stbfile and epserr: the surface or bulk
 //     Green's function of a layered model for every output (src/calcqpi.cc:
 //     203-208, :293-307, :367-375); the principal-layer warning as there.
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
-// Other output modes, spinarr, and analytic or XSF orbitals are reported as
-// unsupported.
+// Other output modes, spinarr, and Wannier90/XSF orbital files are reported
+// as unsupported.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -391,17 +395,16 @@ int main(int argc, char **argv)
         if (lf.probe("prearr")) sprearr = lf.vec("prearr");
         sprearr.resize(maxbands, 1.0);
     }
-    // orbitals from the idl orbital file: src/calcqpi.cc:483-518
-    const size_t T = (size_t)(2 * window + 1) * oversamp;
-    vector<double> tiles(maxbands * T * T, 0.0);
-    if (spectral || nomap) {
-        // the spectral maps and the DOS use no Wannier tiles
-    } else if (!lf.probe("idlorbitalfile")) {
-        std::cerr << "This build takes the Wannier tiles from an idlorbitalfile only.\n";
-        return 1;
-    }
-    {
+    // Wannier tiles (src/calcqpi.cc:483-617): from the idl orbital file, or
+    // analytic orbitals sampled on the tile grid (orbitals=, radius, angle,
+    // anglearr, zheight, pos[i], prearr; window = 0 sizes the window)
+    // (read in every output mode, as the reference does; the spectral maps and
+    // the DOS then leave them unused)
+    vector<vector<double>> wfs;
+    if (lf.probe("idlorbitalfile")) {
+        const size_t T = (size_t)(2 * window + 1) * oversamp;
         string orbfile = lf.str("idlorbitalfile");
+        *log << "Using IDL orbital file for vacuum overlap.\nLoading wave function file " << orbfile << ".\n";
         IdlMap orb;
         if (!read_idl(orbfile, orb)) {
             std::cerr << "Error loading orbitals from file " << orbfile << ", exiting.\n";
@@ -414,7 +417,6 @@ int main(int argc, char **argv)
         vector<double> prearr;
         if (lf.probe("prearr")) prearr = lf.vec("prearr");
         prearr.resize(std::max(orb.sz, maxbands), 1.0);
-        vector<vector<double>> wfs;
         for (size_t k = 0; k < orb.sz; k++) {
             vector<double> t(T * T);
             for (size_t i = 0; i < T; i++)
@@ -422,13 +424,119 @@ int main(int argc, char **argv)
             wfs.push_back(t);
         }
         if ((maxbands >> 1) == orb.sz && spin) {
-            size_t orbs = wfs.size();
-            for (size_t k = 0; k < orbs; k++) wfs.push_back(wfs[k]);
-        }
-        for (size_t o = 0; o < maxbands && o < wfs.size(); o++)
-            std::copy(wfs[o].begin(), wfs[o].end(), tiles.begin() + o * T * T);
-        if (wfs.size() < maxbands)
+            *log << "Found half as many orbitals as needed in spin-polarized calculation, duplicating wave "
+                    "function set.\n";
+            for (size_t k = 0; k < orb.sz; k++) wfs.push_back(wfs[k]);
+        } else if (maxbands > orb.sz) {
             *log << "Warning: Fewer number of orbitals in " << orbfile << " than there are bands, padding with zeros.\n";
+        }
+    } else if (lf.probe("orbitals")) {
+        vector<string> names = lf.list("orbitals");
+        double radius = 0.5, angle = 0.0, zheight = 5.0;
+        if (lf.probe("radius")) radius = lf.num("radius");
+        if (lf.probe("angle")) angle = lf.num("angle");
+        if (lf.probe("zheight")) zheight = lf.num("zheight");
+        *log << "Using gaussian orbitals for vacuum overlap.\n    Parameters:\n             Radius:         " << radius
+             << "\n             Rotation angle: " << angle << "\n             Height:         " << zheight << "\n";
+        vector<double> anglearr;
+        if (lf.probe("anglearr")) anglearr = lf.vec("anglearr");
+        anglearr.resize(names.size(), 0.0);
+        vector<double> prearr;
+        if (lf.probe("prearr")) prearr = lf.vec("prearr");
+        prearr.resize(names.size(), 1.0);
+        struct Orb {
+            string name;
+            double height, x, y, theta, pre;
+        };
+        vector<Orb> orbs;
+        for (size_t i = 0; i < names.size(); i++) {
+            const string key = "pos[" + std::to_string(i) + "]";
+            const double th = (angle + anglearr[i]) * M_PI / 180.0;
+            if (lf.probe(key)) {
+                vector<double> p = lf.vec(key);
+                p.resize(3, 0.0);
+                orbs.push_back({names[i], zheight - p[2], p[0], p[1], th, prearr[i]});
+            } else {
+                orbs.push_back({names[i], zheight, 0.0, 0.0, th, prearr[i]});
+            }
+        }
+        if ((maxbands >> 1) == names.size() && spin) {
+            *log << "Found half as many orbitals as needed in spin-polarized calculation, duplicating wave "
+                    "function set.\n";
+            for (size_t k = 0; k < names.size(); k++) orbs.push_back(orbs[k]);
+        } else if (names.size() < maxbands) {
+            *log << "Warning: Fewer number of orbitals specified than there are bands, padding with zeros.\n";
+            for (size_t i = names.size(); i < maxbands; i++) orbs.push_back({"z", zheight, 0.0, 0.0, 0.0, 0.0});
+        }
+        *log << "Parameters for Wannier functions:\n";
+        for (size_t i = 0; i < orbs.size(); i++)
+            *log << "Orbital " << i << ": " << orbs[i].name << "@(" << orbs[i].x << "," << orbs[i].y << "), height "
+                 << orbs[i].height << ", theta " << orbs[i].theta * 180.0 / M_PI << "deg \n";
+        if (window) {
+            const double val = radius * std::sqrt(std::log(100.0) + std::fabs(zheight));
+            *log << "Recommended window size for r=" << radius << ": " << std::ceil(val * val - zheight * zheight)
+                 << "\nchosen window size: " << window << "\n";
+        } else {
+            window = (long)std::ceil(cqpi_orbital_range(radius, 0.01));
+            *log << "Using recommended window size for r=" << radius << ": " << window << "\n";
+        }
+        const size_t T = (size_t)(2 * window + 1) * oversamp;
+        *log << "Precalculating wave functions.\n";
+        for (const Orb &o : orbs) {
+            vector<double> t(T * T);
+            if (cqpi_orbital_tile(o.name.c_str(), radius, o.height, o.x, o.y, o.theta, o.pre, (size_t)window,
+                                  (size_t)oversamp, t.data())) {
+                std::cerr << "Unknown orbital type " << o.name << ", exiting.\n";
+                return 1;
+            }
+            wfs.push_back(t);
+        }
+        *log << "Done.\n";
+        // largest boundary value relative to each tile's maximum (checkwavefunctions, src/calcqpi.cc:118-127)
+        double worst = 0.0;
+        for (const vector<double> &t : wfs) {
+            double mx = 0.0, mb = 0.0;
+            for (size_t i = 0; i < T; i++)
+                for (size_t j = 0; j < T; j++) {
+                    const double v = std::fabs(t[i * T + j]);
+                    mx = std::max(mx, v);
+                    if (i == 0 || j == 0 || i == T - 1 || j == T - 1) mb = std::max(mb, v);
+                }
+            if (mx > 0.0) worst = std::max(worst, mb / mx);
+        }
+        *log << "Maximum rel. value at boundary: " << worst << "\n";
+        if (worst > 0.05) *log << "Warning: boundary too close, use larger window size!\n";
+    } else if (!(spectral || nomap)) {
+        std::cerr << "This build takes the Wannier tiles from an idlorbitalfile or from analytic orbitals "
+                     "(orbitals=); orbitalfiles (Wannier90/XSF) are not supported.\n";
+        return 1;
+    }
+    const size_t T = (size_t)(2 * window + 1) * oversamp;
+    vector<double> tiles(maxbands * T * T, 0.0);
+    for (size_t o = 0; o < maxbands && o < wfs.size(); o++)
+        std::copy(wfs[o].begin(), wfs[o].end(), tiles.begin() + o * T * T);
+    if (lf.probe("wffile") && !wfs.empty() && !print_only) {
+        // the tiles as an idl file (src/calcqpi.cc:602-618)
+        IdlMap wf;
+        wf.sx = wf.sy = T;
+        wf.sz = wfs.size();
+        wf.data.assign(T * T * wfs.size(), 0.0);
+        for (size_t l = 0; l < wfs.size(); l++)
+            for (size_t i = 0; i < T; i++)
+                for (size_t j = 0; j < T; j++) wf.data[i + j * T + l * T * T] = wfs[l][i * T + j];
+        std::ostringstream nm;
+        nm << "Wannier function file for " << tbfile << " with window=" << window << " and oversamp=" << oversamp;
+        wf.name = nm.str();
+        wf.hdr[0] = wf.hdr[1] = 1.0;  // idl(x, y, z) defaults: scan range 1 x 1, offset and bias 0
+        const time_t now = time(nullptr);
+        wf.date = ctime(&now);
+        if (!wf.date.empty() && wf.date.back() == '\n') wf.date.pop_back();
+        const string wfname = lf.str("wffile");
+        *log << "Writing wave function file to " << wfname << "\n";
+        if (!write_idl(wfname, wf)) {
+            std::cerr << "Error writing " << wfname << "\n";
+            return 1;
+        }
     }
     vector<double> omegas(layers);
     for (long k = 0; k < layers; k++)

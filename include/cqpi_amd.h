@@ -388,6 +388,22 @@ int cqpi_read_hr(const char *path, int scmodel, cqpi_hopping **hops, size_t *nho
                  size_t *bands, char *name, size_t namelen);
 void cqpi_free(void *p);
 
+/* One analytic orbital of the driver's orbitals= mode (src/calcqpi.cc:519-575)
+ * sampled on the Wannier tile grid: tile[i*T + j], T = (2 window + 1) oversamp,
+ * at x = i/oversamp - window - 1/2 (y likewise with j), as
+ * wannierfunctions::precalculate (src/wannierfunctions.cc:436-442) with the
+ * orbital wannierfunctions(name, radius, height, xpos, ypos, theta, 0, prefact)
+ * (src/wannierfunctions.h:56; waves :21-124, shift and rotation :34-40).
+ * name: s, px, py, pz, dx2, dxy, dxz, dyz, dr2, fy3x2, fxyz, fyz2, fz3, fxz2,
+ * fzx2, fxx2, n (4-fold lobe), z (zero).  theta in radians.  Host only;
+ * CQPI_E_ARG for an unknown name or radius <= 0. */
+int cqpi_orbital_tile(const char *name, double radius, double height, double xpos, double ypos, double theta,
+                      double prefact, size_t window, size_t oversamp, double *tile);
+/* radius sqrt(ln(1/p)): where exp(-r^2/radius^2) falls to p
+ * (wannierfunctions::getrange, src/wannierfunctions.cc:382-384); the driver
+ * sizes window = 0 (auto) from it with p = 0.01. */
+double cqpi_orbital_range(double radius, double p);
+
 #ifdef __cplusplus
 }
 #endif

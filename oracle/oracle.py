@@ -228,6 +228,60 @@ def surfacegreen(hops, b, efermi, K, omega, eta, epserr=1e-5, surface=True, shop
     return g, it
 
 
+_ORB_ANG = {
+    # real harmonic times r^l, then divided by r^l: wannierfunctions::swave ...
+    # fxx2wave, src/wannierfunctions.cc:21-124 (polynomial in x, y, z over r^l)
+    "px": (1, lambda x, y, z: np.sqrt(3 / 4 / np.pi) * x),
+    "py": (1, lambda x, y, z: np.sqrt(3 / 4 / np.pi) * y),
+    "pz": (1, lambda x, y, z: np.sqrt(3 / 4 / np.pi) * z),
+    "dx2": (2, lambda x, y, z: np.sqrt(15 / 4 / np.pi) * (x * x - y * y) / 2),
+    "dxy": (2, lambda x, y, z: np.sqrt(15 / 4 / np.pi) * x * y),
+    "dxz": (2, lambda x, y, z: np.sqrt(15 / 4 / np.pi) * x * z),
+    "dyz": (2, lambda x, y, z: np.sqrt(15 / 4 / np.pi) * y * z),
+    "dr2": (2, lambda x, y, z: np.sqrt(5 / np.pi) * (2 * z * z - x * x - y * y) / 4),
+    "fy3x2": (3, lambda x, y, z: np.sqrt(35 / 2 / np.pi) * y * (3 * x * x - y * y) / 4),
+    "fxyz": (3, lambda x, y, z: np.sqrt(105 / 4 / np.pi) * x * y * z),
+    "fyz2": (3, lambda x, y, z: np.sqrt(21 / 2 / np.pi) * y * (4 * z * z - x * x - y * y) / 4),
+    "fz3": (3, lambda x, y, z: np.sqrt(7 / np.pi) * z * (2 * z * z - 3 * x * x - 3 * y * y) / 4),
+    "fxz2": (3, lambda x, y, z: np.sqrt(21 / 2 / np.pi) * x * (4 * z * z - x * x - y * y) / 4),
+    "fzx2": (3, lambda x, y, z: np.sqrt(105 / np.pi) * z * (x * x - y * y) / 4),
+    "fxx2": (3, lambda x, y, z: np.sqrt(35 / 2 / np.pi) * x * (x * x - 3 * y * y) / 4),
+}
+
+
+def orbital_value(name, radius, x, y, z, prefact=1.0, symmetry=4.0):
+    """wannierfunctions::<name>wave at (x, y, z) in the orbital's frame
+    (src/wannierfunctions.cc:21-124)."""
+    x, y, z = np.broadcast_arrays(*(np.asarray(a, np.float64) for a in (x, y, z)))
+    r = np.sqrt(x * x + y * y + z * z)
+    radial = prefact * np.exp(-r / radius)
+    if name == "z":
+        return np.zeros_like(r)
+    if name == "s":
+        return radial / np.sqrt(4 * np.pi)
+    if name == "n":
+        c = np.cos(symmetry * np.arctan2(x, y) / 2)
+        return radial * c * c * r / radius
+    l, f = _ORB_ANG[name]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v = radial * f(x, y, z) / r ** l
+    return np.where(r == 0.0, 0.0, v)
+
+
+def orbital_tile(name, radius, height, xpos, ypos, theta, prefact, window, oversamp):
+    """wannierfunctions::precalculate (src/wannierfunctions.cc:436-442) of the
+    orbital wannierfunctions(name, radius, height, xpos, ypos, theta):
+    getwave(x, y) with z = 0, shifted by (xpos, ypos, height), rotated by theta
+    about z unless s (src/wannierfunctions.cc:126-222, .h:34-40)."""
+    T = (2 * window + 1) * oversamp
+    g = np.arange(T) / oversamp - window - 0.5
+    x, y = np.meshgrid(g - xpos, g - ypos, indexing="ij")
+    z = np.full_like(x, -height)
+    if theta != 0.0 and name != "s":
+        x, y = x * np.cos(theta) + y * np.sin(theta), -x * np.sin(theta) + y * np.cos(theta)
+    return orbital_value(name, radius, x, y, z, prefact)
+
+
 def spf(g0k, n, maxband=0):
     """Spectral function map (src/qpihipkernel.cc:317-329): n x n, x + y*n."""
     K, _, b, _ = g0k.shape

@@ -268,3 +268,54 @@ def test_driver_surface_green_matches_oracle(tmp_path, gpu):
         g, _ = oracle.surfacegreen(hops, 3, wl.efermi, wl.K, w, wl.eta, epserr=1e-10, surface=False)
         ref = oracle.dos(oracle.realspace(g))
         assert abs(d - ref) < 1e-5 * abs(ref)
+
+
+def _orbital_inputs(dirpath, wl, window="0", extra=None):
+    """calcqpi.in with analytic orbitals in place of the idl orbital file."""
+    p = synth.write_inputs(dirpath, wl, extra=extra)
+    lines = [ln for ln in open(p).read().splitlines() if not ln.startswith(("idlorbitalfile", "window"))]
+    lines += ["orbitals=(s,px,dxy);", "radius=0.6;", "angle=10;", "anglearr=(0,5,0);", "zheight=3;",
+              "pos[1]=(0.25,0.0,0.5);", "prearr=(1.0,2.0,1.0);", "window=%s;" % window, 'wffile="wf.idl";']
+    open(p, "w").write("\n".join(lines) + "\n")
+    return p
+
+
+def _oracle_orbital_tiles(window, os_):
+    th = np.deg2rad([10.0, 15.0, 10.0])
+    spec = [("s", 3.0, 0.0, 0.0, 1.0), ("px", 2.5, 0.25, 0.0, 2.0), ("dxy", 3.0, 0.0, 0.0, 1.0)]
+    return np.array([oracle.orbital_tile(n, 0.6, h, x, y, t, pre, window, os_)
+                     for (n, h, x, y, pre), t in zip(spec, th)])
+
+
+def test_driver_analytic_orbitals_keys(tmp_path):
+    """orbitals= mode (src/calcqpi.cc:519-575): window=0 takes
+    ceil(radius sqrt(ln 100)) = 2; the tiles are the oracle's orbitals."""
+    wl = synth.square_3orb(K=16, n=8, E=3)
+    cfg, err = _print_config(_orbital_inputs(str(tmp_path), wl))
+    assert cfg["window"] == 2
+    ref = _oracle_orbital_tiles(2, wl.oversamp)
+    assert abs(cfg["tile_sum"] - ref.sum()) < 1e-9 * np.abs(ref).sum()
+    assert "Unused" not in err
+    cfg, err = _print_config(_orbital_inputs(str(tmp_path / "w1"), wl, window="1"))
+    assert cfg["window"] == 1 and "chosen window size: 1" in err
+    assert "Warning: boundary too close" in err
+
+
+@pytest.mark.gpu
+def test_driver_analytic_orbitals_match_oracle(tmp_path, gpu):
+    wl = synth.square_3orb(K=32, n=8, E=2)
+    p = _orbital_inputs(str(tmp_path), wl)
+    print("case: driver run, analytic orbitals, window auto, K=32, n=8, E=2", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    tiles = _oracle_orbital_tiles(2, wl.oversamp)
+    wf, hdr = synth.read_idl(str(tmp_path / "wf.idl"))
+    assert wf.shape == (20, 20, 3) and hdr["name"].startswith("Wannier function file for model.dat with window=2")
+    assert np.max(np.abs(np.transpose(wf, (2, 0, 1)) - tiles)) < 1e-6 * np.max(np.abs(tiles))  # float32 file
+    data, _ = synth.read_idl(str(tmp_path / "qpi.idl"))
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
+    wl2 = synth.Workload(name=wl.name, bands=3, hops=hops, K=wl.K, n=wl.n, window=2, oversamp=wl.oversamp,
+                         energies=wl.energies, eta=wl.eta, V=wl.V, tiles=tiles)
+    ref = oracle.cldos_run(wl2)
+    for e in range(2):
+        assert rm.rel_errors(data[:, :, e].T.ravel(), ref[e].astype(np.float32).astype(np.float64))[0] < 1e-6
