@@ -50,6 +50,7 @@ SIGNATURES = {
     "cqpi_orbital_tile": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _sz, _sz, _dp]),
     "cqpi_orbital_range": (C.c_double, [C.c_double, C.c_double]),
+    "cqpi_bands_histogram": (C.c_int, [_vp, _sz, _sz, C.c_double, C.c_double, _sz, _dp]),
     "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
     "cqpi_create_spectral": (C.c_int, [_sz, _sz, _sz, _sz, C.c_int, _dp, _sz, _dp, _sz, C.POINTER(_vp)]),
     "cqpi_spf": (C.c_int, [_vp, C.POINTER(_vp), _sz]),
@@ -312,6 +313,13 @@ class AmdQPI:
         """cqpi_run computes the Josephson map (tip gap deltat, tip broadening
         etat; NaN: the run's eta) instead of the LDOS map."""
         _check(lib().cqpi_set_josephson(self._ctx, int(bool(on)), float(deltat), float(etat)))
+
+    def bands_histogram(self, pix, oversamp, le, ue, layers):
+        """Band-structure histogram (bsfile=): pix x pix x layers, [x, y, layer]."""
+        out = np.zeros(layers * pix * pix)
+        _check(lib().cqpi_bands_histogram(self._ctx, int(pix), int(oversamp), float(le), float(ue), int(layers),
+                                          out.ctypes.data_as(_dp)))
+        return out.reshape(layers, pix, pix).transpose(2, 1, 0)
 
     def set_surface(self, mode="surface", epserr=1e-5, shops=None):
         """cqpi_run uses the surface ("surface") or bulk ("bulk") Green's

@@ -1256,6 +1256,29 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     return 0;
 }
 
+int cqpi_bands_histogram(cqpi_ctx *c, size_t pix, size_t oversamp, double le, double ue, size_t layers,
+                         double *hist)
+{
+    g_err.clear();
+    if (!c || !hist) return fail(CQPI_E_ARG, "NULL argument");
+    if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
+    if (!pix || !oversamp || pix * oversamp < 2 || pix * oversamp > 65536 || layers < 2 || !(ue > le))
+        return fail(CQPI_E_ARG, "histogram needs pix*oversamp in [2, 65536], layers >= 2 and ue > le");
+    const size_t n = pix * pix * layers;
+    double *d = nullptr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMalloc((void **)&d, n * sizeof(double)));
+    int r = hipMemset(d, 0, n * sizeof(double)) == hipSuccess ? 0 : CQPI_E_HIP;
+    if (!r)
+        r = launch_bands_hist(c->g.B, c->d_groups, c->ngroups, c->d_tmat, c->efermi, c->scmax, (int)pix,
+                              (int)oversamp, le, ue, (int)layers, d, c->stream);
+    if (!r) r = hipStreamSynchronize(c->stream) == hipSuccess ? 0 : CQPI_E_HIP;
+    if (!r) r = hipMemcpy(hist, d, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? 0 : CQPI_E_HIP;
+    (void)hipFree(d);
+    if (r) return fail(r, "band-structure histogram failed");
+    return 0;
+}
+
 int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *shops, size_t nshops)
 {
     g_err.clear();

@@ -146,6 +146,9 @@ namespace cqpi {
 
 int launch_eigen(int B, const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw,
                  double efermi, int scmax, int K, double *eval, double2 *evec, hipStream_t s);
+// band-structure histogram hist[(x + y pix) + layer pix^2] (zeroed by the caller)
+int launch_bands_hist(int B, const DevRGroup *groups, int ngroups, const double2 *tmat, double efermi, int scmax,
+                      int pix, int os, double le, double ue, int layers, double *hist, hipStream_t s);
 
 // resolvent G0(k,w) from eigen data fused with the transform along ky; writes
 // F1 (the Ry window of every kx row).  omegas: device array of the chunk.

@@ -325,6 +325,53 @@ __global__ void __launch_bounds__(256) k_eigen_g(const DevRGroup *__restrict__ g
     }
 }
 
+// Band-structure histogram (tightbind::calcbst3dhisthr, src/tightbinding.cc:
+// 1158-1193, ux = uy = 1, all bands): the eigenvalues of H(k) at the
+// (pix os)^2 points k = (io/(pix os - 1) - 1/2, jo/(pix os - 1) - 1/2), each
+// added to pixel (io/os, jo/os) of the layers nearest to it as a Lorentzian
+// of full width 2 (ue - le)/layers.  One thread per k; hist[(x + y pix) +
+// layer pix^2] accumulates with atomics (the order of the sums is free).
+template <int B>
+__global__ void __launch_bounds__(128) k_bands_hist(const DevRGroup *__restrict__ groups, int ngroups,
+                                                    const double2 *__restrict__ tmat, double efermi, int scmax,
+                                                    int pix, int os, double le, double ue, int layers,
+                                                    double *__restrict__ hist)
+{
+    const long ospix = (long)pix * os;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ospix * ospix) return;
+    const int io = (int)(idx / ospix), jo = (int)(idx - (long)io * ospix);
+    const double kx = (double)io / (double)(ospix - 1) - 0.5, ky = (double)jo / (double)(ospix - 1) - 0.5;
+    double2 A[B * B];
+    double2 V[B * B];
+#pragma unroll
+    for (int r = 0; r < B * B; r++) A[r] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < B; r++) A[r * B + r].x = (r < scmax) ? -efermi : efermi;
+    for (int g = 0; g < ngroups; g++) {
+        const DevRGroup G = groups[g];
+        double sn, cs;
+        sincos(2.0 * M_PI * (G.rx * kx + G.ry * ky), &sn, &cs);
+        const double2 e = make_double2(cs, sn);
+        const double2 *T = tmat + (size_t)g * B * B;
+        for (int r = 0; r < B * B; r++) A[r] = cadd(A[r], cmul(T[r], e));
+    }
+    jacobi<B>(A, V);
+    const double bin = (ue - le) / (double)(layers - 1), w = 2.0 * (ue - le) / (double)layers;
+    double *px = hist + (long)(io / os) + (long)(jo / os) * pix;
+    const long plane = (long)pix * pix;
+    for (int n = 0; n < B; n++) {
+        const double ev = A[n * B + n].x;
+        if (!(ev > le - 2.0 * bin && ev < ue + 2.0 * bin)) continue;
+        const int lay = (int)round((ev - le) / (ue - le) * (double)(layers - 1));
+        for (int nl = lay - 1; nl <= lay + 1; nl++) {
+            if (nl < 0 || nl >= layers) continue;
+            const double x = (ue - le) * (double)nl / (double)(layers - 1) + le - ev;
+            atomicAdd(px + nl * plane, w / 2.0 / (x * x + w * w / 4.0) / M_PI);
+        }
+    }
+}
+
 template <int B>
 int launch_eigen_t(const DevRGroup *groups, int ngroups, const double2 *tmat, const double2 *tw, double efermi,
                    int scmax, int K, double *eval, double2 *evec, hipStream_t s)
@@ -358,6 +405,26 @@ int launch_eigen(int B, const DevRGroup *groups, int ngroups, const double2 *tma
 #undef CASE
     default: return CQPI_E_ARG;
     }
+}
+
+int launch_bands_hist(int B, const DevRGroup *groups, int ngroups, const double2 *tmat, double efermi, int scmax,
+                      int pix, int os, double le, double ue, int layers, double *hist, hipStream_t s)
+{
+    if (pix < 1 || os < 1 || (long)pix * os < 2 || layers < 2 || !(ue > le)) return CQPI_E_ARG;
+    const long n = (long)pix * os * pix * os;
+    const dim3 grid((unsigned)((n + 127) / 128));
+    switch (B) {
+#define CASE(b)                                                                                                 \
+    case b:                                                                                                     \
+        hipLaunchKernelGGL(k_bands_hist<b>, grid, dim3(128), 0, s, groups, ngroups, tmat, efermi, scmax, pix, os, \
+                           le, ue, layers, hist);                                                               \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+        CASE(11) CASE(12) CASE(14) CASE(16)
+#undef CASE
+    default: return CQPI_E_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
 }  // namespace cqpi

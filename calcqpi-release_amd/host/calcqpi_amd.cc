@@ -30,12 +30,15 @@
 //     prearr: analytic orbitals sampled on the tile grid (src/calcqpi.cc:
 //     519-575); window=0 sizes the window from the radius; wffile writes
 //     the tiles (:602-618);
+//   * bsfile, bslattice, bslayers, bsenergies, bsoversamp: the band-structure
+//     histogram as an idl file (src/calcqpi.cc:308-326);
 //   * green=surface|bulk with stbfile and epserr: the surface or bulk
 //     Green's function of a layered model for every output (src/calcqpi.cc:
 //     203-208, :293-307, :367-375); the principal-layer warning as there.
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
 // Other output modes, spinarr, and Wannier90/XSF orbital files are reported
 // as unsupported.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -156,6 +159,67 @@ struct Config {
 // ---------------------------------------------------------------------------
 // idl files (src/idl.cc:50-95)
 // ---------------------------------------------------------------------------
+// Wall and process-CPU time per named section, sections of one name summed,
+// printed as the reference's timing summary (recordtimings,
+// include/recordtimings.h:26-113).
+class SectionTimes {
+    struct Entry {
+        string name;
+        double real, cpu;
+    };
+    vector<Entry> entries_;
+    double last_real_, last_cpu_;
+    static double now(clockid_t id)
+    {
+        timespec t;
+        clock_gettime(id, &t);
+        return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+    }
+    static string span(double sec)
+    {
+        const long long s = (long long)sec;
+        char buf[64];
+        snprintf(buf, sizeof buf, "%2lldd, %2lldh, %2lldm, %2llds%7lld\u03bcs", s / 86400, (s % 86400) / 3600,
+                 (s % 3600) / 60, s % 60, (long long)((sec - (double)s) * 1e6));
+        return buf;
+    }
+
+  public:
+    SectionTimes() : last_real_(now(CLOCK_REALTIME)), last_cpu_(now(CLOCK_PROCESS_CPUTIME_ID)) {}
+    void mark(const string &name)
+    {
+        const double r = now(CLOCK_REALTIME), c = now(CLOCK_PROCESS_CPUTIME_ID);
+        auto it = std::find_if(entries_.begin(), entries_.end(), [&](const Entry &e) { return e.name == name; });
+        if (it == entries_.end()) entries_.push_back({name, r - last_real_, c - last_cpu_});
+        else {
+            it->real += r - last_real_;
+            it->cpu += c - last_cpu_;
+        }
+        last_real_ = r;
+        last_cpu_ = c;
+    }
+    void print(std::ostream &os) const
+    {
+        double tr = 0.0, tc = 0.0;
+        for (const Entry &e : entries_) {
+            tr += e.real;
+            tc += e.cpu;
+        }
+        const string rule(114, '-');
+        os << "Summary of timings: \n"
+           << "     Section                            CPU time                               Real time\n"
+           << " " << rule << "\n";
+        for (const Entry &e : entries_) {
+            char line[256];
+            snprintf(line, sizeof line, "     %-33s %s (%5.2g%%)   %s (%5.2g%%)", e.name.c_str(), span(e.cpu).c_str(),
+                     tc > 0 ? 100.0 * e.cpu / tc : 0.0, span(e.real).c_str(), tr > 0 ? 100.0 * e.real / tr : 0.0);
+            os << line << "\n";
+        }
+        os << " " << rule << "\n"
+           << "     Total time elapsed                " << span(tc) << "            " << span(tr) << "\n";
+    }
+};
+
 struct IdlMap {
     string name, date;
     size_t sx = 0, sy = 0, sz = 0;
@@ -591,6 +655,7 @@ int main(int argc, char **argv)
              << "         Green's function     : " << (green == CQPI_GREEN_SURFACE ? "surface" : "bulk") << "\n";
 
     auto t0 = std::chrono::steady_clock::now();
+    SectionTimes timings;
     cqpi_ctx *ctx = nullptr;
     if (spectral || nomap) {
         // spectral maps (and the DOS) from a spectral-function context
@@ -635,6 +700,43 @@ int main(int argc, char **argv)
         rc = check(cqpi_set_surface(ctx, green, epserr, shops, nshops), "cqpi_set_surface");
     cqpi_free(hops);
     cqpi_free(shops);
+    if (!rc && lf.probe("bsfile")) {
+        // band-structure histogram: src/calcqpi.cc:308-326
+        long bslattice = 201, bslayers = 21, bsoversamp = 8;
+        double bslenergy = -0.1, bshenergy = 0.1;
+        const string bsfile = lf.str("bsfile");
+        if (lf.probe("bslayers")) bslayers = lf.integer("bslayers");
+        if (lf.probe("bsenergies")) {
+            vector<double> en = lf.vec("bsenergies");
+            if (en.size() >= 2) {
+                bslenergy = en[0];
+                bshenergy = en[1];
+            }
+        }
+        if (lf.probe("bslattice")) bslattice = lf.integer("bslattice");
+        if (lf.probe("bsoversamp")) bsoversamp = lf.integer("bsoversamp");
+        *log << "Calculating band structure histogram in energy range (" << bslenergy << "," << bshenergy << ") with "
+             << bslayers << " layers and oversampling " << bsoversamp << ".\n";
+        IdlMap bs;
+        bs.sx = bs.sy = (size_t)bslattice;
+        bs.sz = (size_t)bslayers;
+        bs.data.assign(bs.sx * bs.sy * bs.sz, 0.0);
+        rc = check(cqpi_bands_histogram(ctx, (size_t)bslattice, (size_t)bsoversamp, bslenergy, bshenergy,
+                                        (size_t)bslayers, bs.data.data()),
+                   "cqpi_bands_histogram");
+        if (!rc) {
+            // idl(pix, pix, layers, 1, 1, le, ue, 0, -1/2, -1/2), named after the model (src/tightbinding.cc:1161,1191)
+            const double h[7] = {1.0, 1.0, -0.5, -0.5, bslenergy, bshenergy, 0.0};
+            std::copy(h, h + 7, bs.hdr);
+            bs.name = tbname;
+            const time_t now = time(nullptr);
+            bs.date = ctime(&now);
+            if (!bs.date.empty() && bs.date.back() == '\n') bs.date.pop_back();
+            if (!write_idl(bsfile, bs)) rc = 1;
+            else *log << "... done and saved to " << bsfile << ".\n";
+        }
+        timings.mark("Bandstructure calculation");
+    }
     if (!rc && !dosfile.empty()) {
         // energies lb + k (hb - lb)/layers, k < layers (src/calcqpi.cc:348-356)
         const double step = (doshenergy - doslenergy) / (double)doslayers;
@@ -648,10 +750,12 @@ int main(int argc, char **argv)
             for (long k = 0; k < doslayers; k++) dosout << dosw[k] << " " << dos[k] << "\n";
             *log << "... done and saved to " << dosfile << ".\n";
         }
+        timings.mark("Calculation of density of states");
     }
     if (nomap) {
         cqpi_destroy(ctx);
         if (rc) return 1;
+        timings.print(*log);
         lf.warn_unused(std::cerr);
         return 0;
     }
@@ -666,9 +770,11 @@ int main(int argc, char **argv)
     }
     IdlMap out;
     out.data.assign((size_t)W * W * layers, 0.0);
+    timings.mark("Initialization of calculation");
     if (!rc) rc = check(cqpi_run(ctx, omegas.data(), omegas.size(), eta, out.data.data()), "cqpi_run");
     cqpi_destroy(ctx);
     if (rc) return 1;
+    timings.mark("Calculation of CLDOS");
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     // src/calcqpi.cc:639-652
@@ -694,6 +800,8 @@ int main(int argc, char **argv)
         return 1;
     }
     *log << "Done.\n";
+    timings.mark("Saving data");
+    timings.print(*log);
     lf.warn_unused(std::cerr);
     return 0;
 }

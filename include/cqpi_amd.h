@@ -258,6 +258,19 @@ int cqpi_set_timing(cqpi_ctx *ctx, int on);
  * cqpi_gradient. */
 int cqpi_set_josephson(cqpi_ctx *ctx, int on, double deltat, double etat);
 
+/* Band-structure histogram of the context's model (bsfile= in the reference
+ * driver: tightbind::calcbst3dhisthr, src/tightbinding.cc:1158-1193, called
+ * from src/calcqpi.cc:308-326 with ux = uy = 1 and all bands).  The
+ * eigenvalues of H(k) at the (pix oversamp)^2 points k = (io/(pix oversamp - 1)
+ * - 1/2, jo/(...) - 1/2) are added to pixel (io/oversamp, jo/oversamp) of the
+ * three layers around round((e - le)/(ue - le) (layers - 1)) as the Lorentzian
+ * w/2 / ((bias - e)^2 + w^2/4) / pi, w = 2 (ue - le)/layers (bias of layer l:
+ * le + (ue - le) l/(layers - 1)), for eigenvalues within two layer spacings of
+ * [le, ue].  hist: pix x pix x layers doubles, x fastest.  The sums run in
+ * any order (atomics): equal to the reference's up to rounding. */
+int cqpi_bands_histogram(cqpi_ctx *ctx, size_t pix, size_t oversamp, double le, double ue, size_t layers,
+                         double *hist);
+
 /* Green's function the LDOS runs use (green= in the reference driver,
  * src/calcqpi.cc:129, :207-208, :367-375). */
 enum {

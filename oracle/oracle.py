@@ -282,6 +282,37 @@ def orbital_tile(name, radius, height, xpos, ypos, theta, prefact, window, overs
     return orbital_value(name, radius, x, y, z, prefact)
 
 
+def bands_histogram(hops, b, efermi, pix, oversamp, le, ue, layers, scmodel=False):
+    """tightbind::calcbst3dhisthr (src/tightbinding.cc:1158-1193, ux = uy = 1,
+    all bands): pix x pix x layers, [x, y, layer]."""
+    ospix = pix * oversamp
+    g = np.arange(ospix) / (ospix - 1) - 0.5
+    kx, ky = np.meshgrid(g, g, indexing="ij")
+    H = np.zeros((ospix, ospix, b, b), complex)
+    scmax = b >> 1 if scmodel else b
+    for o in range(b):
+        H[..., o, o] = -efermi if o < scmax else efermi
+    for h in np.asarray(hops):
+        ph = 2 * np.pi * (h["rx"] * kx + h["ry"] * ky) * (-1 if h["anti"] else 1)
+        H[..., h["o1"], h["o2"]] += (h["tre"] + 1j * h["tim"]) * np.exp(1j * ph)
+    ev = np.linalg.eigvalsh(H)
+    out = np.zeros((pix, pix, layers))
+    binsize = (ue - le) / (layers - 1)
+    w = (ue - le) / layers * 2.0
+    for io in range(ospix):
+        for jo in range(ospix):
+            for e in ev[io, jo]:
+                if not (le - 2 * binsize < e < ue + 2 * binsize):
+                    continue
+                v = (e - le) / (ue - le) * (layers - 1)
+                lay = int(np.sign(v) * np.floor(abs(v) + 0.5))  # C round(): halves away from zero
+                for nl in range(lay - 1, lay + 2):
+                    if 0 <= nl < layers:
+                        x = (ue - le) * nl / (layers - 1) + le - e
+                        out[io // oversamp, jo // oversamp, nl] += w / 2 / (x * x + w * w / 4) / np.pi
+    return out
+
+
 def spf(g0k, n, maxband=0):
     """Spectral function map (src/qpihipkernel.cc:317-329): n x n, x + y*n."""
     K, _, b, _ = g0k.shape

@@ -254,6 +254,11 @@ def test_driver_surface_green_matches_oracle(tmp_path, gpu):
     r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr + r.stdout
     assert "Green's function     : bulk" in r.stdout
+    # the timing summary (include/recordtimings.h:89-112), sections in the order they ran
+    summ = r.stdout[r.stdout.index("Summary of timings"):]
+    order = [summ.index(x) for x in ("Calculation of density of states", "Initialization of calculation",
+                                     "Calculation of CLDOS", "Saving data", "Total time elapsed")]
+    assert order == sorted(order)
     data, hdr = synth.read_idl(str(tmp_path / "qpi.idl"))
     hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
     for e in range(2):

@@ -384,3 +384,22 @@ def test_surface_green_semi_infinite_chain_closed_form(surface):
         cands = [1 / root, -1 / root]
     ref = np.where(np.imag(cands[0]) < 0, cands[0], cands[1])
     assert np.allclose(g[..., 0, 0], ref, rtol=1e-10, atol=0)
+
+
+def test_bands_histogram_flat_band_closed_form():
+    """A flat band e0 (on-site term only): every k adds the same Lorentzian
+    w/2/((bias - e0)^2 + w^2/4)/pi, w = 2 (ue - le)/layers, to the three layers
+    around round((e0 - le)/(ue - le)(L - 1)), os^2 times per pixel
+    (src/tightbinding.cc:1158-1193); nothing elsewhere.  e0 sits exactly half
+    way between two layers, where C's round() goes up."""
+    le, ue, L, pix, os_ = -0.1, 0.1, 11, 3, 2
+    e0 = le + 4.5 * (ue - le) / (L - 1)
+    hops = np.zeros(1, synth.HOP_DTYPE)
+    hops["tre"] = e0
+    h = oracle.bands_histogram(hops, 1, 0.0, pix, os_, le, ue, L)
+    w = 2 * (ue - le) / L
+    ref = np.zeros(L)
+    for nl in (4, 5, 6):
+        x = le + (ue - le) * nl / (L - 1) - e0
+        ref[nl] = os_ * os_ * w / 2 / (x * x + w * w / 4) / np.pi
+    assert np.allclose(h, ref[None, None, :], rtol=1e-12, atol=0)
