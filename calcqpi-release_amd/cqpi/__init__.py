@@ -50,6 +50,7 @@ SIGNATURES = {
     "cqpi_orbital_tile": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _sz, _sz, _dp]),
     "cqpi_orbital_range": (C.c_double, [C.c_double, C.c_double]),
+    "cqpi_xsf_tile": (C.c_int, [C.c_char_p, C.c_double, _sz, _sz, C.c_double, _dp]),
     "cqpi_bands_histogram": (C.c_int, [_vp, _sz, _sz, C.c_double, C.c_double, _sz, _dp]),
     "cqpi_create_list": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _vp, C.POINTER(_sz), C.POINTER(_vp)]),
     "cqpi_create_spectral": (C.c_int, [_sz, _sz, _sz, _sz, C.c_int, _dp, _sz, _dp, _sz, C.POINTER(_vp)]),
@@ -473,6 +474,15 @@ def orbital_tile(name, radius=0.5, height=5.0, xpos=0.0, ypos=0.0, theta=0.0, pr
     _check(lib().cqpi_orbital_tile(name.encode(), float(radius), float(height), float(xpos), float(ypos),
                                    float(theta), float(prefact), int(window), int(oversamp),
                                    out.ctypes.data_as(_dp)))
+    return out
+
+
+def xsf_tile(path, height=5.0, window=2, oversamp=4, prefact=1.0):
+    """Wannier tile of an .xsf orbital file (cqpi_xsf_tile): T x T."""
+    T = (2 * window + 1) * oversamp
+    out = np.zeros((T, T))
+    _check(lib().cqpi_xsf_tile(os.fsencode(path), float(height), int(window), int(oversamp), float(prefact),
+                               out.ctypes.data_as(_dp)))
     return out
 
 

@@ -36,8 +36,9 @@
 //     Green's function of a layered model for every output (src/calcqpi.cc:
 //     203-208, :293-307, :367-375); the principal-layer warning as there.
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
-// Other output modes, spinarr, and Wannier90/XSF orbital files are reported
-// as unsupported.
+//   * orbitalfiles=("a.xsf",...) with zheight: Wannier90/XSF orbital files
+//     (src/calcqpi.cc:576-601).
+// Other output modes and spinarr are reported as unsupported.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -570,9 +571,38 @@ int main(int argc, char **argv)
         }
         *log << "Maximum rel. value at boundary: " << worst << "\n";
         if (worst > 0.05) *log << "Warning: boundary too close, use larger window size!\n";
+    } else if (lf.probe("orbitalfiles")) {
+        // Wannier90/XSF orbital files: src/calcqpi.cc:576-601
+        double zheight = 5.0;
+        if (lf.probe("zheight")) zheight = lf.num("zheight");
+        *log << "Using Wannier90/XSF-orbitals for vacuum overlap.\n    Parameters:\n             zheight:         "
+             << zheight << "\n";
+        vector<string> files = lf.list("orbitalfiles");
+        for (string &f : files)
+            if (f.size() >= 2 && f.front() == '"' && f.back() == '"') f = f.substr(1, f.size() - 2);
+        vector<double> prearr;
+        if (lf.probe("prearr")) prearr = lf.vec("prearr");
+        prearr.resize(files.size(), 1.0);
+        const size_t T = (size_t)(2 * window + 1) * oversamp;
+        for (size_t i = 0; i < files.size(); i++) {
+            *log << "Loading wave function file " << files[i] << ".\n";
+            vector<double> t(T * T);
+            if (cqpi_xsf_tile(files[i].c_str(), zheight, (size_t)window, (size_t)oversamp, prearr[i], t.data())) {
+                std::cerr << "Error loading wave function file " << files[i] << "\n";
+                return 1;
+            }
+            wfs.push_back(t);
+        }
+        if ((maxbands >> 1) == files.size() && spin) {
+            *log << "Found half as many orbitals as needed in spin-polarized calculation, duplicating wave "
+                    "function set.\n";
+            for (size_t k = 0; k < files.size(); k++) wfs.push_back(wfs[k]);
+        } else if (files.size() < maxbands) {
+            *log << "Fewer number of orbital files specified than there are bands, padding with zeros.\n";
+        }
     } else if (!(spectral || nomap)) {
-        std::cerr << "This build takes the Wannier tiles from an idlorbitalfile or from analytic orbitals "
-                     "(orbitals=); orbitalfiles (Wannier90/XSF) are not supported.\n";
+        std::cerr << "The Wannier tiles need an idlorbitalfile, analytic orbitals (orbitals=) or orbital files "
+                     "(orbitalfiles=).\n";
         return 1;
     }
     const size_t T = (size_t)(2 * window + 1) * oversamp;

@@ -412,6 +412,16 @@ void cqpi_free(void *p);
  * CQPI_E_ARG for an unknown name or radius <= 0. */
 int cqpi_orbital_tile(const char *name, double radius, double height, double xpos, double ypos, double theta,
                       double prefact, size_t window, size_t oversamp, double *tile);
+/* Wannier tile of a Wannier90/XCrySDen .xsf orbital file (orbitalfiles= in
+ * the reference driver, src/calcqpi.cc:576-601; wannierfunctions(filename,
+ * height, window, oversamp, prefact), src/wannierfunctions.cc:444-461, with
+ * xcfloader, src/loadxcf.cc:33-349): tile[i*T + j] = prefact * the value of
+ * the data-grid node nearest (nearest in each spanning-vector coordinate, 0
+ * outside) to xu a1 + yu a2 at height zh, xu = i/oversamp - window, yu
+ * likewise, a1, a2 the CONVVEC rows; zh = highest atom z + height (height > 0)
+ * or lowest atom z + height (height < 0).  Host only; CQPI_E_IO if the file
+ * cannot be read or has no 3D_field data grid. */
+int cqpi_xsf_tile(const char *path, double height, size_t window, size_t oversamp, double prefact, double *tile);
 /* radius sqrt(ln(1/p)): where exp(-r^2/radius^2) falls to p
  * (wannierfunctions::getrange, src/wannierfunctions.cc:382-384); the driver
  * sizes window = 0 (auto) from it with p = 0.01. */

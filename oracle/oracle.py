@@ -282,6 +282,55 @@ def orbital_tile(name, radius, height, xpos, ypos, theta, prefact, window, overs
     return orbital_value(name, radius, x, y, z, prefact)
 
 
+def xsf_read(path):
+    """xcfloader::readfile (src/loadxcf.cc:33-185): CONVVEC rows, atom z
+    coordinates, data-grid origin, spanning vectors (rows) and values [i, j, k]."""
+    lines = [ln for ln in open(path).read().splitlines() if "#" not in ln]
+    conv, atoms, grid = np.eye(3), [], None
+    i = 0
+    while i < len(lines):
+        ln = lines[i]
+        if "CONVVEC" in ln:
+            conv = np.array([[float(t) for t in lines[i + r].split()[:3]] for r in (1, 2, 3)])
+            i += 3
+        elif "PRIMCOORD" in ln:
+            n = int(lines[i + 1].split()[0])
+            atoms = [float(lines[i + 2 + a].split()[3]) for a in range(n)]
+            i += 1 + n
+        elif "BEGIN_BLOCK_DATAGRID_3D" in ln:
+            nx, ny, nz = (int(t) for t in lines[i + 3].split()[:3])
+            origin = np.array([float(t) for t in lines[i + 4].split()[:3]])
+            span = np.array([[float(t) for t in lines[i + 5 + a].split()[:3]] for a in range(3)])
+            toks = " ".join(lines[i + 8:]).split()
+            vals = np.array([float(t) for t in toks[:nx * ny * nz]])
+            grid = (origin, span, vals.reshape(nz, ny, nx).transpose(2, 1, 0))
+            break
+        i += 1
+    return conv, np.array(atoms), grid
+
+
+def xsf_tile(path, height, window, oversamp, prefact=1.0):
+    """wannierfunctions(filename, height, window, oversamp, prefact)
+    (src/wannierfunctions.cc:444-461) through xcfloader::getvalue, operator()
+    and getzheight (src/loadxcf.cc:220-236, 330-349)."""
+    conv, atoms, (origin, span, cube) = xsf_read(path)
+    zh = atoms.max() + height if height > 0 else atoms.min() + height if height < 0 else np.nan
+    T = (2 * window + 1) * oversamp
+    g = np.arange(T) / oversamp - window
+    xu, yu = np.meshgrid(g, g, indexing="ij")
+    p = np.stack([xu * conv[0, 0] + yu * conv[1, 0] - origin[0], xu * conv[0, 1] + yu * conv[1, 1] - origin[1],
+                  np.full_like(xu, zh - origin[2])], -1)
+    f = np.linalg.solve(span.T, p.reshape(-1, 3).T).T.reshape(T, T, 3)
+    n = np.array(cube.shape)
+    with np.errstate(invalid="ignore"):
+        idx = np.sign(f * n) * np.floor(np.abs(f * n) + 0.5)  # C round()
+        ok = np.all((idx >= 0) & (idx < n), -1)
+    out = np.zeros((T, T))
+    ii = np.where(ok[..., None], idx, 0).astype(int)
+    out[ok] = cube[ii[..., 0], ii[..., 1], ii[..., 2]][ok]
+    return prefact * out
+
+
 def bands_histogram(hops, b, efermi, pix, oversamp, le, ue, layers, scmodel=False):
     """tightbind::calcbst3dhisthr (src/tightbinding.cc:1158-1193, ux = uy = 1,
     all bands): pix x pix x layers, [x, y, layer]."""

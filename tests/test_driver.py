@@ -324,3 +324,24 @@ def test_driver_analytic_orbitals_match_oracle(tmp_path, gpu):
     ref = oracle.cldos_run(wl2)
     for e in range(2):
         assert rm.rel_errors(data[:, :, e].T.ravel(), ref[e].astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+def test_driver_orbital_files(tmp_path):
+    """orbitalfiles= (src/calcqpi.cc:576-601): one .xsf file per orbital, the
+    rest padded with zero tiles; the tiles are the oracle's."""
+    from test_orbitals import _write_xsf
+    wl = synth.square_3orb(K=16, n=8, E=2)
+    rng = np.random.default_rng(9)
+    files = []
+    for a in range(2):
+        cube = rng.normal(size=(24, 24, 6))
+        _write_xsf(str(tmp_path / ("o%d.xsf" % a)), cube, (-3.0, -3.0, 0.0), np.diag([6.0, 6.0, 3.0]))
+        files.append(str(tmp_path / ("o%d.xsf" % a)))
+    p = synth.write_inputs(str(tmp_path), wl)
+    lines = [ln for ln in open(p).read().splitlines() if not ln.startswith("idlorbitalfile")]
+    lines += ['orbitalfiles=("o0.xsf","o1.xsf");', "zheight=0.4;", "prearr=(1.0,0.5);"]
+    open(p, "w").write("\n".join(lines) + "\n")
+    cfg, err = _print_config(p)
+    ref = sum(pre * oracle.xsf_tile(f, 0.4, wl.window, wl.oversamp).sum() for f, pre in zip(files, (1.0, 0.5)))
+    assert abs(cfg["tile_sum"] - ref) < 1e-9 * max(1.0, abs(ref))
+    assert "padding with zeros" in err and "Unused" not in err
