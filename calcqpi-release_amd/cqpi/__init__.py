@@ -47,6 +47,7 @@ SIGNATURES = {
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_set_surface": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _sz]),
+    "cqpi_set_surface_steps": (C.c_int, [_vp, C.c_int]),
     "cqpi_orbital_tile": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _sz, _sz, _dp]),
     "cqpi_orbital_range": (C.c_double, [C.c_double, C.c_double]),
@@ -330,6 +331,10 @@ class AmdQPI:
         sh = np.ascontiguousarray(shops if shops is not None else np.zeros(0, synth.HOP_DTYPE), dtype=synth.HOP_DTYPE)
         _check(lib().cqpi_set_surface(self._ctx, int(mode), float(epserr), sh.ctypes.data if len(sh) else None,
                                       len(sh)))
+
+    def set_surface_steps(self, maxit):
+        """Step limit of the surface Green's function's decimation (default 500)."""
+        _check(lib().cqpi_set_surface_steps(self._ctx, int(maxit)))
 
     def set_precision(self, mode):
         mode = PRECISIONS.get(mode, mode)

@@ -1279,6 +1279,15 @@ int cqpi_bands_histogram(cqpi_ctx *c, size_t pix, size_t oversamp, double le, do
     return 0;
 }
 
+int cqpi_set_surface_steps(cqpi_ctx *c, int maxit)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (maxit < 1 || maxit > 100000) return fail(CQPI_E_ARG, "the step limit must lie in [1, 100000]");
+    c->surf_maxit = maxit;
+    return 0;
+}
+
 int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *shops, size_t nshops)
 {
     g_err.clear();

@@ -154,9 +154,12 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
 {
     constexpr int NP = B * B;
     const long KK = (long)K * K;
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int e = blockIdx.y;
-    if (k >= KK) return;
+    // lanes past the grid repeat the last point (their results are not stored),
+    // so that the whole wave takes part in the step-count reduction below
+    const bool active = k0 < KK;
+    const long k = active ? k0 : KK - 1;
     const int i = (int)(k / K), j = (int)(k - (long)i * K);
     double2 es[NP], eb[NP], al[NP], be[NP], g[NP], t1[NP];
     build_h<B>(sets.g[1], sets.n[1], sets.t[1], tw, i, j, K, true, efermi, scmax, es);
@@ -208,9 +211,13 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
         }
     }
     resinv<B>(surface ? es : eb, om, eta, g);
+    if (active)
 #pragma unroll
-    for (int r = 0; r < NP; r++) store_c(Gk + ((size_t)e * NP + r) * KK + k, g[r]);
-    atomicMax(stat + e, it);
+        for (int r = 0; r < NP; r++) store_c(Gk + ((size_t)e * NP + r) * KK + k, g[r]);
+    // the wave's largest step count, one atomic per wave (blocks are one wave)
+    int wmax = it;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    if (threadIdx.x == 0) atomicMax(stat + e, wmax);
 }
 
 __global__ void k_zero_int(int *p, int n)

@@ -287,9 +287,14 @@ enum {
  * (stbfile=).  Call after cqpi_set_model; bands <= 5.  It applies to every
  * output of the context: LDOS and Josephson maps, threshold-list maps, the
  * SPF / USPF maps and cqpi_dos.  cqpi_run and cqpi_dos return
- * CQPI_E_CONVERGE if some k needs more than 500 steps.  A run with it cannot
+ * CQPI_E_CONVERGE if some k needs more steps than the limit
+ * (cqpi_set_surface_steps, default 500).  A run with it cannot
  * feed cqpi_gradient.  mode CQPI_GREEN_NORMAL switches it off. */
 int cqpi_set_surface(cqpi_ctx *ctx, int mode, double epserr, const cqpi_hopping *shops, size_t nshops);
+/* Step limit of the decimation (default 500; the reference loops until it
+ * converges, src/qpi.cc:515-541, which a device kernel cannot).  A run in
+ * which some k-point needs more steps returns CQPI_E_CONVERGE. */
+int cqpi_set_surface_steps(cqpi_ctx *ctx, int maxit);
 
 /* For a context of cqpi_create_spectral: mode 1 (SPF) or 2 (USPF) makes
  * cqpi_run compute those maps (n x n per energy) from the device's eigen

@@ -165,3 +165,23 @@ def test_surface_dos(gpu, mode):
                                                                      epserr=1e-10, surface=mode == "surface")[0]))
                     for w in om])
     assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_surface_step_limit_reports_non_convergence(gpu):
+    """A step limit below what the k-points need (forced from the host; every
+    thread still ends after that many steps): the run and the DOS report
+    CQPI_E_CONVERGE, and the default limit runs again afterwards."""
+    wl = synth.layered(3, K=64, n=16, E=2, seed=57, w=1, os_=2, eta=0.02)
+    with cqpi.engine_for(wl) as q:
+        q.set_surface("surface", 1e-10)
+        q.set_surface_steps(2)
+        with pytest.raises(cqpi.CqpiError) as ei:
+            q.run(wl.energies, wl.eta)
+        assert ei.value.code == 7  # CQPI_E_CONVERGE
+        with pytest.raises(cqpi.CqpiError):
+            q.dos([0.0], 0.02)
+        with pytest.raises(cqpi.CqpiError):
+            q.set_surface_steps(0)
+        q.set_surface_steps(500)
+        got = q.run(wl.energies, wl.eta)
+    _assert_close(got, _oracle_maps(wl, True, 1e-10), TOL64)
