@@ -1300,7 +1300,7 @@ int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *s
     }
     if (!(epserr > 0.0)) return fail(CQPI_E_ARG, "epserr must be positive");
     if (nshops && !shops) return fail(CQPI_E_ARG, "shops is NULL");
-    if (c->g.B > 5) return fail(CQPI_E_ARG, "the surface Green's function supports up to 5 bands");
+    if (c->g.B > 8) return fail(CQPI_E_ARG, "the surface Green's function supports up to 8 bands");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     c->h_shops.assign(shops, shops + nshops);
     c->surf_mode = mode;

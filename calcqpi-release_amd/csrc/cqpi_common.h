@@ -166,7 +166,7 @@ int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, 
 // surface (surface != 0) or bulk Green's function of a layered model by
 // decimation into that layout; groups g/t/n: bulk in-plane, surface
 // in-plane, coupling (rz < 0).  stat[e]: the most decimation steps any k
-// needed (maxit + 1: not converged).  B <= 5.
+// needed (maxit + 1: not converged).  B <= 8.
 int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *const t[3], const int n[3],
                       const double2 *tw, double efermi, int scmax, const double *omegas, int E, double eta,
                       double epserr, int maxit, int surface, void *Gk, bool fp32, int *stat, hipStream_t s);

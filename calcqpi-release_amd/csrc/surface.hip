@@ -12,7 +12,8 @@
 //   alpha <- alpha g alpha; beta <- beta g beta;
 // until both Frobenius norms are <= epserr (src/qpi.cc:541); then
 // G(k) = (w + i eta - eps_s)^-1 (surface) or with eps_b (bulk).  One thread per
-// (k, energy) with the b x b blocks in registers; the result goes to the G(k)
+// (k, energy) with the b x b blocks in registers (b <= 5; 6 to 8 spill to
+// scratch); the result goes to the G(k)
 // workspace layout Gk[e][pair][kx][ky] that the first transform pass reads.
 // A run stops a k-point after maxit steps and reports it in stat.
 #include "cqpi_common.h"
@@ -30,10 +31,10 @@ __device__ inline void build_h(const DevRGroup *__restrict__ groups, int ngroups
                                const double2 *__restrict__ tw, int i, int j, int K, bool diag, double efermi,
                                int scmax, double2 *H)
 {
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int r = 0; r < B * B; r++) H[r] = make_double2(0.0, 0.0);
     if (diag)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < B; r++) H[r * B + r].x = r < scmax ? -efermi : efermi;
     const double kx = (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
     const double ky = (j > (K >> 1)) ? ((double)j - (double)K) / (double)K : (double)j / (double)K;
@@ -50,7 +51,7 @@ __device__ inline void build_h(const DevRGroup *__restrict__ groups, int ngroups
             e = make_double2(cs, sn);
         }
         const double2 *T = tmat + (size_t)gi * B * B;
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < B * B; r++) H[r] = cadd(H[r], cmul(T[r], e));
     }
 }
@@ -61,27 +62,27 @@ template <int B>
 __device__ inline void resinv(const double2 *H, double om, double eta, double2 *G)
 {
     double2 M[B * B];
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int r = 0; r < B; r++)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) {
             M[r * B + c] = make_double2((r == c ? om : 0.0) - H[r * B + c].x, (r == c ? eta : 0.0) - H[r * B + c].y);
             G[r * B + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
         }
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int j = 0; j < B; j++) {
         int piv = j;
         double best = cabs2(M[j * B + j]);
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int i = j + 1; i < B; i++)
             if (cabs2(M[i * B + j]) > best) {
                 best = cabs2(M[i * B + j]);
                 piv = i;
             }
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int i = j + 1; i < B; i++)
             if (i == piv)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
                 for (int c = 0; c < B; c++) {
                     double2 t = M[j * B + c];
                     M[j * B + c] = M[i * B + c];
@@ -91,16 +92,16 @@ __device__ inline void resinv(const double2 *H, double om, double eta, double2 *
                     G[i * B + c] = t;
                 }
         const double2 ip = cinv(M[j * B + j]);
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) {
             M[j * B + c] = cmul(M[j * B + c], ip);
             G[j * B + c] = cmul(G[j * B + c], ip);
         }
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int i = 0; i < B; i++) {
             if (i == j) continue;
             const double2 l = M[i * B + j];
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
             for (int c = 0; c < B; c++) {
                 M[i * B + c] = csub(M[i * B + c], cmul(l, M[j * B + c]));
                 G[i * B + c] = csub(G[i * B + c], cmul(l, G[j * B + c]));
@@ -113,12 +114,12 @@ __device__ inline void resinv(const double2 *H, double om, double eta, double2 *
 template <int B>
 __device__ inline void mm(const double2 *X, const double2 *Y, double2 *P)
 {
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int r = 0; r < B; r++)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) {
             double2 a = make_double2(0.0, 0.0);
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
             for (int k = 0; k < B; k++) a = cadd(a, cmul(X[r * B + k], Y[k * B + c]));
             P[r * B + c] = a;
         }
@@ -128,17 +129,17 @@ __device__ inline void mm(const double2 *X, const double2 *Y, double2 *P)
 template <int B>
 __device__ inline void mm_left_inplace(double2 *X, const double2 *Y)
 {
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int r = 0; r < B; r++) {
         double2 row[B];
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) {
             double2 a = make_double2(0.0, 0.0);
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
             for (int k = 0; k < B; k++) a = cadd(a, cmul(X[r * B + k], Y[k * B + c]));
             row[c] = a;
         }
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) X[r * B + c] = row[c];
     }
 }
@@ -165,9 +166,9 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
     build_h<B>(sets.g[1], sets.n[1], sets.t[1], tw, i, j, K, true, efermi, scmax, es);
     build_h<B>(sets.g[0], sets.n[0], sets.t[0], tw, i, j, K, true, efermi, scmax, eb);
     build_h<B>(sets.g[2], sets.n[2], sets.t[2], tw, i, j, K, false, efermi, scmax, al);
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
     for (int r = 0; r < B; r++)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int c = 0; c < B; c++) be[r * B + c] = cconj(al[c * B + r]);
     const double om = omegas[e];
     const double eps2 = epserr * epserr;
@@ -175,23 +176,23 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
     for (;;) {
         resinv<B>(eb, om, eta, g);
         mm<B>(g, be, t1);  // t1 = g beta
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < B; r++)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
             for (int c = 0; c < B; c++) {
                 double2 a = make_double2(0.0, 0.0);  // (alpha g beta)_rc
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
                 for (int q = 0; q < B; q++) a = cadd(a, cmul(al[r * B + q], t1[q * B + c]));
                 es[r * B + c] = cadd(es[r * B + c], a);
                 eb[r * B + c] = cadd(eb[r * B + c], a);
             }
         mm_left_inplace<B>(g, al);  // g <- g alpha
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < B; r++)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
             for (int c = 0; c < B; c++) {
                 double2 a = make_double2(0.0, 0.0);  // (beta g alpha)_rc
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
                 for (int q = 0; q < B; q++) a = cadd(a, cmul(be[r * B + q], g[q * B + c]));
                 eb[r * B + c] = cadd(eb[r * B + c], a);
             }
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
         mm_left_inplace<B>(al, g);   // alpha <- alpha g alpha
         it++;
         double na = 0.0, nb = 0.0;
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < NP; r++) {
             na += cabs2(al[r]);
             nb += cabs2(be[r]);
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
     }
     resinv<B>(surface ? es : eb, om, eta, g);
     if (active)
-#pragma unroll
+#pragma unroll(B <= 5 ? 64 : 1)
         for (int r = 0; r < NP; r++) store_c(Gk + ((size_t)e * NP + r) * KK + k, g[r]);
     // the wave's largest step count, one atomic per wave (blocks are one wave)
     int wmax = it;
@@ -254,7 +255,7 @@ int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *
             hipLaunchKernelGGL((k_surface_gk<double2, b>), grid, dim3(64), 0, s, K, sets, tw, efermi, scmax,        \
                                omegas, eta, epserr, maxit, surface, static_cast<double2 *>(Gk), stat);             \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
     default: return CQPI_E_ARG;
     }

@@ -98,10 +98,10 @@ def test_surface_rejections(gpu):
             q.set_surface(7)
     finally:
         q.close()
-    wl6 = synth.square_norb(6, K=16, n=4, E=1, seed=52, w=1, os_=2)
-    with cqpi.engine_for(wl6) as q:
+    wl10 = synth.bdg_square(5, K=16, n=4, E=1, seed=52)
+    with cqpi.engine_for(wl10) as q:
         with pytest.raises(cqpi.CqpiError):
-            q.set_surface("surface")  # more than 5 bands
+            q.set_surface("surface")  # more than 8 bands
     with cqpi.engine_for(wl) as q:
         q.set_keep_windows(True)
         q.set_surface("bulk", 1e-10)
@@ -185,3 +185,32 @@ def test_surface_step_limit_reports_non_convergence(gpu):
         q.set_surface_steps(500)
         got = q.run(wl.energies, wl.eta)
     _assert_close(got, _oracle_maps(wl, True, 1e-10), TOL64)
+
+
+def test_surface_josephson_map(gpu):
+    """output=josephson with green=surface: the Josephson map of a layered
+    superconducting model from the surface Green's function (scmodel: the
+    -/+ efermi diagonal of the particle and hole halves)."""
+    wl = synth.bdg_square(3, K=64, n=16, E=2, seed=58, w=1, os_=2, eta=0.02)
+    rng = np.random.default_rng(59)
+    M = 0.1 * (rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6)))
+    extra = [(0.0, 0.0, -1.0, a, c, M[a, c].real, M[a, c].imag, 0, 0) for a in range(6) for c in range(6)]
+    extra += [(0.0, 0.0, 1.0, a, c, M[c, a].real, -M[c, a].imag, 0, 0) for a in range(6) for c in range(6)]
+    wl.hops = np.concatenate([np.asarray(wl.hops, synth.HOP_DTYPE), np.array(extra, synth.HOP_DTYPE)])
+    deltat, etat = 0.01, 0.015
+    print("case: josephson + surface, BdG 6 bands, K=64, n=16, E=2", flush=True)
+    with cqpi.engine_for(wl) as q:
+        q.set_josephson(True, deltat, etat)
+        q.set_surface("surface", 1e-10)
+        got = q.run(wl.energies, wl.eta)
+    for e, om in enumerate(wl.energies):
+        g, it = oracle.surfacegreen(wl.hops, 6, wl.efermi, wl.K, om, wl.eta, epserr=1e-10, surface=True,
+                                    scmodel=True)
+        assert it > 1
+        gR = oracle.realspace(g)
+        oc = complex(om, etat)
+        tip = complex(0.0, -np.copysign(1.0, om) * np.pi * deltat) / np.sqrt(oc * oc - deltat * deltat)
+        ref = oracle.wannierjosephson(gR, oracle.tmatrix(wl.V, gR[0, 0]), wl.tiles, wl.n, wl.window, wl.oversamp,
+                                      tip)
+        assert np.abs(ref).max() > 0
+        assert rm.rel_errors(got[e], ref)[0] < TOL64, e
