@@ -49,6 +49,7 @@
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -160,6 +161,36 @@ struct Config {
 // ---------------------------------------------------------------------------
 // idl files (src/idl.cc:50-95)
 // ---------------------------------------------------------------------------
+// Stream buffer that writes through to another one and starts every line
+// with the local time "YYYY-MM-DD.HH:MM:SS " (the reference's log file,
+// AddTimeStamp, include/addtimestamp.h:25-84).
+class StampedLines : public std::streambuf {
+    std::streambuf *sink_;
+    bool at_line_start_ = true;
+
+  protected:
+    int overflow(int c) override
+    {
+        if (c == traits_type::eof()) return sink_->pubsync() == -1 ? c : traits_type::not_eof(c);
+        if (at_line_start_) {
+            char buf[80];
+            const time_t now = time(nullptr);
+            struct tm t;
+            localtime_r(&now, &t);
+            const size_t len = strftime(buf, sizeof buf, "%Y-%m-%d.%X ", &t);
+            if (sink_->sputn(buf, (std::streamsize)len) != (std::streamsize)len) return traits_type::eof();
+        }
+        at_line_start_ = c == '\n';
+        const int r = sink_->sputc((char)c);
+        if (at_line_start_) sink_->pubsync();
+        return r;
+    }
+    int sync() override { return sink_->pubsync(); }
+
+  public:
+    explicit StampedLines(std::streambuf *sink) : sink_(sink) {}
+};
+
 // Wall and process-CPU time per named section, sections of one name summed,
 // printed as the reference's timing summary (recordtimings,
 // include/recordtimings.h:26-113).
@@ -296,10 +327,27 @@ int main(int argc, char **argv)
     }
     std::ofstream logfile;
     std::ostream *log = print_only ? &std::cerr : &std::cout;  // --print-config keeps stdout for the JSON
+    // logfile: every line time-stamped, and error messages go there too
+    // (src/calcqpi.cc:162-170)
+    std::unique_ptr<StampedLines> stamped;
+    std::unique_ptr<std::ostream> stamped_log;
+    struct CerrRestore {
+        std::streambuf *saved = nullptr;
+        ~CerrRestore()
+        {
+            if (saved) std::cerr.rdbuf(saved);
+        }
+    } cerr_restore;
     if (lf.probe("logfile")) {
         logfile.open(lf.str("logfile"), std::ios::app);
-        if (logfile.is_open()) log = &logfile;
+        if (logfile.is_open()) {
+            stamped.reset(new StampedLines(logfile.rdbuf()));
+            stamped_log.reset(new std::ostream(stamped.get()));
+            log = stamped_log.get();
+            if (!print_only) cerr_restore.saved = std::cerr.rdbuf(stamped.get());
+        }
     }
+    *log << "New calculation started, using configuration from " << argv[argi] << "\n";
     // defaults: src/calcqpi.cc:133-137
     long n = 201, layers = 21, oversamp = 4, window = 2, kpoints = 0;
     bool spin = false, scmodel = false;

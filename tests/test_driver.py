@@ -345,3 +345,16 @@ def test_driver_orbital_files(tmp_path):
     ref = sum(pre * oracle.xsf_tile(f, 0.4, wl.window, wl.oversamp).sum() for f, pre in zip(files, (1.0, 0.5)))
     assert abs(cfg["tile_sum"] - ref) < 1e-9 * max(1.0, abs(ref))
     assert "padding with zeros" in err and "Unused" not in err
+
+
+def test_driver_logfile_timestamps(tmp_path):
+    """logfile=: every log line starts with the local time YYYY-MM-DD.HH:MM:SS
+    (include/addtimestamp.h:25-31), appended to the file."""
+    import re
+    wl = synth.square_3orb(K=16, n=8, E=3)
+    p = synth.write_inputs(str(tmp_path), wl, extra={"logfile": '"run.log"'})
+    for _ in range(2):
+        _print_config(p)
+    lines = open(str(tmp_path / "run.log")).read().splitlines()
+    assert sum("New calculation started, using configuration from" in ln for ln in lines) == 2
+    assert lines and all(re.match(r"\d{4}-\d{2}-\d{2}\.\d{2}:\d{2}:\d{2} ", ln) for ln in lines)
