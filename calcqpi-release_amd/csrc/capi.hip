@@ -1360,8 +1360,9 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     // the timing window covers this run, plus a cqpi_compute_eigen called just before it
     if (!c->eigen_timed) reset_stages(c);
     c->eigen_timed = false;
-    // the surface Green's function does not use the eigen data
-    if (!c->eigen_valid && !(c->surf_mode && !c->spec_mode)) {
+    // the surface Green's function (every output, the spectral maps too) does
+    // not use the eigen data
+    if (!c->eigen_valid && !c->surf_mode) {
         c->in_run = true;
         int r = cqpi_compute_eigen(c);
         c->in_run = false;

@@ -47,10 +47,16 @@ enum {
     CQPI_E_CONVERGE = 7  /* surface Green's function not converged */
 };
 
-/* precision of the continuum contraction (the other stages are always FP64) */
+/* Precision of a context's runs.  In every mode the eigensolve, the
+ * T-matrix solve and the periodic (bulk) term are FP64, and the G0(R) window
+ * that the T-matrix reads is stored in FP64.  In the FP32 and mixed modes the
+ * resolvent and both k->R transform passes run on FP32 lines (for
+ * power-of-two k-grids from 64 to 1024; other grids keep FP64 lines), and the
+ * surface Green's function G(k) is stored in FP32; the contraction then runs
+ * as below, with T applied in FP32 and the final sum in FP64. */
 enum {
     CQPI_PREC_FP64 = 0,  /* FP64 everywhere */
-    CQPI_PREC_FP32 = 1,  /* contraction in FP32 */
+    CQPI_PREC_FP32 = 1,  /* contraction in FP32 on the vector ALUs */
     CQPI_PREC_MIXED = 2  /* contraction on MFMA: split-FP16 hi+lo operands, FP32 accumulation */
 };
 
