@@ -45,6 +45,7 @@ SIGNATURES = {
     "cqpi_create": (C.c_int, [_sz, _sz, _sz, _sz, _sz, _sz, _sz, C.c_int, _dp, C.POINTER(_vp)]),
     "cqpi_wannierldos": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_wannierjosephson": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz, C.c_double, C.c_double]),
+    "cqpi_wannierldos_device": (C.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_set_surface": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _sz]),
     "cqpi_set_surface_steps": (C.c_int, [_vp, C.c_int]),
@@ -125,6 +126,15 @@ def lib():
         if not os.path.exists(LIBPATH):
             raise RuntimeError("libcqpi_amd.so not built (run __graft_entry__.build() or "
                                "make -C calcqpi-release_amd): %s" % LIBPATH)
+        # torch carries its own HIP and HSA runtime libraries.  Loaded after
+        # this library has initialised the GPU, that runtime finds no device
+        # ("No HIP GPUs are available"); loaded first, both work side by side.
+        # ldos_tensor() and cqpi.autograd hand device memory to torch, so torch
+        # is loaded here, before the library, when it is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBPATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -209,6 +219,13 @@ class AmdQPI:
         _check(lib().cqpi_wannierldos(self._ctx, _d(scat.view(np.float64)),
                                       ptrs.ctypes.data_as(C.POINTER(_vp)), self.bands))
         self.nlayers = 1
+
+    def wannierldos_device(self, d_T, d_g0win, E, d_ldos):
+        """cqpi_wannierldos_device: device pointers (ints) of E matrices T,
+        E G0(R) windows (layout of g0_window) and the E output maps; nothing
+        is allocated, the kernels run on the context stream and the call
+        returns without waiting."""
+        _check(lib().cqpi_wannierldos_device(self._ctx, int(d_T), int(d_g0win), int(E), int(d_ldos)))
 
     @classmethod
     def from_list(cls, wanniern, kpoints, n, window, oversamp, bands, flags, counts):

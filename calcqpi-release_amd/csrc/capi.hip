@@ -74,6 +74,42 @@ namespace cqpi {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace cqpi
 
+// Fixed device scratch of cqpi_wannierldos_device, the entry point that
+// allocates nothing: the periodic-term table, and per precision the fp16
+// planes and their scales (mixed) or the FP32 copies of the windows and T
+// (FP32).  Every device holds its own copy until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is only a
+// stand-in): it uses the device address hipGetSymbolAddress returned for the
+// current device at context creation.
+static __device__ __align__(256) unsigned char scratch[CQPI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> address of its copy of scratch
+
+// resolves (once per device, that device current) the address of scratch
+int scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return 0;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(CQPI_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return 0;
+}
+
+void *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+
+}  // namespace
+
 // streams an eager run's energy chunk is spread over (3 and 4 measured
 // slower than 2 on the 3-orbital workload: 0.77 and 0.81 ms against 0.755)
 constexpr int kSplit = 2;
@@ -740,6 +776,11 @@ int cqpi_create(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t
     c->kpoints = kpoints;
     c->scmax = g.B;
     (void)hipGetDevice(&c->device);
+    if (scratch_setup(c->device)) {
+        std::string msg = g_err;
+        release(c);
+        return fail(CQPI_E_HIP, msg);
+    }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         release(c);
         return fail(CQPI_E_HIP, "hipStreamCreate failed");
@@ -1116,6 +1157,65 @@ int cqpi_wannierjosephson(cqpi_ctx *c, const double *scat, const double *const *
     if (c && (c->list_mode || c->spectral))
         return fail(CQPI_E_STATE, "the Josephson map needs a context made by cqpi_create");
     return dropin_map(c, scat, g0, tda, tip);
+}
+
+int cqpi_wannierldos_device(cqpi_ctx *c, const double *d_T, const double *d_g0win, size_t E, double *d_ldos)
+{
+    g_err.clear();
+    if (!c || !d_T || !d_g0win || !d_ldos) return fail(CQPI_E_ARG, "NULL argument");
+    if (c->list_mode || c->spectral) return fail(CQPI_E_STATE, "the LDOS map needs a context made by cqpi_create");
+    if (!E || E > 65535) return fail(CQPI_E_ARG, "energies per call must be 1 to 65535");
+    void *base = scratch_of(c->device);
+    if (!base) return fail(CQPI_E_STATE, "no scratch address for the context's device");
+    const Geom &g = c->g;
+    const size_t NP = (size_t)g.B * g.B;
+    // scratch layout: periodic table, then (mixed) scales and planes or (FP32)
+    // narrowed windows and T, each part 256-byte aligned
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_bulk = carve(E * g.os * g.os * sizeof(double));
+    size_t o_a = 0, o_b = 0;
+    if (c->precision == CQPI_PREC_MIXED) {
+        o_a = carve(E * sizeof(double));
+        o_b = carve(E * mix_plane_halves(g, c->mg) * sizeof(_Float16));
+    } else if (c->precision == CQPI_PREC_FP32) {
+        o_a = carve(E * g.nw * g.nw * NP * sizeof(float2));
+        o_b = carve(E * NP * sizeof(float2));
+    }
+    if (off > (size_t)CQPI_NOALLOC_SCRATCH_BYTES)
+        return fail(CQPI_E_ARG, "this call needs " + std::to_string(off) + " bytes of scratch, more than the fixed " +
+                                    std::to_string((size_t)CQPI_NOALLOC_SCRATCH_BYTES) +
+                                    ": pass fewer energies per call");
+    unsigned char *sp = static_cast<unsigned char *>(base);
+    const double2 *G0w = reinterpret_cast<const double2 *>(d_g0win);
+    const double2 *T = reinterpret_cast<const double2 *>(d_T);
+    double *bulk = reinterpret_cast<double *>(sp + o_bulk);
+    const int En = (int)E;
+    hipStream_t s = c->stream;
+    RETCHK(launch_bulk(g, G0w, c->d_tiles, En, bulk, s));
+    int r = 0;
+    switch (c->precision) {
+    case CQPI_PREC_MIXED: {
+        double *scale = reinterpret_cast<double *>(sp + o_a);
+        _Float16 *planes = reinterpret_cast<_Float16 *>(sp + o_b);
+        r = launch_mix_prepare(g, c->mg, G0w, En, scale, planes, s);
+        if (!r) r = launch_contract_mixed(g, c->mg, planes, c->d_afrag, scale, c->sA, T, bulk, En, d_ldos, s);
+        break;
+    }
+    case CQPI_PREC_FP32:
+        r = launch_contract_fp32(g, G0w, T, bulk, c->d_tilesf, En, d_ldos, reinterpret_cast<float2 *>(sp + o_a),
+                                 reinterpret_cast<float2 *>(sp + o_b), s);
+        break;
+    default:
+        r = launch_contract_fp64(g, G0w, T, bulk, c->d_tiles, En, d_ldos, s);
+        break;
+    }
+    if (r) return fail(r, "contraction launch failed");
+    return 0;
 }
 
 int cqpi_retrieve(cqpi_ctx *c, double *ldos) { return cqpi_retrieve_layers(c, 0, 1, ldos); }

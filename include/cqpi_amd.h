@@ -127,6 +127,29 @@ int cqpi_wannierldos(cqpi_ctx *ctx, const double *scat, const double *const *g0,
 int cqpi_wannierjosephson(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda, double tip_re,
                           double tip_im);
 
+/* Size in bytes of the fixed per-device scratch of cqpi_wannierldos_device. */
+#define CQPI_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cqpi_wannierldos for E energies whose inputs are
+ * already in device memory, with no allocation of any kind.
+ *   d_T      E matrices T, b x b complex row-major each
+ *   d_g0win  E windows of G0(R), the layout of cqpi_retrieve_g0_window
+ *            (side nw = 2h+1, h from cqpi_window_half), one after the other
+ *   d_ldos   E maps of W x W doubles (output)
+ * The context's precision applies (FP64, FP32 or mixed; set it beforehand:
+ * cqpi_set_precision allocates what the mode keeps per context).  The
+ * periodic table, and the fp16 planes and scales (mixed) or FP32 copies
+ * (FP32), live in one fixed scratch array of CQPI_NOALLOC_SCRATCH_BYTES per
+ * device that the library holds until the process ends; a call needing more
+ * returns CQPI_E_ARG (pass fewer energies per call).  Every byte of scratch a
+ * call reads is written by that call first.  The kernels run on the
+ * context's stream and the call returns without waiting (cqpi_retrieve does
+ * not apply: the maps are in d_ldos).  Calls that overlap in time on one GPU,
+ * from any context or thread, share that scratch and race. */
+int cqpi_wannierldos_device(cqpi_ctx *ctx, const double *d_T, const double *d_g0win, size_t E, double *d_ldos);
+
 /* Replaces HipQPI::HipQPI(wanniern, kpoints, n, window, oversamp, bands,
  * flaglist) (src/qpihipkernel.h:76, src/qpihipkernel.cc:431-493): a context
  * for the threshold-list contraction.  counts holds the (2w+1)^4 list
