@@ -162,19 +162,80 @@ constexpr int stockham_lanes(int N)
     return mx < 64 ? mx : 64;
 }
 
-// FP64 lines are stored padded: point i at pidx(i) = i + i/8 + i/64, so that
-// the strided Stockham stores of 16-byte points (stride R, then blocks of 8 at
-// stride 64) spread over the LDS banks; a padded line of N points takes
-// padded_len(N) slots.  FP32 lines (8-byte points) measured faster unpadded.
+// Lines are stored padded so that the strided Stockham stores spread over the
+// LDS banks.  FP64 lines (16-byte points): point i at i + i/8 + i/64.  FP32
+// lines (8-byte points): point i at i + i/8, which takes the stores of the
+// first pass (stride 8 points: 16 lanes on one bank) and of the second
+// (blocks of 8 at stride 64) to two lanes a bank, while the loads at stride
+// N/8 stay contiguous; the line stride is odd, so that the lanes reading
+// one slot of several lines fall on different banks.  A padded line of N
+// points takes padded_len(N) slots.
+#ifndef CQPI_FFT32_PAD
+#define CQPI_FFT32_PAD 1
+#endif
 template <typename C>
 __host__ __device__ constexpr int pidx(int i)
 {
-    return sizeof(C) == 16 ? i + (i >> 3) + (i >> 6) : i;
+    return sizeof(C) == 16 ? i + (i >> 3) + (i >> 6) : (CQPI_FFT32_PAD ? i + (i >> 3) : i);
 }
 template <typename C>
 __host__ __device__ constexpr int padded_len(int N)
 {
-    return sizeof(C) == 16 ? N + (N >> 3) + (N >> 6) : N;
+    return sizeof(C) == 16 ? N + (N >> 3) + (N >> 6) : (CQPI_FFT32_PAD ? N + (N >> 3) + 1 : N);
+}
+
+// FP32 lines take their twiddles from a per-pass table in LDS: the pass of
+// span L and radix R (L > 1) holds w^(j r N/(L R)) for j < L, r = 1..R-1 at
+// [tw_pass_off(N, L) + j (R-1) + r - 1], so that the R-1 twiddles of a
+// butterfly are adjacent and the lanes of a wave (consecutive j) read
+// addresses R-1 points apart, not j r apart.  At most N entries in all.
+#ifndef CQPI_FFT32_TWP
+#define CQPI_FFT32_TWP 1
+#endif
+template <typename C>
+__host__ __device__ constexpr bool pass_twiddles()
+{
+    return sizeof(C) == 8 && CQPI_FFT32_TWP;
+}
+__host__ __device__ constexpr int tw_pass_off(int N, int Lstop)
+{
+    int lg = 0;
+    while ((1 << lg) < N) lg++;
+    int L = 1, off = 0;
+    while (lg > 0 && L < Lstop) {
+        const int R = (lg >= 3 && lg != 4) ? 8 : (lg >= 2 ? 4 : 2);
+        if (L > 1) off += L * (R - 1);
+        L *= R;
+        lg -= R == 8 ? 3 : (R == 4 ? 2 : 1);
+    }
+    return off;
+}
+// fills the per-pass table dst (tw_pass_off(N, N) entries) from the natural
+// table tw[m] = w^m, m < N, with the threads of the workgroup
+template <int N, typename C>
+__device__ inline void build_pass_twiddles(C *dst, const C *__restrict__ tw)
+{
+    constexpr int total = tw_pass_off(N, N);
+    for (int m = threadIdx.x; m < total; m += blockDim.x) {
+        int lg = 0;
+        while ((1 << lg) < N) lg++;
+        int L = 1, off = 0, idx = 0;
+        while (lg > 0) {
+            const int R = (lg >= 3 && lg != 4) ? 8 : (lg >= 2 ? 4 : 2);
+            if (L > 1) {
+                const int sz = L * (R - 1);
+                if (m < off + sz) {
+                    const int u = m - off, j = u / (R - 1), r = u - j * (R - 1) + 1;
+                    idx = j * r * (N / (L * R));
+                    break;
+                }
+                off += sz;
+            }
+            L *= R;
+            lg -= R == 8 ? 3 : (R == 4 ? 2 : 1);
+        }
+        dst[m] = tw[idx];
+    }
 }
 
 template <int N, int L, int R, int LN, typename C>
@@ -197,8 +258,14 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
         if (k < NB) {
             const int j = k & (L - 1);
             if constexpr (L > 1) {
+                if constexpr (pass_twiddles<C>()) {
+                    const C *tp = tw + tw_pass_off(N, L) + j * (R - 1);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw[j * r * (N / (L * R))]);
+                    for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tp[r - 1]);
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tw[j * r * (N / (L * R))]);
+                }
             }
             dft_small<R>(v[b], sign);
             const int base = (k - j) * R + j;
@@ -220,7 +287,8 @@ __device__ inline void stockham_rest(C *X, const C *__restrict__ tw, int ll, int
 }
 
 // transform the padded line X (N points, C = double2 or float2, twiddle table
-// of the same type) with lanes ll in [0, stockham_lanes(N))
+// of the same type: the natural table, or for FP32 lines the per-pass table
+// of build_pass_twiddles) with lanes ll in [0, stockham_lanes(N))
 template <int N, typename C>
 __device__ inline void wave_fft(C *X, const C *__restrict__ tw, int ll, int sign)
 {

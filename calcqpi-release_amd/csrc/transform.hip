@@ -167,21 +167,27 @@ typedef _Float16 planes_h8 __attribute__((ext_vector_type(8)));
 // Each item fills 8 consecutive columns, one column group of plane_off (Ly
 // is a multiple of 8), and stores them as one 16-byte hi and one 16-byte lo
 // vector.
+// multiply-high divisors (udiv_magic) of the item decomposition, computed on
+// the host: column groups of a plane, pairs of the chunk, bands
+struct PlaneDiv {
+    unsigned long long ng, npc, b;
+};
+
 template <typename C, bool PAD>
 __device__ inline void write_planes(const C *S, int K, int sy, int e, int p0, int npc, int B, int nw, int h,
-                                    const MixGeom &mg, double s, _Float16 *__restrict__ P)
+                                    const MixGeom &mg, double s, _Float16 *__restrict__ P, const PlaneDiv &dv)
 {
     const size_t plane = (size_t)mg.Ly * nw;
     _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
     const int ng = mg.Ly >> 3;
     const bool pow2 = (K & (K - 1)) == 0;
     for (int t = threadIdx.x; t < 2 * npc * 2 * ng; t += blockDim.x) {
-        const int gi = t % ng;
-        int rest = t / ng;
+        int rest = udiv(t, dv.ng);
+        const int gi = t - rest * ng;
         const int ri = rest & 1;
         rest >>= 1;
-        const int q = rest % npc, ab = rest / npc;
-        const int pr = p0 + q, r = pr / B, c = pr - r * B;
+        const int ab = udiv(rest, dv.npc), q = rest - ab * npc;
+        const int pr = p0 + q, r = udiv(pr, dv.b), c = pr - r * B;
         // a: o = r, p = c;  b: G0[r][c] = G0(-R)^T[c][r], o = c, p = r
         const int row = ab ? 2 * h - sy : sy;
         const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
@@ -262,7 +268,8 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
         __syncthreads();
     }
     if (!P) return;
-    write_planes<double2, false>(S, K, sy, e, p0, npc, B, nw, h, mg, sG * (pow2 ? inv : 1.0), P);
+    const PlaneDiv dv{udiv_magic((unsigned)(mg.Ly >> 3)), udiv_magic((unsigned)npc), udiv_magic((unsigned)B)};
+    write_planes<double2, false>(S, K, sy, e, p0, npc, B, nw, h, mg, sG * (pow2 ? inv : 1.0), P, dv);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,6 +282,16 @@ constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
 constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in registers
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
+// FP32 lines take up to 16 waves a workgroup, so that every orbital pair of
+// up to 4 bands is one line of one chunk (3 bands: 9 waves for 9 lines; with
+// 8 waves the pairs were split 5 + 4 over two chunks, each re-reading the
+// eigen data, and half the waves idled in the line transforms); FP64 lines
+// keep 8 waves (9 FP64 lines do not leave room for two workgroups a CU)
+constexpr int kWaveMaxThreads32 = 1024;
+__host__ __device__ constexpr int wave_max_threads(size_t csize)
+{
+    return csize == 8 ? kWaveMaxThreads32 : kWaveMaxThreads;
+}
 // FP32 pass-1 workgroups are held to 80 VGPRs a lane, so that 6 waves share
 // a SIMD (three workgroups of 512 per CU): measured 20% faster than the
 // compiler's 94 VGPRs (5 waves, two workgroups); FP64 ones keep theirs
@@ -317,8 +334,87 @@ __device__ inline C to_c(double2 z)
 template <int B, int N>
 constexpr bool res_regs() { return B > 0 && B <= kMaxRegB && N <= kWaveMaxThreads; }
 
-template <typename C, int B, int N>
-__global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 : 1) k_res_ky_s(Geom g, const double *__restrict__ eval,
+// G0(k) of every orbital pair of one k-point from its eigen data held in
+// registers: P[n] = v_n v_n^H in its upper triangle (diagonal real), so that
+// G_rc = sum_n f_n P_rc and G_cr = sum_n f_n conj(P_rc) share their products
+template <typename C, int B>
+struct EigenPairs {
+    using R = decltype(C{}.x);
+    R d[B][B];                                // d[n][r] = |v_rn|^2
+    C o[B][B * (B - 1) / 2 > 0 ? B * (B - 1) / 2 : 1];  // o[n][upper (r, c)] = v_rn conj(v_cn)
+};
+
+template <typename C, int B>
+__device__ inline void pairs_from_vectors(EigenPairs<C, B> &P, const double2 *v)  // v[r*B + n]
+{
+    using R = typename EigenPairs<C, B>::R;
+#pragma unroll
+    for (int n = 0; n < B; n++) {
+        int u = 0;
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+            const double2 a = v[r * B + n];
+            P.d[n][r] = (R)(a.x * a.x + a.y * a.y);
+#pragma unroll
+            for (int c = r + 1; c < B; c++, u++) {
+                const double2 z = cmulc(a, v[c * B + n]);
+                P.o[n][u].x = (R)z.x;
+                P.o[n][u].y = (R)z.y;
+            }
+        }
+    }
+}
+
+// S[(r*B + c) * NL + pidx(j)] = sum_n f_n P[n]_rc for all B*B pairs
+template <typename C, int B, int NL>
+__device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &P, const C *f)
+{
+    using R = typename EigenPairs<C, B>::R;
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        R gx = 0, gy = 0;
+#pragma unroll
+        for (int n = 0; n < B; n++) {
+            gx += f[n].x * P.d[n][r];
+            gy += f[n].y * P.d[n][r];
+        }
+        C gz;
+        gz.x = gx;
+        gz.y = gy;
+        S[(size_t)(r * B + r) * NL + cqpi::pidx<C>(j)] = gz;
+    }
+    int u = 0;
+#pragma unroll
+    for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = r + 1; c < B; c++, u++) {
+            // f P = (a - b) + i (c' + d), f conj(P) = (a + b) + i (d - c')
+            R a = 0, b = 0, cc = 0, d = 0;
+#pragma unroll
+            for (int n = 0; n < B; n++) {
+                a += f[n].x * P.o[n][u].x;
+                b += f[n].y * P.o[n][u].y;
+                cc += f[n].x * P.o[n][u].y;
+                d += f[n].y * P.o[n][u].x;
+            }
+            C up, lo;
+            up.x = a - b;
+            up.y = cc + d;
+            lo.x = a + b;
+            lo.y = d - cc;
+            S[(size_t)(r * B + c) * NL + cqpi::pidx<C>(j)] = up;
+            S[(size_t)(c * B + r) * NL + cqpi::pidx<C>(j)] = lo;
+        }
+}
+
+// waves per SIMD pass 1 is compiled for: FP32 lines 6 (80 VGPRs), but 5 for
+// the 9-wave workgroups of one chunk of every pair (two of them fill a CU
+// either way, and 96 VGPRs hold the pair products without spilling)
+template <typename C, bool ALLP>
+constexpr int res_min_waves() { return sizeof(C) == 8 ? (ALLP ? 5 : kResWaves32) : 1; }
+
+template <typename C, int B, int N, bool ALLP = false>
+__global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C, ALLP>())) k_res_ky_s(Geom g, const double *__restrict__ eval,
                                                               const double2 *__restrict__ evec,
                                                               const double *__restrict__ omegas, int E, int EB,
                                                               double eta, const C *__restrict__ tw, int np,
@@ -331,7 +427,9 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
     // FP32 lines read the twiddles from an LDS copy; FP64 ones (which measured
     // slower that way) from the cached global table
     C *Twl = S + (size_t)np * NL;
-    if constexpr (sizeof(C) == 8)
+    if constexpr (cqpi::pass_twiddles<C>())
+        cqpi::build_pass_twiddles<N>(Twl, tw);
+    else if constexpr (sizeof(C) == 8)
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
     const C *Tw = sizeof(C) == 8 ? Twl : tw;
     constexpr bool kRegs = res_regs<B, N>();  // blockDim >= N (launcher)
@@ -343,16 +441,27 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
     const unsigned long long dm = udiv_magic(npc);
 
+    // ALLP (one chunk of every pair): G0(k) from the Hermitian pair
+    // products; a partial chunk from the vectors
+    constexpr bool allp = kRegs && ALLP;
     double ev_r[kRegs ? B : 1];
-    C vv_r[kRegs ? B * B : 1];
+    C vv_r[kRegs && !allp ? B * B : 1];
+    EigenPairs<C, allp ? B : 1> pp;
     if constexpr (kRegs) {
         const int j = threadIdx.x;
         if (j < N) {
             const long k = (long)i * N + j;
 #pragma unroll
             for (int n = 0; n < B; n++) ev_r[n] = eval[n * KK + k];
+            if constexpr (allp) {
+                double2 v[B * B];
 #pragma unroll
-            for (int r = 0; r < B * B; r++) vv_r[r] = to_c<C>(evec[r * KK + k]);
+                for (int r = 0; r < B * B; r++) v[r] = evec[r * KK + k];
+                pairs_from_vectors<C, B>(pp, v);
+            } else {
+#pragma unroll
+                for (int r = 0; r < B * B; r++) vv_r[r] = to_c<C>(evec[r * KK + k]);
+            }
         }
     }
     for (int e = e0; e < e1; e++) {
@@ -363,6 +472,9 @@ __global__ void __launch_bounds__(kWaveMaxThreads, sizeof(C) == 8 ? kResWaves32 
                 C f[B];
 #pragma unroll
                 for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
+                if constexpr (allp)
+                    resolvent_all_pairs<C, B, NL>(S, j, pp, f);
+                else
                 for (int q = 0; q < npc; q++) {
                     const int pr = p0 + q, r = pr / B, c = pr - r * B;
                     C acc = to_c<C>(make_double2(0.0, 0.0));
@@ -485,7 +597,7 @@ template <typename C, int N>
 __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
                                                                  const C *__restrict__ tw, int np, int np1, int rwin,
                                                                  double sG, double2 *__restrict__ G0w,
-                                                                 _Float16 *__restrict__ P)
+                                                                 _Float16 *__restrict__ P, PlaneDiv dv)
 {
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
@@ -494,7 +606,9 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     // FP32 lines read the twiddles from an LDS copy; FP64 ones (which measured
     // slower that way) from the cached global table
     C *Twl = S + (size_t)np * NL;
-    if constexpr (sizeof(C) == 8)
+    if constexpr (cqpi::pass_twiddles<C>())
+        cqpi::build_pass_twiddles<N>(Twl, tw);
+    else if constexpr (sizeof(C) == 8)
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
     const C *Tw = sizeof(C) == 8 ? Twl : tw;
     // neighbouring Ry slots read the same cache lines of F1: keep them on one XCD
@@ -507,13 +621,40 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
     // pair p sits in pass-1 chunk c = p / np1, at c np1 nw + slot npc1(c) + p - c np1
     // of its (e, kx row) block
     const C *in = F1 + (size_t)e * N * nw * NP;
-    const unsigned long long dm = udiv_magic(npc);
-    for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
-        const int i = udiv(t, dm), q = t - i * npc, p = p0 + q;
-        int c0 = 0;
-        while (c0 + np1 <= p) c0 += np1;
-        const int npc1 = min(np1, NP - c0);
-        S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * nw * NP + (size_t)c0 * nw + sy * npc1 + (p - c0)];
+    if (np1 == NP && npc == NP) {
+        // one pass-1 chunk: the slot's pairs are one run of NP values per kx
+        // row; a batch of loads is in flight before their LDS stores
+        constexpr int U = 8;
+        const int tot = N * NP;
+        const C *src = in + (size_t)sy * NP;
+        for (int t0 = threadIdx.x; t0 < tot; t0 += U * blockDim.x) {
+            C v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * blockDim.x;
+                if (t < tot) {
+                    const int i = udiv(t, dv.npc);
+                    v[u] = src[(size_t)i * nw * NP + (t - i * NP)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * blockDim.x;
+                if (t < tot) {
+                    const int i = udiv(t, dv.npc);
+                    S[(size_t)(t - i * NP) * NL + cqpi::pidx<C>(i)] = v[u];
+                }
+            }
+        }
+    } else {
+        const unsigned long long dm = udiv_magic(npc);
+        for (int t = threadIdx.x; t < N * npc; t += blockDim.x) {
+            const int i = udiv(t, dm), q = t - i * npc, p = p0 + q;
+            int c0 = 0;
+            while (c0 + np1 <= p) c0 += np1;
+            const int npc1 = min(np1, NP - c0);
+            S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * nw * NP + (size_t)c0 * nw + sy * npc1 + (p - c0)];
+        }
     }
     lds_barrier();
     for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
@@ -529,7 +670,11 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
             out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
         }
     }
-    if (P) write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P);
+    if (P) {
+        PlaneDiv d2 = dv;
+        if (npc != np) d2.npc = udiv_magic((unsigned)npc);  // the last, shorter chunk
+        write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P, d2);
+    }
 }
 
 constexpr size_t kLdsBudget = 80 * 1024;   // largest line buffer a workgroup may take
@@ -550,20 +695,31 @@ int pairs_per_chunk(int K, int NP)
 // on pass 1, at least one thread per k-point (up to kWaveMaxThreads)
 constexpr size_t kWaveLds = 96 * 1024;
 
-int wave_pairs(int N, int NP, size_t csize)
+int wave_pairs_max(int N, int NP, size_t csize, int maxthreads)
 {
-    const int groups = kWaveMaxThreads / cqpi::stockham_lanes(N);
+    const int groups = maxthreads / cqpi::stockham_lanes(N);
     const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
 
-int wave_threads(int N, int np, bool pass1)
+// FP32 lines: the larger workgroups only where they leave fewer idle lane
+// groups (chunks x lines per chunk), e.g. 9 pairs as one chunk of 9, not 5 + 4
+int wave_pairs(int N, int NP, size_t csize)
+{
+    const int np = wave_pairs_max(N, NP, csize, kWaveMaxThreads);
+    if (csize == 16) return np;
+    const int npw = wave_pairs_max(N, NP, csize, kWaveMaxThreads32);
+    const int slots = (NP + np - 1) / np * np, slotsw = (NP + npw - 1) / npw * npw;
+    return slotsw < slots ? npw : np;
+}
+
+int wave_threads(int N, int np, bool pass1, size_t csize)
 {
     int t = np * cqpi::stockham_lanes(N);
     if (pass1) t = std::max(t, std::min(N, kWaveMaxThreads));
     t = (t + 63) / 64 * 64;
-    return std::min(t, kWaveMaxThreads);
+    return std::min(t, wave_max_threads(csize));
 }
 
 // energies per G(k) block of the workspace path
@@ -581,7 +737,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int NP = g.B * g.B;
     const int np = wave_pairs(N, NP, sizeof(C));
     const int nch = (NP + np - 1) / np;
-    const int thr = wave_threads(N, np, true);
+    const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     if constexpr (!res_regs<B, N>()) {
@@ -601,15 +757,21 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
         return 0;
     }
     // energies per workgroup: FP32 lines, with several workgroups resident
-    // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2.  The
-    // energy blocks are made even (51 energies: 4 blocks of 13, not 12,12,12,12,3)
+    // per CU, want ~16 workgroups per CU (a short tail); FP64 ones ~2; the
+    // 9-wave workgroups of one chunk of every pair ~4 (two blocks of energies
+    // for 512 rows: 0.265 ms against 0.29 with four, 3 orbitals, K = 512).
+    // The energy blocks are made even (51 energies: 4 blocks of 13, not 12,12,12,12,3)
     const int rows = N * nch;
-    const int wg_target = sizeof(C) == 8 ? 4096 : 512;
+    const int wg_target = sizeof(C) == 8 ? (nch == 1 && NP > 4 ? 1024 : 4096) : 512;
     const int nb_want = std::max(1, std::min(E, (wg_target + rows / 2) / rows));
     const int EB = g.eb > 0 ? std::min(E, g.eb) : (E + nb_want - 1) / nb_want;
     const int nb = (E + EB - 1) / EB;
-    hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
-                       eta, tw, np, F1, (const C *)nullptr);
+    if (nch == 1)
+        hipLaunchKernelGGL((k_res_ky_s<C, B, N, true>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E,
+                           EB, eta, tw, np, F1, (const C *)nullptr);
+    else
+        hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E, EB,
+                           eta, tw, np, F1, (const C *)nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -621,7 +783,7 @@ int launch_res_gk(const Geom &g, const C *Gk, const double *omegas, int E, const
     const int NP = g.B * g.B;
     const int np = wave_pairs(N, NP, sizeof(C));
     const int nch = (NP + np - 1) / np;
-    const int thr = wave_threads(N, np, true);
+    const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, 0, N>), dim3(N, E, nch), dim3(thr), lds, s, g, (const double *)nullptr,
@@ -662,8 +824,9 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     rwin = std::min(std::max(rwin, 0), g.h);
+    const PlaneDiv dv{udiv_magic((unsigned)std::max(m.Ly >> 3, 1)), udiv_magic((unsigned)np), udiv_magic((unsigned)g.B)};
     hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, rwin, sG, G0w,
-                       P);
+                       P, dv);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
