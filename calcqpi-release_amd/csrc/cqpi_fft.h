@@ -173,15 +173,18 @@ constexpr int stockham_lanes(int N)
 #ifndef CQPI_FFT32_PAD
 #define CQPI_FFT32_PAD 1
 #endif
-template <typename C>
+// LP = false keeps FP32 lines unpadded (with the natural twiddle table), as
+// the first pass does for K = 1024: there its workgroups take one energy each,
+// and the padded layout measured slower.
+template <typename C, bool LP = true>
 __host__ __device__ constexpr int pidx(int i)
 {
-    return sizeof(C) == 16 ? i + (i >> 3) + (i >> 6) : (CQPI_FFT32_PAD ? i + (i >> 3) : i);
+    return sizeof(C) == 16 ? i + (i >> 3) + (i >> 6) : (LP && CQPI_FFT32_PAD ? i + (i >> 3) : i);
 }
-template <typename C>
+template <typename C, bool LP = true>
 __host__ __device__ constexpr int padded_len(int N)
 {
-    return sizeof(C) == 16 ? N + (N >> 3) + (N >> 6) : (CQPI_FFT32_PAD ? N + (N >> 3) + 1 : N);
+    return sizeof(C) == 16 ? N + (N >> 3) + (N >> 6) : (LP && CQPI_FFT32_PAD ? N + (N >> 3) + 1 : N);
 }
 
 // FP32 lines take their twiddles from a per-pass table in LDS: the pass of
@@ -192,10 +195,10 @@ __host__ __device__ constexpr int padded_len(int N)
 #ifndef CQPI_FFT32_TWP
 #define CQPI_FFT32_TWP 1
 #endif
-template <typename C>
+template <typename C, bool LP = true>
 __host__ __device__ constexpr bool pass_twiddles()
 {
-    return sizeof(C) == 8 && CQPI_FFT32_TWP;
+    return sizeof(C) == 8 && LP && CQPI_FFT32_TWP;
 }
 __host__ __device__ constexpr int tw_pass_off(int N, int Lstop)
 {
@@ -238,7 +241,7 @@ __device__ inline void build_pass_twiddles(C *dst, const C *__restrict__ tw)
     }
 }
 
-template <int N, int L, int R, int LN, typename C>
+template <int N, int L, int R, int LN, bool LP, typename C>
 __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     constexpr int NB = N / R;
@@ -249,7 +252,7 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
         const int k = ll + LN * b;
         if (k < NB)
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = X[pidx<C>(k + r * NB)];
+            for (int r = 0; r < R; r++) v[b][r] = X[pidx<C, LP>(k + r * NB)];
     }
     wave_sync();
 #pragma unroll
@@ -258,7 +261,7 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
         if (k < NB) {
             const int j = k & (L - 1);
             if constexpr (L > 1) {
-                if constexpr (pass_twiddles<C>()) {
+                if constexpr (pass_twiddles<C, LP>()) {
                     const C *tp = tw + tw_pass_off(N, L) + j * (R - 1);
 #pragma unroll
                     for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], tp[r - 1]);
@@ -270,31 +273,31 @@ __device__ inline void stockham_pass(C *X, const C *__restrict__ tw, int ll, int
             dft_small<R>(v[b], sign);
             const int base = (k - j) * R + j;
 #pragma unroll
-            for (int q = 0; q < R; q++) X[pidx<C>(base + q * L)] = v[b][q];
+            for (int q = 0; q < R; q++) X[pidx<C, LP>(base + q * L)] = v[b][q];
         }
     }
     wave_sync();
 }
 
-template <int N, int L, int LGREM, int LN, typename C>
+template <int N, int L, int LGREM, int LN, bool LP, typename C>
 __device__ inline void stockham_rest(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     if constexpr (LGREM > 0) {
         constexpr int R = stockham_radix(LGREM);
-        stockham_pass<N, L, R, LN>(X, tw, ll, sign);
-        stockham_rest<N, L * R, LGREM - ilog2c(R), LN>(X, tw, ll, sign);
+        stockham_pass<N, L, R, LN, LP>(X, tw, ll, sign);
+        stockham_rest<N, L * R, LGREM - ilog2c(R), LN, LP>(X, tw, ll, sign);
     }
 }
 
 // transform the padded line X (N points, C = double2 or float2, twiddle table
-// of the same type: the natural table, or for FP32 lines the per-pass table
-// of build_pass_twiddles) with lanes ll in [0, stockham_lanes(N))
-template <int N, typename C>
+// of the same type: the natural table, or for padded FP32 lines the per-pass
+// table of build_pass_twiddles) with lanes ll in [0, stockham_lanes(N))
+template <int N, bool LP = true, typename C>
 __device__ inline void wave_fft(C *X, const C *__restrict__ tw, int ll, int sign)
 {
     constexpr int LN = stockham_lanes(N);
     constexpr int lg = ilog2n(N);
-    stockham_rest<N, 1, lg, LN>(X, tw, ll, sign);
+    stockham_rest<N, 1, lg, LN, LP>(X, tw, ll, sign);
 }
 
 }  // namespace cqpi

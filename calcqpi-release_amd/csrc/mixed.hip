@@ -331,13 +331,18 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
     const int RW = TPW * CPS + 2 * w, RP = RW | 1, seg = mg.NUg;
     const int cy_lo = by * TPW * CPS;
     const int row_lo = cy_lo - n2 - w + h;
-    // (consecutive threads take consecutive rows: contiguous in the planes)
-    for (int l = threadIdx.x; l < 4 * mg.NPL * RW * seg; l += blockDim.x) {
-        const int r = l % RW, rest = l / RW, g8 = rest % seg, pid = rest / seg;
-        const int row = row_lo + r;
-        half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
-        if (row < nw) v = load8<LDW>(Pe + (size_t)pid * plane, row, col0 + g8 * 8, nw);
-        *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
+    // (consecutive threads take consecutive rows: contiguous in the planes;
+    // multiply-high divisions instead of runtime ones)
+    {
+        const unsigned long long mrw = udiv_magic((unsigned)RW), mseg = udiv_magic((unsigned)seg);
+        for (int l = threadIdx.x; l < 4 * mg.NPL * RW * seg; l += blockDim.x) {
+            const int rest = udiv(l, mrw), r = l - rest * RW;
+            const int pid = udiv(rest, mseg), g8 = rest - pid * seg;
+            const int row = row_lo + r;
+            half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (row < nw) v = load8<LDW>(Pe + (size_t)pid * plane, row, col0 + g8 * 8, nw);
+            *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
+        }
     }
     if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + 4 * mg.NPL * RP * seg * 8) = half8{0, 0, 0, 0, 0, 0, 0, 0};
     __syncthreads();
@@ -541,6 +546,8 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
     const int ZOFF = 4 * NPS * RP * seg * 8;            // 16-byte zero slot
     float *Csw = reinterpret_cast<float *>(Pl + ZOFF + 8) + (size_t)wave * JB * 16 * (16 * NT + 1);
     const int trow = (cy - cy_lo) * seg * 8;
+    const unsigned long long mrw = udiv_magic((unsigned)RW), mseg = udiv_magic((unsigned)seg),
+                             mnps = udiv_magic((unsigned)NPS);
     double val[NB];
 #pragma unroll
     for (int jb = 0; jb < NB; jb++) val[jb] = 0.0;
@@ -549,8 +556,9 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
         __syncthreads();  // the previous spin block's fragments are read
         // (consecutive threads take consecutive rows: contiguous in the planes)
         for (int l = threadIdx.x; l < 4 * NPS * RW * seg; l += blockDim.x) {
-            const int r = l % RW, rest = l / RW, g8 = rest % seg, pid = rest / seg;
-            const int abhl = pid / NPS, pl = sg * NPS + (pid - abhl * NPS);
+            const int rest = udiv(l, mrw), r = l - rest * RW;
+            const int pid = udiv(rest, mseg), g8 = rest - pid * seg;
+            const int abhl = udiv(pid, mnps), pl = sg * NPS + (pid - abhl * NPS);
             const int row = row_lo + r;
             half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
             if (row < nw) v = load8<LDW>(Pe + ((size_t)abhl * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);

@@ -287,10 +287,24 @@ constexpr int kWaveMaxThreads = 512;
 // 8 waves the pairs were split 5 + 4 over two chunks, each re-reading the
 // eigen data, and half the waves idled in the line transforms); FP64 lines
 // keep 8 waves (9 FP64 lines do not leave room for two workgroups a CU)
-constexpr int kWaveMaxThreads32 = 1024;
+#ifndef CQPI_WAVE32_MAX
+#define CQPI_WAVE32_MAX 1024
+#endif
+constexpr int kWaveMaxThreads32 = CQPI_WAVE32_MAX;
 __host__ __device__ constexpr int wave_max_threads(size_t csize)
 {
     return csize == 8 ? kWaveMaxThreads32 : kWaveMaxThreads;
+}
+// the first pass pads its FP32 lines (and takes per-pass twiddles) for
+// K <= 512; at K = 1024 (one energy per workgroup, lines read from the G(k)
+// workspace) the unpadded layout measured faster: 0.78 against 1.0 ms per
+// energy for 10 bands
+__host__ __device__ constexpr bool res_line_padded(int N) { return N <= 512; }
+// LDS slots of one line of the first pass
+inline int res_line_len(int N, size_t csize)
+{
+    return csize == 16 ? cqpi::padded_len<double2>(N)
+                       : (res_line_padded(N) ? cqpi::padded_len<float2, true>(N) : cqpi::padded_len<float2, false>(N));
 }
 // FP32 pass-1 workgroups are held to 80 VGPRs a lane, so that 6 waves share
 // a SIMD (three workgroups of 512 per CU): measured 20% faster than the
@@ -366,7 +380,7 @@ __device__ inline void pairs_from_vectors(EigenPairs<C, B> &P, const double2 *v)
 }
 
 // S[(r*B + c) * NL + pidx(j)] = sum_n f_n P[n]_rc for all B*B pairs
-template <typename C, int B, int NL>
+template <typename C, int B, int NL, bool LP>
 __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &P, const C *f)
 {
     using R = typename EigenPairs<C, B>::R;
@@ -381,7 +395,7 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
         C gz;
         gz.x = gx;
         gz.y = gy;
-        S[(size_t)(r * B + r) * NL + cqpi::pidx<C>(j)] = gz;
+        S[(size_t)(r * B + r) * NL + cqpi::pidx<C, LP>(j)] = gz;
     }
     int u = 0;
 #pragma unroll
@@ -402,8 +416,8 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
             up.y = cc + d;
             lo.x = a + b;
             lo.y = d - cc;
-            S[(size_t)(r * B + c) * NL + cqpi::pidx<C>(j)] = up;
-            S[(size_t)(c * B + r) * NL + cqpi::pidx<C>(j)] = lo;
+            S[(size_t)(r * B + c) * NL + cqpi::pidx<C, LP>(j)] = up;
+            S[(size_t)(c * B + r) * NL + cqpi::pidx<C, LP>(j)] = lo;
         }
 }
 
@@ -423,11 +437,12 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
     constexpr int LN = cqpi::stockham_lanes(N);
-    constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
+    constexpr bool LP = res_line_padded(N);            // padded FP32 lines (K <= 512)
+    constexpr int NL = cqpi::padded_len<C, LP>(N);  // LDS slots per line
     // FP32 lines read the twiddles from an LDS copy; FP64 ones (which measured
     // slower that way) from the cached global table
     C *Twl = S + (size_t)np * NL;
-    if constexpr (cqpi::pass_twiddles<C>())
+    if constexpr (cqpi::pass_twiddles<C, LP>())
         cqpi::build_pass_twiddles<N>(Twl, tw);
     else if constexpr (sizeof(C) == 8)
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
@@ -473,7 +488,7 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
 #pragma unroll
                 for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
                 if constexpr (allp)
-                    resolvent_all_pairs<C, B, NL>(S, j, pp, f);
+                    resolvent_all_pairs<C, B, NL, LP>(S, j, pp, f);
                 else
                 for (int q = 0; q < npc; q++) {
                     const int pr = p0 + q, r = pr / B, c = pr - r * B;
@@ -488,7 +503,7 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
                         }
                         acc = cadd(acc, cmul(f[n], cmulc(vr, vc)));
                     }
-                    S[(size_t)q * NL + cqpi::pidx<C>(j)] = acc;
+                    S[(size_t)q * NL + cqpi::pidx<C, LP>(j)] = acc;
                 }
             }
         }
@@ -498,17 +513,17 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
             const C *gk = Gk + ((size_t)e * NP + p0) * KK + (size_t)i * N;
             for (int t = threadIdx.x; t < npc * N; t += blockDim.x) {
                 const int q = t / N, j = t - q * N;
-                S[(size_t)q * NL + cqpi::pidx<C>(j)] = gk[(size_t)q * KK + j];
+                S[(size_t)q * NL + cqpi::pidx<C, LP>(j)] = gk[(size_t)q * KK + j];
             }
         }
         lds_barrier();
-        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
+        for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N, LP>(S + (size_t)q * NL, Tw, ll, +1);
         lds_barrier();
         // the chunk's part of the (e, kx row) block: one contiguous run
         C *out = F1 + ((size_t)e * N + i) * nw * NP + (size_t)p0 * nw;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int slot = udiv(t, dm), q = t - slot * npc;
-            out[t] = S[(size_t)q * NL + cqpi::pidx<C>((slot - h) & (N - 1))];
+            out[t] = S[(size_t)q * NL + cqpi::pidx<C, LP>((slot - h) & (N - 1))];
         }
         lds_barrier();  // S is rewritten for the next energy
     }
@@ -694,11 +709,21 @@ int pairs_per_chunk(int K, int NP)
 // values per pair), workgroup = enough lane groups for the chunk's lines and,
 // on pass 1, at least one thread per k-point (up to kWaveMaxThreads)
 constexpr size_t kWaveLds = 96 * 1024;
+// FP32 lines: the lines and the twiddle table of a workgroup within 72 KB, so
+// that two workgroups share a CU also at K = 1024 (padded lines of 1153 points)
+#ifndef CQPI_LDS32_KB
+#define CQPI_LDS32_KB 72
+#endif
+constexpr size_t kWaveLds32 = CQPI_LDS32_KB * 1024;
+size_t line_lds_budget(int N, size_t csize)
+{
+    return csize == 16 ? kWaveLds : kWaveLds32 - (size_t)N * csize;
+}
 
 int wave_pairs_max(int N, int NP, size_t csize, int maxthreads)
 {
     const int groups = maxthreads / cqpi::stockham_lanes(N);
-    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
+    const int lines = std::max(1, std::min(std::min(groups, kMaxLines), (int)(line_lds_budget(N, csize) / (csize * (size_t)res_line_len(N, csize)))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
@@ -739,7 +764,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     const int nch = (NP + np - 1) / np;
     const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
-    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
+    const size_t lds = sizeof(C) * ((size_t)np * res_line_len(N, sizeof(C)) + (sizeof(C) == 8 ? N : 0));
     if constexpr (!res_regs<B, N>()) {
         // G(k) through the workspace, a block of energies at a time
         const size_t per_e = (size_t)NP * N * N * sizeof(C);
@@ -785,7 +810,7 @@ int launch_res_gk(const Geom &g, const C *Gk, const double *omegas, int E, const
     const int nch = (NP + np - 1) / np;
     const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
-    const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
+    const size_t lds = sizeof(C) * ((size_t)np * res_line_len(N, sizeof(C)) + (sizeof(C) == 8 ? N : 0));
     hipLaunchKernelGGL((k_res_ky_s<C, 0, N>), dim3(N, E, nch), dim3(thr), lds, s, g, (const double *)nullptr,
                        (const double2 *)nullptr, omegas, E, 1, 0.0, tw, np, F1, Gk);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -810,7 +835,7 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
 // they fit the LDS budget (a split would fetch each row's cache lines twice)
 int kx_pairs(int N, int NP, size_t csize)
 {
-    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(kWaveLds / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
+    const int lines = std::max(1, std::min(1024 / cqpi::stockham_lanes(N), (int)(line_lds_budget(N, csize) / (csize * (size_t)(csize == 16 ? cqpi::padded_len<double2>(N) : cqpi::padded_len<float2>(N))))));
     const int nch = (NP + lines - 1) / lines;
     return (NP + nch - 1) / nch;
 }
