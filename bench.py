@@ -7,16 +7,21 @@ HBM).  Default workload: the 3-orbital square lattice (K=512, 51 energies,
 128x128 cells, os=4, w=2 -> 512x512 pixels), synthetic model drawn from a
 fixed seed.
 
-`value` is the mixed-precision path (split-fp16 MFMA contraction, FP64
-eigensolve / resolvent / transform / T-matrix / periodic term).  The same
-line reports the FP32 and FP64 paths, their ratios, the accuracy against the
-FP64 path and the FP64 CPU oracle, the roofline of the dominant kernel, the
-batched workload (128 uneven instances in one HIP graph), the gradient
-workload and the CPU baseline.  Prints one JSON line.
+`value` is the mixed-precision path: split-fp16 MFMA contraction with FP32
+accumulation, FP32 resolvent and transform lines, FP64 eigensolve, T-matrix,
+periodic term and G0 window.  The same line reports the FP32 and FP64 paths,
+their ratios, the accuracy per energy against the FP64 path and the FP64 CPU
+oracle, the roofline of the dominant kernels, an end-to-end rate that
+includes moving the maps to the host, the 10-band hexagonal workload (K=1024,
+101 energies, 256x256 cells, os=6, w=3), the batched workload (128 uneven
+instances in one HIP graph), the gradient workload, the Josephson,
+threshold-list and surface variants and the CPU baseline.  Prints one JSON
+line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
-                  [--workload NAME] [--energies N]
-  python bench.py --probe        one mixed and one FP64 step after a 256 MiB
+                  [--no-grad] [--no-variants] [--no-hex] [--workload NAME] [--energies N]
+  python bench.py --probe [--workload NAME] [--energies N]
+                                 one mixed and one FP64 step after a 256 MiB
                                  device copy (the traffic calibration), for
                                  rocprofv3 --pmc runs
 """
@@ -47,28 +52,27 @@ def flops_per_point(wl):
     return nsets * (8 * side2 * per * b + 8 * b * b + 8 * b)
 
 
-def stage_bytes(wl, nw, E, csize=16):
-    """Algorithmic HBM bytes of the transform stages for E energies, at the
-    fused minimum: stage 1 (resolvent fused with the ky transform) reads the
-    eigen data once and writes the ky-transformed rows of the window, stage 2
-    reads those rows and writes the G0(R) window (FP64) or, in mixed
-    precision, the fp16 hi/lo planes (the same 16 bytes per complex point).
-    csize: bytes per complex point of the rows (8: FP32 lines)."""
+def fused_transform_bytes(wl, E):
+    """Algorithmic HBM bytes of the resolvent + k->R transform for E energies
+    at the fused minimum: the eigen data read once, K^2 (16 b^2 + 8 b), and
+    the G0(R) window written once per energy, 16 b^2 nk^2 with nk = n + 2w + 2
+    the window side of the reference's copy (src/qpihipkernel.cc:383-393).
+    The intermediate rows between the two passes are not algorithmic bytes:
+    they are overhead traffic."""
     K, b = wl.K, wl.bands
-    eig = K * K * (16 * b * b + 8 * b)
-    f1 = csize * b * b * K * nw
-    win = 16 * b * b * nw * nw
-    return {1: eig + E * f1, 2: E * (f1 + win)}
+    nk = wl.n + 2 * wl.window + 2
+    return K * K * (16 * b * b + 8 * b) + E * 16 * b * b * nk * nk
 
 
-def fft_flops(wl, nw, E):
-    """FP32/FP64 arithmetic of the fused resolvent + ky transform for E
-    energies: the radix-8 line transforms (about 5 N log2 N flops per line of
-    N = K points, b^2 lines per kx row) and the resolvent sandwich (8 flops
-    per pair and band per k-point)."""
+def transform_valu_flops(wl, nw, E):
+    """FP32/FP64 arithmetic of the two transform passes for E energies: the
+    radix-8 line transforms (about 5 N log2 N flops per line of N = K points;
+    b^2 lines per kx row in the first pass, per window column in the second)
+    and the resolvent (8 flops per pair and band per k-point)."""
     K, b = wl.K, wl.bands
     lg = int(np.log2(K))
-    return E * (K * b * b * 5 * K * lg + K * K * b * b * b * 8)
+    line = 5 * K * lg
+    return E * (K * b * b * line + nw * b * b * line + K * K * b * b * b * 8)
 
 
 def rel_errors(x, ref, floor=1e-3):
@@ -77,13 +81,51 @@ def rel_errors(x, ref, floor=1e-3):
     return float(np.max(np.abs(x - ref) / den)), float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
 
 
+def host_cpu():
+    """Model, sockets, cores and threads of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    model = info.get("Model name")
+    if not model:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            model = None
+    def num(key):
+        try:
+            return int(info[key])
+        except (KeyError, ValueError):
+            return None
+    return {"model": model, "sockets": num("Socket(s)"), "cores_per_socket": num("Core(s) per socket"),
+            "threads_per_core": num("Thread(s) per core"), "cpus": num("CPU(s)"),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def load_oracle():
+    """The FP64 CPU oracle, its OpenMP threads bound close (OMP_PROC_BIND is
+    read when the runtime starts)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
 def cpu_baseline(wl, nsample):
     """The FP64 oracle (C restatement of the reference algorithm with the
     reference loop nest for the contraction, OpenMP over cells) timed on this
     host: eigensolve once plus nsample energies, extrapolated linearly to all
     energies.  Also returns the oracle maps of the sampled energies."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+    oracle = load_oracle()
 
     t0 = time.perf_counter()
     ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
@@ -97,21 +139,23 @@ def cpu_baseline(wl, nsample):
     t_e = (time.perf_counter() - t0) / nsample
     E = len(wl.energies)
     total = t_eig + E * t_e
-    info = {"value": wl.points / total, "unit": "points/s", "cores": oracle.num_threads(),
-            "kind": "port",
+    nt = oracle.num_threads()
+    cpu = host_cpu()
+    info = {"value": wl.points / total, "unit": "points/s", "cores": nt, "kind": "port",
             "sample": "oracle eigensolve + %d of %d energies at full size, extrapolated linearly "
-                      "(%.2f s eigensolve, %.2f s/energy, 1 process, %d OpenMP threads)"
-                      % (nsample, E, t_eig, t_e, oracle.num_threads())}
+                      "(%.2f s eigensolve, %.2f s/energy, 1 process, %d OpenMP threads, OMP_PROC_BIND=%s)"
+                      % (nsample, E, t_eig, t_e, nt, os.environ.get("OMP_PROC_BIND")),
+            "host": cpu, "mpi_ranks": 1, "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
     return info, np.array(maps)
 
 
-def time_steps(q, wl, steps, warmup, torch, stage_events=False, eigen=True):
+def time_steps(q, wl, steps, warmup, torch, stage_events=False, eigen=True, stage_reps=5):
     """Wall time per step (eigensolve + all energies) over `steps` steps after
-    `warmup`, then the stage times of one more step.  stage_events: every
-    step records stage events, which keeps each run on one stream (one launch
-    per stage: what a kernel-trace profile of the stages should see).
-    eigen=False: no eigensolve in the step (the surface Green's function does
-    not use it)."""
+    `warmup`, then the stage times of `stage_reps` more steps (median per
+    stage).  stage_events: every step records stage events, which keeps each
+    run on one stream (one launch per stage: what a kernel-trace profile of
+    the stages should see).  eigen=False: no eigensolve in the step (the
+    surface Green's function does not use it)."""
     def step():
         if eigen:
             q.compute_eigen()
@@ -127,8 +171,11 @@ def time_steps(q, wl, steps, warmup, torch, stage_events=False, eigen=True):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     q.set_timing(True)
-    step()
-    st = q.stage_times()
+    runs = []
+    for _ in range(max(stage_reps, 1)):
+        step()
+        runs.append(q.stage_times())
+    st = {k: (float(np.median([r[k][0] for r in runs])), runs[0][k][1]) for k in runs[0]}
     q.set_timing(stage_events)   # (with stage_events the later accuracy runs stay on one stream too)
     return dt, st
 
@@ -225,16 +272,19 @@ def gradient_timing(cqpi, synth, torch, reps=3):
                     "adjoint; maps and dL/drho stay in HBM"}
 
 
-TRAFFIC_MD = os.path.join(ROOT, "profiles", "pmc_traffic_square3orb_K512_E51.md")
+TRAFFIC_MD = {"square_3orb": os.path.join(ROOT, "profiles", "pmc_traffic_square3orb_K512_E51.md"),
+              "hex_5d_spin": os.path.join(ROOT, "profiles", "pmc_traffic_hex10band_K1024_E4.md")}
 
 
-def measured_traffic(kernel, precision):
-    """HBM bytes per launch of `kernel` (name prefix) in the `precision` step,
-    from the committed rocprofv3 --pmc summary of this workload (FETCH_SIZE +
-    WRITE_SIZE, copy-calibrated), or None."""
+def measured_traffic(kernel, precision, workload="square_3orb"):
+    """HBM bytes per launch of `kernel` (name prefix) in the `precision` step
+    of the workload's probe, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), divided by the probe's
+    energies (the summary records them) so that it can be scaled to another
+    energy count; returns (bytes per launch, energies of that launch) or None."""
     try:
-        txt = open(TRAFFIC_MD).read()
-    except OSError:
+        txt = open(TRAFFIC_MD[workload]).read()
+    except (OSError, KeyError):
         return None
     if "```json" not in txt:
         return None
@@ -244,49 +294,143 @@ def measured_traffic(kernel, precision):
         label = name.rsplit(" (", 1)[1].rstrip(")") if " (" in name else None
         typed = "f32" in name or "f64" in name
         if name.startswith(kernel) and label in (None, precision) and (not typed or line in name):
-            return row.get("bytes_per_launch")
+            return row.get("bytes_per_launch"), table.get("energies")
     return None
 
 
-def roofline(wl, q, st, precision, stage=None):
-    """Roofline of the dominant stage's kernel (or of `stage`, an index into
-    the stage list)."""
-    names = list(st)
-    dom = max(range(len(names)), key=lambda i: st[names[i]][0]) if stage is None else stage
-    ms, nl = st[names[dom]]
-    avg_s = ms / max(nl, 1) / 1e3
+def stage_launch_s(st, name):
+    ms, nl = st[name]
+    return ms / max(nl, 1) / 1e3, nl
+
+
+def transform_roofline(wl, q, st, precision, workload):
+    """Roofline of the fused resolvent + k->R transform (the two passes,
+    k_res_ky_s and k_kx_s, one launch each per run with stage events), against
+    the fused minimum bytes; `traffic` is what the two kernels moved in the
+    committed profile (the ky-transformed rows go out to HBM and back between
+    the passes: overhead traffic).  Their limit is the vector ALU and LDS work
+    of the line transforms, reported as valu_*."""
     E = len(wl.energies)
+    t1, n1 = stage_launch_s(st, "resolvent+transform_ky")
+    t2, n2 = stage_launch_s(st, "transform_kx")
+    runs = max(n1, n2, 1)                    # launches of each pass per run (energy chunks)
+    total = t1 * n1 + t2 * n2                # both passes over all E energies
+    per_run = fused_transform_bytes(wl, E)
+    achieved = per_run / total / 1e9
+    t = total / runs
+    traffic = None
+    m1 = measured_traffic("k_res_ky_s", precision, workload)
+    m2 = measured_traffic("k_kx_s", precision, workload)
+    if m1 and m2 and m1[1] and m2[1]:
+        traffic = (m1[0] / m1[1] + m2[0] / m2[1]) * E
     nw = 2 * q.window_half() + 1
-    kern = {1: "k_res_ky_s", 2: "k_kx_s", 4: "k_cmix_areg" if precision == "mixed" else "k_contract"}.get(dom, "")
-    if dom == 4:
-        per_launch = wl.W * wl.W * E / max(nl, 1) * flops_per_point(wl)
-        achieved = per_launch / avg_s / 1e12
-        peak = PEAK_TFLOPS[precision]
-        roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kern, precision)}
-    elif dom in (1, 2):
-        csize = 16 if precision == "fp64" else 8
-        per_launch = stage_bytes(wl, nw, E, csize)[dom] / max(nl, 1)
-        achieved = per_launch / avg_s / 1e9
-        roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern, precision),
-                "algorithmic_bytes_per_launch": per_launch,
-                "note": "algorithmic bytes = fused minimum: eigen data read once and the ky-transformed "
-                        "window rows written once, at the line precision (FP32 in the reduced-precision "
-                        "modes).  The kernel is not HBM-bound: its limit is the vector ALU and LDS work "
-                        "of the line transforms (valu_* fields)"}
-        if dom == 1:
-            fl = fft_flops(wl, nw, E) / max(nl, 1)
-            roof["valu_tflops"] = fl / avg_s / 1e12
-            roof["valu_peak_tflops"] = 78.6   # FP32 or FP64 vector FMA without packed FP32 (not used)
-            roof["valu_frac"] = roof["valu_tflops"] / 78.6
-    else:
-        roof = {"kernel": names[dom], "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": None, "traffic": None}
-    roof["avg_launch_ms"] = avg_s * 1e3
-    roof["launches_per_step"] = nl
-    roof["timing"] = "HIP events on the library stream around the launch, inside the timed run"
-    return roof
+    fl = transform_valu_flops(wl, nw, E)
+    line = "FP64" if precision == "fp64" else "FP32"
+    return {"kernel": "resolvent+transform (k_res_ky_s + k_kx_s, %s lines)" % line, "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic / runs if traffic else None, "algorithmic_bytes_per_launch": per_run / runs,
+            "avg_launch_ms": t * 1e3, "pass_ms": [t1 * 1e3, t2 * 1e3], "launches_per_step": runs,
+            "valu_tflops": fl / total / 1e12, "valu_peak_tflops": 78.6, "valu_frac": fl / total / 1e12 / 78.6,
+            "note": "achieved = fused minimum (eigen data read once + the G0 window, 16 b^2 nk^2 per energy, "
+                    "written once) / the time of both passes; traffic (HBM bytes of both kernels, rocprof "
+                    "FETCH_SIZE + WRITE_SIZE scaled to these energies) also moves the ky-transformed rows "
+                    "between the passes.  The passes are bound by the vector ALU and LDS work of the line "
+                    "transforms (valu_*; FP32 and FP64 both 78.6 TFLOP/s without packed FP32)",
+            "timing": "HIP events on the library stream around each launch, inside the bench run"}
+
+
+def contraction_roofline(wl, st, precision, workload, kernel):
+    """MFMA (mixed) or vector-ALU (FP32, FP64) roofline of the contraction
+    with the useful flops per point F of flops_per_point."""
+    E = len(wl.energies)
+    t, nl = stage_launch_s(st, "contraction")
+    per_launch = wl.W * wl.W * E / max(nl, 1) * flops_per_point(wl)
+    achieved = per_launch / t / 1e12
+    peak = PEAK_TFLOPS[precision]
+    m = measured_traffic(kernel, precision, workload)
+    return {"kernel": kernel, "bound": "mfma" if precision == "mixed" else "valu", "achieved": achieved,
+            "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": m[0] / m[1] * E / max(nl, 1) if m and m[1] else None,
+            "flops_per_point": flops_per_point(wl), "avg_launch_ms": t * 1e3, "launches_per_step": nl,
+            "timing": "HIP events on the library stream around the launch, inside the bench run"}
+
+
+def per_energy_errors(a, r, torch):
+    """[max_rel, rms_rel] per energy of maps a against r (E x W*W, device or host)."""
+    if not torch.is_tensor(a):
+        return [list(rel_errors(a[e], r[e])) for e in range(r.shape[0])]
+    m = torch.amax(torch.abs(r), dim=1, keepdim=True)
+    den = torch.maximum(torch.abs(r), 1e-3 * m)
+    mx = torch.amax(torch.abs(a - r) / den, dim=1)
+    rms = torch.linalg.vector_norm(a - r, dim=1) / torch.linalg.vector_norm(r, dim=1)
+    return [[float(x), float(y)] for x, y in zip(mx.cpu(), rms.cpu())]
+
+
+def hex_timing(cqpi, synth, torch, steps, warmup, with_oracle, cells=16, oracle_energy=50):
+    """The 10-band hexagonal workload (5 d orbitals x spin, SOC, full 10x10 V;
+    K=1024, 101 energies, 256x256 cells, os=6, w=3): FP64 and mixed, each step
+    the eigensolve and all 101 energies; errors of every mixed map against the
+    FP64 one on the device; with_oracle: both against the FP64 oracle on a
+    block of cells x cells cells at the centre of one energy's map (the whole
+    map is beyond the reference loop nest on the host)."""
+    wl = synth.workload("hex_5d_spin")
+    E = len(wl.energies)
+    out = {"workload": "hexagonal lattice, 5 d orbitals x spin (10 bands, SOC lambda=0.05), K=1024, 101 energies "
+                       "in [-0.2,0.2], eta=0.001, full Hermitian 10x10 V, 256x256 cells, os=6, w=3 "
+                       "(1536x1536 pixels), spin blocks", "points": wl.points}
+    qs = {}
+    for prec in ("fp64", "mixed"):
+        q = cqpi.engine_for(wl, prec)
+        dt, st = time_steps(q, wl, steps, warmup, torch, stage_reps=2)
+        q.run(wl.energies, wl.eta, host=False)
+        kern = "k_cmix_st" if prec == "mixed" else "k_contract_st"
+        t_eig, _ = stage_launch_s(st, "eigensolve")
+        K, b = wl.K, wl.bands
+        eig_bytes = K * K * (16 * b * b + 8 * b)
+        out[prec] = {"ms_per_step": dt * 1e3, "points_per_s": wl.points / dt,
+                     "stages_ms": {k: v[0] for k, v in st.items()},
+                     "contraction_roofline": contraction_roofline(wl, st, prec, "hex_5d_spin", kern),
+                     "transform_roofline": transform_roofline(wl, q, st, prec, "hex_5d_spin"),
+                     "eigensolve_roofline": {"kernel": "k_eigen_g<10>", "bound": "hbm",
+                                             "achieved": eig_bytes / t_eig / 1e9, "peak": HBM_PEAK_GBS,
+                                             "unit": "GB/s", "frac": eig_bytes / t_eig / 1e9 / HBM_PEAK_GBS,
+                                             "algorithmic_bytes_per_launch": eig_bytes,
+                                             "note": "writes the eigen data once per model; bound by the FP64 "
+                                                     "vector ALU work of the Jacobi sweeps"}}
+        qs[prec] = q
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    a, r = qs["mixed"].ldos_tensor(), qs["fp64"].ldos_tensor()
+    pe = per_energy_errors(a, r, torch)
+    out["accuracy"] = {"mixed_vs_fp64_gpu_max_rel": max(x[0] for x in pe),
+                       "mixed_vs_fp64_gpu_rms_rel": max(x[1] for x in pe),
+                       "mixed_vs_fp64_gpu_per_energy": pe}
+    if with_oracle:
+        oracle = load_oracle()
+        t0 = time.perf_counter()
+        ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+        om = wl.energies[oracle_energy]
+        gR = oracle.realspace(oracle.greens_k(ev, V, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        c0 = wl.n // 2 - cells // 2
+        ref = oracle.wannierldos_cells(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp,
+                                       (c0, c0 + cells, c0, c0 + cells), wl.maxband, wl.spin)
+        del ev, V, gR
+        os_ = wl.oversamp
+        sl = slice(c0 * os_, (c0 + cells) * os_)
+        blk = ref.reshape(wl.W, wl.W)[sl, sl]
+        acc = out["accuracy"]
+        for prec in ("fp64", "mixed"):
+            got = qs[prec].ldos_tensor()[oracle_energy].cpu().numpy().reshape(wl.W, wl.W)[sl, sl]
+            mx, rms = rel_errors(got.ravel(), blk.ravel())
+            acc["%s_vs_oracle_max_rel" % prec] = mx
+            acc["%s_vs_oracle_rms_rel" % prec] = rms
+        acc["oracle_sample"] = ("energy index %d (omega=%.4f), cells [%d,%d)^2 of 256^2 (%d x %d pixels), "
+                                "oracle eigensolve + transform + reference loop nest on those cells, %.1f s"
+                                % (oracle_energy, om, c0, c0 + cells, cells * os_, cells * os_,
+                                   time.perf_counter() - t0))
+    for q in qs.values():
+        q.close()
+    return out
 
 
 def batch_timing(cqpi, synth, torch, count=128, reps=5):
@@ -330,7 +474,7 @@ def batch_timing(cqpi, synth, torch, count=128, reps=5):
                     "per shape before capture"}
 
 
-def probe(cqpi, synth, torch, workload):
+def probe(cqpi, synth, torch, workload, energies=0):
     """One mixed and one FP64 step after a 256 MiB device copy: the run that
     the rocprofv3 --pmc traffic passes profile.  Stage timing is on, which
     keeps each run on one stream, so every stage is one launch over all
@@ -340,13 +484,35 @@ def probe(cqpi, synth, torch, workload):
     torch.cuda.synchronize()
     del src, dst
     wl = synth.workload(workload)
+    if energies:
+        wl.energies = wl.energies[:energies]
     for prec in ("mixed", "fp64"):
         with cqpi.engine_for(wl, prec) as q:
             q.set_timing(True)
             q.compute_eigen()
             q.run(wl.energies, wl.eta, host=False)
     torch.cuda.synchronize()
-    print("probe done: copy 256 MiB, %s one mixed and one fp64 step" % workload, flush=True)
+    print("probe done: copy 256 MiB, %s (%d energies) one mixed and one fp64 step"
+          % (workload, len(wl.energies)), flush=True)
+
+
+def end_to_end(q, wl, steps, torch):
+    """The mixed step with the maps moved to the host as well (cqpi_run with
+    a host buffer: the driver's path up to the idl write)."""
+    host = np.empty((len(wl.energies), wl.W * wl.W))
+    q.compute_eigen()
+    q.run(wl.energies, wl.eta, host=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        q.compute_eigen()
+        q.run(wl.energies, wl.eta, host=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    del host
+    return {"ms_per_step": dt * 1e3, "points_per_s": wl.points / dt,
+            "map_bytes": wl.points * 8,
+            "note": "mixed step plus the copy of all E maps (FP64) to pageable host memory; value leaves them in HBM"}
 
 
 def main():
@@ -359,6 +525,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-hex", action="store_true", help="skip the 10-band hexagonal workload")
     ap.add_argument("--no-variants", action="store_true", help="skip the Josephson, threshold-list and surface Green's function maps")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--stage-events", action="store_true",
@@ -366,8 +533,8 @@ def main():
     ap.add_argument("--cpu-energies", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU (--gpus 1); the multi-GPU energy sharding is a design, "
-              "not run in this build", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): this build runs on one GPU; the energy sharding over "
+              "several GPUs is a design only", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -378,12 +545,14 @@ def main():
     if cqpi.device_count() < 1:
         raise SystemExit("no HIP device")
     if args.probe:
-        probe(cqpi, synth, torch, args.workload)
+        probe(cqpi, synth, torch, args.workload, args.energies)
         return
     wl = synth.workload(args.workload)
     if args.energies:
         wl.energies = wl.energies[:args.energies]
     E = len(wl.energies)
+    print("bench: %s, %d energies, steps %d warmup %d" % (args.workload, E, args.steps, args.warmup),
+          file=sys.stderr, flush=True)
 
     timed = {}
     for prec in ("fp64", "fp32", "mixed"):
@@ -398,14 +567,16 @@ def main():
     r6 = q6.run(wl.energies, wl.eta, host=True)
     rm_ = qm.run(wl.energies, wl.eta, host=True)
     r3 = q3.run(wl.energies, wl.eta, host=True)
-    errs = [rel_errors(rm_[e], r6[e]) for e in range(E)]
-    errs3 = [rel_errors(r3[e], r6[e]) for e in range(E)]
+    errs = per_energy_errors(rm_, r6, torch)
+    errs3 = per_energy_errors(r3, r6, torch)
     acc = {"mixed_vs_fp64_gpu_max_rel": max(e[0] for e in errs),
            "mixed_vs_fp64_gpu_rms_rel": max(e[1] for e in errs),
            "fp32_vs_fp64_gpu_max_rel": max(e[0] for e in errs3),
-           "floor": "max|d| / max(|ref|, 1e-3 max|ref|) per energy, max over energies"}
+           "mixed_vs_fp64_gpu_per_energy": errs,
+           "floor": "per energy: max|d| / max(|ref|, 1e-3 max|ref|) and ||d|| / ||ref||; maxima over energies"}
 
     default = args.workload == "square_3orb" and not args.energies
+    ws = args.workload if args.workload in TRAFFIC_MD else "square_3orb"
     out = {
         "metric": METRIC,
         "value": wl.points / dt_m,
@@ -417,42 +588,52 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed: fp16x2 split MFMA contraction (f32 accumulate), f64 elsewhere",
+        "dtype": "mixed: split-fp16 (hi+lo) MFMA contraction with f32 accumulation; f32 resolvent and "
+                 "transform lines; f64 eigensolve, T-matrix, periodic term and G0 window",
         "data": "synthetic (seeded model, Gaussian x angular-factor tiles)",
         "config": {"workload": "square lattice 3 orbitals, K=512, 51 energies in [-0.1,0.1], "
                                "eta=0.002, 128x128 cells, os=4, w=2 (512x512 pixels)"
                    if default else "%s: %s, %d energies" % (args.workload, wl.name, E),
                    "kpoints": wl.K, "energies": E, "cells": wl.n, "oversamp": wl.oversamp,
                    "window": wl.window, "bands": wl.bands},
-        "fp64": {"value": wl.W * wl.W * E / dt_6, "ms_per_step": dt_6 * 1e3,
+        "fp64": {"value": wl.points / dt_6, "ms_per_step": dt_6 * 1e3,
                  "stages_ms": {k: v[0] for k, v in st_6.items()},
-                 "roofline": roofline(wl, q6, st_6, "fp64")},
-        "fp32": {"value": wl.W * wl.W * E / dt_3, "ms_per_step": dt_3 * 1e3,
+                 "roofline": transform_roofline(wl, q6, st_6, "fp64", ws),
+                 "contraction_roofline": contraction_roofline(wl, st_6, "fp64", ws, "k_contract_st")},
+        "fp32": {"value": wl.points / dt_3, "ms_per_step": dt_3 * 1e3,
                  "stages_ms": {k: v[0] for k, v in st_3.items()}},
         "speedup_mixed_vs_fp64": dt_6 / dt_m,
         "speedup_fp32_vs_fp64": dt_6 / dt_3,
         "stages_ms": {k: v[0] for k, v in st_m.items()},
         "accuracy": acc,
-        "roofline": roofline(wl, qm, st_m, "mixed"),
-        "contraction_roofline": roofline(wl, qm, st_m, "mixed", 4),
+        "roofline": transform_roofline(wl, qm, st_m, "mixed", ws),
+        "contraction_roofline": contraction_roofline(wl, st_m, "mixed", ws, "k_cmix_areg"),
+        "end_to_end": end_to_end(qm, wl, max(3, args.steps // 4), torch),
     }
     for q, _, _ in timed.values():
         q.close()
+    if not args.no_hex and default:
+        print("bench: 10-band hexagonal workload", file=sys.stderr, flush=True)
+        out["hex_10band"] = hex_timing(cqpi, synth, torch, 2, 1, not args.no_cpu)
     if not args.no_batch:
+        print("bench: batch of 128 instances", file=sys.stderr, flush=True)
         out["batch"] = batch_timing(cqpi, synth, torch)
     if not args.no_grad:
+        print("bench: gradient", file=sys.stderr, flush=True)
         out["gradient"] = gradient_timing(cqpi, synth, torch)
     if not args.no_variants:
+        print("bench: variants", file=sys.stderr, flush=True)
         out["variants"] = variants_timing(cqpi, synth, torch)
     if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         info, maps = cpu_baseline(wl, args.cpu_energies)
         out["cpu_baseline"] = info
-        e6 = [rel_errors(r6[e], maps[e]) for e in range(len(maps))]
-        em = [rel_errors(rm_[e], maps[e]) for e in range(len(maps))]
+        e6 = per_energy_errors(r6[:len(maps)], maps, torch)
+        em = per_energy_errors(rm_[:len(maps)], maps, torch)
         acc["fp64_gpu_vs_oracle_max_rel"] = max(e[0] for e in e6)
         acc["mixed_vs_oracle_max_rel"] = max(e[0] for e in em)
         acc["mixed_vs_oracle_rms_rel"] = max(e[1] for e in em)
-        acc["oracle_energies"] = len(maps)
+        acc["vs_oracle_per_energy"] = {"energies": list(range(len(maps))), "fp64": e6, "mixed": em}
     print(json.dumps(out), flush=True)
 
 

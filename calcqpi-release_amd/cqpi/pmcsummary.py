@@ -1,7 +1,7 @@
 """Summarise rocprofv3 --pmc runs of `bench.py --probe` into HBM traffic and
 MFMA figures per kernel (markdown with a JSON block that bench.py reads).
 
-    python -m cqpi.pmcsummary FETCH.csv WRITE.csv [MFMA.csv] > summary.md
+    python -m cqpi.pmcsummary FETCH.csv WRITE.csv [MFMA.csv] [--energies N] > summary.md
     python -m cqpi.pmcsummary --stats KERNEL_STATS.csv > summary.md
 
 The probe first copies 256 MiB with a plain device copy (the runtime's
@@ -116,6 +116,11 @@ def main(argv):
     if argv and argv[0] == "--stats":
         stats(argv[1])
         return
+    energies = None
+    if "--energies" in argv:
+        k = argv.index("--energies")
+        energies = int(argv[k + 1])
+        argv = argv[:k] + argv[k + 2:]
     fetch = read(argv[0], "FETCH_SIZE")
     write = read(argv[1], "WRITE_SIZE")
     ff = calibration(fetch, COPY_BYTES / 1024)
@@ -147,7 +152,10 @@ def main(argv):
     print("Copy calibration: FETCH_SIZE x %.3f, WRITE_SIZE x %.3f (256 MiB copy read and written)." % (ff, fw))
     print()
     print("```json")
-    print(json.dumps({"calibration": {"fetch": ff, "write": fw}, "kernels": table}, indent=1))
+    doc = {"calibration": {"fetch": ff, "write": fw}, "kernels": table}
+    if energies:
+        doc["energies"] = energies  # energies of each probe launch
+    print(json.dumps(doc, indent=1))
     print("```")
 
 

@@ -413,9 +413,23 @@ static inline void zgemm_small(size_t b, const cplx *A, const cplx *B, cplx *C)
         }
 }
 
+/* The same loop nest restricted to the cells i in [i0, i1), j in [j0, j1) of
+ * the n x n map (the other pixels stay 0): a sample of a map too large for
+ * the whole loop nest on the host. */
+void or_wannierldos_cells(size_t K, size_t n, size_t window, size_t oversamp, size_t b, size_t maxband,
+                          int spin, const double *Tin, const double *g0_in, const double *wf,
+                          const int *wfzero, size_t i0, size_t i1, size_t j0, size_t j1, double *ldos);
+
 void or_wannierldos(size_t K, size_t n, size_t window, size_t oversamp, size_t b, size_t maxband,
                     int spin, const double *Tin, const double *g0_in, const double *wf,
                     const int *wfzero, double *ldos)
+{
+    or_wannierldos_cells(K, n, window, oversamp, b, maxband, spin, Tin, g0_in, wf, wfzero, 0, n, 0, n, ldos);
+}
+
+void or_wannierldos_cells(size_t K, size_t n, size_t window, size_t oversamp, size_t b, size_t maxband,
+                          int spin, const double *Tin, const double *g0_in, const double *wf,
+                          const int *wfzero, size_t i0, size_t i1, size_t j0, size_t j1, double *ldos)
 {
     const cplx *T = (const cplx *)Tin, *g0 = (const cplx *)g0_in;
     size_t W = oversamp * n, TS = (2 * window + 1) * oversamp, bb = b * b;
@@ -428,9 +442,9 @@ void or_wannierldos(size_t K, size_t n, size_t window, size_t oversamp, size_t b
 #pragma omp parallel
     {
         cplx *buffer = (cplx *)malloc(bb * sizeof(cplx)), *cl = (cplx *)malloc(bb * sizeof(cplx));
-#pragma omp for schedule(static)
-        for (size_t i = 0; i < n; i++)
-            for (size_t j = 0; j < n; j++) {
+#pragma omp for schedule(static) collapse(2)
+        for (size_t i = i0; i < i1; i++)
+            for (size_t j = j0; j < j1; j++) {
                 long ipos = (long)i - (long)n2, jpos = (long)j - (long)n2;
                 for (int nni1 = -w; nni1 <= w; nni1++)
                     for (int nnj1 = -w; nnj1 <= w; nnj1++) {

@@ -43,6 +43,8 @@ def lib():
         L.or_tmatrix.argtypes = [sz, _dp, _dp, _dp]
         L.or_tmatrix.restype = C.c_int
         L.or_wannierldos.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip, _dp]
+        L.or_wannierldos_cells.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip, sz, sz, sz, sz,
+                                           _dp]
         L.or_wannierjosephson.argtypes = [sz, sz, sz, sz, sz, sz, C.c_int, _dp, _dp, _dp, _ip,
                                           C.c_double, C.c_double, _dp]
         L.or_mkthresholdlist.argtypes = [_dp, sz, sz, sz, C.c_double, sz, C.c_int, C.c_void_p, sz,
@@ -157,6 +159,22 @@ def wannierldos(g0R, T, tiles, n, window, oversamp, maxband=0, spin=False, wfzer
     lib().or_wannierldos(K, n, window, oversamp, b, maxband, int(spin),
                          _d(_c(T, np.complex128).view(np.float64)),
                          _d(_c(g0R, np.complex128).view(np.float64)), _d(tiles), wz, _d(out))
+    return out
+
+
+def wannierldos_cells(g0R, T, tiles, n, window, oversamp, cells, maxband=0, spin=False):
+    """or_wannierldos on the cells i in [i0, i1), j in [j0, j1) only
+    (cells = (i0, i1, j0, j1)); the other pixels of the W*W map are 0."""
+    K, _, b, _ = g0R.shape
+    W = n * oversamp
+    out = np.zeros(W * W)
+    if not maxband:
+        maxband = b
+    i0, i1, j0, j1 = cells
+    lib().or_wannierldos_cells(K, n, window, oversamp, b, maxband, int(spin),
+                               _d(_c(T, np.complex128).view(np.float64)),
+                               _d(_c(g0R, np.complex128).view(np.float64)), _d(_c(tiles, np.float64)), None,
+                               i0, i1, j0, j1, _d(out))
     return out
 
 

@@ -403,3 +403,21 @@ def test_bands_histogram_flat_band_closed_form():
         x = le + (ue - le) * nl / (L - 1) - e0
         ref[nl] = os_ * os_ * w / 2 / (x * x + w * w / 4) / np.pi
     assert np.allclose(h, ref[None, None, :], rtol=1e-12, atol=0)
+
+
+def test_cell_block_contraction_equals_full_map_on_the_block():
+    """or_wannierldos_cells computes exactly the full loop nest's pixels of
+    its cells and leaves the rest 0 (the sample the bench compares the
+    10-band workload on)."""
+    wl = synth.square_norb(3, K=16, n=8, E=1, seed=61, w=1, os_=2, eta=0.02)
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    full = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp).reshape(wl.W, wl.W)  # [y, x]
+    part = oracle.wannierldos_cells(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, (2, 5, 3, 7)).reshape(wl.W, wl.W)
+    os_ = wl.oversamp
+    xs, ys = slice(2 * os_, 5 * os_), slice(3 * os_, 7 * os_)
+    assert np.array_equal(part[ys, xs], full[ys, xs])
+    mask = np.ones_like(part, bool)
+    mask[ys, xs] = False
+    assert not np.any(part[mask])

@@ -32,11 +32,20 @@ def test_bench_reads_measured_traffic():
     import bench
     f32 = bench.measured_traffic("k_res_ky_s", "mixed")
     f64 = bench.measured_traffic("k_res_ky_s", "fp64")
-    assert f32 and f64 and f32 != f64         # the line type selects the row
-    txt = open(bench.TRAFFIC_MD).read()
+    assert f32 and f64 and f32[0] != f64[0]   # the line type selects the row
+    assert f32[1] == 51                        # the probe's energies are recorded
+    txt = open(bench.TRAFFIC_MD["square_3orb"]).read()
     rows = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])["kernels"]
     w32 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f32")]
     w64 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f64")]
     assert w32 and w64 and w32[0] < 0.6 * w64[0]   # FP32 rows: half the bytes written
-    assert bench.measured_traffic("k_cmix_areg", "mixed") > 0
+    assert bench.measured_traffic("k_cmix_areg", "mixed")[0] > 0
     assert bench.measured_traffic("no_such_kernel", "mixed") is None
+
+
+def test_fused_transform_bytes_follow_the_window_of_the_reference():
+    import bench
+    from cqpi import synth
+    wl = synth.square_3orb()
+    nk = 128 + 2 * 2 + 2
+    assert bench.fused_transform_bytes(wl, 51) == 512 * 512 * (16 * 9 + 8 * 3) + 51 * 16 * 9 * nk * nk
