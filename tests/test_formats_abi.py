@@ -149,3 +149,19 @@ def test_threshold_list_builder_rejects_short_buffer():
         wl.tiles.ctypes.data_as(C.POINTER(C.c_double)), 3, 1, 2, 0.0, 0, 0, flags.ctypes.data, 3,
         counts.ctypes.data_as(C.POINTER(C.c_size_t)), C.byref(total))
     assert rc == 1 and total.value > 3
+
+
+def test_plugin_shim_compiles_against_the_interface():
+    """The GPUQPI shim header (the file a calcQPI maintainer includes) compiles
+    with the interface's stand-in headers and -Wall -Wextra -Werror."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no g++")
+    shim = os.path.join(ROOT, "tests", "shim")
+    r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter",
+                        "-I", shim, "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.join(ROOT, "calcqpi-release_amd", "host"), os.path.join(shim, "shim_check.cc")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
