@@ -383,9 +383,18 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, per * (eo + E) * g.os * g.os, c->gpumem));
     double *bulk = c->d_bulk + per * eo * g.os * g.os;
     if (c->list_mode) {
-        // the periodic part and the map from the threshold list (FP64)
+        // the periodic part (FP64) and the map from the threshold list, in
+        // FP64 or on windows and T narrowed to FP32
+        float2 *G0f = nullptr, *Tf = nullptr;
+        if (c->precision == CQPI_PREC_FP32) {
+            RETCHK(dev_grow(c->d_G0f, c->c_G0f, (eo + E) * g.nw * g.nw * NP, c->gpumem));
+            RETCHK(dev_grow(c->d_Tf, c->c_Tf, (eo + E) * NP, c->gpumem));
+            G0f = c->d_G0f + eo * g.nw * g.nw * NP;
+            Tf = c->d_Tf + eo * NP;
+        }
         int st = stage_begin(c, 4);
-        int r = launch_contract_list(g, G0w, T, c->d_lcoef, c->d_lr1off, c->d_lpair, c->lnact, E, bulk, ldos, s);
+        int r = launch_contract_list(g, G0w, T, c->d_lcoef, c->d_lr1off, c->d_lpair, c->lnact, E, bulk, ldos, s,
+                                     G0f, Tf);
         stage_end(c, st);
         if (r) return fail(r, "threshold-list contraction launch failed");
         return 0;
@@ -994,7 +1003,7 @@ int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, s
     g_err.clear();
     if (!out || !counts) return fail(CQPI_E_ARG, "NULL argument");
     *out = nullptr;
-    if (bands > 5) return fail(CQPI_E_ARG, "the threshold-list contraction supports up to 5 bands");
+    if (bands > 16) return fail(CQPI_E_ARG, "the threshold-list contraction supports up to 16 bands");
     const size_t side = 2 * window + 1, NR = side * side, npair = NR * NR;
     const size_t NP = bands * bands, npix = oversamp * oversamp;
     size_t total = 0;
@@ -1098,8 +1107,8 @@ int cqpi_set_precision(cqpi_ctx *c, int precision)
         return fail(CQPI_E_ARG, "precision mode not available");
     if (precision == CQPI_PREC_MIXED && c->josephson)
         return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
-    if (precision != CQPI_PREC_FP64 && c->list_mode)
-        return fail(CQPI_E_ARG, "the threshold-list contraction is computed in FP64 only");
+    if (precision == CQPI_PREC_MIXED && c->list_mode)
+        return fail(CQPI_E_ARG, "the threshold-list contraction is computed in FP64 or FP32 (it is no matrix product)");
     if (precision == CQPI_PREC_FP32 && !c->d_tilesf) {
         std::vector<float> tf(c->h_tiles.begin(), c->h_tiles.end());
         RETCHK(dev_grow(c->d_tilesf, c->c_tilesf, tf.size(), c->gpumem));

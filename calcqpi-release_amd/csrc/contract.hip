@@ -444,6 +444,102 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const dou
     }
 }
 
+// The list map for any band count up to kListMaxB, in FP64 or (C = float2,
+// windows and T narrowed) FP32: the same sum with one column o1 of
+// Y = T G0(-pos-R1) in registers at a time, so that its size does not grow
+// with b^2; the row G0(pos+R2)_{o2,.} is then read once per o1 (from cache).
+constexpr int kListMaxB = 16;
+
+template <typename C>
+__global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C *__restrict__ G0w,
+                                                               const C *__restrict__ Tm,
+                                                               const double *__restrict__ bulk,
+                                                               const double *__restrict__ coef,
+                                                               const int *__restrict__ r1off,
+                                                               const int *__restrict__ lpair, double *__restrict__ ldos)
+{
+    using Re = decltype(C{}.x);
+    __shared__ double red[kWaves - 1][kListPx][64];
+    const int B = g.B, NP = B * B;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cx = blockIdx.x * kCells + (lane & (kCells - 1));
+    const int cy = blockIdx.y * kCells + (lane >> 3);
+    const int e = blockIdx.z;
+    const bool cell_ok = cx < g.n && cy < g.n;
+    const int os = g.os, w = g.w, nw = g.nw, h = g.h, W = g.W;
+    const int side = 2 * w + 1, NR = side * side, npix = os * os;
+    const int n2 = g.n >> 1;
+    // cells past the map edge compute with cell 0 (in the window) and store nothing
+    const int px = (cell_ok ? cx : 0) - n2, py = (cell_ok ? cy : 0) - n2;
+    const C *G0 = G0w + (size_t)e * nw * nw * NP;
+    const C *Te = Tm + (size_t)e * NP;
+    double *out = ldos + (size_t)e * W * W;
+
+    for (int pc = 0; pc < npix; pc += kListPx) {
+        double val[kListPx];
+#pragma unroll
+        for (int q = 0; q < kListPx; q++) val[q] = 0.0;
+        for (int r1 = wave; r1 < NR; r1 += kWaves) {
+            const int rx1 = r1 / side - w, ry1 = r1 % side - w;
+            const C *GB = G0 + ((size_t)(-px - rx1 + h) * nw + (-py - ry1 + h)) * NP;
+            const int k0 = r1off[r1], k1 = r1off[r1 + 1];
+            if (k0 == k1) continue;
+            for (int o1 = 0; o1 < B; o1++) {
+                C Y[kListMaxB];  // column o1 of T G0(-pos-R1)
+#pragma unroll
+                for (int q = 0; q < kListMaxB; q++) {
+                    Re yx = 0, yy = 0;
+                    if (q < B)
+                        for (int p = 0; p < B; p++) {
+                            const C t = Te[q * B + p], gb = GB[p * B + o1];
+                            yx += t.x * gb.x - t.y * gb.y;
+                            yy += t.x * gb.y + t.y * gb.x;
+                        }
+                    Y[q].x = yx;
+                    Y[q].y = yy;
+                }
+                for (int k = k0; k < k1; k++) {
+                    const int r2 = lpair[k] - r1 * NR;
+                    const int rx2 = r2 / side - w, ry2 = r2 % side - w;
+                    const C *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
+                    const double *cf = coef + (size_t)k * NP * npix + pc;
+                    for (int o2 = 0; o2 < B; o2++) {
+                        Re cim = 0;  // Im (G0(pos+R2) Y)_{o2,o1}
+#pragma unroll
+                        for (int q = 0; q < kListMaxB; q++)
+                            if (q < B) {
+                                const C a = GA[o2 * B + q];
+                                cim += a.x * Y[q].y + a.y * Y[q].x;
+                            }
+                        const double *cp = cf + (size_t)(o2 * B + o1) * npix;
+#pragma unroll
+                        for (int q = 0; q < kListPx; q++)
+                            if (pc + q < npix) val[q] += cp[q] * (double)cim;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the previous pass's partial sums are read
+        if (wave > 0)
+#pragma unroll
+            for (int q = 0; q < kListPx; q++) red[wave - 1][q][lane] = val[q];
+        __syncthreads();
+        if (wave == 0 && cell_ok) {
+#pragma unroll
+            for (int q = 0; q < kListPx; q++) {
+                const int pix = pc + q;
+                if (pix >= npix) break;
+                double v = val[q];
+#pragma unroll
+                for (int v2 = 0; v2 < kWaves - 1; v2++) v += red[v2][q][lane];
+                const int ii = pix / os, jj = pix - ii * os;
+                out[(size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W] = -(v + bulk[(size_t)e * npix + pix]);
+            }
+        }
+    }
+}
+
 // periodic part of the list map: bulk[e][pix] = sum_(pairs, o1, o2) F Im G0(R2-R1)_{o2,o1}
 constexpr int kListBulkThreads = 256;
 
@@ -493,12 +589,27 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
 
 int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
                          const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
-                         hipStream_t s)
+                         hipStream_t s, float2 *G0f, float2 *Tf)
 {
+    if (g.B > kListMaxB) return CQPI_E_ARG;
     hipLaunchKernelGGL(k_bulk_list, dim3(g.os * g.os, E), dim3(kListBulkThreads), 0, s, g, G0w, coef, lpair, nact,
                        bulk);
     if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
     const int nb = (g.n + kCells - 1) / kCells;
+    if (G0f) {
+        // FP32: windows and T narrowed, the periodic part stays FP64
+        const size_t na = (size_t)E * g.nw * g.nw * g.B * g.B, nt = (size_t)E * g.B * g.B;
+        hipLaunchKernelGGL(k_narrow, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, G0w, na, T, nt, G0f, Tf);
+        if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+        hipLaunchKernelGGL(k_contract_list_g<float2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g,
+                           (const float2 *)G0f, (const float2 *)Tf, bulk, coef, r1off, lpair, ldos);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
+    if (g.B > 5) {
+        hipLaunchKernelGGL(k_contract_list_g<double2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
+                           coef, r1off, lpair, ldos);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     switch (g.B) {
 #define CASE(b)                                                                                                    \
     case b:                                                                                                        \

@@ -211,12 +211,14 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
 // complex periodic table of launch_bulk_josephson (needs 2 maxband <= b)
 int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, const double *bulk,
                          const double *tiles, int E, double *ldos, hipStream_t s, const double2 *tips = nullptr);
-// threshold-list contraction (b <= 5): coef[k][o2][o1][pix] per active window
+// threshold-list contraction: coef[k][o2][o1][pix] per active window
 // pair k = lpair[k] = r1*(2w+1)^2 + r2, pairs of one r1 contiguous from
 // r1off[r1]; writes the periodic part to bulk (E x os^2) and the maps to ldos
+// (up to 16 bands; G0f, Tf != NULL: FP32, with E*nw*nw*b*b and E*b*b
+// elements of scratch for the narrowed windows and T)
 int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
                          const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
-                         hipStream_t s);
+                         hipStream_t s, float2 *G0f = nullptr, float2 *Tf = nullptr);
 // spectral-function maps n x n per energy (mode 1: SPF, 2: USPF; pos: mb x 3 =
 // x, y, prefactor): from G0(k) blocks g0k[K*K][b*b] (one energy) or, with
 // g0k == NULL, from the eigen data for the E energies omegas (device)

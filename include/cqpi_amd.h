@@ -156,7 +156,7 @@ int cqpi_wannierldos_device(cqpi_ctx *ctx, const double *d_T, const double *d_g0
  * lengths of the window pairs (xn1, yn1, xn2, yn2), R = xn - w, in the order
  * ((xn1*(2w+1) + yn1)*(2w+1) + xn2)*(2w+1) + yn2 (IDX4CC); flags holds the
  * entries of all lists one after the other, as the reference flattens them
- * (src/qpihipkernel.cc:452-480).  Needs bands <= 5; FP64 only.  The context
+ * (src/qpihipkernel.cc:452-480).  Up to 16 bands; FP64 or FP32.  The context
  * serves cqpi_wannierldoslist and cqpi_run (maps from the list). */
 int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, size_t oversamp, size_t bands,
                      const cqpi_flagentry *flags, const size_t *counts, cqpi_ctx **out);

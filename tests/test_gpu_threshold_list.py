@@ -21,9 +21,10 @@ def _g0(wl, omega):
     return gR, oracle.tmatrix(wl.V, gR[0, 0])
 
 
-def _dropin(wl, flags, counts, gR, T):
+def _dropin(wl, flags, counts, gR, T, prec="fp64"):
     q = cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, flags, counts)
     try:
+        q.set_precision(prec)
         q.wannierldoslist(T, gR)
         gR[:] = 0  # the caller may overwrite g0 once the call returned
         return q.retrieveResult()
@@ -100,7 +101,7 @@ def test_list_context_rejects_other_uses(gpu):
         with pytest.raises(cqpi.CqpiError):
             q.wannierldos(T, gR)
         with pytest.raises(cqpi.CqpiError):
-            q.set_precision("fp32")
+            q.set_precision("mixed")   # no matrix product to put on MFMA
     finally:
         q.close()
     bad = flags.copy()
@@ -108,7 +109,7 @@ def test_list_context_rejects_other_uses(gpu):
     with pytest.raises(cqpi.CqpiError):
         cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, 1, 2, 3, bad, counts)
     with pytest.raises(cqpi.CqpiError):
-        cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, 1, 2, 6, flags, counts)
+        cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, 1, 2, 17, flags, counts)
     # a plain context has no list
     q = cqpi.AmdQPI(wl.W, wl.K, wl.n, 1, 2, 3, 3, False, wl.tiles)
     try:
@@ -116,3 +117,58 @@ def test_list_context_rejects_other_uses(gpu):
             q.wannierldoslist(T, gR)
     finally:
         q.close()
+
+
+def _hex_small():
+    wl = synth.hex_5d_spin(K=16, n=6, E=1)
+    wl.window, wl.oversamp = 1, 2
+    wl.tiles = np.concatenate([synth.gaussian_tiles(["xz", "yz", "xy", "x2y2", "z2"], 0.4, 0.5, 1, 2, "hex")] * 2)
+    return wl
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["6orb", "hex10-spin", "3orb"])
+@pytest.mark.parametrize("threshold", [0.05, -1.0])
+def test_dropin_list_many_bands_and_fp32(gpu, case, prec, threshold):
+    """Band counts beyond the register kernel (6; 10 with spin halves, the
+    hexagonal 5 d x spin model) and the FP32 list map (windows and T narrowed)
+    against the oracle: FP64 1e-10, FP32 1e-4."""
+    if case == "6orb":
+        wl = synth.square_norb(6, K=32, n=8, E=1, seed=90, w=1, os_=2, eta=0.02)
+        spin = False
+    elif case == "3orb":
+        wl = synth.square_norb(3, K=32, n=10, E=1, seed=83, w=2, os_=4, eta=0.01)
+        spin = False
+    else:
+        wl = _hex_small()
+        spin = True
+    gR, T = _g0(wl, 0.03)
+    flags, counts = cqpi.make_threshold_list(wl.tiles, wl.window, wl.oversamp, threshold, spin=spin)
+    print("case: threshold list %s %s threshold=%g entries=%d" % (case, prec, threshold, len(flags)), flush=True)
+    ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, wl.window, wl.oversamp)
+    got = _dropin(wl, flags, counts, gR.copy(), T, prec)
+    tol = TOL64 if prec == "fp64" else 1e-4
+    assert np.abs(ref).max() > 0
+    assert rm.rel_errors(got, ref)[0] < tol
+    if threshold < 0:
+        dense = oracle.wannierldos(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, spin=spin)
+        assert rm.rel_errors(got, dense)[0] < tol
+
+
+def test_full_chain_list_many_bands_fp32(gpu):
+    """cqpi_run on a 6-band list context in FP32 (FP32 transform lines, FP32
+    list map) against the oracle chain at 1e-4."""
+    wl = synth.square_norb(6, K=64, n=8, E=2, seed=91, w=1, os_=2, eta=0.02, elo=-0.1, ehi=0.1)
+    flags, counts = cqpi.make_threshold_list(wl.tiles, 1, 2, 0.04)
+    q = cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, flags, counts)
+    try:
+        q.set_model(wl.hops, wl.efermi, wl.scmodel)
+        q.set_impurity(wl.V)
+        q.set_precision("fp32")
+        got = q.run(wl.energies, wl.eta)
+    finally:
+        q.close()
+    for e, om in enumerate(wl.energies):
+        gR, T = _g0(wl, om)
+        ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, 1, 2)
+        assert rm.rel_errors(got[e], ref)[0] < 1e-4, e
