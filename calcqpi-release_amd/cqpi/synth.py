@@ -368,9 +368,18 @@ def square_norb(b, K, n, E, seed, elo=-0.3, ehi=0.3, w=2, os_=4, eta=0.002, V=No
         name="square %d-orbital seed %d" % (b, seed), bands=b,
         hops=hops_from_blocks(blocks, b), K=K, n=n, window=w, oversamp=os_,
         energies=np.linspace(elo, ehi, E), eta=eta, V=np.asarray(V, np.complex128),
-        tiles=gaussian_tiles(ORB_SETS[b], 0.4, 0.6, w, os_),
+        tiles=gaussian_tiles(orb_set(b), 0.4, 0.6, w, os_),
         meta={"seed": seed},
     )
+
+
+def orb_set(b):
+    """Angular factors of b tiles: the fixed sets for 1, 3, 5, 6, 7 orbitals,
+    otherwise s, xz, yz, xy, x2y2, z2 repeated."""
+    if b in ORB_SETS:
+        return ORB_SETS[b]
+    base = ["s", "xz", "yz", "xy", "x2y2", "z2"]
+    return [base[i % len(base)] for i in range(b)]
 
 
 def bdg_square(nb, K, n, E, seed, delta=0.08, w=1, os_=2, eta=0.01, efermi=0.05, elo=-0.06, ehi=0.06):

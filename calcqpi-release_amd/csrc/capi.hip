@@ -1409,7 +1409,8 @@ int cqpi_set_surface(cqpi_ctx *c, int mode, double epserr, const cqpi_hopping *s
     }
     if (!(epserr > 0.0)) return fail(CQPI_E_ARG, "epserr must be positive");
     if (nshops && !shops) return fail(CQPI_E_ARG, "shops is NULL");
-    if (c->g.B > 8) return fail(CQPI_E_ARG, "the surface Green's function supports up to 8 bands");
+    if (c->g.B > 12 && c->g.B != 14 && c->g.B != 16)
+        return fail(CQPI_E_ARG, "the surface Green's function supports 1-12, 14 and 16 bands");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     c->h_shops.assign(shops, shops + nshops);
     c->surf_mode = mode;

@@ -12,8 +12,9 @@
 //   alpha <- alpha g alpha; beta <- beta g beta;
 // until both Frobenius norms are <= epserr (src/qpi.cc:541); then
 // G(k) = (w + i eta - eps_s)^-1 (surface) or with eps_b (bulk).  One thread per
-// (k, energy) with the b x b blocks in registers (b <= 5; 6 to 8 spill to
-// scratch); the result goes to the G(k)
+// (k, energy) with the b x b blocks in registers for b <= 4, one 16-lane
+// group per (k, energy) with a row per lane for 5 <= b <= 16 (no scratch up
+// to 14 bands); the result goes to the G(k)
 // workspace layout Gk[e][pair][kx][ky] that the first transform pass reads.
 // A run stops a k-point after maxit steps and reports it in stat.
 #include "cqpi_common.h"
@@ -221,6 +222,241 @@ __global__ void __launch_bounds__(64) k_surface_gk(int K, SurfSets sets, const d
     if (threadIdx.x == 0) atomicMax(stat + e, wmax);
 }
 
+// ---------------------------------------------------------------------------
+// 5 <= B <= 16 bands: one 16-lane group per (k, energy), lane r holding row r
+// of eps_s, eps_b, alpha, beta and g in registers (no scratch).  A product
+// X Y stages Y's rows in the group's LDS and each lane forms its row of X Y
+// from them; the inverse is Gauss-Jordan with partial pivoting, the pivot
+// found by a shuffle reduction over the group, rows exchanged through LDS
+// and the pivot row broadcast through LDS.  Lanes r >= B shadow row B-1
+// and store nothing.  Four items a wave; a wave loops until all four have
+// converged (a converged group skips the work).
+// ---------------------------------------------------------------------------
+constexpr int kGroup = 16;
+
+__device__ inline void group_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// row rr of the in-plane block (diag: with the -/+ efermi diagonal) or coupling
+template <int B>
+__device__ inline void build_h_row(const DevRGroup *__restrict__ groups, int ngroups, const double2 *__restrict__ tmat,
+                                   const double2 *__restrict__ tw, int i, int j, int K, bool diag, double efermi,
+                                   int scmax, int rr, double2 *H)
+{
+#pragma unroll
+    for (int c = 0; c < B; c++) H[c] = make_double2(0.0, 0.0);
+    if (diag)
+#pragma unroll
+        for (int c = 0; c < B; c++)
+            if (c == rr) H[c].x = rr < scmax ? -efermi : efermi;
+    const double kx = (i > (K >> 1)) ? ((double)i - (double)K) / (double)K : (double)i / (double)K;
+    const double ky = (j > (K >> 1)) ? ((double)j - (double)K) / (double)K : (double)j / (double)K;
+    const unsigned kmask = (K & (K - 1)) == 0 ? (unsigned)K - 1u : 0u;
+    for (int gi = 0; gi < ngroups; gi++) {
+        const DevRGroup G = groups[gi];
+        double2 e;
+        if (G.integral) {
+            const unsigned m = (unsigned)G.irx * (unsigned)i + (unsigned)G.iry * (unsigned)j;
+            e = tw[kmask ? (m & kmask) : (m % (unsigned)K)];
+        } else {
+            double sn, cs;
+            sincos(2.0 * M_PI * (G.rx * kx + G.ry * ky), &sn, &cs);
+            e = make_double2(cs, sn);
+        }
+        const double2 *T = tmat + ((size_t)gi * B + rr) * B;
+#pragma unroll
+        for (int c = 0; c < B; c++) H[c] = cadd(H[c], cmul(T[c], e));
+    }
+}
+
+// P = X Y, X the lane's row, Y staged row-major in S (B x B).  The k loop
+// stays rolled (X[k] picked by compares): unrolled, the loads of all B staged
+// rows were hoisted ahead and held B^2 values in registers (spills for b > 5)
+template <int B>
+__device__ inline void row_times(const double2 *X, const double2 *S, double2 *P)
+{
+#pragma unroll
+    for (int c = 0; c < B; c++) P[c] = make_double2(0.0, 0.0);
+#pragma unroll 1
+    for (int k = 0; k < B; k++) {
+        double2 x = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q == k) x = X[q];
+#pragma unroll
+        for (int c = 0; c < B; c++) P[c] = cadd(P[c], cmul(x, S[k * B + c]));
+    }
+}
+
+template <int B>
+__device__ inline void stage_row(double2 *S, int r, const double2 *row)
+{
+    if (r < B)
+#pragma unroll
+        for (int c = 0; c < B; c++) S[r * B + c] = row[c];
+}
+
+// G = (w + i eta - H)^-1 by rows; S1, S2 (B x B) and PV (2B) are the group's LDS
+template <int B>
+__device__ inline void row_resinv(const double2 *H, double om, double eta, int r, int rr, double2 *G, double2 *S1,
+                                  double2 *S2, double2 *PV)
+{
+    double2 M[B];
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+        M[c] = make_double2((c == rr ? om : 0.0) - H[c].x, (c == rr ? eta : 0.0) - H[c].y);
+        G[c] = make_double2(c == rr ? 1.0 : 0.0, 0.0);
+    }
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+        // pivot: the largest |M_ij| over the rows i >= j
+        double v = (r < B && rr >= j) ? cabs2(M[j]) : -1.0;
+        int piv = rr;
+#pragma unroll
+        for (int off = kGroup / 2; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off, kGroup);
+            const int op = __shfl_xor(piv, off, kGroup);
+            if (ov > v || (ov == v && op < piv)) {
+                v = ov;
+                piv = op;
+            }
+        }
+        if (piv != j) {
+            // exchange rows j and piv through LDS
+            stage_row<B>(S1, r, M);
+            stage_row<B>(S2, r, G);
+            group_sync();
+            const int src = rr == j ? piv : (rr == piv ? j : rr);
+            if (rr == j || rr == piv)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    M[c] = S1[src * B + c];
+                    G[c] = S2[src * B + c];
+                }
+            group_sync();
+        }
+        if (rr == j && r < B) {
+            const double2 ip = cinv(M[j]);
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                M[c] = cmul(M[c], ip);
+                G[c] = cmul(G[c], ip);
+                PV[c] = M[c];
+                PV[B + c] = G[c];
+            }
+        }
+        group_sync();
+        if (rr != j) {
+            const double2 l = M[j];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                M[c] = csub(M[c], cmul(l, PV[c]));
+                G[c] = csub(G[c], cmul(l, PV[B + c]));
+            }
+        }
+        group_sync();
+    }
+}
+
+template <typename C, int B>
+__global__ void __launch_bounds__(64) k_surface_gk_g(int K, SurfSets sets, const double2 *__restrict__ tw,
+                                                     double efermi, int scmax, const double *__restrict__ omegas,
+                                                     double eta, double epserr, int maxit, int surface,
+                                                     C *__restrict__ Gk, int *__restrict__ stat)
+{
+    constexpr int NP = B * B;
+    __shared__ double2 sm[64 / kGroup][2 * NP + 2 * B];
+    const int lane = threadIdx.x & 63, grp = lane / kGroup, r = lane % kGroup;
+    const int rr = r < B ? r : B - 1;
+    const long KK = (long)K * K;
+    const long k0 = (long)blockIdx.x * (64 / kGroup) + grp;
+    const int e = blockIdx.y;
+    const bool active = k0 < KK;
+    const long k = active ? k0 : KK - 1;
+    const int i = (int)(k / K), j = (int)(k - (long)i * K);
+    double2 *S1 = sm[grp], *S2 = S1 + NP, *PV = S2 + NP;
+    double2 es[B], eb[B], al[B], be[B], g[B], t1[B];
+    build_h_row<B>(sets.g[1], sets.n[1], sets.t[1], tw, i, j, K, true, efermi, scmax, rr, es);
+    build_h_row<B>(sets.g[0], sets.n[0], sets.t[0], tw, i, j, K, true, efermi, scmax, rr, eb);
+    build_h_row<B>(sets.g[2], sets.n[2], sets.t[2], tw, i, j, K, false, efermi, scmax, rr, al);
+    // beta = alpha^H: row rr of beta is the conjugated column rr of alpha
+    stage_row<B>(S1, r, al);
+    group_sync();
+#pragma unroll
+    for (int c = 0; c < B; c++) be[c] = cconj(S1[c * B + rr]);
+    group_sync();
+    const double om = omegas[e];
+    const double eps2 = epserr * epserr;
+    int it = 0;
+    bool done = false;
+    for (;;) {
+        if (!done) {
+            row_resinv<B>(eb, om, eta, r, rr, g, S1, S2, PV);  // g = (w + i eta - eps_b)^-1
+            stage_row<B>(S1, r, be);
+            group_sync();
+            row_times<B>(g, S1, t1);  // t1 = g beta
+            group_sync();
+            stage_row<B>(S2, r, t1);
+            group_sync();
+            double2 a[B];
+            row_times<B>(al, S2, a);  // alpha g beta
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                es[c] = cadd(es[c], a[c]);
+                eb[c] = cadd(eb[c], a[c]);
+            }
+            stage_row<B>(S1, r, al);
+            group_sync();
+            row_times<B>(g, S1, a);  // g alpha
+            group_sync();
+#pragma unroll
+            for (int c = 0; c < B; c++) g[c] = a[c];
+            stage_row<B>(S1, r, g);
+            group_sync();
+            row_times<B>(be, S1, a);  // beta g alpha
+#pragma unroll
+            for (int c = 0; c < B; c++) eb[c] = cadd(eb[c], a[c]);
+            row_times<B>(be, S2, a);  // beta <- beta g beta
+#pragma unroll
+            for (int c = 0; c < B; c++) be[c] = a[c];
+            row_times<B>(al, S1, a);  // alpha <- alpha g alpha
+#pragma unroll
+            for (int c = 0; c < B; c++) al[c] = a[c];
+            group_sync();
+            it++;
+            double na = 0.0, nb = 0.0;
+            if (r < B)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    na += cabs2(al[c]);
+                    nb += cabs2(be[c]);
+                }
+#pragma unroll
+            for (int off = kGroup / 2; off > 0; off >>= 1) {
+                na += __shfl_xor(na, off, kGroup);
+                nb += __shfl_xor(nb, off, kGroup);
+            }
+            if (!(na > eps2 || nb > eps2)) done = true;  // both Frobenius norms <= epserr
+            else if (it >= maxit) {
+                it = maxit + 1;  // not converged
+                done = true;
+            }
+        }
+        if (__all(done)) break;
+    }
+    row_resinv<B>(surface ? es : eb, om, eta, r, rr, g, S1, S2, PV);
+    if (active && r < B)
+#pragma unroll
+        for (int c = 0; c < B; c++) store_c(Gk + ((size_t)e * NP + r * B + c) * KK + k, g[c]);
+    int wmax = it;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    if (threadIdx.x == 0) atomicMax(stat + e, wmax);
+}
+
 __global__ void k_zero_int(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,6 +481,7 @@ int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *
     hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((E + 63) / 64)), dim3(64), 0, s, stat, E);
     const long KK = (long)K * K;
     const dim3 grid((unsigned)((KK + 63) / 64), (unsigned)E);
+    const dim3 gridg((unsigned)((KK + 64 / kGroup - 1) / (64 / kGroup)), (unsigned)E);  // four (k, e) a wave
     switch (B) {
 #define CASE(b)                                                                                                    \
     case b:                                                                                                        \
@@ -255,8 +492,19 @@ int launch_surface_gk(int B, int K, const DevRGroup *const g[3], const double2 *
             hipLaunchKernelGGL((k_surface_gk<double2, b>), grid, dim3(64), 0, s, K, sets, tw, efermi, scmax,        \
                                omegas, eta, epserr, maxit, surface, static_cast<double2 *>(Gk), stat);             \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
+#define CASEG(b)                                                                                                   \
+    case b:                                                                                                        \
+        if (fp32)                                                                                                  \
+            hipLaunchKernelGGL((k_surface_gk_g<float2, b>), gridg, dim3(64), 0, s, K, sets, tw, efermi, scmax,      \
+                               omegas, eta, epserr, maxit, surface, static_cast<float2 *>(Gk), stat);              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_surface_gk_g<double2, b>), gridg, dim3(64), 0, s, K, sets, tw, efermi, scmax,     \
+                               omegas, eta, epserr, maxit, surface, static_cast<double2 *>(Gk), stat);             \
+        break;
+        CASEG(5) CASEG(6) CASEG(7) CASEG(8) CASEG(9) CASEG(10) CASEG(11) CASEG(12) CASEG(14) CASEG(16)
+#undef CASEG
     default: return CQPI_E_ARG;
     }
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;

@@ -923,11 +923,11 @@ int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, 
         const double2 *t = static_cast<const double2 *>(tw);
         double2 *f = static_cast<double2 *>(F1);
         switch (g.B) {
-        case 1: return launch_res_t<1>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
-        case 2: return launch_res_t<2>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
-        case 3: return launch_res_t<3>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
-        case 4: return launch_res_t<4>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
-        case 5: return launch_res_t<5>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+#define CASE(b) \
+    case b: return launch_res_t<b>(g, nullptr, nullptr, omegas, E, 0.0, t, f, s, gk);
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+            CASE(11) CASE(12) CASE(14) CASE(16)
+#undef CASE
         default: return CQPI_E_ARG;
         }
     }

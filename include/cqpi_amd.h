@@ -313,7 +313,7 @@ enum {
  * -/+ efermi diagonal) and the coupling to the next layer (rz < 0) are
  * decimated until the Frobenius norms of both couplings are <= epserr; shops
  * (NULL/0: the model's own hoppings) gives the surface layer's in-plane block
- * (stbfile=).  Call after cqpi_set_model; bands <= 8.  It applies to every
+ * (stbfile=).  Call after cqpi_set_model; bands 1-12, 14, 16.  It applies to every
  * output of the context: LDOS and Josephson maps, threshold-list maps, the
  * SPF / USPF maps and cqpi_dos.  cqpi_run and cqpi_dos return
  * CQPI_E_CONVERGE if some k needs more steps than the limit

@@ -50,6 +50,22 @@ def test_surface_chain_matches_oracle(gpu, mode, b):
     _assert_close(_run(wl, mode, 1e-10), _oracle_maps(wl, mode == "surface", 1e-10), TOL64)
 
 
+@pytest.mark.parametrize("mode", ["surface", "bulk"])
+@pytest.mark.parametrize("b", [4, 5, 7, 8, 10])
+def test_surface_chain_many_bands(gpu, mode, b):
+    """4 bands on the per-thread kernel; 5 to 10 on the lane-group kernel
+    (a row of each block per lane, no scratch), e.g. BdG models of 4 and 5
+    orbitals."""
+    wl = synth.layered(b, K=32, n=8, E=2, seed=60 + b, w=1, os_=2, eta=0.03)
+    _assert_close(_run(wl, mode, 1e-10), _oracle_maps(wl, mode == "surface", 1e-10), TOL64)
+
+
+@pytest.mark.parametrize("b", [5, 10])
+def test_surface_many_bands_fp32_default_epserr(gpu, b):
+    wl = synth.layered(b, K=64, n=8, E=2, seed=70 + b, w=1, os_=2, eta=0.03)
+    _assert_close(_run(wl, "surface", 1e-5, "fp32"), _oracle_maps(wl, True, 1e-5), TOL_RED)
+
+
 def test_surface_layer_model(gpu):
     """stbfile=: the surface layer's own in-plane terms (here the on-site
     energies shifted) for eps_s; the bulk and coupling blocks from the model."""
@@ -100,8 +116,7 @@ def test_surface_rejections(gpu):
         q.close()
     wl10 = synth.bdg_square(5, K=16, n=4, E=1, seed=52)
     with cqpi.engine_for(wl10) as q:
-        with pytest.raises(cqpi.CqpiError):
-            q.set_surface("surface")  # more than 8 bands
+        q.set_surface("surface")  # 10 bands (a BdG model of 5 orbitals): accepted
     with cqpi.engine_for(wl) as q:
         q.set_keep_windows(True)
         q.set_surface("bulk", 1e-10)
