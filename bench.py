@@ -184,7 +184,7 @@ def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
     """The reference's other two contraction kinds at the 3-orbital geometry
     (K=512, 51 energies, 128x128 cells, os=4, w=2), whole chain on the device:
     the Josephson map of a superconducting model (3 orbitals x particle-hole,
-    6 bands; FP64 and FP32) and the threshold-list map (FP64) of the 3-orbital
+    6 bands; FP64, FP32 and mixed) and the threshold-list map (FP64) of the 3-orbital
     workload with the driver's list rule at the given threshold; and the LDOS
     map from the surface Green's function of a layered 3-orbital model."""
     out = {}
@@ -193,7 +193,7 @@ def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
                        "51 energies in [-0.1,0.1], eta=0.002, 128x128 cells, os=4, w=2; tip deltat=0.001, "
                        "etat=eta"}
     maps = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "mixed"):
         q = cqpi.engine_for(wlj, prec)
         q.set_josephson(True, deltat=0.001)
         dt, st = time_steps(q, wlj, steps, warmup, torch)
@@ -201,8 +201,9 @@ def variants_timing(cqpi, synth, torch, steps=5, warmup=3, threshold=0.01):
                      "stages_ms": {k: v[0] for k, v in st.items()}}
         maps[prec] = q.run(wlj.energies, wlj.eta, host=True)
         q.close()
-    jos["fp32_vs_fp64_gpu_max_rel"] = max(rel_errors(maps["fp32"][e], maps["fp64"][e])[0]
-                                          for e in range(len(wlj.energies)))
+    for prec in ("fp32", "mixed"):
+        jos[prec + "_vs_fp64_gpu_max_rel"] = max(rel_errors(maps[prec][e], maps["fp64"][e])[0]
+                                                 for e in range(len(wlj.energies)))
     out["josephson"] = jos
     wl = synth.workload("square_3orb")
     flags, counts = cqpi.make_threshold_list(wl.tiles, wl.window, wl.oversamp, threshold)

@@ -376,8 +376,6 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
     const Geom &g = c->g;
     if (!s) s = c->stream;
     const int NP = g.B * g.B;
-    if (tips && c->precision == CQPI_PREC_MIXED)
-        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
     if (tips && 2 * g.maxb > g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
     const size_t per = tips ? 2 : 1;  // doubles per periodic-table entry
     RETCHK(dev_grow(c->d_bulk, c->c_bulk, per * (eo + E) * g.os * g.os, c->gpumem));
@@ -417,7 +415,7 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
         }
         if (!r)
             r = launch_contract_mixed(g, c->mg, c->d_planes + eo * mix_plane_halves(g, c->mg), c->d_afrag,
-                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s);
+                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s, tips);
         break;
     }
     case CQPI_PREC_FP32:
@@ -1105,8 +1103,6 @@ int cqpi_set_precision(cqpi_ctx *c, int precision)
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
     if (precision != CQPI_PREC_FP64 && precision != CQPI_PREC_FP32 && precision != CQPI_PREC_MIXED)
         return fail(CQPI_E_ARG, "precision mode not available");
-    if (precision == CQPI_PREC_MIXED && c->josephson)
-        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
     if (precision == CQPI_PREC_MIXED && c->list_mode)
         return fail(CQPI_E_ARG, "the threshold-list contraction is computed in FP64 or FP32 (it is no matrix product)");
     if (precision == CQPI_PREC_FP32 && !c->d_tilesf) {
@@ -1534,8 +1530,6 @@ int cqpi_set_josephson(cqpi_ctx *c, int on, double deltat, double etat)
     if (on && (c->list_mode || c->spectral))
         return fail(CQPI_E_ARG, "the Josephson map needs a context made by cqpi_create");
     if (on && 2 * c->g.maxb > c->g.B) return fail(CQPI_E_ARG, "the Josephson map needs 2*maxband <= bands");
-    if (on && c->precision == CQPI_PREC_MIXED)
-        return fail(CQPI_E_ARG, "the Josephson map is computed in FP64 or FP32 only");
     c->josephson = on != 0;
     c->jos_deltat = deltat;
     c->jos_etat = std::isnan(etat) ? -1.0 : etat;

@@ -41,11 +41,16 @@ struct Cplx<float> {
 // Real = double: the FP64 contraction.  Real = float: the same loop nest in
 // FP32 (G0 window, T and tiles narrowed to FP32, FP32 accumulation; the
 // periodic table stays FP64).
-template <typename Real, int B, int P>
+//
+// J: the Josephson map (the b gather from the particle-hole column o +
+// maxband, complex periodic table and tip factor; as k_contract_st below), for
+// windows whose staged rows do not fit the LDS.
+template <typename Real, int B, int P, bool J>
 __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename Cplx<Real>::type *__restrict__ G0w,
                                                         const typename Cplx<Real>::type *__restrict__ Tm,
                                                         const double *__restrict__ bulk,
-                                                        const Real *__restrict__ tiles, double *__restrict__ ldos)
+                                                        const Real *__restrict__ tiles, double *__restrict__ ldos,
+                                                        const double2 *__restrict__ tips)
 {
     using C = typename Cplx<Real>::type;
     const int lane = threadIdx.x & 63;
@@ -77,9 +82,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
             ii[p] = pix / os;
             jj[p] = pix - ii[p] * os;
         }
-        Real val[P];
+        Real val[P], valr[P];  // Im and (J) Re of sum_s a^T T b
 #pragma unroll
-        for (int p = 0; p < P; p++) val[p] = 0;
+        for (int p = 0; p < P; p++) val[p] = valr[p] = 0;
+        const int jo = J ? g.maxb : 0;  // column offset of the b gather
         for (int sg = 0; sg < nsets; sg++) {
             const int olo = sg * nper, ohi = olo + nper;
             C acca[P][B], accb[P][B];
@@ -103,7 +109,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
 #pragma unroll
                         for (int q = 0; q < B; q++) {
                             const C ga = GA[o * B + q];
-                            const C gb = GB[q * B + o];
+                            const C gb = GB[q * B + o + jo];
 #pragma unroll
                             for (int p = 0; p < P; p++) {
                                 acca[p][q].x = fma(wt[p], ga.x, acca[p][q].x);
@@ -130,6 +136,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
                         tby += trow[q2].x * accb[p][q2].y + trow[q2].y * accb[p][q2].x;
                     }
                     val[p] += acca[p][q1].x * tby + acca[p][q1].y * tbx;
+                    if (J) valr[p] += acca[p][q1].x * tbx - acca[p][q1].y * tby;
                 }
             }
         }
@@ -137,7 +144,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract(Geom g, const typename
         for (int p = 0; p < P; p++) {
             if (!ok[p]) continue;
             int X = cx * os + ii[p], Y = cy * os + jj[p];
-            out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+            if (J) {
+                const double2 bz = reinterpret_cast<const double2 *>(bulk)[(size_t)e * npix + ii[p] * os + jj[p]];
+                const double2 tp = tips[e];
+                const double zr = (double)valr[p] + bz.x, zi = (double)val[p] + bz.y;
+                out[(size_t)X + (size_t)Y * W] = -(tp.x * zi + tp.y * zr);
+            } else {
+                out[(size_t)X + (size_t)Y * W] = -((double)val[p] + bulk[(size_t)e * npix + ii[p] * os + jj[p]]);
+            }
         }
     }
 }
@@ -302,8 +316,12 @@ int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double
     constexpr int P = B <= 1 ? 8 : (B <= 3 ? 4 : (B <= 12 ? 2 : 1));
     int nb = (g.n + kCells - 1) / kCells;
     const size_t lds = contract_st_lds(g, sizeof(double2));
+    if (tips && lds > 150 * 1024) {
+        hipLaunchKernelGGL((k_contract<double, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T,
+                           bulk, tiles, ldos, tips);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     if (tips) {
-        if (lds > 150 * 1024) return CQPI_E_ARG;  // the Josephson map has no per-lane gather variant
         hipLaunchKernelGGL((k_contract_st<double, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0w,
                            T, bulk, tiles, ldos, tips);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -313,8 +331,8 @@ int launch_c64(const Geom &g, const double2 *G0w, const double2 *T, const double
                            T, bulk, tiles, ldos, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
-    hipLaunchKernelGGL((k_contract<double, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
-                       tiles, ldos);
+    hipLaunchKernelGGL((k_contract<double, B, P, false>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
+                       tiles, ldos, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -326,8 +344,12 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
     constexpr int P = B <= 6 ? 4 : 2;
     int nb = (g.n + kCells - 1) / kCells;
     const size_t lds = contract_st_lds(g, sizeof(float2));
+    if (tips && lds > 150 * 1024) {
+        hipLaunchKernelGGL((k_contract<float, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf,
+                           bulk, tiles, ldos, tips);
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    }
     if (tips) {
-        if (lds > 150 * 1024) return CQPI_E_ARG;
         hipLaunchKernelGGL((k_contract_st<float, B, P, true>), dim3(nb, nb, E), dim3(64 * kWaves), lds, s, g, G0f, Tf,
                            bulk, tiles, ldos, tips);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -337,8 +359,8 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
                            Tf, bulk, tiles, ldos, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
-    hipLaunchKernelGGL((k_contract<float, B, P>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf, bulk,
-                       tiles, ldos);
+    hipLaunchKernelGGL((k_contract<float, B, P, false>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0f, Tf, bulk,
+                       tiles, ldos, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 

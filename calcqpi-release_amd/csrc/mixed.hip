@@ -131,13 +131,21 @@ constexpr int kJB = 4;  // sub-pixel rows (jj) per accumulator block
 // one wave = one tile: CB cells along x starting at cx0, one cell row cy,
 // all sub-pixels.  Rows of the GEMM are the pixels (c, ii), c < CB, ii < os,
 // of one sub-pixel row jj; the jj loop runs in blocks of kJB accumulators.
-template <int NT, int LDW, int JB>
+//
+// J: the Josephson map of a superconducting model (src/qpihipkernel.cc:171-271,
+// CPU src/qpi.cc:953-1033): the b gather reads the particle-hole column
+// o + maxband (the planes of G0(-R)[p][o + maxband]), both the real and the
+// imaginary part of sum_s a^T T b' are kept, and per pixel
+//   rho = -Im[ tip (sum_s a^T T b' + bulk') ]
+// with the complex periodic table bulk' (E x os^2 double2) and tips[e].
+template <int NT, int LDW, int JB, bool J>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                     const half8 *__restrict__ Afrag,
                                                     const double *__restrict__ scale, double sA,
                                                     const double2 *__restrict__ Tm,
                                                     const double *__restrict__ bulk,
-                                                    double *__restrict__ ldos)
+                                                    double *__restrict__ ldos,
+                                                    const double2 *__restrict__ tips)
 {
     __shared__ double2 Ts[16 * 16];
     constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
@@ -161,9 +169,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
     const int col = lane & 15, kq = lane >> 4;
     const int col0 = cx0 - n2 - w + h + mg.OFF;
     const int rows = mg.CB * os;
-    double val[NB];
+    double val[NB], valr[NB];  // Im and (J) Re of sum_s a^T T b
 #pragma unroll
-    for (int jb = 0; jb < NB; jb++) val[jb] = 0.0;
+    for (int jb = 0; jb < NB; jb++) val[jb] = valr[jb] = 0.0;
+    const int jo = J ? g.maxb : 0;  // orbital offset of the b planes
 
     for (int sg = 0; sg < mg.nsets; sg++) {
 #pragma unroll
@@ -201,7 +210,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                     } else if (nn < 4 * B) {
                         ab = 1;
                         int m2 = nn - 2 * B;
-                        pl = (o * B + (m2 >> 1)) * 2 + (m2 & 1);
+                        pl = ((o + jo) * B + (m2 >> 1)) * 2 + (m2 & 1);
                     }
                     if (kok && ab >= 0) {
                         bh[t] = load8<LDW>(Pe + ((size_t)(ab * 2 + 0) * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
@@ -242,14 +251,16 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                 const int i = lane >> 4, m = lane & 15;
                 if (i < JB && jj0 + i < os && m < rows) {
                     const float *Cr = Cs[wave][i][m];
-                    double acc = 0.0;
+                    double acc = 0.0, accr = 0.0;
                     for (int p = 0; p < B; p++) {
                         double2 tb = make_double2(0.0, 0.0);
                         for (int q = 0; q < B; q++)
                             tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
                         acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
+                        if (J) accr += (double)Cr[2 * p] * tb.x - (double)Cr[2 * p + 1] * tb.y;
                     }
                     val[jb] += acc;
+                    if (J) valr[jb] += accr;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -268,7 +279,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
         const int c = m / os, ii = m - c * os, cx = cx0 + c;
         if (cx >= n) continue;
         const int X = cx * os + ii, Y = cy * os + jj;
-        out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+        if (J) {
+            const double2 bz = reinterpret_cast<const double2 *>(bulk)[(size_t)e * os * os + ii * os + jj];
+            const double2 tp = tips[e];
+            const double zr = valr[jb] * unscale + bz.x, zi = val[jb] * unscale + bz.y;
+            out[(size_t)X + (size_t)Y * W] = -(tp.x * zi + tp.y * zr);
+        } else {
+            out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+        }
     }
 }
 
@@ -508,13 +526,16 @@ size_t cmix_st_lds(const MixGeom &mg, const Geom &g, int NT, int JB)
     return (planes * RP * mg.NUg * 8 + 8) * sizeof(_Float16) + (size_t)kStWaves * JB * 16 * (16 * NT + 1) * sizeof(float);
 }
 
-template <int NT, int LDW, int JB>
+// J: the Josephson map, as in k_cmix (the b planes of the spin block staged
+// from the particle-hole columns o + maxband).
+template <int NT, int LDW, int JB, bool J>
 __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                           const half8 *__restrict__ Afrag,
                                                           const double *__restrict__ scale, double sA,
                                                           const double2 *__restrict__ Tm,
                                                           const double *__restrict__ bulk,
-                                                          double *__restrict__ ldos)
+                                                          double *__restrict__ ldos,
+                                                          const double2 *__restrict__ tips)
 {
     constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
     __shared__ double2 Ts[16 * 16];
@@ -548,9 +569,10 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
     const int trow = (cy - cy_lo) * seg * 8;
     const unsigned long long mrw = udiv_magic((unsigned)RW), mseg = udiv_magic((unsigned)seg),
                              mnps = udiv_magic((unsigned)NPS);
-    double val[NB];
+    double val[NB], valr[NB];  // Im and (J) Re of sum_s a^T T b
 #pragma unroll
-    for (int jb = 0; jb < NB; jb++) val[jb] = 0.0;
+    for (int jb = 0; jb < NB; jb++) val[jb] = valr[jb] = 0.0;
+    const int jo = J ? g.maxb : 0;  // orbital offset of the b planes
 
     for (int sg = 0; sg < mg.nsets; sg++) {
         __syncthreads();  // the previous spin block's fragments are read
@@ -558,7 +580,8 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
         for (int l = threadIdx.x; l < 4 * NPS * RW * seg; l += blockDim.x) {
             const int rest = udiv(l, mrw), r = l - rest * RW;
             const int pid = udiv(rest, mseg), g8 = rest - pid * seg;
-            const int abhl = udiv(pid, mnps), pl = sg * NPS + (pid - abhl * NPS);
+            const int abhl = udiv(pid, mnps);
+            const int pl = sg * NPS + (pid - abhl * NPS) + (abhl >= 2 ? jo * B * 2 : 0);
             const int row = row_lo + r;
             half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
             if (row < nw) v = load8<LDW>(Pe + ((size_t)abhl * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
@@ -633,14 +656,16 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
                 const int i = lane >> 4, m = lane & 15;
                 if (i < JB && jj0 + i < os && m < rows) {
                     const float *Cr = Csw + ((size_t)i * 16 + m) * (16 * NT + 1);
-                    double acc = 0.0;
+                    double acc = 0.0, accr = 0.0;
                     for (int p = 0; p < B; p++) {
                         double2 tb = make_double2(0.0, 0.0);
                         for (int q = 0; q < B; q++)
                             tb = cadd(tb, cmul(Ts[p * B + q], make_double2(Cr[2 * B + 2 * q], Cr[2 * B + 2 * q + 1])));
                         acc += (double)Cr[2 * p] * tb.y + (double)Cr[2 * p + 1] * tb.x;
+                        if (J) accr += (double)Cr[2 * p] * tb.x - (double)Cr[2 * p + 1] * tb.y;
                     }
                     val[jb] += acc;
+                    if (J) valr[jb] += accr;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -659,42 +684,75 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
         const int c = m / os, ii = m - c * os, cx = cx0 + c;
         if (cx >= n) continue;
         const int X = cx * os + ii, Y = cy * os + jj;
-        out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+        if (J) {
+            const double2 bz = reinterpret_cast<const double2 *>(bulk)[(size_t)e * os * os + ii * os + jj];
+            const double2 tp = tips[e];
+            const double zr = valr[jb] * unscale + bz.x, zi = val[jb] * unscale + bz.y;
+            out[(size_t)X + (size_t)Y * W] = -(tp.x * zi + tp.y * zr);
+        } else {
+            out[(size_t)X + (size_t)Y * W] = -(val[jb] * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+        }
     }
 }
 
-template <int NT, int JB>
-int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
-              const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+template <int NT, int JB, bool J>
+void launch_st_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
+                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
 {
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kStWaves - 1) / kStWaves, E);
     dim3 blk(64 * kStWaves);
     const size_t lds = cmix_st_lds(mg, g, NT, JB);
+#define CQPI_ST(ldw) \
+    hipLaunchKernelGGL((k_cmix_st<NT, ldw, JB, J>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips)
     switch (mg.LDW) {
-    case 8: hipLaunchKernelGGL((k_cmix_st<NT, 8, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 4: hipLaunchKernelGGL((k_cmix_st<NT, 4, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 2: hipLaunchKernelGGL((k_cmix_st<NT, 2, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    default: hipLaunchKernelGGL((k_cmix_st<NT, 1, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
+    case 8: CQPI_ST(8); break;
+    case 4: CQPI_ST(4); break;
+    case 2: CQPI_ST(2); break;
+    default: CQPI_ST(1); break;
     }
+#undef CQPI_ST
+}
+
+template <int NT, int JB>
+int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
+              const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+{
+    if (tips)
+        launch_st_j<NT, JB, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s);
+    else
+        launch_st_j<NT, JB, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
+template <int NT, bool J>
+void launch_nt_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
+                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+{
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves - 1) / kWaves, E);
+    dim3 blk(64 * kWaves);
+    // accumulator blocks of JB sub-pixel rows: fewer for wide N, so that the
+    // accumulators leave room for more than one wave per SIMD
+    constexpr int JB = NT >= 3 ? 2 : 4;
+#define CQPI_NT(ldw) \
+    hipLaunchKernelGGL((k_cmix<NT, ldw, JB, J>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips)
+    switch (mg.LDW) {
+    case 8: CQPI_NT(8); break;
+    case 4: CQPI_NT(4); break;
+    case 2: CQPI_NT(2); break;
+    default: CQPI_NT(1); break;
+    }
+#undef CQPI_NT
 }
 
 template <int NT>
 int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag, const double *scale,
-              double sA, const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+              double sA, const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
 {
-    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves - 1) / kWaves, E);
-    dim3 blk(64 * kWaves);
     const half8 *A = (const half8 *)Afrag;
-    // accumulator blocks of JB sub-pixel rows: fewer for wide N, so that the
-    // accumulators leave room for more than one wave per SIMD
-    constexpr int JB = NT >= 3 ? 2 : 4;
-    switch (mg.LDW) {
-    case 8: hipLaunchKernelGGL((k_cmix<NT, 8, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 4: hipLaunchKernelGGL((k_cmix<NT, 4, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    case 2: hipLaunchKernelGGL((k_cmix<NT, 2, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    default: hipLaunchKernelGGL((k_cmix<NT, 1, JB>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos); break;
-    }
+    if (tips)
+        launch_nt_j<NT, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s);
+    else
+        launch_nt_j<NT, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -780,34 +838,36 @@ int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int
 
 int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
-                          double *ldos, hipStream_t s)
+                          double *ldos, hipStream_t s, const double2 *tips)
 {
     const half8 *A = (const half8 *)Afrag;
+    if (tips && 2 * g.maxb > g.B) return CQPI_E_ARG;
     // register-resident tile fragments for the common small shapes:
     // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital).
     // 3 orbitals: two waves share a cell row (half the fragments each, three
     // waves a SIMD instead of two: 4% faster)
-    if (try_areg<1, 4, 4, 4, 3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
+    // (LDOS maps only: the Josephson map takes the staged or the generic kernel)
+    if (!tips && (try_areg<1, 4, 4, 4, 3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
         try_areg<1, 4, 4, 2, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
         try_areg<2, 4, 4, 7, 5, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
-        try_areg<1, 2, 8, 1, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s))
+        try_areg<1, 2, 8, 1, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s)))
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     // staged B operand when the spin block's plane rows fit the LDS
     constexpr size_t kStLdsMax = 150 * 1024;
     if (g.os <= 16 && g.B <= 16) {
         switch (mg.NT) {
-        case 1: if (cmix_st_lds(mg, g, 1, 4) <= kStLdsMax) return launch_st<1, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-        case 2: if (cmix_st_lds(mg, g, 2, 4) <= kStLdsMax) return launch_st<2, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-        case 3: if (cmix_st_lds(mg, g, 3, 2) <= kStLdsMax) return launch_st<3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
-        case 4: if (cmix_st_lds(mg, g, 4, 2) <= kStLdsMax) return launch_st<4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s); break;
+        case 1: if (cmix_st_lds(mg, g, 1, 4) <= kStLdsMax) return launch_st<1, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
+        case 2: if (cmix_st_lds(mg, g, 2, 4) <= kStLdsMax) return launch_st<2, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
+        case 3: if (cmix_st_lds(mg, g, 3, 2) <= kStLdsMax) return launch_st<3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
+        case 4: if (cmix_st_lds(mg, g, 4, 2) <= kStLdsMax) return launch_st<4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
         default: break;
         }
     }
     switch (mg.NT) {
-    case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
-    case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
-    case 3: return launch_nt<3>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
-    case 4: return launch_nt<4>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, s);
+    case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
+    case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
+    case 3: return launch_nt<3>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
+    case 4: return launch_nt<4>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
     default: return CQPI_E_ARG;
     }
 }

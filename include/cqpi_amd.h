@@ -122,8 +122,8 @@ int cqpi_wannierldos(cqpi_ctx *ctx, const double *scat, const double *const *g0,
  * particle-hole block.  The context must have been created with
  * 2*maxband <= bands (the reference passes maxband = bands/2).  Arguments as
  * cqpi_wannierldos; tip = tip_re + i tip_im (the driver's energy-dependent
- * tip factor, src/calcqpi.cc:705-709).  FP64 or FP32 precision; retrieve the
- * map with cqpi_retrieve. */
+ * tip factor, src/calcqpi.cc:705-709).  Any precision mode; retrieve the map
+ * with cqpi_retrieve. */
 int cqpi_wannierjosephson(cqpi_ctx *ctx, const double *scat, const double *const *g0, size_t tda, double tip_re,
                           double tip_im);
 
@@ -283,7 +283,7 @@ int cqpi_set_timing(cqpi_ctx *ctx, int on);
  * the tip factor of energy w
  *   tip(w) = i (-sign(w) pi deltat) / sqrt((w + i etat)^2 - deltat^2)
  * (principal root; etat NaN: the run's eta, as the driver's default).  Needs
- * 2*maxband <= bands and FP64 or FP32 precision; a run with it on cannot feed
+ * 2*maxband <= bands (any precision mode); a run with it on cannot feed
  * cqpi_gradient. */
 int cqpi_set_josephson(cqpi_ctx *ctx, int on, double deltat, double etat);
 

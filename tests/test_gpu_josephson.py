@@ -5,7 +5,12 @@ src/qpi.cc:953-1033, called through the C ABI.
 G0(R) comes from a Bogoliubov-de Gennes model built from a seeded normal-state
 model: H_BdG(k) = [[H(k), D], [D^H, -H(-k)^*]] with constant s-wave pairing D,
 inverted per k and transformed by the pinned oracle transform, so the
-particle-hole block the map reads is nonzero."""
+particle-hole block the map reads is nonzero.
+
+Precision modes: FP64 to 1e-10; FP32 and mixed (split-fp16 MFMA products
+with FP32 accumulation) to 1e-4, in the metric of refmath.rel_errors.  The
+wide-window case (w = 17) takes the per-lane gather kernels in FP64 and FP32
+(the staged rows exceed the LDS) and the generic mixed kernel."""
 import numpy as np
 import pytest
 
@@ -18,10 +23,12 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-10
 TOL32 = 1e-4
+TOL = {"fp64": TOL64, "fp32": TOL32, "mixed": TOL32}
+_REF = {}
 
 
-def _bdg(nb, K=16, n=8, seed=51, delta=0.08, omega=0.03):
-    wl = synth.square_norb(nb, K=K, n=n, E=1, seed=seed, w=1, os_=2, eta=0.02)
+def _bdg(nb, K=16, n=8, seed=51, delta=0.08, omega=0.03, w=1):
+    wl = synth.square_norb(nb, K=K, n=n, E=1, seed=seed, w=w, os_=2, eta=0.02)
     k = synth.kgrid(K)
     kx, ky = np.meshgrid(k, k, indexing="ij")
     H = synth.hk_numpy(wl.hops, nb, wl.efermi, kx, ky)
@@ -79,6 +86,34 @@ def test_dropin_josephson_fp32(gpu):
     assert rm.rel_errors(got, ref)[0] < TOL32
 
 
+@pytest.mark.parametrize("nb,spin", [(1, False), (3, False), (6, True)])
+def test_dropin_josephson_mixed(gpu, nb, spin):
+    """The staged mixed-precision kernel with the b planes taken from the
+    particle-hole columns o + maxband (spin: per spin block)."""
+    wl, gR, T = _bdg(nb)
+    tip = _tip(omega=-0.02)
+    ref = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip, spin=spin)
+    got = _run(wl, gR, T, tip, spin, "mixed")
+    assert np.abs(ref).max() > 0
+    assert rm.rel_errors(got, ref)[0] < TOL32
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_dropin_josephson_wide_window(gpu, prec):
+    """w = 17 with 6 bands: (8 + 2w)^2 staged window rows of 6 slots per map
+    side exceed the LDS in FP64 (338 KB) and FP32 (169 KB), so the map runs
+    on the per-lane gather kernel; mixed precision runs the generic (global
+    B operand) kernel."""
+    wl, gR, T = _bdg(3, K=48, w=17)
+    tip = _tip()
+    if "wide" not in _REF:  # ~13 s of oracle work, shared by the three modes
+        _REF["wide"] = oracle.wannierjosephson(gR, T, wl.tiles, wl.n, wl.window, wl.oversamp, tip)
+    ref = _REF["wide"]
+    got = _run(wl, gR, T, tip, False, prec)
+    assert np.abs(ref).max() > 0
+    assert rm.rel_errors(got, ref)[0] < TOL[prec]
+
+
 def test_dropin_josephson_then_ldos(gpu):
     """The two maps share one context: a Josephson call followed by an LDOS
     call gives each map its own periodic table."""
@@ -110,13 +145,6 @@ def test_josephson_rejects_what_it_cannot_compute(gpu):
             q.wannierjosephson(T, gR, 1.0)
     finally:
         q.close()
-    q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, b, b // 2, False, wl.tiles)
-    try:
-        q.set_precision("mixed")
-        with pytest.raises(cqpi.CqpiError):
-            q.wannierjosephson(T, gR, 1.0)
-    finally:
-        q.close()
 
 
 def _oracle_chain_josephson(wl, deltat, etat=None):
@@ -133,7 +161,7 @@ def _oracle_chain_josephson(wl, deltat, etat=None):
     return np.array(out)
 
 
-@pytest.mark.parametrize("prec,etat", [("fp64", None), ("fp64", 0.004), ("fp32", None)])
+@pytest.mark.parametrize("prec,etat", [("fp64", None), ("fp64", 0.004), ("fp32", None), ("mixed", None)])
 def test_full_chain_josephson(gpu, prec, etat):
     """cqpi_run with the Josephson map selected: eigensolve, resolvent,
     transforms and T-matrix on the device, tip factor per energy."""
@@ -143,7 +171,7 @@ def test_full_chain_josephson(gpu, prec, etat):
     with cqpi.engine_for(wl, prec) as q:
         q.set_josephson(True, deltat=0.01, etat=float("nan") if etat is None else etat)
         got = q.run(wl.energies, wl.eta)
-    tol = TOL64 if prec == "fp64" else TOL32
+    tol = TOL[prec]
     for e in range(len(wl.energies)):
         assert np.abs(ref[e]).max() > 0
         assert rm.rel_errors(got[e], ref[e])[0] < tol, e
@@ -157,5 +185,3 @@ def test_full_chain_josephson_not_differentiated(gpu):
         maps = q.run(wl.energies, wl.eta)
         with pytest.raises(cqpi.CqpiError):
             q.gradient(np.ones_like(maps))
-        with pytest.raises(cqpi.CqpiError):
-            q.set_precision("mixed")
