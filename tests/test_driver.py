@@ -358,3 +358,19 @@ def test_driver_logfile_timestamps(tmp_path):
     lines = open(str(tmp_path / "run.log")).read().splitlines()
     assert sum("New calculation started, using configuration from" in ln for ln in lines) == 2
     assert lines and all(re.match(r"\d{4}-\d{2}-\d{2}\.\d{2}:\d{2}:\d{2} ", ln) for ln in lines)
+
+
+def test_driver_logfile_receives_error_messages(tmp_path):
+    """With logfile= set, error messages go to the log with the time stamp
+    (src/calcqpi.cc:162-170 routes cerr there): an input without tbfile
+    stops at the model check, before any device work, and the log holds the
+    time-stamped "No tightbinding model specified" line."""
+    import re
+    p = tmp_path / "calcqpi.in"
+    p.write_text('logfile="run.log";\nlattice=8;\nkpoints=16;\nlayers=2;\n')
+    r = subprocess.run([DRIVER, str(p)], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert r.returncode != 0
+    assert "No tightbinding model" not in r.stderr  # it went to the log
+    lines = open(str(tmp_path / "run.log")).read().splitlines()
+    assert any(re.match(r"\d{4}-\d{2}-\d{2}\.\d{2}:\d{2}:\d{2} No tightbinding model specified \.\.\. exiting\.$",
+                        ln) for ln in lines), lines

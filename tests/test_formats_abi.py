@@ -114,6 +114,41 @@ def test_idl_roundtrip_between_writers(tmp_path):
     assert np.array_equal(got2, d) and list(h2) == list(hdr)
 
 
+def test_idl_bytes_follow_the_file_format(tmp_path):
+    """Known-answer bytes written out by hand from the format of idl::write
+    (src/idl.cc:78-95): name, date, the three sizes, scan range x and y,
+    offset x and y, junction bias low and high, setpoint, one per line as
+    an ostream prints doubles (6 significant digits), then float32 values with
+    x fastest, then y, then the layer.  The header values are those of the
+    driver's map idl(n, n, L, 2.0, 2.0, lb, hb, 0.0, -1.0, -1.0)
+    (src/calcqpi.cc:655; constructor src/idl.cc:662-674), so the file order
+    puts the offset (-1, -1) before the bias (lb, hb).  Both writers must
+    produce these bytes and both readers must read them back."""
+    import struct
+    sx, sy, sz = 3, 2, 2
+    head = b"map\nThu Dec  3 10:00:00 2026\n3\n2\n2\n2\n2\n-1\n-1\n-0.125\n0.125\n0\n"
+    vals = [100.0 * k + 10.0 * j + i for k in range(sz) for j in range(sy) for i in range(sx)]
+    expect = head + struct.pack("<12f", *vals)
+    d = np.zeros((sx, sy, sz))
+    for i in range(sx):
+        for j in range(sy):
+            for k in range(sz):
+                d[i, j, k] = 100.0 * k + 10.0 * j + i
+    hdr = (2.0, 2.0, -1.0, -1.0, -0.125, 0.125, 0.0)
+    p1, p2 = str(tmp_path / "a.idl"), str(tmp_path / "b.idl")
+    synth.write_idl(p1, d, name="map", date="Thu Dec  3 10:00:00 2026", hdr=hdr)
+    oracle.idl_write(p2, d, name="map", date="Thu Dec  3 10:00:00 2026", hdr=hdr)
+    assert open(p1, "rb").read() == expect
+    assert open(p2, "rb").read() == expect
+    p3 = tmp_path / "c.idl"
+    p3.write_bytes(expect)
+    got, h = synth.read_idl(str(p3))
+    assert np.array_equal(got, d)
+    assert (h["scan_range"], h["offset"], h["bias"], h["setpoint"]) == ([2.0, 2.0], [-1.0, -1.0], [-0.125, 0.125], 0.0)
+    got2, h2 = oracle.idl_read(str(p3))
+    assert np.array_equal(got2, d) and list(h2) == list(hdr)
+
+
 def test_write_inputs(tmp_path):
     wl = synth.square_3orb(K=16, n=8, E=5)
     p = synth.write_inputs(str(tmp_path), wl)

@@ -386,6 +386,43 @@ def test_surface_green_semi_infinite_chain_closed_form(surface):
     assert np.allclose(g[..., 0, 0], ref, rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("t_perp", [0.0, 0.4 * np.exp(0.3j)])
+def test_surface_green_with_surface_model_closed_form(t_perp):
+    """stbfile (src/qpi.cc:488-554): the surface layer's in-plane Hamiltonian
+    comes from the surface model, here the bulk band shifted by delta, while
+    the layers below keep the bulk band e(k).  The surface site then sees the
+    semi-infinite bulk chain below it through the interlayer hopping t:
+      G_s = 1 / (z - e(k) - delta - |t|^2 g0),
+      g0 = (z - e - sqrt((z - e)^2 - 4|t|^2)) / (2|t|^2)  (Im g0 < 0),
+    and for uncoupled layers (t = 0) G_s = 1/(z - e(k) - delta).  The bulk
+    Green's function does not see the surface model."""
+    delta = 0.07
+    wl, hops = _layered_1band(t_perp) if t_perp else (synth.square_1orb(K=8, n=8, E=1), None)
+    if hops is None:
+        hops = np.array(list(wl.hops), synth.HOP_DTYPE)
+    shift = np.zeros(1, synth.HOP_DTYPE)
+    shift["tre"] = delta
+    shops = np.concatenate([np.array(list(wl.hops), synth.HOP_DTYPE), shift])
+    om, eta = 0.2, 0.05
+    g, it = oracle.surfacegreen(hops, 1, wl.efermi, 8, om, eta, epserr=1e-14, surface=True, shops=shops)
+    k = synth.kgrid(8)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    e = synth.hk_numpy(wl.hops, 1, wl.efermi, kx, ky)[..., 0, 0]
+    z = om + 1j * eta - e
+    t2 = abs(t_perp) ** 2
+    if t2:
+        root = np.sqrt(z * z - 4 * t2)
+        c0, c1 = (z - root) / (2 * t2), (z + root) / (2 * t2)
+        g0 = np.where(np.imag(c0) < 0, c0, c1)
+        ref = 1 / (z - delta - t2 * g0)
+    else:
+        ref = 1 / (z - delta)
+    assert np.allclose(g[..., 0, 0], ref, rtol=1e-10, atol=0)
+    gb, _ = oracle.surfacegreen(hops, 1, wl.efermi, 8, om, eta, epserr=1e-14, surface=False, shops=shops)
+    gb0, _ = oracle.surfacegreen(hops, 1, wl.efermi, 8, om, eta, epserr=1e-14, surface=False)
+    assert np.array_equal(gb, gb0)
+
+
 def test_bands_histogram_flat_band_closed_form():
     """A flat band e0 (on-site term only): every k adds the same Lorentzian
     w/2/((bias - e0)^2 + w^2/4)/pi, w = 2 (ue - le)/layers, to the three layers
@@ -403,6 +440,39 @@ def test_bands_histogram_flat_band_closed_form():
         x = le + (ue - le) * nl / (L - 1) - e0
         ref[nl] = os_ * os_ * w / 2 / (x * x + w * w / 4) / np.pi
     assert np.allclose(h, ref[None, None, :], rtol=1e-12, atol=0)
+
+
+def test_bands_histogram_dispersive_band_on_the_reference_k_grid():
+    """The histogram's k-grid (src/tightbinding.cc:1158-1193): sub-pixel io of
+    ospix = pix*os samples k = io/(ospix-1) - 1/2, the ends included, and
+    pixel (x, y) = (io/os, jo/os) takes the band energies of (kx(io), ky(jo)).
+    With the anisotropic band e(k) = 2 tx cos 2pi kx + 2 ty cos 2pi ky and
+    ospix = 3 the samples are kx, ky in {-1/2, 0, 1/2}, cos in {-1, 1, -1}, so
+    pixel (x, y) holds one energy 2 tx c(x) + 2 ty c(y) in {-0.3, -0.1, 0.1,
+    0.3}, each the centre of a layer: that layer gets 2/(pi w) and its two
+    neighbours w/2/(bin^2 + w^2/4)/pi.  A grid io/ospix - 1/2, swapped axes or
+    a dropped sub-pixel would move energies off the layer centres or between
+    pixels."""
+    le, ue, L = -1.0, 1.0, 21
+    tx, ty = 0.1, 0.05
+    hops = np.zeros(4, synth.HOP_DTYPE)
+    for i, (rx, ry, t) in enumerate([(1, 0, tx), (-1, 0, tx), (0, 1, ty), (0, -1, ty)]):
+        hops[i]["rx"], hops[i]["ry"], hops[i]["tre"] = rx, ry, t
+    h = oracle.bands_histogram(hops, 1, 0.0, 3, 1, le, ue, L)
+    binsize = (ue - le) / (L - 1)
+    w = 2 * (ue - le) / L
+    c = [-1.0, 1.0, -1.0]
+    for x in range(3):
+        for y in range(3):
+            e = 2 * tx * c[x] + 2 * ty * c[y]
+            lay = int(round((e - le) / binsize))
+            ref = np.zeros(L)
+            ref[lay] = 2 / (np.pi * w)
+            ref[lay - 1] = ref[lay + 1] = w / 2 / (binsize ** 2 + w * w / 4) / np.pi
+            assert np.allclose(h[x, y], ref, rtol=1e-9, atol=1e-12), (x, y, e)
+    # the same grid summed over 3 x 3 sub-pixels of one pixel
+    h1 = oracle.bands_histogram(hops, 1, 0.0, 1, 3, le, ue, L)
+    assert np.allclose(h1[0, 0], h.sum(axis=(0, 1)), rtol=1e-12, atol=0)
 
 
 def test_cell_block_contraction_equals_full_map_on_the_block():
