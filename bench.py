@@ -12,14 +12,15 @@ accumulation, FP32 resolvent and transform lines, FP64 eigensolve, T-matrix,
 periodic term and G0 window.  The same line reports the FP32 and FP64 paths,
 their ratios, the accuracy per energy against the FP64 path and the FP64 CPU
 oracle, the roofline of the dominant kernels, an end-to-end rate that
-includes moving the maps to the host, the 10-band hexagonal workload (K=1024,
+includes moving the maps to the host, the single-orbital workload (K=256, 11
+energies, 64x64 cells, os=8, w=1), the 10-band hexagonal workload (K=1024,
 101 energies, 256x256 cells, os=6, w=3), the batched workload (128 uneven
 instances in one HIP graph), the gradient workload, the Josephson,
 threshold-list and surface variants and the CPU baseline.  Prints one JSON
 line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
-                  [--no-grad] [--no-variants] [--no-hex] [--workload NAME] [--energies N]
+                  [--no-grad] [--no-variants] [--no-hex] [--no-config1] [--workload NAME] [--energies N]
   python bench.py --probe [--workload NAME] [--energies N]
                                  one mixed and one FP64 step after a 256 MiB
                                  device copy (the traffic calibration), for
@@ -434,6 +435,59 @@ def hex_timing(cqpi, synth, torch, steps, warmup, with_oracle, cells=16, oracle_
     return out
 
 
+def config1_timing(cqpi, synth, torch, steps, warmup, with_oracle):
+    """The single-orbital workload (K=256, 11 energies in [-0.5,0.5], eta=0.01,
+    64x64 cells, os=8, w=1): FP64, FP32 and mixed steps (eigensolve + all
+    energies) with stage times; its contraction (88 useful flops per point) is
+    bound by HBM: the algorithmic bytes per energy are the map written once
+    (8 W^2) plus the G0 window read once (16 b^2 nw^2).  with_oracle: every
+    energy of every map against the FP64 oracle (the whole workload takes the
+    oracle about a second)."""
+    wl = synth.workload("square_1orb")
+    E = len(wl.energies)
+    out = {"workload": "square lattice 1 orbital (t=-1, t'=0.3, mu=0.5), K=256, 11 energies in [-0.5,0.5], "
+                       "eta=0.01, 64x64 cells, os=8, w=1 (512x512 pixels), V=1.0", "points": wl.points}
+    qs = {}
+    for prec in ("fp64", "fp32", "mixed"):
+        q = cqpi.engine_for(wl, prec)
+        dt, st = time_steps(q, wl, steps, warmup, torch)
+        q.run(wl.energies, wl.eta, host=False)
+        t, nl = stage_launch_s(st, "contraction")
+        nw = 2 * q.window_half() + 1
+        per_launch = E * (8 * wl.W * wl.W + 16 * wl.bands ** 2 * nw * nw) / max(nl, 1)
+        out[prec] = {"ms_per_step": dt * 1e3, "points_per_s": wl.points / dt,
+                     "stages_ms": {k: v[0] for k, v in st.items()},
+                     "contraction_roofline": {
+                         "bound": "hbm", "achieved": per_launch / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": per_launch / t / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": per_launch,
+                         "avg_launch_ms": t * 1e3, "launches_per_step": nl,
+                         "flops_per_point": flops_per_point(wl),
+                         "note": "map written once (8 W^2 B) + G0 window read once (16 b^2 nw^2 B) per energy"},
+                     "transform_roofline": transform_roofline(wl, q, st, prec, "square_1orb")}
+        qs[prec] = q
+    for prec in ("fp32", "mixed"):
+        out["speedup_%s_vs_fp64" % prec] = out["fp64"]["ms_per_step"] / out[prec]["ms_per_step"]
+    acc = {}
+    r = qs["fp64"].ldos_tensor()
+    for prec in ("fp32", "mixed"):
+        pe = per_energy_errors(qs[prec].ldos_tensor(), r, torch)
+        acc["%s_vs_fp64_gpu_max_rel" % prec] = max(x[0] for x in pe)
+    if with_oracle:
+        oracle = load_oracle()
+        t0 = time.perf_counter()
+        ref = oracle.cldos_run(wl)
+        acc["oracle_s"] = time.perf_counter() - t0
+        for prec in ("fp64", "fp32", "mixed"):
+            pe = per_energy_errors(qs[prec].ldos_tensor().cpu().numpy(), ref, torch)
+            acc["%s_vs_oracle_max_rel" % prec] = max(x[0] for x in pe)
+            acc["%s_vs_oracle_rms_rel" % prec] = max(x[1] for x in pe)
+            acc["%s_vs_oracle_per_energy" % prec] = pe
+    out["accuracy"] = acc
+    for q in qs.values():
+        q.close()
+    return out
+
+
 def batch_timing(cqpi, synth, torch, count=128, reps=5):
     """Batched workload: `count` independent instances with uneven shapes
     (orbitals {1,3,5}, k-grids {128,256,512}, 16-64 energies, maps
@@ -527,6 +581,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-hex", action="store_true", help="skip the 10-band hexagonal workload")
+    ap.add_argument("--no-config1", action="store_true", help="skip the single-orbital workload")
     ap.add_argument("--no-variants", action="store_true", help="skip the Josephson, threshold-list and surface Green's function maps")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--stage-events", action="store_true",
@@ -613,6 +668,9 @@ def main():
     }
     for q, _, _ in timed.values():
         q.close()
+    if not args.no_config1 and default:
+        print("bench: single-orbital workload", file=sys.stderr, flush=True)
+        out["config1_1orb"] = config1_timing(cqpi, synth, torch, max(5, args.steps // 2), 5, not args.no_cpu)
     if not args.no_hex and default:
         print("bench: 10-band hexagonal workload", file=sys.stderr, flush=True)
         out["hex_10band"] = hex_timing(cqpi, synth, torch, 2, 1, not args.no_cpu)
