@@ -368,7 +368,7 @@ int mix_buffers(cqpi_ctx *c, int E)
 // device.  planes_ready: the transform already wrote the mixed-precision
 // planes and scales (whole-chain path); otherwise they are made from G0w.
 int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *ldos, bool planes_ready,
-             size_t eo = 0, hipStream_t s = nullptr, const double2 *tips = nullptr)
+             size_t eo = 0, hipStream_t s = nullptr, const double2 *tips = nullptr, bool bulk_ready = false)
 {
     // eo: offset of these energies in the chunk buffers (periodic term, planes,
     // scales, FP32 copies); s: stream (default: the context stream); tips:
@@ -397,25 +397,29 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
         if (r) return fail(r, "threshold-list contraction launch failed");
         return 0;
     }
-    int st = stage_begin(c, 3);
-    if (tips)
-        RETCHK(launch_bulk_josephson(g, G0w, c->d_tiles, E, reinterpret_cast<double2 *>(bulk), s));
-    else
-        RETCHK(launch_bulk(g, G0w, c->d_tiles, E, bulk, s));
-    stage_end(c, st);
+    int st = 0;
+    if (!bulk_ready) {
+        st = stage_begin(c, 3);
+        if (tips)
+            RETCHK(launch_bulk_josephson(g, G0w, c->d_tiles, E, reinterpret_cast<double2 *>(bulk), s));
+        else
+            RETCHK(launch_bulk(g, G0w, c->d_tiles, E, bulk, s));
+        stage_end(c, st);
+    }
     st = stage_begin(c, 4);
     int r = 0;
     switch (c->precision) {
     case CQPI_PREC_MIXED: {
+        const bool tf = !tips && mix_tfold(g, c->mg);
         if (!planes_ready) {
             RETCHK(mix_buffers(c, (int)(eo + E)));
             r = launch_mix_prepare(g, c->mg, G0w, E, c->d_mscale + eo,
-                                   c->d_planes + eo * mix_plane_halves(g, c->mg), s);
+                                   c->d_planes + eo * mix_plane_halves(g, c->mg), s, tf);
             c->up_scale.clear();  // the device scales now differ from the last upload
         }
         if (!r)
             r = launch_contract_mixed(g, c->mg, c->d_planes + eo * mix_plane_halves(g, c->mg), c->d_afrag,
-                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s, tips);
+                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s, tips, tf);
         break;
     }
     case CQPI_PREC_FP32:
@@ -554,6 +558,11 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     // reduced-precision modes run the resolvent and both transform passes in
     // FP32 where the Stockham path applies (F1 then holds float2)
     const bool t32 = c->precision != CQPI_PREC_FP64 && transform32_supported(g);
+    // the T-folded contraction reads the FP32 window, which the FP32 kx pass
+    // writes into the plane buffer; after an FP64 kx pass the contraction
+    // packs it from the FP64 window itself
+    const bool tf = plan.mixed && !c->josephson && mix_tfold(g, c->mg);
+    const bool planes_ready = plan.mixed && (!tf || t32);
     float2 *F1f = reinterpret_cast<float2 *>(c->d_F1) + eo * f1e;
     double2 *F1d = c->d_F1 + eo * f1e;
     int st = stage_begin(c, 1);
@@ -588,15 +597,22 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
     st = stage_begin(c, 2);
     if (t32)
         RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes,
-                                     plan.rwin, s));
+                                     plan.rwin, s, tf ? reinterpret_cast<float2 *>(planes) : nullptr));
     else
-        RETCHK(launch_transform_kx(g, plan.mixed ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
+        RETCHK(launch_transform_kx(g, planes_ready ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
     stage_end(c, st);
     st = stage_begin(c, 3);
-    RETCHK(launch_tmatrix(g, G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, s));
+    // the T-matrix and (but for the threshold list, whose kernel forms it) the
+    // periodic term in one launch
+    const bool tb = !c->list_mode;
+    if (tb)
+        RETCHK(launch_tmatrix_bulk(g, G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, c->d_tiles,
+                                   c->d_bulk + (c->josephson ? 2 : 1) * eo * g.os * g.os, c->josephson, s));
+    else
+        RETCHK(launch_tmatrix(g, G0w, c->d_V, ec, c->d_T + e0 * NP, c->d_status + e0, s));
     stage_end(c, st);
-    RETCHK(contract(c, G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, plan.mixed, eo, s,
-                    c->josephson ? c->d_tips + e0 : nullptr));
+    RETCHK(contract(c, G0w, c->d_T + e0 * NP, ec, c->d_ldos + e0 * WW, planes_ready, eo, s,
+                    c->josephson ? c->d_tips + e0 : nullptr, tb));
     return 0;
 }
 
@@ -1207,8 +1223,9 @@ int cqpi_wannierldos_device(cqpi_ctx *c, const double *d_T, const double *d_g0wi
     case CQPI_PREC_MIXED: {
         double *scale = reinterpret_cast<double *>(sp + o_a);
         _Float16 *planes = reinterpret_cast<_Float16 *>(sp + o_b);
-        r = launch_mix_prepare(g, c->mg, G0w, En, scale, planes, s);
-        if (!r) r = launch_contract_mixed(g, c->mg, planes, c->d_afrag, scale, c->sA, T, bulk, En, d_ldos, s);
+        const bool tf = mix_tfold(g, c->mg);
+        r = launch_mix_prepare(g, c->mg, G0w, En, scale, planes, s, tf);
+        if (!r) r = launch_contract_mixed(g, c->mg, planes, c->d_afrag, scale, c->sA, T, bulk, En, d_ldos, s, nullptr, tf);
         break;
     }
     case CQPI_PREC_FP32:
@@ -1461,7 +1478,9 @@ int cqpi_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, double *ld
     if (!c || (!omegas && E)) return fail(CQPI_E_ARG, "NULL argument");
     if (!c->has_model) return fail(CQPI_E_STATE, "cqpi_set_model has not been called");
     if (E == 0) return fail(CQPI_E_ARG, "no energies");
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // no stream synchronisation here: an eigensolve enqueued just before
+    // keeps running while this run is prepared and enqueued (the previous
+    // run ended synchronised in finish_run; stage events wait for themselves)
     stage_collect(c);
     // the timing window covers this run, plus a cqpi_compute_eigen called just before it
     if (!c->eigen_timed) reset_stages(c);

@@ -181,8 +181,10 @@ int launch_resolvent_ky32(const Geom &g, const double *eval, const double2 *evec
                           double eta, const float2 *tw, float2 *F1, void *work, size_t work_bytes, hipStream_t s);
 // rwin: half-width of the part of the window written to G0w (g.h: all of it;
 // the mixed contraction needs only |R| <= 2w, for the T-matrix and the bulk term)
+// W32: write the whole window in FP32 as W32[e][sy][sx][pair] instead of the
+// fp16 planes (the T-folded contraction's input)
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
-                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s);
+                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s, float2 *W32 = nullptr);
 // T[e] = (1 - V G0w[e](0))^{-1} V; status[e] = 1 if singular
 int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T,
                    int *status, hipStream_t s);
@@ -193,6 +195,9 @@ int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, d
 // sum w_o1 w_o2 G0(R2-R1)_{o2, o1+maxband}
 int launch_bulk_josephson(const Geom &g, const double2 *G0w, const double *tiles, int E, double2 *bulk,
                           hipStream_t s);
+// launch_tmatrix and launch_bulk (or launch_bulk_josephson) in one launch
+int launch_tmatrix_bulk(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T, int *status,
+                        const double *tiles, void *bulk, bool josephson, hipStream_t s);
 // adjoint of the whole chain (see adjoint.hip)
 size_t adjoint_pix_blocks(const Geom &g);
 // doubles of the energy-independent weight-overlap table of the periodic term

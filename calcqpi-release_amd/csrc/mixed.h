@@ -33,11 +33,14 @@ namespace cqpi {
 MixGeom mix_geometry(const Geom &g);
 size_t mix_plane_halves(const Geom &g, const MixGeom &mg);
 std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const double *tiles, double &sA);
+// shapes served by the T-folded contraction, whose operand buffer P holds the
+// FP32 window W32[e][sy][sx][pair] instead of the fp16 planes
+bool mix_tfold(const Geom &g, const MixGeom &mg);
 int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int E, double *scale,
-                       _Float16 *P, hipStream_t s);
+                       _Float16 *P, hipStream_t s, bool tfold = false);
 // tips: device tip factors of the E energies for the Josephson map (bulk then
 // holds the complex periodic table, E x os^2 double2); NULL: the LDOS map
 int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
-                          double *ldos, hipStream_t s, const double2 *tips = nullptr);
+                          double *ldos, hipStream_t s, const double2 *tips = nullptr, bool tfold = false);
 }  // namespace cqpi

@@ -510,6 +510,299 @@ bool try_areg(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *
     return true;
 }
 
+// T-folded contraction for one orbital block (LDOS maps, BT <= 5 bands, every
+// orbital with a Wannier function).  With c = T b formed before the products,
+//   c_p = sum_{R1,o1} (T G0(-pos-R1))[p][o1] w_o1,   rho = -Im sum_p a_p c_p - bulk,
+// so no per-pixel T sandwich remains: the workgroup stages, from the FP32
+// window W32[e][sy][sx][pair] (pair = r*BT + c holds G0(R)[r][c],
+// R = (sx - h, sy - h)), the a rows G0(pos+R)[o][p] and the folded c rows
+// (T G0(-pos-R))[p][o] = sum_q T_pq G0(-pos-R)[q][o], split into fp16 hi/lo.
+// The MFMA then forms C^T[ch][pix]: its A operand is a window fragment whose
+// row is the channel ch = 4p + j (j: a_re, a_im, c_re, c_im), its B operand
+// the tile fragment whose column is the pixel, so that lane (kq, col) holds
+// the four channels of p = 4t + kq of pixel col and the pixel's sum over p is
+// a sum over the four rows of lanes.  The lo parts are stored unscaled (an
+// fp16 residual is exact to 2^-24 absolute, far below the hi part's scale), so
+// that the three products hi*hi, lo*hi and hi*lo share one accumulator.
+// Geometry is compile-time: OS sub-pixels, window WW, BT bands, CB cells per
+// tile (CB*OS = 16 rows), K = (dy, o, dx) in groups of 8 columns.
+template <int OS, int WW>
+struct TfGeom {
+    static constexpr int CB = 16 / OS;
+    static constexpr int NU = CB + 2 * WW;
+    static constexpr int NUG = (NU + 7) / 8;
+    static constexpr int W2 = 2 * WW + 1;
+};
+
+// waves per workgroup and per SIMD of k_cmix_tf (eight-wave workgroups at
+// four waves a SIMD measured 6 % slower for 3 bands)
+constexpr int tf_wg_waves(int) { return kWaves; }
+constexpr int tf_simd_waves(int bt) { return areg_waves(bt); }
+
+template <int OS, int WW, int BT, int TPW, int JS>
+__global__ void __launch_bounds__(64 * tf_wg_waves(BT)) __attribute__((amdgpu_waves_per_eu(tf_simd_waves(BT), tf_simd_waves(BT)))) k_cmix_tf(Geom g, const float2 *__restrict__ W32,
+                                                         const half8 *__restrict__ Afrag,
+                                                         const double *__restrict__ scale, double sA,
+                                                         const double2 *__restrict__ Tm,
+                                                         const double *__restrict__ bulk,
+                                                         double *__restrict__ ldos)
+{
+    using TG = TfGeom<OS, WW>;
+    constexpr int CB = TG::CB, NUG = TG::NUG, SEG8 = NUG * 8;
+    constexpr int KG = TG::W2 * BT * NUG, KS = (KG + 3) / 4, NT = (4 * BT + 15) / 16;
+    constexpr int OSW = OS / JS, CPS = tf_wg_waves(BT) / JS;
+    constexpr int NP = BT * BT, NPL = NP * 2;
+    constexpr int RW = TPW * CPS + 2 * WW, RP = RW | 1;
+    constexpr int PSTR = RP * SEG8;             // halves per staged plane
+    constexpr int LOFF = NPL * PSTR, ZOFF = 4 * NPL * PSTR;
+    __shared__ float2 Tf[NP];
+    __shared__ float sTsh;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j0 = (wave % JS) * OSW, wrow = wave / JS;
+    int bx, by, e;
+    xcd_block(bx, by, e);
+    const int n = g.n, h = g.h, nw = g.nw;
+    const int n2 = n >> 1;
+    const int cx0 = bx * CB;
+    const double sG = scale[e];
+    const double2 *Te = Tm + (size_t)e * NP;
+    if (threadIdx.x < NP) Tf[threadIdx.x] = make_float2((float)Te[threadIdx.x].x, (float)Te[threadIdx.x].y);
+    if (threadIdx.x == 0) {
+        // power-of-two scale of the folded rows: |(T G0)_po| <= max_p sum_q |T_pq| * max|G0|,
+        // and max|G0| sG < 2^13
+        double ts = 0.0;
+        for (int p = 0; p < BT; p++) {
+            double r = 0.0;
+            for (int q = 0; q < BT; q++) r += fabs(Te[p * BT + q].x) + fabs(Te[p * BT + q].y);
+            ts = fmax(ts, r);
+        }
+        double sT = 1.0;
+        if (ts > 0.0) {
+            int ex;
+            frexp(ts * 8192.0 / sG, &ex);
+            sT = ldexp(1.0, 13 - ex);
+        }
+        sTsh = (float)sT;
+    }
+
+    __syncthreads();
+    const float sGf = (float)sG, sTf = sTsh;
+    const double unscale = 1.0 / (sA * sA * sG * (double)sTf);
+    const int W = g.W;
+    double *out = ldos + (size_t)e * W * W;
+
+    // Stage the rows of the workgroup's tiles: plane rows row_lo .. + RW - 1,
+    // window columns sx0 .. + 8 NUG - 1, for every (o, p): the a rows and the
+    // folded c rows, hi and lo, re and im.  LDS layout [plane][row (RP)][8 NUG]
+    // with plane = ((a|c) * 2 + hi|lo) * NPL + (o BT + p) * 2 + re|im.
+    extern __shared__ __align__(16) unsigned char smem_m[];
+    _Float16 *Pl = reinterpret_cast<_Float16 *>(smem_m);
+    const int cy_lo = by * TPW * CPS;
+    const int row_lo = cy_lo - n2 - WW + h;
+    const int sx0 = cx0 - n2 - WW + h;
+    const float2 *We = W32 + (size_t)e * nw * nw * NP;
+    for (int it = threadIdx.x; it < RW * NUG * NP; it += blockDim.x) {
+        const int op = it % NP, rest = it / NP;
+        const int o = op / BT, p = op - o * BT;
+        const int g8 = rest % NUG, r = rest / NUG;
+        const int row = row_lo + r;
+        const int rowc = min(max(row, 0), nw - 1);
+        const bool rok = row == rowc;
+        const float2 *arow = We + (size_t)rowc * nw * NP + op;
+        const float2 *brow = We + (size_t)(2 * h - rowc) * nw * NP + o;
+        // every load first (indices clamped into the window, values outside it
+        // dropped below), then the arithmetic
+        float2 av[8], bv[8][BT];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int sxc = min(max(sx0 + g8 * 8 + u, 0), nw - 1);
+            av[u] = arow[(size_t)sxc * NP];
+#pragma unroll
+            for (int q = 0; q < BT; q++) bv[u][q] = brow[(size_t)(2 * h - sxc) * NP + q * BT];
+        }
+        float2 c0[BT];
+#pragma unroll
+        for (int q = 0; q < BT; q++) c0[q] = Tf[p * BT + q];
+        half8 ahr, alr, ahi, ali, chr, clr, chi, cli;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int sx = sx0 + g8 * 8 + u;
+            const bool ok = rok && sx >= 0 && sx < nw;
+            float yr = 0.f, yi = 0.f;
+#pragma unroll
+            for (int q = 0; q < BT; q++) {
+                yr += c0[q].x * bv[u][q].x - c0[q].y * bv[u][q].y;
+                yi += c0[q].x * bv[u][q].y + c0[q].y * bv[u][q].x;
+            }
+            const float xr = ok ? av[u].x * sGf : 0.f, xi = ok ? av[u].y * sGf : 0.f;
+            yr = ok ? yr * sTf : 0.f;
+            yi = ok ? yi * sTf : 0.f;
+            // v - hi is exact in FP32 (power-of-two scales)
+            _Float16 t;
+            t = (_Float16)xr; ahr[u] = t; alr[u] = (_Float16)(xr - (float)t);
+            t = (_Float16)xi; ahi[u] = t; ali[u] = (_Float16)(xi - (float)t);
+            t = (_Float16)yr; chr[u] = t; clr[u] = (_Float16)(yr - (float)t);
+            t = (_Float16)yi; chi[u] = t; cli[u] = (_Float16)(yi - (float)t);
+        }
+        _Float16 *dst = Pl + (size_t)(op * 2) * PSTR + r * SEG8 + g8 * 8;
+        *reinterpret_cast<half8 *>(dst) = ahr;
+        *reinterpret_cast<half8 *>(dst + PSTR) = ahi;
+        *reinterpret_cast<half8 *>(dst + LOFF) = alr;
+        *reinterpret_cast<half8 *>(dst + LOFF + PSTR) = ali;
+        *reinterpret_cast<half8 *>(dst + 2 * LOFF) = chr;
+        *reinterpret_cast<half8 *>(dst + 2 * LOFF + PSTR) = chi;
+        *reinterpret_cast<half8 *>(dst + 3 * LOFF) = clr;
+        *reinterpret_cast<half8 *>(dst + 3 * LOFF + PSTR) = cli;
+    }
+    if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + ZOFF) = half8{0, 0, 0, 0, 0, 0, 0, 0};
+    __syncthreads();
+
+    half8 ah[OSW][KS], al[OSW][KS];
+#pragma unroll
+    for (int jj = 0; jj < OSW; jj++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const size_t fa = (((size_t)(j0 + jj) * KS + ks) * 2) * 64;
+            ah[jj][ks] = Afrag[fa + lane];
+            al[jj][ks] = Afrag[fa + 64 + lane];
+        }
+    // per lane and K step: the staged offset of its window fragment (channel
+    // ch = t*16 + col) in the hi plane at tile row 0, or the zero slot where
+    // the K group or the channel is padding
+    const int col = lane & 15, kq = lane >> 4;
+    int qoff[KS][NT];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const int kg = ks * 4 + kq;
+        const int g8 = kg % NUG, r = kg / NUG;
+        const int dy = r / BT, o = r % BT;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int ch = t * 16 + col, p = ch >> 2, j = ch & 3;
+            const int pid = (j >> 1) * 2 * NPL + (o * BT + p) * 2 + (j & 1);
+            qoff[ks][t] = (kg < KG && p < BT) ? (pid * RP + dy) * SEG8 + g8 * 8 : -1;
+        }
+    }
+    // the pixel of this lane's column and the periodic term of its sub-pixel rows
+    const int c = col / OS, ii = col - c * OS, cx = cx0 + c;
+    double bk[OSW];
+#pragma unroll
+    for (int i = 0; i < OSW; i++) bk[i] = bulk[(size_t)e * OS * OS + ii * OS + j0 + i];
+
+    for (int tt = 0; tt < TPW; tt++) {
+        const int cy = (by * TPW + tt) * CPS + wrow;
+        if (cy >= n) break;  // uniform per wave; no block-wide barrier follows
+        const int trow = (cy - cy_lo) * SEG8;
+        half8 bh[KS][NT], bl[KS][NT];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int q = qoff[ks][t];
+                const int a0 = q >= 0 ? q + trow : ZOFF, a1 = q >= 0 ? q + trow + LOFF : ZOFF;
+                bh[ks][t] = *reinterpret_cast<const half8 *>(Pl + a0);
+                bl[ks][t] = *reinterpret_cast<const half8 *>(Pl + a1);
+            }
+        float4v acc[OSW][NT];
+#pragma unroll
+        for (int i = 0; i < OSW; i++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+            for (int i = 0; i < OSW; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[ks][t], ah[i][ks], acc[i][t], 0, 0, 0);
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl[ks][t], ah[i][ks], acc[i][t], 0, 0, 0);
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[ks][t], al[i][ks], acc[i][t], 0, 0, 0);
+                }
+        }
+        // epilogue: a_re c_im + a_im c_re of this lane's p, summed over the
+        // four lane rows (p = 4t + kq).  FP32 adds rounding of the order of the
+        // accumulation's own; the row sums use the lane swaps of gfx950
+        // (a swap's two results are, per lane, x[l] and x[l ^ 16] (x[l ^ 32]))
+        float val[OSW];
+#pragma unroll
+        for (int i = 0; i < OSW; i++) {
+            float part = acc[i][0][0] * acc[i][0][3] + acc[i][0][1] * acc[i][0][2];
+#pragma unroll
+            for (int t = 1; t < NT; t++) part += acc[i][t][0] * acc[i][t][3] + acc[i][t][1] * acc[i][t][2];
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            part = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+            val[i] = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+        }
+        // lane row kq stores the sub-pixel rows j0 + 4 ib + kq
+#pragma unroll
+        for (int ib = 0; ib < (OSW + 3) / 4; ib++) {
+            float mine = 0.f;
+            double b = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++)
+                if (kq == k2 && ib * 4 + k2 < OSW) {
+                    mine = val[ib * 4 + k2];
+                    b = bk[ib * 4 + k2];
+                }
+            const int i = ib * 4 + kq;
+            if (i < OSW && cx < n) {
+                const int X = cx * OS + ii, Y = cy * OS + j0 + i;
+                out[(size_t)X + (size_t)Y * W] = -((double)mine * unscale + b);
+            }
+        }
+    }
+}
+
+template <int OS, int WW, int BT, int TPW, int JS>
+void launch_tf(const Geom &g, const float2 *W32, const half8 *A, const double *scale, double sA, const double2 *T,
+               const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    using TG = TfGeom<OS, WW>;
+    constexpr int CPS = tf_wg_waves(BT) / JS, RW = TPW * CPS + 2 * WW, RP = RW | 1;
+    dim3 grid((g.n + TG::CB - 1) / TG::CB, (g.n + CPS * TPW - 1) / (CPS * TPW), E);
+    const size_t lds = ((size_t)4 * 2 * BT * BT * RP * TG::NUG * 8 + 8) * sizeof(_Float16);  // + the zero slot
+    hipLaunchKernelGGL((k_cmix_tf<OS, WW, BT, TPW, JS>), grid, dim3(64 * tf_wg_waves(BT)), lds, s, g, W32, A, scale, sA, T,
+                       bulk, ldos);
+}
+
+template <int OS, int WW, int BT, int JS>
+bool tf_shape(const Geom &g, const MixGeom &mg)
+{
+    using TG = TfGeom<OS, WW>;
+    // (the compile-time geometry must be the context's)
+    return g.os == OS && g.w == WW && g.B == BT && g.maxb == BT && !g.spin && mg.nsets == 1 && mg.CB == TG::CB &&
+           mg.NUg == TG::NUG && mg.nper == BT;
+}
+
+template <int OS, int WW, int BT, int JS>
+bool try_tf(const Geom &g, const MixGeom &mg, const float2 *W32, const half8 *A, const double *scale, double sA,
+            const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    if (!tf_shape<OS, WW, BT, JS>(g, mg)) return false;
+    switch (g.tpw) {
+    case 2: launch_tf<OS, WW, BT, 2, JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
+    case 4: launch_tf<OS, WW, BT, 4, JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
+    default: launch_tf<OS, WW, BT, kTPW * JS * 4 / tf_wg_waves(BT), JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
+    }
+    return true;
+}
+
+// the FP64 window [e][sx][sy][pair] of the drop-in path as the FP32 window
+// W32[e][sy][sx][pair] of the T-folded contraction
+__global__ void k_pack32(Geom g, const double2 *__restrict__ G0w, float2 *__restrict__ W32)
+{
+    const int sy = blockIdx.x, e = blockIdx.y, nw = g.nw, NP = g.B * g.B;
+    const double2 *G = G0w + (size_t)e * nw * nw * NP;
+    float2 *o = W32 + ((size_t)e * nw + sy) * nw * NP;
+    for (int t = threadIdx.x; t < nw * NP; t += blockDim.x) {
+        const int sx = t / NP, q = t - sx * NP;
+        const double2 z = G[((size_t)sx * nw + sy) * NP + q];
+        o[t] = make_float2((float)z.x, (float)z.y);
+    }
+}
+
 // Generic shapes with the B operand staged in LDS (spin blocks, many bands,
 // any os): one workgroup of kStWaves waves covers kStWaves consecutive cell
 // rows (one per wave) and one tile of CB cells; for each spin block it copies
@@ -800,6 +1093,8 @@ std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const
         std::frexp(mx, &ex);
         sA = std::ldexp(1.0, 13 - ex);
     }
+    // the T-folded contraction keeps lo parts unscaled (one accumulator for the three products)
+    const double loscale = mix_tfold(g, mg) ? 1.0 : (double)kLoScale;
     std::vector<_Float16> F((size_t)mg.nsets * g.os * mg.KS * 2 * 64 * 8);
     const int rows = mg.CB * g.os, T = g.T, w = g.w, os = g.os;
     for (int sg = 0; sg < mg.nsets; sg++)
@@ -819,7 +1114,7 @@ std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const
                             }
                         }
                         _Float16 hi = (_Float16)v;
-                        _Float16 lo = (_Float16)((v - (double)hi) * kLoScale);
+                        _Float16 lo = (_Float16)((v - (double)hi) * loscale);
                         size_t base = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64 * 8;
                         F[base + (size_t)lane * 8 + j] = hi;
                         F[base + 64 * 8 + (size_t)lane * 8 + j] = lo;
@@ -827,21 +1122,48 @@ std::vector<_Float16> mix_tile_fragments(const Geom &g, const MixGeom &mg, const
     return F;
 }
 
+// shapes of the T-folded contraction (k_cmix_tf): one orbital block, os 4
+// (w=2: K steps 2/4/7 for 1/3/5 orbitals) or os 8 (w=1, 1 orbital)
+// (os, w, bands, waves per cell row)
+#define CQPI_TF_SHAPES(X) X(4, 2, 3, 2) X(4, 2, 1, 1) X(4, 2, 5, 1) X(8, 1, 1, 1)
+
+bool mix_tfold(const Geom &g, const MixGeom &mg)
+{
+#define CQPI_TF_IS(os, ww, bt, js) if (tf_shape<os, ww, bt, js>(g, mg)) return true;
+    CQPI_TF_SHAPES(CQPI_TF_IS)
+#undef CQPI_TF_IS
+    return false;
+}
+
 int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int E, double *scale,
-                       _Float16 *P, hipStream_t s)
+                       _Float16 *P, hipStream_t s, bool tfold)
 {
     size_t per_e = (size_t)g.nw * g.nw * g.B * g.B;
     hipLaunchKernelGGL(k_wscale, dim3(E), dim3(1024), 0, s, G0w, per_e, scale);
-    hipLaunchKernelGGL(k_pack, dim3(g.nw, 2 * mg.NPL, E), dim3(128), 0, s, g, mg, G0w, scale, P);
+    if (tfold)
+        hipLaunchKernelGGL(k_pack32, dim3(g.nw, E), dim3(256), 0, s, g, G0w, reinterpret_cast<float2 *>(P));
+    else
+        hipLaunchKernelGGL(k_pack, dim3(g.nw, 2 * mg.NPL, E), dim3(128), 0, s, g, mg, G0w, scale, P);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
 int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
-                          double *ldos, hipStream_t s, const double2 *tips)
+                          double *ldos, hipStream_t s, const double2 *tips, bool tfold)
 {
     const half8 *A = (const half8 *)Afrag;
     if (tips && 2 * g.maxb > g.B) return CQPI_E_ARG;
+    if (tfold) {
+        // P holds the FP32 window W32 (see k_cmix_tf)
+        if (tips) return CQPI_E_ARG;
+        const float2 *W32 = reinterpret_cast<const float2 *>(P);
+#define CQPI_TF_RUN(os, ww, bt, js) \
+        if (try_tf<os, ww, bt, js>(g, mg, W32, A, scale, sA, T, bulk, E, ldos, s)) \
+            return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+        CQPI_TF_SHAPES(CQPI_TF_RUN)
+#undef CQPI_TF_RUN
+        return CQPI_E_ARG;
+    }
     // register-resident tile fragments for the common small shapes:
     // os=4 (w=2: K steps 2/4/7 for 1/3/5 orbitals) and os=8 (w=1, 1 orbital).
     // 3 orbitals: two waves share a cell row (half the fragments each, three

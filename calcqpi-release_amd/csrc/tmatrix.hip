@@ -18,11 +18,9 @@ constexpr int kMaxB = 16;
 // registers for small MB: the row exchange of the pivoting is done by compares
 // so that every index is static); MB = kMaxB serves any B <= kMaxB
 template <int MB>
-__global__ void __launch_bounds__(64) k_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
-                          int E, double2 *__restrict__ Tout, int *__restrict__ status)
+__device__ inline void tmatrix_one(const Geom &g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
+                                   int e, double2 *__restrict__ Tout, int *__restrict__ status)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
     const int B = MB < kMaxB ? MB : g.B, NP = B * B;
     const double2 *G = G0w + ((size_t)e * g.nw * g.nw + (size_t)g.h * g.nw + g.h) * NP;
     double2 M[MB * MB], X[MB * MB];
@@ -92,23 +90,51 @@ __global__ void __launch_bounds__(64) k_tmatrix(Geom g, const double2 *__restric
     status[e] = singular;
 }
 
+template <int MB>
+__global__ void __launch_bounds__(64) k_tmatrix(Geom g, const double2 *__restrict__ G0w, const double2 *__restrict__ V,
+                          int E, double2 *__restrict__ Tout, int *__restrict__ status)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    tmatrix_one<MB>(g, G0w, V, e, Tout, status);
+}
+
 constexpr int kBulkThreads = 256;
 
 // J = false: bulk[e][pix] = sum w_o1 w_o2 Im G0(R2-R1)_{o2,o1} (double).
 // J = true: the Josephson map's complex table, column o1 + maxband (double2).
-template <bool J>
+// One workgroup per (sub-pixel, energy); the tile values w_o(x - R) of its
+// sub-pixel are staged in LDS first ([o][R], maxb (2w+1)^2 doubles).
+// MB > 0: the workgroups with blockIdx.x == os^2 compute the T-matrix of
+// their energy instead (thread 0), so that one launch serves both.
+template <bool J, int MB>
 __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__restrict__ G0w,
-                                                      const double *__restrict__ tiles, void *__restrict__ out)
+                                                      const double *__restrict__ tiles, void *__restrict__ out,
+                                                      const double2 *__restrict__ V, double2 *__restrict__ Tout,
+                                                      int *__restrict__ status)
 {
     __shared__ double red[kBulkThreads], redr[J ? kBulkThreads : 1];
+    extern __shared__ double wv[];
     const int pix = blockIdx.x, e = blockIdx.y;
     const int os = g.os, w = g.w, T = g.T, B = g.B, NP = B * B, nw = g.nw, h = g.h;
+    if constexpr (MB > 0) {
+        if (pix == os * os) {
+            if (threadIdx.x == 0) tmatrix_one<MB>(g, G0w, V, e, Tout, status);
+            return;
+        }
+    }
     const int ii = pix / os, jj = pix - ii * os;
     const int x = ii + w * os, y = jj + w * os;
     const int side = 2 * w + 1, NW = side * side;
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
     const int cofs = J ? g.maxb : 0;
+    for (int t = threadIdx.x; t < g.maxb * NW; t += blockDim.x) {
+        const int o = t / NW, a = t - o * NW;
+        const int rx = a / side - w, ry = a - (a / side) * side - w;
+        wv[t] = tiles[((size_t)o * T + (x - rx * os)) * T + (y - ry * os)];
+    }
+    __syncthreads();
     const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
     double acc = 0.0, accr = 0.0;
     for (int t = threadIdx.x; t < NW * NW; t += blockDim.x) {
@@ -116,14 +142,13 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__
         int r1x = a1 / side - w, r1y = a1 % side - w;
         int r2x = a2 / side - w, r2y = a2 % side - w;
         const double2 *G = G0 + ((size_t)(r2x - r1x + h) * nw + (r2y - r1y + h)) * NP;
-        int x1 = x - r1x * os, y1 = y - r1y * os, x2 = x - r2x * os, y2 = y - r2y * os;
         for (int sg = 0; sg < nsets; sg++) {
             int off = sg * nper;
             for (int o1 = off; o1 < off + nper; o1++) {
-                double f1 = tiles[((size_t)o1 * T + x1) * T + y1];
+                const double f1 = wv[o1 * NW + a1];
                 if (f1 == 0.0) continue;
                 for (int o2 = off; o2 < off + nper; o2++) {
-                    double f2 = tiles[((size_t)o2 * T + x2) * T + y2];
+                    const double f2 = wv[o2 * NW + a2];
                     const double2 gv = G[o2 * B + o1 + cofs];
                     acc += f1 * f2 * gv.y;
                     if (J) accr += f1 * f2 * gv.x;
@@ -149,6 +174,8 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(Geom g, const double2 *__
     }
 }
 
+size_t bulk_lds(const Geom &g) { return (size_t)g.maxb * (2 * g.w + 1) * (2 * g.w + 1) * sizeof(double); }
+
 }  // namespace
 
 namespace cqpi {
@@ -173,7 +200,8 @@ int launch_tmatrix(const Geom &g, const double2 *G0w, const double2 *V, int E, d
 int launch_bulk(const Geom &g, const double2 *G0w, const double *tiles, int E, double *bulk,
                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bulk<false>, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
+    hipLaunchKernelGGL((k_bulk<false, 0>), dim3(g.os * g.os, E), dim3(kBulkThreads), bulk_lds(g), s, g, G0w, tiles,
+                       bulk, nullptr, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -181,8 +209,34 @@ int launch_bulk_josephson(const Geom &g, const double2 *G0w, const double *tiles
                           hipStream_t s)
 {
     if (2 * g.maxb > g.B) return CQPI_E_ARG;
-    hipLaunchKernelGGL(k_bulk<true>, dim3(g.os * g.os, E), dim3(kBulkThreads), 0, s, g, G0w, tiles, bulk);
+    hipLaunchKernelGGL((k_bulk<true, 0>), dim3(g.os * g.os, E), dim3(kBulkThreads), bulk_lds(g), s, g, G0w, tiles,
+                       bulk, nullptr, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
+
+// the T-matrix and the periodic (bulk) term of E energies in one launch
+int launch_tmatrix_bulk(const Geom &g, const double2 *G0w, const double2 *V, int E, double2 *T, int *status,
+                        const double *tiles, void *bulk, bool josephson, hipStream_t s)
+{
+    if (g.B > kMaxB || (josephson && 2 * g.maxb > g.B)) return CQPI_E_ARG;
+    const dim3 grid(g.os * g.os + 1, E), blk(kBulkThreads);
+    const size_t lds = bulk_lds(g);
+#define CQPI_TB(mb)                                                                                               \
+    if (josephson)                                                                                                \
+        hipLaunchKernelGGL((k_bulk<true, mb>), grid, blk, lds, s, g, G0w, tiles, bulk, V, T, status);             \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_bulk<false, mb>), grid, blk, lds, s, g, G0w, tiles, bulk, V, T, status);
+    switch (g.B) {
+    case 1: CQPI_TB(1) break;
+    case 2: CQPI_TB(2) break;
+    case 3: CQPI_TB(3) break;
+    case 4: CQPI_TB(4) break;
+    case 5: CQPI_TB(5) break;
+    default: CQPI_TB(kMaxB) break;
+    }
+#undef CQPI_TB
+    return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+}
+
 
 }  // namespace cqpi

@@ -612,7 +612,8 @@ template <typename C, int N>
 __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
                                                                  const C *__restrict__ tw, int np, int np1, int rwin,
                                                                  double sG, double2 *__restrict__ G0w,
-                                                                 _Float16 *__restrict__ P, PlaneDiv dv)
+                                                                 _Float16 *__restrict__ P, PlaneDiv dv,
+                                                                 float2 *__restrict__ W32)
 {
     extern __shared__ __align__(16) unsigned char smem_s[];
     C *S = reinterpret_cast<C *>(smem_s);
@@ -689,6 +690,17 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom
         PlaneDiv d2 = dv;
         if (npc != np) d2.npc = udiv_magic((unsigned)npc);  // the last, shorter chunk
         write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P, d2);
+    }
+    if (W32) {
+        // the whole window row sy in FP32, W32[e][sy][sx][pair] (one contiguous run)
+        const float invf = (float)inv;
+        const unsigned long long mq = udiv_magic((unsigned)npc);
+        float2 *ow = W32 + ((size_t)e * nw + sy) * nw * NP + p0;
+        for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
+            const int sx = udiv(t, mq), q = t - sx * npc;
+            const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
+            ow[(size_t)sx * NP + q] = make_float2((float)z.x * invf, (float)z.y * invf);
+        }
     }
 }
 
@@ -842,7 +854,7 @@ int kx_pairs(int N, int NP, size_t csize)
 
 template <typename C, int N>
 int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw, double sG, double2 *G0w,
-                _Float16 *P, int rwin, hipStream_t s)
+                _Float16 *P, int rwin, hipStream_t s, float2 *W32 = nullptr)
 {
     const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
     const int np1 = wave_pairs(N, NP, sizeof(C));  // pass 1's chunks fix the F1 layout
@@ -851,7 +863,7 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
     rwin = std::min(std::max(rwin, 0), g.h);
     const PlaneDiv dv{udiv_magic((unsigned)std::max(m.Ly >> 3, 1)), udiv_magic((unsigned)np), udiv_magic((unsigned)g.B)};
     hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, rwin, sG, G0w,
-                       P, dv);
+                       P, dv, W32);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -967,17 +979,17 @@ size_t resolvent_work_bytes(const Geom &g, int E, bool fp32)
 }
 
 int launch_transform_kx32(const Geom &g, const MixGeom *mg, const float2 *F1, int E, const float2 *tw, double sG,
-                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s)
+                          double2 *G0w, _Float16 *P, int rwin, hipStream_t s, float2 *W32)
 {
     MixGeom m{};
     if (mg) m = *mg;
-    _Float16 *Pm = mg ? P : nullptr;
+    _Float16 *Pm = mg && !W32 ? P : nullptr;
     switch (g.K) {
-    case 64: return launch_kx_s<float2, 64>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
-    case 128: return launch_kx_s<float2, 128>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
-    case 256: return launch_kx_s<float2, 256>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
-    case 512: return launch_kx_s<float2, 512>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
-    case 1024: return launch_kx_s<float2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s);
+    case 64: return launch_kx_s<float2, 64>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s, W32);
+    case 128: return launch_kx_s<float2, 128>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s, W32);
+    case 256: return launch_kx_s<float2, 256>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s, W32);
+    case 512: return launch_kx_s<float2, 512>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s, W32);
+    case 1024: return launch_kx_s<float2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, rwin, s, W32);
     default: return CQPI_E_ARG;
     }
 }
