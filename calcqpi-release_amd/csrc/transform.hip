@@ -423,12 +423,13 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
 
 // waves per SIMD pass 1 is compiled for: FP32 lines 6 (80 VGPRs), but 5 for
 // the 9-wave workgroups of one chunk of every pair (two of them fill a CU
-// either way, and 96 VGPRs hold the pair products without spilling)
-template <typename C, bool ALLP>
-constexpr int res_min_waves() { return sizeof(C) == 8 ? (ALLP ? 5 : kResWaves32) : 1; }
+// either way, and 96 VGPRs hold the pair products without spilling) and 4
+// for 4 and 5 bands (their eigenvectors alone take 32 and 50 VGPRs)
+template <typename C, bool ALLP, int B>
+constexpr int res_min_waves() { return sizeof(C) == 8 ? (ALLP ? 5 : (B >= 4 ? 4 : kResWaves32)) : 1; }
 
 template <typename C, int B, int N, bool ALLP = false>
-__global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C, ALLP>())) k_res_ky_s(Geom g, const double *__restrict__ eval,
+__global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C, ALLP, B>())) k_res_ky_s(Geom g, const double *__restrict__ eval,
                                                               const double2 *__restrict__ evec,
                                                               const double *__restrict__ omegas, int E, int EB,
                                                               double eta, const C *__restrict__ tw, int np,
@@ -489,21 +490,27 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
                 for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
                 if constexpr (allp)
                     resolvent_all_pairs<C, B, NL, LP>(S, j, pp, f);
-                else
-                for (int q = 0; q < npc; q++) {
-                    const int pr = p0 + q, r = pr / B, c = pr - r * B;
-                    C acc = to_c<C>(make_double2(0.0, 0.0));
+                else {
+                    // the chunk's pairs p0 .. p0 + npc - 1: every (r, c) unrolled,
+                    // so that the eigenvector components are static registers,
+                    // with a test on the chunk (uniform over the workgroup);
+                    // G_rc = sum_n (f_n v_rn) conj(v_cn)
 #pragma unroll
-                    for (int n = 0; n < B; n++) {
-                        C vr = vv_r[n], vc = vv_r[n];
+                    for (int r = 0; r < B; r++) {
+                        if (r * B + B <= p0 || r * B >= p0 + npc) continue;
+                        C fv[B];
 #pragma unroll
-                        for (int x = 0; x < B; x++) {
-                            if (x == r) vr = vv_r[x * B + n];
-                            if (x == c) vc = vv_r[x * B + n];
+                        for (int n = 0; n < B; n++) fv[n] = cmul(f[n], vv_r[r * B + n]);
+#pragma unroll
+                        for (int c = 0; c < B; c++) {
+                            const int pr = r * B + c;
+                            if (pr < p0 || pr >= p0 + npc) continue;
+                            C acc = cmulc(fv[0], vv_r[c * B]);
+#pragma unroll
+                            for (int n = 1; n < B; n++) acc = cadd(acc, cmulc(fv[n], vv_r[c * B + n]));
+                            S[(size_t)(pr - p0) * NL + cqpi::pidx<C, LP>(j)] = acc;
                         }
-                        acc = cadd(acc, cmul(f[n], cmulc(vr, vc)));
                     }
-                    S[(size_t)q * NL + cqpi::pidx<C, LP>(j)] = acc;
                 }
             }
         }

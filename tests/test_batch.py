@@ -56,6 +56,28 @@ def test_batch_graph_matches_eager_and_oracle(gpu, precision):
             assert mx < tol, (i, e, mx, rms)
 
 
+@pytest.mark.skip(reason="a captured per-instance fork/join (energy chunks split over an "
+                         "instance's two streams inside the batch graph) ended in a host "
+                         "segmentation fault in its first run, also for this 3-instance batch "
+                         "and with non-blocking extra streams; batch instances run on one "
+                         "stream until its cause is shown")
+def test_batch_with_split_energy_chunks(gpu):
+    # 8 or more energies: each instance's chunk would be split over its two
+    # streams, joined to the batch branch inside the captured graph
+    spec = [(3, 32, 8, 12, 2, 4), (1, 64, 16, 8, 1, 8), (5, 16, 6, 9, 1, 2)]
+    wls = [synth.square_norb(b, K=K, n=n, E=E, seed=7300 + i, w=w, os_=os_, eta=0.02)
+           for i, (b, K, n, E, w, os_) in enumerate(spec)]
+    print("case: batch of %d instances with split chunks" % len(wls), flush=True)
+    for precision in ("fp64", "mixed"):
+        eager = [_eager(wl, precision) for wl in wls]
+        with cqpi.Batch(wls, precision) as b:
+            for rep in range(3):
+                b.run()
+                for i in range(len(wls)):
+                    assert np.array_equal(b.maps(i), eager[i]), (precision, rep, i)
+                _eager(wls[1], precision)
+
+
 def test_batch_reports_singular_instance(gpu):
     wl = synth.square_1orb(K=16, n=4, E=1)
     ev, V = oracle.calchamiltonian(wl.hops, 1, 0.0, wl.K)
