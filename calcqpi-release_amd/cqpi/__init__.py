@@ -63,6 +63,8 @@ SIGNATURES = {
     "cqpi_wannierldoslist": (C.c_int, [_vp, _dp, C.POINTER(_vp), _sz]),
     "cqpi_make_threshold_list": (C.c_int, [_dp, _sz, _sz, _sz, C.c_double, _sz, C.c_int, _vp, _sz,
                                            C.POINTER(_sz), C.POINTER(_sz)]),
+    "cqpi_make_threshold_list_spinarr": (C.c_int, [_dp, _sz, _sz, _sz, C.c_double, C.POINTER(C.c_int), _sz, _vp,
+                                                   _sz, C.POINTER(_sz), C.POINTER(_sz)]),
     "cqpi_retrieve": (C.c_int, [_vp, _dp]),
     "cqpi_printinfo": (C.c_int, [_vp, C.c_char_p, _sz]),
     "cqpi_destroy": (None, [_vp]),
@@ -521,6 +523,25 @@ def make_threshold_list(tiles, window, oversamp, threshold, maxband=0, spin=Fals
     _check(lib().cqpi_make_threshold_list(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
                                           flags.ctypes.data, total.value, counts.ctypes.data_as(C.POINTER(_sz)),
                                           C.byref(total)))
+    return flags[:total.value], counts
+
+
+def make_threshold_list_spinarr(tiles, window, oversamp, threshold, spinarr):
+    """cqpi_make_threshold_list_spinarr (src/qpi.cc:213-253): the list over
+    orbital pairs with equal nonzero spin labels; (flags FLAG_DTYPE, counts
+    per window pair in IDX4CC order)."""
+    tiles = np.ascontiguousarray(tiles, np.float64)
+    norb = tiles.shape[0]
+    sp = np.ascontiguousarray(spinarr, np.int32)
+    spp = sp.ctypes.data_as(C.POINTER(C.c_int))
+    counts = np.zeros((2 * window + 1) ** 4, np.uintp)
+    total = _sz(0)
+    _check(lib().cqpi_make_threshold_list_spinarr(_d(tiles), norb, window, oversamp, float(threshold), spp, len(sp),
+                                                  None, 0, counts.ctypes.data_as(C.POINTER(_sz)), C.byref(total)))
+    flags = np.zeros(max(total.value, 1), FLAG_DTYPE)
+    _check(lib().cqpi_make_threshold_list_spinarr(_d(tiles), norb, window, oversamp, float(threshold), spp, len(sp),
+                                                  flags.ctypes.data, total.value,
+                                                  counts.ctypes.data_as(C.POINTER(_sz)), C.byref(total)))
     return flags[:total.value], counts
 
 

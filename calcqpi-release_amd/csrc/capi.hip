@@ -1114,6 +1114,54 @@ int cqpi_make_threshold_list(const double *wf, size_t norb, size_t window, size_
     return 0;
 }
 
+int cqpi_make_threshold_list_spinarr(const double *wf, size_t norb, size_t window, size_t oversamp,
+                                     double threshold, const int *spinarr, size_t nspinarr, cqpi_flagentry *out,
+                                     size_t cap, size_t *counts, size_t *total)
+{
+    g_err.clear();
+    if (!wf || !counts || !total || (!spinarr && nspinarr)) return fail(CQPI_E_ARG, "NULL argument");
+    // the spin label of every tile: spinarr resized to norb, missing labels 0
+    // (no entries), as src/qpi.cc:217
+    std::vector<int> spin(norb, 0);
+    for (size_t o = 0; o < std::min(norb, nspinarr); o++) spin[o] = spinarr[o];
+    const size_t T = (2 * window + 1) * oversamp, wins = 2 * window + 1;
+    auto wv = [&](size_t o, size_t x, size_t y) { return wf[(o * T + x) * T + y]; };
+    double maxval = 0.0;
+    for (size_t l = 0; l < norb; l++)
+        for (size_t i = 0; i < T; i++)
+            for (size_t j = 0; j < T; j++) maxval = std::max(maxval, wv(l, i, j) * wv(l, i, j));
+    const double cut = threshold * maxval;
+    size_t cnt = 0;
+    for (size_t xn1 = 0; xn1 < wins; xn1++)
+        for (size_t yn1 = 0; yn1 < wins; yn1++)
+            for (size_t xn2 = 0; xn2 < wins; xn2++)
+                for (size_t yn2 = 0; yn2 < wins; yn2++) {
+                    const size_t first = cnt;
+                    for (size_t o1 = 0; o1 < norb; o1++) {
+                        if (!spin[o1]) continue;
+                        for (size_t o2 = 0; o2 < norb; o2++) {
+                            if (spin[o2] != spin[o1]) continue;
+                            for (size_t i = 0; i < oversamp; i++)
+                                for (size_t j = 0; j < oversamp; j++) {
+                                    const size_t x = i + (wins - 1) * oversamp, y = j + (wins - 1) * oversamp;
+                                    const double f = wv(o1, x - xn1 * oversamp, y - yn1 * oversamp) *
+                                                     wv(o2, x - xn2 * oversamp, y - yn2 * oversamp);
+                                    if (std::fabs(f) > cut) {
+                                        if (out && cnt < cap)
+                                            out[cnt] = cqpi_flagentry{(uint32_t)i, (uint32_t)j, (uint32_t)o1,
+                                                                      (uint32_t)o2, f};
+                                        cnt++;
+                                    }
+                                }
+                        }
+                    }
+                    counts[((xn1 * wins + yn1) * wins + xn2) * wins + yn2] = cnt - first;
+                }
+    *total = cnt;
+    if (out && cnt > cap) return fail(CQPI_E_ARG, "cap is smaller than the list; *total holds its length");
+    return 0;
+}
+
 int cqpi_set_precision(cqpi_ctx *c, int precision)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");

@@ -38,7 +38,8 @@
 // Extension key: precision=fp64|fp32|mixed; (contraction precision, default fp64).
 //   * orbitalfiles=("a.xsf",...) with zheight: Wannier90/XSF orbital files
 //     (src/calcqpi.cc:576-601).
-// Other output modes and spinarr are reported as unsupported.
+//   * spinarr: the threshold list's spin labels (src/calcqpi.cc:626-628).
+// Other output modes are reported as unsupported.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -391,9 +392,13 @@ int main(int argc, char **argv)
         }
     }
     if (lf.probe("threshold")) threshold = lf.num("threshold");
+    // spin label per orbital for the threshold list (src/calcqpi.cc:626-628;
+    // read as the reference's getintvector, atoi per entry)
+    bool have_spinarr = false;
+    vector<int> spinarr;
     if (lf.probe("spinarr")) {
-        std::cerr << "spinarr is not supported by this build.\n";
-        return 1;
+        have_spinarr = true;
+        for (auto &v : lf.list("spinarr")) spinarr.push_back(std::atoi(v.c_str()));
     }
     if (lf.probe("deltat")) deltat = lf.num("deltat");
     if (lf.probe("etat")) etat = lf.num("etat");
@@ -713,7 +718,14 @@ int main(int argc, char **argv)
             std::cout << (i ? ", " : "") << "[" << vscat[i].real() << ", " << vscat[i].imag() << "]";
         double tsum = 0;
         for (double v : tiles) tsum += v;
-        std::cout << "], \"tile_sum\": " << tsum << "}\n";
+        std::cout << "], \"spinarr\": ";
+        if (!have_spinarr) std::cout << "null";
+        else {
+            std::cout << "[";
+            for (size_t i = 0; i < spinarr.size(); i++) std::cout << (i ? ", " : "") << spinarr[i];
+            std::cout << "]";
+        }
+        std::cout << ", \"tile_sum\": " << tsum << "}\n";
         cqpi_free(hops);
         cqpi_free(shops);
         lf.warn_unused(std::cerr);
@@ -746,15 +758,20 @@ int main(int argc, char **argv)
         const size_t wins = (size_t)(2 * window + 1);
         vector<size_t> counts(wins * wins * wins * wins);
         size_t total = 0;
-        if (check(cqpi_make_threshold_list(tiles.data(), maxbands, (size_t)window, (size_t)oversamp, threshold,
-                                           maxbands, spin, nullptr, 0, counts.data(), &total),
-                  "cqpi_make_threshold_list"))
-            return 1;
+        // with spinarr: the spin-label rule over every orbital (src/qpi.cc:213-253)
+        auto build = [&](cqpi_flagentry *out, size_t cap) {
+            if (have_spinarr)
+                return check(cqpi_make_threshold_list_spinarr(tiles.data(), maxbands, (size_t)window,
+                                                              (size_t)oversamp, threshold, spinarr.data(),
+                                                              spinarr.size(), out, cap, counts.data(), &total),
+                             "cqpi_make_threshold_list_spinarr");
+            return check(cqpi_make_threshold_list(tiles.data(), maxbands, (size_t)window, (size_t)oversamp,
+                                                  threshold, maxbands, spin, out, cap, counts.data(), &total),
+                         "cqpi_make_threshold_list");
+        };
+        if (build(nullptr, 0)) return 1;
         vector<cqpi_flagentry> flags(std::max<size_t>(total, 1));
-        if (check(cqpi_make_threshold_list(tiles.data(), maxbands, (size_t)window, (size_t)oversamp, threshold,
-                                           maxbands, spin, flags.data(), total, counts.data(), &total),
-                  "cqpi_make_threshold_list"))
-            return 1;
+        if (build(flags.data(), total)) return 1;
         const size_t entries = wins * wins * wins * wins * (size_t)(oversamp * oversamp) * maxbands * maxbands;
         *log << "Threshold list has " << total << "/" << entries << " entries (" << 100.0 * total / entries
              << "%), requiring " << total * sizeof(cqpi_flagentry) / 1024.0 / 1024.0 / 1024.0 << "GB of memory.\n";

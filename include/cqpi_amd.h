@@ -181,6 +181,17 @@ int cqpi_make_threshold_list(const double *wf, size_t norb, size_t window, size_
                              size_t maxband, int spin, cqpi_flagentry *out, size_t cap, size_t *counts,
                              size_t *total);
 
+/* The driver's list builder with a spin label per orbital, mkthresholdlist(
+ * ..., spinarr) (src/qpi.cc:213-253; called at src/calcqpi.cc:626-628 when
+ * the input file has spinarr=): entries over every orbital pair o1, o2 < norb
+ * whose labels are equal and nonzero, in the reference's order.  spinarr has
+ * nspinarr labels; it is resized to norb (missing labels are 0, and an
+ * orbital labelled 0 has no entries).  Other arguments and results as
+ * cqpi_make_threshold_list. */
+int cqpi_make_threshold_list_spinarr(const double *wf, size_t norb, size_t window, size_t oversamp,
+                                     double threshold, const int *spinarr, size_t nspinarr, cqpi_flagentry *out,
+                                     size_t cap, size_t *counts, size_t *total);
+
 /* Replaces HipQPI::HipQPI(kpoints, n, bands, maxbands, spin, pos, prearr)
  * (src/qpihipkernel.h:77, src/qpihipkernel.cc:498-533): a context for the
  * spectral-function maps, n x n (n <= kpoints; the driver sets n = kpoints

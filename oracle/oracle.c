@@ -621,6 +621,55 @@ size_t or_mkthresholdlist(const double *wf, size_t norb, size_t window, size_t o
 }
 
 /* ------------------------------------------------------------------------- */
+/* Threshold list with spin labels: mkthresholdlist(..., spinarr)            */
+/* src/qpi.cc:213-253.  spinarr is resized to norb (src/qpi.cc:217: missing  */
+/* labels 0); entries for every o1 with a nonzero label and every o2 with the */
+/* same label.  Returns the total count.                                      */
+/* ------------------------------------------------------------------------- */
+size_t or_mkthresholdlist_spinarr(const double *wf, size_t norb, size_t window, size_t oversamp, double threshold,
+                                  const int *spinarr, size_t nspinarr, or_flag *out, size_t cap, size_t *counts)
+{
+    size_t TS = (2 * window + 1) * oversamp, wins = 2 * window + 1, count = 0;
+    double maxval = 0.0;
+    int *sp = (int *)calloc(norb ? norb : 1, sizeof(int));
+    for (size_t o = 0; o < norb && o < nspinarr; o++) sp[o] = spinarr[o];
+#define WF(o, x, y) wf[((size_t)(o) * TS + (size_t)(x)) * TS + (size_t)(y)]
+    for (size_t l = 0; l < norb; l++)
+        for (size_t i = 0; i < TS; i++)
+            for (size_t j = 0; j < TS; j++) {
+                double v = WF(l, i, j) * WF(l, i, j);
+                if (v > maxval) maxval = v;
+            }
+    for (size_t xn1 = 0; xn1 < wins; xn1++)
+        for (size_t yn1 = 0; yn1 < wins; yn1++)
+            for (size_t xn2 = 0; xn2 < wins; xn2++)
+                for (size_t yn2 = 0; yn2 < wins; yn2++) {
+                    size_t before = count;
+                    for (size_t o1 = 0; o1 < norb; o1++)
+                        if (sp[o1])
+                            for (size_t o2 = 0; o2 < norb; o2++)
+                                if (sp[o1] == sp[o2])
+                                    for (size_t i = 0; i < oversamp; i++)
+                                        for (size_t j = 0; j < oversamp; j++) {
+                                            size_t x = i + (wins - 1) * oversamp, y = j + (wins - 1) * oversamp;
+                                            double f = WF(o1, x - xn1 * oversamp, y - yn1 * oversamp) *
+                                                       WF(o2, x - xn2 * oversamp, y - yn2 * oversamp);
+                                            if (fabs(f) > threshold * maxval) {
+                                                if (out && count < cap) {
+                                                    or_flag e = {(uint32_t)i, (uint32_t)j, (uint32_t)o1, (uint32_t)o2, f};
+                                                    out[count] = e;
+                                                }
+                                                count++;
+                                            }
+                                        }
+                    if (counts) counts[((xn1 * wins + yn1) * wins + xn2) * wins + yn2] = count - before;
+                }
+#undef WF
+    free(sp);
+    return count;
+}
+
+/* ------------------------------------------------------------------------- */
 /* LDOS from a threshold list: tmatrix::calcwannierldos(oversamp, window,      */
 /* flaglist) src/qpi.cc:1088-1135 (GPU kernel src/qpihipkernel.cc:273-315).   */
 /* flags/counts as produced by or_mkthresholdlist (any list of that layout).  */

@@ -50,6 +50,9 @@ def lib():
         L.or_mkthresholdlist.argtypes = [_dp, sz, sz, sz, C.c_double, sz, C.c_int, C.c_void_p, sz,
                                          C.POINTER(sz)]
         L.or_mkthresholdlist.restype = sz
+        L.or_mkthresholdlist_spinarr.argtypes = [_dp, sz, sz, sz, C.c_double, _ip, sz, C.c_void_p, sz,
+                                                 C.POINTER(sz)]
+        L.or_mkthresholdlist_spinarr.restype = sz
         L.or_wannierldoslist.argtypes = [sz, sz, sz, sz, sz, _dp, _dp, C.c_void_p, C.POINTER(sz), _dp]
         L.or_spf.argtypes = [sz, sz, sz, sz, _dp, _dp]
         L.or_surfacegreen.argtypes = [C.c_void_p, sz, C.c_void_p, sz, sz, C.c_double, sz, sz, C.c_double,
@@ -215,6 +218,23 @@ def mkthresholdlist(tiles, window, oversamp, threshold, maxband=0, spin=False):
     out = np.zeros(max(total, 1), FLAG_DTYPE)
     lib().or_mkthresholdlist(_d(tiles), norb, window, oversamp, float(threshold), maxband, int(spin),
                              out.ctypes.data, total, counts.ctypes.data_as(C.POINTER(C.c_size_t)))
+    return out[:total], counts
+
+
+def mkthresholdlist_spinarr(tiles, window, oversamp, threshold, spinarr):
+    """src/qpi.cc:213-253 (spin label per orbital): (entries FLAG_DTYPE,
+    counts per window pair in IDX4CC order)."""
+    tiles = _c(tiles, np.float64)
+    norb = tiles.shape[0]
+    sp = np.ascontiguousarray(spinarr, np.int32)
+    wins = 2 * window + 1
+    counts = np.zeros(wins ** 4, np.uintp)
+    total = lib().or_mkthresholdlist_spinarr(_d(tiles), norb, window, oversamp, float(threshold),
+                                             sp.ctypes.data_as(_ip), len(sp), None, 0,
+                                             counts.ctypes.data_as(C.POINTER(C.c_size_t)))
+    out = np.zeros(max(total, 1), FLAG_DTYPE)
+    lib().or_mkthresholdlist_spinarr(_d(tiles), norb, window, oversamp, float(threshold), sp.ctypes.data_as(_ip),
+                                     len(sp), out.ctypes.data, total, counts.ctypes.data_as(C.POINTER(C.c_size_t)))
     return out[:total], counts
 
 

@@ -114,6 +114,11 @@ def test_driver_josephson_and_threshold_keys(tmp_path):
     p = synth.write_inputs(str(tmp_path / "t"), wl3, extra={"threshold": "0.05", "etat": "0.003"})
     cfg, _ = _print_config(p)
     assert cfg["output"] == "wannier" and cfg["threshold"] == 0.05 and cfg["etat"] == 0.003
+    assert cfg["spinarr"] is None
+    # spinarr: integers as the reference's getintvector reads them (atoi)
+    p = synth.write_inputs(str(tmp_path / "s"), wl3, extra={"threshold": "0.05", "spinarr": "(1,2,1.7)"})
+    cfg, err = _print_config(p)
+    assert cfg["spinarr"] == [1, 2, 1] and "Unused" not in err
 
 
 def test_driver_josephson_needs_scmodel(tmp_path):
@@ -155,6 +160,28 @@ def test_driver_threshold_matches_oracle(tmp_path, gpu):
     data, _ = synth.read_idl(str(tmp_path / "qpi.idl"))
     hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
     flags, counts = oracle.mkthresholdlist(wl.tiles, wl.window, wl.oversamp, 0.03)
+    ev, vec = oracle.calchamiltonian(hops, b, wl.efermi, wl.K)
+    for e, om in enumerate(wl.energies):
+        gR = oracle.realspace(oracle.greens_k(ev, vec, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, wl.window, wl.oversamp)
+        got = data[:, :, e].T.ravel()
+        assert rm.rel_errors(got, ref.astype(np.float32).astype(np.float64))[0] < 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_spinarr_threshold_matches_oracle(tmp_path, gpu):
+    """spinarr= (src/calcqpi.cc:626-628): the list of the spin-label rule,
+    then the list map, against the oracle."""
+    wl = synth.square_3orb(K=32, n=8, E=2)
+    p = synth.write_inputs(str(tmp_path), wl, extra={"threshold": "0.03", "spinarr": "(1,2,1)"})
+    print("case: driver run, threshold list 0.03 with spinarr (1,2,1), K=32, n=8, E=2", flush=True)
+    r = subprocess.run([DRIVER, p], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Threshold list has" in r.stdout
+    data, _ = synth.read_idl(str(tmp_path / "qpi.idl"))
+    hops, b, _ = oracle.read_hr(str(tmp_path / "model.dat"))
+    flags, counts = oracle.mkthresholdlist_spinarr(wl.tiles, wl.window, wl.oversamp, 0.03, [1, 2, 1])
     ev, vec = oracle.calchamiltonian(hops, b, wl.efermi, wl.K)
     for e, om in enumerate(wl.energies):
         gR = oracle.realspace(oracle.greens_k(ev, vec, om, wl.eta))

@@ -175,6 +175,40 @@ def test_threshold_list_builder_matches_oracle(threshold, spin):
     assert len(got) > 0
 
 
+@pytest.mark.parametrize("spinarr", [[1, 1, 2, 2], [2, 0, 2, 1], [1, 2], [3, 3, 3, 3, 3, 9]])
+def test_threshold_list_spinarr_builder_matches_oracle(spinarr):
+    """cqpi_make_threshold_list_spinarr builds the same entries, in the same
+    order, as the oracle's restatement of mkthresholdlist(..., spinarr): labels
+    resized to the orbital count, zero labels without entries."""
+    tiles = synth.square_norb(4, K=8, n=4, E=1, seed=73, w=1, os_=3).tiles
+    got, gc = cqpi.make_threshold_list_spinarr(tiles, 1, 3, 0.01, spinarr)
+    ref, rc = oracle.mkthresholdlist_spinarr(tiles, 1, 3, 0.01, spinarr)
+    assert np.array_equal(gc, rc)
+    assert np.array_equal(got, ref)
+    assert len(got) > 0
+    # the selection rule: only equal nonzero labels pair up
+    lab = (list(spinarr) + [0] * 4)[:4]
+    assert all(lab[e["o1"]] != 0 and lab[e["o1"]] == lab[e["o2"]] for e in got)
+
+
+def test_threshold_list_spinarr_pins():
+    """Pins of the spin-label rule against the other list rule: one common
+    label is the plain list over every orbital (same entries, same order);
+    two halves labelled 1 and 2 hold the spin list's entries (its order
+    interleaves the halves); all labels 0 give no entry."""
+    tiles = synth.square_norb(4, K=8, n=4, E=1, seed=74, w=1, os_=2).tiles
+    one, oc = oracle.mkthresholdlist_spinarr(tiles, 1, 2, 0.02, [1, 1, 1, 1])
+    plain, pc = oracle.mkthresholdlist(tiles, 1, 2, 0.02)
+    assert np.array_equal(one, plain) and np.array_equal(oc, pc)
+    halves, hc = oracle.mkthresholdlist_spinarr(tiles, 1, 2, 0.02, [1, 1, 2, 2])
+    spin, sc = oracle.mkthresholdlist(tiles, 1, 2, 0.02, spin=True)
+    assert np.array_equal(hc, sc)
+    key = lambda a: sorted((int(e["i"]), int(e["j"]), int(e["o1"]), int(e["o2"]), float(e["factor"])) for e in a)
+    assert key(halves) == key(spin)
+    none, nc = oracle.mkthresholdlist_spinarr(tiles, 1, 2, 0.02, [0, 0, 0, 0])
+    assert len(none) == 0 and not nc.any()
+
+
 def test_threshold_list_builder_rejects_short_buffer():
     wl = synth.square_norb(3, K=8, n=4, E=1, seed=71, w=1, os_=2)
     counts = np.zeros(81, np.uintp)

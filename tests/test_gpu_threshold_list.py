@@ -75,6 +75,21 @@ def test_dropin_list_spin_and_duplicates(gpu):
     assert rm.rel_errors(got, ref)[0] < TOL64
 
 
+def test_dropin_spinarr_list(gpu):
+    """A list built by the spin-label rule (src/qpi.cc:213-253, driver key
+    spinarr): orbitals 0 and 2 share label 1, orbital 1 has label 2, orbital
+    3 none; the map against the oracle's list contraction of the same list."""
+    wl = synth.square_norb(4, K=32, n=8, E=1, seed=88, w=1, os_=2, eta=0.01)
+    gR, T = _g0(wl, 0.03)
+    flags, counts = cqpi.make_threshold_list_spinarr(wl.tiles, 1, 2, 0.01, [1, 2, 1, 0])
+    print("case: spinarr list entries=%d" % len(flags), flush=True)
+    assert len(flags) > 0 and not (flags["o1"] == 3).any()
+    ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, 1, 2)
+    got = _dropin(wl, flags, counts, gR.copy(), T)
+    assert np.abs(ref).max() > 0
+    assert rm.rel_errors(got, ref)[0] < TOL64
+
+
 def test_full_chain_list(gpu):
     """cqpi_run on a list context: the device chain up to T, then the list map."""
     wl = synth.square_norb(3, K=32, n=8, E=3, seed=87, w=2, os_=4, eta=0.01, elo=-0.1, ehi=0.1)
