@@ -184,8 +184,8 @@ struct cqpi_ctx {
     size_t c_dos = 0, c_dosp = 0, c_domega = 0;
     // threshold list (cqpi_create_list): per-pair factor table and pair index
     bool list_mode = false;
-    double *d_lcoef = nullptr;
-    int *d_lr1off = nullptr, *d_lpair = nullptr;
+    double *d_lcoef = nullptr, *d_lccoef = nullptr;
+    int *d_lr1off = nullptr, *d_lpair = nullptr, *d_lcoff = nullptr, *d_lco2 = nullptr;
     int lnact = 0;
     // surface / bulk Green's function of a layered model (cqpi_set_surface):
     // 0 off (G0(k) from the eigen data), 1 surface layer, 2 bulk.  Hopping
@@ -276,7 +276,8 @@ void release(cqpi_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_tiles, c->d_groups, c->d_tmat, c->d_eval, c->d_evec, c->d_V, c->d_tw, c->d_tw32,
-                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_lcoef, c->d_lr1off,
+                    c->d_F1, c->d_gkw, c->d_G0w, c->d_bulk, c->d_T, c->d_tips, c->d_lcoef, c->d_lccoef,
+                    c->d_lcoff, c->d_lco2, c->d_lr1off,
                     c->d_lpair, c->d_spos, c->d_g0k, c->d_dos, c->d_dosp, c->d_domega,
                     c->d_sgroups, c->d_stmat, c->d_sstat, c->d_status, c->d_omega, c->d_ldos,
                     c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
@@ -391,7 +392,8 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
             Tf = c->d_Tf + eo * NP;
         }
         int st = stage_begin(c, 4);
-        int r = launch_contract_list(g, G0w, T, c->d_lcoef, c->d_lr1off, c->d_lpair, c->lnact, E, bulk, ldos, s,
+        int r = launch_contract_list(g, G0w, T, c->d_lcoef, c->d_lcoff, c->d_lco2, c->d_lccoef, c->d_lr1off,
+                                     c->d_lpair, c->lnact, E, bulk, ldos, s,
                                      G0f, Tf);
         stage_end(c, st);
         if (r) return fail(r, "threshold-list contraction launch failed");
@@ -1042,14 +1044,41 @@ int cqpi_create_list(size_t wanniern, size_t kpoints, size_t n, size_t window, s
         }
     }
     for (size_t r = 0; r < NR; r++) r1off[r + 1] += r1off[r];
+    // the orbital pairs present per active pair: combos ordered by (k, o1, o2),
+    // coff[k B + o1] the first of (k, o1); ccoef their factors per sub-pixel
+    const size_t nact = lpair.size();
+    std::vector<int> coff(nact * bands + 1, 0), co2;
+    std::vector<double> ccoef;
+    for (size_t k = 0; k < nact; k++)
+        for (size_t o1 = 0; o1 < bands; o1++) {
+            coff[k * bands + o1] = (int)co2.size();
+            for (size_t o2 = 0; o2 < bands; o2++) {
+                const double *cp = &coef[(k * NP + o2 * bands + o1) * npix];
+                if (std::none_of(cp, cp + npix, [](double v) { return v != 0.0; })) continue;
+                co2.push_back((int)o2);
+                ccoef.insert(ccoef.end(), cp, cp + npix);
+            }
+        }
+    coff[nact * bands] = (int)co2.size();
     const size_t T = side * oversamp;
     std::vector<double> zero_tiles(bands * T * T, 0.0);  // the list carries the factors
     RETCHK(cqpi_create(wanniern, kpoints, n, window, oversamp, bands, bands, 0, zero_tiles.data(), out));
     cqpi_ctx *c = *out;
     c->list_mode = true;
     c->lnact = (int)lpair.size();
-    size_t cap_coef = 0, cap_off = 0, cap_pair = 0;
+    size_t cap_coef = 0, cap_off = 0, cap_pair = 0, cap_cc = 0, cap_co = 0, cap_c2 = 0;
     int r = dev_grow(c->d_lcoef, cap_coef, std::max<size_t>(coef.size(), 1), c->gpumem);
+    if (!r) r = dev_grow(c->d_lccoef, cap_cc, std::max<size_t>(ccoef.size(), 1), c->gpumem);
+    if (!r) r = dev_grow(c->d_lcoff, cap_co, coff.size(), c->gpumem);
+    if (!r) r = dev_grow(c->d_lco2, cap_c2, std::max<size_t>(co2.size(), 1), c->gpumem);
+    if (!r && !ccoef.empty() &&
+        hipMemcpy(c->d_lccoef, ccoef.data(), ccoef.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
+    if (!r && hipMemcpy(c->d_lcoff, coff.data(), coff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
+    if (!r && !co2.empty() &&
+        hipMemcpy(c->d_lco2, co2.data(), co2.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail(CQPI_E_HIP, "threshold-list upload failed");
     if (!r) r = dev_grow(c->d_lr1off, cap_off, NR + 1, c->gpumem);
     if (!r) r = dev_grow(c->d_lpair, cap_pair, std::max<size_t>(lpair.size(), 1), c->gpumem);
     if (!r && !coef.empty() &&

@@ -368,9 +368,11 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
 // Threshold-list contraction (src/qpihipkernel.cc:273-315, CPU src/qpi.cc:1088-1135):
 //   rho(pos, x) = - sum_(R1,R2) sum_(o1,o2) F[R1,R2,o2,o1,x]
 //                   Im[ (G0(pos+R2) T G0(-pos-R1))_{o2,o1} + G0(R2-R1)_{o2,o1} ]
-// with F the list's factors summed into a dense table per active window pair
-// (coef[k][o2][o1][pix], pair k = (r1, r2) = lpair[k], pairs of one r1
-// contiguous from r1off[r1]).  The list selects arbitrary (R1, o1, R2, o2)
+// with F the list's factors summed per active window pair (pair k = (r1, r2)
+// = lpair[k], pairs of one r1 contiguous from r1off[r1]) and per orbital pair
+// present: combos coff[k B + o1] .. coff[k B + o1 + 1] - 1 hold o2 = co2[cb]
+// and the factors of every sub-pixel, ccoef[cb][pix] (the periodic part
+// reads the dense table coef[k][o2][o1][pix]).  The list selects arbitrary (R1, o1, R2, o2)
 // pairs, so the sum does not factorise as the dense map does: per pair the
 // lane forms C = G0(pos+R2) Y with Y = T G0(-pos-R1) kept from its R1.
 // Lanes are the 8 x 8 cells, the waves split the R1 values, kListPx
@@ -383,7 +385,9 @@ template <int B>
 __global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const double2 *__restrict__ G0w,
                                                              const double2 *__restrict__ Tm,
                                                              const double *__restrict__ bulk,
-                                                             const double *__restrict__ coef,
+                                                             const int *__restrict__ coff,
+                                                             const int *__restrict__ co2,
+                                                             const double *__restrict__ ccoef,
                                                              const int *__restrict__ r1off,
                                                              const int *__restrict__ lpair, double *__restrict__ ldos)
 {
@@ -426,19 +430,19 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const dou
                 const int r2 = lpair[k] - r1 * NR;
                 const int rx2 = r2 / side - w, ry2 = r2 % side - w;
                 const double2 *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
-                const double *cf = coef + (size_t)k * NP * npix + pc;
+                // only the (o2, o1) present in the pair's entries
 #pragma unroll
-                for (int o2 = 0; o2 < B; o2++) {
-                    double2 arow[B];
-#pragma unroll
-                    for (int q = 0; q < B; q++) arow[q] = GA[o2 * B + q];
-#pragma unroll
-                    for (int o1 = 0; o1 < B; o1++) {
+                for (int o1 = 0; o1 < B; o1++) {
+                    const int c1 = coff[k * B + o1 + 1];
+                    for (int cb = coff[k * B + o1]; cb < c1; cb++) {
+                        const int o2 = co2[cb];
                         double cim = 0.0;  // Im (G0(pos+R2) Y)_{o2,o1}
 #pragma unroll
-                        for (int q = 0; q < B; q++)
-                            cim += arow[q].x * Y[q * B + o1].y + arow[q].y * Y[q * B + o1].x;
-                        const double *cp = cf + (size_t)(o2 * B + o1) * npix;
+                        for (int q = 0; q < B; q++) {
+                            const double2 a = GA[o2 * B + q];
+                            cim += a.x * Y[q * B + o1].y + a.y * Y[q * B + o1].x;
+                        }
+                        const double *cp = ccoef + (size_t)cb * npix + pc;
 #pragma unroll
                         for (int q = 0; q < kListPx; q++)
                             if (pc + q < npix) val[q] += cp[q] * cim;
@@ -476,7 +480,9 @@ template <typename C>
 __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C *__restrict__ G0w,
                                                                const C *__restrict__ Tm,
                                                                const double *__restrict__ bulk,
-                                                               const double *__restrict__ coef,
+                                                               const int *__restrict__ coff,
+                                                               const int *__restrict__ co2,
+                                                               const double *__restrict__ ccoef,
                                                                const int *__restrict__ r1off,
                                                                const int *__restrict__ lpair, double *__restrict__ ldos)
 {
@@ -522,11 +528,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
                     Y[q].y = yy;
                 }
                 for (int k = k0; k < k1; k++) {
+                    const int c1 = coff[k * B + o1 + 1];
+                    if (coff[k * B + o1] == c1) continue;
                     const int r2 = lpair[k] - r1 * NR;
                     const int rx2 = r2 / side - w, ry2 = r2 % side - w;
                     const C *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
-                    const double *cf = coef + (size_t)k * NP * npix + pc;
-                    for (int o2 = 0; o2 < B; o2++) {
+                    // only the o2 present with this o1 in the pair's entries
+                    for (int cb = coff[k * B + o1]; cb < c1; cb++) {
+                        const int o2 = co2[cb];
                         Re cim = 0;  // Im (G0(pos+R2) Y)_{o2,o1}
 #pragma unroll
                         for (int q = 0; q < kListMaxB; q++)
@@ -534,7 +543,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
                                 const C a = GA[o2 * B + q];
                                 cim += a.x * Y[q].y + a.y * Y[q].x;
                             }
-                        const double *cp = cf + (size_t)(o2 * B + o1) * npix;
+                        const double *cp = ccoef + (size_t)cb * npix + pc;
 #pragma unroll
                         for (int q = 0; q < kListPx; q++)
                             if (pc + q < npix) val[q] += cp[q] * (double)cim;
@@ -610,6 +619,7 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
 }
 
 int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
+                         const int *coff, const int *co2, const double *ccoef,
                          const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
                          hipStream_t s, float2 *G0f, float2 *Tf)
 {
@@ -624,19 +634,19 @@ int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, co
         hipLaunchKernelGGL(k_narrow, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, G0w, na, T, nt, G0f, Tf);
         if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
         hipLaunchKernelGGL(k_contract_list_g<float2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g,
-                           (const float2 *)G0f, (const float2 *)Tf, bulk, coef, r1off, lpair, ldos);
+                           (const float2 *)G0f, (const float2 *)Tf, bulk, coff, co2, ccoef, r1off, lpair, ldos);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     if (g.B > 5) {
         hipLaunchKernelGGL(k_contract_list_g<double2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
-                           coef, r1off, lpair, ldos);
+                           coff, co2, ccoef, r1off, lpair, ldos);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     switch (g.B) {
 #define CASE(b)                                                                                                    \
     case b:                                                                                                        \
-        hipLaunchKernelGGL(k_contract_list<b>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk, coef,    \
-                           r1off, lpair, ldos);                                                                    \
+        hipLaunchKernelGGL(k_contract_list<b>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk, coff,    \
+                           co2, ccoef, r1off, lpair, ldos);                                                        \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
 #undef CASE

@@ -218,10 +218,13 @@ int launch_contract_fp64(const Geom &g, const double2 *G0w, const double2 *T, co
                          const double *tiles, int E, double *ldos, hipStream_t s, const double2 *tips = nullptr);
 // threshold-list contraction: coef[k][o2][o1][pix] per active window
 // pair k = lpair[k] = r1*(2w+1)^2 + r2, pairs of one r1 contiguous from
-// r1off[r1]; writes the periodic part to bulk (E x os^2) and the maps to ldos
+// r1off[r1]; the same factors per orbital pair present, the combos
+// coff[k*b + o1] .. coff[k*b + o1 + 1] - 1 with o2 = co2[cb] and factors
+// ccoef[cb][pix]; writes the periodic part to bulk (E x os^2) and the maps to ldos
 // (up to 16 bands; G0f, Tf != NULL: FP32, with E*nw*nw*b*b and E*b*b
 // elements of scratch for the narrowed windows and T)
 int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, const double *coef,
+                         const int *coff, const int *co2, const double *ccoef,
                          const int *r1off, const int *lpair, int nact, int E, double *bulk, double *ldos,
                          hipStream_t s, float2 *G0f = nullptr, float2 *Tf = nullptr);
 // spectral-function maps n x n per energy (mode 1: SPF, 2: USPF; pos: mb x 3 =
