@@ -223,8 +223,8 @@ struct cqpi_ctx {
     double sA = 1.0;
     _Float16 *d_afrag = nullptr;
     size_t c_afrag = 0;
-    double *d_mscale = nullptr;
-    size_t c_mscale = 0;
+    double *d_mscale = nullptr, *d_cscale = nullptr;
+    size_t c_mscale = 0, c_cscale = 0;
     _Float16 *d_planes = nullptr;
     size_t c_planes = 0;
 
@@ -280,7 +280,7 @@ void release(cqpi_ctx *c)
                     c->d_lcoff, c->d_lco2, c->d_lr1off,
                     c->d_lpair, c->d_spos, c->d_g0k, c->d_dos, c->d_dosp, c->d_domega,
                     c->d_sgroups, c->d_stmat, c->d_sstat, c->d_status, c->d_omega, c->d_ldos,
-                    c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
+                    c->d_tilesf, c->d_G0f, c->d_Tf, c->d_afrag, c->d_mscale, c->d_cscale, c->d_planes, c->d_twn, c->d_abar, c->d_bbar, c->d_Tpart,
                     c->d_Tbar, c->d_Gbar, c->d_Vbar, c->d_Hacc, c->d_Rlist, c->d_SR, c->d_rhobar,
                     c->d_rhosum, c->d_ry, c->d_U, c->d_Q};
     for (void *p : bufs)
@@ -361,6 +361,7 @@ int upload_identity_V(cqpi_ctx *c)
 int mix_buffers(cqpi_ctx *c, int E)
 {
     RETCHK(dev_grow(c->d_mscale, c->c_mscale, (size_t)E, c->gpumem));
+    RETCHK(dev_grow(c->d_cscale, c->c_cscale, (size_t)E, c->gpumem));
     RETCHK(dev_grow(c->d_planes, c->c_planes, (size_t)E * mix_plane_halves(c->g, c->mg), c->gpumem));
     return 0;
 }
@@ -421,7 +422,7 @@ int contract(cqpi_ctx *c, const double2 *G0w, const double2 *T, int E, double *l
         }
         if (!r)
             r = launch_contract_mixed(g, c->mg, c->d_planes + eo * mix_plane_halves(g, c->mg), c->d_afrag,
-                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s, tips, tf);
+                                      c->d_mscale + eo, c->sA, T, bulk, E, ldos, s, tips, tf, c->d_cscale + eo);
         break;
     }
     case CQPI_PREC_FP32:
@@ -1267,7 +1268,8 @@ int cqpi_wannierldos_device(cqpi_ctx *c, const double *d_T, const double *d_g0wi
     if (!base) return fail(CQPI_E_STATE, "no scratch address for the context's device");
     const Geom &g = c->g;
     const size_t NP = (size_t)g.B * g.B;
-    // scratch layout: periodic table, then (mixed) scales and planes or (FP32)
+    // scratch layout: periodic table, then (mixed) the plane scales, the scales
+    // of the T-folded b planes and the planes or (FP32)
     // narrowed windows and T, each part 256-byte aligned
     size_t off = 0;
     auto carve = [&off](size_t bytes) {
@@ -1277,8 +1279,10 @@ int cqpi_wannierldos_device(cqpi_ctx *c, const double *d_T, const double *d_g0wi
     };
     const size_t o_bulk = carve(E * g.os * g.os * sizeof(double));
     size_t o_a = 0, o_b = 0;
+    size_t o_c = 0;
     if (c->precision == CQPI_PREC_MIXED) {
         o_a = carve(E * sizeof(double));
+        o_c = carve(E * sizeof(double));
         o_b = carve(E * mix_plane_halves(g, c->mg) * sizeof(_Float16));
     } else if (c->precision == CQPI_PREC_FP32) {
         o_a = carve(E * g.nw * g.nw * NP * sizeof(float2));
@@ -1302,7 +1306,9 @@ int cqpi_wannierldos_device(cqpi_ctx *c, const double *d_T, const double *d_g0wi
         _Float16 *planes = reinterpret_cast<_Float16 *>(sp + o_b);
         const bool tf = mix_tfold(g, c->mg);
         r = launch_mix_prepare(g, c->mg, G0w, En, scale, planes, s, tf);
-        if (!r) r = launch_contract_mixed(g, c->mg, planes, c->d_afrag, scale, c->sA, T, bulk, En, d_ldos, s, nullptr, tf);
+        if (!r)
+            r = launch_contract_mixed(g, c->mg, planes, c->d_afrag, scale, c->sA, T, bulk, En, d_ldos, s, nullptr, tf,
+                                      reinterpret_cast<double *>(sp + o_c));
         break;
     }
     case CQPI_PREC_FP32:

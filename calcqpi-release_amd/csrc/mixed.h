@@ -39,8 +39,12 @@ bool mix_tfold(const Geom &g, const MixGeom &mg);
 int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int E, double *scale,
                        _Float16 *P, hipStream_t s, bool tfold = false);
 // tips: device tip factors of the E energies for the Josephson map (bulk then
-// holds the complex periodic table, E x os^2 double2); NULL: the LDOS map
+// holds the complex periodic table, E x os^2 double2); NULL: the LDOS map.
+// cscale (LDOS maps, E doubles of device scratch, may be NULL): the kernels
+// other than the T-folded one first fold T into the b planes (in place) and
+// keep their scale there.
 int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
-                          double *ldos, hipStream_t s, const double2 *tips = nullptr, bool tfold = false);
+                          double *ldos, hipStream_t s, const double2 *tips = nullptr, bool tfold = false,
+                          double *cscale = nullptr);
 }  // namespace cqpi

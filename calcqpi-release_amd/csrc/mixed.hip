@@ -95,6 +95,63 @@ __global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
     }
 }
 
+// Folds T into the b planes for the LDOS map: the planes (o, p) of ab = 1
+// become c = T b, (T G0(-R))[p][o] = sum_q T_pq G0(-R)[q][o], so that the
+// contraction's epilogue needs no per-pixel T sandwich.  One lane per
+// (row, column, orbital o) of one energy; the b values of its column are
+// rebuilt from hi + lo 2^-12 in FP32, the sums taken in FP32 and split
+// again.  cscale[e] receives the new scale of the c values: scale[e]
+// 2^-ex with max_p sum_q |T_pq| < 2^ex, so that they stay below 2^13.
+__global__ void k_fold_c(Geom g, MixGeom mg, const double *__restrict__ scale, const double2 *__restrict__ Tm,
+                         _Float16 *__restrict__ P, double *__restrict__ cscale)
+{
+    constexpr int kMaxFoldB = 16;
+    const int e = blockIdx.y, B = g.B, nw = g.nw, mb = g.maxb;
+    const double2 *Te = Tm + (size_t)e * B * B;
+    double ts = 0.0;
+    for (int p = 0; p < B; p++) {
+        double r = 0.0;
+        for (int q = 0; q < B; q++) r += fabs(Te[p * B + q].x) + fabs(Te[p * B + q].y);
+        ts = fmax(ts, r);
+    }
+    int ex = 0;
+    if (ts > 0.0) frexp(ts, &ex);
+    const float f = (float)ldexp(1.0, -ex);  // c scale / b scale
+    if (blockIdx.x == 0 && threadIdx.x == 0) cscale[e] = scale[e] * (double)f;
+    const size_t item = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)nw * mg.Ly;
+    if (item >= cells * mb) return;
+    const int o = (int)(item / cells);
+    const size_t rc = item - (size_t)o * cells;
+    const int row = (int)(rc / mg.Ly), col = (int)(rc - (size_t)row * mg.Ly);
+    const size_t plane = (size_t)mg.Ly * nw, off = plane_off(row, col, nw);
+    const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
+    _Float16 *Pw = P + (size_t)e * 4 * mg.NPL * plane;
+    const _Float16 *bh = Pe + (size_t)(2 * mg.NPL) * plane, *bl = Pe + (size_t)(3 * mg.NPL) * plane;
+    float br[kMaxFoldB], bi[kMaxFoldB];
+    for (int q = 0; q < B; q++) {
+        const size_t pr = (size_t)((o * B + q) * 2) * plane + off, pi = pr + plane;
+        br[q] = (float)bh[pr] + (float)bl[pr] * (1.0f / kLoScale);
+        bi[q] = (float)bh[pi] + (float)bl[pi] * (1.0f / kLoScale);
+    }
+    for (int p = 0; p < B; p++) {
+        float cr = 0.f, ci = 0.f;
+        for (int q = 0; q < B; q++) {
+            const float tr = (float)Te[p * B + q].x, ti = (float)Te[p * B + q].y;
+            cr += tr * br[q] - ti * bi[q];
+            ci += tr * bi[q] + ti * br[q];
+        }
+        cr *= f;
+        ci *= f;
+        const size_t pr = (size_t)((o * B + p) * 2) * plane + off, pi = pr + plane;
+        const _Float16 hr = (_Float16)cr, hi = (_Float16)ci;
+        Pw[(size_t)(2 * mg.NPL) * plane + pr] = hr;
+        Pw[(size_t)(3 * mg.NPL) * plane + pr] = (_Float16)((cr - (float)hr) * (float)kLoScale);
+        Pw[(size_t)(2 * mg.NPL) * plane + pi] = hi;
+        Pw[(size_t)(3 * mg.NPL) * plane + pi] = (_Float16)((ci - (float)hi) * (float)kLoScale);
+    }
+}
+
 // the 8 halves at columns col .. col+7 of row `row` of plane pl (col a
 // multiple of LDW; the run may span two column groups of plane_off)
 template <int LDW>
@@ -138,6 +195,9 @@ constexpr int kJB = 4;  // sub-pixel rows (jj) per accumulator block
 // imaginary part of sum_s a^T T b' are kept, and per pixel
 //   rho = -Im[ tip (sum_s a^T T b' + bulk') ]
 // with the complex periodic table bulk' (E x os^2 double2) and tips[e].
+//
+// cscale != NULL (LDOS map): the b planes hold c = T b (k_fold_c, scale
+// cscale[e]); the epilogue then sums Im(a_p c_p) with no T sandwich.
 template <int NT, int LDW, int JB, bool J>
 __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                     const half8 *__restrict__ Afrag,
@@ -145,7 +205,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                                                     const double2 *__restrict__ Tm,
                                                     const double *__restrict__ bulk,
                                                     double *__restrict__ ldos,
-                                                    const double2 *__restrict__ tips)
+                                                    const double2 *__restrict__ tips,
+                                                    const double *__restrict__ cscale)
 {
     __shared__ double2 Ts[16 * 16];
     constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
@@ -252,6 +313,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
                 if (i < JB && jj0 + i < os && m < rows) {
                     const float *Cr = Cs[wave][i][m];
                     double acc = 0.0, accr = 0.0;
+                    if (!J && cscale) {
+                        for (int p = 0; p < B; p++)
+                            acc += (double)Cr[2 * p] * (double)Cr[2 * B + 2 * p + 1] +
+                                   (double)Cr[2 * p + 1] * (double)Cr[2 * B + 2 * p];
+                    } else
                     for (int p = 0; p < B; p++) {
                         double2 tb = make_double2(0.0, 0.0);
                         for (int q = 0; q < B; q++)
@@ -268,8 +334,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_cmix(Geom g, MixGeom mg, const 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
-    const double sc = sA * scale[e];
-    const double unscale = 1.0 / (sc * sc);
+    const double unscale = 1.0 / (sA * sA * scale[e] * (!J && cscale ? cscale[e] : scale[e]));
     const int W = g.W;
     double *out = ldos + (size_t)e * W * W;
 #pragma unroll
@@ -821,6 +886,7 @@ size_t cmix_st_lds(const MixGeom &mg, const Geom &g, int NT, int JB)
 
 // J: the Josephson map, as in k_cmix (the b planes of the spin block staged
 // from the particle-hole columns o + maxband).
+// cscale: as k_cmix.
 template <int NT, int LDW, int JB, bool J>
 __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
                                                           const half8 *__restrict__ Afrag,
@@ -828,7 +894,8 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
                                                           const double2 *__restrict__ Tm,
                                                           const double *__restrict__ bulk,
                                                           double *__restrict__ ldos,
-                                                          const double2 *__restrict__ tips)
+                                                          const double2 *__restrict__ tips,
+                                                          const double *__restrict__ cscale)
 {
     constexpr int NB = 16 / JB;  // jj blocks (os <= 16)
     __shared__ double2 Ts[16 * 16];
@@ -950,6 +1017,11 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
                 if (i < JB && jj0 + i < os && m < rows) {
                     const float *Cr = Csw + ((size_t)i * 16 + m) * (16 * NT + 1);
                     double acc = 0.0, accr = 0.0;
+                    if (!J && cscale) {
+                        for (int p = 0; p < B; p++)
+                            acc += (double)Cr[2 * p] * (double)Cr[2 * B + 2 * p + 1] +
+                                   (double)Cr[2 * p + 1] * (double)Cr[2 * B + 2 * p];
+                    } else
                     for (int p = 0; p < B; p++) {
                         double2 tb = make_double2(0.0, 0.0);
                         for (int q = 0; q < B; q++)
@@ -966,8 +1038,7 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
-    const double sc = sA * scale[e];
-    const double unscale = 1.0 / (sc * sc);
+    const double unscale = 1.0 / (sA * sA * scale[e] * (!J && cscale ? cscale[e] : scale[e]));
     const int W = g.W;
     double *out = ldos + (size_t)e * W * W;
 #pragma unroll
@@ -990,13 +1061,15 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
 
 template <int NT, int JB, bool J>
 void launch_st_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
-                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
+                 const double *cscale)
 {
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kStWaves - 1) / kStWaves, E);
     dim3 blk(64 * kStWaves);
     const size_t lds = cmix_st_lds(mg, g, NT, JB);
 #define CQPI_ST(ldw) \
-    hipLaunchKernelGGL((k_cmix_st<NT, ldw, JB, J>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips)
+    hipLaunchKernelGGL((k_cmix_st<NT, ldw, JB, J>), grid, blk, lds, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips, \
+                       cscale)
     switch (mg.LDW) {
     case 8: CQPI_ST(8); break;
     case 4: CQPI_ST(4); break;
@@ -1008,18 +1081,20 @@ void launch_st_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half
 
 template <int NT, int JB>
 int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
-              const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+              const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
+              const double *cscale)
 {
     if (tips)
-        launch_st_j<NT, JB, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s);
+        launch_st_j<NT, JB, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, nullptr);
     else
-        launch_st_j<NT, JB, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s);
+        launch_st_j<NT, JB, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s, cscale);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
 template <int NT, bool J>
 void launch_nt_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
-                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+                 const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
+                 const double *cscale)
 {
     dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kWaves - 1) / kWaves, E);
     dim3 blk(64 * kWaves);
@@ -1027,7 +1102,7 @@ void launch_nt_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half
     // accumulators leave room for more than one wave per SIMD
     constexpr int JB = NT >= 3 ? 2 : 4;
 #define CQPI_NT(ldw) \
-    hipLaunchKernelGGL((k_cmix<NT, ldw, JB, J>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips)
+    hipLaunchKernelGGL((k_cmix<NT, ldw, JB, J>), grid, blk, 0, s, g, mg, P, A, scale, sA, T, bulk, ldos, tips, cscale)
     switch (mg.LDW) {
     case 8: CQPI_NT(8); break;
     case 4: CQPI_NT(4); break;
@@ -1039,13 +1114,14 @@ void launch_nt_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half
 
 template <int NT>
 int launch_nt(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag, const double *scale,
-              double sA, const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s)
+              double sA, const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
+              const double *cscale)
 {
     const half8 *A = (const half8 *)Afrag;
     if (tips)
-        launch_nt_j<NT, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s);
+        launch_nt_j<NT, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, nullptr);
     else
-        launch_nt_j<NT, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s);
+        launch_nt_j<NT, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s, cscale);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -1149,7 +1225,7 @@ int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int
 
 int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, const void *Afrag,
                           const double *scale, double sA, const double2 *T, const double *bulk, int E,
-                          double *ldos, hipStream_t s, const double2 *tips, bool tfold)
+                          double *ldos, hipStream_t s, const double2 *tips, bool tfold, double *cscale)
 {
     const half8 *A = (const half8 *)Afrag;
     if (tips && 2 * g.maxb > g.B) return CQPI_E_ARG;
@@ -1174,22 +1250,29 @@ int launch_contract_mixed(const Geom &g, const MixGeom &mg, const _Float16 *P, c
         try_areg<2, 4, 4, 7, 5, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s) ||
         try_areg<1, 2, 8, 1, 1, 1>(g, mg, P, A, scale, sA, T, bulk, E, ldos, s)))
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+    // LDOS maps: T folded into the b planes first (no per-pixel T sandwich)
+    if (!tips && cscale) {
+        const size_t items = (size_t)g.nw * mg.Ly * g.maxb;
+        hipLaunchKernelGGL(k_fold_c, dim3((unsigned)((items + 255) / 256), E), dim3(256), 0, s, g, mg, scale, T,
+                           const_cast<_Float16 *>(P), cscale);
+        if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    }
     // staged B operand when the spin block's plane rows fit the LDS
     constexpr size_t kStLdsMax = 150 * 1024;
     if (g.os <= 16 && g.B <= 16) {
         switch (mg.NT) {
-        case 1: if (cmix_st_lds(mg, g, 1, 4) <= kStLdsMax) return launch_st<1, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
-        case 2: if (cmix_st_lds(mg, g, 2, 4) <= kStLdsMax) return launch_st<2, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
-        case 3: if (cmix_st_lds(mg, g, 3, 2) <= kStLdsMax) return launch_st<3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
-        case 4: if (cmix_st_lds(mg, g, 4, 2) <= kStLdsMax) return launch_st<4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s); break;
+        case 1: if (cmix_st_lds(mg, g, 1, 4) <= kStLdsMax) return launch_st<1, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, cscale); break;
+        case 2: if (cmix_st_lds(mg, g, 2, 4) <= kStLdsMax) return launch_st<2, 4>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, cscale); break;
+        case 3: if (cmix_st_lds(mg, g, 3, 2) <= kStLdsMax) return launch_st<3, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, cscale); break;
+        case 4: if (cmix_st_lds(mg, g, 4, 2) <= kStLdsMax) return launch_st<4, 2>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, cscale); break;
         default: break;
         }
     }
     switch (mg.NT) {
-    case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
-    case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
-    case 3: return launch_nt<3>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
-    case 4: return launch_nt<4>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s);
+    case 1: return launch_nt<1>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s, cscale);
+    case 2: return launch_nt<2>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s, cscale);
+    case 3: return launch_nt<3>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s, cscale);
+    case 4: return launch_nt<4>(g, mg, P, Afrag, scale, sA, T, bulk, E, ldos, tips, s, cscale);
     default: return CQPI_E_ARG;
     }
 }
