@@ -214,27 +214,31 @@ __host__ __device__ constexpr int tw_pass_off(int N, int Lstop)
     return off;
 }
 // fills the per-pass table dst (tw_pass_off(N, N) entries) from the natural
-// table tw[m] = w^m, m < N, with the threads of the workgroup (the pass
-// structure is unrolled at compile time)
-template <int N, int L, int LGREM, typename C>
-__device__ inline void build_pass_twiddles_rec(C *dst, const C *__restrict__ tw)
-{
-    if constexpr (LGREM > 0) {
-        constexpr int R = stockham_radix(LGREM);
-        if constexpr (L > 1) {
-            constexpr int off = tw_pass_off(N, L), sz = L * (R - 1), st = N / (L * R);
-            for (int u = threadIdx.x; u < sz; u += blockDim.x) {
-                const int j = u / (R - 1), r = u - j * (R - 1) + 1;
-                dst[off + u] = tw[j * r * st];
-            }
-        }
-        build_pass_twiddles_rec<N, L * R, LGREM - ilog2c(R)>(dst, tw);
-    }
-}
+// table tw[m] = w^m, m < N, with the threads of the workgroup
 template <int N, typename C>
 __device__ inline void build_pass_twiddles(C *dst, const C *__restrict__ tw)
 {
-    build_pass_twiddles_rec<N, 1, ilog2n(N)>(dst, tw);
+    constexpr int total = tw_pass_off(N, N);
+    for (int m = threadIdx.x; m < total; m += blockDim.x) {
+        int lg = 0;
+        while ((1 << lg) < N) lg++;
+        int L = 1, off = 0, idx = 0;
+        while (lg > 0) {
+            const int R = (lg >= 3 && lg != 4) ? 8 : (lg >= 2 ? 4 : 2);
+            if (L > 1) {
+                const int sz = L * (R - 1);
+                if (m < off + sz) {
+                    const int u = m - off, j = u / (R - 1), r = u - j * (R - 1) + 1;
+                    idx = j * r * (N / (L * R));
+                    break;
+                }
+                off += sz;
+            }
+            L *= R;
+            lg -= R == 8 ? 3 : (R == 4 ? 2 : 1);
+        }
+        dst[m] = tw[idx];
+    }
 }
 
 template <int N, int L, int R, int LN, bool LP, typename C>
