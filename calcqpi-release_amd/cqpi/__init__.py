@@ -88,6 +88,9 @@ SIGNATURES = {
     "cqpi_gradient_device": (C.c_int, [_vp, _vp, _dp, _dp, _dp]),
     "cqpi_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cqpi_get_tuning": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cqpi_set_tuning_ext": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cqpi_get_tuning_ext": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "cqpi_autotune": (C.c_int, [_vp, _dp, _sz, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cqpi_batch_create": (C.c_int, [_vp, _sz, C.c_int, C.POINTER(_vp)]),
     "cqpi_batch_autotune": (C.c_int, [_vp, C.c_int]),
@@ -448,6 +451,15 @@ class AmdQPI:
         t, e = C.c_int(), C.c_int()
         _check(lib().cqpi_get_tuning(self._ctx, C.byref(t), C.byref(e)))
         return t.value, e.value
+
+    def set_tuning_ext(self, tpw=0, eb=0, js=0, streams=0):
+        _check(lib().cqpi_set_tuning_ext(self._ctx, int(tpw), int(eb), int(js), int(streams)))
+
+    def tuning_ext(self):
+        """(tpw, eb, js, streams) of the context."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().cqpi_get_tuning_ext(self._ctx, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, omegas, eta, reps=3):
         """Times candidate launch tunings for this shape on the device and keeps

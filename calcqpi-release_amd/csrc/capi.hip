@@ -1,6 +1,7 @@
 // C ABI of the continuum-LDOS library: context, GPUQPI drop-in entry points
 // and the whole-chain device path.  See include/cqpi_amd.h for the contract.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -635,7 +636,7 @@ int enqueue_chunks(cqpi_ctx *c, const RunPlan &plan)
         // parts of at least 4 energies (the G(k) workspace path has one
         // workspace: no split)
         const int parts = std::min(kSplit, ec / 4);
-        const bool split = parts >= 2 && !c->timing && !c->single_stream && !plan.work_bytes;
+        const bool split = parts >= 2 && !c->timing && !c->single_stream && !plan.work_bytes && c->g.streams != 1;
         if (split && !c->ev_fork) {
             bool ok = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
             for (int i = 0; ok && i < kSplit - 1; i++)
@@ -1815,7 +1816,7 @@ namespace {
 
 // best launch tuning per problem shape, shared by every context of the process
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<int, int>> g_tune;
+std::map<std::string, std::array<int, 4>> g_tune;  // tpw, eb, js, streams
 
 std::string tune_key(const cqpi_ctx *c, size_t E)
 {
@@ -1858,6 +1859,26 @@ int cqpi_set_tuning(cqpi_ctx *c, int tpw, int eb)
     return 0;
 }
 
+int cqpi_set_tuning_ext(cqpi_ctx *c, int tpw, int eb, int js, int streams)
+{
+    RETCHK(cqpi_set_tuning(c, tpw, eb));
+    if (js < 0 || js > 2) return fail(CQPI_E_ARG, "js must be 0, 1 or 2");
+    if (streams < 0 || streams > 2) return fail(CQPI_E_ARG, "streams must be 0, 1 or 2");
+    c->g.js = js;
+    c->g.streams = streams;
+    return 0;
+}
+
+int cqpi_get_tuning_ext(cqpi_ctx *c, int *tpw, int *eb, int *js, int *streams)
+{
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (tpw) *tpw = c->g.tpw;
+    if (eb) *eb = c->g.eb;
+    if (js) *js = c->g.js;
+    if (streams) *streams = c->g.streams;
+    return 0;
+}
+
 int cqpi_get_tuning(cqpi_ctx *c, int *tpw, int *eb)
 {
     if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
@@ -1876,8 +1897,10 @@ int cqpi_autotune(cqpi_ctx *c, const double *omegas, size_t E, double eta, int r
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) {
-            c->g.tpw = it->second.first;
-            c->g.eb = it->second.second;
+            c->g.tpw = it->second[0];
+            c->g.eb = it->second[1];
+            c->g.js = it->second[2];
+            c->g.streams = it->second[3];
             if (tpw) *tpw = c->g.tpw;
             if (eb) *eb = c->g.eb;
             return 0;
@@ -1904,6 +1927,8 @@ int cqpi_autotune(cqpi_ctx *c, const double *omegas, size_t E, double eta, int r
         if (r) break;
         c->g.eb = cand;
         c->g.tpw = 0;
+        c->g.js = 0;
+        c->g.streams = 0;
         float ms = 0.f;
         r = time_run(c, plan, reps, ms);
         if (!r && ms < best) {
@@ -1925,14 +1950,47 @@ int cqpi_autotune(cqpi_ctx *c, const double *omegas, size_t E, double eta, int r
             }
         }
     }
+    // waves per cell row of the T-folded contraction (its default is per shape)
+    int best_js = 0;
+    if (!r && plan.mixed && !c->josephson && mix_tfold(c->g, c->mg)) {
+        const int jss[] = {1, 2};
+        for (int cand : jss) {
+            c->g.eb = best_eb;
+            c->g.tpw = best_tpw;
+            c->g.js = cand;
+            float ms = 0.f;
+            r = time_run(c, plan, reps, ms);
+            if (r) break;
+            if (ms < best) {
+                best = ms;
+                best_js = cand;
+            }
+        }
+    }
+    // one stream instead of the split over two (where a run would split)
+    int best_streams = 0;
+    if (!r && plan.Ec >= 8 && !plan.work_bytes && !c->single_stream) {
+        c->g.eb = best_eb;
+        c->g.tpw = best_tpw;
+        c->g.js = best_js;
+        c->g.streams = 1;
+        float ms = 0.f;
+        r = time_run(c, plan, reps, ms);
+        if (!r && ms < best) {
+            best = ms;
+            best_streams = 1;
+        }
+    }
     c->timing = timing;
     c->g.eb = best_eb;
     c->g.tpw = best_tpw;
+    c->g.js = best_js;
+    c->g.streams = best_streams;
     if (!r) r = finish_run(c, plan);
     if (r) return r;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[key] = {best_tpw, best_eb};
+        g_tune[key] = {best_tpw, best_eb, best_js, best_streams};
     }
     if (tpw) *tpw = best_tpw;
     if (eb) *eb = best_eb;

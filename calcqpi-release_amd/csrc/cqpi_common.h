@@ -71,6 +71,8 @@ struct Geom {
     // launch tuning (0: built-in heuristic), chosen by the autotuner
     int tpw;      // mixed contraction: cell rows per wave (2, 4, 8)
     int eb;       // first transform pass: energies per workgroup
+    int js;       // T-folded mixed contraction: waves per cell row (1, 2; 0: per shape)
+    int streams;  // eager runs: streams an energy chunk is split over (1, 2; 0: 2 from 8 energies)
 };
 
 // Workgroup barrier that orders LDS accesses only.  __syncthreads also waits

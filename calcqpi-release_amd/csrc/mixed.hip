@@ -842,15 +842,26 @@ bool tf_shape(const Geom &g, const MixGeom &mg)
 }
 
 template <int OS, int WW, int BT, int JS>
-bool try_tf(const Geom &g, const MixGeom &mg, const float2 *W32, const half8 *A, const double *scale, double sA,
-            const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+void run_tf(const Geom &g, const float2 *W32, const half8 *A, const double *scale, double sA, const double2 *T,
+            const double *bulk, int E, double *ldos, hipStream_t s)
 {
-    if (!tf_shape<OS, WW, BT, JS>(g, mg)) return false;
     switch (g.tpw) {
     case 2: launch_tf<OS, WW, BT, 2, JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
     case 4: launch_tf<OS, WW, BT, 4, JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
     default: launch_tf<OS, WW, BT, kTPW * JS * 4 / tf_wg_waves(BT), JS>(g, W32, A, scale, sA, T, bulk, E, ldos, s); break;
     }
+}
+
+// JS: the shape's default waves per cell row; g.js (autotuner) may pick 1 or 2
+template <int OS, int WW, int BT, int JS>
+bool try_tf(const Geom &g, const MixGeom &mg, const float2 *W32, const half8 *A, const double *scale, double sA,
+            const double2 *T, const double *bulk, int E, double *ldos, hipStream_t s)
+{
+    if (!tf_shape<OS, WW, BT, JS>(g, mg)) return false;
+    if ((g.js ? g.js : JS) == 1)
+        run_tf<OS, WW, BT, 1>(g, W32, A, scale, sA, T, bulk, E, ldos, s);
+    else
+        run_tf<OS, WW, BT, 2>(g, W32, A, scale, sA, T, bulk, E, ldos, s);
     return true;
 }
 

@@ -385,12 +385,20 @@ int cqpi_gradient_device(cqpi_ctx *ctx, const double *d_rhobar, double *g_hops, 
  * do not depend on either. */
 int cqpi_set_tuning(cqpi_ctx *ctx, int tpw, int eb);
 int cqpi_get_tuning(cqpi_ctx *ctx, int *tpw, int *eb);
+/* The same with the other two tuned knobs: js = waves per cell row of the
+ * T-folded mixed contraction (0 = the shape's default, or 1, 2), streams =
+ * streams an eager run splits an energy chunk over (0 = two from 8 energies,
+ * 1 = one, 2 = two).  Results do not depend on any of them. */
+int cqpi_set_tuning_ext(cqpi_ctx *ctx, int tpw, int eb, int js, int streams);
+int cqpi_get_tuning_ext(cqpi_ctx *ctx, int *tpw, int *eb, int *js, int *streams);
 
 /* Autotuner: times candidate tunings on the device for this context's shape
  * and E energies (reps timed runs each, eigen data cached), keeps the fastest
  * and remembers it for the process (contexts of the same shape then reuse it
- * without timing).  The maps of the last timed run are left in the context.
- * *tpw and *eb (may be NULL) receive the choice. */
+ * without timing).  Knobs, each timed with the others fixed: eb, then (mixed)
+ * tpw and js, then streams.  The maps of the last timed run are left in the
+ * context.  *tpw and *eb (may be NULL) receive the choice; cqpi_get_tuning_ext
+ * reads all four. */
 int cqpi_autotune(cqpi_ctx *ctx, const double *omegas, size_t E, double eta, int reps, int *tpw, int *eb);
 
 /* ---------------------------------------------------------------------- */

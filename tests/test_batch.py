@@ -98,20 +98,30 @@ def test_batch_reports_singular_instance(gpu):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autotune_leaves_results_unchanged(gpu, precision):
-    wl = synth.square_norb(3, K=64, n=16, E=6, seed=7200, w=2, os_=4, eta=0.01)
+    # 10 energies: the two-stream split applies, so the streams knob is timed
+    wl = synth.square_norb(3, K=64, n=16, E=10, seed=7200, w=2, os_=4, eta=0.01)
     print("case: autotune %s" % precision, flush=True)
     base = _eager(wl, precision)
     with cqpi.engine_for(wl, precision) as q:
         tpw, eb = q.autotune(wl.energies, wl.eta, reps=2)
         assert tpw in (0, 2, 4, 8) and 0 <= eb <= len(wl.energies)
         assert q.tuning() == (tpw, eb)
+        ext = q.tuning_ext()
+        assert ext[:2] == (tpw, eb) and ext[2] in (0, 1, 2) and ext[3] in (0, 1)
         assert np.array_equal(q.run(wl.energies, wl.eta), base)
         for t, e in ((2, 1), (4, 3), (8, 0)):
             q.set_tuning(t, e)
             assert np.array_equal(q.run(wl.energies, wl.eta), base), (t, e)
+        # every waves-per-row and stream choice gives the same maps
+        for js in (1, 2):
+            for st in (1, 2):
+                q.set_tuning_ext(4, 0, js, st)
+                assert q.tuning_ext() == (4, 0, js, st)
+                assert np.array_equal(q.run(wl.energies, wl.eta), base), (js, st)
     # a second context of the same shape reuses the remembered tuning
     with cqpi.engine_for(wl, precision) as q2:
         assert q2.autotune(wl.energies, wl.eta, reps=2) == (tpw, eb)
+        assert q2.tuning_ext() == ext
 
 
 def test_batch_autotune_then_run(gpu):
