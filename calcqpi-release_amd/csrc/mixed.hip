@@ -118,13 +118,14 @@ __global__ void k_fold_c(Geom g, MixGeom mg, const double *__restrict__ scale, c
     if (ts > 0.0) frexp(ts, &ex);
     const float f = (float)ldexp(1.0, -ex);  // c scale / b scale
     if (blockIdx.x == 0 && threadIdx.x == 0) cscale[e] = scale[e] * (double)f;
+    // items in the planes' own order ([column group][row][8 columns]), so
+    // that the lanes of a wave touch contiguous memory
     const size_t item = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cells = (size_t)nw * mg.Ly;
     if (item >= cells * mb) return;
     const int o = (int)(item / cells);
-    const size_t rc = item - (size_t)o * cells;
-    const int row = (int)(rc / mg.Ly), col = (int)(rc - (size_t)row * mg.Ly);
-    const size_t plane = (size_t)mg.Ly * nw, off = plane_off(row, col, nw);
+    const size_t off = item - (size_t)o * cells;
+    const size_t plane = (size_t)mg.Ly * nw;
     const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
     _Float16 *Pw = P + (size_t)e * 4 * mg.NPL * plane;
     const _Float16 *bh = Pe + (size_t)(2 * mg.NPL) * plane, *bl = Pe + (size_t)(3 * mg.NPL) * plane;
