@@ -663,7 +663,7 @@ def main():
         "stages_ms": {k: v[0] for k, v in st_m.items()},
         "accuracy": acc,
         "roofline": transform_roofline(wl, qm, st_m, "mixed", ws),
-        "contraction_roofline": contraction_roofline(wl, st_m, "mixed", ws, "k_cmix_areg"),
+        "contraction_roofline": contraction_roofline(wl, st_m, "mixed", ws, "k_cmix_tf"),
         "end_to_end": end_to_end(qm, wl, max(3, args.steps // 4), torch),
     }
     for q, _, _ in timed.values():
