@@ -614,9 +614,11 @@ int launch_gk(int B, int N, const double *eval, const double2 *evec, const doubl
 }
 
 // pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk);
-// the window is written in FP64 whatever C is
+// the window is written in FP64 whatever C is.  FP32 lines are compiled for 7
+// waves per SIMD (72 VGPRs: three 9-wave workgroups a CU; 6 % faster than the
+// compiler's 78 VGPRs and two workgroups; 8 waves spilled)
 template <typename C, int N>
-__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
+__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 1)) k_kx_s(Geom g, MixGeom mg, const C *__restrict__ F1,
                                                                  const C *__restrict__ tw, int np, int np1, int rwin,
                                                                  double sG, double2 *__restrict__ G0w,
                                                                  _Float16 *__restrict__ P, PlaneDiv dv,
