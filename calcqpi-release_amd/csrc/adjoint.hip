@@ -496,6 +496,68 @@ __global__ void __launch_bounds__(64) k_adj_tmatrix(Geom g, const double2 *__res
 
 constexpr int kThreads = 256;
 
+// Resolvent adjoint in the eigenbasis.  With G(k) = V F V^H (V the
+// eigenvectors, F = diag f_n, f_n = 1/(w + i eta - e_n)),
+//   Hbar = G^H Gbar G^H = V F* (V^H Gbar V) F* V^H,
+// and V does not depend on the energy, so a k-point accumulates
+//   Z += F* (V^H Gbar V) F*          (2 b^3 complex products per energy)
+// and forms Hbar = V Z V^H once, after its last energy.  vv[r*B+n] = V_rn,
+// gb(p) returns Gbar_rl at p = r*B + l.
+template <int B, typename GbAt>
+__device__ inline void adj_res_accumulate(const double2 *vv, const double2 *cf, GbAt gb, double2 *zacc)
+{
+#pragma unroll
+    for (int n = 0; n < B; n++) {
+        double2 wn[B];  // conj(f_n) (Gbar V)[:, n]
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int l = 0; l < B; l++) acc = cadd(acc, cmul(gb(r * B + l), vv[l * B + n]));
+            wn[r] = cmul(acc, cf[n]);
+        }
+#pragma unroll
+        for (int m = 0; m < B; m++) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < B; r++) acc = cadd(acc, cmul(cconj(vv[r * B + m]), wn[r]));
+            zacc[m * B + n] = cadd(zacc[m * B + n], cmul(cf[m], acc));
+        }
+    }
+}
+
+// hb = V Z V^H
+template <int B>
+__device__ inline void adj_res_finish(const double2 *vv, const double2 *zacc, double2 *hb)
+{
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+        double2 t[B];  // (Z V^H)[:, c]
+#pragma unroll
+        for (int m = 0; m < B; m++) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int n = 0; n < B; n++) acc = cadd(acc, cmulc(zacc[m * B + n], vv[c * B + n]));
+            t[m] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int m = 0; m < B; m++) acc = cadd(acc, cmul(vv[r * B + m], t[m]));
+            hb[r * B + c] = acc;
+        }
+    }
+}
+
+// conj(f_n) of the energy omega
+template <int B>
+__device__ inline void adj_res_cf(const double *ev, double omega, double eta, double2 *cf)
+{
+#pragma unroll
+    for (int nn = 0; nn < B; nn++) cf[nn] = cconj(cinv(make_double2(omega - ev[nn], eta)));
+}
+
 // adjoint of the kx pass: F1bar[e][i][sy][pair] = K^-2 sum_sx Gbar[e][sx][sy][pair] e^{-2 pi i i (sx-h)/K}
 __global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__restrict__ Gbar,
                                                    const double2 *__restrict__ twn, int np,
@@ -527,8 +589,9 @@ __global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__re
 }
 
 // adjoint of the ky pass fused with the resolvent adjoint, summed over energies:
-// Gbar(k) for one kx row, then Hacc(k) += G(k)^H Gbar(k) G(k)^H with G(k) rebuilt
-// from the eigen data.  One workgroup per row, all orbital pairs in LDS.
+// Gbar(k) for one kx row, then Hacc(k) += G(k)^H Gbar(k) G(k)^H, accumulated
+// in the eigenbasis (adj_res_accumulate).  One workgroup per row, all orbital
+// pairs in LDS.
 template <int B, int JPT>
 __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__restrict__ eval,
                                                    const double2 *__restrict__ evec,
@@ -547,7 +610,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
     for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
     // eigen data of the thread's k-points, kept for the energy loop
     double ev[JPT][B];
-    double2 vv[JPT][NP];
+    double2 vv[JPT][NP], zacc[JPT][NP];
 #pragma unroll
     for (int t = 0; t < JPT; t++) {
         const int j = threadIdx.x + t * kThreads;
@@ -555,13 +618,11 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
 #pragma unroll
         for (int nn = 0; nn < B; nn++) ev[t][nn] = eval[nn * KK + k];
 #pragma unroll
-        for (int r = 0; r < NP; r++) vv[t][r] = evec[r * KK + k];
+        for (int r = 0; r < NP; r++) {
+            vv[t][r] = evec[r * KK + k];
+            zacc[t][r] = make_double2(0.0, 0.0);
+        }
     }
-    double2 hacc[JPT][NP];
-#pragma unroll
-    for (int t = 0; t < JPT; t++)
-#pragma unroll
-        for (int p = 0; p < NP; p++) hacc[t][p] = make_double2(0.0, 0.0);
     for (int e = e0; e < e1; e++) {
         __syncthreads();
         const double2 *in = F1bar + ((size_t)e * K + i) * (size_t)nw * NP;
@@ -578,42 +639,9 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
         for (int t = 0; t < JPT; t++) {
             const int j = threadIdx.x + t * kThreads;
             if (j >= K) break;
-            const long k = (long)i * K + j;
-            (void)k;
-            double2 f[B];
-#pragma unroll
-            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - ev[t][nn], eta));
-            double2 G[NP], Gb[NP], X[NP];
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int nn = 0; nn < B; nn++)
-                        acc = cadd(acc, cmul(f[nn], cmulc(vv[t][r * B + nn], vv[t][c * B + nn])));
-                    G[r * B + c] = acc;
-                    Gb[r * B + c] = S[(size_t)(r * B + c) * K + j];
-                }
-            // X = Gb G^H, then Hbar = G^H X
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int l = 0; l < B; l++) acc = cadd(acc, cmulc(Gb[r * B + l], G[c * B + l]));
-                    X[r * B + c] = acc;
-                }
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int l = 0; l < B; l++) acc = cadd(acc, cmul(cconj(G[l * B + r]), X[l * B + c]));
-                    hacc[t][r * B + c] = cadd(hacc[t][r * B + c], acc);
-                }
+            double2 cf[B];
+            adj_res_cf<B>(ev[t], omega, eta, cf);
+            adj_res_accumulate<B>(vv[t], cf, [&](int p) { return S[(size_t)p * K + j]; }, zacc[t]);
         }
     }
 #pragma unroll
@@ -621,8 +649,10 @@ __global__ void __launch_bounds__(kThreads) k_adj_ky(Geom g, const double *__res
         const int j = threadIdx.x + t * kThreads;
         if (j >= K) break;
         const long k = (long)i * K + j;
+        double2 hb[NP];
+        adj_res_finish<B>(vv[t], zacc[t], hb);
 #pragma unroll
-        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hacc[t][p];
+        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hb[p];
     }
 }
 
@@ -651,17 +681,15 @@ __global__ void __launch_bounds__(kAdjWaveMaxThreads) k_adj_ky_s(Geom g, const d
     const int j = threadIdx.x;  // this thread's k-point (blockDim >= N)
     const bool has_k = j < N;
     const long k = (long)i * N + (has_k ? j : 0);
-    // the row's eigenvectors stay in LDS after the lines (the workgroup holds
-    // the CU's LDS either way), which keeps the thread's registers unspilled
-    double2 *Vl = S + (size_t)NP * NL;
+    // the eigenvectors and the eigenbasis accumulators in registers
     double ev[B];
-    double2 hacc[NP];
+    double2 vv[NP], zacc[NP];
 #pragma unroll
     for (int nn = 0; nn < B; nn++) ev[nn] = eval[nn * KK + k];
 #pragma unroll
     for (int r = 0; r < NP; r++) {
-        if (has_k) Vl[(size_t)r * N + j] = evec[r * KK + k];
-        hacc[r] = make_double2(0.0, 0.0);
+        vv[r] = evec[r * KK + k];
+        zacc[r] = make_double2(0.0, 0.0);
     }
     for (int e = e0; e < e1; e++) {
         const double2 *in = F1bar + ((size_t)e * N + i) * (size_t)nw * NP;
@@ -675,47 +703,89 @@ __global__ void __launch_bounds__(kAdjWaveMaxThreads) k_adj_ky_s(Geom g, const d
         for (int q = grp; q < NP; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, twn, ll, -1);
         lds_barrier();
         if (has_k) {
-            const double omega = omegas[e];
-            double2 f[B];
-#pragma unroll
-            for (int nn = 0; nn < B; nn++) f[nn] = cinv(make_double2(omega - ev[nn], eta));
-            double2 G[NP];
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int nn = 0; nn < B; nn++)
-                        acc = cadd(acc, cmul(f[nn], cmulc(Vl[(size_t)(r * B + nn) * N + j], Vl[(size_t)(c * B + nn) * N + j])));
-                    G[r * B + c] = acc;
-                }
-            // Hbar = G^H (Gb G^H), one column of X = Gb G^H at a time; Gb read from LDS
+            double2 cf[B];
+            adj_res_cf<B>(ev, omegas[e], eta, cf);
             const double2 *gb = S + cqpi::pidx<double2>(j);
-#pragma unroll
-            for (int c = 0; c < B; c++) {
-                double2 X[B];
-#pragma unroll
-                for (int r = 0; r < B; r++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int l = 0; l < B; l++) acc = cadd(acc, cmulc(gb[(size_t)(r * B + l) * NL], G[c * B + l]));
-                    X[r] = acc;
-                }
-#pragma unroll
-                for (int r = 0; r < B; r++) {
-                    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int l = 0; l < B; l++) acc = cadd(acc, cmul(cconj(G[l * B + r]), X[l]));
-                    hacc[r * B + c] = cadd(hacc[r * B + c], acc);
-                }
-            }
+            adj_res_accumulate<B>(vv, cf, [&](int p) { return gb[(size_t)p * NL]; }, zacc);
         }
         lds_barrier();  // S is rewritten for the next energy
     }
-    if (has_k)
+    if (has_k) {
+        double2 hb[NP];
+        adj_res_finish<B>(vv, zacc, hb);
 #pragma unroll
-        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hacc[p];
+        for (int p = 0; p < NP; p++) Hacc[((size_t)blockIdx.y * NP + p) * KK + k] = hb[p];
+    }
+}
+
+// The ky adjoint for rows whose b^2 lines do not fit one workgroup (b >= 4
+// with K >= 256, or K = 1024), in two kernels over a block of energies:
+//   k_adj_ky_lines: Gbar(k) lines, a chunk of orbital pairs per workgroup,
+//                   into the workspace Gk[el][pair][kx][ky];
+//   k_adj_res_k:    Hbar(k) = G(k)^H Gbar(k) G(k)^H per k-point (one thread
+//                   each), summed over the block's energies of its group and
+//                   added to Hacc[group][pair][k].
+__global__ void __launch_bounds__(kThreads) k_adj_ky_lines(Geom g, const double2 *__restrict__ F1bar,
+                                                         const double2 *__restrict__ twn, int np,
+                                                         double2 *__restrict__ Gk)
+{
+    extern __shared__ double2 S[];
+    const int K = g.K, i = blockIdx.x, el = blockIdx.y, NP = g.B * g.B, p0 = blockIdx.z * np;
+    const int npc = min(np, NP - p0);
+    const int nw = g.nw, h = g.h;
+    const long KK = (long)K * K;
+    int lgK = 0;
+    while ((1 << lgK) < K) lgK++;
+    double2 *Tw = S + (size_t)np * K;
+    for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
+    const double2 *in = F1bar + ((size_t)el * K + i) * (size_t)nw * NP + p0;
+    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
+        const int m = t / npc, q = t - m * npc;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int sy = (m + h) & (K - 1); sy < nw; sy += K) acc = cadd(acc, in[(size_t)sy * NP + q]);
+        S[(size_t)q * K + brev(m, lgK)] = acc;
+    }
+    __syncthreads();
+    cqpi::fft_lines(S, npc, K, lgK, Tw, -1);
+    double2 *out = Gk + ((size_t)el * NP + p0) * KK + (size_t)i * K;
+    for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
+        const int q = t / K, j = t - q * K;
+        out[(size_t)q * KK + j] = S[(size_t)q * K + j];
+    }
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) k_adj_res_k(Geom g, const double *__restrict__ eval,
+                                                   const double2 *__restrict__ evec,
+                                                   const double *__restrict__ omegas, int eb, double eta,
+                                                   const double2 *__restrict__ Gk, int add,
+                                                   double2 *__restrict__ Hacc)
+{
+    constexpr int NP = B * B;
+    const long KK = (long)g.K * g.K;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.y, NG = gridDim.y;
+    if (k >= KK) return;
+    double ev[B];
+    double2 vv[NP], zacc[NP];
+#pragma unroll
+    for (int nn = 0; nn < B; nn++) ev[nn] = eval[nn * KK + k];
+#pragma unroll
+    for (int r = 0; r < NP; r++) {
+        vv[r] = evec[r * KK + k];
+        zacc[r] = make_double2(0.0, 0.0);
+    }
+    for (int el = grp; el < eb; el += NG) {
+        double2 cf[B];
+        adj_res_cf<B>(ev, omegas[el], eta, cf);
+        const double2 *gb = Gk + (size_t)el * NP * KK + k;
+        adj_res_accumulate<B>(vv, cf, [&](int p) { return gb[(size_t)p * KK]; }, zacc);
+    }
+    double2 hb[NP];
+    adj_res_finish<B>(vv, zacc, hb);
+    double2 *out = Hacc + (size_t)grp * NP * KK + k;
+#pragma unroll
+    for (int p = 0; p < NP; p++) out[(size_t)p * KK] = add ? cadd(out[(size_t)p * KK], hb[p]) : hb[p];
 }
 
 __device__ inline double kcoord(int i, int K)
@@ -755,45 +825,46 @@ __device__ inline double2 hop_phase(double x, int k, int K, const double2 *__res
 }
 
 // Hopping sums S(R) = sum_k Hbar(k) e^{-2 pi i R.k} factorise over kx and ky.
-// Stage 1, one workgroup per (kx row, orbital pair): summing the energy groups,
-//   U[kx][r][pair] = sum_ky Hbar(kx, ky) e^{-2 pi i ry_r ky}  for the distinct ry,
-//   U[kx][nry][pair] = sum_ky Hbar(kx, ky)                       (for the trace)
-// Hacc is [group][pair][k], so a workgroup reads contiguous rows.
-__global__ void __launch_bounds__(256) k_adj_hrow(Geom g, const double2 *__restrict__ Hacc, int NG,
-                                                  const double *__restrict__ ry, int nry,
-                                                  const double2 *__restrict__ twn, double2 *__restrict__ U)
+// The y phases first, once per call: Pt[r][ky] = e^{-2 pi i ry_r ky}.
+__global__ void k_adj_phase(Geom g, const double *__restrict__ ry, int nry, const double2 *__restrict__ twn,
+                            double2 *__restrict__ Pt)
 {
-    __shared__ double2 part[4][kMaxRy + 1];  // per-wave sums (256 threads)
+    const int K = g.K;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nry * K) return;
+    const int r = idx / K, ky = idx - r * K;
+    Pt[idx] = hop_phase(ry[r], ky, K, twn);
+}
+
+// Stage 1, one wave per (kx row, orbital pair): summing the energy groups,
+//   U[kx][r][pair] = sum_ky Hbar(kx, ky) Pt[r][ky]   for the distinct ry,
+//   U[kx][nry][pair] = sum_ky Hbar(kx, ky)            (for the trace)
+// Hacc is [group][pair][k], so a wave reads a contiguous row.
+__global__ void __launch_bounds__(64) k_adj_hrow(Geom g, const double2 *__restrict__ Hacc, int NG,
+                                                 const double2 *__restrict__ Pt, int nry,
+                                                 double2 *__restrict__ U)
+{
     const int kx = blockIdx.x, pr = blockIdx.y;
     const int NP = g.B * g.B, K = g.K;
     const long KK = (long)K * K;
     double2 acc[kMaxRy + 1];
 #pragma unroll
     for (int r = 0; r <= kMaxRy; r++) acc[r] = make_double2(0.0, 0.0);
-    for (int ky = threadIdx.x; ky < K; ky += blockDim.x) {
+    for (int ky = threadIdx.x; ky < K; ky += 64) {
         double2 hv = make_double2(0.0, 0.0);
         for (int gi = 0; gi < NG; gi++) hv = cadd(hv, Hacc[((size_t)gi * NP + pr) * KK + (long)kx * K + ky]);
 #pragma unroll
         for (int r = 0; r < kMaxRy; r++)
-            if (r < nry) acc[r] = cadd(acc[r], cmul(hv, hop_phase(ry[r], ky, K, twn)));
+            if (r < nry) acc[r] = cadd(acc[r], cmul(hv, Pt[r * K + ky]));
         acc[kMaxRy] = cadd(acc[kMaxRy], hv);
     }
-    // wave sums by shuffles, then one barrier and the sum over the 4 waves
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int r = 0; r <= kMaxRy; r++) {
         if (r < nry || r == kMaxRy) {
             const double re = wsum_reduce(acc[r].x), im = wsum_reduce(acc[r].y);
-            if (lane == 0) part[wave][r] = make_double2(re, im);
+            const int slot = r == kMaxRy ? nry : r;
+            if (threadIdx.x == 0) U[((size_t)kx * (nry + 1) + slot) * NP + pr] = make_double2(re, im);
         }
-    }
-    __syncthreads();
-    const int r = threadIdx.x;
-    if (r <= kMaxRy && (r < nry || r == kMaxRy)) {
-        double2 s = part[0][r];
-        for (int wv = 1; wv < 4; wv++) s = cadd(s, part[wv][r]);
-        const int slot = r == kMaxRy ? nry : r;
-        U[((size_t)kx * (nry + 1) + slot) * NP + pr] = s;
     }
 }
 
@@ -815,10 +886,14 @@ __global__ void __launch_bounds__(256) k_adj_hcol(Geom g, const double2 *__restr
     if (threadIdx.x == 0) SR[(size_t)ridx * NP + pr] = s;
 }
 
-// energy groups of the ky adjoint: enough workgroups to fill the chip
+// energy groups of the ky adjoint (row workgroups x groups): enough
+// workgroups to fill the chip.  The Stockham kernel (b <= 3, 64 <= K <= 512)
+// holds a CU with one workgroup, so two workgroups per CU (512) suffice; fewer
+// groups also shrink the Hacc slices written there and read by k_adj_hrow.
 int cqpi_energy_groups(const Geom &g, int E)
 {
-    int ng = 2048 / std::max(g.K, 1);
+    const bool stockham = g.B <= 3 && g.K >= 64 && g.K <= 512;
+    const int ng = (stockham ? 512 : 2048) / std::max(g.K, 1);
     return std::max(1, std::min(E, ng));
 }
 
@@ -862,7 +937,7 @@ int launch_ky_s(const Geom &g, const double *eval, const double2 *evec, const do
     const int NG = cqpi_energy_groups(g, E), EG = (E + NG - 1) / NG;
     int thr = std::max(NP * cqpi::stockham_lanes(N), N);
     thr = std::min((thr + 63) / 64 * 64, kAdjWaveMaxThreads);  // >= N: one thread per k-point
-    const size_t lds = sizeof(double2) * (size_t)NP * (cqpi::padded_len<double2>(N) + N);
+    const size_t lds = sizeof(double2) * (size_t)NP * cqpi::padded_len<double2>(N);
     hipLaunchKernelGGL((k_adj_ky_s<B, N>), dim3(N, NG), dim3(thr), lds, s, g, eval, evec, omegas, E, EG, eta, F1bar,
                        twn, Hacc);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
@@ -897,6 +972,46 @@ int launch_ky_t(const Geom &g, const double *eval, const double2 *evec, const do
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
+// the ky adjoint in one kernel per row for b <= 3: the Stockham kernel
+// (64 <= K <= 512) or the row's b^2 lines and twiddles within 96 KB of LDS;
+// otherwise (and for every b >= 4, whose per-thread accumulators would spill
+// beside the row's lines) the two-kernel path through the Gbar(k) workspace
+bool ky_fused(const Geom &g)
+{
+    const size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
+    return g.B <= 3 && ((g.K >= 64 && g.K <= 512) || (lds <= 96 * 1024 && g.K <= 4 * kThreads));
+}
+
+constexpr int kAdjMaxB = 5;
+constexpr size_t kAdjGkBudget = (size_t)1 << 30;  // Gbar(k) workspace of the two-kernel ky path
+
+template <int B>
+int launch_ky_chunked(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E,
+                      double eta, const double2 *F1bar, const double2 *twn, double2 *Hacc, double2 *Gk,
+                      size_t gk_bytes, hipStream_t s)
+{
+    constexpr int NP = B * B;
+    const long KK = (long)g.K * g.K;
+    const size_t per_e = (size_t)NP * KK * sizeof(double2);
+    if (!Gk || gk_bytes < per_e) return CQPI_E_ARG;
+    const int EBw = (int)std::min<size_t>((size_t)E, gk_bytes / per_e);
+    int npmax = (int)((32 * 1024) / (sizeof(double2) * (size_t)g.K));
+    if (npmax < 1) npmax = 1;
+    const int nch = (NP + npmax - 1) / npmax, np = (NP + nch - 1) / nch;
+    const size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
+    const int NG = cqpi_energy_groups(g, E);
+    const unsigned nkb = (unsigned)((KK + 255) / 256);
+    for (int ea = 0; ea < E; ea += EBw) {
+        const int eb = std::min(EBw, E - ea);
+        hipLaunchKernelGGL(k_adj_ky_lines, dim3(g.K, eb, nch), dim3(kThreads), lds, s, g,
+                           F1bar + (size_t)ea * g.K * g.nw * NP, twn, np, Gk);
+        hipLaunchKernelGGL(k_adj_res_k<B>, dim3(nkb, NG), dim3(256), 0, s, g, eval, evec, omegas + ea, eb, eta,
+                           (const double2 *)Gk, ea > 0 ? 1 : 0, Hacc);
+        if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
+    }
+    return 0;
+}
+
 }  // namespace
 
 namespace cqpi {
@@ -914,9 +1029,18 @@ size_t adjoint_pix_blocks(const Geom &g)
 
 bool adjoint_supported(const Geom &g)
 {
-    bool pow2 = (g.K & (g.K - 1)) == 0;
-    size_t lds = sizeof(double2) * ((size_t)g.B * g.B * g.K + (size_t)(g.K >> 1));
-    return pow2 && g.B <= 4 && g.w <= 4 && g.K <= 4 * kThreads && lds <= 96 * 1024 && win_lds(g) <= 64 * 1024;
+    const bool pow2 = (g.K & (g.K - 1)) == 0;
+    const size_t pix = (size_t)2 * (kCells + 2 * g.w) * (kCells + 2 * g.w) * g.B * sizeof(double2);
+    return pow2 && g.B <= kAdjMaxB && g.w <= 4 && g.K <= 4 * kThreads && pix <= 64 * 1024 &&
+           win_lds(g) <= 64 * 1024;
+}
+
+size_t adjoint_workspace_bytes(const Geom &g, int E)
+{
+    if (ky_fused(g)) return 0;
+    const size_t per_e = (size_t)g.B * g.B * g.K * g.K * sizeof(double2);
+    const size_t eb = std::max<size_t>(1, std::min<size_t>((size_t)E, kAdjGkBudget / per_e));
+    return eb * per_e;
 }
 
 int adjoint_energy_groups(const Geom &g, int E) { return cqpi_energy_groups(g, E); }
@@ -925,7 +1049,8 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
                    const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
                    double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
                    double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
-                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q, hipStream_t s)
+                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q,
+                   double2 *Gk, size_t gk_bytes, hipStream_t s)
 {
     if (!adjoint_supported(g) || nry > kMaxRy) return CQPI_E_ARG;
     int nblk = 0, r = CQPI_E_ARG;
@@ -934,6 +1059,7 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case 2: r = launch_pix_t<2>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
     case 3: r = launch_pix_t<3>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
     case 4: r = launch_pix_t<4>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
+    case 5: r = launch_pix_t<5>(g, G0w, T, tiles, rhobar, E, abar, bbar, Tpart, nblk, s); break;
     }
     if (r) return r;
     const int NP = g.B * g.B;
@@ -943,6 +1069,7 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case 2: r = launch_win_t<2>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
     case 3: r = launch_win_t<3>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
     case 4: r = launch_win_t<4>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
+    case 5: r = launch_win_t<5>(g, tiles, abar, bbar, rhosum, Q, E, Gbar, s); break;
     }
     if (r) return r;
     switch (g.B) {
@@ -950,7 +1077,7 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     case b:                                                                                                       \
         hipLaunchKernelGGL(k_adj_tmatrix<b>, dim3((E + 63) / 64), dim3(64), 0, s, g, G0w, V, T, Tbar, E, Gbar, Vbar); \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4)
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
 #undef CASE
     default: return CQPI_E_ARG;
     }
@@ -961,15 +1088,28 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
     hipLaunchKernelGGL(k_adj_kx, dim3(g.nw, E, nch), dim3(kThreads), lds, s, g, Gbar, twn, np, F1bar);
     if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
-    switch (g.B) {
-    case 1: r = launch_ky_t<1>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
-    case 2: r = launch_ky_t<2>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
-    case 3: r = launch_ky_t<3>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
-    case 4: r = launch_ky_t<4>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+    if (ky_fused(g)) {
+        switch (g.B) {
+        case 1: r = launch_ky_t<1>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+        case 2: r = launch_ky_t<2>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+        case 3: r = launch_ky_t<3>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, s); break;
+        default: r = CQPI_E_ARG;
+        }
+    } else {
+#define CASE(b) \
+    case b: r = launch_ky_chunked<b>(g, eval, evec, omegas, E, eta, F1bar, twn, Hacc, Gk, gk_bytes, s); break;
+        switch (g.B) {
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+        default: r = CQPI_E_ARG;
+        }
+#undef CASE
     }
     if (r) return r;
-    hipLaunchKernelGGL(k_adj_hrow, dim3(g.K, NP), dim3(256), 0, s, g, Hacc, cqpi_energy_groups(g, E), ryvals, nry,
-                       twn, U);
+    // the y phase table follows the K (nry + 1) b^2 sums in U
+    double2 *Pt = U + (size_t)g.K * (nry + 1) * NP;
+    if (nry > 0)
+        hipLaunchKernelGGL(k_adj_phase, dim3((nry * g.K + 255) / 256), dim3(256), 0, s, g, ryvals, nry, twn, Pt);
+    hipLaunchKernelGGL(k_adj_hrow, dim3(g.K, NP), dim3(64), 0, s, g, Hacc, cqpi_energy_groups(g, E), Pt, nry, U);
     hipLaunchKernelGGL(k_adj_hcol, dim3(nRl, NP), dim3(256), 0, s, g, U, nry, Rl, twn, SR);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }

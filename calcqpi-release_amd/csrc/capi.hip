@@ -248,6 +248,12 @@ struct cqpi_ctx {
     size_t c_rhobar = 0, c_rhosum = 0, c_ry = 0, c_Q = 0;
     double2 *d_U = nullptr;
     size_t c_U = 0;
+    // last uploaded hopping vectors and y offsets, and the pinned read-back
+    // buffer of the adjoint's sums
+    std::vector<double2> up_R;
+    std::vector<double> up_ry;
+    double2 *h_grad = nullptr;
+    size_t c_hgrad = 0;
 
     // uploads of the last prepared run (skipped when unchanged) and the pinned
     // status read-back buffer
@@ -290,6 +296,7 @@ void release(cqpi_ctx *c)
     if (c->h_T) (void)hipHostFree(c->h_T);
     if (c->h_g0k) (void)hipHostFree(c->h_g0k);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_grad) (void)hipHostFree(c->h_grad);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
@@ -1671,7 +1678,7 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     if (c->list_mode) return fail(CQPI_E_ARG, "the gradient is of the dense LDOS map, not a threshold-list map");
     const Geom &g = c->g;
     if (!adjoint_supported(g))
-        return fail(CQPI_E_ARG, "the adjoint needs a power-of-two k-grid (<= 1024) and at most 4 bands");
+        return fail(CQPI_E_ARG, "the adjoint needs a power-of-two k-grid (<= 1024), at most 5 bands and window <= 4");
     const int B = g.B, NP = B * B;
     const size_t E = c->nlayers, WW = (size_t)g.W * g.W;
     const int nsets = g.spin ? 2 : 1;
@@ -1716,8 +1723,19 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     const size_t nblk = adjoint_pix_blocks(g);
     const size_t NG = (size_t)adjoint_energy_groups(g, (int)E);
     if (!on_device) RETCHK(dev_grow(c->d_rhobar, c->c_rhobar, E * WW, c->gpumem));
+    // the hopping vectors, y offsets and energies are uploaded only when they
+    // differ from the last upload (the energies normally are the forward run's)
+    auto same2 = [](const std::vector<double2> &a, const std::vector<double2> &b) {
+        if (a.size() != b.size()) return false;
+        for (size_t i = 0; i < a.size(); i++)
+            if (a[i].x != b[i].x || a[i].y != b[i].y) return false;
+        return true;
+    };
+    const bool newR = c->c_Rlist < R.size() || !same2(R, c->up_R);
+    const bool newry = c->c_ry < ryv.size() || ryv != c->up_ry;
+    const bool newom = c->c_omega < E || c->up_omegas != c->last_omegas;
     RETCHK(dev_grow(c->d_ry, c->c_ry, ryv.size(), c->gpumem));
-    RETCHK(dev_grow(c->d_U, c->c_U, (size_t)g.K * (nry + 1) * NP, c->gpumem));
+    RETCHK(dev_grow(c->d_U, c->c_U, (size_t)g.K * (nry + 1) * NP + (size_t)nry * g.K, c->gpumem));
     RETCHK(dev_grow(c->d_abar, c->c_abar, E * nsets * WW * B, c->gpumem));
     RETCHK(dev_grow(c->d_bbar, c->c_bbar, E * nsets * WW * B, c->gpumem));
     RETCHK(dev_grow(c->d_Tpart, c->c_Tpart, E * nblk * NP, c->gpumem));
@@ -1730,22 +1748,46 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     RETCHK(dev_grow(c->d_Rlist, c->c_Rlist, R.size(), c->gpumem));
     RETCHK(dev_grow(c->d_SR, c->c_SR, R.size() * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Q, c->c_Q, adjoint_q_size(g), c->gpumem));
+    // the Gbar(k) workspace of the many-line ky adjoint shares the G(k) workspace
+    const size_t gkb = adjoint_workspace_bytes(g, (int)E);
+    if (gkb) RETCHK(dev_grow(c->d_gkw, c->c_gkw, gkb / sizeof(double2), c->gpumem));
     const double *drb = rhobar;
     if (!on_device) {
         HIPCHK(hipMemcpyAsync(c->d_rhobar, rhobar, E * WW * sizeof(double), hipMemcpyHostToDevice, c->stream));
         drb = c->d_rhobar;
     }
-    HIPCHK(hipMemcpyAsync(c->d_Rlist, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_ry, ryv.data(), ryv.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_omega, c->last_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (newR) {
+        c->up_R = R;
+        HIPCHK(hipMemcpyAsync(c->d_Rlist, c->up_R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (newry) {
+        c->up_ry = ryv;
+        HIPCHK(hipMemcpyAsync(c->d_ry, c->up_ry.data(), ryv.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (newom) {
+        RETCHK(dev_grow(c->d_omega, c->c_omega, E, c->gpumem));
+        c->up_omegas = c->last_omegas;
+        HIPCHK(hipMemcpyAsync(c->d_omega, c->up_omegas.data(), E * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    }
     RETCHK(launch_adjoint(g, c->d_G0w, c->d_T, c->d_V, c->d_tiles, drb, c->d_eval, c->d_evec, c->d_omega, (int)E,
                           c->last_eta, c->d_twn, c->d_ry, nry, c->d_Rlist, (int)R.size(), c->d_abar, c->d_bbar,
                           c->d_Tpart, c->d_Tbar, c->d_rhosum, c->d_Gbar, c->d_Vbar, c->d_F1, c->d_Hacc, c->d_U,
-                          c->d_SR, c->d_Q, c->stream));
-    std::vector<double2> SR(R.size() * NP), Vb(E * NP);
-    HIPCHK(hipMemcpyAsync(SR.data(), c->d_SR, SR.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(Vb.data(), c->d_Vbar, Vb.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+                          c->d_SR, c->d_Q, c->d_gkw, c->c_gkw * sizeof(double2), c->stream));
+    const size_t nsr = R.size() * NP, ngrad = nsr + E * NP;
+    if (c->c_hgrad < ngrad) {
+        if (c->h_grad) (void)hipHostFree(c->h_grad);
+        c->h_grad = nullptr;
+        c->c_hgrad = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_grad, ngrad * sizeof(double2), hipHostMallocDefault));
+        c->c_hgrad = ngrad;
+    }
+    HIPCHK(hipMemcpyAsync(c->h_grad, c->d_SR, nsr * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_grad + nsr, c->d_Vbar, E * NP * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const double2 *SR = c->h_grad, *Vb = c->h_grad + nsr;
     // (the adjoint reused the F1 buffer; the kept windows and T stay valid)
     if (g_hops)
         for (size_t i = 0; i < c->h_hops.size(); i++) {

@@ -206,13 +206,18 @@ size_t adjoint_pix_blocks(const Geom &g);
 size_t adjoint_q_size(const Geom &g);
 bool adjoint_supported(const Geom &g);
 int adjoint_energy_groups(const Geom &g, int E);
+// bytes of the Gbar(k) workspace the ky adjoint needs for E energies (0 when
+// each row's lines fit one workgroup); launch_adjoint takes it as Gk
+size_t adjoint_workspace_bytes(const Geom &g, int E);
 // Rl[r] = (rx, index into ryvals) of each distinct hopping vector, plus a last
-// entry (0, nry) for the plain k sums; SR[r][pair] receives the sums
+// entry (0, nry) for the plain k sums; SR[r][pair] receives the sums; U holds
+// K (nry + 1) b^2 + nry K values (the kx-row sums, then the y phase table)
 int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const double2 *V, const double *tiles,
                    const double *rhobar, const double *eval, const double2 *evec, const double *omegas, int E,
                    double eta, const double2 *twn, const double *ryvals, int nry, const double2 *Rl, int nRl,
                    double2 *abar, double2 *bbar, double2 *Tpart, double2 *Tbar, double *rhosum, double2 *Gbar,
-                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q, hipStream_t s);
+                   double2 *Vbar, double2 *F1bar, double2 *Hacc, double2 *U, double2 *SR, double *Q,
+                   double2 *Gk, size_t gk_bytes, hipStream_t s);
 // contraction: ldos[e][x + y*W].  With tips != NULL (device, one per energy)
 // the Josephson map instead: particle-hole column o + maxband, bulk the
 // complex periodic table of launch_bulk_josephson (needs 2 maxband <= b)

@@ -366,8 +366,10 @@ int cqpi_set_keep_windows(cqpi_ctx *ctx, int on);
  *   *g_mu = dL/defermi;
  *   g_V (b x b complex) with dL = Re sum conj(g_V) dV, i.e. dL/dRe V_ij =
  *     Re g_V_ij and dL/dIm V_ij = Im g_V_ij.
- * Any output pointer may be NULL.  Needs a power-of-two k-grid, bands <= 4,
- * window <= 4. */
+ * Any output pointer may be NULL.  Needs a power-of-two k-grid (<= 1024),
+ * bands <= 5, window <= 4.  Where a k row's b^2 lines exceed one workgroup's
+ * LDS (b >= 4 with K >= 256, or K = 1024) the call grows the G(k) workspace
+ * by a block of energies of Gbar(k), at most 1 GiB. */
 int cqpi_gradient(cqpi_ctx *ctx, const double *rhobar, double *g_hops, double *g_mu, double *g_V);
 
 /* As cqpi_gradient with rhobar in device memory (e.g. computed by the caller

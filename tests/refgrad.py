@@ -22,17 +22,21 @@ def ldos(tre, tim, mu, Vre, Vim, hops, b, K, n, w, os_, omegas, eta, tiles):
     k = kgrid(K)
     KX, KY = np.meshgrid(k, k, indexing="ij")
     nh = len(hops)
-    # one-hot phase tensor P[h, kx, ky, o1, o2]
-    P = np.zeros((nh, K, K, b, b), np.complex128)
+    # phase of each hopping P[h, kx, ky], added into its orbital pair o1*b + o2
+    P = np.empty((nh, K, K), np.complex128)
+    pair = np.empty(nh, np.int64)
     for i, hp in enumerate(hops):
         arg = 2 * np.pi * (hp["rx"] * KX + hp["ry"] * KY)
         if hp["anti"]:
             arg = -arg
-        P[i, :, :, hp["o1"], hp["o2"]] = np.exp(1j * arg)
+        P[i] = np.exp(1j * arg)
+        pair[i] = hp["o1"] * b + hp["o2"]
     P = torch.from_numpy(P)
     t = torch.complex(tre, tim)
     eye = torch.eye(b, dtype=torch.complex128)
-    H = torch.einsum("h,hxyab->xyab", t, P) - torch.complex(mu, torch.zeros_like(mu)) * eye
+    Hp = torch.zeros((K, K, b * b), dtype=torch.complex128).index_add(
+        2, torch.from_numpy(pair), (t[:, None, None] * P).permute(1, 2, 0))
+    H = Hp.reshape(K, K, b, b) - torch.complex(mu, torch.zeros_like(mu)) * eye
     V = torch.complex(Vre, Vim)
     h = max((n >> 1) + w, 2 * w)
     nw = 2 * h + 1

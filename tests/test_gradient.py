@@ -103,6 +103,47 @@ def test_gradient_matches_torch_autograd(gpu, E):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("b,K", [(4, 32), (5, 32), (5, 256), (2, 1024)])
+def test_gradient_multiband_matches_torch_autograd(gpu, b, K):
+    # 4 and 5 bands take the two-kernel ky adjoint through the Gbar(k)
+    # workspace; 2 bands at K = 1024 the one-kernel path with 4 k-points a thread
+    wl = synth.square_norb(b, K=K, n=8, E=2, seed=43 + b, w=2, os_=4, elo=-0.05, ehi=0.05, eta=0.01)
+    wl.efermi = 0.02
+    target = synth.perturbed(wl)
+    print("case: gradient, %d-orbital square lattice K=%d n=8 E=2" % (b, K), flush=True)
+    tgt = oracle.cldos_run(target)
+    with cqpi.engine_for(wl) as q:
+        _, (gh, gmu, gV) = _gpu_loss_and_grad(q, wl, tgt)
+    p = _torch_params(wl)
+    L = ((_torch_rho(p, wl) - torch.from_numpy(tgt)) ** 2).sum()
+    L.backward()
+    ref_h = np.stack([p[0].grad.numpy(), p[1].grad.numpy()], axis=1)
+    assert np.linalg.norm(gh - ref_h) < 1e-9 * np.linalg.norm(ref_h)
+    assert abs(gmu - float(p[2].grad)) < 1e-9 * max(1.0, abs(float(p[2].grad)))
+    ref_V = p[3].grad.numpy() + 1j * p[4].grad.numpy()
+    assert np.linalg.norm(gV - ref_V) < 1e-9 * np.linalg.norm(ref_V)
+
+
+@pytest.mark.gpu
+def test_gradient_energy_blocks_add_up(gpu):
+    # 3 bands at K = 1024: the two-kernel ky adjoint, whose 1 GiB workspace
+    # holds 7 energies of Gbar(k); 8 energies run as blocks of 7 and 1, whose
+    # sums must equal the gradients of the two energy subsets run apart
+    wl = synth.square_norb(3, K=1024, n=8, E=8, seed=47, w=2, os_=4, elo=-0.05, ehi=0.05, eta=0.01)
+    print("case: gradient energy blocks, 3-orbital K=1024 n=8 E=8", flush=True)
+    tgt = np.zeros((8, wl.W * wl.W))  # L = sum rho^2
+    parts = []
+    with cqpi.engine_for(wl) as q:
+        for sl in (slice(0, 8), slice(0, 7), slice(7, 8)):
+            w2 = synth.Workload(**{**wl.__dict__, "energies": wl.energies[sl]})
+            parts.append(_gpu_loss_and_grad(q, w2, tgt[sl])[1])
+    full, a, b = parts
+    for i in range(3):
+        ref = a[i] + b[i]
+        assert np.linalg.norm(np.asarray(full[i] - ref)) <= 1e-10 * np.linalg.norm(np.asarray(ref)), i
+
+
+@pytest.mark.gpu
 def test_gradient_matches_central_differences(gpu):
     wl, target = _job()
     print("case: gradient vs finite differences, K=32 n=8 E=3", flush=True)

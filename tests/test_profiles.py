@@ -39,7 +39,7 @@ def test_bench_reads_measured_traffic():
     w32 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f32")]
     w64 = [r["write_bytes"] for k, r in rows.items() if k.startswith("k_res_ky_s<f64")]
     assert w32 and w64 and w32[0] < 0.6 * w64[0]   # FP32 rows: half the bytes written
-    assert bench.measured_traffic("k_cmix_areg", "mixed")[0] > 0
+    assert bench.measured_traffic("k_cmix_tf", "mixed")[0] > 0
     assert bench.measured_traffic("no_such_kernel", "mixed") is None
 
 
