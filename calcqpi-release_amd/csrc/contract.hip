@@ -375,23 +375,32 @@ int launch_c32(const Geom &g, const float2 *G0f, const float2 *Tf, const double 
 // reads the dense table coef[k][o2][o1][pix]).  The list selects arbitrary (R1, o1, R2, o2)
 // pairs, so the sum does not factorise as the dense map does: per pair the
 // lane forms C = G0(pos+R2) Y with Y = T G0(-pos-R1) kept from its R1.
-// Lanes are the 8 x 8 cells, the waves split the R1 values, kListPx
-// sub-pixels are accumulated per pass and summed over the waves in LDS.
-// The periodic part (position independent) comes from k_bulk_list.
+// Lanes are the 8 x 8 cells, the waves split the R1 values, the sub-pixels
+// are accumulated in registers (at most kListPx per pass) and summed over the
+// waves in LDS.  The periodic part (position independent) comes from
+// k_bulk_list.
 // ---------------------------------------------------------------------------
 constexpr int kListPx = 16;
 
-template <int B>
-__global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const double2 *__restrict__ G0w,
-                                                             const double2 *__restrict__ Tm,
-                                                             const double *__restrict__ bulk,
-                                                             const int *__restrict__ coff,
-                                                             const int *__restrict__ co2,
-                                                             const double *__restrict__ ccoef,
-                                                             const int *__restrict__ r1off,
-                                                             const int *__restrict__ lpair, double *__restrict__ ldos)
+// FP64 for B <= 5 and os <= 4, the loads regrouped: per active
+// pair the lane loads its b x b block G0(pos+R2) at once (independent loads)
+// and forms, per o1 present, Im (G0(pos+R2) Y)_{o2,o1} for every o2; a combo
+// then selects its o2, so no load waits on the combo's orbital.  The combo
+// factors are uniform (scalar) loads of a compile-time count NPIX = os^2, so
+// that they issue together (a run-time bound puts a branch, and a wait,
+// before each).
+template <int B, int NPIX>
+__global__ void __launch_bounds__(64 * kWaves) k_contract_list_s(Geom g, const double2 *__restrict__ G0w,
+                                                               const double2 *__restrict__ Tm,
+                                                               const double *__restrict__ bulk,
+                                                               const int *__restrict__ coff,
+                                                               const int *__restrict__ co2,
+                                                               const double *__restrict__ ccoef,
+                                                               const int *__restrict__ r1off,
+                                                               const int *__restrict__ lpair,
+                                                               double *__restrict__ ldos)
 {
-    __shared__ double red[kWaves - 1][kListPx][64];
+    __shared__ double red[kWaves - 1][NPIX][64];
     constexpr int NP = B * B;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -400,72 +409,77 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const dou
     const int e = blockIdx.z;
     const bool cell_ok = cx < g.n && cy < g.n;
     const int os = g.os, w = g.w, nw = g.nw, h = g.h, W = g.W;
-    const int side = 2 * w + 1, NR = side * side, npix = os * os;
+    const int side = 2 * w + 1, NR = side * side;
     const int n2 = g.n >> 1;
-    // cells past the map edge compute with cell 0 (in the window) and store nothing
     const int px = (cell_ok ? cx : 0) - n2, py = (cell_ok ? cy : 0) - n2;
     const double2 *G0 = G0w + (size_t)e * nw * nw * NP;
     const double2 *Te = Tm + (size_t)e * NP;
     double *out = ldos + (size_t)e * W * W;
 
-    for (int pc = 0; pc < npix; pc += kListPx) {
-        double val[kListPx];
+    double val[NPIX];
 #pragma unroll
-        for (int q = 0; q < kListPx; q++) val[q] = 0.0;
-        for (int r1 = wave; r1 < NR; r1 += kWaves) {
-            const int rx1 = r1 / side - w, ry1 = r1 % side - w;
-            const double2 *GB = G0 + ((size_t)(-px - rx1 + h) * nw + (-py - ry1 + h)) * NP;
-            double2 Y[NP];  // T G0(-pos-R1)
+    for (int q = 0; q < NPIX; q++) val[q] = 0.0;
+    for (int r1 = wave; r1 < NR; r1 += kWaves) {
+        const int k0 = r1off[r1], k1 = r1off[r1 + 1];
+        if (k0 == k1) continue;
+        const int rx1 = r1 / side - w, ry1 = r1 % side - w;
+        const double2 *GB = G0 + ((size_t)(-px - rx1 + h) * nw + (-py - ry1 + h)) * NP;
+        double2 Y[NP];  // T G0(-pos-R1)
 #pragma unroll
-            for (int q = 0; q < B; q++)
+        for (int q = 0; q < B; q++)
 #pragma unroll
-                for (int o1 = 0; o1 < B; o1++) {
-                    double2 acc = make_double2(0.0, 0.0);
+            for (int o1 = 0; o1 < B; o1++) {
+                double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-                    for (int p = 0; p < B; p++) acc = cadd(acc, cmul(Te[q * B + p], GB[p * B + o1]));
-                    Y[q * B + o1] = acc;
+                for (int p = 0; p < B; p++) acc = cadd(acc, cmul(Te[q * B + p], GB[p * B + o1]));
+                Y[q * B + o1] = acc;
+            }
+        for (int k = k0; k < k1; k++) {
+            const int r2 = lpair[k] - r1 * NR;
+            const int rx2 = r2 / side - w, ry2 = r2 % side - w;
+            const double2 *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
+            double2 ga[NP];
+#pragma unroll
+            for (int t = 0; t < NP; t++) ga[t] = GA[t];
+#pragma unroll
+            for (int o1 = 0; o1 < B; o1++) {
+                const int c0 = coff[k * B + o1], c1 = coff[k * B + o1 + 1];
+                if (c0 == c1) continue;
+                double cimv[B];  // Im (G0(pos+R2) Y)_{o2,o1} for every o2
+#pragma unroll
+                for (int o2 = 0; o2 < B; o2++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; q++)
+                        acc += ga[o2 * B + q].x * Y[q * B + o1].y + ga[o2 * B + q].y * Y[q * B + o1].x;
+                    cimv[o2] = acc;
                 }
-            const int k1 = r1off[r1 + 1];
-            for (int k = r1off[r1]; k < k1; k++) {
-                const int r2 = lpair[k] - r1 * NR;
-                const int rx2 = r2 / side - w, ry2 = r2 % side - w;
-                const double2 *GA = G0 + ((size_t)(px + rx2 + h) * nw + (py + ry2 + h)) * NP;
-                // only the (o2, o1) present in the pair's entries
+                for (int cb = c0; cb < c1; cb++) {
+                    const int o2 = co2[cb];
+                    double cim = cimv[0];
 #pragma unroll
-                for (int o1 = 0; o1 < B; o1++) {
-                    const int c1 = coff[k * B + o1 + 1];
-                    for (int cb = coff[k * B + o1]; cb < c1; cb++) {
-                        const int o2 = co2[cb];
-                        double cim = 0.0;  // Im (G0(pos+R2) Y)_{o2,o1}
+                    for (int o = 1; o < B; o++)
+                        if (o2 == o) cim = cimv[o];
+                    const double *cp = ccoef + (size_t)cb * NPIX;
 #pragma unroll
-                        for (int q = 0; q < B; q++) {
-                            const double2 a = GA[o2 * B + q];
-                            cim += a.x * Y[q * B + o1].y + a.y * Y[q * B + o1].x;
-                        }
-                        const double *cp = ccoef + (size_t)cb * npix + pc;
-#pragma unroll
-                        for (int q = 0; q < kListPx; q++)
-                            if (pc + q < npix) val[q] += cp[q] * cim;
-                    }
+                    for (int q = 0; q < NPIX; q++) val[q] += cp[q] * cim;
                 }
             }
         }
-        __syncthreads();  // the previous pass's partial sums are read
-        if (wave > 0)
+    }
+    __syncthreads();
+    if (wave > 0)
 #pragma unroll
-            for (int q = 0; q < kListPx; q++) red[wave - 1][q][lane] = val[q];
-        __syncthreads();
-        if (wave == 0 && cell_ok) {
+        for (int q = 0; q < NPIX; q++) red[wave - 1][q][lane] = val[q];
+    __syncthreads();
+    if (wave == 0 && cell_ok) {
 #pragma unroll
-            for (int q = 0; q < kListPx; q++) {
-                const int pix = pc + q;
-                if (pix >= npix) break;
-                double v = val[q];
+        for (int q = 0; q < NPIX; q++) {
+            double v = val[q];
 #pragma unroll
-                for (int v2 = 0; v2 < kWaves - 1; v2++) v += red[v2][q][lane];
-                const int ii = pix / os, jj = pix - ii * os;
-                out[(size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W] = -(v + bulk[(size_t)e * npix + pix]);
-            }
+            for (int v2 = 0; v2 < kWaves - 1; v2++) v += red[v2][q][lane];
+            const int ii = q / os, jj = q - ii * os;
+            out[(size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W] = -(v + bulk[(size_t)e * NPIX + q]);
         }
     }
 }
@@ -476,7 +490,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list(Geom g, const dou
 // with b^2; the row G0(pos+R2)_{o2,.} is then read once per o1 (from cache).
 constexpr int kListMaxB = 16;
 
-template <typename C>
+// NPX > 0: os^2 = NPX <= 16 sub-pixels, the factor loads of a combo
+// unconditional (they issue together); 0: any os, in passes of kListPx
+template <typename C, int NPX>
 __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C *__restrict__ G0w,
                                                                const C *__restrict__ Tm,
                                                                const double *__restrict__ bulk,
@@ -487,7 +503,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
                                                                const int *__restrict__ lpair, double *__restrict__ ldos)
 {
     using Re = decltype(C{}.x);
-    __shared__ double red[kWaves - 1][kListPx][64];
+    constexpr int PX = NPX > 0 ? NPX : kListPx;  // sub-pixels per pass
+    __shared__ double red[kWaves - 1][PX][64];
     const int B = g.B, NP = B * B;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -504,10 +521,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
     const C *Te = Tm + (size_t)e * NP;
     double *out = ldos + (size_t)e * W * W;
 
-    for (int pc = 0; pc < npix; pc += kListPx) {
-        double val[kListPx];
+    for (int pc = 0; pc < npix; pc += PX) {
+        double val[PX];
 #pragma unroll
-        for (int q = 0; q < kListPx; q++) val[q] = 0.0;
+        for (int q = 0; q < PX; q++) val[q] = 0.0;
         for (int r1 = wave; r1 < NR; r1 += kWaves) {
             const int rx1 = r1 / side - w, ry1 = r1 % side - w;
             const C *GB = G0 + ((size_t)(-px - rx1 + h) * nw + (-py - ry1 + h)) * NP;
@@ -543,10 +560,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
                                 const C a = GA[o2 * B + q];
                                 cim += a.x * Y[q].y + a.y * Y[q].x;
                             }
-                        const double *cp = ccoef + (size_t)cb * npix + pc;
+                        const double *cp = ccoef + (size_t)cb * (NPX > 0 ? NPX : npix) + pc;
 #pragma unroll
-                        for (int q = 0; q < kListPx; q++)
-                            if (pc + q < npix) val[q] += cp[q] * (double)cim;
+                        for (int q = 0; q < PX; q++)
+                            if (NPX > 0 || pc + q < npix) val[q] += cp[q] * (double)cim;
                     }
                 }
             }
@@ -554,11 +571,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_contract_list_g(Geom g, const C
         __syncthreads();  // the previous pass's partial sums are read
         if (wave > 0)
 #pragma unroll
-            for (int q = 0; q < kListPx; q++) red[wave - 1][q][lane] = val[q];
+            for (int q = 0; q < PX; q++) red[wave - 1][q][lane] = val[q];
         __syncthreads();
         if (wave == 0 && cell_ok) {
 #pragma unroll
-            for (int q = 0; q < kListPx; q++) {
+            for (int q = 0; q < PX; q++) {
                 const int pix = pc + q;
                 if (pix >= npix) break;
                 double v = val[q];
@@ -600,6 +617,20 @@ __global__ void __launch_bounds__(kListBulkThreads) k_bulk_list(Geom g, const do
     if (threadIdx.x == 0) bulk[(size_t)e * npix + pix] = red[0];
 }
 
+template <typename C>
+void list_g(const Geom &g, int nb, int E, const C *G0w, const C *T, const double *bulk, const int *coff,
+            const int *co2, const double *ccoef, const int *r1off, const int *lpair, double *ldos, hipStream_t s)
+{
+    const dim3 grid(nb, nb, E), blk(64 * kWaves);
+    switch (g.os) {
+    case 1: hipLaunchKernelGGL((k_contract_list_g<C, 1>), grid, blk, 0, s, g, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos); break;
+    case 2: hipLaunchKernelGGL((k_contract_list_g<C, 4>), grid, blk, 0, s, g, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos); break;
+    case 3: hipLaunchKernelGGL((k_contract_list_g<C, 9>), grid, blk, 0, s, g, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos); break;
+    case 4: hipLaunchKernelGGL((k_contract_list_g<C, 16>), grid, blk, 0, s, g, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos); break;
+    default: hipLaunchKernelGGL((k_contract_list_g<C, 0>), grid, blk, 0, s, g, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos); break;
+    }
+}
+
 }  // namespace
 
 namespace cqpi {
@@ -633,25 +664,31 @@ int launch_contract_list(const Geom &g, const double2 *G0w, const double2 *T, co
         const size_t na = (size_t)E * g.nw * g.nw * g.B * g.B, nt = (size_t)E * g.B * g.B;
         hipLaunchKernelGGL(k_narrow, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, G0w, na, T, nt, G0f, Tf);
         if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
-        hipLaunchKernelGGL(k_contract_list_g<float2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g,
-                           (const float2 *)G0f, (const float2 *)Tf, bulk, coff, co2, ccoef, r1off, lpair, ldos);
+        list_g<float2>(g, nb, E, (const float2 *)G0f, (const float2 *)Tf, bulk, coff, co2, ccoef, r1off, lpair,
+                       ldos, s);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
     if (g.B > 5) {
-        hipLaunchKernelGGL(k_contract_list_g<double2>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk,
-                           coff, co2, ccoef, r1off, lpair, ldos);
+        list_g<double2>(g, nb, E, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos, s);
         return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
-    switch (g.B) {
-#define CASE(b)                                                                                                    \
-    case b:                                                                                                        \
-        hipLaunchKernelGGL(k_contract_list<b>, dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T, bulk, coff,    \
-                           co2, ccoef, r1off, lpair, ldos);                                                        \
+    if (g.os <= 4) {
+        switch (g.B * 8 + g.os) {
+#define CASE(b, o)                                                                                                 \
+    case b * 8 + o:                                                                                                \
+        hipLaunchKernelGGL((k_contract_list_s<b, o * o>), dim3(nb, nb, E), dim3(64 * kWaves), 0, s, g, G0w, T,     \
+                           bulk, coff, co2, ccoef, r1off, lpair, ldos);                                            \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#define CASES(b) CASE(b, 1) CASE(b, 2) CASE(b, 3) CASE(b, 4)
+            CASES(1) CASES(2) CASES(3) CASES(4) CASES(5)
+#undef CASES
 #undef CASE
-    default: return CQPI_E_ARG;
+        default: return CQPI_E_ARG;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
     }
+    // more sub-pixels than one pass holds: the generic kernel in passes
+    list_g<double2>(g, nb, E, G0w, T, bulk, coff, co2, ccoef, r1off, lpair, ldos, s);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
