@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
 // FP64 from the FP64 eigenvalues and then rounded).
 // ---------------------------------------------------------------------------
 constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
-constexpr int kMaxRegB = 5;   // band counts whose per-k eigen data stays in registers
+constexpr int kMaxRegB = 6;   // band counts whose per-k eigen data stays in registers
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
 constexpr int kWaveMaxThreads = 512;
 // FP32 lines take up to 16 waves a workgroup, so that every orbital pair of
@@ -848,6 +848,7 @@ int launch_res_sb(const Geom &g, const double *eval, const double2 *evec, const 
     case 3: return launch_res_s<C, 3, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     case 4: return launch_res_s<C, 4, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     case 5: return launch_res_s<C, 5, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
+    case 6: return launch_res_s<C, 6, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     default: return launch_res_s<C, 0, N>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
     }
 }
