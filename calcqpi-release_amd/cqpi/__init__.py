@@ -49,6 +49,8 @@ SIGNATURES = {
     "cqpi_set_josephson": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "cqpi_set_surface": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _sz]),
     "cqpi_set_surface_steps": (C.c_int, [_vp, C.c_int]),
+    "cqpi_set_pair_symmetry": (C.c_int, [_vp, C.c_int]),
+    "cqpi_get_pair_symmetry": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cqpi_orbital_tile": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _sz, _sz, _dp]),
     "cqpi_orbital_range": (C.c_double, [C.c_double, C.c_double]),
@@ -357,6 +359,17 @@ class AmdQPI:
     def set_surface_steps(self, maxit):
         """Step limit of the surface Green's function's decimation (default 500)."""
         _check(lib().cqpi_set_surface_steps(self._ctx, int(maxit)))
+
+    def set_pair_symmetry(self, enable):
+        """Lets runs transform only the orbital pairs r <= c where the model
+        has H(k)^T = H(-k) (default on)."""
+        _check(lib().cqpi_set_pair_symmetry(self._ctx, int(bool(enable))))
+
+    def pair_symmetry(self):
+        """(model, used): the model has H(k)^T = H(-k); the last run used it."""
+        m, u = C.c_int(0), C.c_int(0)
+        _check(lib().cqpi_get_pair_symmetry(self._ctx, C.byref(m), C.byref(u)))
+        return bool(m.value), bool(u.value)
 
     def set_precision(self, mode):
         mode = PRECISIONS.get(mode, mode)

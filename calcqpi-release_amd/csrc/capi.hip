@@ -143,6 +143,11 @@ struct cqpi_ctx {
     int ngroups = 0;
     double efermi = 0.0;
     int scmax = 0;
+    // H(k)^T = H(-k) for the model (cqpi_set_model), and whether the
+    // transform passes may use it (cqpi_set_pair_symmetry)
+    bool tsym_model = false;
+    bool tsym_on = true;
+    bool tsym_used = false;  // the last prepared run used it
     double *d_eval = nullptr;
     size_t c_eval = 0;
     double2 *d_evec = nullptr;
@@ -455,6 +460,7 @@ struct RunPlan {
     bool mixed = false;
     size_t work_bytes = 0;  // G(k) workspace of the resolvent pass (0: not used)
     int rwin = 0;           // half-width of the G0 window part that is written
+    bool tsym = false;      // the transform passes use G0(R)_cr = G0(-R)_rc
 };
 
 // allocations and uploads of a run (synchronous); no kernel launch
@@ -541,6 +547,14 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
                                   c->stream));
         }
     }
+    // the transform passes carry only the pairs r <= c where the model has
+    // H(k)^T = H(-k): on the Stockham path, with G0(k) from the eigen data (not
+    // the decimation workspace) and no fp16 planes to write (their writer
+    // has no mirrored copies)
+    const bool planes_written = plan.mixed && (c->josephson || !mix_tfold(g, c->mg));
+    plan.tsym = c->tsym_model && c->tsym_on && !c->surf_mode && g.B > 1 && transform32_supported(g) &&
+                !planes_written;
+    c->tsym_used = plan.tsym;
     if (c->precision == CQPI_PREC_FP32) {
         RETCHK(dev_grow(c->d_G0f, c->c_G0f, Ec * g.nw * g.nw * NP, c->gpumem));
         RETCHK(dev_grow(c->d_Tf, c->c_Tf, Ec * NP, c->gpumem));
@@ -561,6 +575,10 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
 int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo, hipStream_t s)
 {
     const Geom &g = c->g;
+    // the transform passes' geometry: pairs r <= c only where the model allows
+    // it (prepare_run)
+    Geom gt = g;
+    gt.ts = plan.tsym ? 1 : 0;
     const int NP = g.B * g.B;
     const size_t WW = (size_t)g.W * g.W;
     const size_t f1e = (size_t)g.K * g.nw * NP, we = (size_t)g.nw * g.nw * NP;
@@ -599,18 +617,18 @@ int enqueue_range(cqpi_ctx *c, const RunPlan &plan, size_t e0, int ec, size_t eo
                 RETCHK(launch_resolvent_ky_gk(g, c->d_gkw, om, eb, c->d_tw, F1d + (size_t)ea * f1e, false, s));
         }
     } else if (t32)
-        RETCHK(launch_resolvent_ky32(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f, c->d_gkw,
-                                     plan.work_bytes, s));
+        RETCHK(launch_resolvent_ky32(gt, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw32, F1f,
+                                     c->d_gkw, plan.work_bytes, s));
     else
-        RETCHK(launch_resolvent_ky(g, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, F1d, c->d_gkw,
+        RETCHK(launch_resolvent_ky(gt, c->d_eval, c->d_evec, c->d_omega + e0, ec, plan.eta, c->d_tw, F1d, c->d_gkw,
                                    plan.work_bytes, s));
     stage_end(c, st);
     st = stage_begin(c, 2);
     if (t32)
-        RETCHK(launch_transform_kx32(g, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes,
+        RETCHK(launch_transform_kx32(gt, plan.mixed ? &c->mg : nullptr, F1f, ec, c->d_tw32, plan.sG, G0w, planes,
                                      plan.rwin, s, tf ? reinterpret_cast<float2 *>(planes) : nullptr));
     else
-        RETCHK(launch_transform_kx(g, planes_ready ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
+        RETCHK(launch_transform_kx(gt, planes_ready ? &c->mg : nullptr, F1d, ec, c->d_tw, plan.sG, G0w, planes, s));
     stage_end(c, st);
     st = stage_begin(c, 3);
     // the T-matrix and (but for the threshold list, whose kernel forms it) the
@@ -1443,6 +1461,32 @@ static int upload_surface_groups(cqpi_ctx *c)
     return 0;
 }
 
+// H(k)^T = H(-k) on the k-grid: every hopping block at an integral R has its
+// transpose at -R, exactly (real hoppings of a Hermitian model), no
+// particle-hole term.  Then G0(k)^T = G0(-k) at every energy, and the
+// transform of the pair (r, c) at R is the pair (c, r) at -R.
+static bool transpose_symmetric(const cqpi_hopping *hops, size_t nhops, const std::vector<DevRGroup> &G,
+                                const std::vector<double2> &T, int B, bool scmodel)
+{
+    if (scmodel) return false;
+    for (size_t i = 0; i < nhops; i++)
+        if (hops[i].anti) return false;
+    std::map<std::pair<double, double>, size_t> at;
+    for (size_t i = 0; i < G.size(); i++) {
+        if (!G[i].integral) return false;
+        at.emplace(std::make_pair(G[i].rx, G[i].ry), i);
+    }
+    for (size_t i = 0; i < G.size(); i++) {
+        auto it = at.find(std::make_pair(-G[i].rx + 0.0, -G[i].ry + 0.0));
+        if (it == at.end()) return false;
+        const double2 *a = &T[i * B * B], *m = &T[it->second * B * B];
+        for (int r = 0; r < B; r++)
+            for (int q = 0; q < B; q++)
+                if (!(a[r * B + q].x == m[q * B + r].x && a[r * B + q].y == m[q * B + r].y)) return false;
+    }
+    return true;
+}
+
 int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double efermi, int scmodel)
 {
     g_err.clear();
@@ -1461,6 +1505,7 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     c->ngroups = (int)G.size();
     c->efermi = efermi;
     c->scmax = scmodel ? (B >> 1) : B;
+    c->tsym_model = transpose_symmetric(hops, nhops, G, T, B, scmodel != 0);
     c->has_model = true;
     c->h_hops.assign(hops, hops + nhops);
     c->windows_kept = false;
@@ -1489,6 +1534,23 @@ int cqpi_bands_histogram(cqpi_ctx *c, size_t pix, size_t oversamp, double le, do
     if (!r) r = hipMemcpy(hist, d, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? 0 : CQPI_E_HIP;
     (void)hipFree(d);
     if (r) return fail(r, "band-structure histogram failed");
+    return 0;
+}
+
+int cqpi_set_pair_symmetry(cqpi_ctx *c, int enable)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    c->tsym_on = enable != 0;
+    return 0;
+}
+
+int cqpi_get_pair_symmetry(const cqpi_ctx *c, int *model, int *used)
+{
+    g_err.clear();
+    if (!c) return fail(CQPI_E_ARG, "ctx is NULL");
+    if (model) *model = c->tsym_model ? 1 : 0;
+    if (used) *used = c->tsym_used ? 1 : 0;
     return 0;
 }
 

@@ -73,7 +73,43 @@ struct Geom {
     int eb;       // first transform pass: energies per workgroup
     int js;       // T-folded mixed contraction: waves per cell row (1, 2; 0: per shape)
     int streams;  // eager runs: streams an energy chunk is split over (1, 2; 0: 2 from 8 energies)
+    // transform passes of the Stockham path: 1 where H(k)^T = H(-k) (real
+    // hoppings), so that G0(k)^T = G0(-k) and G0(R)_cr = G0(-R)_rc; the passes
+    // then transform only the pairs r <= c and pass 2 writes each of them
+    // also as (c, r) at -R.  0: every pair is transformed.
+    int ts;
 };
+
+// pairs (lines) that the transform passes carry per (energy, row)
+__host__ __device__ inline int transform_pairs(const Geom &g)
+{
+    return g.ts ? g.B * (g.B + 1) / 2 : g.B * g.B;
+}
+
+// index of the pair (r, c), r <= c, in the row-major list of such pairs
+__host__ __device__ constexpr int upper_index(int r, int c, int B)
+{
+    return r * B - r * (r - 1) / 2 + (c - r);
+}
+
+// orbitals (r, c) of transformed pair q: q = r B + c, or with ts the q-th
+// pair r <= c in row-major order
+__device__ inline void transform_pair_rc(int q, int B, int ts, int &r, int &c)
+{
+    if (!ts) {
+        r = q / B;
+        c = q - r * B;
+        return;
+    }
+    int row = 0, len = B;
+    while (q >= len) {
+        q -= len;
+        row++;
+        len--;
+    }
+    r = row;
+    c = row + q;
+}
 
 // Workgroup barrier that orders LDS accesses only.  __syncthreads also waits
 // for the workgroup's outstanding global stores; where a barrier only guards

@@ -379,8 +379,9 @@ __device__ inline void pairs_from_vectors(EigenPairs<C, B> &P, const double2 *v)
     }
 }
 
-// S[(r*B + c) * NL + pidx(j)] = sum_n f_n P[n]_rc for all B*B pairs
-template <typename C, int B, int NL, bool LP>
+// S[(r*B + c) * NL + pidx(j)] = sum_n f_n P[n]_rc for all B*B pairs, or
+// (UP) S[upper_index(r, c) * NL + pidx(j)] for the pairs r <= c only
+template <typename C, int B, int NL, bool LP, bool UP = false>
 __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &P, const C *f)
 {
     using R = typename EigenPairs<C, B>::R;
@@ -395,7 +396,7 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
         C gz;
         gz.x = gx;
         gz.y = gy;
-        S[(size_t)(r * B + r) * NL + cqpi::pidx<C, LP>(j)] = gz;
+        S[(size_t)(UP ? upper_index(r, r, B) : r * B + r) * NL + cqpi::pidx<C, LP>(j)] = gz;
     }
     int u = 0;
 #pragma unroll
@@ -411,13 +412,18 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
                 cc += f[n].x * P.o[n][u].y;
                 d += f[n].y * P.o[n][u].x;
             }
-            C up, lo;
+            C up;
             up.x = a - b;
             up.y = cc + d;
-            lo.x = a + b;
-            lo.y = d - cc;
-            S[(size_t)(r * B + c) * NL + cqpi::pidx<C, LP>(j)] = up;
-            S[(size_t)(c * B + r) * NL + cqpi::pidx<C, LP>(j)] = lo;
+            if constexpr (UP) {
+                S[(size_t)upper_index(r, c, B) * NL + cqpi::pidx<C, LP>(j)] = up;
+            } else {
+                C lo;
+                lo.x = a + b;
+                lo.y = d - cc;
+                S[(size_t)(r * B + c) * NL + cqpi::pidx<C, LP>(j)] = up;
+                S[(size_t)(c * B + r) * NL + cqpi::pidx<C, LP>(j)] = lo;
+            }
         }
 }
 
@@ -449,8 +455,8 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
         for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
     const C *Tw = sizeof(C) == 8 ? Twl : tw;
     constexpr bool kRegs = res_regs<B, N>();  // blockDim >= N (launcher)
-    const int nb = B > 0 ? B : g.B, NP = nb * nb;
-    const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NP - p0);
+    const int nb = B > 0 ? B : g.B, NP = nb * nb, NT = transform_pairs(g);
+    const int i = blockIdx.x, p0 = blockIdx.z * np, npc = min(np, NT - p0);
     const int e0 = blockIdx.y * EB, e1 = min(E, e0 + EB);
     const long KK = (long)N * N;
     const int nw = g.nw, h = g.h;
@@ -488,22 +494,28 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
                 C f[B];
 #pragma unroll
                 for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
-                if constexpr (allp)
-                    resolvent_all_pairs<C, B, NL, LP>(S, j, pp, f);
-                else {
+                if constexpr (allp) {
+                    if (g.ts)
+                        resolvent_all_pairs<C, B, NL, LP, true>(S, j, pp, f);
+                    else
+                        resolvent_all_pairs<C, B, NL, LP>(S, j, pp, f);
+                } else {
                     // the chunk's pairs p0 .. p0 + npc - 1: every (r, c) unrolled,
                     // so that the eigenvector components are static registers,
                     // with a test on the chunk (uniform over the workgroup);
-                    // G_rc = sum_n (f_n v_rn) conj(v_cn)
+                    // G_rc = sum_n (f_n v_rn) conj(v_cn).  With g.ts the pairs
+                    // are those with r <= c, numbered by upper_index.
+                    const bool ts = g.ts;
 #pragma unroll
                     for (int r = 0; r < B; r++) {
-                        if (r * B + B <= p0 || r * B >= p0 + npc) continue;
+                        const int rlo = ts ? upper_index(r, r, B) : r * B, rhi = ts ? rlo + B - r : r * B + B;
+                        if (rhi <= p0 || rlo >= p0 + npc) continue;
                         C fv[B];
 #pragma unroll
                         for (int n = 0; n < B; n++) fv[n] = cmul(f[n], vv_r[r * B + n]);
 #pragma unroll
                         for (int c = 0; c < B; c++) {
-                            const int pr = r * B + c;
+                            const int pr = ts ? (c >= r ? upper_index(r, c, B) : -1) : r * B + c;
                             if (pr < p0 || pr >= p0 + npc) continue;
                             C acc = cmulc(fv[0], vv_r[c * B]);
 #pragma unroll
@@ -517,17 +529,19 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
         if constexpr (!kRegs) {
             // G(k) of this energy and the chunk's pairs from the workspace
             // (k_gk), one line per pair, read along ky
-            const C *gk = Gk + ((size_t)e * NP + p0) * KK + (size_t)i * N;
+            const C *gk = Gk + (size_t)e * NP * KK + (size_t)i * N;
             for (int t = threadIdx.x; t < npc * N; t += blockDim.x) {
                 const int q = t / N, j = t - q * N;
-                S[(size_t)q * NL + cqpi::pidx<C, LP>(j)] = gk[(size_t)q * KK + j];
+                int r, c;
+                transform_pair_rc(p0 + q, nb, g.ts, r, c);
+                S[(size_t)q * NL + cqpi::pidx<C, LP>(j)] = gk[(size_t)(r * nb + c) * KK + j];
             }
         }
         lds_barrier();
         for (int q = grp; q < npc; q += ngrp) cqpi::wave_fft<N, LP>(S + (size_t)q * NL, Tw, ll, +1);
         lds_barrier();
         // the chunk's part of the (e, kx row) block: one contiguous run
-        C *out = F1 + ((size_t)e * N + i) * nw * NP + (size_t)p0 * nw;
+        C *out = F1 + ((size_t)e * N + i) * nw * NT + (size_t)p0 * nw;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int slot = udiv(t, dm), q = t - slot * npc;
             out[t] = S[(size_t)q * NL + cqpi::pidx<C, LP>((slot - h) & (N - 1))];
@@ -613,6 +627,21 @@ int launch_gk(int B, int N, const double *eval, const double2 *evec, const doubl
     }
 }
 
+// window slots of transformed pair q: pf = r B + c at R, and (ts, r < c)
+// pm = c B + r at -R; pm = -1 where there is no mirrored copy
+__device__ inline void kx_pair_slots(int q, int B, int ts, int &pf, int &pm)
+{
+    pm = -1;
+    if (!ts) {
+        pf = q;
+        return;
+    }
+    int r, c;
+    transform_pair_rc(q, B, ts, r, c);
+    pf = r * B + c;
+    if (r != c) pm = c * B + r;
+}
+
 // pass 2 on the Stockham path: one workgroup per (Ry slot, energy, pair chunk);
 // the window is written in FP64 whatever C is.  FP32 lines are compiled for 7
 // waves per SIMD (72 VGPRs: three 9-wave workgroups a CU; 6 % faster than the
@@ -639,19 +668,21 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
     // neighbouring Ry slots read the same cache lines of F1: keep them on one XCD
     int sy, e, z;
     xcd_block(sy, e, z);
-    const int B = g.B, NP = B * B, p0 = z * np;
-    const int npc = min(np, NP - p0);
+    // NT transformed pairs per (energy, row) of F1 (NP, or with g.ts the
+    // pairs r <= c); the window holds all NP
+    const int B = g.B, NP = B * B, NT = transform_pairs(g), p0 = z * np;
+    const int npc = min(np, NT - p0);
     const int nw = g.nw, h = g.h;
     const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
     // pair p sits in pass-1 chunk c = p / np1, at c np1 nw + slot npc1(c) + p - c np1
     // of its (e, kx row) block
-    const C *in = F1 + (size_t)e * N * nw * NP;
-    if (np1 == NP && npc == NP) {
-        // one pass-1 chunk: the slot's pairs are one run of NP values per kx
+    const C *in = F1 + (size_t)e * N * nw * NT;
+    if (np1 == NT && npc == NT) {
+        // one pass-1 chunk: the slot's pairs are one run of NT values per kx
         // row; a batch of loads is in flight before their LDS stores
         constexpr int U = 8;
-        const int tot = N * NP;
-        const C *src = in + (size_t)sy * NP;
+        const int tot = N * NT;
+        const C *src = in + (size_t)sy * NT;
         for (int t0 = threadIdx.x; t0 < tot; t0 += U * blockDim.x) {
             C v[U];
 #pragma unroll
@@ -659,7 +690,7 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
                 const int t = t0 + u * blockDim.x;
                 if (t < tot) {
                     const int i = udiv(t, dv.npc);
-                    v[u] = src[(size_t)i * nw * NP + (t - i * NP)];
+                    v[u] = src[(size_t)i * nw * NT + (t - i * NT)];
                 }
             }
 #pragma unroll
@@ -667,7 +698,7 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
                 const int t = t0 + u * blockDim.x;
                 if (t < tot) {
                     const int i = udiv(t, dv.npc);
-                    S[(size_t)(t - i * NP) * NL + cqpi::pidx<C>(i)] = v[u];
+                    S[(size_t)(t - i * NT) * NL + cqpi::pidx<C>(i)] = v[u];
                 }
             }
         }
@@ -677,8 +708,8 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
             const int i = udiv(t, dm), q = t - i * npc, p = p0 + q;
             int c0 = 0;
             while (c0 + np1 <= p) c0 += np1;
-            const int npc1 = min(np1, NP - c0);
-            S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * nw * NP + (size_t)c0 * nw + sy * npc1 + (p - c0)];
+            const int npc1 = min(np1, NT - c0);
+            S[(size_t)q * NL + cqpi::pidx<C>(i)] = in[(size_t)i * nw * NT + (size_t)c0 * nw + sy * npc1 + (p - c0)];
         }
     }
     lds_barrier();
@@ -686,13 +717,18 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
     lds_barrier();
     const double inv = 1.0 / ((double)N * (double)N);
     double2 *out = G0w + (size_t)e * nw * nw * NP;
-    // the window in FP64: whole, or (rwin < h) only |R| <= rwin
+    // the window in FP64: whole, or (rwin < h) only |R| <= rwin.  With g.ts
+    // a pair (r, c), r < c, is also G0(-R)_cr: slot (2h - sx, 2h - sy)
     if (abs(sy - h) <= rwin) {
         const int sx0 = h - rwin, nx = 2 * rwin + 1;
         for (int t = threadIdx.x; t < nx * npc; t += blockDim.x) {
             const int sx = sx0 + t / npc, q = t - (sx - sx0) * npc;
             const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
-            out[((size_t)sx * nw + sy) * NP + p0 + q] = make_double2(inv * (double)z.x, inv * (double)z.y);
+            const double2 v = make_double2(inv * (double)z.x, inv * (double)z.y);
+            int pf, pm;
+            kx_pair_slots(p0 + q, B, g.ts, pf, pm);
+            out[((size_t)sx * nw + sy) * NP + pf] = v;
+            if (pm >= 0) out[((size_t)(2 * h - sx) * nw + (2 * h - sy)) * NP + pm] = v;
         }
     }
     if (P) {
@@ -701,14 +737,20 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
         write_planes<C, true>(S, N, sy, e, p0, npc, B, nw, h, mg, sG * inv, P, d2);
     }
     if (W32) {
-        // the whole window row sy in FP32, W32[e][sy][sx][pair] (one contiguous run)
+        // the whole window row sy in FP32, W32[e][sy][sx][pair] (one contiguous
+        // run); with g.ts the pairs r < c also go to row 2h - sy as (c, r)
         const float invf = (float)inv;
         const unsigned long long mq = udiv_magic((unsigned)npc);
-        float2 *ow = W32 + ((size_t)e * nw + sy) * nw * NP + p0;
+        float2 *ow = W32 + ((size_t)e * nw + sy) * nw * NP;
+        float2 *om = W32 + ((size_t)e * nw + (2 * h - sy)) * nw * NP;
         for (int t = threadIdx.x; t < nw * npc; t += blockDim.x) {
             const int sx = udiv(t, mq), q = t - sx * npc;
             const C z = S[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
-            ow[(size_t)sx * NP + q] = make_float2((float)z.x * invf, (float)z.y * invf);
+            const float2 v = make_float2((float)z.x * invf, (float)z.y * invf);
+            int pf, pm;
+            kx_pair_slots(p0 + q, B, g.ts, pf, pm);
+            ow[(size_t)sx * NP + pf] = v;
+            if (pm >= 0) om[(size_t)(2 * h - sx) * NP + pm] = v;
         }
     }
 }
@@ -780,9 +822,9 @@ template <typename C, int B, int N>
 int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const double *omegas, int E, double eta,
                  const C *tw, C *F1, void *work, size_t work_bytes, hipStream_t s)
 {
-    const int NP = g.B * g.B;
-    const int np = wave_pairs(N, NP, sizeof(C));
-    const int nch = (NP + np - 1) / np;
+    const int NP = g.B * g.B, NT = transform_pairs(g);
+    const int np = wave_pairs(N, NT, sizeof(C));
+    const int nch = (NT + np - 1) / np;
     const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     const size_t lds = sizeof(C) * ((size_t)np * res_line_len(N, sizeof(C)) + (sizeof(C) == 8 ? N : 0));
@@ -797,7 +839,7 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
             const int r = launch_gk<C>(g.B, N, eval, evec, omegas + ea, eb, eta, Gk, s);
             if (r) return r;
             hipLaunchKernelGGL((k_res_ky_s<C, B, N>), dim3(N, eb, nch), dim3(thr), lds, s, g, eval, evec,
-                               omegas + ea, eb, 1, eta, tw, np, F1 + (size_t)ea * N * g.nw * NP, (const C *)Gk);
+                               omegas + ea, eb, 1, eta, tw, np, F1 + (size_t)ea * N * g.nw * NT, (const C *)Gk);
             if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
         }
         return 0;
@@ -808,11 +850,15 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     // for 512 rows: 0.265 ms against 0.29 with four, 3 orbitals, K = 512).
     // The energy blocks are made even (51 energies: 4 blocks of 13, not 12,12,12,12,3)
     const int rows = N * nch;
-    const int wg_target = sizeof(C) == 8 ? (nch == 1 && NP > 4 ? 1024 : 4096) : 512;
+    const int wg_target = sizeof(C) == 8 ? (nch == 1 && NT > 4 ? 1024 : 4096) : 512;
     const int nb_want = std::max(1, std::min(E, (wg_target + rows / 2) / rows));
     const int EB = g.eb > 0 ? std::min(E, g.eb) : (E + nb_want - 1) / nb_want;
     const int nb = (E + EB - 1) / EB;
-    if (nch == 1)
+    // one chunk of every pair: G0(k) from the Hermitian pair products, except
+    // for the 10 and 15 pairs r <= c of 4 and 5 bands, whose products do not
+    // fit the registers (the pair products of 5 bands spilled: 0.74 -> 1.40 ms
+    // for 40 energies, K = 512)
+    if (nch == 1 && (B <= 3 || !g.ts))
         hipLaunchKernelGGL((k_res_ky_s<C, B, N, true>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E,
                            EB, eta, tw, np, F1, (const C *)nullptr);
     else
@@ -826,9 +872,9 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
 template <typename C, int N>
 int launch_res_gk(const Geom &g, const C *Gk, const double *omegas, int E, const C *tw, C *F1, hipStream_t s)
 {
-    const int NP = g.B * g.B;
-    const int np = wave_pairs(N, NP, sizeof(C));
-    const int nch = (NP + np - 1) / np;
+    const int NT = transform_pairs(g);
+    const int np = wave_pairs(N, NT, sizeof(C));
+    const int nch = (NT + np - 1) / np;
     const int thr = wave_threads(N, np, true, sizeof(C));
     if (thr / cqpi::stockham_lanes(N) < 1) return CQPI_E_ARG;
     const size_t lds = sizeof(C) * ((size_t)np * res_line_len(N, sizeof(C)) + (sizeof(C) == 8 ? N : 0));
@@ -866,8 +912,10 @@ template <typename C, int N>
 int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw, double sG, double2 *G0w,
                 _Float16 *P, int rwin, hipStream_t s, float2 *W32 = nullptr)
 {
-    const int NP = g.B * g.B, np = kx_pairs(N, NP, sizeof(C)), nch = (NP + np - 1) / np;
-    const int np1 = wave_pairs(N, NP, sizeof(C));  // pass 1's chunks fix the F1 layout
+    // pairs with planes are only written without the pair symmetry
+    if (g.ts && P) return CQPI_E_ARG;
+    const int NT = transform_pairs(g), np = kx_pairs(N, NT, sizeof(C)), nch = (NT + np - 1) / np;
+    const int np1 = wave_pairs(N, NT, sizeof(C));  // pass 1's chunks fix the F1 layout
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
     rwin = std::min(std::max(rwin, 0), g.h);
@@ -924,7 +972,8 @@ int launch_resolvent_ky(const Geom &g, const double *eval, const double2 *evec, 
         case 1024: return launch_res_sb<double2, 1024>(g, eval, evec, omegas, E, eta, tw, F1, work, work_bytes, s);
         }
     }
-    if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
+    // the LDS-line passes transform every pair
+    if ((size_t)g.K * sizeof(double2) > kLdsBudget || g.ts) return CQPI_E_ARG;
     switch (g.B) {
 #define CASE(b) \
     case b: return launch_res_t<b>(g, eval, evec, omegas, E, eta, tw, F1, s);
@@ -940,7 +989,7 @@ int launch_resolvent_ky_gk(const Geom &g, const void *Gk, const double *omegas, 
 {
     if (!wave_path(g.K)) {
         // other grids: the LDS-line pass (FP64 only)
-        if (fp32 || (size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
+        if (fp32 || (size_t)g.K * sizeof(double2) > kLdsBudget || g.ts) return CQPI_E_ARG;
         const double2 *gk = static_cast<const double2 *>(Gk);
         const double2 *t = static_cast<const double2 *>(tw);
         double2 *f = static_cast<double2 *>(F1);
@@ -1019,7 +1068,7 @@ int launch_transform_kx(const Geom &g, const MixGeom *mg, const double2 *F1, int
         case 1024: return launch_kx_s<double2, 1024>(g, m, F1, E, tw, sG, G0w, Pm, g.h, s);
         }
     }
-    if ((size_t)g.K * sizeof(double2) > kLdsBudget) return CQPI_E_ARG;
+    if ((size_t)g.K * sizeof(double2) > kLdsBudget || g.ts) return CQPI_E_ARG;
     int NP = g.B * g.B;
     int np = pairs_per_chunk(g.K, NP);
     int nch = (NP + np - 1) / np;

@@ -336,6 +336,21 @@ int cqpi_set_surface(cqpi_ctx *ctx, int mode, double epserr, const cqpi_hopping 
  * which some k-point needs more steps returns CQPI_E_CONVERGE. */
 int cqpi_set_surface_steps(cqpi_ctx *ctx, int maxit);
 
+/* Pair symmetry of the k->R transform (no counterpart in the reference, which
+ * transforms every orbital pair, src/qpi.cc:557-578).  Where every hopping
+ * block at an integral R has its exact transpose at -R and there is no
+ * particle-hole term (real hoppings of a Hermitian model), H(k)^T = H(-k), so
+ * G0(k)^T = G0(-k) and G0(R)_cr = G0(-R)_rc: the transform passes of
+ * cqpi_run then carry only the b(b+1)/2 pairs r <= c and write each pair
+ * also as (c, r) at -R.  cqpi_set_model detects the property; enable (default
+ * 1) lets runs use it (power-of-two k-grids from 64 to 1024, G0 from the
+ * eigen data, and in mixed mode the T-folded contraction).  The maps agree
+ * with those of enable = 0 to rounding.  cqpi_get_pair_symmetry: model = 1
+ * where the model has the property, used = 1 where the last run used it
+ * (either pointer may be NULL). */
+int cqpi_set_pair_symmetry(cqpi_ctx *ctx, int enable);
+int cqpi_get_pair_symmetry(const cqpi_ctx *ctx, int *model, int *used);
+
 /* For a context of cqpi_create_spectral: mode 1 (SPF) or 2 (USPF) makes
  * cqpi_run compute those maps (n x n per energy) from the device's eigen
  * data instead of host G0(k) (output=spf|uspf of the reference driver,
