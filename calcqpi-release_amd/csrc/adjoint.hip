@@ -242,44 +242,47 @@ constexpr int kWinTY = 8;   // slots per workgroup along y
 constexpr int kWinPix = 4;  // sub-pixel rows staged per phase
 constexpr int kStageCols = kWinTX * kWinTY / kWinPix;  // cell columns staged at once
 
-// Gbar on the slot window, one thread per slot holding all b x b entries:
-//   Gbar(S)_{o,p} += sum_R sum_x w_o(x;R) abar^{s(o)}_p(cell S-R, x)      (a gather transposed)
-//   Gbar(S)_{p,o} += sum_R sum_x w_o(x;R) bbar^{s(o)}_p(cell -S-R, x)     (b gather transposed)
-//   Gbar(S)_{r,c} += -i sum_x rhosum(x) Q[r,c][S][x]                      (periodic term)
-// with cells relative to the map centre (cell index = position + n/2) and
-// s(o) the spin block of orbital o.  A workgroup owns kWinTX x kWinTY slots;
-// the adjoints of the cells they reach (the tile plus a w halo, and its mirror
-// image for b) are staged in LDS one strip of kWinPix x 1 sub-pixels at a
-// time, so each adjoint value is read from memory about once per tile.
-// Writes every slot of the window (plain stores, no accumulation).
+// Gbar on the slot window, one thread per slot, split over workgroups by
+// gather and band p (a workgroup per energy, gather and p; 2 waves each, so
+// that the launch has about 6b times the waves of one holding every b x b
+// entry of a slot, which ran at under two waves per SIMD):
+//   gather a: Gbar_a(S)_{o,p} = sum_R sum_x w_o(x;R) abar^{s(o)}_p(cell S-R, x)
+//                               - i sum_x rhosum(x) Q[o,p][S][x]   (periodic term)
+//   gather b: Gbar_b(S)_{p,o} = sum_R sum_x w_o(x;R) bbar^{s(o)}_p(cell -S-R, x)
+// Gbar = Gbar_a + Gbar_b (k_adj_kx reads the sum).  Cells are relative to the
+// map centre (cell index = position + n/2), s(o) is the spin block of orbital
+// o.  A workgroup owns kWinTX x kWinTY slots; the adjoints of the cells they
+// reach (the tile plus a w halo; for b its mirror image) are staged in LDS
+// one strip of kWinPix x 1 sub-pixels at a time, so each adjoint value is
+// read from memory about once per tile.  Every entry of both halves is
+// written by exactly one workgroup (plain stores, no accumulation).
 template <int B>
 __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double *__restrict__ tiles,
                                                             const double2 *__restrict__ abar,
                                                             const double2 *__restrict__ bbar,
                                                             const double *__restrict__ rhosum,
                                                             const double *__restrict__ Q,
-                                                            double2 *__restrict__ Gbar)
+                                                            double2 *__restrict__ Gbar_a,
+                                                            double2 *__restrict__ Gbar_b)
 {
-    extern __shared__ double2 stage[];  // [a|b][kWinPix][RY][RX] (planes padded to RC), weights [R][kWinPix][B]
+    extern __shared__ double2 stage[];  // [kWinPix][RY][RX] (planes padded to RC), weights [R][kWinPix][B]
     const int os = g.os, w = g.w, T = g.T, W = g.W, n = g.n, nw = g.nw, h = g.h, n2 = n >> 1;
     // RC: stride of the staged sub-pixel planes, padded by two values so the
     // planes of one cell start on different LDS banks
     const int RX = kWinTX + 2 * w, RY = kWinTY + 2 * w, RC = RX * RY + 2;
     const int side = 2 * w + 1;
-    double *wl = reinterpret_cast<double *>(stage + 2 * kWinPix * RC);
+    double *wl = reinterpret_cast<double *>(stage + kWinPix * RC);
     const int tx = threadIdx.x % kWinTX, ty = threadIdx.x / kWinTX;
     const int S0x = (int)blockIdx.x * kWinTX - h, S0y = (int)blockIdx.y * kWinTY - h;
     const int Sx = S0x + tx, Sy = S0y + ty;
-    const int e = blockIdx.z;
+    const int z = blockIdx.z, p = z % B, ab = (z / B) & 1, e = z / (2 * B);
     const int nsets = g.spin ? 2 : 1;
     const int nper = g.spin ? (g.maxb >> 1) : g.maxb;
-    // cell origins of the staged regions: a reads cells S-R, b reads cells -S-R
-    const int ax0 = S0x - w + n2, ay0 = S0y - w + n2;
-    const int bx0 = -S0x - (kWinTX - 1) - w + n2, by0 = -S0y - (kWinTY - 1) - w + n2;
-    const double2 *la = stage, *lb = stage + kWinPix * RC;
-    // this thread's offsets inside the staged regions for R = 0
-    const int oa = (ty + w) * RX + tx + w;
-    const int ob = (kWinTY - 1 - ty + w) * RX + (kWinTX - 1 - tx) + w;
+    // cell origin of the staged region: a reads cells S-R, b reads cells -S-R;
+    // this thread's offset inside it for R = 0
+    const int x0 = ab ? -S0x - (kWinTX - 1) - w + n2 : S0x - w + n2;
+    const int y0 = ab ? -S0y - (kWinTY - 1) - w + n2 : S0y - w + n2;
+    const int off = ab ? (kWinTY - 1 - ty + w) * RX + (kWinTX - 1 - tx) + w : (ty + w) * RX + tx + w;
     const size_t TT = (size_t)T * T;
     // staging lanes: sub-pixel row of the strip, first cell column; for the
     // weights (row wi, orbital wo) and the first R row wr0
@@ -288,96 +291,80 @@ __global__ void __launch_bounds__(kWinTX *kWinTY) k_adj_win(Geom g, const double
     const bool wact = threadIdx.x < kWgtStep * kWinPix * B;
     const int wio = threadIdx.x % (kWinPix * B), wr0 = threadIdx.x / (kWinPix * B);
     const int wsi = wio / B, wso = wio % B;
-    double2 acc[B * B];
+    double2 acc[B];
 #pragma unroll
-    for (int i = 0; i < B * B; i++) acc[i] = make_double2(0.0, 0.0);
+    for (int i = 0; i < B; i++) acc[i] = make_double2(0.0, 0.0);
 
     for (int sg = 0; sg < nsets; sg++) {
         const int olo = sg * nper, ohi = olo + nper;
-#pragma unroll
-        for (int p = 0; p < B; p++) {
-            const double2 *sa = abar + (((size_t)e * nsets + sg) * B + p) * W * W;
-            const double2 *sb = bbar + (((size_t)e * nsets + sg) * B + p) * W * W;
-            for (int jj = 0; jj < os; jj++)
-                for (int ii0 = 0; ii0 < os; ii0 += kWinPix) {
-                    const int npx = min(kWinPix, os - ii0);
-                    __syncthreads();
-                    // lanes: the strip's sub-pixel row si fastest, then the cell
-                    // column, so a wave reads consecutive pixels of a map row
-#pragma unroll
-                    for (int ab = 0; ab < 2; ab++) {
-                        const double2 *src = ab ? sb : sa;
-                        const int x0 = ab ? bx0 : ax0, y0 = ab ? by0 : ay0;
-                        double2 *dst = stage + (size_t)(ab * kWinPix + si) * RC;
-                        for (int ry = 0; ry < RY; ry++) {
-                            const int cy = y0 + ry;
-                            const bool rowok = si < npx && cy >= 0 && cy < n;
-                            for (int rx = scx; rx < RX; rx += kStageCols) {
-                                const int cx = x0 + rx;
-                                double2 v = make_double2(0.0, 0.0);
-                                if (rowok && cx >= 0 && cx < n)
-                                    v = src[(size_t)(cx * os + ii0 + si) + (size_t)(cy * os + jj) * W];
-                                dst[ry * RX + rx] = v;
-                            }
-                        }
+        const double2 *src = (ab ? bbar : abar) + (((size_t)e * nsets + sg) * B + p) * W * W;
+        for (int jj = 0; jj < os; jj++)
+            for (int ii0 = 0; ii0 < os; ii0 += kWinPix) {
+                const int npx = min(kWinPix, os - ii0);
+                __syncthreads();
+                // lanes: the strip's sub-pixel row si fastest, then the cell
+                // column, so a wave reads consecutive pixels of a map row
+                double2 *dst = stage + (size_t)si * RC;
+                for (int ry = 0; ry < RY; ry++) {
+                    const int cy = y0 + ry;
+                    const bool rowok = si < npx && cy >= 0 && cy < n;
+                    for (int rx = scx; rx < RX; rx += kStageCols) {
+                        const int cx = x0 + rx;
+                        double2 v = make_double2(0.0, 0.0);
+                        if (rowok && cx >= 0 && cx < n)
+                            v = src[(size_t)(cx * os + ii0 + si) + (size_t)(cy * os + jj) * W];
+                        dst[ry * RX + rx] = v;
                     }
-                    // weights w_o(x;R) of this strip as wl[R][row][o], zero outside
-                    // the spin block and past the strip's last sub-pixel; lanes
-                    // (row, o) fastest, then the R column
-                    if (wact)
-                        for (int rxw = 0; rxw < side; rxw++)
-                            for (int ryw = wr0; ryw < side; ryw += kWgtStep) {
-                                double v = 0.0;
-                                if (wsi < npx && wso >= olo && wso < ohi)
-                                    v = tiles[(size_t)wso * TT + (size_t)(ii0 + wsi + (2 * w - rxw) * os) * T + jj +
-                                              (2 * w - ryw) * os];
-                                wl[(rxw * side + ryw) * (kWinPix * B) + wio] = v;
-                            }
-                    __syncthreads();
-                    // all kWinPix rows of the strip per R, so that each wait on
-                    // LDS covers 4 kWinPix B FMAs; rows past the strip's last
-                    // sub-pixel are staged as zeros with zero weights
-                    for (int rxi = -w; rxi <= w; rxi++) {
-                        for (int ryi = -w; ryi <= w; ryi++) {
-                            const int d = -ryi * RX - rxi;
-                            double2 va[kWinPix], vb[kWinPix];
+                }
+                // weights w_o(x;R) of this strip as wl[R][row][o], zero outside
+                // the spin block and past the strip's last sub-pixel; lanes
+                // (row, o) fastest, then the R column
+                if (wact)
+                    for (int rxw = 0; rxw < side; rxw++)
+                        for (int ryw = wr0; ryw < side; ryw += kWgtStep) {
+                            double v = 0.0;
+                            if (wsi < npx && wso >= olo && wso < ohi)
+                                v = tiles[(size_t)wso * TT + (size_t)(ii0 + wsi + (2 * w - rxw) * os) * T + jj +
+                                          (2 * w - ryw) * os];
+                            wl[(rxw * side + ryw) * (kWinPix * B) + wio] = v;
+                        }
+                __syncthreads();
+                // all kWinPix rows of the strip per R, so that each wait on
+                // LDS covers 2 kWinPix B FMAs; rows past the strip's last
+                // sub-pixel are staged as zeros with zero weights
+                for (int rxi = -w; rxi <= w; rxi++) {
+                    for (int ryi = -w; ryi <= w; ryi++) {
+                        const int d = -ryi * RX - rxi;
+                        double2 va[kWinPix];
 #pragma unroll
-                            for (int i = 0; i < kWinPix; i++) {
-                                va[i] = la[i * RC + oa + d];
-                                vb[i] = lb[i * RC + ob + d];
-                            }
-                            // the kWinPix x B weights of this R, contiguous
-                            const double *wr = wl + ((rxi + w) * side + (ryi + w)) * (kWinPix * B);
+                        for (int i = 0; i < kWinPix; i++) va[i] = stage[i * RC + off + d];
+                        // the kWinPix x B weights of this R, contiguous
+                        const double *wr = wl + ((rxi + w) * side + (ryi + w)) * (kWinPix * B);
 #pragma unroll
-                            for (int i = 0; i < kWinPix; i++) {
+                        for (int i = 0; i < kWinPix; i++) {
 #pragma unroll
-                                for (int o = 0; o < B; o++) {
-                                    const double wo = wr[i * B + o];
-                                    acc[o * B + p] = cadd(acc[o * B + p], cscale(wo, va[i]));
-                                    acc[p * B + o] = cadd(acc[p * B + o], cscale(wo, vb[i]));
-                                }
-                            }
+                            for (int o = 0; o < B; o++) acc[o] = cadd(acc[o], cscale(wr[i * B + o], va[i]));
                         }
                     }
                 }
-        }
+            }
     }
     if (Sx > h || Sy > h) return;
-    if (abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
+    if (!ab && abs(Sx) <= 2 * w && abs(Sy) <= 2 * w) {
         const int D = 4 * w + 1, npix = os * os;
         const double *rs = rhosum + (size_t)e * npix;
         const double *q = Q + ((size_t)(Sx + 2 * w) * D + (Sy + 2 * w)) * npix;
 #pragma unroll
-        for (int pr = 0; pr < B * B; pr++) {
-            const double *qp = q + (size_t)pr * D * D * npix;
+        for (int o = 0; o < B; o++) {
+            const double *qp = q + (size_t)(o * B + p) * D * D * npix;
             double re = 0.0;
             for (int pix = 0; pix < npix; pix++) re += rs[pix] * qp[pix];
-            acc[pr].y -= re;
+            acc[o].y -= re;
         }
     }
-    double2 *out = Gbar + ((size_t)e * nw * nw + (size_t)(Sx + h) * nw + (Sy + h)) * (B * B);
+    double2 *out = (ab ? Gbar_b : Gbar_a) + ((size_t)e * nw * nw + (size_t)(Sx + h) * nw + (Sy + h)) * (B * B);
 #pragma unroll
-    for (int i = 0; i < B * B; i++) out[i] = acc[i];
+    for (int o = 0; o < B; o++) out[ab ? p * B + o : o * B + p] = acc[o];
 }
 
 // Gbar(0) += T^H Tbar T^H;  Vbar[e] = M^-H Tbar (1 + G(0) T)^H.  One thread
@@ -560,6 +547,7 @@ __device__ inline void adj_res_cf(const double *ev, double omega, double eta, do
 
 // adjoint of the kx pass: F1bar[e][i][sy][pair] = K^-2 sum_sx Gbar[e][sx][sy][pair] e^{-2 pi i i (sx-h)/K}
 __global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__restrict__ Gbar,
+                                                   const double2 *__restrict__ Gbar_b,
                                                    const double2 *__restrict__ twn, int np,
                                                    double2 *__restrict__ F1bar)
 {
@@ -571,11 +559,14 @@ __global__ void __launch_bounds__(kThreads) k_adj_kx(Geom g, const double2 *__re
     while ((1 << lgK) < K) lgK++;
     double2 *Tw = S + (size_t)np * K;
     for (int m = threadIdx.x; m < (K >> 1); m += blockDim.x) Tw[m] = twn[m];
-    const double2 *in = Gbar + (size_t)e * nw * nw * NP;
+    const double2 *in = Gbar + (size_t)e * nw * nw * NP, *inb = Gbar_b + (size_t)e * nw * nw * NP;
     for (int t = threadIdx.x; t < K * npc; t += blockDim.x) {
         int m = t / npc, q = t - m * npc;
         double2 acc = make_double2(0.0, 0.0);
-        for (int sx = imod(m + h, K); sx < nw; sx += K) acc = cadd(acc, in[((size_t)sx * nw + sy) * NP + p0 + q]);
+        for (int sx = imod(m + h, K); sx < nw; sx += K) {
+            const size_t idx = ((size_t)sx * nw + sy) * NP + p0 + q;
+            acc = cadd(acc, cadd(in[idx], inb[idx]));
+        }
         S[(size_t)q * K + brev(m, lgK)] = acc;
     }
     __syncthreads();
@@ -913,7 +904,7 @@ int launch_pix_t(const Geom &g, const double2 *G0w, const double2 *T, const doub
 
 size_t win_lds(const Geom &g)
 {
-    return sizeof(double2) * 2 * kWinPix * ((size_t)(kWinTX + 2 * g.w) * (kWinTY + 2 * g.w) + 2) +
+    return sizeof(double2) * kWinPix * ((size_t)(kWinTX + 2 * g.w) * (kWinTY + 2 * g.w) + 2) +
            sizeof(double) * kWinPix * (size_t)(2 * g.w + 1) * (2 * g.w + 1) * g.B;
 }
 
@@ -923,9 +914,10 @@ int launch_win_t(const Geom &g, const double *tiles, const double2 *abar, const 
 {
     const long nq = (long)cqpi::adjoint_q_size(g);
     hipLaunchKernelGGL(k_adj_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, tiles, Q);
-    dim3 grid((g.nw + kWinTX - 1) / kWinTX, (g.nw + kWinTY - 1) / kWinTY, E);
+    // z: (energy, gather, band p); Gbar holds the a half, then the b half
+    dim3 grid((g.nw + kWinTX - 1) / kWinTX, (g.nw + kWinTY - 1) / kWinTY, E * 2 * B);
     hipLaunchKernelGGL(k_adj_win<B>, grid, dim3(kWinTX * kWinTY), win_lds(g), s, g, tiles, abar, bbar, rhosum, Q,
-                       Gbar);
+                       Gbar, Gbar + (size_t)E * g.nw * g.nw * B * B);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 
@@ -1086,7 +1078,8 @@ int launch_adjoint(const Geom &g, const double2 *G0w, const double2 *T, const do
     if (npmax < 1) npmax = 1;
     int nch = (NP + npmax - 1) / npmax, np = (NP + nch - 1) / nch;
     size_t lds = sizeof(double2) * ((size_t)np * g.K + (size_t)(g.K >> 1));
-    hipLaunchKernelGGL(k_adj_kx, dim3(g.nw, E, nch), dim3(kThreads), lds, s, g, Gbar, twn, np, F1bar);
+    hipLaunchKernelGGL(k_adj_kx, dim3(g.nw, E, nch), dim3(kThreads), lds, s, g, Gbar,
+                       Gbar + (size_t)E * g.nw * g.nw * NP, twn, np, F1bar);
     if (hipGetLastError() != hipSuccess) return CQPI_E_HIP;
     if (ky_fused(g)) {
         switch (g.B) {

@@ -1803,7 +1803,8 @@ static int gradient_impl(cqpi_ctx *c, const double *rhobar, bool on_device, doub
     RETCHK(dev_grow(c->d_Tpart, c->c_Tpart, E * nblk * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Tbar, c->c_Tbar, E * NP, c->gpumem));
     RETCHK(dev_grow(c->d_rhosum, c->c_rhosum, E * g.os * g.os, c->gpumem));
-    RETCHK(dev_grow(c->d_Gbar, c->c_Gbar, E * (size_t)g.nw * g.nw * NP, c->gpumem));
+    // Gbar in two halves: the a and the b gather of the window adjoint
+    RETCHK(dev_grow(c->d_Gbar, c->c_Gbar, 2 * E * (size_t)g.nw * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Vbar, c->c_Vbar, E * NP, c->gpumem));
     RETCHK(dev_grow(c->d_F1, c->c_F1, E * (size_t)g.K * g.nw * NP, c->gpumem));
     RETCHK(dev_grow(c->d_Hacc, c->c_Hacc, NG * g.K * g.K * NP, c->gpumem));
