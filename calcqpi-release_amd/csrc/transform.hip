@@ -281,6 +281,7 @@ __global__ void __launch_bounds__(kThreads) k_transform_kx(Geom g, MixGeom mg, c
 constexpr int kMinWaveK = 64, kMaxWaveK = 1024;
 constexpr int kMaxRegB = 6;   // band counts whose per-k eigen data stays in registers
 constexpr int kMaxLines = 16; // orbital pairs (lines) per workgroup
+constexpr int kMaxSymPairs = 64;  // b^2 of the paired-row pass 2 (b <= 8)
 constexpr int kWaveMaxThreads = 512;
 // FP32 lines take up to 16 waves a workgroup, so that every orbital pair of
 // up to 4 bands is one line of one chunk (3 bands: 9 waves for 9 lines; with
@@ -685,13 +686,13 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
         const C *src = in + (size_t)sy * NT;
         for (int t0 = threadIdx.x; t0 < tot; t0 += U * blockDim.x) {
             C v[U];
+            // every v[u] loaded (the index clamped), so that the batch stays
+            // in registers (a conditionally written array went to scratch in FP64)
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int t = t0 + u * blockDim.x;
-                if (t < tot) {
-                    const int i = udiv(t, dv.npc);
-                    v[u] = src[(size_t)i * nw * NT + (t - i * NT)];
-                }
+                const int t = min(t0 + u * (int)blockDim.x, tot - 1);
+                const int i = udiv(t, dv.npc);
+                v[u] = src[(size_t)i * nw * NT + (t - i * NT)];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -755,6 +756,110 @@ __global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 7 : 
     }
 }
 
+// pass 2 with the pair symmetry (g.ts), one workgroup per mirror pair of Ry
+// slots {sy, 2h - sy} (sy <= h) and energy: the NT lines r <= c of both rows
+// are transformed together (NT for the centre row sy = h), so that every
+// window record of the two rows, all b^2 pairs of a slot, is written whole
+// by one workgroup, G0(R)_cr taken as G0(-R)_rc from the other row at -Rx.
+// (Writing the pairs c > r from the workgroup of the mirror row left every
+// record split between two workgroups: 4x the window's bytes written in
+// FP64.)  F1 holds the NT pairs in one pass-1 chunk; no fp16 planes.
+template <typename C, int N>
+__global__ void __launch_bounds__(N >= 1024 ? 512 : 1024, (sizeof(C) == 8 ? 6 : 1)) k_kx_sym(Geom g, const C *__restrict__ F1,
+                                                                   const C *__restrict__ tw, int rwin,
+                                                                   double2 *__restrict__ G0w,
+                                                                   float2 *__restrict__ W32)
+{
+    extern __shared__ __align__(16) unsigned char smem_s[];
+    __shared__ int ptab[kMaxSymPairs];
+    C *S = reinterpret_cast<C *>(smem_s);
+    constexpr int LN = cqpi::stockham_lanes(N);
+    constexpr int NL = cqpi::padded_len<C>(N);  // LDS slots per line
+    const int B = g.B, NP = B * B, NT = transform_pairs(g);
+    C *Twl = S + (size_t)2 * NT * NL;
+    if constexpr (cqpi::pass_twiddles<C>())
+        cqpi::build_pass_twiddles<N>(Twl, tw);
+    else if constexpr (sizeof(C) == 8)
+        for (int m = threadIdx.x; m < N; m += blockDim.x) Twl[m] = tw[m];
+    const C *Tw = sizeof(C) == 8 ? Twl : tw;
+    // pair pf = r B + c of a record: line upper_index(min, max), from the
+    // mirror row at -Rx where r > c (bit 16)
+    for (int pf = threadIdx.x; pf < NP; pf += blockDim.x) {
+        const int r = pf / B, c = pf - r * B;
+        ptab[pf] = r <= c ? upper_index(r, c, B) : (upper_index(c, r, B) | (1 << 16));
+    }
+    int sy, e, z;
+    xcd_block(sy, e, z);
+    const int nw = g.nw, h = g.h, sm = 2 * h - sy;
+    const int nrow = sm != sy ? 2 : 1;
+    const int grp = threadIdx.x / LN, ll = threadIdx.x - grp * LN, ngrp = blockDim.x / LN;
+    const C *in = F1 + (size_t)e * N * nw * NT;
+    // lines t NT + q: pair q of row t (t = 0: sy, t = 1: sm); a batch of
+    // loads in flight before their LDS stores
+    {
+        constexpr int U = 8;
+        const unsigned long long mq = udiv_magic((unsigned)NT);
+        const int tot = N * NT;
+        for (int t = 0; t < nrow; t++) {
+            const C *src = in + (size_t)(t ? sm : sy) * NT;
+            C *dst = S + (size_t)t * NT * NL;
+            for (int t0 = threadIdx.x; t0 < tot; t0 += U * blockDim.x) {
+                C v[U];
+                // every v[u] loaded (the index clamped), so that the batch
+                // stays in registers (a conditionally written array went to
+                // scratch in FP64)
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int k = min(t0 + u * (int)blockDim.x, tot - 1);
+                    const int i = udiv(k, mq);
+                    v[u] = src[(size_t)i * nw * NT + (k - i * NT)];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int k = t0 + u * blockDim.x;
+                    if (k < tot) {
+                        const int i = udiv(k, mq);
+                        dst[(size_t)(k - i * NT) * NL + cqpi::pidx<C>(i)] = v[u];
+                    }
+                }
+            }
+        }
+    }
+    lds_barrier();
+    for (int q = grp; q < nrow * NT; q += ngrp) cqpi::wave_fft<N>(S + (size_t)q * NL, Tw, ll, +1);
+    lds_barrier();
+    const double inv = 1.0 / ((double)N * (double)N);
+    const unsigned long long mp = udiv_magic((unsigned)NP);
+    for (int t = 0; t < nrow; t++) {
+        const int y = t ? sm : sy;
+        const C *own = S + (size_t)t * NT * NL, *mir = S + (size_t)(nrow - 1 - t) * NT * NL;
+        // FP64 window: whole, or (rwin < h) only |R| <= rwin
+        if (abs(y - h) <= rwin) {
+            const int sx0 = h - rwin, nx = 2 * rwin + 1;
+            double2 *out = G0w + ((size_t)e * nw * nw + y) * NP;
+            for (int k = threadIdx.x; k < nx * NP; k += blockDim.x) {
+                const int ix = udiv(k, mp), pf = k - ix * NP, sx = sx0 + ix;
+                const int pt = ptab[pf], q = pt & 0xffff;
+                const C zv = (pt >> 16) ? mir[(size_t)q * NL + cqpi::pidx<C>((h - sx) & (N - 1))]
+                                        : own[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
+                out[(size_t)sx * nw * NP + pf] = make_double2(inv * (double)zv.x, inv * (double)zv.y);
+            }
+        }
+        if (W32) {
+            // the whole window row y in FP32, W32[e][y][sx][pair] (one contiguous run)
+            const float invf = (float)inv;
+            float2 *ow = W32 + ((size_t)e * nw + y) * nw * NP;
+            for (int k = threadIdx.x; k < nw * NP; k += blockDim.x) {
+                const int sx = udiv(k, mp), pf = k - sx * NP;
+                const int pt = ptab[pf], q = pt & 0xffff;
+                const C zv = (pt >> 16) ? mir[(size_t)q * NL + cqpi::pidx<C>((h - sx) & (N - 1))]
+                                        : own[(size_t)q * NL + cqpi::pidx<C>((sx - h) & (N - 1))];
+                ow[k] = make_float2((float)zv.x * invf, (float)zv.y * invf);
+            }
+        }
+    }
+}
+
 constexpr size_t kLdsBudget = 80 * 1024;   // largest line buffer a workgroup may take
 constexpr size_t kLdsTarget = 32 * 1024;   // preferred: several workgroups per CU
 
@@ -778,6 +883,9 @@ constexpr size_t kWaveLds = 96 * 1024;
 #define CQPI_LDS32_KB 72
 #endif
 constexpr size_t kWaveLds32 = CQPI_LDS32_KB * 1024;
+// FP64 lines of the paired-row pass 2: up to 12 (3 bands at K = 512) in one
+// workgroup, one workgroup a CU
+constexpr size_t kSymLds64 = 128 * 1024;
 size_t line_lds_budget(int N, size_t csize)
 {
     return csize == 16 ? kWaveLds : kWaveLds32 - (size_t)N * csize;
@@ -916,9 +1024,18 @@ int launch_kx_s(const Geom &g, const MixGeom &m, const C *F1, int E, const C *tw
     if (g.ts && P) return CQPI_E_ARG;
     const int NT = transform_pairs(g), np = kx_pairs(N, NT, sizeof(C)), nch = (NT + np - 1) / np;
     const int np1 = wave_pairs(N, NT, sizeof(C));  // pass 1's chunks fix the F1 layout
+    rwin = std::min(std::max(rwin, 0), g.h);
+    if (g.ts && np1 == NT && g.B * g.B <= kMaxSymPairs) {
+        // both rows of a mirror pair in one workgroup where their lines fit
+        const int tsym = (2 * NT * cqpi::stockham_lanes(N) + 63) / 64 * 64;
+        const size_t lsym = sizeof(C) * ((size_t)2 * NT * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
+        if (tsym <= (N >= 1024 ? 512 : 1024) && lsym <= (sizeof(C) == 8 ? kWaveLds32 : kSymLds64)) {
+            hipLaunchKernelGGL((k_kx_sym<C, N>), dim3(g.h + 1, E, 1), dim3(tsym), lsym, s, g, F1, tw, rwin, G0w, W32);
+            return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+        }
+    }
     const int thr = std::min(N >= 1024 ? 512 : 1024, (np * cqpi::stockham_lanes(N) + 63) / 64 * 64);
     const size_t lds = sizeof(C) * ((size_t)np * cqpi::padded_len<C>(N) + (sizeof(C) == 8 ? N : 0));
-    rwin = std::min(std::max(rwin, 0), g.h);
     const PlaneDiv dv{udiv_magic((unsigned)std::max(m.Ly >> 3, 1)), udiv_magic((unsigned)np), udiv_magic((unsigned)g.B)};
     hipLaunchKernelGGL((k_kx_s<C, N>), dim3(g.nw, E, nch), dim3(thr), lds, s, g, m, F1, tw, np, np1, rwin, sG, G0w,
                        P, dv, W32);
