@@ -65,15 +65,18 @@ def fused_transform_bytes(wl, E):
     return K * K * (16 * b * b + 8 * b) + E * 16 * b * b * nk * nk
 
 
-def transform_valu_flops(wl, nw, E):
+def transform_valu_flops(wl, nw, E, pairs=None):
     """FP32/FP64 arithmetic of the two transform passes for E energies: the
     radix-8 line transforms (about 5 N log2 N flops per line of N = K points;
-    b^2 lines per kx row in the first pass, per window column in the second)
-    and the resolvent (8 flops per pair and band per k-point)."""
+    one line per transformed orbital pair per kx row in the first pass and per
+    window column in the second) and the resolvent (8 flops per transformed
+    pair and band per k-point).  pairs: the transformed pairs, b^2, or
+    b (b + 1) / 2 where the run used the pair symmetry."""
     K, b = wl.K, wl.bands
+    npair = b * b if pairs is None else pairs
     lg = int(np.log2(K))
     line = 5 * K * lg
-    return E * (K * b * b * line + nw * b * b * line + K * K * b * b * b * 8)
+    return E * (K * npair * line + nw * npair * line + K * K * npair * b * 8)
 
 
 def rel_errors(x, ref, floor=1e-3):
@@ -264,14 +267,28 @@ def gradient_timing(cqpi, synth, torch, reps=3):
         if it:
             fw.append(t1 - t0)
             bw.append(t2 - t1)
+    # the same forward with every orbital pair transformed (the adjoint's
+    # transforms always carry every pair)
+    q.set_pair_symmetry(False)
+    fa = []
+    for it in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q.compute_eigen()
+        q.run(wl.energies, wl.eta, host=False)
+        torch.cuda.synchronize()
+        if it:
+            fa.append(time.perf_counter() - t0)
     q.close()
-    f, b = float(np.median(fw)), float(np.median(bw))
+    f, b, f_all = float(np.median(fw)), float(np.median(bw)), float(np.median(fa))
     return {"workload": "square lattice 3 orbitals, K=512, 21 energies, 64x64 cells, os=4, w=2; "
                         "L2 map loss; gradients w.r.t. %d hopping amplitudes (re, im), mu and V" % len(wl.hops),
             "forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
             "gradient_over_forward": (f + b) / f,
-            "note": "FP64; forward = eigensolve + all energies; backward = dL/drho on the device + "
-                    "adjoint; maps and dL/drho stay in HBM"}
+            "forward_every_pair_ms": f_all * 1e3, "gradient_over_forward_every_pair": (f_all + b) / f_all,
+            "note": "FP64; forward = eigensolve + all energies (the transform with the pair symmetry of "
+                    "the real hoppings; forward_every_pair_ms: with it switched off); backward = dL/drho on "
+                    "the device + adjoint; maps and dL/drho stay in HBM"}
 
 
 TRAFFIC_MD = {"square_3orb": os.path.join(ROOT, "profiles", "pmc_traffic_square3orb_K512_E51.md"),
@@ -307,7 +324,8 @@ def stage_launch_s(st, name):
 
 def transform_roofline(wl, q, st, precision, workload):
     """Roofline of the fused resolvent + k->R transform (the two passes,
-    k_res_ky_s and k_kx_s, one launch each per run with stage events), against
+    k_res_ky_s and k_kx_s or, with the pair symmetry, k_kx_sym; one launch
+    each per run with stage events), against
     the fused minimum bytes; `traffic` is what the two kernels moved in the
     committed profile (the ky-transformed rows go out to HBM and back between
     the passes: overhead traffic).  Their limit is the vector ALU and LDS work
@@ -326,9 +344,12 @@ def transform_roofline(wl, q, st, precision, workload):
     if m1 and m2 and m1[1] and m2[1]:
         traffic = (m1[0] / m1[1] + m2[0] / m2[1]) * E
     nw = 2 * q.window_half() + 1
-    fl = transform_valu_flops(wl, nw, E)
+    sym = q.pair_symmetry()[1]
+    b = wl.bands
+    fl = transform_valu_flops(wl, nw, E, b * (b + 1) // 2 if sym else b * b)
     line = "FP64" if precision == "fp64" else "FP32"
-    return {"kernel": "resolvent+transform (k_res_ky_s + k_kx_s, %s lines)" % line, "bound": "hbm",
+    return {"kernel": "resolvent+transform (k_res_ky_s + %s, %s lines)" % ("k_kx_sym" if sym else "k_kx_s", line),
+            "pair_symmetry": bool(sym), "bound": "hbm",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic / runs if traffic else None, "algorithmic_bytes_per_launch": per_run / runs,
             "avg_launch_ms": t * 1e3, "pass_ms": [t1 * 1e3, t2 * 1e3], "launches_per_step": runs,
