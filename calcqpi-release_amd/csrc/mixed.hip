@@ -668,58 +668,69 @@ __global__ void __launch_bounds__(64 * tf_wg_waves(BT)) __attribute__((amdgpu_wa
     const int row_lo = cy_lo - n2 - WW + h;
     const int sx0 = cx0 - n2 - WW + h;
     const float2 *We = W32 + (size_t)e * nw * nw * NP;
-    for (int it = threadIdx.x; it < RW * NUG * NP; it += blockDim.x) {
-        const int op = it % NP, rest = it / NP;
-        const int o = op / BT, p = op - o * BT;
+    // one item per (row, column group, orbital o): the b row G0(-pos-R)[q][o]
+    // is loaded once for the BT folds (T G0)[p][o], p < BT
+    for (int it = threadIdx.x; it < RW * NUG * BT; it += blockDim.x) {
+        const int o = it % BT, rest = it / BT;
         const int g8 = rest % NUG, r = rest / NUG;
         const int row = row_lo + r;
         const int rowc = min(max(row, 0), nw - 1);
         const bool rok = row == rowc;
-        const float2 *arow = We + (size_t)rowc * nw * NP + op;
+        const float2 *arow = We + (size_t)rowc * nw * NP + o * BT;
         const float2 *brow = We + (size_t)(2 * h - rowc) * nw * NP + o;
         // every load first (indices clamped into the window, values outside it
         // dropped below), then the arithmetic
-        float2 av[8], bv[8][BT];
+        float2 av[8][BT], bv[8][BT];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int sxc = min(max(sx0 + g8 * 8 + u, 0), nw - 1);
-            av[u] = arow[(size_t)sxc * NP];
 #pragma unroll
-            for (int q = 0; q < BT; q++) bv[u][q] = brow[(size_t)(2 * h - sxc) * NP + q * BT];
+            for (int q = 0; q < BT; q++) {
+                av[u][q] = arow[(size_t)sxc * NP + q];
+                bv[u][q] = brow[(size_t)(2 * h - sxc) * NP + q * BT];
+            }
         }
-        float2 c0[BT];
-#pragma unroll
-        for (int q = 0; q < BT; q++) c0[q] = Tf[p * BT + q];
-        half8 ahr, alr, ahi, ali, chr, clr, chi, cli;
+        bool okv[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int sx = sx0 + g8 * 8 + u;
-            const bool ok = rok && sx >= 0 && sx < nw;
-            float yr = 0.f, yi = 0.f;
-#pragma unroll
-            for (int q = 0; q < BT; q++) {
-                yr += c0[q].x * bv[u][q].x - c0[q].y * bv[u][q].y;
-                yi += c0[q].x * bv[u][q].y + c0[q].y * bv[u][q].x;
-            }
-            const float xr = ok ? av[u].x * sGf : 0.f, xi = ok ? av[u].y * sGf : 0.f;
-            yr = ok ? yr * sTf : 0.f;
-            yi = ok ? yi * sTf : 0.f;
-            // v - hi is exact in FP32 (power-of-two scales)
-            _Float16 t;
-            t = (_Float16)xr; ahr[u] = t; alr[u] = (_Float16)(xr - (float)t);
-            t = (_Float16)xi; ahi[u] = t; ali[u] = (_Float16)(xi - (float)t);
-            t = (_Float16)yr; chr[u] = t; clr[u] = (_Float16)(yr - (float)t);
-            t = (_Float16)yi; chi[u] = t; cli[u] = (_Float16)(yi - (float)t);
+            okv[u] = rok && sx >= 0 && sx < nw;
         }
-        _Float16 *dst = Pl + (size_t)(op * 2) * PSTR + r * SEG8 + g8 * 8;
-        *reinterpret_cast<half8 *>(dst) = ahr;
-        *reinterpret_cast<half8 *>(dst + PSTR) = ahi;
-        *reinterpret_cast<half8 *>(dst + LOFF) = alr;
-        *reinterpret_cast<half8 *>(dst + LOFF + PSTR) = ali;
-        *reinterpret_cast<half8 *>(dst + 2 * LOFF) = chr;
-        *reinterpret_cast<half8 *>(dst + 2 * LOFF + PSTR) = chi;
-        *reinterpret_cast<half8 *>(dst + 3 * LOFF) = clr;
-        *reinterpret_cast<half8 *>(dst + 3 * LOFF + PSTR) = cli;
+#pragma unroll
+        for (int p = 0; p < BT; p++) {
+            float2 c0[BT];
+#pragma unroll
+            for (int q = 0; q < BT; q++) c0[q] = Tf[p * BT + q];
+            half8 ahr, alr, ahi, ali, chr, clr, chi, cli;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const bool ok = okv[u];
+                float yr = 0.f, yi = 0.f;
+#pragma unroll
+                for (int q = 0; q < BT; q++) {
+                    yr += c0[q].x * bv[u][q].x - c0[q].y * bv[u][q].y;
+                    yi += c0[q].x * bv[u][q].y + c0[q].y * bv[u][q].x;
+                }
+                const float xr = ok ? av[u][p].x * sGf : 0.f, xi = ok ? av[u][p].y * sGf : 0.f;
+                yr = ok ? yr * sTf : 0.f;
+                yi = ok ? yi * sTf : 0.f;
+                // v - hi is exact in FP32 (power-of-two scales)
+                _Float16 t;
+                t = (_Float16)xr; ahr[u] = t; alr[u] = (_Float16)(xr - (float)t);
+                t = (_Float16)xi; ahi[u] = t; ali[u] = (_Float16)(xi - (float)t);
+                t = (_Float16)yr; chr[u] = t; clr[u] = (_Float16)(yr - (float)t);
+                t = (_Float16)yi; chi[u] = t; cli[u] = (_Float16)(yi - (float)t);
+            }
+            _Float16 *dst = Pl + (size_t)((o * BT + p) * 2) * PSTR + r * SEG8 + g8 * 8;
+            *reinterpret_cast<half8 *>(dst) = ahr;
+            *reinterpret_cast<half8 *>(dst + PSTR) = ahi;
+            *reinterpret_cast<half8 *>(dst + LOFF) = alr;
+            *reinterpret_cast<half8 *>(dst + LOFF + PSTR) = ali;
+            *reinterpret_cast<half8 *>(dst + 2 * LOFF) = chr;
+            *reinterpret_cast<half8 *>(dst + 2 * LOFF + PSTR) = chi;
+            *reinterpret_cast<half8 *>(dst + 3 * LOFF) = clr;
+            *reinterpret_cast<half8 *>(dst + 3 * LOFF + PSTR) = cli;
+        }
     }
     if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + ZOFF) = half8{0, 0, 0, 0, 0, 0, 0, 0};
     __syncthreads();
