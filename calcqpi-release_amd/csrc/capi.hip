@@ -549,8 +549,10 @@ int prepare_run(cqpi_ctx *c, const double *omegas, size_t E, double eta, RunPlan
     }
     // the transform passes carry only the pairs r <= c where the model has
     // H(k)^T = H(-k): on the Stockham path, with G0(k) from the eigen data (not
-    // the decimation workspace) and no fp16 planes to write (their writer
-    // has no mirrored copies)
+    // the decimation workspace) and no fp16 planes to write (their writer in
+    // the kx pass has no mirrored copies; packing the planes from a whole
+    // FP64 window instead cost more than the symmetry saved: mixed Josephson
+    // map of 6 bands 5.3 -> 7.6 ms)
     const bool planes_written = plan.mixed && (c->josephson || !mix_tfold(g, c->mg));
     plan.tsym = c->tsym_model && c->tsym_on && !c->surf_mode && g.B > 1 && transform32_supported(g) &&
                 !planes_written;
@@ -1461,16 +1463,14 @@ static int upload_surface_groups(cqpi_ctx *c)
     return 0;
 }
 
-// H(k)^T = H(-k) on the k-grid: every hopping block at an integral R has its
-// transpose at -R, exactly (real hoppings of a Hermitian model), no
-// particle-hole term.  Then G0(k)^T = G0(-k) at every energy, and the
-// transform of the pair (r, c) at R is the pair (c, r) at -R.
-static bool transpose_symmetric(const cqpi_hopping *hops, size_t nhops, const std::vector<DevRGroup> &G,
-                                const std::vector<double2> &T, int B, bool scmodel)
+// H(k)^T = H(-k) on the k-grid: every block of H(k) = sum_R T_R e^{2 pi i k.R}
+// (the groups k_eigen sums, hole-block terms at their conjugate-phase R) sits
+// at an integral R and has its exact transpose at -R (real hoppings of a
+// Hermitian model, and a particle-hole model built from them with real
+// pairing); the diagonal -/+ efermi is real.  Then G0(k)^T = G0(-k) at every
+// energy, and the transform of the pair (r, c) at R is the pair (c, r) at -R.
+static bool transpose_symmetric(const std::vector<DevRGroup> &G, const std::vector<double2> &T, int B)
 {
-    if (scmodel) return false;
-    for (size_t i = 0; i < nhops; i++)
-        if (hops[i].anti) return false;
     std::map<std::pair<double, double>, size_t> at;
     for (size_t i = 0; i < G.size(); i++) {
         if (!G[i].integral) return false;
@@ -1505,7 +1505,7 @@ int cqpi_set_model(cqpi_ctx *c, const cqpi_hopping *hops, size_t nhops, double e
     c->ngroups = (int)G.size();
     c->efermi = efermi;
     c->scmax = scmodel ? (B >> 1) : B;
-    c->tsym_model = transpose_symmetric(hops, nhops, G, T, B, scmodel != 0);
+    c->tsym_model = transpose_symmetric(G, T, B);
     c->has_model = true;
     c->h_hops.assign(hops, hops + nhops);
     c->windows_kept = false;

@@ -338,8 +338,9 @@ int cqpi_set_surface_steps(cqpi_ctx *ctx, int maxit);
 
 /* Pair symmetry of the k->R transform (no counterpart in the reference, which
  * transforms every orbital pair, src/qpi.cc:557-578).  Where every hopping
- * block at an integral R has its exact transpose at -R and there is no
- * particle-hole term (real hoppings of a Hermitian model), H(k)^T = H(-k), so
+ * block at an integral R (the hole block's terms at their conjugate-phase R)
+ * has its exact transpose at -R (real hoppings of a Hermitian model, also as
+ * the normal blocks of a particle-hole model with real pairing), H(k)^T = H(-k), so
  * G0(k)^T = G0(-k) and G0(R)_cr = G0(-R)_rc: the transform passes of
  * cqpi_run then carry only the b(b+1)/2 pairs r <= c and write each pair
  * also as (c, r) at -R.  cqpi_set_model detects the property; enable (default

@@ -98,3 +98,22 @@ def test_planes_path_keeps_every_pair(gpu):
     rho, (model, used) = _maps(wl, "mixed", True)
     assert model and not used
     _close(rho, oracle.cldos_run(wl), TOL["mixed"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_particle_hole_model(gpu, prec):
+    # the particle-hole model of real normal-state hoppings and real s-wave
+    # pairing has H(k)^T = H(-k) as well: 21 of 36 pairs, LDOS and Josephson maps
+    wl = synth.bdg_square(3, K=64, n=8, E=3, seed=61)
+    on, (model, used) = _maps(wl, prec, True)
+    assert model and used
+    off, _ = _maps(wl, prec, False)
+    _close(on, off, TOL_ONOFF[prec])
+    _close(on, oracle.cldos_run(wl), TOL[prec])
+    with cqpi.engine_for(wl, prec) as q:
+        q.set_josephson(True, deltat=0.01)
+        jon = q.run(wl.energies, wl.eta)
+        assert q.pair_symmetry() == (True, True)
+        q.set_pair_symmetry(False)
+        joff = q.run(wl.energies, wl.eta)
+    _close(jon, joff, TOL_ONOFF[prec])
