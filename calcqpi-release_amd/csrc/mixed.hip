@@ -984,7 +984,32 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
                     hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
                     cr[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
                 }
+            // the tile fragments of the next K step are loaded while this
+            // step's MFMAs run (they come from L2, one pair per sub-pixel row)
+            half8 ahn[JB], aln[JB];
+#pragma unroll
+            for (int i = 0; i < JB; i++) {
+                const int jj = min(jj0 + i, os - 1);
+                const size_t fa = ((((size_t)sg * os + jj) * mg.KS + 0) * 2) * 64;
+                ahn[i] = Afrag[fa + lane];
+                aln[i] = Afrag[fa + 64 + lane];
+            }
             for (int ks = 0; ks < mg.KS; ks++) {
+                half8 ahc[JB], alc[JB];
+#pragma unroll
+                for (int i = 0; i < JB; i++) {
+                    ahc[i] = ahn[i];
+                    alc[i] = aln[i];
+                }
+                if (ks + 1 < mg.KS) {
+#pragma unroll
+                    for (int i = 0; i < JB; i++) {
+                        const int jj = min(jj0 + i, os - 1);
+                        const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks + 1) * 2) * 64;
+                        ahn[i] = Afrag[fa + lane];
+                        aln[i] = Afrag[fa + 64 + lane];
+                    }
+                }
                 const int kg = ks * 4 + kq;
                 const bool kok = kg < mg.KG;
                 const int g8 = kg % mg.NUg, r = kg / mg.NUg;
@@ -1011,9 +1036,7 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
                 for (int i = 0; i < JB; i++) {
                     const int jj = jj0 + i;
                     if (jj >= os) break;
-                    const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks) * 2) * 64;
-                    const half8 ah = Afrag[fa + lane];
-                    const half8 al = Afrag[fa + 64 + lane];
+                    const half8 ah = ahc[i], al = alc[i];
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
                         hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[i][t], 0, 0, 0);
