@@ -432,11 +432,14 @@ __device__ inline void resolvent_all_pairs(C *S, int j, const EigenPairs<C, B> &
 // the 9-wave workgroups of one chunk of every pair (two of them fill a CU
 // either way, and 96 VGPRs hold the pair products without spilling) and 4
 // for 4 and 5 bands (their eigenvectors alone take 32 and 50 VGPRs)
-template <typename C, bool ALLP, int B>
-constexpr int res_min_waves() { return sizeof(C) == 8 ? (ALLP ? 5 : (B >= 4 ? 4 : kResWaves32)) : 1; }
+// (UPT: the 6 pairs r <= c of 3 bands under the pair symmetry, 8-wave
+// workgroups at 6 waves per SIMD: three a CU)
+template <typename C, bool ALLP, int B, bool UPT = false>
+constexpr int res_min_waves() { return sizeof(C) == 8 ? (UPT ? 6 : (ALLP ? 5 : (B >= 4 ? 4 : kResWaves32))) : 1; }
 
-template <typename C, int B, int N, bool ALLP = false>
-__global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C, ALLP, B>())) k_res_ky_s(Geom g, const double *__restrict__ eval,
+// UPT: one chunk of the pairs r <= c only (g.ts), resolved at compile time
+template <typename C, int B, int N, bool ALLP = false, bool UPT = false>
+__global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C, ALLP, B, UPT>())) k_res_ky_s(Geom g, const double *__restrict__ eval,
                                                               const double2 *__restrict__ evec,
                                                               const double *__restrict__ omegas, int E, int EB,
                                                               double eta, const C *__restrict__ tw, int np,
@@ -495,7 +498,9 @@ __global__ void __launch_bounds__(wave_max_threads(sizeof(C)), (res_min_waves<C,
                 C f[B];
 #pragma unroll
                 for (int n = 0; n < B; n++) f[n] = res_factor<C>(omega - ev_r[n], eta);
-                if constexpr (allp) {
+                if constexpr (allp && UPT) {
+                    resolvent_all_pairs<C, B, NL, LP, true>(S, j, pp, f);
+                } else if constexpr (allp) {
                     if (g.ts)
                         resolvent_all_pairs<C, B, NL, LP, true>(S, j, pp, f);
                     else
@@ -966,6 +971,13 @@ int launch_res_s(const Geom &g, const double *eval, const double2 *evec, const d
     // for the 10 and 15 pairs r <= c of 4 and 5 bands, whose products do not
     // fit the registers (the pair products of 5 bands spilled: 0.74 -> 1.40 ms
     // for 40 energies, K = 512)
+    if constexpr (B == 3 && sizeof(C) == 8) {
+        if (nch == 1 && g.ts) {
+            hipLaunchKernelGGL((k_res_ky_s<C, B, N, true, true>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec,
+                               omegas, E, EB, eta, tw, np, F1, (const C *)nullptr);
+            return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
+        }
+    }
     if (nch == 1 && (B <= 3 || !g.ts))
         hipLaunchKernelGGL((k_res_ky_s<C, B, N, true>), dim3(N, nb, nch), dim3(thr), lds, s, g, eval, evec, omegas, E,
                            EB, eta, tw, np, F1, (const C *)nullptr);

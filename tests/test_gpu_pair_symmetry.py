@@ -117,3 +117,32 @@ def test_particle_hole_model(gpu, prec):
         q.set_pair_symmetry(False)
         joff = q.run(wl.energies, wl.eta)
     _close(jon, joff, TOL_ONOFF[prec])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_threshold_list_context(gpu, prec):
+    # the list contraction reads the same window: symmetric passes, list map
+    # against the oracle's list contraction of the oracle window
+    wl = synth.square_norb(3, K=64, n=8, E=2, seed=87, w=2, os_=4, eta=0.01, elo=-0.1, ehi=0.1)
+    flags, counts = cqpi.make_threshold_list(wl.tiles, 2, 4, 0.04)
+    maps = {}
+    for sym in (True, False):
+        print("case: list context K=64 prec=%s sym=%d" % (prec, sym), flush=True)
+        q = cqpi.AmdQPI.from_list(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, flags, counts)
+        try:
+            q.set_model(wl.hops, wl.efermi, wl.scmodel)
+            q.set_impurity(wl.V)
+            q.set_precision(prec)
+            q.set_pair_symmetry(sym)
+            maps[sym] = q.run(wl.energies, wl.eta)
+            assert q.pair_symmetry() == (True, sym)
+        finally:
+            q.close()
+    _close(maps[True], maps[False], TOL_ONOFF[prec])
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    for e, om in enumerate(wl.energies):
+        gR = oracle.realspace(oracle.greens_k(ev, V, om, wl.eta))
+        T = oracle.tmatrix(wl.V, gR[0, 0])
+        ref = oracle.wannierldoslist(gR, T, flags, counts, wl.n, 2, 4)
+        mx, rms = rm.rel_errors(maps[True][e], ref)
+        assert mx < TOL[prec], "energy %d: max_rel %.3e" % (e, mx)
