@@ -29,14 +29,23 @@ __device__ inline void rotate(double2 *A, double2 *V, int b, int p, int q)
     double r = sqrt(cabs2(apq));
     if (r == 0.0) return;
     double app = A[p * b + p].x, aqq = A[q * b + q].x;
-    double2 ph = make_double2(apq.x / r, apq.y / r);  // e^{i phi}
-    double theta = (aqq - app) / (2.0 * r);
-    double t;
-    if (fabs(theta) > 1e150)
+    // one reciprocal of r for the phase and theta, one of tau for t and c:
+    // t = sign(theta) / tau, tau = |theta| + sqrt(theta^2 + 1) >= 1, and
+    // c = 1 / sqrt(t^2 + 1) = tau / sqrt(tau^2 + 1)
+    const double ir = 1.0 / r;
+    double2 ph = make_double2(apq.x * ir, apq.y * ir);  // e^{i phi}
+    double theta = (aqq - app) * (0.5 * ir);
+    double t, c;
+    if (fabs(theta) > 1e150) {
         t = 0.5 / theta;
-    else
-        t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-    double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        c = 1.0 / sqrt(t * t + 1.0);
+    } else {
+        const double tau = fabs(theta) + sqrt(theta * theta + 1.0);
+        const double it = 1.0 / tau;
+        t = theta >= 0.0 ? it : -it;
+        c = 1.0 / sqrt(it * it + 1.0);
+    }
+    double s = t * c;
     double2 sph = cscale(s, ph);          // s e^{i phi}
     double2 sphc = cconj(sph);            // s e^{-i phi}
     double2 cphc = cscale(c, cconj(ph));  // c e^{-i phi}
