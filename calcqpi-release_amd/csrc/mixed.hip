@@ -1105,6 +1105,195 @@ __global__ void __launch_bounds__(64 * kStWaves) k_cmix_st(Geom g, MixGeom mg, c
     }
 }
 
+// LDOS maps with T folded into the b planes (k_fold_c), staged as k_cmix_st,
+// with the channels of band p on four consecutive columns (a_re, a_im, c_re,
+// c_im): Im(a_p c_p) = a_re c_im + a_im c_re is formed inside each group of
+// four lanes by lane shuffles, summed over the four bands of a column tile by
+// two more and over the tiles in registers.  No LDS transpose: the LDS holds
+// only the staged planes, so JB sub-pixel rows (up to the whole cell) share
+// each B fragment.  Lane (kq, col < 4) owns pixel 4 kq + col of the tile.
+template <int NT, int LDW, int JB>
+__global__ void __launch_bounds__(64 * kStWaves) k_cmix_stq(Geom g, MixGeom mg, const _Float16 *__restrict__ P,
+                                                           const half8 *__restrict__ Afrag,
+                                                           const double *__restrict__ scale, double sA,
+                                                           const double *__restrict__ bulk,
+                                                           double *__restrict__ ldos,
+                                                           const double *__restrict__ cscale)
+{
+    constexpr int NB = (16 + JB - 1) / JB;  // jj blocks (os <= 16)
+    extern __shared__ __align__(16) unsigned char smem_st[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int bx, by, e;
+    xcd_block(bx, by, e);
+    const int B = g.B, os = g.os, w = g.w, n = g.n, h = g.h, nw = g.nw;
+    const int n2 = n >> 1;
+    const int cy_lo = by * kStWaves;
+    const int cy_raw = cy_lo + wave;
+    const bool cy_ok = cy_raw < n;
+    const int cy = cy_ok ? cy_raw : n - 1;
+    const int cx0 = bx * mg.CB;
+    const size_t plane = (size_t)mg.Ly * nw;
+    const _Float16 *Pe = P + (size_t)e * 4 * mg.NPL * plane;
+    const int col = lane & 15, kq = lane >> 4;
+    const int col0 = cx0 - n2 - w + h + mg.OFF;
+    const int rows = mg.CB * os;
+    const int RW = kStWaves + 2 * w, RP = RW | 1, seg = mg.NUg;
+    const int row_lo = cy_lo - n2 - w + h;
+    const int NPS = mg.nper * B * 2;                    // planes of one spin block per (a|b, hi|lo)
+    _Float16 *Pl = reinterpret_cast<_Float16 *>(smem_st);
+    const int LOFF = NPS * RP * seg * 8;                // hi -> lo plane of the same (o, p)
+    const int ZOFF = 4 * NPS * RP * seg * 8;            // 16-byte zero slot
+    const int trow = (cy - cy_lo) * seg * 8;
+    const unsigned long long mrw = udiv_magic((unsigned)RW), mseg = udiv_magic((unsigned)seg),
+                             mnps = udiv_magic((unsigned)NPS);
+    // this lane's pixel; its sum over the spin blocks is kept in the map
+    // itself (one lane owns each pixel): raw after the first block, scaled
+    // with the periodic term after the last
+    const double unscale = 1.0 / (sA * sA * scale[e] * cscale[e]);
+    const int W = g.W;
+    double *out = ldos + (size_t)e * W * W;
+    const int m = 4 * kq + col;
+    const int mc = m / os, ii = m - mc * os, cx = cx0 + mc;
+    const bool owner = col < 4 && m < rows && cy_ok && cx < n;
+
+    for (int sg = 0; sg < mg.nsets; sg++) {
+        __syncthreads();  // the previous spin block's fragments are read
+        for (int l = threadIdx.x; l < 4 * NPS * RW * seg; l += blockDim.x) {
+            const int rest = udiv(l, mrw), r = l - rest * RW;
+            const int pid = udiv(rest, mseg), g8 = rest - pid * seg;
+            const int abhl = udiv(pid, mnps);
+            const int pl = sg * NPS + (pid - abhl * NPS);
+            const int row = row_lo + r;
+            half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (row < nw) v = load8<LDW>(Pe + ((size_t)abhl * mg.NPL + pl) * plane, row, col0 + g8 * 8, nw);
+            *reinterpret_cast<half8 *>(Pl + ((size_t)pid * RP + r) * seg * 8 + g8 * 8) = v;
+        }
+        if (threadIdx.x == 0) *reinterpret_cast<half8 *>(Pl + ZOFF) = half8{0, 0, 0, 0, 0, 0, 0, 0};
+        __syncthreads();
+#pragma unroll 1
+        for (int jb = 0; jb < NB; jb++) {
+            const int jj0 = jb * JB;
+            if (jj0 >= os) break;
+            float4v hh[JB][NT], cr[JB][NT];
+#pragma unroll
+            for (int i = 0; i < JB; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    hh[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+                    cr[i][t] = float4v{0.f, 0.f, 0.f, 0.f};
+                }
+            // the tile fragments of the next K step are loaded while this
+            // step's MFMAs run
+            half8 ahn[JB], aln[JB];
+#pragma unroll
+            for (int i = 0; i < JB; i++) {
+                const int jj = min(jj0 + i, os - 1);
+                const size_t fa = ((((size_t)sg * os + jj) * mg.KS + 0) * 2) * 64;
+                ahn[i] = Afrag[fa + lane];
+                aln[i] = Afrag[fa + 64 + lane];
+            }
+            for (int ks = 0; ks < mg.KS; ks++) {
+                half8 ahc[JB], alc[JB];
+#pragma unroll
+                for (int i = 0; i < JB; i++) {
+                    ahc[i] = ahn[i];
+                    alc[i] = aln[i];
+                }
+                if (ks + 1 < mg.KS) {
+#pragma unroll
+                    for (int i = 0; i < JB; i++) {
+                        const int jj = min(jj0 + i, os - 1);
+                        const size_t fa = ((((size_t)sg * os + jj) * mg.KS + ks + 1) * 2) * 64;
+                        ahn[i] = Afrag[fa + lane];
+                        aln[i] = Afrag[fa + 64 + lane];
+                    }
+                }
+                const int kg = ks * 4 + kq;
+                const bool kok = kg < mg.KG;
+                const int g8 = kg % mg.NUg, r = kg / mg.NUg;
+                const int ol = r % mg.nper, dy = r / mg.nper;
+                half8 bh[NT], bl[NT];
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    // column t*16 + col = 4 p + j: j = a_re, a_im, c_re, c_im
+                    const int nn = t * 16 + col, p = nn >> 2, j = nn & 3;
+                    const bool ok = kok && p < B;
+                    const int pl = (ol * B + p) * 2 + (j & 1);
+                    const int q = ((((j >> 1) * 2) * NPS + pl) * RP + dy) * seg * 8 + g8 * 8 + trow;
+                    bh[t] = *reinterpret_cast<const half8 *>(Pl + (ok ? q : ZOFF));
+                    bl[t] = *reinterpret_cast<const half8 *>(Pl + (ok ? q + LOFF : ZOFF));
+                }
+#pragma unroll
+                for (int i = 0; i < JB; i++) {
+                    const half8 ah = ahc[i], al = alc[i];
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        hh[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[i][t], 0, 0, 0);
+                        cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], cr[i][t], 0, 0, 0);
+                        cr[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], cr[i][t], 0, 0, 0);
+                    }
+                }
+            }
+            // epilogue: lane (kq, col) holds pixel rows 4 kq + r of column
+            // t*16 + col; every lane of a group of four gets the band's
+            // a_re c_im + a_im c_re, the groups' sums then add over xor 4, 8
+#pragma unroll
+            for (int i = 0; i < JB; i++) {
+                float sr[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float v = hh[i][t][r] + cr[i][t][r] * (1.0f / kLoScale);
+                        const float x1 = __shfl_xor(v, 1), x2 = __shfl_xor(v, 2), x3 = __shfl_xor(v, 3);
+                        sr[r] += v * x3 + x1 * x2;
+                    }
+                float mine = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    float x = sr[r];
+                    x += __shfl_xor(x, 4);
+                    x += __shfl_xor(x, 8);
+                    if ((col & 3) == r) mine = x;
+                }
+                const int jj = jj0 + i;
+                if (owner && jj < os) {
+                    double *o = out + (size_t)(cx * os + ii) + (size_t)(cy * os + jj) * W;
+                    double v = (double)mine;
+                    if (sg > 0) v += *o;
+                    *o = sg + 1 < mg.nsets ? v : -(v * unscale + bulk[(size_t)e * os * os + ii * os + jj]);
+                }
+            }
+        }
+    }
+}
+
+size_t cmix_stq_lds(const MixGeom &mg, const Geom &g)
+{
+    const int RP = (kStWaves + 2 * g.w) | 1;
+    const size_t planes = (size_t)4 * mg.nper * g.B * 2;
+    return (planes * RP * mg.NUg * 8 + 8) * sizeof(_Float16);
+}
+
+template <int NT, int JB>
+void launch_stq(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
+                const double *bulk, int E, double *ldos, hipStream_t s, const double *cscale)
+{
+    dim3 grid((g.n + mg.CB - 1) / mg.CB, (g.n + kStWaves - 1) / kStWaves, E);
+    dim3 blk(64 * kStWaves);
+    const size_t lds = cmix_stq_lds(mg, g);
+#define CQPI_STQ(ldw) \
+    hipLaunchKernelGGL((k_cmix_stq<NT, ldw, JB>), grid, blk, lds, s, g, mg, P, A, scale, sA, bulk, ldos, cscale)
+    switch (mg.LDW) {
+    case 8: CQPI_STQ(8); break;
+    case 4: CQPI_STQ(4); break;
+    case 2: CQPI_STQ(2); break;
+    default: CQPI_STQ(1); break;
+    }
+#undef CQPI_STQ
+}
+
 template <int NT, int JB, bool J>
 void launch_st_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
                  const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
@@ -1125,6 +1314,11 @@ void launch_st_j(const Geom &g, const MixGeom &mg, const _Float16 *P, const half
 #undef CQPI_ST
 }
 
+// folded LDOS maps: sub-pixel rows per B fragment of the shuffle epilogue
+#ifndef CQPI_STQ_JB
+#define CQPI_STQ_JB 3
+#endif
+
 template <int NT, int JB>
 int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *A, const double *scale, double sA,
               const double2 *T, const double *bulk, int E, double *ldos, const double2 *tips, hipStream_t s,
@@ -1132,6 +1326,8 @@ int launch_st(const Geom &g, const MixGeom &mg, const _Float16 *P, const half8 *
 {
     if (tips)
         launch_st_j<NT, JB, true>(g, mg, P, A, scale, sA, T, bulk, E, ldos, tips, s, nullptr);
+    else if (cscale)
+        launch_stq<NT, CQPI_STQ_JB>(g, mg, P, A, scale, sA, bulk, E, ldos, s, cscale);
     else
         launch_st_j<NT, JB, false>(g, mg, P, A, scale, sA, T, bulk, E, ldos, nullptr, s, cscale);
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
