@@ -406,7 +406,7 @@ def hex_timing(cqpi, synth, torch, steps, warmup, with_oracle, cells=16, oracle_
         q = cqpi.engine_for(wl, prec)
         dt, st = time_steps(q, wl, steps, warmup, torch, stage_reps=2)
         q.run(wl.energies, wl.eta, host=False)
-        kern = "k_cmix_st" if prec == "mixed" else "k_contract_st"
+        kern = "k_cmix_stq" if prec == "mixed" else "k_contract_st"
         t_eig, _ = stage_launch_s(st, "eigensolve")
         K, b = wl.K, wl.bands
         eig_bytes = K * K * (16 * b * b + 8 * b)
