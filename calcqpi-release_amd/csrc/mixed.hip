@@ -68,30 +68,59 @@ __global__ void __launch_bounds__(1024) k_wscale(const double2 *__restrict__ G0w
 // out by plane_off), rows along x: ab=0 holds G0(R)[o][p], ab=1 holds
 // G0(-R)[p][o], R = (idx-OFF-h, sy-h); entries outside the window are zero.
 // (The whole-chain path writes these planes from the transform kernel; this
-// kernel serves the GPUQPI drop-in path.)
-__global__ void k_pack(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
-                       const double *__restrict__ scale, _Float16 *__restrict__ P)
+// kernel serves the GPUQPI drop-in path.)  One workgroup per (row sy, energy,
+// chunk of kPackPairs source pairs): the a row sy and the b row 2h - sy of the
+// chunk's pairs are read as contiguous runs of their window records into LDS,
+// then each item builds 8 consecutive columns of one plane and stores them as
+// one 16-byte hi and one 16-byte lo vector.  A gather of each value from its
+// own record, one plane row per workgroup, took 2.1 ms more for the 6-band
+// Josephson map of 51 energies.
+// kp pairs per chunk: 8, fewer where the two staged rows would pass 64 KB of LDS
+constexpr int kPackPairs = 8;
+constexpr size_t kPackLds = 64 * 1024;
+
+__global__ void __launch_bounds__(256) k_pack_rows(Geom g, MixGeom mg, const double2 *__restrict__ G0w,
+                                                   const double *__restrict__ scale, _Float16 *__restrict__ P,
+                                                   int kp)
 {
+    extern __shared__ double2 Sp[];  // [ab][sx][kp]
     const int sy = blockIdx.x, e = blockIdx.z;
-    const int B = g.B, NP = B * B, nw = g.nw;
-    const int ab = blockIdx.y / mg.NPL, pl = blockIdx.y % mg.NPL;
-    const int ri = pl & 1, op = pl >> 1, o = op / B, p = op % B;
+    const int B = g.B, NP = B * B, nw = g.nw, h = g.h;
+    const int q0 = blockIdx.y * kp, nq = min(kp, NP - q0);
     const double s = scale[e];
     const double2 *G = G0w + (size_t)e * nw * nw * NP;
     const size_t plane = (size_t)mg.Ly * nw;
-    _Float16 *hi = P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane;
-    _Float16 *lo = P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane;
-    for (int idx = threadIdx.x; idx < mg.Ly; idx += blockDim.x) {
-        int sx = idx - mg.OFF;
-        double v = 0.0;
-        if (sx >= 0 && sx < nw) {
-            double2 z = ab == 0 ? G[((size_t)sx * nw + sy) * NP + o * B + p]
-                                : G[((size_t)(2 * g.h - sx) * nw + (2 * g.h - sy)) * NP + p * B + o];
-            v = (ri ? z.y : z.x) * s;
+    // ab = 0: G(sx, sy)[q]; ab = 1: G(2h - sx, 2h - sy)[q], q = q0 + j
+    for (int t = threadIdx.x; t < 2 * nw * nq; t += blockDim.x) {
+        const int ab = t / (nw * nq), rest = t - ab * nw * nq, sx = rest / nq, j = rest - sx * nq;
+        const size_t rec = ab ? ((size_t)(2 * h - sx) * nw + (2 * h - sy)) : ((size_t)sx * nw + sy);
+        Sp[((size_t)ab * nw + sx) * kp + j] = G[rec * NP + q0 + j];
+    }
+    __syncthreads();
+    const int ng = mg.Ly >> 3;
+    // items (ab, j, re|im, column group); source pair q = q0 + j = r B + c
+    // goes to plane (r B + c) for a (o = r, p = c) and (c B + r) for b
+    // (the b plane (o, p) holds G(-R)[p][o])
+    for (int t = threadIdx.x; t < 2 * nq * 2 * ng; t += blockDim.x) {
+        const int gi = t % ng, rest = t / ng, ri = rest & 1, abj = rest >> 1, ab = abj / nq, j = abj - ab * nq;
+        const int q = q0 + j, r = q / B, c = q - r * B;
+        const int pl = (ab ? (c * B + r) : (r * B + c)) * 2 + ri;
+        half8 hv, lv;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int sx = gi * 8 + u - mg.OFF;
+            double v = 0.0;
+            if (sx >= 0 && sx < nw) {
+                const double2 z = Sp[((size_t)ab * nw + sx) * kp + j];
+                v = (ri ? z.y : z.x) * s;
+            }
+            const _Float16 hh = (_Float16)v;
+            hv[u] = hh;
+            lv[u] = (_Float16)((v - (double)hh) * kLoScale);
         }
-        _Float16 h = (_Float16)v;
-        hi[plane_off(sy, idx, nw)] = h;
-        lo[plane_off(sy, idx, nw)] = (_Float16)((v - (double)h) * kLoScale);
+        const size_t o = plane_off(sy, gi * 8, nw);
+        *reinterpret_cast<half8 *>(P + ((((size_t)e * 2 + ab) * 2 + 0) * mg.NPL + pl) * plane + o) = hv;
+        *reinterpret_cast<half8 *>(P + ((((size_t)e * 2 + ab) * 2 + 1) * mg.NPL + pl) * plane + o) = lv;
     }
 }
 
@@ -1460,8 +1489,13 @@ int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int
     hipLaunchKernelGGL(k_wscale, dim3(E), dim3(1024), 0, s, G0w, per_e, scale);
     if (tfold)
         hipLaunchKernelGGL(k_pack32, dim3(g.nw, E), dim3(256), 0, s, g, G0w, reinterpret_cast<float2 *>(P));
-    else
-        hipLaunchKernelGGL(k_pack, dim3(g.nw, 2 * mg.NPL, E), dim3(128), 0, s, g, mg, G0w, scale, P);
+    else {
+        const size_t row = sizeof(double2) * 2 * g.nw;  // LDS of one staged pair
+        if (row > 160 * 1024) return CQPI_E_ARG;
+        const int kp = (int)std::max<size_t>(1, std::min<size_t>(kPackPairs, kPackLds / row));
+        hipLaunchKernelGGL(k_pack_rows, dim3(g.nw, (g.B * g.B + kp - 1) / kp, E), dim3(256), row * kp, s, g, mg,
+                           G0w, scale, P, kp);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CQPI_E_HIP;
 }
 

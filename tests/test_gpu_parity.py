@@ -323,3 +323,26 @@ def test_reduced_precision_dropin_path(gpu, prec):
     finally:
         q.close()
     assert rm.rel_errors(got, ref)[0] < TOLMIX
+
+
+def test_mixed_dropin_wide_window_packs_in_short_chunks(gpu):
+    """A window of more than 256 rows (n = 264: 267 rows) stages 7 of the 9
+    pairs per pack workgroup, to stay within 64 KB of LDS, and 2 in the last
+    chunk; os = 2 is not a T-folded shape, so the drop-in mixed map packs fp16
+    planes.  It matches the FP64 drop-in map to the mixed tolerance."""
+    wl = synth.square_norb(3, K=512, n=264, E=1, seed=93, w=1, os_=2, eta=0.02)
+    ev, V = oracle.calchamiltonian(wl.hops, wl.bands, wl.efermi, wl.K)
+    gR = oracle.realspace(oracle.greens_k(ev, V, wl.energies[0], wl.eta))
+    T = oracle.tmatrix(wl.V, gR[0, 0])
+    maps = {}
+    for prec in ("fp64", "mixed"):
+        q = cqpi.AmdQPI(wl.W, wl.K, wl.n, wl.window, wl.oversamp, wl.bands, wl.maxband, wl.spin, wl.tiles)
+        try:
+            q.set_precision(prec)
+            assert 2 * q.window_half() + 1 == 267
+            q.wannierldos(T, gR)
+            maps[prec] = q.retrieveResult()
+        finally:
+            q.close()
+    assert np.abs(maps["fp64"]).max() > 0
+    assert rm.rel_errors(maps["mixed"], maps["fp64"])[0] < TOLMIX
