@@ -35,33 +35,54 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int kLoScale = 4096;  // lo parts are stored times 2^12
 
-// per-energy power-of-two scale: max|G0 window| * s in [2^12, 2^13)
-__global__ void __launch_bounds__(1024) k_wscale(const double2 *__restrict__ G0w, size_t per_e,
-                                                double *__restrict__ scale)
+// per-energy power-of-two scale: max|G0 window| * s in [2^12, 2^13).  The
+// maximum is taken by several workgroups per energy (grid x), each folding its
+// slice and raising scale[e], read as a 64-bit integer, with one atomic max:
+// non-negative doubles order like their bit patterns.  k_wscale_clear zeroes
+// scale[] first and k_wscale_fin turns each maximum into its scale in place.
+// (One 1024-lane workgroup per energy took 78.6 us for the 133-row window of
+// 3 bands and one energy; one workgroup per 2048 values, 6.2 us plus 4.7 us
+// for the clear and the finish.)
+__global__ void k_wscale_clear(double *__restrict__ scale, int E)
 {
-    __shared__ double red[1024];
-    const double2 *G = G0w + (size_t)blockIdx.x * per_e;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) scale[e] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_wscale(const double2 *__restrict__ G0w, size_t per_e,
+                                               double *__restrict__ scale)
+{
+    __shared__ double red[256];
+    const double2 *G = G0w + (size_t)blockIdx.y * per_e;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
     double m = 0.0;
-    for (size_t i = threadIdx.x; i < per_e; i += blockDim.x) {
-        double2 v = G[i];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_e; i += stride) {
+        const double2 v = G[i];
         m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
     }
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double mx = red[0];
-        double sc = 1.0;
-        if (mx > 0.0) {
-            int ex;
-            frexp(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
-            sc = ldexp(1.0, 13 - ex);
-        }
-        scale[blockIdx.x] = sc;
+    if (threadIdx.x == 0 && red[0] > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long *>(scale) + blockIdx.y,
+                  (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_wscale_fin(double *__restrict__ scale, int E)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const double mx = scale[e];
+    double sc = 1.0;
+    if (mx > 0.0) {
+        int ex;
+        frexp(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
+        sc = ldexp(1.0, 13 - ex);
     }
+    scale[e] = sc;
 }
 
 // planes P[e][ab][hl][(o*B+p)*2+ri] of nw rows sy and Ly columns idx (laid
@@ -1486,7 +1507,11 @@ int launch_mix_prepare(const Geom &g, const MixGeom &mg, const double2 *G0w, int
                        _Float16 *P, hipStream_t s, bool tfold)
 {
     size_t per_e = (size_t)g.nw * g.nw * g.B * g.B;
-    hipLaunchKernelGGL(k_wscale, dim3(E), dim3(1024), 0, s, G0w, per_e, scale);
+    // workgroups per energy: one per 2048 values, at most 256
+    const int nblk = (int)std::max<size_t>(1, std::min<size_t>(256, per_e / 2048));
+    hipLaunchKernelGGL(k_wscale_clear, dim3((E + 255) / 256), dim3(256), 0, s, scale, E);
+    hipLaunchKernelGGL(k_wscale, dim3(nblk, E), dim3(256), 0, s, G0w, per_e, scale);
+    hipLaunchKernelGGL(k_wscale_fin, dim3((E + 255) / 256), dim3(256), 0, s, scale, E);
     if (tfold)
         hipLaunchKernelGGL(k_pack32, dim3(g.nw, E), dim3(256), 0, s, g, G0w, reinterpret_cast<float2 *>(P));
     else {
